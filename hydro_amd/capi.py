@@ -1,0 +1,227 @@
+"""ctypes mirror of include/hydro_gpu.h (the C-ABI of libhydrogpu.so).
+
+The structures below follow the header field by field; `HG_ABI_VERSION`
+guards against a stale library.  Everything here is plumbing: the compute
+lives in the HIP kernels behind the C-ABI.
+"""
+import ctypes
+import os
+
+HG_ABI_VERSION = 1
+HG_MAX_PHASES = 3
+
+HG_OK, HG_ERR_NAN, HG_ERR_NOT_CONVERGED, HG_ERR_BAD_ARG, HG_ERR_HIP, \
+    HG_ERR_UNSUPPORTED, HG_ERR_PARSE = range(7)
+STATUS_NAMES = {0: "HG_OK", 1: "HG_ERR_NAN", 2: "HG_ERR_NOT_CONVERGED",
+                3: "HG_ERR_BAD_ARG", 4: "HG_ERR_HIP", 5: "HG_ERR_UNSUPPORTED",
+                6: "HG_ERR_PARSE"}
+
+SOLVERS = {"lu": 0, "lu_relaxed": 1, "gauss_seidel": 2, "jacobi": 3,
+           "lu_rb": 4, "cg": 5, "fcg_learned": 6}
+HG_PREC_FP64, HG_PREC_MIXED = 0, 1
+
+HG_FIELD_VELOCITY_X = 0
+HG_FIELD_VELOCITY_Y = 1
+HG_FIELD_PRESSURE = 2
+HG_FIELD_VOLUME_FLUX = 3
+HG_FIELD_PARTIAL_DENSITY_0 = 4
+HG_FIELD_VOLUME_FRACTION_0 = 7
+HG_FIELD_DENSITY = 10
+HG_FIELD_DENSITY_SMOOTH = 11
+HG_FIELD_VISCOSITY_SMOOTH = 12
+HG_FIELD_FORCE_X = 13
+HG_FIELD_FORCE_Y = 14
+HG_FIELD_VOLUME_FLUX_ITER = 15
+
+_d2 = ctypes.c_double * 2
+_d3 = ctypes.c_double * HG_MAX_PHASES
+_i4 = ctypes.c_int32 * 4
+
+
+class HgConfig(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("a", _d2), ("b", _d2),
+        ("dt", ctypes.c_double),
+        ("num_phases", ctypes.c_int32),
+        ("density", _d3), ("viscosity", _d3),
+        ("gravity", _d2), ("force", _d2),
+        ("sigma", ctypes.c_double),
+        ("bc_kind", _i4),
+        ("bc_velocity", _d2 * 4),
+        ("initial_velocity", _d2),
+        ("has_initial_sin", ctypes.c_int32),
+        ("initial_sin_n", _d2),
+        ("initial_sin_lambda", ctypes.c_double),
+        ("initial_sin_phase", ctypes.c_double),
+        ("initial_volume_fraction", _d3),
+        ("a1", _d2), ("b1", _d2), ("a2", _d2), ("b2", _d2),
+        ("ic", _d2), ("ir", ctypes.c_double),
+        ("ic2", _d2), ("ir2", ctypes.c_double),
+        ("initial_volume_fraction_smooth_times", ctypes.c_int32),
+        ("density_smooth_times", ctypes.c_int32),
+        ("viscosity_smooth_times", ctypes.c_int32),
+        ("force_smooth_times", ctypes.c_int32),
+        ("fluid_enable", ctypes.c_int32),
+        ("advection_enable", ctypes.c_int32),
+        ("velocity_relaxation_factor", ctypes.c_double),
+        ("pressure_relaxation_factor", ctypes.c_double),
+        ("rhie_chow_factor", ctypes.c_double),
+        ("convergence_tolerance", ctypes.c_double),
+        ("num_iterations_limit", ctypes.c_int32),
+        ("time_second_order", ctypes.c_int32),
+        ("simpler", ctypes.c_int32),
+        ("force_geometric_average", ctypes.c_int32),
+        ("guess_extrapolation", ctypes.c_double),
+        ("meshvel", _d2),
+        ("has_pressure_fixed_point", ctypes.c_int32),
+        ("pressure_fixed_point", _d2),
+        ("pressure_fixed_value", ctypes.c_double),
+        ("linear_solver_velocity", ctypes.c_int32),
+        ("linear_solver_pressure", ctypes.c_int32),
+        ("lu_relaxed_tolerance", ctypes.c_double),
+        ("lu_relaxed_num_iters_limit", ctypes.c_int32),
+        ("lu_relaxed_relaxation_factor", ctypes.c_double),
+        ("advection_dt_factor", ctypes.c_double),
+        ("tvd_split", ctypes.c_int32),
+        ("sharp", ctypes.c_double),
+        ("pressure_tolerance_gpu", ctypes.c_double),
+        ("pressure_max_iters_gpu", ctypes.c_int32),
+        ("precision_gpu", ctypes.c_int32),
+        ("learned_levels_gpu", ctypes.c_int32),
+        ("learned_channels_gpu", ctypes.c_int32),
+        ("unsupported_flags", ctypes.c_int32),
+    ]
+
+
+class HgIterStats(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_int32),
+                ("convergence_indicator", ctypes.c_double),
+                ("velocity_solve_iters", ctypes.c_int32 * 2),
+                ("pressure_solve_iters", ctypes.c_int32),
+                ("pressure_solve_diff", ctypes.c_double),
+                ("pressure_converged", ctypes.c_int32)]
+
+
+class HgStepStats(ctypes.Structure):
+    _fields_ = [("simple_iterations", ctypes.c_int32),
+                ("convergence_indicator", ctypes.c_double),
+                ("advection_substeps", ctypes.c_int32),
+                ("pressure_solve_iters_total", ctypes.c_int64),
+                ("time", ctypes.c_double)]
+
+
+class HgStats(ctypes.Structure):
+    _fields_ = [(n, _d3) for n in ("volume", "mass", "pd_min", "pd_max",
+                                   "cx", "cy", "vx", "vy")]
+
+
+class HgSolveOpts(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("tolerance", ctypes.c_double),
+                ("max_iters", ctypes.c_int32),
+                ("relaxation_factor", ctypes.c_double),
+                ("precision", ctypes.c_int32)]
+
+
+class HgSolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("final_diff", ctypes.c_double),
+                ("rel_residual", ctypes.c_double), ("converged", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
+
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libhydrogpu.so")
+
+# Every entry point include/hydro_gpu.h declares.
+EXPORTS = [
+    "hg_config_default", "hg_config_parse", "hg_solver_from_name",
+    "hg_create", "hg_destroy", "hg_last_error", "hg_num_cells", "hg_num_faces",
+    "hg_set_field", "hg_get_field", "hg_refresh",
+    "hg_fluid_start_step", "hg_fluid_iterate", "hg_fluid_is_converged",
+    "hg_fluid_finish_step", "hg_advect_substep", "hg_update_properties",
+    "hg_calc_stats", "hg_step", "hg_last_step_timing",
+    "hg_stencil_solve", "hg_poisson_create", "hg_poisson_destroy",
+    "hg_poisson_solve",
+]
+
+_P = ctypes.POINTER
+_dp = _P(ctypes.c_double)
+
+
+def _declare(lib):
+    vp = ctypes.c_void_p
+    sig = {
+        "hg_config_default": (None, [_P(HgConfig)]),
+        "hg_config_parse": (ctypes.c_int, [_P(HgConfig), ctypes.c_char_p,
+                                           ctypes.c_char_p, ctypes.c_size_t]),
+        "hg_solver_from_name": (ctypes.c_int32, [ctypes.c_char_p]),
+        "hg_create": (ctypes.c_int, [_P(HgConfig), _P(vp)]),
+        "hg_destroy": (None, [vp]),
+        "hg_last_error": (ctypes.c_char_p, [vp]),
+        "hg_num_cells": (ctypes.c_int32, [vp]),
+        "hg_num_faces": (ctypes.c_int32, [vp]),
+        "hg_set_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
+        "hg_get_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
+        "hg_refresh": (ctypes.c_int, [vp]),
+        "hg_fluid_start_step": (ctypes.c_int, [vp]),
+        "hg_fluid_iterate": (ctypes.c_int, [vp, _P(HgIterStats)]),
+        "hg_fluid_is_converged": (ctypes.c_int32, [vp]),
+        "hg_fluid_finish_step": (ctypes.c_int, [vp]),
+        "hg_advect_substep": (ctypes.c_int, [vp]),
+        "hg_update_properties": (ctypes.c_int, [vp]),
+        "hg_calc_stats": (ctypes.c_int, [vp, _P(HgStats)]),
+        "hg_step": (ctypes.c_int, [vp, _P(HgStepStats)]),
+        "hg_last_step_timing": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+        "hg_stencil_solve": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32] + [_dp] * 7 +
+                             [_P(HgSolveOpts), _P(HgSolveStats)]),
+        "hg_poisson_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, _dp, _P(vp)]),
+        "hg_poisson_destroy": (None, [vp]),
+        "hg_poisson_solve": (ctypes.c_int, [vp, _dp, _dp, _P(HgSolveOpts), _P(HgSolveStats)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_LIB = None
+
+
+def load(path=None):
+    """Loads libhydrogpu.so; raises if it is missing (no fallback path)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(p)
+    _declare(lib)
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def default_config(lib=None):
+    lib = lib or load()
+    cfg = HgConfig()
+    lib.hg_config_default(ctypes.byref(cfg))
+    return cfg
+
+
+def parse_config(text, cfg=None, lib=None):
+    """Applies .hydroconf `set` lines to cfg (defaults first if cfg is None)."""
+    lib = lib or load()
+    cfg = cfg if cfg is not None else default_config(lib)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.hg_config_parse(ctypes.byref(cfg), text.encode(), err, 512)
+    if rc != HG_OK:
+        raise ValueError(err.value.decode())
+    return cfg
+
+
+def dptr(arr):
+    """Pointer to a C-contiguous float64 numpy array."""
+    return arr.ctypes.data_as(_dp)

@@ -1,0 +1,289 @@
+/*
+ * hydro_gpu.h -- C-ABI of the MI355X hydro2d time-step library (libhydrogpu.so).
+ *
+ * Plain C types only: no exceptions, no C++ or torch types cross this
+ * boundary.  Host buffers passed in are copied, never retained.  Every call
+ * returns an hg_status; hg_last_error() gives per-context detail.  A context
+ * owns one HIP stream and all of its device buffers and holds no global
+ * mutable state, so one context per host thread is safe (the reference runs
+ * one experiment per std::thread, source/control/console.cpp:162-178).
+ *
+ * Field layout is the reference's raw layout (source/hydro2dmpi/mesh.hpp:552-561,
+ * 657-713): cell (i,j) -> i + Nx*j; faces are the (Nx+1)*Ny x-faces
+ * (i + (Nx+1)*j) followed by the Nx*(Ny+1) y-faces (i + Nx*j).
+ */
+#ifndef HYDRO_GPU_H_
+#define HYDRO_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HG_ABI_VERSION 1
+#define HG_MAX_PHASES 3
+
+typedef enum hg_status {
+  HG_OK = 0,
+  HG_ERR_NAN = 1,            /* "NaN pressure"/"NaN field" (fluid.hpp:795-797,1164-1166) */
+  HG_ERR_NOT_CONVERGED = 2,  /* solver hit its iteration limit (stats say so) */
+  HG_ERR_BAD_ARG = 3,
+  HG_ERR_HIP = 4,
+  HG_ERR_UNSUPPORTED = 5,    /* a configuration this build does not run */
+  HG_ERR_PARSE = 6           /* .hydroconf text could not be read */
+} hg_status;
+
+/* Linear solver names: the values of linear_solver_velocity /
+ * linear_solver_pressure (hydro2d.hpp:195-248).  The first four are the
+ * reference's own; the rest are new values of the same keys. */
+typedef enum hg_solver {
+  HG_SOLVER_LU = 0,           /* "lu": one symmetric Gauss-Seidel sweep (linear.hpp:527-567) */
+  HG_SOLVER_LU_RELAXED = 1,   /* "lu_relaxed" (linear.hpp:578-651) */
+  HG_SOLVER_GAUSS_SEIDEL = 2, /* "gauss_seidel": lexicographic SOR (linear.hpp:671-716) */
+  HG_SOLVER_JACOBI = 3,       /* "jacobi" (linear.hpp:736-783) */
+  HG_SOLVER_LU_RB = 4,        /* "lu_rb": red-black ordered symmetric GS sweep (new) */
+  HG_SOLVER_CG = 5,           /* "cg": Jacobi-PCG to a relative residual (new) */
+  HG_SOLVER_FCG_LEARNED = 6   /* "fcg_learned": flexible CG + learned preconditioner (new) */
+} hg_solver;
+
+typedef enum hg_precision {
+  HG_PREC_FP64 = 0,           /* every kernel in FP64 */
+  HG_PREC_MIXED = 1           /* FP32 stencils / SpMV, FP64 residual, dots and updates */
+} hg_precision;
+
+typedef enum hg_bc_kind {
+  HG_BC_WALL = 0              /* "wall vx vy": no-slip wall moving with (vx,vy) */
+} hg_bc_kind;
+
+enum { HG_SIDE_LEFT = 0, HG_SIDE_RIGHT = 1, HG_SIDE_BOTTOM = 2, HG_SIDE_TOP = 3 };
+
+/* Every parameter of the reference's hydro2d module that the time step
+ * reads (hydro2d.hpp:267-615,928-972), with the names of the .hydroconf keys.
+ * hg_config_default() fills the values of examples/general.hydroconf. */
+typedef struct hg_config {
+  int32_t abi_version;                 /* = HG_ABI_VERSION */
+  int32_t nx, ny;                      /* Nx, Ny */
+  double a[2], b[2];                   /* A, B: domain corners */
+  double dt;                           /* dt */
+  int32_t num_phases;                  /* num_phases (1..3) */
+  double density[HG_MAX_PHASES];       /* density_i */
+  double viscosity[HG_MAX_PHASES];     /* viscosity_i */
+  double gravity[2];                   /* gravity */
+  double force[2];                     /* force */
+  double sigma;                        /* sigma (surface tension) */
+  int32_t bc_kind[4];                  /* condition_left/right/bottom/top kind */
+  double bc_velocity[4][2];            /* ... and their velocity */
+  /* initial conditions */
+  double initial_velocity[2];          /* initial_velocity */
+  int32_t has_initial_sin;             /* initial_sin_n present */
+  double initial_sin_n[2];             /* initial_sin_n */
+  double initial_sin_lambda;           /* initial_sin_lambda */
+  double initial_sin_phase;            /* initial_sin_phase */
+  double initial_volume_fraction[HG_MAX_PHASES]; /* initial_volume_fraction_i */
+  double a1[2], b1[2];                 /* A1, B1: block with phase 1 */
+  double a2[2], b2[2];                 /* A2, B2: block with phase 2 */
+  double ic[2], ir;                    /* IC, IR: disc with phase 1 */
+  double ic2[2], ir2;                  /* IC2, IR2 */
+  int32_t initial_volume_fraction_smooth_times;
+  int32_t density_smooth_times;
+  int32_t viscosity_smooth_times;
+  int32_t force_smooth_times;
+  /* fluid solver */
+  int32_t fluid_enable;                /* fluid_enable */
+  int32_t advection_enable;            /* advection_enable */
+  double velocity_relaxation_factor;
+  double pressure_relaxation_factor;
+  double rhie_chow_factor;
+  double convergence_tolerance;
+  int32_t num_iterations_limit;
+  int32_t time_second_order;
+  int32_t simpler;                     /* must be 0 */
+  int32_t force_geometric_average;     /* must be 0 */
+  double guess_extrapolation;
+  double meshvel[2];
+  int32_t has_pressure_fixed_point;    /* pressure_fixed_point present */
+  double pressure_fixed_point[2];
+  double pressure_fixed_value;
+  /* linear solvers */
+  int32_t linear_solver_velocity;      /* hg_solver */
+  int32_t linear_solver_pressure;      /* hg_solver */
+  double lu_relaxed_tolerance;
+  int32_t lu_relaxed_num_iters_limit;
+  double lu_relaxed_relaxation_factor;
+  /* advection */
+  double advection_dt_factor;
+  int32_t tvd_split;
+  double sharp;
+  /* new keys of the GPU path (no reference counterpart) */
+  double pressure_tolerance_gpu;       /* relative residual for cg / fcg_learned (1e-8) */
+  int32_t pressure_max_iters_gpu;      /* iteration cap for cg / fcg_learned */
+  int32_t precision_gpu;               /* hg_precision */
+  int32_t learned_levels_gpu;          /* levels of the learned preconditioner (4) */
+  int32_t learned_channels_gpu;        /* channels at the finest level (16) */
+  /* Bits of hg_unsupported: features the script switched on that this path
+   * does not run (hg_create refuses them with HG_ERR_UNSUPPORTED). */
+  int32_t unsupported_flags;
+} hg_config;
+
+typedef enum hg_unsupported {
+  HG_UNSUP_DT_AUTO = 1 << 0,        /* dt_auto 1 (hydro2d.hpp:1536-1544) */
+  HG_UNSUP_COMPRESSIBLE = 1 << 1,   /* compressible_enable 1 */
+  HG_UNSUP_HEAT = 1 << 2,           /* heat_enable 1 */
+  HG_UNSUP_RADIATION = 1 << 3,      /* radiation_enable 1 */
+  HG_UNSUP_CHEMISTRY = 1 << 4,      /* chem_intensity != 0 */
+  HG_UNSUP_ANTIDIFFUSION = 1 << 5,  /* antidiffusion_factor set */
+  HG_UNSUP_SETTLING = 1 << 6,       /* enable_settling_i 1 */
+  HG_UNSUP_ADVECTION = 1 << 7,      /* advection_solver other than tvd */
+  HG_UNSUP_RIGID_BOX = 1 << 8,      /* box_A/box_B holding cells */
+  HG_UNSUP_CONDITION = 1 << 9,      /* inlet/outlet boundary conditions */
+  HG_UNSUP_IMAGE_INIT = 1 << 10,    /* img_init / imgu_init / imgv_init */
+  HG_UNSUP_OTHER = 1 << 11          /* initial_pois, deforming_velocity, meshvel_auto */
+} hg_unsupported;
+
+typedef enum hg_field {
+  HG_FIELD_VELOCITY_X = 0,     /* cells, time_curr layer */
+  HG_FIELD_VELOCITY_Y = 1,
+  HG_FIELD_PRESSURE = 2,
+  HG_FIELD_VOLUME_FLUX = 3,    /* faces, time_curr layer */
+  HG_FIELD_PARTIAL_DENSITY_0 = 4,   /* +i for phase i */
+  HG_FIELD_VOLUME_FRACTION_0 = 7,   /* +i for phase i (read only) */
+  HG_FIELD_DENSITY = 10,            /* unsmoothed mixture density (read only) */
+  HG_FIELD_DENSITY_SMOOTH = 11,     /* (read only) */
+  HG_FIELD_VISCOSITY_SMOOTH = 12,   /* (read only) */
+  HG_FIELD_FORCE_X = 13,            /* (read only) */
+  HG_FIELD_FORCE_Y = 14,            /* (read only) */
+  HG_FIELD_VOLUME_FLUX_ITER = 15,   /* faces, iter_curr layer (advection input) */
+  HG_FIELD_COUNT = 16
+} hg_field;
+
+typedef struct hg_iter_stats {
+  int32_t iteration;            /* SIMPLE iteration count after this call ("s") */
+  double convergence_indicator; /* "Rs" (fluid.hpp:174-179) */
+  int32_t velocity_solve_iters[2];
+  int32_t pressure_solve_iters; /* sweeps / CG iterations of the pressure solve */
+  double pressure_solve_diff;   /* final max|corr| (SOR/Jacobi) or relative residual (CG) */
+  int32_t pressure_converged;   /* 1 if the pressure solver met its tolerance */
+} hg_iter_stats;
+
+typedef struct hg_step_stats {
+  int32_t simple_iterations;
+  double convergence_indicator;     /* Rs after the last SIMPLE iteration */
+  int32_t advection_substeps;
+  int64_t pressure_solve_iters_total;
+  double time;                      /* fluid time after the step */
+} hg_step_stats;
+
+typedef struct hg_stats {           /* CalcStat (hydro2d.hpp:1432-1529), per phase */
+  double volume[HG_MAX_PHASES];
+  double mass[HG_MAX_PHASES];
+  double pd_min[HG_MAX_PHASES];
+  double pd_max[HG_MAX_PHASES];
+  double cx[HG_MAX_PHASES], cy[HG_MAX_PHASES];
+  double vx[HG_MAX_PHASES], vy[HG_MAX_PHASES];
+} hg_stats;
+
+/* ---- configuration (the .hydroconf interface) ------------------------- */
+
+/* Fills the defaults of examples/general.hydroconf. */
+void hg_config_default(hg_config* cfg);
+
+/* Applies "set <int|double|string|bool|vect> <name> <value>" lines
+ * (console.cpp:276-314) to cfg.  Unknown keys are ignored like the reference
+ * stores them unused; a known key with an unreadable value is HG_ERR_PARSE. */
+hg_status hg_config_parse(hg_config* cfg, const char* text, char* err, size_t errlen);
+
+/* Maps a solver name ("lu", "gauss_seidel", "cg", ...) to hg_solver; -1 if unknown. */
+int32_t hg_solver_from_name(const char* name);
+
+/* ---- context lifecycle (replaces hydro<Mesh>::hydro, hydro2d.hpp:928-972) */
+
+typedef struct hg_ctx hg_ctx;
+
+hg_status hg_create(const hg_config* cfg, hg_ctx** out);
+void hg_destroy(hg_ctx* ctx);
+const char* hg_last_error(const hg_ctx* ctx);
+int32_t hg_num_cells(const hg_ctx* ctx);
+int32_t hg_num_faces(const hg_ctx* ctx);
+
+/* Field transfer (host <-> HBM).  set_field writes both time layers of the
+ * field; call hg_refresh() afterwards to recompute what the constructor
+ * derives from it (initial volume flux, fluid properties, stats). */
+hg_status hg_set_field(hg_ctx* ctx, int32_t field, const double* host, size_t n);
+hg_status hg_get_field(hg_ctx* ctx, int32_t field, double* host, size_t n);
+hg_status hg_refresh(hg_ctx* ctx);
+
+/* ---- stepping at the reference's granularity --------------------------- */
+
+/* FluidSimple::StartStep / MakeIteration / IsConverged / FinishStep
+ * (fluid.hpp:793-1169, solver.hpp:733-736). */
+hg_status hg_fluid_start_step(hg_ctx* ctx);
+hg_status hg_fluid_iterate(hg_ctx* ctx, hg_iter_stats* stats);
+int32_t hg_fluid_is_converged(const hg_ctx* ctx);
+hg_status hg_fluid_finish_step(hg_ctx* ctx);
+/* One AdvectionSolverMultiExplicit sub-step (advection.hpp:417-545). */
+hg_status hg_advect_substep(hg_ctx* ctx);
+/* hydro::UpdateFluidProperties (hydro2d.hpp:1404-1430). */
+hg_status hg_update_properties(hg_ctx* ctx);
+/* hydro::CalcStat reductions (hydro2d.hpp:1432-1529). */
+hg_status hg_calc_stats(hg_ctx* ctx, hg_stats* stats);
+/* hydro::step (hydro2d.hpp:1531-1621): SIMPLE loop, advection sub-steps,
+ * property update, stats. */
+hg_status hg_step(hg_ctx* ctx, hg_step_stats* stats);
+/* Wall-clock of the last hg_step split by the reference's timer regions
+ * (step.fluid, step.advection, step.fluid_properties), in ms, HIP events. */
+hg_status hg_last_step_timing(const hg_ctx* ctx, double* fluid_ms, double* advection_ms,
+                              double* properties_ms);
+
+/* ---- solve level (replaces LinearSolver<Scal,IdxCell,Expr>::Solve,
+ * linear.hpp:302-310) ------------------------------------------------------
+ * A 5-point system on an nx*ny grid: row r means
+ *   s[r]*x[r-nx] + w[r]*x[r-1] + c[r]*x[r] + e[r]*x[r+1] + n[r]*x[r+nx] + k[r] = 0
+ * (terms outside the grid must be 0).  x is overwritten with the solution
+ * from a zero initial guess, as every reference solver starts from 0. */
+typedef struct hg_solve_opts {
+  int32_t solver;            /* hg_solver */
+  double tolerance;          /* SOR/Jacobi: max|corr|; CG: relative residual */
+  int32_t max_iters;         /* SOR/Jacobi: num_iters_limit; CG: iteration cap */
+  double relaxation_factor;  /* SOR/Jacobi omega */
+  int32_t precision;         /* hg_precision */
+} hg_solve_opts;
+
+typedef struct hg_solve_stats {
+  int32_t iterations;
+  double final_diff;         /* max|corr| (SOR/Jacobi) */
+  double rel_residual;       /* ||b - A x||_2 / ||b||_2 in FP64, b = -k */
+  int32_t converged;
+  double ms;                 /* device time of the solve (HIP events) */
+} hg_solve_stats;
+
+hg_status hg_stencil_solve(int32_t nx, int32_t ny, const double* s, const double* w,
+                           const double* c, const double* e, const double* n,
+                           const double* k, double* x, const hg_solve_opts* opts,
+                           hg_solve_stats* stats);
+
+/* ---- variable-coefficient pressure Poisson (div(1/rho grad p) = div u*) --
+ * The SIMPLE pressure-correction system (fluid.hpp:946-1027) in face form:
+ * kx[(nx+1)*ny], ky[nx*(ny+1)] are the face coefficients A/(d*a_f) (>0,
+ * zero on walls), rhs[nx*ny] the cell right-hand side.  Operator:
+ *   (A p)_c = sum_f k_f (p_c - p_nb)  (symmetric positive semi-definite).
+ * A pressure-solve object keeps the operator and the learned preconditioner
+ * resident in HBM so repeated solves do not re-upload. */
+typedef struct hg_poisson hg_poisson;
+
+hg_status hg_poisson_create(int32_t nx, int32_t ny, const double* kx, const double* ky,
+                            hg_poisson** out);
+void hg_poisson_destroy(hg_poisson* ps);
+/* Solves A p = rhs (rhs projected to zero mean) from p = 0 to the relative
+ * residual opts->tolerance (true FP64 residual).  opts->solver is
+ * HG_SOLVER_CG (Jacobi-PCG) or HG_SOLVER_FCG_LEARNED; opts->precision
+ * HG_PREC_MIXED runs FP32 inner solves under FP64 iterative refinement. */
+hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
+                           const hg_solve_opts* opts, hg_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* HYDRO_GPU_H_ */
