@@ -1,0 +1,79 @@
+// cg.h -- conjugate-gradient pressure solvers on the GPU (host interface).
+//
+// The operator is the SIMPLE pressure-correction matrix (fluid.hpp:946-1027)
+// in face form: (A p)_c = sum_f k_f (p_c - p_nb), k_f > 0 on inner faces,
+// k_f = 0 on walls, so A is symmetric positive semi-definite (singular, with
+// the constants as null space, when no pressure point is fixed).  Given as
+// kx[(nx+1)*ny], ky[nx*(ny+1)].  Right-hand sides are projected to zero mean
+// when the system is singular.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace hg {
+
+class LearnedPrecond;
+
+struct CgResult {
+  int iterations = 0;     // total inner iterations
+  int outer = 0;          // refinement rounds (mixed)
+  double rel_residual = 0.;  // true FP64 ||b - A x|| / ||b||
+  bool converged = false;
+  double ms = 0.;
+  bool used_fallback = false;  // learned preconditioner rejected
+};
+
+class CgSolver {
+ public:
+  CgSolver(int nx, int ny, hipStream_t stream);
+  ~CgSolver();
+  CgSolver(const CgSolver&) = delete;
+  CgSolver& operator=(const CgSolver&) = delete;
+
+  // Face coefficients already in device memory (FP64), copied internally;
+  // also builds the FP32 copies and the inverse diagonal.
+  void SetOperator(const double* d_kx, const double* d_ky);
+  // Solves A x = b (device pointers, FP64) from x = 0.
+  // precision: 0 FP64 Jacobi-PCG; 1 FP32 inner PCG under FP64 refinement.
+  // With `pc` set, the inner iteration is flexible CG preconditioned by the
+  // learned network (falls back to Jacobi when it is not a descent operator).
+  CgResult Solve(const double* d_b, double* d_x, double tol, int max_iters, int precision,
+                 bool project, LearnedPrecond* pc = nullptr);
+  // y = A x (FP64), for tests and residual checks.
+  void Apply(const double* d_x, double* d_y);
+  const double* kx() const { return kx_; }
+  const double* ky() const { return ky_; }
+  int nx() const { return nx_; }
+  int ny() const { return ny_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  int nx_, ny_;
+  long n_;
+  hipStream_t stream_;
+  double *kx_ = nullptr, *ky_ = nullptr, *dinv_ = nullptr;
+  float *kx32_ = nullptr, *ky32_ = nullptr, *dinv32_ = nullptr;
+  // FP64 work vectors
+  double *r_ = nullptr, *z_ = nullptr, *pa_ = nullptr, *pb_ = nullptr, *q_ = nullptr;
+  double* b_ = nullptr;
+  // FP32 work vectors
+  float *r32_ = nullptr, *z32_ = nullptr, *pa32_ = nullptr, *pb32_ = nullptr, *q32_ = nullptr;
+  float *e32_ = nullptr, *zprev32_ = nullptr;
+  double* part_ = nullptr;   // block partials
+  void* scal_ = nullptr;     // device scalars (CgScalars)
+  void* host_scal_ = nullptr;  // pinned mirror
+  int blocks_;
+  hipEvent_t ev0_, ev1_;
+
+  CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project);
+  CgResult SolveMixed(const double* b, double* x, double tol, int max_iters, bool project,
+                      LearnedPrecond* pc);
+  int InnerFp32(double tol_inner, int max_iters, LearnedPrecond* pc, bool* pc_failed);
+  double ResidualFp64(const double* b, const double* x, double* r);  // returns ||r||^2
+  double Dot64(const double* a, const double* b);
+  void ReadScalars();
+};
+
+}  // namespace hg

@@ -1,0 +1,788 @@
+// context.hip -- the C-ABI of libhydrogpu.so: context lifecycle, field
+// transfer and the time step driven at the reference's granularity
+// (hydro<Mesh>::step, hydro2d.hpp:1531-1621; FluidSimple, fluid.hpp:421-1207;
+// AdvectionSolverMultiExplicit, advection.hpp:368-555).
+//
+// All fields live in HBM for the life of the context; the host only
+// sequences kernels on the context's stream and reads back the few scalars
+// the reference's control flow branches on (the SIMPLE convergence
+// indicator, solver iteration counts, NaN flags).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hydro_gpu.h"
+#include "cg.h"
+#include "host_geometry.h"
+#include "kernels.h"
+#include "learned_net.h"
+
+#define HG_CHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") +            \
+                                                   hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+namespace {
+
+struct HgError : std::runtime_error {
+  hg_status code;
+  HgError(hg_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// GetDerivativeApproxCoeffs (solver.hpp:815-861): Lagrange weights of the
+// time derivative at 0 from samples at args (BDF2 / BDF1).
+std::vector<double> DerivCoeffs(double target, const std::vector<double>& args) {
+  size_t size = args.size();
+  std::vector<double> res(size);
+  for (size_t i = 0; i < size; ++i) {
+    double denom = 1., numer = 0.;
+    for (size_t j = 0; j < size; ++j) {
+      if (j == i) continue;
+      denom *= args[i] - args[j];
+      double term = 1.;
+      for (size_t k = 0; k < size; ++k)
+        if (k != i && k != j) term *= target - args[k];
+      numer += term;
+    }
+    res[i] = numer / denom;
+  }
+  return res;
+}
+
+// ---- small elementwise kernels -------------------------------------------
+__global__ void KExtrapolate(long n, const double* __restrict__ tc, const double* __restrict__ tp,
+                             double ge, double* __restrict__ out) {
+  long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = tc[i] + (tc[i] - tp[i]) * ge;
+}
+__global__ void KNegateFaces(long nf, const double* __restrict__ fcoef, double* __restrict__ k) {
+  long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (f < nf) k[f] = -fcoef[f];
+}
+__global__ void KNeg(long n, const double* __restrict__ a, double* __restrict__ b) {
+  long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) b[i] = -a[i];
+}
+__global__ void KAddTo(long n, const double* __restrict__ base, const double* __restrict__ x,
+                       double* __restrict__ out) {
+  long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = base[i] + x[i];
+}
+
+inline dim3 G(long n) { return dim3(unsigned((n + 255) / 256)); }
+
+}  // namespace
+
+struct hg_ctx {
+  hg_config cfg{};
+  hg::HostMesh mesh;
+  hg::Geo geo{};
+  hg::Bc bc{};
+  hg::Phases ph{};
+  hipStream_t st = nullptr;
+  long nc = 0, nf = 0;
+  int np = 1;
+  std::vector<void*> allocs;
+  // layers
+  double *u_tc[2], *u_tp[2], *u_ic[2], *u_ip[2];
+  double *p_tc, *p_tp, *p_ic, *p_ip;
+  double *fl_tc, *fl_tp, *fl_ic, *fl_ip;
+  // properties
+  double *rho, *rho_s, *mu, *mu_s, *fx, *fy, *tmp1, *tmp2;
+  double *pd[3], *pd_new[3], *alpha[3];
+  // per-step faces
+  double *ffex, *ffey, *ffmu, *frx, *fry, *ffrx, *ffry;
+  // per-iteration
+  double *gpx, *gpy, *g0x, *g0y, *g1x, *g1y;
+  double *As, *Aw, *Ac, *Ae, *An, *k0, *k1, *x0, *x1;
+  double *ap, *fstar, *fcoef;
+  double *Ps, *Pw, *Pc, *Pe, *Pn, *Pk, *pcorr, *pb, *kface, *zeros;
+  double *xn_d, *yn_d, *stat_part, *stat_out;
+  unsigned long long* rs_d;
+  int* nan_d;
+  unsigned long long* gs_info;
+  hg::JacobiWork jw{};
+  double* jdiff_host_dummy = nullptr;
+  // pinned host mirrors
+  unsigned long long* rs_h = nullptr;
+  int* flag_h = nullptr;
+  unsigned long long* info_h = nullptr;
+  double* stat_h = nullptr;
+  int pinned = -1;
+  double bdf[3] = {0, 0, 0};
+  double t_fluid = 0., t_adv = 0., dt = 0., dt_adv = 0.;
+  int fluid_iter = 0, cd_iter = 0;
+  double last_rs = 1.;
+  hg_iter_stats last_iter{};
+  std::string err;
+  std::unique_ptr<hg::CgSolver> cg;
+  std::unique_ptr<hg::LearnedNet> net;
+  hipEvent_t ev[4];
+  double timing[3] = {0, 0, 0};
+
+  double* Alloc(long n) {
+    void* p = nullptr;
+    HG_CHECK(hipMalloc(&p, size_t(std::max<long>(n, 1)) * sizeof(double)));
+    HG_CHECK(hipMemsetAsync(p, 0, size_t(std::max<long>(n, 1)) * sizeof(double), st));
+    allocs.push_back(p);
+    return (double*)p;
+  }
+  ~hg_ctx() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : allocs) (void)hipFree(p);
+    if (rs_h) (void)hipHostFree(rs_h);
+    if (flag_h) (void)hipHostFree(flag_h);
+    if (info_h) (void)hipHostFree(info_h);
+    if (stat_h) (void)hipHostFree(stat_h);
+    cg.reset();
+    net.reset();
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  void Sync() { HG_CHECK(hipStreamSynchronize(st)); }
+};
+
+namespace {
+
+hg_status Guard(hg_ctx* ctx, const std::function<void()>& fn);
+
+void Upload(hg_ctx* c, double* d, const std::vector<double>& h) {
+  HG_CHECK(hipMemcpyAsync(d, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->st));
+}
+
+void Smooth(hg_ctx* c, const double* in, double* out, int times) {
+  // out = GetSmoothField(in, times); in and out may alias.
+  if (times <= 0) {
+    if (out != in)
+      HG_CHECK(hipMemcpyAsync(out, in, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+    return;
+  }
+  const double* src = in;
+  for (int r = 0; r < times; ++r) {
+    double* dst = (r == times - 1) ? c->tmp2 : ((r & 1) ? c->tmp2 : c->tmp1);
+    if (dst == src) dst = (src == c->tmp1) ? c->tmp2 : c->tmp1;
+    hg::LaunchSmooth(c->st, c->geo, src, dst);
+    src = dst;
+  }
+  HG_CHECK(hipMemcpyAsync(out, src, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+}
+
+void UpdateProperties(hg_ctx* c) {
+  const double* pdv[3] = {c->pd[0], c->pd[1], c->pd[2]};
+  double* al[3] = {c->alpha[0], c->alpha[1], c->alpha[2]};
+  hg::LaunchProperties(c->st, c->geo, c->ph, pdv, al, c->rho, c->mu, c->fx, c->fy);
+  Smooth(c, c->rho, c->rho_s, c->cfg.density_smooth_times);
+  Smooth(c, c->mu, c->mu_s, c->cfg.viscosity_smooth_times);
+  Smooth(c, c->fx, c->fx, c->cfg.force_smooth_times);
+  Smooth(c, c->fy, c->fy, c->cfg.force_smooth_times);
+}
+
+void CalcStats(hg_ctx* c, hg_stats* s) {
+  std::memset(s, 0, sizeof(*s));
+  for (int i = 0; i < c->np; ++i) {
+    hg::LaunchStats(c->st, c->geo, c->xn_d, c->yn_d, c->alpha[i], c->pd[i], c->u_tc[0],
+                    c->u_tc[1], c->stat_part, c->stat_out);
+    HG_CHECK(hipMemcpyAsync(c->stat_h, c->stat_out, 7 * 8, hipMemcpyDeviceToHost, c->st));
+    c->Sync();
+    const double* v = c->stat_h;
+    double volume = v[0];
+    s->volume[i] = volume;
+    s->mass[i] = volume * c->cfg.density[i];
+    s->cx[i] = v[1] / volume;
+    s->cy[i] = v[2] / volume;
+    s->vx[i] = v[3] / volume;
+    s->vy[i] = v[4] / volume;
+    s->pd_min[i] = v[5];
+    s->pd_max[i] = v[6];
+  }
+}
+
+void InitialFlux(hg_ctx* c) {
+  hg::LaunchInitialFlux(c->st, c->geo, c->bc, c->u_tc[0], c->u_tc[1], c->cfg.meshvel[0],
+                        c->cfg.meshvel[1], c->fl_tc);
+  HG_CHECK(hipMemcpyAsync(c->fl_tp, c->fl_tc, c->nf * 8, hipMemcpyDeviceToDevice, c->st));
+}
+
+bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
+  HG_CHECK(hipMemsetAsync(c->nan_d, 0, sizeof(int), c->st));
+  hg::LaunchNanCheck(c->st, a, b, d, int(c->nc), c->nan_d);
+  HG_CHECK(hipMemcpyAsync(c->flag_h, c->nan_d, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  c->Sync();
+  return *c->flag_h != 0;
+}
+
+void StartStep(hg_ctx* c) {
+  c->fluid_iter = 0;
+  c->cd_iter = 0;
+  if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
+    throw HgError(HG_ERR_NAN, "NaN initial pressure");
+  const double ge = c->cfg.guess_extrapolation;
+  for (int d = 0; d < 2; ++d)
+    KExtrapolate<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->u_tc[d], c->u_tp[d], ge, c->u_ic[d]);
+  KExtrapolate<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->p_tc, c->p_tp, ge, c->p_ic);
+  KExtrapolate<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fl_tc, c->fl_tp, ge, c->fl_ic);
+  // inputs that stay fixed through the SIMPLE loop (fluid.hpp:602-641)
+  hg::LaunchForceFaces(c->st, c->geo, c->fx, c->fy, c->mu_s, c->ffex, c->ffey, c->ffmu);
+  hg::LaunchRestoredForce(c->st, c->geo, c->ffex, c->ffey, c->frx, c->fry);
+  hg::LaunchFacesDeriv0Pair(c->st, c->geo, c->frx, c->fry, c->ffrx, c->ffry);
+}
+
+void SolveVelocity(hg_ctx* c, hg::Rows rows, hg_iter_stats* is) {
+  const int nx = c->geo.nx, ny = c->geo.ny;
+  const hg_config& cfg = c->cfg;
+  switch (cfg.linear_solver_velocity) {
+    case HG_SOLVER_LU:
+      hg::LaunchLuWavefront(c->st, nx, ny, rows, c->k0, c->k1, c->x0, c->x1, c->u_ip[0],
+                            c->u_ip[1], c->u_ic[0], c->u_ic[1]);
+      is->velocity_solve_iters[0] = is->velocity_solve_iters[1] = 0;
+      break;
+    case HG_SOLVER_LU_RB:
+      hg::LaunchLuRb(c->st, nx, ny, rows, c->k0, c->k1, c->x0, c->x1, c->u_ip[0], c->u_ip[1],
+                     c->u_ic[0], c->u_ic[1]);
+      is->velocity_solve_iters[0] = is->velocity_solve_iters[1] = 0;
+      break;
+    case HG_SOLVER_GAUSS_SEIDEL: {
+      const double* kk[2] = {c->k0, c->k1};
+      double* xx[2] = {c->x0, c->x1};
+      for (int d = 0; d < 2; ++d) {
+        hg::LaunchGsWavefront(c->st, nx, ny, rows, kk[d], xx[d], cfg.lu_relaxed_tolerance,
+                              cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
+                              c->gs_info);
+        HG_CHECK(hipMemcpyAsync(c->info_h, c->gs_info, 16, hipMemcpyDeviceToHost, c->st));
+        c->Sync();
+        is->velocity_solve_iters[d] = int(c->info_h[0]);
+        KAddTo<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->u_ip[d], xx[d], c->u_ic[d]);
+      }
+      break;
+    }
+    default:
+      throw HgError(HG_ERR_UNSUPPORTED,
+                    "linear_solver_velocity: supported values are lu, lu_rb, gauss_seidel");
+  }
+}
+
+void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
+  const int nx = c->geo.nx, ny = c->geo.ny;
+  const hg_config& cfg = c->cfg;
+  hg::Rows rows{c->Ps, c->Pw, c->Pc, c->Pe, c->Pn};
+  is->pressure_converged = 1;
+  switch (cfg.linear_solver_pressure) {
+    case HG_SOLVER_GAUSS_SEIDEL: {
+      hg::LaunchGsWavefront(c->st, nx, ny, rows, c->Pk, c->pcorr, cfg.lu_relaxed_tolerance,
+                            cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
+                            c->gs_info);
+      HG_CHECK(hipMemcpyAsync(c->info_h, c->gs_info, 16, hipMemcpyDeviceToHost, c->st));
+      c->Sync();
+      is->pressure_solve_iters = int(c->info_h[0]);
+      double d;
+      std::memcpy(&d, &c->info_h[1], 8);
+      is->pressure_solve_diff = d;
+      is->pressure_converged = !(d > cfg.lu_relaxed_tolerance);
+      break;
+    }
+    case HG_SOLVER_JACOBI: {
+      hg::JacobiWork& w = c->jw;
+      const int limit = cfg.lu_relaxed_num_iters_limit;
+      HG_CHECK(hipMemsetAsync(w.xa, 0, c->nc * 8, c->st));
+      HG_CHECK(hipMemsetAsync(w.diff, 0, size_t(limit + 2) * 8, c->st));
+      HG_CHECK(hipMemsetAsync(w.state, 0, 4 * sizeof(int), c->st));
+      int first = 0;
+      while (first <= limit) {
+        int n = std::min(64, limit + 1 - first);
+        hg::LaunchJacobiSweeps(c->st, nx, ny, rows, c->Pk, w, first, n, cfg.lu_relaxed_tolerance,
+                               limit, cfg.lu_relaxed_relaxation_factor);
+        first += n;
+        HG_CHECK(hipMemcpyAsync(c->flag_h, w.state, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        c->Sync();
+        if (c->flag_h[0]) break;
+      }
+      int res_buf = c->flag_h[3];
+      int iters = c->flag_h[2];
+      HG_CHECK(hipMemcpyAsync(c->pcorr, res_buf ? w.xb : w.xa, c->nc * 8,
+                              hipMemcpyDeviceToDevice, c->st));
+      unsigned long long bits;
+      int last_sweep = std::min(first, limit + 1) - 1;
+      // the final diff is that of the last executed sweep
+      HG_CHECK(hipMemcpyAsync(&bits, w.diff + std::max(0, std::min(iters, last_sweep)), 8,
+                              hipMemcpyDeviceToHost, c->st));
+      c->Sync();
+      double d;
+      std::memcpy(&d, &bits, 8);
+      is->pressure_solve_iters = iters;
+      is->pressure_solve_diff = d;
+      is->pressure_converged = !(d > cfg.lu_relaxed_tolerance);
+      break;
+    }
+    case HG_SOLVER_LU:
+      hg::LaunchLuWavefront(c->st, nx, ny, rows, c->Pk, nullptr, c->pcorr, nullptr, nullptr,
+                            nullptr, nullptr, nullptr);
+      is->pressure_solve_iters = 0;
+      break;
+    case HG_SOLVER_LU_RB:
+      // one right-hand side: the second lane solves the same system into tmp1
+      hg::LaunchLuRb(c->st, nx, ny, rows, c->Pk, c->Pk, c->x0, c->x1, c->zeros, c->zeros,
+                     c->pcorr, c->tmp1);
+      is->pressure_solve_iters = 0;
+      break;
+    case HG_SOLVER_CG:
+    case HG_SOLVER_FCG_LEARNED: {
+      if (c->pinned >= 0)
+        throw HgError(HG_ERR_UNSUPPORTED, "cg with pressure_fixed_point is not supported");
+      KNegateFaces<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fcoef, c->kface);
+      KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
+      c->cg->SetOperator(c->kface, c->kface + c->geo.nfx);
+      hg::LearnedPrecond* pc = nullptr;
+      if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED && c->net && c->net->Trained())
+        pc = c->net.get();
+      hg::CgResult r = c->cg->Solve(c->pb, c->pcorr, cfg.pressure_tolerance_gpu,
+                                    cfg.pressure_max_iters_gpu, cfg.precision_gpu, true, pc);
+      is->pressure_solve_iters = r.iterations;
+      is->pressure_solve_diff = r.rel_residual;
+      is->pressure_converged = r.converged ? 1 : 0;
+      break;
+    }
+    default:
+      throw HgError(HG_ERR_UNSUPPORTED, "linear_solver_pressure value not supported");
+  }
+}
+
+void Iterate(hg_ctx* c, hg_iter_stats* is) {
+  std::memset(is, 0, sizeof(*is));
+  const hg_config& cfg = c->cfg;
+  // iter_prev <- iter_curr (fluid.hpp:815-818, conv_diff.hpp:132-134); the
+  // iter_curr buffers are rewritten in full below.
+  std::swap(c->p_ip, c->p_ic);
+  std::swap(c->fl_ip, c->fl_ic);
+  std::swap(c->u_ip[0], c->u_ic[0]);
+  std::swap(c->u_ip[1], c->u_ic[1]);
+  hg::LaunchCellGrads(c->st, c->geo, c->bc, c->p_ip, c->u_ip[0], c->u_ip[1], c->gpx, c->gpy,
+                      c->g0x, c->g0y, c->g1x, c->g1y);
+  hg::MomentumArgs m{};
+  m.u0 = c->u_ip[0];
+  m.u1 = c->u_ip[1];
+  m.g0x = c->g0x; m.g0y = c->g0y; m.g1x = c->g1x; m.g1y = c->g1y;
+  m.gpx = c->gpx; m.gpy = c->gpy; m.frx = c->frx; m.fry = c->fry;
+  m.ffmu = c->ffmu;
+  m.flux = c->fl_ip;
+  m.rho = c->rho_s;
+  m.utp0 = c->u_tp[0]; m.utp1 = c->u_tp[1]; m.utc0 = c->u_tc[0]; m.utc1 = c->u_tc[1];
+  m.bdf0 = c->bdf[0]; m.bdf1 = c->bdf[1]; m.bdf2 = c->bdf[2];
+  m.relax = cfg.velocity_relaxation_factor;
+  m.s = c->As; m.w = c->Aw; m.c = c->Ac; m.e = c->Ae; m.n = c->An; m.k0 = c->k0; m.k1 = c->k1;
+  hg::LaunchMomentum(c->st, c->geo, c->bc, m);
+  hg::Rows vrows{c->As, c->Aw, c->Ac, c->Ae, c->An};
+  SolveVelocity(c, vrows, is);
+  c->cd_iter += 1;
+  hg::LaunchDiag(c->st, c->geo, c->As, c->Aw, c->Ac, c->Ae, c->An, c->ap);
+  hg::LaunchRhieChow(c->st, c->geo, c->bc, c->u_ic[0], c->u_ic[1], c->gpx, c->gpy, c->ffrx,
+                     c->ffry, c->ffex, c->ffey, c->p_ip, c->ap, cfg.rhie_chow_factor,
+                     cfg.meshvel[0], cfg.meshvel[1], c->fstar, c->fcoef);
+  hg::LaunchPressureRows(c->st, c->geo, c->fstar, c->fcoef, c->pinned, cfg.pressure_fixed_value,
+                         c->Ps, c->Pw, c->Pc, c->Pe, c->Pn, c->Pk);
+  SolvePressure(c, is);
+  HG_CHECK(hipMemsetAsync(c->rs_d, 0, 8, c->st));
+  hg::LaunchCorrectCells(c->st, c->geo, c->pcorr, c->ap, c->p_ip,
+                         cfg.pressure_relaxation_factor, c->p_ic, c->u_ic[0], c->u_ic[1],
+                         c->u_ip[0], c->u_ip[1], c->rs_d);
+  hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic);
+  HG_CHECK(hipMemcpyAsync(c->rs_h, c->rs_d, 8, hipMemcpyDeviceToHost, c->st));
+  c->Sync();
+  double rs;
+  std::memcpy(&rs, c->rs_h, 8);
+  c->last_rs = rs;
+  c->fluid_iter += 1;
+  is->iteration = c->fluid_iter;
+  is->convergence_indicator = rs;
+  c->last_iter = *is;
+}
+
+bool IsConverged(const hg_ctx* c) {
+  double ind = (c->cd_iter == 0) ? 1. : c->last_rs;
+  return c->fluid_iter >= c->cfg.num_iterations_limit || ind < c->cfg.convergence_tolerance;
+}
+
+void FinishStep(hg_ctx* c) {
+  std::swap(c->p_tp, c->p_tc);
+  HG_CHECK(hipMemcpyAsync(c->p_tc, c->p_ic, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+  std::swap(c->fl_tp, c->fl_tc);
+  HG_CHECK(hipMemcpyAsync(c->fl_tc, c->fl_ic, c->nf * 8, hipMemcpyDeviceToDevice, c->st));
+  for (int d = 0; d < 2; ++d) {
+    std::swap(c->u_tp[d], c->u_tc[d]);
+    HG_CHECK(hipMemcpyAsync(c->u_tc[d], c->u_ic[d], c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+  }
+  c->t_fluid += c->dt;
+  if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
+}
+
+void AdvectSubstep(hg_ctx* c) {
+  for (int i = 0; i < c->np; ++i) {
+    hg::LaunchAdvGrad(c->st, c->geo, c->pd[i], c->g0x, c->g0y);
+    hg::LaunchAdvUpdate(c->st, c->geo, c->pd[i], c->g0x, c->g0y, c->fl_ic, c->dt_adv,
+                        c->pd_new[i]);
+    std::swap(c->pd[i], c->pd_new[i]);
+  }
+  c->t_adv += c->dt_adv;
+}
+
+hg_status Guard(hg_ctx* ctx, const std::function<void()>& fn) {
+  try {
+    fn();
+    return HG_OK;
+  } catch (const HgError& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return HG_ERR_HIP;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
+  if (!cfg_in || !out) return HG_ERR_BAD_ARG;
+  *out = nullptr;
+  const hg_config& cfg = *cfg_in;
+  if (cfg.abi_version != HG_ABI_VERSION) return HG_ERR_BAD_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HG_ERR_HIP;
+  std::unique_ptr<hg_ctx> c(new hg_ctx());
+  c->cfg = cfg;
+  hg_status rc = Guard(c.get(), [&]() {
+    if (cfg.nx < 2 || cfg.ny < 2) throw HgError(HG_ERR_BAD_ARG, "Nx, Ny must be >= 2");
+    if (cfg.num_phases < 1 || cfg.num_phases > 3) throw HgError(HG_ERR_BAD_ARG, "num_phases 1..3");
+    if (cfg.unsupported_flags)
+      throw HgError(HG_ERR_UNSUPPORTED, "script enables features outside this path (flags " +
+                                            std::to_string(cfg.unsupported_flags) + ")");
+    if (cfg.simpler || cfg.force_geometric_average)
+      throw HgError(HG_ERR_UNSUPPORTED, "simpler / force_geometric_average are not supported");
+    if (cfg.sigma != 0. && cfg.num_phases >= 2)
+      throw HgError(HG_ERR_UNSUPPORTED, "sigma != 0 is not supported");
+    if (std::abs(cfg.sharp) > 1e-10) throw HgError(HG_ERR_UNSUPPORTED, "sharp != 0 is not supported");
+    if (cfg.tvd_split) throw HgError(HG_ERR_UNSUPPORTED, "tvd_split is not supported");
+    if (long(cfg.nx) * cfg.ny > (1L << 30)) throw HgError(HG_ERR_BAD_ARG, "mesh too large");
+    HG_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    for (auto& e : c->ev) HG_CHECK(hipEventCreate(&e));
+    c->np = cfg.num_phases;
+    c->mesh.Init(cfg.nx, cfg.ny, cfg.a, cfg.b);
+    c->geo = c->mesh.geo;
+    for (int d = 0; d < 2; ++d)
+      for (int s = 0; s < 4; ++s) c->bc.vel[d][s] = cfg.bc_velocity[s][d];
+    c->ph.n = c->np;
+    for (int i = 0; i < 3; ++i) {
+      c->ph.rho[i] = cfg.density[i];
+      c->ph.mu[i] = cfg.viscosity[i];
+    }
+    for (int d = 0; d < 2; ++d) {
+      c->ph.grav[d] = cfg.gravity[d];
+      c->ph.force[d] = cfg.force[d];
+    }
+    c->dt = cfg.dt;
+    c->dt_adv = cfg.dt * cfg.advection_dt_factor;
+    std::vector<double> co = DerivCoeffs(0., {-2. * c->dt, -c->dt, 0.});
+    if (!cfg.time_second_order) {
+      std::vector<double> c2 = DerivCoeffs(0., {-c->dt, 0.});
+      co = {0., c2[0], c2[1]};
+    }
+    for (int i = 0; i < 3; ++i) c->bdf[i] = co[i];
+    const long nc = long(cfg.nx) * cfg.ny;
+    const long nf = long(cfg.nx + 1) * cfg.ny + long(cfg.nx) * (cfg.ny + 1);
+    c->nc = nc;
+    c->nf = nf;
+    for (int d = 0; d < 2; ++d) {
+      c->u_tc[d] = c->Alloc(nc); c->u_tp[d] = c->Alloc(nc);
+      c->u_ic[d] = c->Alloc(nc); c->u_ip[d] = c->Alloc(nc);
+    }
+    c->p_tc = c->Alloc(nc); c->p_tp = c->Alloc(nc); c->p_ic = c->Alloc(nc); c->p_ip = c->Alloc(nc);
+    c->fl_tc = c->Alloc(nf); c->fl_tp = c->Alloc(nf); c->fl_ic = c->Alloc(nf); c->fl_ip = c->Alloc(nf);
+    for (double** p : {&c->rho, &c->rho_s, &c->mu, &c->mu_s, &c->fx, &c->fy, &c->tmp1, &c->tmp2})
+      *p = c->Alloc(nc);
+    for (int i = 0; i < 3; ++i) {
+      c->pd[i] = c->Alloc(i < c->np ? nc : 1);
+      c->pd_new[i] = c->Alloc(i < c->np ? nc : 1);
+      c->alpha[i] = c->Alloc(i < c->np ? nc : 1);
+    }
+    for (double** p : {&c->ffex, &c->ffey, &c->ffmu, &c->ffrx, &c->ffry, &c->fstar, &c->fcoef,
+                       &c->kface})
+      *p = c->Alloc(nf);
+    for (double** p : {&c->frx, &c->fry, &c->gpx, &c->gpy, &c->g0x, &c->g0y, &c->g1x, &c->g1y,
+                       &c->As, &c->Aw, &c->Ac, &c->Ae, &c->An, &c->k0, &c->k1, &c->x0, &c->x1,
+                       &c->ap, &c->Ps, &c->Pw, &c->Pc, &c->Pe, &c->Pn, &c->Pk, &c->pcorr, &c->pb,
+                       &c->zeros})
+      *p = c->Alloc(nc);
+    c->xn_d = c->Alloc(cfg.nx + 1);
+    c->yn_d = c->Alloc(cfg.ny + 1);
+    c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
+    c->stat_out = c->Alloc(8);
+    c->rs_d = (unsigned long long*)c->Alloc(1);
+    c->nan_d = (int*)c->Alloc(1);
+    c->gs_info = (unsigned long long*)c->Alloc(2);
+    c->jw.xa = c->Alloc(nc);
+    c->jw.xb = c->Alloc(nc);
+    c->jw.diff = (unsigned long long*)c->Alloc(cfg.lu_relaxed_num_iters_limit + 2);
+    c->jw.state = (int*)c->Alloc(2);
+    HG_CHECK(hipHostMalloc((void**)&c->rs_h, 8));
+    HG_CHECK(hipHostMalloc((void**)&c->flag_h, 4 * sizeof(int)));
+    HG_CHECK(hipHostMalloc((void**)&c->info_h, 16));
+    HG_CHECK(hipHostMalloc((void**)&c->stat_h, 8 * 8));
+    if (cfg.linear_solver_pressure == HG_SOLVER_CG ||
+        cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED)
+      c->cg.reset(new hg::CgSolver(cfg.nx, cfg.ny, c->st));
+    Upload(c.get(), c->xn_d, c->mesh.xn);
+    Upload(c.get(), c->yn_d, c->mesh.yn);
+
+    // ---- initial state (hydro ctor: InitFluidSolver, InitAdvectionSolver,
+    //      UpdateFluidProperties; hydro2d.hpp:267-615, 928-972)
+    std::vector<double> cx(nc), cy(nc);
+    for (int j = 0; j < cfg.ny; ++j)
+      for (int i = 0; i < cfg.nx; ++i) c->mesh.CellCenter(i, j, cx[i + long(cfg.nx) * j],
+                                                          cy[i + long(cfg.nx) * j]);
+    std::vector<double> u0(nc, cfg.initial_velocity[0]), u1(nc, cfg.initial_velocity[1]);
+    if (cfg.has_initial_sin) {
+      double pi = std::atan(1.) * 4.;
+      double kx = cfg.initial_sin_n[0] * (2. * pi / cfg.initial_sin_lambda);
+      double ky = cfg.initial_sin_n[1] * (2. * pi / cfg.initial_sin_lambda);
+      for (long q = 0; q < nc; ++q) {
+        double dotv = 0.;
+        dotv += kx * cx[q];
+        dotv += ky * cy[q];
+        double sv = std::sin(dotv - cfg.initial_sin_phase);
+        u0[q] *= sv;
+        u1[q] *= sv;
+      }
+    }
+    Upload(c.get(), c->u_tc[0], u0);
+    Upload(c.get(), c->u_tc[1], u1);
+    Upload(c.get(), c->u_tp[0], u0);
+    Upload(c.get(), c->u_tp[1], u1);
+    if (cfg.has_pressure_fixed_point) {
+      auto dist = [&](long q) {
+        double dx = cx[q] - cfg.pressure_fixed_point[0], dy = cy[q] - cfg.pressure_fixed_point[1];
+        double s = 0.;
+        s += dx * dx;
+        s += dy * dy;
+        return std::sqrt(s);
+      };
+      long best = 0;
+      for (long q = 0; q < nc; ++q)
+        if (dist(q) < dist(best)) best = q;
+      c->pinned = int(best);
+    }
+    // partial densities before smoothing
+    std::vector<std::vector<double>> pdh(c->np);
+    for (int i = 0; i < c->np; ++i)
+      pdh[i].assign(nc, cfg.density[i] * cfg.initial_volume_fraction[i]);
+    for (long q = 0; q < nc; ++q) {
+      double x = cx[q], y = cy[q];
+      auto inside = [&](const double* lo, const double* hi) {
+        return !(x < lo[0] || hi[0] < x) && !(y < lo[1] || hi[1] < y);
+      };
+      auto dist2 = [&](const double* ctr) {
+        double dx = ctr[0] - x, dy = ctr[1] - y, s = 0.;
+        s += dx * dx;
+        s += dy * dy;
+        return std::sqrt(s);
+      };
+      if (inside(cfg.a2, cfg.b2)) {
+        if (c->np < 3) throw HgError(HG_ERR_BAD_ARG, "block A2-B2 holds cells but num_phases < 3");
+        pdh[2][q] = cfg.density[2];
+      } else if (c->np >= 2 && inside(cfg.a1, cfg.b1)) {
+        pdh[1][q] = cfg.density[1];
+      } else if (c->np >= 2 && dist2(cfg.ic) < cfg.ir) {
+        pdh[1][q] = cfg.density[1];
+      } else if (c->np >= 2 && dist2(cfg.ic2) < cfg.ir2) {
+        pdh[1][q] = cfg.density[1];
+      }
+    }
+    for (int i = 0; i < c->np; ++i) Upload(c.get(), c->pd[i], pdh[i]);
+    for (int i = 1; i < c->np; ++i)
+      Smooth(c.get(), c->pd[i], c->pd[i], cfg.initial_volume_fraction_smooth_times);
+    hg::LaunchPhase0Init(c->st, c->geo, c->ph, c->pd[1], c->pd[2], c->pd[0]);
+    InitialFlux(c.get());
+    UpdateProperties(c.get());
+    c->Sync();
+  });
+  if (rc != HG_OK) {
+    // keep the message reachable: hand back the context only on success
+    static thread_local std::string last;
+    last = c->err;
+    std::fprintf(stderr, "hg_create: %s\n", last.c_str());
+    return rc;
+  }
+  *out = c.release();
+  return HG_OK;
+}
+
+void hg_destroy(hg_ctx* ctx) { delete ctx; }
+
+const char* hg_last_error(const hg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+int32_t hg_num_cells(const hg_ctx* ctx) { return ctx ? int32_t(ctx->nc) : 0; }
+int32_t hg_num_faces(const hg_ctx* ctx) { return ctx ? int32_t(ctx->nf) : 0; }
+
+static double* FieldPtr(hg_ctx* c, int f, bool write, long* n) {
+  *n = c->nc;
+  switch (f) {
+    case HG_FIELD_VELOCITY_X: return c->u_tc[0];
+    case HG_FIELD_VELOCITY_Y: return c->u_tc[1];
+    case HG_FIELD_PRESSURE: return c->p_tc;
+    case HG_FIELD_VOLUME_FLUX: *n = c->nf; return c->fl_tc;
+    default: break;
+  }
+  if (f >= HG_FIELD_PARTIAL_DENSITY_0 && f < HG_FIELD_PARTIAL_DENSITY_0 + c->np)
+    return c->pd[f - HG_FIELD_PARTIAL_DENSITY_0];
+  if (write) return nullptr;
+  if (f >= HG_FIELD_VOLUME_FRACTION_0 && f < HG_FIELD_VOLUME_FRACTION_0 + c->np)
+    return c->alpha[f - HG_FIELD_VOLUME_FRACTION_0];
+  switch (f) {
+    case HG_FIELD_DENSITY: return c->rho;
+    case HG_FIELD_DENSITY_SMOOTH: return c->rho_s;
+    case HG_FIELD_VISCOSITY_SMOOTH: return c->mu_s;
+    case HG_FIELD_FORCE_X: return c->fx;
+    case HG_FIELD_FORCE_Y: return c->fy;
+    case HG_FIELD_VOLUME_FLUX_ITER: *n = c->nf; return c->fl_ic;
+    default: return nullptr;
+  }
+}
+
+hg_status hg_set_field(hg_ctx* c, int32_t field, const double* host, size_t n) {
+  if (!c || !host) return HG_ERR_BAD_ARG;
+  long len;
+  double* d = FieldPtr(c, field, true, &len);
+  if (!d || size_t(len) != n) {
+    c->err = "hg_set_field: bad field or size";
+    return HG_ERR_BAD_ARG;
+  }
+  return Guard(c, [&]() {
+    HG_CHECK(hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->st));
+    double* prev = nullptr;
+    if (field == HG_FIELD_VELOCITY_X) prev = c->u_tp[0];
+    if (field == HG_FIELD_VELOCITY_Y) prev = c->u_tp[1];
+    if (field == HG_FIELD_PRESSURE) prev = c->p_tp;
+    if (field == HG_FIELD_VOLUME_FLUX) prev = c->fl_tp;
+    if (prev) HG_CHECK(hipMemcpyAsync(prev, host, n * 8, hipMemcpyHostToDevice, c->st));
+    c->Sync();
+  });
+}
+
+hg_status hg_get_field(hg_ctx* c, int32_t field, double* host, size_t n) {
+  if (!c || !host) return HG_ERR_BAD_ARG;
+  long len;
+  double* d = FieldPtr(c, field, false, &len);
+  if (!d || size_t(len) != n) {
+    c->err = "hg_get_field: bad field or size";
+    return HG_ERR_BAD_ARG;
+  }
+  return Guard(c, [&]() {
+    HG_CHECK(hipMemcpyAsync(host, d, n * 8, hipMemcpyDeviceToHost, c->st));
+    c->Sync();
+  });
+}
+
+hg_status hg_refresh(hg_ctx* c) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    InitialFlux(c);
+    UpdateProperties(c);
+    c->Sync();
+  });
+}
+
+hg_status hg_fluid_start_step(hg_ctx* c) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() { StartStep(c); });
+}
+
+hg_status hg_fluid_iterate(hg_ctx* c, hg_iter_stats* s) {
+  if (!c) return HG_ERR_BAD_ARG;
+  hg_iter_stats tmp;
+  return Guard(c, [&]() { Iterate(c, s ? s : &tmp); });
+}
+
+int32_t hg_fluid_is_converged(const hg_ctx* c) { return c && IsConverged(c) ? 1 : 0; }
+
+hg_status hg_fluid_finish_step(hg_ctx* c) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() { FinishStep(c); });
+}
+
+hg_status hg_advect_substep(hg_ctx* c) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    AdvectSubstep(c);
+    c->Sync();
+  });
+}
+
+hg_status hg_update_properties(hg_ctx* c) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    UpdateProperties(c);
+    c->Sync();
+  });
+}
+
+hg_status hg_calc_stats(hg_ctx* c, hg_stats* s) {
+  if (!c || !s) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() { CalcStats(c, s); });
+}
+
+hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
+  if (!c) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    hg_step_stats st{};
+    HG_CHECK(hipEventRecord(c->ev[0], c->st));
+    StartStep(c);
+    if (c->cfg.fluid_enable) {
+      while (!IsConverged(c)) {
+        hg_iter_stats is;
+        Iterate(c, &is);
+        st.pressure_solve_iters_total += is.pressure_solve_iters;
+        st.convergence_indicator = is.convergence_indicator;
+      }
+    }
+    st.simple_iterations = c->fluid_iter;
+    FinishStep(c);
+    HG_CHECK(hipEventRecord(c->ev[1], c->st));
+    if (c->cfg.advection_enable) {
+      while (c->t_adv < c->t_fluid - 0.5 * c->dt_adv) {
+        AdvectSubstep(c);
+        st.advection_substeps += 1;
+      }
+    }
+    HG_CHECK(hipEventRecord(c->ev[2], c->st));
+    UpdateProperties(c);
+    HG_CHECK(hipEventRecord(c->ev[3], c->st));
+    c->Sync();
+    float a = 0, b = 0, d = 0;
+    HG_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HG_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    HG_CHECK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+    c->timing[0] = a;
+    c->timing[1] = b;
+    c->timing[2] = d;
+    st.time = c->t_fluid;
+    if (s) *s = st;
+  });
+}
+
+hg_status hg_last_step_timing(const hg_ctx* c, double* f, double* a, double* p) {
+  if (!c) return HG_ERR_BAD_ARG;
+  if (f) *f = c->timing[0];
+  if (a) *a = c->timing[1];
+  if (p) *p = c->timing[2];
+  return HG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// device_common.h -- geometry and boundary helpers shared by the HIP kernels.
+//
+// The mesh is the reference's uniform structured 2D mesh (mesh.hpp:722-736,
+// mesh2d.hpp:20-434).  All geometric quantities a kernel needs are constant
+// per face direction / boundary side; the host derives them once with the
+// reference's own formulas (host_geometry.h) and hands them to kernels by
+// value in `Geo`.  Kernels written against these scalars evaluate every
+// expression in the reference's operation order, dropping only additions of
+// exact zeros (products with the zero component of an axis-aligned surface
+// or normal), which cannot change a result except for the sign of a zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hg {
+
+enum Side { kLeft = 0, kRight = 1, kBottom = 2, kTop = 3 };
+
+struct Geo {
+  int nx, ny;
+  int nfx;            // number of x-faces, (nx+1)*ny; y-faces follow
+  double V;           // cell volume (CalcVolume)
+  double sx;          // x-face surface, x component (= area of an x-face)
+  double sy;          // y-face surface, y component
+  double ax, ay;      // x-face / y-face areas (CalcArea)
+  double tx0, tx1;    // x-face: x component of vect to cell 0 / cell 1
+  double ty0, ty1;    // y-face: y component of vect to cell 0 / cell 1
+  double dnx, dny;    // |to1 - to0| for x- / y-faces
+  double bal[4];      // boundary derivative factor (1/|to_valid|)*(+-1) per side
+  double ex_dist[4];  // extrapolation: |to_valid| per side
+  double ex_dot[4];   // extrapolation: S_out(opposite face) . n_outward / per side
+  double ex_den[4];   // extrapolation: 1/dist - S_out(self) . n_outward / V
+  double rdist_x, rdist_y;  // |x_cell - x_face| for x- / y-faces
+};
+
+__device__ __forceinline__ int XFace(const Geo& g, int i, int j) { return i + (g.nx + 1) * j; }
+__device__ __forceinline__ int YFace(const Geo& g, int i, int j) { return g.nfx + i + g.nx * j; }
+
+// Inner-face interpolation u[cm]*(1-0.5) + u[cp]*0.5 (solver.hpp:415-427).
+__device__ __forceinline__ double Mid(double um, double up) { return um * 0.5 + up * 0.5; }
+
+// Extrapolation boundary condition (solver.hpp:446-464) on side `s`, with
+// `uc` the cell value and `uopp` the value on the cell's opposite face.
+__device__ __forceinline__ double Extrap(const Geo& g, int s, double uc, double uopp) {
+  double nom = uc / g.ex_dist[s];
+  nom += uopp * g.ex_dot[s] / g.V;
+  return nom / g.ex_den[s];
+}
+
+// Face values of a cell field at the four faces (W, E, S, N) of cell (i,j).
+// kind: 0 = value condition (bv per side), 1 = zero derivative,
+// 2 = extrapolation.
+template <int kind>
+__device__ __forceinline__ void CellFaces(const Geo& g, const double* __restrict__ u, int i,
+                                          int j, const double* bv, double f[4]) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = i + nx * j;
+  const double uc = u[c];
+  double fw = 0, fe = 0, fs = 0, fn = 0;
+  if (i > 0) fw = Mid(u[c - 1], uc);
+  if (i < nx - 1) fe = Mid(uc, u[c + 1]);
+  if (j > 0) fs = Mid(u[c - nx], uc);
+  if (j < ny - 1) fn = Mid(uc, u[c + nx]);
+  if (kind == 0) {
+    if (i == 0) fw = bv[kLeft];
+    if (i == nx - 1) fe = bv[kRight];
+    if (j == 0) fs = bv[kBottom];
+    if (j == ny - 1) fn = bv[kTop];
+  } else if (kind == 1) {
+    if (i == 0) fw = uc;
+    if (i == nx - 1) fe = uc;
+    if (j == 0) fs = uc;
+    if (j == ny - 1) fn = uc;
+  } else {
+    if (i == 0) fw = Extrap(g, kLeft, uc, fe);
+    if (i == nx - 1) fe = Extrap(g, kRight, uc, fw);
+    if (j == 0) fs = Extrap(g, kBottom, uc, fn);
+    if (j == ny - 1) fn = Extrap(g, kTop, uc, fs);
+  }
+  f[0] = fw;
+  f[1] = fe;
+  f[2] = fs;
+  f[3] = fn;
+}
+
+// Gradient from the four face values (solver.hpp:658-677).
+__device__ __forceinline__ void GradFromFaces(const Geo& g, const double f[4], double& gx,
+                                              double& gy) {
+  double sxv = (-g.sx) * f[0];
+  sxv += g.sx * f[1];
+  double syv = (-g.sy) * f[2];
+  syv += g.sy * f[3];
+  gx = sxv / g.V;
+  gy = syv / g.V;
+}
+
+// Wave-level reductions (64 lanes).
+__device__ __forceinline__ double WaveSum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double WaveMax(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double WaveMin(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+}  // namespace hg

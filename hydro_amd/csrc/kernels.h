@@ -1,0 +1,108 @@
+// kernels.h -- launchers of the HIP kernels (step, solvers, learned net).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace hg {
+
+// Velocity boundary values per side (wall velocity), per component.
+struct Bc {
+  double vel[2][4];
+};
+
+struct Phases {
+  int n;
+  double rho[3];
+  double mu[3];
+  double grav[2];
+  double force[2];
+};
+
+struct MomentumArgs {
+  const double *u0, *u1, *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry;
+  const double *ffmu, *flux, *rho, *utp0, *utp1, *utc0, *utc1;
+  double bdf0, bdf1, bdf2, relax;
+  double *s, *w, *c, *e, *n, *k0, *k1;
+};
+
+// ---- step kernels (step_kernels.hip) ----
+void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
+                      const double* mu, double* ffex, double* ffey, double* ffmu);
+void LaunchRestoredForce(hipStream_t st, const Geo& g, const double* ffex, const double* ffey,
+                         double* frx, double* fry);
+void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const double* ax, const double* ay,
+                           double* fx, double* fy);
+void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
+                     const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
+                     double* g0y, double* g1x, double* g1y);
+void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a);
+void LaunchDiag(hipStream_t st, const Geo& g, const double* s, const double* w, const double* c,
+                const double* e, const double* n, double* ap);
+void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                    const double* u1, const double* gpx, const double* gpy, const double* ffrx,
+                    const double* ffry, const double* ffex, const double* ffey, const double* p,
+                    const double* ap, double rc, double mvx, double mvy, double* fstar,
+                    double* fcoef);
+void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
+                        int pinned, double pval, double* s, double* w, double* c, double* e,
+                        double* n, double* k);
+void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
+                        const double* pprev, double alpha_p, double* pcurr, double* u0,
+                        double* u1, const double* up0, const double* up1,
+                        unsigned long long* rs);
+void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
+                        const double* pc, double* flux);
+void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                       const double* u1, double mvx, double mvy, double* flux);
+void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, double* gy);
+void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
+                     const double* gy, const double* flux, double dt, double* un);
+void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
+                      double* const alpha[3], double* rho, double* mu, double* fx, double* fy);
+void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out);
+void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const double* pd1,
+                      const double* pd2, double* pd0);
+void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
+                    int* flag);
+int StatsBlocks(const Geo& g);
+void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,
+                 const double* alpha, const double* pd, const double* u0, const double* u1,
+                 double* part, double* out7);
+
+// ---- 5-point solvers (solvers.hip) ----
+// Rows: s x[r-nx] + w x[r-1] + c x[r] + e x[r+1] + n x[r+nx] + k = 0.
+struct Rows {
+  const double *s, *w, *c, *e, *n;
+};
+
+// Exact-order symmetric Gauss-Seidel sweep from zero (linear.hpp:527-567) for
+// up to two right-hand sides sharing the coefficients, in one workgroup.
+// x_out[r] = base[r] + x[r] when base != nullptr (the delta-form update).
+void LaunchLuWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k0,
+                       const double* k1, double* x0, double* x1, const double* base0,
+                       const double* base1, double* out0, double* out1);
+// Exact-order lexicographic SOR from zero (linear.hpp:685-715), all sweeps in
+// one persistent workgroup.  info[0] = iterations (reference count),
+// info[1] = final max|corr| (as double bits).
+void LaunchGsWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k, double* x,
+                       double tol, int limit, double omega, unsigned long long* info);
+// Jacobi (linear.hpp:750-783), batched sweeps with a device-side stop test.
+struct JacobiWork {
+  double* xa;
+  double* xb;
+  unsigned long long* diff;  // [max_sweeps + 1]
+  int* state;                // [0] done flag, [1] counter, [2] iterations, [3] result buffer
+};
+void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k, JacobiWork w,
+                        int first_sweep, int nsweeps, double tol, int limit, double omega);
+// Red-black ordered symmetric GS sweep ("lu_rb"), two right-hand sides.
+void LaunchLuRb(hipStream_t st, int nx, int ny, Rows a, const double* k0, const double* k1,
+                double* xb0, double* xb1, const double* base0, const double* base1,
+                double* out0, double* out1);
+
+// ---- Poisson / CG (cg.hip) ----
+struct CgSystem;  // opaque, defined in cg.hip
+
+}  // namespace hg

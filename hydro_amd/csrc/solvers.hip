@@ -1,0 +1,323 @@
+// solvers.hip -- 5-point linear solvers of the reference, on the GPU.
+//
+// Rows: s x[r-nx] + w x[r-1] + c x[r] + e x[r+1] + n x[r+nx] + k = 0, raw
+// index r = i + nx*j.  The reference's "lu" and "gauss_seidel" sweep the
+// cells in raw order (linear.hpp:536-563, 692-709), so cell (i,j) needs the
+// new values of (i-1,j) and (i,j-1): the anti-diagonal i+j is a wavefront.
+// One workgroup runs the whole solve: thread t owns a block of columns and
+// handles row j = step - t at each step; the west value crosses threads
+// through LDS, the south value stays in registers, and one barrier separates
+// steps.  Each cell computes exactly the reference's expression, so the
+// result equals the serial sweep bit for bit.  Compiled with
+// -ffp-contract=off.
+
+#include "kernels.h"
+
+namespace hg {
+
+namespace {
+
+constexpr int kMaxCw = 4;        // columns per thread (nx <= 4096)
+constexpr int kMaxThreads = 1024;
+
+inline int ThreadsFor(int nx, int* cw) {
+  int c = (nx + kMaxThreads - 1) / kMaxThreads;
+  if (c < 1) c = 1;
+  *cw = c;
+  int t = (nx + c - 1) / c;
+  return ((t + 63) / 64) * 64;
+}
+
+// Forward half of the symmetric sweep: x = (-k - s x_S - w x_W) / c.
+// Backward half: x -= (n x_N + e x_E) / c.  NR = number of right-hand sides.
+template <int NR>
+__global__ void __launch_bounds__(kMaxThreads)
+KLuWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k0,
+             const double* __restrict__ k1, double* __restrict__ x0, double* __restrict__ x1,
+             const double* __restrict__ base0, const double* __restrict__ base1,
+             double* __restrict__ out0, double* __restrict__ out1) {
+  extern __shared__ double lds[];  // [2 parities][NR][T]
+  const int T = blockDim.x;
+  const int t = threadIdx.x;
+  const int c0 = t * cw;
+  const int c1 = min(nx, c0 + cw);
+  const bool active = c0 < nx;
+  const double* kk[2] = {k0, k1};
+  double* xx[2] = {x0, x1};
+  double south[NR][kMaxCw];
+  // ---- forward
+  for (int step = 0; step < ny + T - 1; ++step) {
+    const int j = step - t;
+    const int par = step & 1;
+    if (active && j >= 0 && j < ny) {
+      double west[NR];
+      for (int q = 0; q < NR; ++q) west[q] = (t > 0) ? lds[((par ^ 1) * NR + q) * T + t - 1] : 0.;
+      for (int i = c0; i < c1; ++i) {
+        const int r = i + nx * j;
+        const double cs = a.s[r], cw_ = a.w[r], cc = a.c[r];
+        for (int q = 0; q < NR; ++q) {
+          double sum = 0;
+          if (j > 0) sum += cs * south[q][i - c0];
+          if (i > 0) sum += cw_ * west[q];
+          const double v = (-kk[q][r] - sum) / cc;
+          xx[q][r] = v;
+          south[q][i - c0] = v;
+          west[q] = v;
+        }
+      }
+      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = west[q];
+    }
+    __syncthreads();
+  }
+  // ---- backward (thread T-1 first, rows from the top)
+  double north[NR][kMaxCw];
+  for (int step = 0; step < ny + T - 1; ++step) {
+    const int j = ny - 1 - (step - (T - 1 - t));
+    const int par = step & 1;
+    if (active && j >= 0 && j < ny) {
+      double east[NR];
+      for (int q = 0; q < NR; ++q)
+        east[q] = (t < T - 1) ? lds[((par ^ 1) * NR + q) * T + t + 1] : 0.;
+      for (int i = c1 - 1; i >= c0; --i) {
+        const int r = i + nx * j;
+        const double cn = a.n[r], ce = a.e[r], cc = a.c[r];
+        for (int q = 0; q < NR; ++q) {
+          double sum = 0;
+          if (j < ny - 1) sum += cn * north[q][i - c0];
+          if (i < nx - 1) sum += ce * east[q];
+          double v = xx[q][r];
+          v -= sum / cc;
+          xx[q][r] = v;
+          north[q][i - c0] = v;
+          east[q] = v;
+        }
+      }
+      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = east[q];
+    }
+    __syncthreads();
+  }
+  // ---- delta-form update out = base + x
+  if (base0) {
+    const double* bb[2] = {base0, base1};
+    double* oo[2] = {out0, out1};
+    for (int r = t; r < nx * ny; r += T)
+      for (int q = 0; q < NR; ++q) oo[q][r] = bb[q][r] + xx[q][r];
+  }
+}
+
+// Lexicographic SOR, every sweep in this workgroup (linear.hpp:685-715).
+__global__ void __launch_bounds__(kMaxThreads)
+KGsWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k,
+             double* __restrict__ x, double tol, int limit, double omega,
+             unsigned long long* __restrict__ info) {
+  extern __shared__ double lds[];  // [2][T] west values, then reduction scratch
+  double* red = lds + 2 * blockDim.x;
+  const int T = blockDim.x;
+  const int t = threadIdx.x;
+  const int c0 = t * cw;
+  const int c1 = min(nx, c0 + cw);
+  const bool active = c0 < nx;
+  for (int r = t; r < nx * ny; r += T) x[r] = 0.;
+  __syncthreads();
+  double south[kMaxCw];
+  long iter = 0;
+  double diff = 0.;
+  while (true) {
+    double dmax = 0.;
+    for (int step = 0; step < ny + T - 1; ++step) {
+      const int j = step - t;
+      const int par = step & 1;
+      if (active && j >= 0 && j < ny) {
+        double west = (t > 0) ? lds[(par ^ 1) * T + t - 1] : 0.;
+        for (int i = c0; i < c1; ++i) {
+          const int r = i + nx * j;
+          double sum = 0.;
+          if (j > 0) sum += a.s[r] * south[i - c0];
+          if (i > 0) sum += a.w[r] * west;
+          if (i < nx - 1) sum += a.e[r] * x[r + 1];
+          if (j < ny - 1) sum += a.n[r] * x[r + nx];
+          const double xo = x[r];
+          const double value = -(k[r] + sum) / a.c[r];
+          const double corr = value - xo;
+          const double ac = fabs(corr);
+          dmax = (dmax < ac) ? ac : dmax;  // std::max(diff, |corr|)
+          const double xn = xo + corr * omega;
+          x[r] = xn;
+          south[i - c0] = xn;
+          west = xn;
+        }
+        lds[par * T + t] = west;
+      }
+      __syncthreads();
+    }
+    // workgroup max of |corr|
+    double m = WaveMax(dmax);
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    m = red[0];
+    for (int q = 1; q < T / 64; ++q) m = fmax(m, red[q]);
+    __syncthreads();
+    diff = m;
+    // do { sweep } while (diff > tol && iter++ < limit)
+    if (!(diff > tol)) break;
+    if (!(iter++ < limit)) break;
+  }
+  if (t == 0) {
+    info[0] = (unsigned long long)iter;
+    info[1] = (unsigned long long)__double_as_longlong(diff);
+  }
+}
+
+// One Jacobi sweep (linear.hpp:757-776) with a device-side stop test run by
+// the last block to finish; later launches see `done` and return.
+__global__ void __launch_bounds__(256)
+KJacobiSweep(int nx, int ny, Rows a, const double* __restrict__ k, JacobiWork w, int sweep,
+             double tol, int limit, double omega) {
+  volatile int* state = w.state;
+  if (state[0]) return;
+  const double* __restrict__ x = (sweep & 1) ? w.xb : w.xa;
+  double* __restrict__ nxt = (sweep & 1) ? w.xa : w.xb;
+  double dmax = 0.;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nx * ny; r += gridDim.x * blockDim.x) {
+    const int i = r % nx, j = r / nx;
+    double sum = 0.;
+    if (j > 0) sum += a.s[r] * x[r - nx];
+    if (i > 0) sum += a.w[r] * x[r - 1];
+    if (i < nx - 1) sum += a.e[r] * x[r + 1];
+    if (j < ny - 1) sum += a.n[r] * x[r + nx];
+    const double value = -(k[r] + sum) / a.c[r];
+    const double corr = value - x[r];
+    const double ac = fabs(corr);
+    dmax = (dmax < ac) ? ac : dmax;
+    nxt[r] = x[r] + corr * omega;
+  }
+  double m = WaveMax(dmax);
+  __shared__ double red[4];
+  __shared__ int last;
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&w.diff[sweep], (unsigned long long)__double_as_longlong(m));
+    __threadfence();
+    last = (atomicAdd(&w.state[1], 1) == int(gridDim.x) - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    const double diff = __longlong_as_double(
+        (long long)atomicMax(&w.diff[sweep], 0ull));
+    w.state[1] = 0;
+    // do { sweep } while (diff > tol && iter++ < limit), iter == sweep here
+    bool cont = (diff > tol) && (sweep < limit);
+    w.state[2] = (diff > tol) ? sweep + 1 : sweep;  // reference's printed iter
+    w.state[3] = (sweep + 1) & 1;                   // buffer holding the result
+    if (!cont) state[0] = 1;
+    __threadfence();
+  }
+}
+
+// "lu_rb", first kernel: black cells ((i+j) odd).  Red neighbours are the
+// forward red values -k/c computed on the fly.
+__global__ void __launch_bounds__(256)
+KLuRbBlack(int nx, int ny, Rows a, const double* __restrict__ k0, const double* __restrict__ k1,
+           double* __restrict__ xb0, double* __restrict__ xb1) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nx * ny) return;
+  const int i = r % nx, j = r / nx;
+  if (((i + j) & 1) == 0) return;
+  double s0 = 0., s1 = 0.;
+  if (j > 0) {
+    const int q = r - nx;
+    s0 += a.s[r] * ((-k0[q] - 0.) / a.c[q]);
+    s1 += a.s[r] * ((-k1[q] - 0.) / a.c[q]);
+  }
+  if (i > 0) {
+    const int q = r - 1;
+    s0 += a.w[r] * ((-k0[q] - 0.) / a.c[q]);
+    s1 += a.w[r] * ((-k1[q] - 0.) / a.c[q]);
+  }
+  if (i < nx - 1) {
+    const int q = r + 1;
+    s0 += a.e[r] * ((-k0[q] - 0.) / a.c[q]);
+    s1 += a.e[r] * ((-k1[q] - 0.) / a.c[q]);
+  }
+  if (j < ny - 1) {
+    const int q = r + nx;
+    s0 += a.n[r] * ((-k0[q] - 0.) / a.c[q]);
+    s1 += a.n[r] * ((-k1[q] - 0.) / a.c[q]);
+  }
+  xb0[r] = (-k0[r] - s0) / a.c[r];
+  xb1[r] = (-k1[r] - s1) / a.c[r];
+}
+
+// "lu_rb", second kernel: backward update of red cells from black
+// neighbours (N, E, W, S order) and the delta-form update for all cells.
+__global__ void __launch_bounds__(256)
+KLuRbRed(int nx, int ny, Rows a, const double* __restrict__ k0, const double* __restrict__ k1,
+         const double* __restrict__ xb0, const double* __restrict__ xb1,
+         const double* __restrict__ base0, const double* __restrict__ base1,
+         double* __restrict__ out0, double* __restrict__ out1) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nx * ny) return;
+  const int i = r % nx, j = r / nx;
+  if ((i + j) & 1) {
+    out0[r] = base0[r] + xb0[r];
+    out1[r] = base1[r] + xb1[r];
+    return;
+  }
+  double x0 = (-k0[r] - 0.) / a.c[r];
+  double x1 = (-k1[r] - 0.) / a.c[r];
+  double s0 = 0., s1 = 0.;
+  if (j < ny - 1) { s0 += a.n[r] * xb0[r + nx]; s1 += a.n[r] * xb1[r + nx]; }
+  if (i < nx - 1) { s0 += a.e[r] * xb0[r + 1]; s1 += a.e[r] * xb1[r + 1]; }
+  if (i > 0) { s0 += a.w[r] * xb0[r - 1]; s1 += a.w[r] * xb1[r - 1]; }
+  if (j > 0) { s0 += a.s[r] * xb0[r - nx]; s1 += a.s[r] * xb1[r - nx]; }
+  x0 -= s0 / a.c[r];
+  x1 -= s1 / a.c[r];
+  out0[r] = base0[r] + x0;
+  out1[r] = base1[r] + x1;
+}
+
+}  // namespace
+
+void LaunchLuWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k0,
+                       const double* k1, double* x0, double* x1, const double* base0,
+                       const double* base1, double* out0, double* out1) {
+  int cw;
+  int T = ThreadsFor(nx, &cw);
+  if (k1) {
+    size_t lds = size_t(2) * 2 * T * sizeof(double);
+    KLuWavefront<2><<<1, T, lds, st>>>(nx, ny, cw, a, k0, k1, x0, x1, base0, base1, out0, out1);
+  } else {
+    size_t lds = size_t(2) * T * sizeof(double);
+    KLuWavefront<1><<<1, T, lds, st>>>(nx, ny, cw, a, k0, k0, x0, x0, base0, base0, out0, out0);
+  }
+}
+
+void LaunchGsWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k, double* x,
+                       double tol, int limit, double omega, unsigned long long* info) {
+  int cw;
+  int T = ThreadsFor(nx, &cw);
+  size_t lds = size_t(2) * T * sizeof(double) + 16 * sizeof(double);
+  KGsWavefront<<<1, T, lds, st>>>(nx, ny, cw, a, k, x, tol, limit, omega, info);
+}
+
+void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k, JacobiWork w,
+                        int first_sweep, int nsweeps, double tol, int limit, double omega) {
+  long n = long(nx) * ny;
+  int blocks = int(std::min<long>((n + 255) / 256, 2048));
+  for (int s = first_sweep; s < first_sweep + nsweeps; ++s)
+    KJacobiSweep<<<blocks, 256, 0, st>>>(nx, ny, a, k, w, s, tol, limit, omega);
+}
+
+void LaunchLuRb(hipStream_t st, int nx, int ny, Rows a, const double* k0, const double* k1,
+                double* xb0, double* xb1, const double* base0, const double* base1,
+                double* out0, double* out1) {
+  long n = long(nx) * ny;
+  dim3 grid(unsigned((n + 255) / 256));
+  KLuRbBlack<<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1);
+  KLuRbRed<<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1, base0, base1, out0, out1);
+}
+
+}  // namespace hg

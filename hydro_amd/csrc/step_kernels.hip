@@ -1,0 +1,753 @@
+// step_kernels.hip -- FP64 kernels of the hydro2d time step.
+//
+// One kernel per stage of FluidSimple::MakeIteration (fluid.hpp:814-1158),
+// AdvectionSolverMultiExplicit::MakeIteration (advection.hpp:440-539) and
+// hydro::UpdateFluidProperties (hydro2d.hpp:1404-1430).  Every value is
+// computed with the reference's operation order (this file is compiled with
+// -ffp-contract=off), so on power-of-two meshes the FP64 results equal the
+// reference's bit for bit.  Fields are SoA FP64, x fastest; faces are the
+// x-faces followed by the y-faces.
+
+#include "kernels.h"
+
+namespace hg {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline dim3 Grid(long n) { return dim3(unsigned((n + kBlock - 1) / kBlock)); }
+
+// ---------------------------------------------------------------------------
+// Face interpolation of the per-step inputs (fluid.hpp:602-641).
+__global__ void __launch_bounds__(kBlock)
+KForceFaces(Geo g, const double* __restrict__ fx, const double* __restrict__ fy,
+            const double* __restrict__ mu, double* __restrict__ ffex, double* __restrict__ ffey,
+            double* __restrict__ ffmu) {
+  const int nf = g.nfx + g.nx * (g.ny + 1);
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int cm, cp;
+  if (f < g.nfx) {
+    int i = f % (g.nx + 1), j = f / (g.nx + 1);
+    cm = (i > 0) ? (i - 1) + g.nx * j : -1;
+    cp = (i < g.nx) ? i + g.nx * j : -1;
+  } else {
+    int r = f - g.nfx;
+    int i = r % g.nx, j = r / g.nx;
+    cm = (j > 0) ? i + g.nx * (j - 1) : -1;
+    cp = (j < g.ny) ? i + g.nx * j : -1;
+  }
+  if (cm >= 0 && cp >= 0) {
+    ffex[f] = Mid(fx[cm], fx[cp]);
+    ffey[f] = Mid(fy[cm], fy[cp]);
+    ffmu[f] = Mid(mu[cm], mu[cp]);
+  } else {
+    int c = cm >= 0 ? cm : cp;  // zero-derivative condition: cell value
+    ffex[f] = fx[c];
+    ffey[f] = fy[c];
+    ffmu[f] = mu[c];
+  }
+}
+
+// Restored force (fluid.hpp:612-626): sum_f S_f (f_f . n_f) |x_c - x_f| / V.
+__global__ void __launch_bounds__(kBlock)
+KRestoredForce(Geo g, const double* __restrict__ ffex, const double* __restrict__ ffey,
+               double* __restrict__ frx, double* __restrict__ fry) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  // x-faces: S = (sx, -0), n = (1, -0): S*(f.n) has only an x component.
+  double sx = 0., sy = 0.;
+  sx += g.sx * ffex[fw] * g.rdist_x;
+  sx += g.sx * ffex[fe] * g.rdist_x;
+  sy += g.sy * ffey[fs] * g.rdist_y;
+  sy += g.sy * ffey[fn] * g.rdist_y;
+  frx[c] = sx / g.V;
+  fry[c] = sy / g.V;
+}
+
+// Face interpolation with zero-derivative condition for a pair of cell fields.
+__global__ void __launch_bounds__(kBlock)
+KFacesDeriv0Pair(Geo g, const double* __restrict__ ax, const double* __restrict__ ay,
+                 double* __restrict__ fx, double* __restrict__ fy) {
+  const int nf = g.nfx + g.nx * (g.ny + 1);
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int cm, cp;
+  if (f < g.nfx) {
+    int i = f % (g.nx + 1), j = f / (g.nx + 1);
+    cm = (i > 0) ? (i - 1) + g.nx * j : -1;
+    cp = (i < g.nx) ? i + g.nx * j : -1;
+  } else {
+    int r = f - g.nfx;
+    int i = r % g.nx, j = r / g.nx;
+    cm = (j > 0) ? i + g.nx * (j - 1) : -1;
+    cp = (j < g.ny) ? i + g.nx * j : -1;
+  }
+  if (cm >= 0 && cp >= 0) {
+    fx[f] = Mid(ax[cm], ax[cp]);
+    fy[f] = Mid(ay[cm], ay[cp]);
+  } else {
+    int c = cm >= 0 ? cm : cp;
+    fx[f] = ax[c];
+    fy[f] = ay[c];
+  }
+}
+
+// Pressure gradient (extrapolation condition) and velocity gradients (value
+// condition) at cells (fluid.hpp:827-832, 840-842; conv_diff.hpp:135).
+__global__ void __launch_bounds__(kBlock)
+KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict__ u0,
+           const double* __restrict__ u1, double* __restrict__ gpx, double* __restrict__ gpy,
+           double* __restrict__ g0x, double* __restrict__ g0y, double* __restrict__ g1x,
+           double* __restrict__ g1y) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  double f[4];
+  CellFaces<2>(g, p, i, j, nullptr, f);
+  GradFromFaces(g, f, gpx[c], gpy[c]);
+  CellFaces<0>(g, u0, i, j, bc.vel[0], f);
+  GradFromFaces(g, f, g0x[c], g0y[c]);
+  CellFaces<0>(g, u1, i, j, bc.vel[1], f);
+  GradFromFaces(g, f, g1x[c], g1y[c]);
+}
+
+struct MomIn {
+  const double *u0, *u1, *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry;
+  const double *ffmu, *flux, *rho, *utp0, *utp1, *utc0, *utc1;
+  double bdf0, bdf1, bdf2, relax;
+};
+struct MomOut {
+  double *s, *w, *c, *e, *n, *k0, *k1;
+};
+
+// Face-normal gradient component of a cell gradient interpolated to a face
+// with the zero-derivative condition (fluid.hpp:842).
+__device__ __forceinline__ double FaceGrad(const double* __restrict__ gv, int c, int nb,
+                                           bool inner) {
+  return inner ? Mid(gv[nb < c ? nb : c], gv[nb < c ? c : nb]) : gv[c];
+}
+
+// Explicit viscosity + force assembly + momentum rows (fluid.hpp:835-874,
+// conv_diff.hpp:130-243).  Row coefficients are shared by both velocity
+// components (their BCs, fluxes and densities coincide); constants differ.
+__global__ void __launch_bounds__(kBlock)
+KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+  const int cw = c - 1, ce = c + 1, cs = c - nx, cn = c + nx;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  const double V = g.V;
+  const double muw = in.ffmu[fw], mue = in.ffmu[fe], mus = in.ffmu[fs], mun = in.ffmu[fn];
+
+  // --- explicit viscosity (transpose stress), component by component
+  double fox = 0., foy = 0.;
+  {
+    // d = 0: weights mu_f * S_out[0] on the W/E faces.
+    double ww = muw * (-g.sx), we = mue * g.sx;
+    double gwx = FaceGrad(in.g0x, c, cw, hw), gex = FaceGrad(in.g0x, c, ce, he);
+    double gwy = FaceGrad(in.g0y, c, cw, hw), gey = FaceGrad(in.g0y, c, ce, he);
+    double sx = gwx * ww;
+    sx += gex * we;
+    double sy = gwy * ww;
+    sy += gey * we;
+    fox += sx / V;
+    foy += sy / V;
+    // d = 1: weights mu_f * S_out[1] on the S/N faces.
+    double ws = mus * (-g.sy), wn = mun * g.sy;
+    double gsx = FaceGrad(in.g1x, c, cs, hs), gnx = FaceGrad(in.g1x, c, cn, hn);
+    double gsy = FaceGrad(in.g1y, c, cs, hs), gny = FaceGrad(in.g1y, c, cn, hn);
+    sx = gsx * ws;
+    sx += gnx * wn;
+    sy = gsy * ws;
+    sy += gny * wn;
+    fox += sx / V;
+    foy += sy / V;
+  }
+  // --- force assembly: += -grad p + restored force (+ zero surface tension
+  //     and zero volume-source compensation on this path)
+  fox += in.gpx[c] * (-1.) + in.frx[c];
+  foy += in.gpy[c] * (-1.) + in.fry[c];
+
+  // --- face expressions.  Convective (upwind deferred, threshold 1e-10,
+  //     solver.hpp:201-243) and diffusive (solver.hpp:283-341).
+  const int nbi[4] = {cw, ce, cs, cn};
+  const int fid[4] = {fw, fe, fs, fn};
+  const bool inner[4] = {hw, he, hs, hn};
+  const double mu[4] = {muw, mue, mus, mun};
+  const double outf[4] = {-1., 1., -1., 1.};
+  const int side[4] = {kLeft, kRight, kBottom, kTop};
+  double Cc = 0., Dc = 0., Cnb[4] = {0, 0, 0, 0}, Dnb[4] = {0, 0, 0, 0};
+  double Ck0 = 0., Ck1 = 0., Dk0 = 0., Dk1 = 0.;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool xface = k < 2;
+    const double A = xface ? g.ax : g.ay;
+    const double F = in.flux[fid[k]];
+    const double o = outf[k];
+    const bool self_m = (k == 1 || k == 3);
+    if (inner[k]) {
+      const int cm = self_m ? c : nbi[k];
+      const int cp = self_m ? nbi[k] : c;
+      double am, ap, ak0, ak1;
+      if (F > 1e-10) {
+        am = 1.;
+        ap = 0.;
+        ak0 = -(xface ? in.g0x[cm] * g.tx0 : in.g0y[cm] * g.ty0);
+        ak1 = -(xface ? in.g1x[cm] * g.tx0 : in.g1y[cm] * g.ty0);
+      } else if (F < -1e-10) {
+        am = 0.;
+        ap = 1.;
+        ak0 = -(xface ? in.g0x[cp] * g.tx1 : in.g0y[cp] * g.ty1);
+        ak1 = -(xface ? in.g1x[cp] * g.tx1 : in.g1y[cp] * g.ty1);
+      } else {
+        am = 0.5;
+        ap = 0.5;
+        ak0 = 0.;
+        ak1 = 0.;
+      }
+      const double cam = am * F, cap = ap * F;
+      const double al = 1. / (xface ? g.dnx : g.dny);
+      const double dcm = ((-al) * (-mu[k])) * A;
+      const double dcp = (al * (-mu[k])) * A;
+      const double dck = (0. * (-mu[k])) * A;
+      Cc += (self_m ? cam : cap) * o;
+      Cnb[k] = (self_m ? cap : cam) * o;
+      Dc += (self_m ? dcm : dcp) * o;
+      Dnb[k] = (self_m ? dcp : dcm) * o;
+      Ck0 += (ak0 * F) * o;
+      Ck1 += (ak1 * F) * o;
+      Dk0 += dck * o;
+      Dk1 += dck * o;
+    } else {
+      const int sd = side[k];
+      const double al = g.bal[sd];
+      const double v0 = bc.vel[0][sd], v1 = bc.vel[1][sd];
+      Ck0 += (v0 * F) * o;
+      Ck1 += (v1 * F) * o;
+      Dc += (((-al) * (-mu[k])) * A) * o;
+      Dk0 += (((al * v0) * (-mu[k])) * A) * o;
+      Dk1 += (((al * v1) * (-mu[k])) * A) * o;
+    }
+  }
+  const double rho = in.rho[c];
+  const double cen = (Dc / V) + ((Cc / V + in.bdf2) * rho);
+  double nb[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nb[k] = (Dnb[k] / V) + ((Cnb[k] / V) * rho);
+  // constants, then the delta form (row evaluated at the previous iterate in
+  // the order S, W, C, E, N) and under-relaxation of the diagonal
+  const double un0 = in.bdf0 * in.utp0[c] + in.bdf1 * in.utc0[c];
+  const double un1 = in.bdf0 * in.utp1[c] + in.bdf1 * in.utc1[c];
+  double k0 = ((Dk0 / V) + ((Ck0 / V + un0) * rho)) + (-fox);
+  double k1 = ((Dk1 / V) + ((Ck1 / V + un1) * rho)) + (-foy);
+  const double* u0 = in.u0;
+  const double* u1 = in.u1;
+  if (hs) { k0 += u0[cs] * nb[2]; k1 += u1[cs] * nb[2]; }
+  if (hw) { k0 += u0[cw] * nb[0]; k1 += u1[cw] * nb[0]; }
+  k0 += u0[c] * cen;
+  k1 += u1[c] * cen;
+  if (he) { k0 += u0[ce] * nb[1]; k1 += u1[ce] * nb[1]; }
+  if (hn) { k0 += u0[cn] * nb[3]; k1 += u1[cn] * nb[3]; }
+  out.s[c] = hs ? nb[2] : 0.;
+  out.w[c] = hw ? nb[0] : 0.;
+  out.c[c] = cen / in.relax;
+  out.e[c] = he ? nb[1] : 0.;
+  out.n[c] = hn ? nb[3] : 0.;
+  out.k0[c] = k0;
+  out.k1[c] = k1;
+}
+
+// Diagonal coefficient a_P = mean of the rows' CoeffSum (fluid.hpp:876-884).
+__global__ void __launch_bounds__(kBlock)
+KDiag(Geo g, const double* __restrict__ s, const double* __restrict__ w,
+      const double* __restrict__ cc, const double* __restrict__ e, const double* __restrict__ n,
+      double* __restrict__ ap) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  double cs = 0.;
+  if (j > 0) cs += s[c];
+  if (i > 0) cs += w[c];
+  cs += cc[c];
+  if (i < g.nx - 1) cs += e[c];
+  if (j < g.ny - 1) cs += n[c];
+  double sum = 0.;
+  sum += cs;
+  sum += cs;
+  ap[c] = sum / 2.;
+}
+
+// Momentum interpolation (Rhie-Chow) and volume-flux rows
+// (fluid.hpp:886-970): flux* and the face coefficient -A/(d a_f).
+__global__ void __launch_bounds__(kBlock)
+KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict__ u1,
+          const double* __restrict__ gpx, const double* __restrict__ gpy,
+          const double* __restrict__ ffrx, const double* __restrict__ ffry,
+          const double* __restrict__ ffex, const double* __restrict__ ffey,
+          const double* __restrict__ p, const double* __restrict__ ap, double rc, double mvx,
+          double mvy, double* __restrict__ fstar, double* __restrict__ fcoef) {
+  const int nf = g.nfx + g.nx * (g.ny + 1);
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const bool xface = f < g.nfx;
+  int cm, cp, sd;
+  if (xface) {
+    int i = f % (g.nx + 1), j = f / (g.nx + 1);
+    cm = (i > 0) ? (i - 1) + g.nx * j : -1;
+    cp = (i < g.nx) ? i + g.nx * j : -1;
+    sd = (i == 0) ? kLeft : kRight;
+  } else {
+    int r = f - g.nfx;
+    int i = r % g.nx, j = r / g.nx;
+    cm = (j > 0) ? i + g.nx * (j - 1) : -1;
+    cp = (j < g.ny) ? i + g.nx * j : -1;
+    sd = (j == 0) ? kBottom : kTop;
+  }
+  const double S = xface ? g.sx : g.sy;
+  double fs, fc = 0.;
+  if (cm >= 0 && cp >= 0) {
+    // u*_f . S with u*_f = (u[cm] + u[cp]) / 2 component-wise
+    const double ufn = xface ? Mid(u0[cm], u0[cp]) : Mid(u1[cm], u1[cp]);
+    const double vfi = ufn * S;
+    const double gpf = xface ? Mid(gpx[cm], gpx[cp]) : Mid(gpy[cm], gpy[cp]);
+    const double frf = xface ? ffrx[f] : ffry[f];
+    const double wide = (gpf - frf) * S;
+    const double dn = xface ? g.dnx : g.dny;
+    const double A = xface ? g.ax : g.ay;
+    const double fef = xface ? ffex[f] : ffey[f];
+    const double compact = (p[cp] - p[cm]) / dn * A - fef * S;
+    const double af = Mid(ap[cm], ap[cp]);
+    fs = vfi + rc * (wide - compact) / af + 0;
+    fc = -A / (dn * af);
+  } else {
+    const double vb = xface ? bc.vel[0][sd] : bc.vel[1][sd];
+    fs = vb * S;
+  }
+  fs -= (xface ? mvx : mvy) * S;
+  fstar[f] = fs;
+  fcoef[f] = fc;
+}
+
+// Pressure-correction rows (fluid.hpp:972-1027) with an optional fixed
+// pressure cell (Expression::SetKnownValue, linear.hpp:237-249).
+__global__ void __launch_bounds__(kBlock)
+KPressureRows(Geo g, const double* __restrict__ fstar, const double* __restrict__ fcoef,
+              int pinned, double pval, double* __restrict__ s, double* __restrict__ w,
+              double* __restrict__ cc, double* __restrict__ e, double* __restrict__ n,
+              double* __restrict__ k) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  double cen = 0., kon = 0.;
+  double nw = 0., ne = 0., ns = 0., nn = 0.;
+  if (i > 0) { cen += fcoef[fw] * -1.; nw = (-fcoef[fw]) * -1.; }
+  kon += fstar[fw] * -1.;
+  if (i < nx - 1) { cen += (-fcoef[fe]) * 1.; ne = fcoef[fe] * 1.; }
+  kon += fstar[fe] * 1.;
+  if (j > 0) { cen += fcoef[fs] * -1.; ns = (-fcoef[fs]) * -1.; }
+  kon += fstar[fs] * -1.;
+  if (j < ny - 1) { cen += (-fcoef[fn]) * 1.; nn = fcoef[fn] * 1.; }
+  kon += fstar[fn] * 1.;
+  kon += -(0. * g.V);  // volume source is zero on this path
+  if (pinned >= 0) {
+    if (c == pinned) {
+      nw = ne = ns = nn = 0.;
+      cen = 1.;
+      kon = -pval;
+    } else {
+      if (c + nx == pinned) { kon += pval * nn; nn = 0.; }
+      if (c + 1 == pinned && i < nx - 1) { kon += pval * ne; ne = 0.; }
+      if (c == pinned + 1 && i > 0) { kon += pval * nw; nw = 0.; }
+      if (c == pinned + nx) { kon += pval * ns; ns = 0.; }
+    }
+  }
+  s[c] = ns;
+  w[c] = nw;
+  cc[c] = cen;
+  e[c] = ne;
+  n[c] = nn;
+  k[c] = kon;
+}
+
+// Correction (fluid.hpp:1033-1057) for cells, with the convergence
+// indicator max_c |u_curr - u_prev| (solver.hpp:804-813) reduced into *rs
+// (non-negative doubles ordered as unsigned integers).
+__global__ void __launch_bounds__(kBlock)
+KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ ap,
+              const double* __restrict__ pprev, double alpha_p, double* __restrict__ pcurr,
+              double* __restrict__ u0, double* __restrict__ u1, const double* __restrict__ up0,
+              const double* __restrict__ up1, unsigned long long* __restrict__ rs) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.;
+  if (c < g.nx * g.ny) {
+    const int i = c % g.nx, j = c / g.nx;
+    pcurr[c] = pprev[c] + alpha_p * pc[c];
+    double f[4], gx, gy;
+    CellFaces<2>(g, pc, i, j, nullptr, f);
+    GradFromFaces(g, f, gx, gy);
+    const double a = -ap[c];
+    const double n0 = u0[c] + gx / a;
+    const double n1 = u1[c] + gy / a;
+    u0[c] = n0;
+    u1[c] = n1;
+    const double d0 = up0[c] - n0, d1 = up1[c] - n1;
+    double s2 = 0.;
+    s2 += d0 * d0;
+    s2 += d1 * d1;
+    d = sqrt(s2);
+    if (!(d == d)) d = 0.;  // std::max(res, NaN) keeps res
+  }
+  d = WaveMax(d);
+  __shared__ double red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, red[w]);
+    atomicMax(rs, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Divergence-free volume flux: the flux row evaluated at p' (fluid.hpp:1053-1056).
+__global__ void __launch_bounds__(kBlock)
+KCorrectFaces(Geo g, const double* __restrict__ fstar, const double* __restrict__ fcoef,
+              const double* __restrict__ pc, double* __restrict__ flux) {
+  const int nf = g.nfx + g.nx * (g.ny + 1);
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int cm, cp;
+  if (f < g.nfx) {
+    int i = f % (g.nx + 1), j = f / (g.nx + 1);
+    cm = (i > 0) ? (i - 1) + g.nx * j : -1;
+    cp = (i < g.nx) ? i + g.nx * j : -1;
+  } else {
+    int r = f - g.nfx;
+    int i = r % g.nx, j = r / g.nx;
+    cm = (j > 0) ? i + g.nx * (j - 1) : -1;
+    cp = (j < g.ny) ? i + g.nx * j : -1;
+  }
+  double v = fstar[f];
+  if (cm >= 0 && cp >= 0) {
+    v += pc[cm] * (-fcoef[f]);
+    v += pc[cp] * fcoef[f];
+  }
+  flux[f] = v;
+}
+
+// Initial volume flux u_f . S - meshvel . S (fluid.hpp:770-784).
+__global__ void __launch_bounds__(kBlock)
+KInitialFlux(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict__ u1,
+             double mvx, double mvy, double* __restrict__ flux) {
+  const int nf = g.nfx + g.nx * (g.ny + 1);
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const bool xface = f < g.nfx;
+  int cm, cp, sd;
+  if (xface) {
+    int i = f % (g.nx + 1), j = f / (g.nx + 1);
+    cm = (i > 0) ? (i - 1) + g.nx * j : -1;
+    cp = (i < g.nx) ? i + g.nx * j : -1;
+    sd = (i == 0) ? kLeft : kRight;
+  } else {
+    int r = f - g.nfx;
+    int i = r % g.nx, j = r / g.nx;
+    cm = (j > 0) ? i + g.nx * (j - 1) : -1;
+    cp = (j < g.ny) ? i + g.nx * j : -1;
+    sd = (j == 0) ? kBottom : kTop;
+  }
+  const double S = xface ? g.sx : g.sy;
+  double un;
+  if (cm >= 0 && cp >= 0)
+    un = xface ? Mid(u0[cm], u0[cp]) : Mid(u1[cm], u1[cp]);
+  else
+    un = xface ? bc.vel[0][sd] : bc.vel[1][sd];
+  double v = un * S;
+  v -= (xface ? mvx : mvy) * S;
+  flux[f] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Advection (advection.hpp:440-478): gradient of the zero-derivative face
+// interpolation, then Superbee face values and the explicit update.
+
+__global__ void __launch_bounds__(kBlock)
+KAdvGrad(Geo g, const double* __restrict__ u, double* __restrict__ gx, double* __restrict__ gy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  double f[4];
+  CellFaces<1>(g, u, c % g.nx, c / g.nx, nullptr, f);
+  GradFromFaces(g, f, gx[c], gy[c]);
+}
+
+__device__ __forceinline__ double Superbee(double p, double q) {
+  if (p > 0. && q > 0.) return fmax(fmin(2 * p, q), fmin(p, 2 * q));
+  if (p < 0. && q < 0.) return -fmax(fmin(-2 * p, -q), fmin(-p, -2 * q));
+  return 0.;
+}
+
+// Superbee face value between P (cell 0) and E (cell 1) (solver.hpp:576-592).
+__device__ __forceinline__ double SuperbeeFace(double uP, double uE, double gPn, double gEn,
+                                               double r0, double r1, double vf) {
+  if (vf > 1e-8) return uP + 0.5 * Superbee(uE - uP, -4. * (gPn * r0) - (uE - uP));
+  if (vf < -1e-8) return uE - 0.5 * Superbee(uE - uP, 4. * (gEn * r1) - (uE - uP));
+  return 0.5 * (uP + uE);
+}
+
+__global__ void __launch_bounds__(kBlock)
+KAdvUpdate(Geo g, const double* __restrict__ u, const double* __restrict__ gx,
+           const double* __restrict__ gy, const double* __restrict__ flux, double dt,
+           double* __restrict__ un) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  const double uc = u[c];
+  double vw = flux[fw], ve = flux[fe], vs = flux[fs], vn = flux[fn];
+  double uw = i > 0 ? SuperbeeFace(u[c - 1], uc, gx[c - 1], gx[c], g.tx0, g.tx1, vw) : uc;
+  double ue = i < nx - 1 ? SuperbeeFace(uc, u[c + 1], gx[c], gx[c + 1], g.tx0, g.tx1, ve) : uc;
+  double us = j > 0 ? SuperbeeFace(u[c - nx], uc, gy[c - nx], gy[c], g.ty0, g.ty1, vs) : uc;
+  double uN = j < ny - 1 ? SuperbeeFace(uc, u[c + nx], gy[c], gy[c + nx], g.ty0, g.ty1, vn) : uc;
+  double fsum = 0.;
+  fsum += uw * vw * -1.;
+  fsum += ue * ve * 1.;
+  fsum += us * vs * -1.;
+  fsum += uN * vn * 1.;
+  un[c] = uc + -dt / g.V * fsum;
+}
+
+// ---------------------------------------------------------------------------
+// Fluid properties (hydro2d.hpp:980-997, 1221-1246, 1308-1318).
+__global__ void __launch_bounds__(kBlock)
+KProperties(Geo g, Phases ph, const double* __restrict__ pd0, const double* __restrict__ pd1,
+            const double* __restrict__ pd2, double* __restrict__ a0, double* __restrict__ a1,
+            double* __restrict__ a2, double* __restrict__ rho, double* __restrict__ mu,
+            double* __restrict__ fx, double* __restrict__ fy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  double al[3] = {0, 0, 0};
+  const double* pd[3] = {pd0, pd1, pd2};
+  double sum = 0.;
+  for (int i = 0; i < ph.n; ++i) {
+    al[i] = pd[i][c] / ph.rho[i];
+    sum += al[i];
+  }
+  for (int i = 0; i < ph.n; ++i) al[i] /= sum;
+  double r = 0., m = 0.;
+  for (int i = 0; i < ph.n; ++i) r += ph.rho[i] * al[i];
+  for (int i = 0; i < ph.n; ++i) m += ph.mu[i] * al[i];
+  a0[c] = al[0];
+  if (ph.n > 1) a1[c] = al[1];
+  if (ph.n > 2) a2[c] = al[2];
+  rho[c] = r;
+  mu[c] = m;
+  fx[c] = ph.grav[0] * r + ph.force[0];
+  fy[c] = ph.grav[1] * r + ph.force[1];
+}
+
+// One pass of GetSmoothField (solver.hpp:636-656): average of the
+// zero-derivative face interpolation.
+__global__ void __launch_bounds__(kBlock)
+KSmooth(Geo g, const double* __restrict__ u, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  double f[4];
+  CellFaces<1>(g, u, c % g.nx, c / g.nx, nullptr, f);
+  double s = 0.;
+  s += f[0];
+  s += f[1];
+  s += f[2];
+  s += f[3];
+  out[c] = s / 4.;
+}
+
+// Initial partial density of phase 0 compensating the others (hydro2d.hpp:556-565).
+__global__ void __launch_bounds__(kBlock)
+KPhase0Init(Geo g, Phases ph, const double* __restrict__ pd1, const double* __restrict__ pd2,
+            double* __restrict__ pd0) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  double vs = 0.;
+  if (ph.n > 1) vs += pd1[c] / ph.rho[1];
+  if (ph.n > 2) vs += pd2[c] / ph.rho[2];
+  pd0[c] = (1. - vs) * ph.rho[0];
+}
+
+// NaN test of FinishStep (fluid.hpp:1164-1166, conv_diff.hpp:255-257).
+__global__ void __launch_bounds__(kBlock)
+KNanCheck(const double* __restrict__ a, const double* __restrict__ b,
+          const double* __restrict__ c3, int n, int* __restrict__ flag) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double v = a[c] * 0. + b[c] * 0. + c3[c] * 0.;
+  if (!(v == 0.)) atomicOr(flag, 1);
+}
+
+// CalcStat partial sums (hydro2d.hpp:1432-1467) for one phase: per block,
+// volume, centroid, velocity sums and pd min/max; a second pass folds the
+// block partials in a fixed order.
+__global__ void __launch_bounds__(kBlock)
+KStatsPartial(Geo g, const double* __restrict__ xn, const double* __restrict__ yn,
+              const double* __restrict__ alpha, const double* __restrict__ pd,
+              const double* __restrict__ u0, const double* __restrict__ u1,
+              double* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[7] = {0, 0, 0, 0, 0, 1e10, -1e10};
+  if (c < g.nx * g.ny) {
+    const int i = c % g.nx, j = c / g.nx;
+    double cx = 0., cy = 0.;
+    cx += xn[i]; cy += yn[j];
+    cx += xn[i + 1]; cy += yn[j];
+    cx += xn[i + 1]; cy += yn[j + 1];
+    cx += xn[i]; cy += yn[j + 1];
+    cx /= 4.;
+    cy /= 4.;
+    const double a = alpha[c];
+    const double w = a * g.V;
+    v[0] = a * g.V;
+    v[1] = cx * w;
+    v[2] = cy * w;
+    v[3] = u0[c] * w;
+    v[4] = u1[c] * w;
+    v[5] = pd[c];
+    v[6] = pd[c];
+  }
+  __shared__ double red[7][kBlock / 64];
+  for (int q = 0; q < 7; ++q) {
+    double x = q < 5 ? WaveSum(v[q]) : (q == 5 ? WaveMin(v[q]) : WaveMax(v[q]));
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const int q = threadIdx.x;
+    double x = red[q][0];
+    for (int w = 1; w < kBlock / 64; ++w)
+      x = q < 5 ? x + red[q][w] : (q == 5 ? fmin(x, red[q][w]) : fmax(x, red[q][w]));
+    part[blockIdx.x * 7 + q] = x;
+  }
+}
+
+__global__ void KStatsFinal(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= 7) return;
+  double x = q < 5 ? 0. : (q == 5 ? 1e10 : -1e10);
+  for (int b = 0; b < nblocks; ++b) {
+    double y = part[b * 7 + q];
+    x = q < 5 ? x + y : (q == 5 ? fmin(x, y) : fmax(x, y));
+  }
+  out[q] = x;
+}
+
+}  // namespace
+
+// ---- launchers --------------------------------------------------------------
+
+void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
+                      const double* mu, double* ffex, double* ffey, double* ffmu) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KForceFaces<<<Grid(nf), kBlock, 0, st>>>(g, fx, fy, mu, ffex, ffey, ffmu);
+}
+void LaunchRestoredForce(hipStream_t st, const Geo& g, const double* ffex, const double* ffey,
+                         double* frx, double* fry) {
+  KRestoredForce<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ffex, ffey, frx, fry);
+}
+void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const double* ax, const double* ay,
+                           double* fx, double* fy) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KFacesDeriv0Pair<<<Grid(nf), kBlock, 0, st>>>(g, ax, ay, fx, fy);
+}
+void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
+                     const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
+                     double* g0y, double* g1x, double* g1y) {
+  KCellGrads<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x, g0y,
+                                                          g1x, g1y);
+}
+void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a) {
+  MomIn in{a.u0,   a.u1,   a.g0x,  a.g0y,  a.g1x,  a.g1y,  a.gpx,  a.gpy,  a.frx, a.fry,
+           a.ffmu, a.flux, a.rho,  a.utp0, a.utp1, a.utc0, a.utc1, a.bdf0, a.bdf1, a.bdf2,
+           a.relax};
+  MomOut out{a.s, a.w, a.c, a.e, a.n, a.k0, a.k1};
+  KMomentum<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, in, out);
+}
+void LaunchDiag(hipStream_t st, const Geo& g, const double* s, const double* w, const double* c,
+                const double* e, const double* n, double* ap) {
+  KDiag<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, s, w, c, e, n, ap);
+}
+void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                    const double* u1, const double* gpx, const double* gpy, const double* ffrx,
+                    const double* ffry, const double* ffex, const double* ffey, const double* p,
+                    const double* ap, double rc, double mvx, double mvy, double* fstar,
+                    double* fcoef) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KRhieChow<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap,
+                                         rc, mvx, mvy, fstar, fcoef);
+}
+void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
+                        int pinned, double pval, double* s, double* w, double* c, double* e,
+                        double* n, double* k) {
+  KPressureRows<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, fstar, fcoef, pinned, pval, s, w,
+                                                             c, e, n, k);
+}
+void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
+                        const double* pprev, double alpha_p, double* pcurr, double* u0,
+                        double* u1, const double* up0, const double* up1,
+                        unsigned long long* rs) {
+  KCorrectCells<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, pc, ap, pprev, alpha_p, pcurr, u0,
+                                                             u1, up0, up1, rs);
+}
+void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
+                        const double* pc, double* flux) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KCorrectFaces<<<Grid(nf), kBlock, 0, st>>>(g, fstar, fcoef, pc, flux);
+}
+void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                       const double* u1, double mvx, double mvy, double* flux) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KInitialFlux<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, mvx, mvy, flux);
+}
+void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, double* gy) {
+  KAdvGrad<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, gx, gy);
+}
+void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
+                     const double* gy, const double* flux, double dt, double* un) {
+  KAdvUpdate<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, gx, gy, flux, dt, un);
+}
+void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
+                      double* const alpha[3], double* rho, double* mu, double* fx, double* fy) {
+  KProperties<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ph, pd[0], pd[1], pd[2], alpha[0],
+                                                           alpha[1], alpha[2], rho, mu, fx, fy);
+}
+void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out) {
+  KSmooth<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, out);
+}
+void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const double* pd1,
+                      const double* pd2, double* pd0) {
+  KPhase0Init<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ph, pd1, pd2, pd0);
+}
+void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
+                    int* flag) {
+  KNanCheck<<<Grid(n), kBlock, 0, st>>>(a, b, c, n, flag);
+}
+int StatsBlocks(const Geo& g) { return int((long(g.nx) * g.ny + kBlock - 1) / kBlock); }
+void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,
+                 const double* alpha, const double* pd, const double* u0, const double* u1,
+                 double* part, double* out7) {
+  int nb = StatsBlocks(g);
+  KStatsPartial<<<nb, kBlock, 0, st>>>(g, xn, yn, alpha, pd, u0, u1, part);
+  KStatsFinal<<<1, 64, 0, st>>>(part, nb, out7);
+}
+
+}  // namespace hg

@@ -1,0 +1,62 @@
+"""Case scripts shared by the tests (.hydroconf text on top of the
+defaults of examples/general.hydroconf, which hg_config_default holds)."""
+
+
+def cavity(n=64, dt=1e-3, viscosity=0.01, solver_p="gauss_seidel", solver_v="lu", extra=""):
+    """Lid-driven cavity, unit square, lid velocity 1 (Re = 1/viscosity)."""
+    return f"""
+set int Nx {n}
+set int Ny {n}
+set double dt {dt}
+set double viscosity_0 {viscosity}
+set string condition_top "wall 1 0"
+set string linear_solver_pressure {solver_p}
+set string linear_solver_velocity {solver_v}
+{extra}
+"""
+
+
+def dam(nx=256, ny=128, dt=1e-4, solver_p="gauss_seidel", solver_v="lu", extra=""):
+    """Broken dam, 2x1 domain, water column 0.5 x 0.5 in the left corner:
+    density ratio 1000, viscosity ratio 100, gravity -9.81."""
+    return f"""
+set vect A (0, 0)
+set vect B (2, 1)
+set int Nx {nx}
+set int Ny {ny}
+set int num_phases 2
+set double density_0 1
+set double density_1 1000
+set double viscosity_0 1e-5
+set double viscosity_1 1e-3
+set vect gravity (0, -9.81)
+set vect A1 (0, 0)
+set vect B1 (0.5, 0.5)
+set double dt {dt}
+set string linear_solver_pressure {solver_p}
+set string linear_solver_velocity {solver_v}
+{extra}
+"""
+
+
+def rt(nx=1024, ny=2048, dt=2e-4, solver_p="cg", solver_v="lu_rb", precision="fp64", extra=""):
+    """Rayleigh-Taylor, [0,1]x[0,2], Atwood 0.5 (heavy 3 on top of light 1),
+    g = (0,-1).  The interface is set through the partial-density fields."""
+    return f"""
+set vect A (0, 0)
+set vect B (1, 2)
+set int Nx {nx}
+set int Ny {ny}
+set int num_phases 2
+set double density_0 1
+set double density_1 3
+set double viscosity_0 1e-3
+set double viscosity_1 1e-3
+set vect gravity (0, -1)
+set int initial_volume_fraction_smooth_times 0
+set double dt {dt}
+set string linear_solver_pressure {solver_p}
+set string linear_solver_velocity {solver_v}
+set string precision_gpu {precision}
+{extra}
+"""
