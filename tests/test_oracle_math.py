@@ -1,0 +1,213 @@
+"""Pins for the oracle that the mathematics fixes (CPU only).
+
+Each check is chosen so that a plausible slip (a dropped term, a sign, a
+transposed index) fails it: closed forms of the BDF weights and the
+uniform-mesh geometry, brute-force dense sequential sweeps on small systems,
+dense direct solves for CG, and invariants of the SIMPLE step.
+"""
+import numpy as np
+import pytest
+
+from hydro_amd import capi
+from oracle import pyoracle
+from tests import cases
+
+
+def test_bdf_coefficients_closed_form():
+    dt = 1e-3
+    c2 = pyoracle.bdf_coeffs(dt, True)
+    np.testing.assert_allclose(c2, np.array([0.5, -2.0, 1.5]) / dt, rtol=1e-14)
+    c1 = pyoracle.bdf_coeffs(dt, False)
+    np.testing.assert_allclose(c1, np.array([0.0, -1.0, 1.0]) / dt, rtol=1e-14)
+
+
+def test_uniform_geometry():
+    o = pyoracle.Oracle(capi.parse_config(cases.dam(16, 8)))  # [0,2]x[0,1], h = 1/8
+    (cx, cy, vol), (fx, fy, fa, sx, sy) = o.geometry()
+    h = 1.0 / 8
+    np.testing.assert_array_equal(vol, np.full(16 * 8, h * h))
+    i = np.arange(16 * 8) % 16
+    j = np.arange(16 * 8) // 16
+    np.testing.assert_array_equal(cx, (i + 0.5) * h)
+    np.testing.assert_array_equal(cy, (j + 0.5) * h)
+    nfx = 17 * 8
+    np.testing.assert_array_equal(fa, np.full(fa.size, h))
+    np.testing.assert_array_equal(sx[:nfx], np.full(nfx, h))      # x-faces: S = (h, 0)
+    np.testing.assert_array_equal(sy[nfx:], np.full(fa.size - nfx, h))  # y-faces: S = (0, h)
+    assert np.all(sy[:nfx] == 0) and np.all(sx[nfx:] == 0)
+
+
+def random_rows(nx, ny, seed, spd=False):
+    rng = np.random.default_rng(seed)
+    n = nx * ny
+    s, w, e, nn = (-rng.uniform(0.2, 1.0, n) for _ in range(4))
+    i = np.arange(n) % nx
+    j = np.arange(n) // nx
+    s[j == 0] = 0
+    w[i == 0] = 0
+    e[i == nx - 1] = 0
+    nn[j == ny - 1] = 0
+    if spd:  # symmetric: w[r] = e[r-1], s[r] = n[r-nx]
+        w[1:] = e[:-1]
+        w[i == 0] = 0
+        s[nx:] = nn[:-nx]
+    c = -(s + w + e + nn) + rng.uniform(0.05, 0.5, n)
+    k = rng.normal(size=n)
+    return s, w, c, e, nn, k
+
+
+def dense(nx, ny, s, w, c, e, nn):
+    n = nx * ny
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, r] = c[r]
+        if r >= nx: A[r, r - nx] = s[r]
+        if r % nx: A[r, r - 1] = w[r]
+        if r % nx != nx - 1: A[r, r + 1] = e[r]
+        if r + nx < n: A[r, r + nx] = nn[r]
+    return A
+
+
+def test_lu_is_one_symmetric_gauss_seidel_sweep():
+    nx, ny = 5, 4
+    s, w, c, e, nn, k = random_rows(nx, ny, 1)
+    A = dense(nx, ny, s, w, c, e, nn)
+    b = -k
+    D = np.diag(np.diag(A))
+    L = np.tril(A, -1)
+    U = np.triu(A, 1)
+    expect = np.linalg.solve(D + U, D @ np.linalg.solve(D + L, b))
+    x, _ = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, capi.SOLVERS["lu"])
+    np.testing.assert_allclose(x, expect, rtol=1e-12, atol=1e-14)
+
+
+def test_gauss_seidel_sweeps_match_dense_loop():
+    nx, ny = 4, 4
+    s, w, c, e, nn, k = random_rows(nx, ny, 2)
+    A = dense(nx, ny, s, w, c, e, nn)
+    omega, sweeps = 1.3, 3
+    x = np.zeros(nx * ny)
+    for _ in range(sweeps):
+        for r in range(nx * ny):
+            off = A[r] @ x - A[r, r] * x[r]
+            val = -(k[r] + off) / A[r, r]
+            x[r] += (val - x[r]) * omega
+    # tolerance 0 and limit sweeps-1 -> exactly `sweeps` sweeps
+    xo, st = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, capi.SOLVERS["gauss_seidel"],
+                                    tol=0.0, max_iters=sweeps - 1, omega=omega)
+    assert st.iterations == sweeps
+    np.testing.assert_allclose(xo, x, rtol=1e-12, atol=1e-14)
+
+
+def test_jacobi_sweeps_match_dense_loop():
+    nx, ny = 4, 3
+    s, w, c, e, nn, k = random_rows(nx, ny, 3)
+    A = dense(nx, ny, s, w, c, e, nn)
+    omega, sweeps = 0.7, 4
+    x = np.zeros(nx * ny)
+    for _ in range(sweeps):
+        x = x + omega * ((-k - (A @ x - np.diag(A) * x)) / np.diag(A) - x)
+    xo, st = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, capi.SOLVERS["jacobi"],
+                                    tol=0.0, max_iters=sweeps - 1, omega=omega)
+    np.testing.assert_allclose(xo, x, rtol=1e-12, atol=1e-14)
+
+
+def test_lu_rb_is_red_black_ordered_sgs():
+    nx, ny = 6, 5
+    s, w, c, e, nn, k = random_rows(nx, ny, 4)
+    A = dense(nx, ny, s, w, c, e, nn)
+    i = np.arange(nx * ny) % nx
+    j = np.arange(nx * ny) // nx
+    order = np.concatenate([np.where((i + j) % 2 == 0)[0], np.where((i + j) % 2 == 1)[0]])
+    P = A[np.ix_(order, order)]
+    b = -k[order]
+    D = np.diag(np.diag(P))
+    y = np.linalg.solve(D + np.triu(P, 1), D @ np.linalg.solve(D + np.tril(P, -1), b))
+    expect = np.empty_like(y)
+    expect[order] = y
+    x, _ = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, capi.SOLVERS["lu_rb"])
+    np.testing.assert_allclose(x, expect, rtol=1e-12, atol=1e-14)
+
+
+def test_cg_matches_dense_solve():
+    nx, ny = 8, 8
+    s, w, c, e, nn, k = random_rows(nx, ny, 5, spd=True)
+    A = dense(nx, ny, s, w, c, e, nn)
+    assert np.allclose(A, A.T)
+    x, st = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, capi.SOLVERS["cg"], tol=1e-13,
+                                   max_iters=1000)
+    np.testing.assert_allclose(x, np.linalg.solve(A, -k), rtol=1e-10)
+
+
+def test_poisson_cg_singular_neumann():
+    rng = np.random.default_rng(6)
+    nx, ny = 12, 10
+    kx = rng.uniform(0.1, 2.0, (nx + 1) * ny)
+    ky = rng.uniform(0.1, 2.0, nx * (ny + 1))
+    kx.reshape(ny, nx + 1)[:, [0, nx]] = 0.0
+    ky.reshape(ny + 1, nx)[[0, ny], :] = 0.0
+    rhs = rng.normal(size=nx * ny)
+    p, st = pyoracle.poisson_solve(nx, ny, kx, ky, rhs, tol=1e-12)
+    # dense operator (A p)_c = sum k_f (p_c - p_nb)
+    n = nx * ny
+    A = np.zeros((n, n))
+    for jj in range(ny):
+        for ii in range(nx):
+            r = ii + nx * jj
+            for kf, nb in ((kx[ii + (nx + 1) * jj], r - 1), (kx[ii + 1 + (nx + 1) * jj], r + 1),
+                           (ky[ii + nx * jj], r - nx), (ky[ii + nx * (jj + 1)], r + nx)):
+                if kf:
+                    A[r, r] += kf
+                    A[r, nb] -= kf
+    b = rhs - rhs.mean()
+    np.testing.assert_allclose(A @ p, b, atol=1e-9 * np.linalg.norm(b))
+
+
+def test_pressure_system_symmetric_zero_row_sums():
+    o = pyoracle.Oracle(capi.parse_config(cases.dam(16, 8)))
+    o.start_step()
+    o.iterate()
+    s, w, c, e, nn, k, _ = o.system(1)
+    nx = 16
+    np.testing.assert_allclose(w[1:][np.arange(1, 128) % nx != 0],
+                               e[:-1][np.arange(0, 127) % nx != nx - 1], rtol=0, atol=0)
+    np.testing.assert_array_equal(s[nx:], nn[:-nx])
+    rowsum = ((w + e) + s + nn) + c
+    assert np.max(np.abs(rowsum)) <= 1e-12 * np.max(np.abs(c))
+
+
+def test_quiescent_fluid_stays_at_rest():
+    o = pyoracle.Oracle(capi.parse_config(cases.cavity(16).replace('"wall 1 0"', '"wall 0 0"')))
+    for _ in range(3):
+        o.step()
+    assert np.all(o.get(capi.HG_FIELD_VELOCITY_X) == 0)
+    assert np.all(o.get(capi.HG_FIELD_VELOCITY_Y) == 0)
+
+
+def test_advection_conserves_volume_and_fractions_sum_to_one():
+    o = pyoracle.Oracle(capi.parse_config(cases.dam(32, 16)))
+    v0 = o.stats().volume[1]
+    for _ in range(5):
+        o.step()
+    st = o.stats()
+    assert abs(st.volume[1] - v0) <= 1e-12 * v0
+    a0 = o.get(capi.HG_FIELD_VOLUME_FRACTION_0)
+    a1 = o.get(capi.HG_FIELD_VOLUME_FRACTION_0 + 1)
+    np.testing.assert_allclose(a0 + a1, 1.0, atol=1e-15)
+
+
+def test_corrected_flux_is_divergence_free_when_solved_tightly():
+    text = cases.cavity(16, solver_p="cg", extra="set double pressure_tolerance_gpu 1e-13")
+    o = pyoracle.Oracle(capi.parse_config(text))
+    o.start_step()
+    o.iterate()
+    fl = o.get(capi.HG_FIELD_VOLUME_FLUX_ITER)
+    nx = ny = 16
+    nfx = (nx + 1) * ny
+    div = np.zeros(nx * ny)
+    for jj in range(ny):
+        for ii in range(nx):
+            r = ii + nx * jj
+            div[r] = (fl[ii + 1 + (nx + 1) * jj] - fl[ii + (nx + 1) * jj] +
+                      fl[nfx + ii + nx * (jj + 1)] - fl[nfx + ii + nx * jj])
+    assert np.max(np.abs(div)) < 1e-10 * np.max(np.abs(fl))
