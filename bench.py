@@ -1,0 +1,182 @@
+"""Benchmark of the hydro2d time step on one MI355X.
+
+Workload: Rayleigh-Taylor instability on the 1024x2048 grid (Atwood 0.5,
+heavy fluid on top, cosine interface of amplitude 0.01 plus seeded 1e-4
+noise).  A step is hydro::step (hydro2d.hpp:1531-1621): the SIMPLE loop
+(momentum rows, red-black symmetric GS velocity sweep, Rhie-Chow, pressure
+solve to 1e-8 relative residual, correction), 10 TVD advection sub-steps and
+the property update, all on the GPU with fields resident in HBM.
+
+Reported: steps/s of the headline path (mixed precision, learned
+preconditioner when trained) next to the plain FP64 path, and the pressure
+solve to 1e-8 (ms) for both on the same SIMPLE pressure system.
+`roofline` is for the dominant kernels (one CG iteration: SpMV + update),
+`cpu_baseline` times the reference module built from its sources
+(oracle/_ref/hydro_ref_fast) on one step of the same grid.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_path(text, steps, warmup, label):
+    from hydro_amd.hydro import Hydro
+    from hydro_amd import workloads
+    h = Hydro(text)
+    workloads.set_rt_state(h)
+    for i in range(warmup):
+        st = h.step()
+        log(f"{label} warmup step {i + 1}: SIMPLE {st.simple_iterations}, "
+            f"pressure iters {st.pressure_solve_iters_total}")
+    t0 = time.perf_counter()
+    iters = []
+    for i in range(steps):
+        st = h.step()
+        iters.append((st.simple_iterations, st.pressure_solve_iters_total))
+    dt = time.perf_counter() - t0
+    log(f"{label}: {steps} steps in {dt:.3f} s, (SIMPLE, pressure iters) per step {iters}")
+    return h, dt, iters
+
+
+def cpu_baseline(seconds_cap=150):
+    """One step of the reference module (its default solvers, OpenMP) on the
+    same 1024x2048 grid with the RT densities; flat interface at y = 1
+    (the reference has no way to read the perturbed state)."""
+    from oracle import pyoracle
+    exe = pyoracle.REF_BIN_FAST
+    if not os.path.exists(exe):
+        return None
+    from tests import cases
+    general = os.path.join(ROOT, "oracle", "_ref", "general.hydroconf")
+    text = open(general).read() + cases.rt(solver_p="gauss_seidel", solver_v="lu") + """
+set vect A1 (0, 1)
+set vect B1 (1, 2)
+set bool no_output 1
+"""
+    text = text.replace("set string precision_gpu fp64", "")
+    path = "/tmp/hydro_bench_ref.conf"
+    open(path, "w").write(text)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    try:
+        r = subprocess.run([exe, path, "1", "-", "0"], capture_output=True, text=True,
+                           timeout=seconds_cap, env=env)
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": "steps/s", "cores": cores, "kind": "reference",
+                "sample": f"1 step of 1024x2048 RT-like case did not finish in {seconds_cap} s"}
+    m = re.search(r"STEP s=1 iters=(\d+) t=([0-9.]+)", r.stdout)
+    if not m:
+        return None
+    t = float(m.group(2))
+    sweeps = [int(x) for x in re.findall(r"iter = (\d+), diff", r.stdout)]
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": cores, "kind": "reference",
+            "sample": f"1 step of the reference module (gauss_seidel + lu, OpenMP) on 1024x2048 "
+                      f"two-phase RT-like state: {m.group(1)} SIMPLE iterations, "
+                      f"SOR sweeps {sweeps}, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from hydro_amd import capi, workloads
+    from hydro_amd.hydro import Poisson
+    from tests import cases
+
+    nx, ny = 1024, 2048
+    ncell = nx * ny
+    text64 = cases.rt(nx, ny, solver_p="cg", solver_v="lu_rb", precision="fp64")
+    textmx = cases.rt(nx, ny, solver_p="fcg_learned", solver_v="lu_rb", precision="mixed")
+
+    h64, t64, it64 = run_path(text64, args.steps, args.warmup, "fp64")
+    # the SIMPLE pressure system of the last step, for the solve comparison
+    kf = h64.get(capi.HG_FIELD_PRESSURE_FACE_COEF)
+    rhs = h64.get(capi.HG_FIELD_PRESSURE_RHS)
+    h64.close()
+    hmx, tmx, itmx = run_path(textmx, args.steps, args.warmup, "mixed")
+    hmx.close()
+
+    nfx = (nx + 1) * ny
+    ps = Poisson(nx, ny, kf[:nfx], kf[nfx:])
+    solves = {}
+    for label, solver, prec in (("fp64", "cg", 0), ("cg_mixed", "cg", 1),
+                                ("learned_mixed", "fcg_learned", 1)):
+        ms, its, res = [], None, None
+        for rep in range(4):
+            _, st = ps.solve(rhs, solver=solver, tol=1e-8, precision=prec)
+            if rep > 0:
+                ms.append(st.ms)
+            its, res = st.iterations, st.rel_residual
+        solves[label] = {"ms": float(np.median(ms)), "iterations": int(its),
+                         "rel_residual": float(res)}
+        log(f"pressure solve {label}: {solves[label]}")
+    ps.close()
+
+    # roofline of one FP64 CG iteration: SpMV reads z, p_old, kx, ky and
+    # writes p, q (6 words); the update reads p, q, D^-1, x, r and writes
+    # x, r, z (8 words): 112 B per cell per iteration.
+    bytes_iter = 112.0 * ncell
+    ms_iter = solves["fp64"]["ms"] / solves["fp64"]["iterations"]
+    achieved = bytes_iter / (ms_iter * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "FP64 CG iteration (KSpmv + KUpdate), 112 B/cell"}
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    steps_mx = args.steps / tmx
+    steps_64 = args.steps / t64
+    out = {
+        "metric": "steps/s & pressure-solve ms to 1e-8 resid: learned-PC+mixed vs plain FP64",
+        "value": round(steps_mx, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tmx / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32 (mixed)",
+        "data": "synthetic",
+        "config": {"workload": "Rayleigh-Taylor 1024x2048, Atwood 0.5, cos amplitude 0.01 + "
+                               "1e-4 seeded noise; lu_rb velocity, cg pressure to 1e-8",
+                   "grid": [nx, ny],
+                   "path": "fcg_learned: FP64 flexible CG, FP32 multigrid V-cycle preconditioner"
+                           " (+ learned correction when trained)"},
+        "steps_per_s_fp64": round(steps_64, 4),
+        "ms_per_step_fp64": round(1e3 * t64 / args.steps, 3),
+        "pressure_solve_ms": dict(solves, speedup_learned_mixed_vs_fp64=round(
+            solves["fp64"]["ms"] / solves["learned_mixed"]["ms"], 3)),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -32,6 +32,8 @@ HG_FIELD_VISCOSITY_SMOOTH = 12
 HG_FIELD_FORCE_X = 13
 HG_FIELD_FORCE_Y = 14
 HG_FIELD_VOLUME_FLUX_ITER = 15
+HG_FIELD_PRESSURE_FACE_COEF = 16
+HG_FIELD_PRESSURE_RHS = 17
 
 _d2 = ctypes.c_double * 2
 _d3 = ctypes.c_double * HG_MAX_PHASES

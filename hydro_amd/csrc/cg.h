@@ -14,7 +14,7 @@
 
 namespace hg {
 
-class LearnedPrecond;
+class Precond;
 
 struct CgResult {
   int iterations = 0;     // total inner iterations
@@ -37,10 +37,10 @@ class CgSolver {
   void SetOperator(const double* d_kx, const double* d_ky);
   // Solves A x = b (device pointers, FP64) from x = 0.
   // precision: 0 FP64 Jacobi-PCG; 1 FP32 inner PCG under FP64 refinement.
-  // With `pc` set, the inner iteration is flexible CG preconditioned by the
-  // learned network (falls back to Jacobi when it is not a descent operator).
+  // With `pc` set: flexible CG with FP64 Krylov vectors preconditioned by
+  // pc (FP32 output); falls back to Jacobi-PCG when pc breaks descent.
   CgResult Solve(const double* d_b, double* d_x, double tol, int max_iters, int precision,
-                 bool project, LearnedPrecond* pc = nullptr);
+                 bool project, Precond* pc = nullptr);
   // y = A x (FP64), for tests and residual checks.
   void Apply(const double* d_x, double* d_y);
   const double* kx() const { return kx_; }
@@ -69,8 +69,12 @@ class CgSolver {
 
   CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project);
   CgResult SolveMixed(const double* b, double* x, double tol, int max_iters, bool project,
-                      LearnedPrecond* pc);
-  int InnerFp32(double tol_inner, int max_iters, LearnedPrecond* pc, bool* pc_failed);
+                      Precond* pc);
+  CgResult SolveFlexible(const double* b, double* x, double tol, int max_iters, bool project,
+                         Precond* pc);
+  void Prepare(const double* b, double* x, bool project);
+  int JacobiPcg64(double* x, int max_iters);
+  int InnerFp32(double tol_inner, int max_iters, Precond* pc, bool* pc_failed);
   double ResidualFp64(const double* b, const double* x, double* r);  // returns ||r||^2
   double Dot64(const double* a, const double* b);
   void ReadScalars();

@@ -1,17 +1,25 @@
-// cg.hip -- Jacobi-PCG (FP64) and FP32-inner / FP64-outer iterative
-// refinement for the variable-coefficient pressure Poisson system.
+// cg.hip -- Krylov solvers for the variable-coefficient pressure system.
 //
-// New solvers (no reference counterpart): the values "cg" and "fcg_learned"
-// of linear_solver_pressure.  Per iteration two kernels:
-//   KSpmv:   p = z + beta p_old (recomputed at the stencil points), q = A p,
-//            partial (p, q); the last block forms alpha = (r, z) / (p, q);
-//   KUpdate: x += alpha p, r -= alpha q, z = D^-1 r, partials (r, z), (r, r);
-//            the last block forms beta, counts the iteration and raises `done`
-//            once ||r|| <= tol ||b|| or the cap is reached.
-// Reductions are deterministic: fixed grid, per-block partials folded by the
-// last block in block order.  Dots accumulate in FP64 for FP32 vectors too.
-// Kernels return at once when `done` is set, so the host can queue batches
-// of iterations and poll the flag rarely.
+// New solvers (no reference counterpart), values of linear_solver_pressure:
+//   "cg"           Jacobi-preconditioned CG, FP64 throughout (the plain
+//                  FP64 path); with precision "mixed", FP32 inner CG under
+//                  FP64 iterative refinement.
+//   "fcg_learned"  flexible CG (Notay's FCG(1)) with FP64 Krylov vectors,
+//                  residuals, dot products and updates, preconditioned by an
+//                  FP32 multigrid V-cycle plus, once trained, the learned
+//                  network's correction.  If the preconditioner ever breaks
+//                  the descent property ((r,z) <= 0 or (p,Ap) <= 0) the solve
+//                  continues with Jacobi-PCG from the current iterate, to the
+//                  same tolerance.
+//
+// Work distribution: a persistent grid of at most 1024 workgroups walks
+// 64x4 tiles (tile.h), each XCD a contiguous band.  Reductions use no
+// atomics: each workgroup writes its partial sums, and the kernel that needs
+// the scalar folds all partials itself, in block order, in every workgroup
+// (so every workgroup derives bit-identical alpha / beta / stop decisions).
+// Partial buffers alternate by iteration parity so a fold never races with
+// the next producer.  Workgroup 0 records the scalars for the host, which
+// queues batches of iterations and polls rarely.
 
 #include <algorithm>
 #include <cmath>
@@ -22,20 +30,19 @@
 #include "cg.h"
 #include "device_common.h"
 #include "learned.h"
+#include "tile.h"
 
 namespace hg {
 
 namespace {
 
-constexpr int kB = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kB = kTileThreads;
+constexpr int kMaxPersist = 1024;
 
 struct CgScalars {
-  double rz, pq, alpha, beta, rr, bb, tol2, pad0;
-  int iter, max_iter, done, fcg;
-  unsigned int cnt1, cnt2, cnt3, pad1;
-  double zq;  // FCG: (z_new, q_old)
-  double pad2[3];
+  double rz, pq, alpha, beta, rr, bb, tol2, zq;
+  int iter, max_iter, done, fail;
+  double mean;
 };
 
 #define HG_CHECK(x)                                                                  \
@@ -45,324 +52,407 @@ struct CgScalars {
                                                    hipGetErrorString(e_) + " at " #x); \
   } while (0)
 
-template <class T>
-__device__ __forceinline__ double D(T v) { return double(v); }
+// Persistent tile walk: workgroup b serves tiles of its XCD's band.
+struct Walk {
+  TileGrid tg;
+  int nx, ny;
+};
 
-// Block sum of up to 3 values into part[blockIdx*3 + q]; returns true in
-// thread 0 of the last block to finish, with the folded totals in tot[].
+template <class F>
+__device__ __forceinline__ void ForTiles(const Walk& w, F&& f) {
+  const int nt = w.tg.ntx * w.tg.nty;
+  const int nb = gridDim.x;                    // multiple of 8
+  const int xcd = blockIdx.x % kXcds;
+  const int bpx = nb / kXcds;                  // workgroups per XCD
+  const int lb = blockIdx.x / kXcds;
+  for (int k = lb; k < w.tg.per; k += bpx) {
+    const int tile = xcd * w.tg.per + k;
+    if (tile >= nt) break;
+    const int tx = tile % w.tg.ntx, ty = tile / w.tg.ntx;
+    const int i = tx * kTX + (threadIdx.x & (kTX - 1));
+    const int j = ty * kTY + (threadIdx.x >> 6);
+    if (i < w.nx && j < w.ny) f(i, j);
+  }
+}
+
+// Writes this workgroup's sum of v (per quantity) to part[q * nb + block].
 template <int NQ>
-__device__ bool BlockReduce(double v[NQ], double* part, unsigned int* cnt, double tot[NQ]) {
+__device__ __forceinline__ void PutPartials(const double (&v)[NQ], double* part) {
   __shared__ double red[NQ][kB / 64];
-  __shared__ bool last;
   for (int q = 0; q < NQ; ++q) {
-    double s = WaveSum(v[q]);
+    const double s = WaveSum(v[q]);
     if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < NQ; ++q) {
-      double s = 0.;
-      for (int w = 0; w < kB / 64; ++w) s += red[q][w];
-      part[blockIdx.x * 3 + q] = s;
-    }
-    __threadfence();
-    last = (atomicAdd(cnt, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  // last block: fold the partials in block order (deterministic)
-  __threadfence();
-  for (int q = 0; q < NQ; ++q) {
+  if (threadIdx.x < NQ) {
+    const int q = threadIdx.x;
     double s = 0.;
-    for (int b = threadIdx.x; b < int(gridDim.x); b += kB) s += part[b * 3 + q];
-    s = WaveSum(s);
-    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s;
+    for (int w = 0; w < kB / 64; ++w) s += red[q][w];
+    part[q * gridDim.x + blockIdx.x] = s;
   }
+}
+
+// Sum of `n` partials, identical in every workgroup and thread.
+__device__ __forceinline__ double Fold(const double* part, int n) {
+  __shared__ double red[kB / 64];
+  __shared__ double tot;
+  double s = 0.;
+  for (int b = threadIdx.x; b < n; b += kB) s += part[b];
+  s = WaveSum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 0; q < NQ; ++q) {
-      double s = 0.;
-      for (int w = 0; w < kB / 64; ++w) s += red[q][w];
-      tot[q] = s;
-    }
-    *cnt = 0;
+    double t = 0.;
+    for (int w = 0; w < kB / 64; ++w) t += red[w];
+    tot = t;
   }
-  return threadIdx.x == 0;
+  __syncthreads();
+  const double r = tot;
+  __syncthreads();
+  return r;
 }
 
-// (A p)_c = d_c p_c - sum_nb k_f p_nb, face form.
-template <class T>
-__device__ __forceinline__ T ApplyAt(int nx, int ny, int nfx, const T* __restrict__ kx,
-                                     const T* __restrict__ ky, int c, int i, int j, T pc, T pw,
-                                     T pe, T ps, T pn) {
+// (A p)_c = d_c p_c - sum_nb k_f p_nb on the face form.
+template <class T, class F>
+__device__ __forceinline__ T ApplyFn(int nx, int ny, const T* __restrict__ kx,
+                                     const T* __restrict__ ky, int i, int j, F&& P) {
   const T kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-  const T ks = ky[nfx * 0 + i + nx * j], kn = ky[i + nx * (j + 1)];
-  T d = ((kw + ke) + ks) + kn;
-  return d * pc - kw * pw - ke * pe - ks * ps - kn * pn;
+  const T ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+  T v = (((kw + ke) + ks) + kn) * P(i, j);
+  if (i > 0) v -= kw * P(i - 1, j);
+  if (i < nx - 1) v -= ke * P(i + 1, j);
+  if (j > 0) v -= ks * P(i, j - 1);
+  if (j < ny - 1) v -= kn * P(i, j + 1);
+  return v;
 }
 
+// Partial-buffer layout: upd[2][2 * nb] holds (rz, rr) of the update of
+// iteration k in slot k & 1; spmv[2][nb] holds (p, q) of iteration k in
+// slot k & 1; dir[2 * nb] holds (z, q_old), (r, z) of FCG.
+struct Parts {
+  double* upd[2];
+  double* spmv[2];
+  double* dir;
+  double* misc;
+  int nb;
+};
+
+// ---- Jacobi-PCG (T = double or float vectors) ------------------------------
+// Iteration k: KSpmv reads the update partials of k-1 (stop test, beta) and
+// writes (p, q) partials; KUpdate forms alpha and writes (r, z), (r, r).
 template <class T>
 __global__ void __launch_bounds__(kB)
-KSpmv(int nx, int ny, const T* __restrict__ kx, const T* __restrict__ ky,
-      const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
-      T* __restrict__ q, double* part, CgScalars* sc) {
+KPcgSpmv(Walk w, int k, const T* __restrict__ kx, const T* __restrict__ ky,
+         const T* __restrict__ z, const T* __restrict__ pold, T* __restrict__ pnew,
+         T* __restrict__ q, Parts pt, CgScalars* sc) {
   if (sc->done) return;
-  const double beta = sc->beta;
+  const int nb = pt.nb;
+  // stop test and beta from the previous update (or the start partials)
+  const double rz_new = Fold(pt.upd[(k + 1) & 1], nb);
+  const double rr = Fold(pt.upd[(k + 1) & 1] + nb, nb);
+  const bool stop = !(rr > sc->tol2 * sc->bb) || k >= sc->max_iter || !(rr == rr);
+  if (stop) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iter = k;
+      sc->rr = rr;
+    }
+    return;
+  }
+  double beta = 0.;
+  if (k > 0) beta = rz_new / Fold(pt.upd[k & 1], nb);
   const T b = T(beta);
-  const long n = long(nx) * ny;
   double acc = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    const int i = int(c % nx), j = int(c / nx);
-    auto P = [&](long cc) -> T { return z[cc] + b * pold[cc]; };
-    const T pc = P(c);
-    const T pw = i > 0 ? P(c - 1) : T(0);
-    const T pe = i < nx - 1 ? P(c + 1) : T(0);
-    const T ps = j > 0 ? P(c - nx) : T(0);
-    const T pn = j < ny - 1 ? P(c + nx) : T(0);
-    const T qv = ApplyAt<T>(nx, ny, 0, kx, ky, int(c), i, j, pc, pw, pe, ps, pn);
+  ForTiles(w, [&](int i, int j) {
+    auto P = [&](int ii, int jj) {
+      const int cc = ii + w.nx * jj;
+      return z[cc] + b * pold[cc];
+    };
+    const T pc = P(i, j);
+    const T qv = ApplyFn<T>(w.nx, w.ny, kx, ky, i, j, P);
+    const int c = i + w.nx * j;
     pnew[c] = pc;
     q[c] = qv;
-    acc += D(pc) * D(qv);
-  }
-  double v[1] = {acc}, tot[1];
-  if (BlockReduce<1>(v, part, &sc->cnt1, tot)) {
-    sc->pq = tot[0];
-    sc->alpha = sc->rz / tot[0];
-  }
+    acc += double(pc) * double(qv);
+  });
+  double v[1] = {acc};
+  PutPartials<1>(v, pt.spmv[k & 1]);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kB)
-KUpdate(long n, const T* __restrict__ p, const T* __restrict__ q, const T* __restrict__ dinv,
-        T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, double* part, CgScalars* sc) {
+KPcgUpdate(Walk w, int k, const T* __restrict__ p, const T* __restrict__ q,
+           const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+           Parts pt, CgScalars* sc) {
   if (sc->done) return;
-  const T a = T(sc->alpha);
-  double rz = 0., rr = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
+  const int nb = pt.nb;
+  const double rz = Fold(pt.upd[(k + 1) & 1], nb);
+  const double pq = Fold(pt.spmv[k & 1], nb);
+  const T a = T(rz / pq);
+  double srz = 0., srr = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
     x[c] += a * p[c];
     const T rv = r[c] - a * q[c];
     r[c] = rv;
     const T zv = rv * dinv[c];
     z[c] = zv;
-    rz += D(rv) * D(zv);
-    rr += D(rv) * D(rv);
-  }
-  double v[2] = {rz, rr}, tot[2];
-  if (BlockReduce<2>(v, part, &sc->cnt2, tot)) {
-    sc->beta = tot[0] / sc->rz;
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->iter += 1;
-    if (tot[1] <= sc->tol2 * sc->bb || sc->iter >= sc->max_iter || !(tot[1] == tot[1]))
-      sc->done = 1;
-  }
+    srz += double(rv) * double(zv);
+    srr += double(rv) * double(rv);
+  });
+  double v[2] = {srz, srr};
+  PutPartials<2>(v, pt.upd[k & 1]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->iter = k + 1;
 }
 
-// Start of a (P)CG run: z = D^-1 r, p_old = 0 (beta = 0), rz = (r, z),
-// bb = (r, r).
+// Start: z = D^-1 r, p_old = 0, partials (r, z), (r, r) in upd slot 1
+// (the slot iteration 0 reads as "previous").
 template <class T>
 __global__ void __launch_bounds__(kB)
-KStart(long n, const T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ z,
-       T* __restrict__ pold, double* part, CgScalars* sc) {
-  double rz = 0., rr = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
+KPcgStart(Walk w, const T* __restrict__ r, const T* __restrict__ dinv, T* __restrict__ z,
+          T* __restrict__ pold, Parts pt) {
+  double srz = 0., srr = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
     const T zv = r[c] * dinv[c];
     z[c] = zv;
     pold[c] = T(0);
-    rz += D(r[c]) * D(zv);
-    rr += D(r[c]) * D(r[c]);
+    srz += double(r[c]) * double(zv);
+    srr += double(r[c]) * double(r[c]);
+  });
+  double v[2] = {srz, srr};
+  PutPartials<2>(v, pt.upd[1]);
+}
+
+// ---- FCG(1) with an FP32 preconditioner output -----------------------------
+// Check after update k-1 (one workgroup): stop test from (r, r).
+__global__ void __launch_bounds__(kB)
+KFcgCheck(int k, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  const double rr = Fold(pt.upd[(k + 1) & 1] + pt.nb, pt.nb);
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    sc->iter = k;
+    if (!(rr > sc->tol2 * sc->bb) || k >= sc->max_iter || !(rr == rr) || sc->fail) sc->done = 1;
   }
-  double v[2] = {rz, rr}, tot[2];
-  if (BlockReduce<2>(v, part, &sc->cnt3, tot)) {
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->bb = tot[1];
-    sc->beta = 0.;
-    sc->iter = 0;
-    sc->done = (tot[1] == 0.) ? 1 : 0;
+}
+
+// (z, q_old), (r, z).
+__global__ void __launch_bounds__(kB)
+KFcgDir(Walk w, const float* __restrict__ z, const double* __restrict__ qold,
+        const double* __restrict__ r, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  double zq = 0., rz = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    const double zv = double(z[c]);
+    zq += zv * qold[c];
+    rz += r[c] * zv;
+  });
+  double v[2] = {zq, rz};
+  PutPartials<2>(v, pt.dir);
+}
+
+// p = z + beta p_old with beta = -(z, q_old) / (p_old, q_old); q = A p.
+__global__ void __launch_bounds__(kB)
+KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict__ ky,
+         const float* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
+         double* __restrict__ q, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  const int nb = pt.nb;
+  const double rz = Fold(pt.dir + nb, nb);
+  if (!(rz > 0.)) {  // preconditioner not positive on r
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->fail = 1;
+      sc->done = 1;
+    }
+    return;
   }
-}
-
-// Sum of a vector (for the zero-mean projection) -> sc->pq.
-__global__ void __launch_bounds__(kB)
-KSum(long n, const double* __restrict__ a, double* part, CgScalars* sc) {
-  double s = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) s += a[c];
-  double v[1] = {s}, tot[1];
-  if (BlockReduce<1>(v, part, &sc->cnt1, tot)) sc->pq = tot[0];
-}
-
-__global__ void __launch_bounds__(kB)
-KProject(long n, const double* __restrict__ a, double* __restrict__ out, CgScalars* sc) {
-  const double mean = sc->pq / double(n);
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB)
-    out[c] = a[c] - mean;
-}
-
-// r = b - A x in FP64, rr = (r, r) -> sc->rr.
-__global__ void __launch_bounds__(kB)
-KResidual(int nx, int ny, const double* __restrict__ kx, const double* __restrict__ ky,
-          const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
-          double* part, CgScalars* sc) {
-  const long n = long(nx) * ny;
+  double beta = 0.;
+  if (k > 0) beta = -Fold(pt.dir, nb) / Fold(pt.spmv[(k + 1) & 1], nb);
   double acc = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    const int i = int(c % nx), j = int(c / nx);
-    const double ax = ApplyAt<double>(nx, ny, 0, kx, ky, int(c), i, j, x[c],
-                                      i > 0 ? x[c - 1] : 0., i < nx - 1 ? x[c + 1] : 0.,
-                                      j > 0 ? x[c - nx] : 0., j < ny - 1 ? x[c + nx] : 0.);
-    const double rv = b[c] - ax;
+  ForTiles(w, [&](int i, int j) {
+    auto P = [&](int ii, int jj) {
+      const int cc = ii + w.nx * jj;
+      return double(z[cc]) + beta * pold[cc];
+    };
+    const double pc = P(i, j);
+    const double qv = ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
+    const int c = i + w.nx * j;
+    pnew[c] = pc;
+    q[c] = qv;
+    acc += pc * qv;
+  });
+  double v[1] = {acc};
+  PutPartials<1>(v, pt.spmv[k & 1]);
+}
+
+// x += alpha p, r -= alpha q with alpha = (r, z) / (p, q); partial (r, r).
+__global__ void __launch_bounds__(kB)
+KFcgUpdate(Walk w, int k, const double* __restrict__ p, const double* __restrict__ q,
+           double* __restrict__ x, double* __restrict__ r, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  const int nb = pt.nb;
+  const double rz = Fold(pt.dir + nb, nb);
+  const double pq = Fold(pt.spmv[k & 1], nb);
+  if (!(pq > 0.)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->fail = 1;
+      sc->done = 1;
+    }
+    return;
+  }
+  const double a = rz / pq;
+  double rr = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    x[c] += a * p[c];
+    const double rv = r[c] - a * q[c];
+    r[c] = rv;
+    rr += rv * rv;
+  });
+  double v[2] = {0., rr};
+  PutPartials<2>(v, pt.upd[k & 1]);
+}
+
+// ---- helpers --------------------------------------------------------------
+__global__ void __launch_bounds__(kB)
+KSumPart(Walk w, const double* __restrict__ a, Parts pt) {
+  double s = 0.;
+  ForTiles(w, [&](int i, int j) { s += a[i + w.nx * j]; });
+  double v[1] = {s};
+  PutPartials<1>(v, pt.misc);
+}
+
+__global__ void __launch_bounds__(kB)
+KProject(Walk w, const double* __restrict__ a, double* __restrict__ out, Parts pt) {
+  const double mean = Fold(pt.misc, pt.nb) / (double(w.nx) * double(w.ny));
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    out[c] = a[c] - mean;
+  });
+}
+
+// r = b - A x (FP64) and its partial (r, r) into misc.
+__global__ void __launch_bounds__(kB)
+KResidual(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
+          const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
+          Parts pt) {
+  double acc = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    auto P = [&](int ii, int jj) { return x[ii + w.nx * jj]; };
+    const double rv = b[c] - ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
     r[c] = rv;
     acc += rv * rv;
+  });
+  double v[1] = {acc};
+  PutPartials<1>(v, pt.misc);
+}
+
+// Folds misc into sc->rr (and sc->bb when set_bb); also copies it to the
+// update slot 1 (r, r) so iteration 0 of FCG sees it.
+__global__ void __launch_bounds__(kB)
+KTakeRR(Parts pt, CgScalars* sc, int set_bb) {
+  const double rr = Fold(pt.misc, pt.nb);
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    if (set_bb) sc->bb = rr;
   }
-  double v[1] = {acc}, tot[1];
-  if (BlockReduce<1>(v, part, &sc->cnt1, tot)) sc->rr = tot[0];
 }
 
 __global__ void __launch_bounds__(kB)
-KApply(int nx, int ny, const double* __restrict__ kx, const double* __restrict__ ky,
+KApply(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
        const double* __restrict__ x, double* __restrict__ y) {
-  const long n = long(nx) * ny;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    const int i = int(c % nx), j = int(c / nx);
-    y[c] = ApplyAt<double>(nx, ny, 0, kx, ky, int(c), i, j, x[c], i > 0 ? x[c - 1] : 0.,
-                           i < nx - 1 ? x[c + 1] : 0., j > 0 ? x[c - nx] : 0.,
-                           j < ny - 1 ? x[c + nx] : 0.);
-  }
+  ForTiles(w, [&](int i, int j) {
+    auto P = [&](int ii, int jj) { return x[ii + w.nx * jj]; };
+    y[i + w.nx * j] = ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
+  });
 }
 
 __global__ void __launch_bounds__(kB)
-KDinv(int nx, int ny, const double* __restrict__ kx, const double* __restrict__ ky,
-      double* __restrict__ dinv, float* __restrict__ dinv32, float* __restrict__ kx32,
-      float* __restrict__ ky32) {
-  const long n = long(nx) * ny;
-  const long nfx = long(nx + 1) * ny, nfy = long(nx) * (ny + 1);
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    const int i = int(c % nx), j = int(c / nx);
+KDinv(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
+      double* __restrict__ dinv, float* __restrict__ dinv32) {
+  ForTiles(w, [&](int i, int j) {
+    const int nx = w.nx;
     const double d = ((kx[i + (nx + 1) * j] + kx[i + 1 + (nx + 1) * j]) + ky[i + nx * j]) +
                      ky[i + nx * (j + 1)];
     const double di = d > 0. ? 1. / d : 0.;
-    dinv[c] = di;
-    dinv32[c] = float(di);
-  }
-  for (long f = blockIdx.x * long(kB) + threadIdx.x; f < nfx; f += long(gridDim.x) * kB)
-    kx32[f] = float(kx[f]);
-  for (long f = blockIdx.x * long(kB) + threadIdx.x; f < nfy; f += long(gridDim.x) * kB)
-    ky32[f] = float(ky[f]);
+    dinv[i + nx * j] = di;
+    dinv32[i + nx * j] = float(di);
+  });
+}
+
+__global__ void KCastFaces(long n, const double* __restrict__ a, float* __restrict__ b) {
+  long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (f < n) b[f] = float(a[f]);
 }
 
 __global__ void __launch_bounds__(kB)
-KZero(long n, double* __restrict__ a) {
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) a[c] = 0.;
+KZero(Walk w, double* __restrict__ a) {
+  ForTiles(w, [&](int i, int j) { a[i + w.nx * j] = 0.; });
 }
 
 __global__ void __launch_bounds__(kB)
-KCastDown(long n, const double* __restrict__ a, float* __restrict__ b, float* __restrict__ zero) {
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
+KCastDown(Walk w, const double* __restrict__ a, float* __restrict__ b, float* __restrict__ zero) {
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
     b[c] = float(a[c]);
     zero[c] = 0.f;
-  }
+  });
 }
 
 __global__ void __launch_bounds__(kB)
-KAddUp(long n, const float* __restrict__ e, double* __restrict__ x) {
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB)
-    x[c] += double(e[c]);
+KAddUp(Walk w, const float* __restrict__ e, double* __restrict__ x) {
+  ForTiles(w, [&](int i, int j) { x[i + w.nx * j] += double(e[i + w.nx * j]); });
 }
 
-__global__ void __launch_bounds__(kB)
-KSetupScalars(CgScalars* sc, double tol2, int max_iter) {
+__global__ void KSetup(CgScalars* sc, double tol2, int max_iter) {
   if (threadIdx.x == 0) {
     sc->tol2 = tol2;
     sc->max_iter = max_iter;
     sc->done = 0;
+    sc->fail = 0;
     sc->iter = 0;
-    sc->cnt1 = sc->cnt2 = sc->cnt3 = 0;
   }
 }
 
-// ---- flexible CG with a (possibly nonlinear) preconditioner --------------
-// FCG(1) (Notay 2000): p = z - ((z, q_old)/(p_old, q_old)) p_old.  z comes
-// from the learned network; (z, q_old) is reduced here and stored in sc->zq.
+// Sets the FP32 inner solve's reference norm: bb = (r, r) from upd slot 1.
 __global__ void __launch_bounds__(kB)
-KFcgDirection(long n, const float* __restrict__ z, const float* __restrict__ pold,
-              const float* __restrict__ qold, const float* __restrict__ r, double* part,
-              CgScalars* sc) {
-  if (sc->done) return;
-  double zq = 0., rz = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    zq += D(z[c]) * D(qold[c]);
-    rz += D(r[c]) * D(z[c]);
-  }
-  double v[2] = {zq, rz}, tot[2];
-  if (BlockReduce<2>(v, part, &sc->cnt3, tot)) {
-    sc->beta = (sc->iter == 0 || sc->pq == 0.) ? 0. : -tot[0] / sc->pq;
-    sc->rz = tot[1];
-    sc->zq = tot[0];
-    // a non-positive (r, z) means the preconditioner is not SPD-like here
-    if (!(tot[1] > 0.)) sc->fcg = 2;
-  }
-}
-
-// r -= alpha q, x += alpha p; rr -> stop test (FCG flavour: z comes later).
-__global__ void __launch_bounds__(kB)
-KFcgUpdate(long n, const float* __restrict__ p, const float* __restrict__ q,
-           float* __restrict__ x, float* __restrict__ r, double* part, CgScalars* sc) {
-  if (sc->done) return;
-  const float a = float(sc->alpha);
-  double rr = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    x[c] += a * p[c];
-    const float rv = r[c] - a * q[c];
-    r[c] = rv;
-    rr += D(rv) * D(rv);
-  }
-  double v[1] = {rr}, tot[1];
-  if (BlockReduce<1>(v, part, &sc->cnt2, tot)) {
-    sc->rr = tot[0];
-    sc->iter += 1;
-    if (tot[0] <= sc->tol2 * sc->bb || sc->iter >= sc->max_iter || !(tot[0] == tot[0]))
-      sc->done = 1;
-  }
-}
-
-// p = z + beta p_old (plain, no stencil), q = A p, partial (p, q).
-__global__ void __launch_bounds__(kB)
-KFcgSpmv(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
-         const float* __restrict__ z, const float* __restrict__ pold, float* __restrict__ pnew,
-         float* __restrict__ q, double* part, CgScalars* sc) {
-  if (sc->done) return;
-  const float b = float(sc->beta);
-  const long n = long(nx) * ny;
-  double acc = 0.;
-  for (long c = blockIdx.x * long(kB) + threadIdx.x; c < n; c += long(gridDim.x) * kB) {
-    const int i = int(c % nx), j = int(c / nx);
-    auto P = [&](long cc) -> float { return z[cc] + b * pold[cc]; };
-    const float pc = P(c);
-    const float qv = ApplyAt<float>(nx, ny, 0, kx, ky, int(c), i, j, pc,
-                                    i > 0 ? P(c - 1) : 0.f, i < nx - 1 ? P(c + 1) : 0.f,
-                                    j > 0 ? P(c - nx) : 0.f, j < ny - 1 ? P(c + nx) : 0.f);
-    pnew[c] = pc;
-    q[c] = qv;
-    acc += D(pc) * D(qv);
-  }
-  double v[1] = {acc}, tot[1];
-  if (BlockReduce<1>(v, part, &sc->cnt1, tot)) {
-    sc->pq = tot[0];
-    sc->alpha = sc->rz / tot[0];
-    if (!(tot[0] > 0.)) sc->fcg = 2;
-  }
+KInnerBB(Parts pt, CgScalars* sc) {
+  const double rr = Fold(pt.upd[1] + pt.nb, pt.nb);
+  if (threadIdx.x == 0) sc->bb = rr;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
 
+namespace {
+struct CgImpl {
+  Walk w;
+  int nb;
+  Parts pt;
+};
+}  // namespace
+
+static Walk MakeWalk(int nx, int ny, int* nb) {
+  Walk w;
+  w.tg = MakeTileGrid(nx, ny);
+  w.nx = nx;
+  w.ny = ny;
+  int tiles = w.tg.ntx * w.tg.nty;
+  int b = std::min(tiles, kMaxPersist);
+  b = std::max(kXcds, (b + kXcds - 1) / kXcds * kXcds);
+  *nb = b;
+  return w;
+}
+
 CgSolver::CgSolver(int nx, int ny, hipStream_t stream) : nx_(nx), ny_(ny), stream_(stream) {
   n_ = long(nx) * ny;
   long nfx = long(nx + 1) * ny, nfy = long(nx) * (ny + 1);
-  blocks_ = int(std::min<long>((n_ + kB - 1) / kB, kMaxBlocks));
-  auto A = [](void** p, size_t bytes) { HG_CHECK(hipMalloc(p, bytes)); };
+  MakeWalk(nx, ny, &blocks_);
+  auto A = [](void** p, size_t bytes) { HG_CHECK(hipMalloc(p, std::max<size_t>(bytes, 8))); };
   A((void**)&kx_, nfx * 8);
   A((void**)&ky_, nfy * 8);
   A((void**)&dinv_, n_ * 8);
@@ -371,7 +461,9 @@ CgSolver::CgSolver(int nx, int ny, hipStream_t stream) : nx_(nx), ny_(ny), strea
   A((void**)&dinv32_, n_ * 4);
   for (double** p : {&r_, &z_, &pa_, &pb_, &q_, &b_}) A((void**)p, n_ * 8);
   for (float** p : {&r32_, &z32_, &pa32_, &pb32_, &q32_, &e32_, &zprev32_}) A((void**)p, n_ * 4);
-  A((void**)&part_, size_t(kMaxBlocks) * 3 * 8);
+  // partials: upd[2] x 2nb, spmv[2] x nb, dir 2nb, misc nb
+  A((void**)&part_, size_t(blocks_) * 10 * 8);
+  HG_CHECK(hipMemset(part_, 0, size_t(blocks_) * 10 * 8));
   A(&scal_, sizeof(CgScalars));
   HG_CHECK(hipMemset(scal_, 0, sizeof(CgScalars)));
   HG_CHECK(hipHostMalloc(&host_scal_, sizeof(CgScalars)));
@@ -390,11 +482,27 @@ CgSolver::~CgSolver() {
   (void)hipEventDestroy(ev1_);
 }
 
+static Parts MakeParts(double* base, int nb) {
+  Parts p;
+  p.nb = nb;
+  p.upd[0] = base;
+  p.upd[1] = base + 2 * nb;
+  p.spmv[0] = base + 4 * nb;
+  p.spmv[1] = base + 5 * nb;
+  p.dir = base + 6 * nb;
+  p.misc = base + 8 * nb;
+  return p;
+}
+
 void CgSolver::SetOperator(const double* d_kx, const double* d_ky) {
   long nfx = long(nx_ + 1) * ny_, nfy = long(nx_) * (ny_ + 1);
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
   HG_CHECK(hipMemcpyAsync(kx_, d_kx, nfx * 8, hipMemcpyDeviceToDevice, stream_));
   HG_CHECK(hipMemcpyAsync(ky_, d_ky, nfy * 8, hipMemcpyDeviceToDevice, stream_));
-  KDinv<<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx_, ky_, dinv_, dinv32_, kx32_, ky32_);
+  KDinv<<<nb, kB, 0, stream_>>>(w, kx_, ky_, dinv_, dinv32_);
+  KCastFaces<<<unsigned((nfx + 255) / 256), 256, 0, stream_>>>(nfx, kx_, kx32_);
+  KCastFaces<<<unsigned((nfy + 255) / 256), 256, 0, stream_>>>(nfy, ky_, ky32_);
 }
 
 void CgSolver::ReadScalars() {
@@ -403,20 +511,31 @@ void CgSolver::ReadScalars() {
 }
 
 void CgSolver::Apply(const double* x, double* y) {
-  KApply<<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx_, ky_, x, y);
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  KApply<<<nb, kB, 0, stream_>>>(w, kx_, ky_, x, y);
 }
 
 double CgSolver::ResidualFp64(const double* b, const double* x, double* r) {
-  KResidual<<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx_, ky_, b, x, r, part_, (CgScalars*)scal_);
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
+  KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b, x, r, pt);
+  KTakeRR<<<1, kB, 0, stream_>>>(pt, (CgScalars*)scal_, 0);
   ReadScalars();
   return ((CgScalars*)host_scal_)->rr;
 }
 
 CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, int precision,
-                         bool project, LearnedPrecond* pc) {
+                         bool project, Precond* pc) {
   HG_CHECK(hipEventRecord(ev0_, stream_));
-  CgResult res = (precision == 0 && !pc) ? SolveFp64(b, x, tol, max_iters, project)
-                                         : SolveMixed(b, x, tol, max_iters, project, pc);
+  CgResult res;
+  if (pc)
+    res = SolveFlexible(b, x, tol, max_iters, project, pc);
+  else if (precision == 0)
+    res = SolveFp64(b, x, tol, max_iters, project);
+  else
+    res = SolveMixed(b, x, tol, max_iters, project, nullptr);
   HG_CHECK(hipEventRecord(ev1_, stream_));
   HG_CHECK(hipEventSynchronize(ev1_));
   float ms = 0.f;
@@ -425,36 +544,58 @@ CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, 
   return res;
 }
 
+// b_ <- b (projected to zero mean when singular); x <- 0; r_ <- b_;
+// sc->bb = ||b_||^2; (r, r) also placed in update slot 1.
+void CgSolver::Prepare(const double* b, double* x, bool project) {
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
+  CgScalars* sc = (CgScalars*)scal_;
+  if (project) {
+    KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
+    KProject<<<nb, kB, 0, stream_>>>(w, b, b_, pt);
+  } else {
+    HG_CHECK(hipMemcpyAsync(b_, b, n_ * 8, hipMemcpyDeviceToDevice, stream_));
+  }
+  KZero<<<nb, kB, 0, stream_>>>(w, x);
+  KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b_, x, r_, pt);
+  KTakeRR<<<1, kB, 0, stream_>>>(pt, sc, 1);
+  HG_CHECK(hipMemcpyAsync(pt.upd[1] + nb, pt.misc, size_t(nb) * 8, hipMemcpyDeviceToDevice,
+                          stream_));
+}
+
+// Jacobi-PCG on A x = b_ continuing from the current x and r_ (FP64).
+int CgSolver::JacobiPcg64(double* x, int max_iters) {
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
+  CgScalars* sc = (CgScalars*)scal_;
+  CgScalars* hs = (CgScalars*)host_scal_;
+  KPcgStart<double><<<nb, kB, 0, stream_>>>(w, r_, dinv_, z_, pa_, pt);
+  double* pold = pa_;
+  double* pnew = pb_;
+  int k = 0;
+  const int batch = 32;
+  while (true) {
+    for (int it = 0; it < batch; ++it, ++k) {
+      KPcgSpmv<double><<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z_, pold, pnew, q_, pt, sc);
+      KPcgUpdate<double><<<nb, kB, 0, stream_>>>(w, k, pnew, q_, dinv_, x, r_, z_, pt, sc);
+      std::swap(pold, pnew);
+    }
+    ReadScalars();
+    if (hs->done || k >= max_iters + batch) break;
+  }
+  return hs->iter;
+}
+
 CgResult CgSolver::SolveFp64(const double* b, double* x, double tol, int max_iters,
                              bool project) {
   CgScalars* sc = (CgScalars*)scal_;
   CgScalars* hs = (CgScalars*)host_scal_;
-  KSetupScalars<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
-  if (project) {
-    KSum<<<blocks_, kB, 0, stream_>>>(n_, b, part_, sc);
-    KProject<<<blocks_, kB, 0, stream_>>>(n_, b, b_, sc);
-  } else {
-    HG_CHECK(hipMemcpyAsync(b_, b, n_ * 8, hipMemcpyDeviceToDevice, stream_));
-  }
-  HG_CHECK(hipMemcpyAsync(r_, b_, n_ * 8, hipMemcpyDeviceToDevice, stream_));
-  KZero<<<blocks_, kB, 0, stream_>>>(n_, x);
-  KStart<double><<<blocks_, kB, 0, stream_>>>(n_, r_, dinv_, z_, pa_, part_, sc);
-  double* pold = pa_;
-  double* pnew = pb_;
-  int launched = 0;
-  const int batch = 32;
-  while (true) {
-    for (int it = 0; it < batch; ++it) {
-      KSpmv<double><<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx_, ky_, z_, pold, pnew, q_, part_, sc);
-      KUpdate<double><<<blocks_, kB, 0, stream_>>>(n_, pnew, q_, dinv_, x, r_, z_, part_, sc);
-      std::swap(pold, pnew);
-    }
-    launched += batch;
-    ReadScalars();
-    if (hs->done || launched >= max_iters + batch) break;
-  }
+  KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
+  Prepare(b, x, project);
   CgResult res;
-  res.iterations = hs->iter;
+  res.iterations = JacobiPcg64(x, max_iters);
   double bb = hs->bb;
   double rr = ResidualFp64(b_, x, r_);
   res.rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.;
@@ -463,101 +604,107 @@ CgResult CgSolver::SolveFp64(const double* b, double* x, double tol, int max_ite
   return res;
 }
 
-int CgSolver::InnerFp32(double tol_inner, int max_iters, LearnedPrecond* pc, bool* pc_failed) {
-  // r32 holds the residual; solve A e = r32 from e = 0.
+CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max_iters,
+                                 bool project, Precond* pc) {
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
   CgScalars* hs = (CgScalars*)host_scal_;
-  KSetupScalars<<<1, 64, 0, stream_>>>(sc, tol_inner * tol_inner, max_iters);
-  if (!pc) {
-    KStart<float><<<blocks_, kB, 0, stream_>>>(n_, r32_, dinv32_, z32_, pa32_, part_, sc);
-    float* pold = pa32_;
-    float* pnew = pb32_;
-    int launched = 0;
-    const int batch = 32;
-    while (true) {
-      for (int it = 0; it < batch; ++it) {
-        KSpmv<float><<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx32_, ky32_, z32_, pold, pnew, q32_,
-                                                  part_, sc);
-        KUpdate<float><<<blocks_, kB, 0, stream_>>>(n_, pnew, q32_, dinv32_, e32_, r32_, z32_,
-                                                    part_, sc);
-        std::swap(pold, pnew);
-      }
-      launched += batch;
-      ReadScalars();
-      if (hs->done || launched >= max_iters + batch) break;
-    }
-    return hs->iter;
-  }
-  // flexible CG with the learned preconditioner
-  KStart<float><<<blocks_, kB, 0, stream_>>>(n_, r32_, dinv32_, z32_, pa32_, part_, sc);
-  HG_CHECK(hipMemsetAsync(q32_, 0, n_ * 4, stream_));
-  float* pold = pa32_;
-  float* pnew = pb32_;
-  int launched = 0;
-  const int batch = 8;
-  *pc_failed = false;
+  KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
+  Prepare(b, x, project);
+  HG_CHECK(hipMemsetAsync(q_, 0, n_ * 8, stream_));
+  HG_CHECK(hipMemsetAsync(pa_, 0, n_ * 8, stream_));
+  double* pold = pa_;
+  double* pnew = pb_;
+  int k = 0;
+  const int batch = 4;
+  CgResult res;
   while (true) {
-    for (int it = 0; it < batch; ++it) {
-      pc->Apply(r32_, z32_, stream_, (const int*)&sc->done);
-      KFcgDirection<<<blocks_, kB, 0, stream_>>>(n_, z32_, pold, q32_, r32_, part_, sc);
-      KFcgSpmv<<<blocks_, kB, 0, stream_>>>(nx_, ny_, kx32_, ky32_, z32_, pold, pnew, q32_, part_,
-                                            sc);
-      KFcgUpdate<<<blocks_, kB, 0, stream_>>>(n_, pnew, q32_, e32_, r32_, part_, sc);
+    for (int it = 0; it < batch; ++it, ++k) {
+      KFcgCheck<<<1, kB, 0, stream_>>>(k, pt, sc);
+      pc->Apply(r_, z32_, stream_, &sc->done);
+      KFcgDir<<<nb, kB, 0, stream_>>>(w, z32_, q_, r_, pt, sc);
+      KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pt, sc);
+      KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pt, sc);
       std::swap(pold, pnew);
     }
-    launched += batch;
     ReadScalars();
-    if (hs->fcg == 2) {
-      *pc_failed = true;
-      return hs->iter;
+    if (hs->done || k >= max_iters + batch) break;
+  }
+  res.iterations = hs->iter;
+  if (hs->fail) {
+    // the preconditioner broke the descent property: finish with
+    // Jacobi-PCG from the current iterate (same FP64 tolerance; sc->bb
+    // still holds ||b||^2 from Prepare)
+    res.used_fallback = true;
+    const int left = std::max(1, max_iters - res.iterations);
+    KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, left);
+    KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b_, x, r_, pt);
+    res.iterations += JacobiPcg64(x, left);
+  }
+  double bb = hs->bb;
+  double rr = ResidualFp64(b_, x, r_);
+  res.rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.;
+  res.converged = res.rel_residual <= tol;
+  res.outer = 1;
+  return res;
+}
+
+int CgSolver::InnerFp32(double tol_inner, int max_iters, Precond*, bool*) {
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
+  CgScalars* sc = (CgScalars*)scal_;
+  CgScalars* hs = (CgScalars*)host_scal_;
+  KSetup<<<1, 64, 0, stream_>>>(sc, tol_inner * tol_inner, max_iters);
+  KPcgStart<float><<<nb, kB, 0, stream_>>>(w, r32_, dinv32_, z32_, pa32_, pt);
+  KInnerBB<<<1, kB, 0, stream_>>>(pt, sc);
+  float* pold = pa32_;
+  float* pnew = pb32_;
+  int k = 0;
+  const int batch = 32;
+  while (true) {
+    for (int it = 0; it < batch; ++it, ++k) {
+      KPcgSpmv<float><<<nb, kB, 0, stream_>>>(w, k, kx32_, ky32_, z32_, pold, pnew, q32_, pt, sc);
+      KPcgUpdate<float><<<nb, kB, 0, stream_>>>(w, k, pnew, q32_, dinv32_, e32_, r32_, z32_, pt,
+                                                sc);
+      std::swap(pold, pnew);
     }
-    if (hs->done || launched >= max_iters + batch) break;
+    ReadScalars();
+    if (hs->done || k >= max_iters + batch) break;
   }
   return hs->iter;
 }
 
+// FP32 inner Jacobi-PCG under FP64 iterative refinement ("cg", mixed).
 CgResult CgSolver::SolveMixed(const double* b, double* x, double tol, int max_iters,
-                              bool project, LearnedPrecond* pc) {
+                              bool project, Precond*) {
+  int nb;
+  Walk w = MakeWalk(nx_, ny_, &nb);
+  Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
   CgResult res;
-  KSetupScalars<<<1, 64, 0, stream_>>>(sc, 0., 1);
-  if (project) {
-    KSum<<<blocks_, kB, 0, stream_>>>(n_, b, part_, sc);
-    KProject<<<blocks_, kB, 0, stream_>>>(n_, b, b_, sc);
-  } else {
-    HG_CHECK(hipMemcpyAsync(b_, b, n_ * 8, hipMemcpyDeviceToDevice, stream_));
-  }
-  KZero<<<blocks_, kB, 0, stream_>>>(n_, x);
-  double bb = ResidualFp64(b_, x, r_);  // r = b, ||b||^2
+  KSetup<<<1, 64, 0, stream_>>>(sc, 0., 1);
+  Prepare(b, x, project);
+  ReadScalars();
+  double bb = ((CgScalars*)host_scal_)->bb;
   if (bb == 0.) {
     res.converged = true;
     return res;
   }
   double rr = bb;
-  const double inner_tol = pc ? 1e-3 : 1e-4;
   int total = 0;
-  bool fallback = false;
   while (std::sqrt(rr / bb) > tol && total < max_iters && res.outer < 50) {
     if (project) {
-      KSum<<<blocks_, kB, 0, stream_>>>(n_, r_, part_, sc);
-      KProject<<<blocks_, kB, 0, stream_>>>(n_, r_, r_, sc);
+      KSumPart<<<nb, kB, 0, stream_>>>(w, r_, pt);
+      KProject<<<nb, kB, 0, stream_>>>(w, r_, r_, pt);
     }
-    KCastDown<<<blocks_, kB, 0, stream_>>>(n_, r_, r32_, e32_);
-    // the inner solve only needs to gain `inner_tol` relative to the current
-    // residual, but never more than the overall target requires
+    KCastDown<<<nb, kB, 0, stream_>>>(w, r_, r32_, e32_);
     double need = tol * std::sqrt(bb / rr);
-    double itol = std::max(inner_tol, need * 0.5);
-    bool pc_failed = false;
-    int it = InnerFp32(itol, max_iters - total, fallback ? nullptr : pc, &pc_failed);
-    total += it;
-    if (pc_failed) {
-      // the network broke the descent property: restart this round with
-      // Jacobi-PCG, keeping the FP64 tolerance
-      fallback = true;
-      res.used_fallback = true;
-      continue;
-    }
-    KAddUp<<<blocks_, kB, 0, stream_>>>(n_, e32_, x);
+    double itol = std::max(1e-4, need * 0.5);
+    total += InnerFp32(itol, max_iters - total, nullptr, nullptr);
+    KAddUp<<<nb, kB, 0, stream_>>>(w, e32_, x);
     rr = ResidualFp64(b_, x, r_);
     res.outer += 1;
     if (!(rr == rr)) break;

@@ -23,6 +23,7 @@
 #include "host_geometry.h"
 #include "kernels.h"
 #include "learned_net.h"
+#include "mg.h"
 
 #define HG_CHECK(x)                                                                  \
   do {                                                                               \
@@ -106,7 +107,7 @@ struct hg_ctx {
   double *As, *Aw, *Ac, *Ae, *An, *k0, *k1, *x0, *x1;
   double *ap, *fstar, *fcoef;
   double *Ps, *Pw, *Pc, *Pe, *Pn, *Pk, *pcorr, *pb, *kface, *zeros;
-  double *xn_d, *yn_d, *stat_part, *stat_out;
+  double *xn_d, *yn_d, *stat_part, *stat_out, *rs_part;
   unsigned long long* rs_d;
   int* nan_d;
   unsigned long long* gs_info;
@@ -126,6 +127,7 @@ struct hg_ctx {
   std::string err;
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
+  std::unique_ptr<hg::MgPrecond> mg;
   hipEvent_t ev[4];
   double timing[3] = {0, 0, 0};
 
@@ -145,6 +147,7 @@ struct hg_ctx {
     if (stat_h) (void)hipHostFree(stat_h);
     cg.reset();
     net.reset();
+    mg.reset();
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -340,9 +343,12 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
       KNegateFaces<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fcoef, c->kface);
       KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
       c->cg->SetOperator(c->kface, c->kface + c->geo.nfx);
-      hg::LearnedPrecond* pc = nullptr;
-      if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED && c->net && c->net->Trained())
-        pc = c->net.get();
+      hg::Precond* pc = nullptr;
+      if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED) {
+        c->mg->Build(c->kface, c->kface + c->geo.nfx, c->st);
+        pc = (c->net && c->net->Trained()) ? static_cast<hg::Precond*>(c->net.get())
+                                           : static_cast<hg::Precond*>(c->mg.get());
+      }
       hg::CgResult r = c->cg->Solve(c->pb, c->pcorr, cfg.pressure_tolerance_gpu,
                                     cfg.pressure_max_iters_gpu, cfg.precision_gpu, true, pc);
       is->pressure_solve_iters = r.iterations;
@@ -389,10 +395,9 @@ void Iterate(hg_ctx* c, hg_iter_stats* is) {
   hg::LaunchPressureRows(c->st, c->geo, c->fstar, c->fcoef, c->pinned, cfg.pressure_fixed_value,
                          c->Ps, c->Pw, c->Pc, c->Pe, c->Pn, c->Pk);
   SolvePressure(c, is);
-  HG_CHECK(hipMemsetAsync(c->rs_d, 0, 8, c->st));
   hg::LaunchCorrectCells(c->st, c->geo, c->pcorr, c->ap, c->p_ip,
                          cfg.pressure_relaxation_factor, c->p_ic, c->u_ic[0], c->u_ic[1],
-                         c->u_ip[0], c->u_ip[1], c->rs_d);
+                         c->u_ip[0], c->u_ip[1], c->rs_part, (double*)c->rs_d);
   hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic);
   HG_CHECK(hipMemcpyAsync(c->rs_h, c->rs_d, 8, hipMemcpyDeviceToHost, c->st));
   c->Sync();
@@ -526,6 +531,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     c->yn_d = c->Alloc(cfg.ny + 1);
     c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
     c->stat_out = c->Alloc(8);
+    c->rs_part = c->Alloc((nc + 255) / 256);
     c->rs_d = (unsigned long long*)c->Alloc(1);
     c->nan_d = (int*)c->Alloc(1);
     c->gs_info = (unsigned long long*)c->Alloc(2);
@@ -540,6 +546,8 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     if (cfg.linear_solver_pressure == HG_SOLVER_CG ||
         cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED)
       c->cg.reset(new hg::CgSolver(cfg.nx, cfg.ny, c->st));
+    if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED)
+      c->mg.reset(new hg::MgPrecond(cfg.nx, cfg.ny));
     Upload(c.get(), c->xn_d, c->mesh.xn);
     Upload(c.get(), c->yn_d, c->mesh.yn);
 
@@ -652,6 +660,13 @@ static double* FieldPtr(hg_ctx* c, int f, bool write, long* n) {
     case HG_FIELD_FORCE_X: return c->fx;
     case HG_FIELD_FORCE_Y: return c->fy;
     case HG_FIELD_VOLUME_FLUX_ITER: *n = c->nf; return c->fl_ic;
+    case HG_FIELD_PRESSURE_FACE_COEF:
+      *n = c->nf;
+      KNegateFaces<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fcoef, c->kface);
+      return c->kface;
+    case HG_FIELD_PRESSURE_RHS:
+      KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
+      return c->pb;
     default: return nullptr;
   }
 }

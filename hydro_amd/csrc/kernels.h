@@ -50,8 +50,8 @@ void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const
                         double* n, double* k);
 void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
                         const double* pprev, double alpha_p, double* pcurr, double* u0,
-                        double* u1, const double* up0, const double* up1,
-                        unsigned long long* rs);
+                        double* u1, const double* up0, const double* up1, double* part,
+                        double* rs);
 void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         const double* pc, double* flux);
 void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,

@@ -7,12 +7,12 @@
 
 namespace hg {
 
-class LearnedNet : public LearnedPrecond {
+class LearnedNet : public Precond {
  public:
   LearnedNet(int nx, int ny, int levels, int channels, hipStream_t st);
   ~LearnedNet() override;
   bool Trained() const { return trained_; }
-  void Apply(const float* r, float* z, hipStream_t st, const int* d_done) override;
+  void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;
 
  private:
   int nx_, ny_, levels_, channels_;

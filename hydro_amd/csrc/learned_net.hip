@@ -6,6 +6,6 @@ namespace hg {
 LearnedNet::LearnedNet(int nx, int ny, int levels, int channels, hipStream_t st)
     : nx_(nx), ny_(ny), levels_(levels), channels_(channels), st_(st) {}
 LearnedNet::~LearnedNet() {}
-void LearnedNet::Apply(const float*, float*, hipStream_t, const int*) {}
+void LearnedNet::Apply(const double*, float*, hipStream_t, const int*) {}
 
 }  // namespace hg

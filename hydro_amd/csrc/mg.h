@@ -1,0 +1,48 @@
+// mg.h -- FP32 cell-centred multigrid V-cycle for the pressure operator.
+//
+// Face-form operator (A p)_c = sum_f k_f (p_c - p_nb).  Coarse levels halve
+// both dimensions while they stay even: coarse face coefficient = mean of
+// the two fine faces it covers (rediscretisation; in 2D k_f = A/(d a_f) does
+// not scale with h), restriction = sum of the four children's residuals,
+// prolongation = injection, smoother = damped Jacobi (omega 0.8, 2 pre +
+// 2 post sweeps), coarsest grid (<= 1024 cells) relaxed in one workgroup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "learned.h"
+#include "tile.h"
+
+namespace hg {
+
+struct MgLevel {
+  int nx, ny;
+  TileGrid tg;
+  float *kx = nullptr, *ky = nullptr, *dinv = nullptr;
+  float *r = nullptr, *xa = nullptr, *xb = nullptr;
+};
+
+class MgPrecond : public Precond {
+ public:
+  MgPrecond(int nx, int ny);
+  ~MgPrecond();
+  MgPrecond(const MgPrecond&) = delete;
+  MgPrecond& operator=(const MgPrecond&) = delete;
+  // FP64 face coefficients of the finest level (device pointers).
+  void Build(const double* kx, const double* ky, hipStream_t st);
+  // z = V(r): r is the FP64 residual of the finest level; z FP32.  Does
+  // nothing when *d_done is set.
+  void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;
+  int levels() const { return int(lv_.size()); }
+  const MgLevel& level(int l) const { return lv_[l]; }
+  float omega = 0.8f;
+  int coarse_sweeps = 40;
+
+ private:
+  std::vector<MgLevel> lv_;
+  void Cycle(int l, const double* r64, float* out, hipStream_t st, const int* d_done);
+};
+
+}  // namespace hg

@@ -1,0 +1,320 @@
+// mg.hip -- FP32 multigrid V-cycle preconditioner (see mg.h).
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "mg.h"
+
+#define HG_CHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") +            \
+                                                   hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+namespace hg {
+
+namespace {
+
+constexpr int kCoarsestMax = 1024;
+
+__device__ __forceinline__ bool Done(const int* d) { return d && *d; }
+
+// A x at (i,j) for face-form coefficients.
+template <class T>
+__device__ __forceinline__ float Ax(int nx, int ny, const float* __restrict__ kx,
+                                    const float* __restrict__ ky, const T* __restrict__ x, int i,
+                                    int j) {
+  const int c = i + nx * j;
+  const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+  const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+  const float xc = float(x[c]);
+  float v = (((kw + ke) + ks) + kn) * xc;
+  if (i > 0) v -= kw * float(x[c - 1]);
+  if (i < nx - 1) v -= ke * float(x[c + 1]);
+  if (j > 0) v -= ks * float(x[c - nx]);
+  if (j < ny - 1) v -= kn * float(x[c + nx]);
+  return v;
+}
+
+__global__ void __launch_bounds__(kTileThreads)
+KCastCoeff(int nx, int ny, const double* __restrict__ kx, const double* __restrict__ ky,
+           float* __restrict__ kx32, float* __restrict__ ky32, float* __restrict__ dinv) {
+  const long nfx = long(nx + 1) * ny, nfy = long(nx) * (ny + 1), n = long(nx) * ny;
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (long f = blockIdx.x * long(blockDim.x) + threadIdx.x; f < nfx; f += stride)
+    kx32[f] = float(kx[f]);
+  for (long f = blockIdx.x * long(blockDim.x) + threadIdx.x; f < nfy; f += stride)
+    ky32[f] = float(ky[f]);
+  for (long c = blockIdx.x * long(blockDim.x) + threadIdx.x; c < n; c += stride) {
+    const int i = int(c % nx), j = int(c / nx);
+    const double d = ((kx[i + long(nx + 1) * j] + kx[i + 1 + long(nx + 1) * j]) +
+                      ky[i + long(nx) * j]) + ky[i + long(nx) * (j + 1)];
+    dinv[c] = d > 0. ? float(1. / d) : 0.f;
+  }
+}
+
+// Coarse coefficients: mean of the two fine faces a coarse face covers.
+__global__ void __launch_bounds__(256)
+KCoarsen(int nxf, int nyf, const float* __restrict__ kxf, const float* __restrict__ kyf,
+         int nxc, int nyc, float* __restrict__ kxc, float* __restrict__ kyc,
+         float* __restrict__ dinvc) {
+  const long nfx = long(nxc + 1) * nyc, nfy = long(nxc) * (nyc + 1), n = long(nxc) * nyc;
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (long f = blockIdx.x * long(blockDim.x) + threadIdx.x; f < nfx; f += stride) {
+    const int I = int(f % (nxc + 1)), J = int(f / (nxc + 1));
+    kxc[f] = 0.5f * (kxf[2 * I + (nxf + 1) * (2 * J)] + kxf[2 * I + (nxf + 1) * (2 * J + 1)]);
+  }
+  for (long f = blockIdx.x * long(blockDim.x) + threadIdx.x; f < nfy; f += stride) {
+    const int I = int(f % nxc), J = int(f / nxc);
+    kyc[f] = 0.5f * (kyf[2 * I + nxf * (2 * J)] + kyf[2 * I + 1 + nxf * (2 * J)]);
+  }
+  (void)nyf;
+  (void)n;
+}
+
+__global__ void __launch_bounds__(256)
+KDinv32(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
+        float* __restrict__ dinv) {
+  const long n = long(nx) * ny;
+  for (long c = blockIdx.x * long(blockDim.x) + threadIdx.x; c < n;
+       c += long(gridDim.x) * blockDim.x) {
+    const int i = int(c % nx), j = int(c / nx);
+    const float d = ((kx[i + (nx + 1) * j] + kx[i + 1 + (nx + 1) * j]) + ky[i + nx * j]) +
+                    ky[i + nx * (j + 1)];
+    dinv[c] = d > 0.f ? 1.f / d : 0.f;
+  }
+}
+
+// First pre-smoothing sweep from zero: x = omega D^-1 r.
+template <class RT>
+__global__ void __launch_bounds__(kTileThreads)
+KSmoothFirst(int nx, int ny, TileGrid tg, const RT* __restrict__ r,
+             const float* __restrict__ dinv, float omega, float* __restrict__ x,
+             const int* done) {
+  if (Done(done)) return;
+  int i, j;
+  if (!TileCell(tg, nx, ny, i, j)) return;
+  const int c = i + nx * j;
+  x[c] = omega * dinv[c] * float(r[c]);
+}
+
+// Damped Jacobi sweep: xn = x + omega D^-1 (r - A x).
+template <class RT>
+__global__ void __launch_bounds__(kTileThreads)
+KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
+        const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x,
+        float omega, float* __restrict__ xn, const int* done) {
+  if (Done(done)) return;
+  int i, j;
+  if (!TileCell(tg, nx, ny, i, j)) return;
+  const int c = i + nx * j;
+  xn[c] = x[c] + omega * dinv[c] * (float(r[c]) - Ax(nx, ny, kx, ky, x, i, j));
+}
+
+// Residual of the fine level summed over each coarse cell's four children.
+template <class RT>
+__global__ void __launch_bounds__(kTileThreads)
+KResRestrict(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
+             const RT* __restrict__ r, const float* __restrict__ x, int nxc, int nyc,
+             TileGrid tgc, float* __restrict__ rc, const int* done) {
+  if (Done(done)) return;
+  int I, J;
+  if (!TileCell(tgc, nxc, nyc, I, J)) return;
+  float s = 0.f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
+    s += float(r[i + nx * j]) - Ax(nx, ny, kx, ky, x, i, j);
+  }
+  rc[I + nxc * J] = s;
+}
+
+// Prolongation by injection fused with a Jacobi sweep:
+// y = x + P e, xn = y + omega D^-1 (r - A y).
+template <class RT>
+__global__ void __launch_bounds__(kTileThreads)
+KProlongJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx,
+               const float* __restrict__ ky, const float* __restrict__ dinv,
+               const RT* __restrict__ r, const float* __restrict__ x,
+               const float* __restrict__ e, int nxc, float omega, float* __restrict__ xn,
+               const int* done) {
+  if (Done(done)) return;
+  int i, j;
+  if (!TileCell(tg, nx, ny, i, j)) return;
+  auto Y = [&](int ii, int jj) { return x[ii + nx * jj] + e[(ii >> 1) + nxc * (jj >> 1)]; };
+  const int c = i + nx * j;
+  const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+  const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+  const float yc = Y(i, j);
+  float ay = (((kw + ke) + ks) + kn) * yc;
+  if (i > 0) ay -= kw * Y(i - 1, j);
+  if (i < nx - 1) ay -= ke * Y(i + 1, j);
+  if (j > 0) ay -= ks * Y(i, j - 1);
+  if (j < ny - 1) ay -= kn * Y(i, j + 1);
+  xn[c] = yc + omega * dinv[c] * (float(r[c]) - ay);
+}
+
+// Coarsest level: Jacobi sweeps in one workgroup, x and r in LDS.
+__global__ void __launch_bounds__(kCoarsestMax)
+KCoarsest(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
+          const float* __restrict__ dinv, const float* __restrict__ r, float omega, int sweeps,
+          float* __restrict__ out, const int* done) {
+  if (Done(done)) return;
+  __shared__ float xs[2][kCoarsestMax];
+  const int n = nx * ny;
+  const int c = threadIdx.x;
+  const bool act = c < n;
+  float kw = 0, ke = 0, ks = 0, kn = 0, di = 0, rc = 0;
+  int i = 0, j = 0;
+  if (act) {
+    i = c % nx;
+    j = c / nx;
+    kw = kx[i + (nx + 1) * j];
+    ke = kx[i + 1 + (nx + 1) * j];
+    ks = ky[i + nx * j];
+    kn = ky[i + nx * (j + 1)];
+    di = dinv[c];
+    rc = r[c];
+    xs[0][c] = omega * di * rc;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int s = 1; s < sweeps; ++s) {
+    if (act) {
+      const float* x = xs[cur];
+      float v = (((kw + ke) + ks) + kn) * x[c];
+      if (i > 0) v -= kw * x[c - 1];
+      if (i < nx - 1) v -= ke * x[c + 1];
+      if (j > 0) v -= ks * x[c - nx];
+      if (j < ny - 1) v -= kn * x[c + nx];
+      xs[cur ^ 1][c] = x[c] + omega * di * (rc - v);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (act) out[c] = xs[cur][c];
+}
+
+inline int Blocks1D(long n) { return int(std::min<long>((n + 255) / 256, 4096)); }
+
+}  // namespace
+
+MgPrecond::MgPrecond(int nx, int ny) {
+  int cx = nx, cy = ny;
+  while (true) {
+    MgLevel L;
+    L.nx = cx;
+    L.ny = cy;
+    L.tg = MakeTileGrid(cx, cy);
+    long n = long(cx) * cy;
+    HG_CHECK(hipMalloc(&L.kx, size_t(cx + 1) * cy * 4));
+    HG_CHECK(hipMalloc(&L.ky, size_t(cx) * (cy + 1) * 4));
+    HG_CHECK(hipMalloc(&L.dinv, n * 4));
+    HG_CHECK(hipMalloc(&L.r, n * 4));
+    HG_CHECK(hipMalloc(&L.xa, n * 4));
+    HG_CHECK(hipMalloc(&L.xb, n * 4));
+    lv_.push_back(L);
+    if ((cx & 1) || (cy & 1) || n <= kCoarsestMax / 2 || cx < 4 || cy < 4) break;
+    cx /= 2;
+    cy /= 2;
+  }
+  if (long(lv_.back().nx) * lv_.back().ny > kCoarsestMax)
+    coarse_sweeps = -1;  // coarsest too large for one workgroup: relax with grid kernels
+}
+
+MgPrecond::~MgPrecond() {
+  for (auto& L : lv_)
+    for (void* p : {(void*)L.kx, (void*)L.ky, (void*)L.dinv, (void*)L.r, (void*)L.xa, (void*)L.xb})
+      if (p) (void)hipFree(p);
+}
+
+void MgPrecond::Build(const double* kx, const double* ky, hipStream_t st) {
+  MgLevel& L0 = lv_[0];
+  KCastCoeff<<<Blocks1D(long(L0.nx + 1) * L0.ny), 256, 0, st>>>(L0.nx, L0.ny, kx, ky, L0.kx,
+                                                                 L0.ky, L0.dinv);
+  for (size_t l = 1; l < lv_.size(); ++l) {
+    MgLevel& F = lv_[l - 1];
+    MgLevel& C = lv_[l];
+    KCoarsen<<<Blocks1D(long(C.nx + 1) * (C.ny + 1)), 256, 0, st>>>(F.nx, F.ny, F.kx, F.ky,
+                                                                    C.nx, C.ny, C.kx, C.ky,
+                                                                    C.dinv);
+    KDinv32<<<Blocks1D(long(C.nx) * C.ny), 256, 0, st>>>(C.nx, C.ny, C.kx, C.ky, C.dinv);
+  }
+}
+
+void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, const int* done) {
+  MgLevel& L = lv_[l];
+  const bool coarsest = (l == int(lv_.size()) - 1);
+  const int blocks = L.tg.blocks();
+  if (coarsest && coarse_sweeps > 0) {
+    // r of the coarsest level is always FP32 (lv_.size() > 1 is guaranteed for
+    // fine grids; a single-level hierarchy relaxes the FP64 residual below)
+    if (l > 0) {
+      KCoarsest<<<1, kCoarsestMax, 0, st>>>(L.nx, L.ny, L.kx, L.ky, L.dinv, L.r, omega,
+                                             coarse_sweeps, out, done);
+      return;
+    }
+  }
+  if (l == 0)
+    KSmoothFirst<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, r64, L.dinv, omega,
+                                                          L.xa, done);
+  else
+    KSmoothFirst<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.r, L.dinv, omega,
+                                                         L.xa, done);
+  if (coarsest) {
+    // no coarser level (odd or tiny grids): seven more Jacobi sweeps ending
+    // in `out`, ping-ponging between xa and xb
+    float* cur = L.xa;
+    const int nsw = 7;
+    for (int s = 0; s < nsw; ++s) {
+      float* dst = (s == nsw - 1) ? out : (cur == L.xa ? L.xb : L.xa);
+      if (dst == cur) dst = (cur == L.xa) ? L.xb : L.xa;
+      if (l == 0)
+        KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
+                                                         cur, omega, dst, done);
+      else
+        KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
+                                                        cur, omega, dst, done);
+      cur = dst;
+    }
+    if (cur != out)
+      HG_CHECK(hipMemcpyAsync(out, cur, size_t(L.nx) * L.ny * 4, hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  MgLevel& C = lv_[l + 1];
+  if (l == 0) {
+    KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
+                                                     L.xa, omega, L.xb, done);
+    KResRestrict<double><<<C.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.kx, L.ky, r64,
+                                                                 L.xb, C.nx, C.ny, C.tg, C.r,
+                                                                 done);
+  } else {
+    KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
+                                                    L.xa, omega, L.xb, done);
+    KResRestrict<float><<<C.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.kx, L.ky, L.r,
+                                                                L.xb, C.nx, C.ny, C.tg, C.r,
+                                                                done);
+  }
+  Cycle(l + 1, nullptr, C.xb, st, done);
+  if (l == 0) {
+    KProlongJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+                                                            r64, L.xb, C.xb, C.nx, omega, L.xa,
+                                                            done);
+    KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
+                                                     L.xa, omega, out, done);
+  } else {
+    KProlongJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+                                                           L.r, L.xb, C.xb, C.nx, omega, L.xa,
+                                                           done);
+    KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
+                                                    L.xa, omega, out, done);
+  }
+}
+
+void MgPrecond::Apply(const double* r, float* z, hipStream_t st, const int* d_done) {
+  Cycle(0, r, z, st, d_done);
+}
+
+}  // namespace hg

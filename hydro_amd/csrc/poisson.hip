@@ -17,6 +17,7 @@
 #include "cg.h"
 #include "kernels.h"
 #include "learned_net.h"
+#include "mg.h"
 
 #define HG_CHECK(x)                                                                  \
   do {                                                                               \
@@ -42,11 +43,13 @@ struct hg_poisson {
   hipStream_t st = nullptr;
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
+  std::unique_ptr<hg::MgPrecond> mg;
   double *b = nullptr, *x = nullptr;
   ~hg_poisson() {
     if (st) (void)hipStreamSynchronize(st);
     cg.reset();
     net.reset();
+    mg.reset();
     if (b) (void)hipFree(b);
     if (x) (void)hipFree(x);
     if (st) (void)hipStreamDestroy(st);
@@ -181,6 +184,8 @@ hg_status hg_poisson_create(int32_t nx, int32_t ny, const double* kx, const doub
     HG_CHECK(hipMemcpyAsync(dkx.p, kx, nfx * 8, hipMemcpyHostToDevice, ps->st));
     HG_CHECK(hipMemcpyAsync(dky.p, ky, nfy * 8, hipMemcpyHostToDevice, ps->st));
     ps->cg->SetOperator(dkx.p, dky.p);
+    ps->mg.reset(new hg::MgPrecond(nx, ny));
+    ps->mg->Build(dkx.p, dky.p, ps->st);
     HG_CHECK(hipMalloc(&ps->b, N * 8));
     HG_CHECK(hipMalloc(&ps->x, N * 8));
     HG_CHECK(hipStreamSynchronize(ps->st));
@@ -202,9 +207,10 @@ hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
   try {
     const size_t N = size_t(ps->nx) * ps->ny;
     HG_CHECK(hipMemcpyAsync(ps->b, rhs, N * 8, hipMemcpyHostToDevice, ps->st));
-    hg::LearnedPrecond* pc = nullptr;
-    if (opts->solver == HG_SOLVER_FCG_LEARNED && ps->net && ps->net->Trained())
-      pc = ps->net.get();
+    hg::Precond* pc = nullptr;
+    if (opts->solver == HG_SOLVER_FCG_LEARNED)
+      pc = (ps->net && ps->net->Trained()) ? static_cast<hg::Precond*>(ps->net.get())
+                                           : static_cast<hg::Precond*>(ps->mg.get());
     hg::CgResult r = ps->cg->Solve(ps->b, ps->x, opts->tolerance, opts->max_iters,
                                    opts->precision, true, pc);
     HG_CHECK(hipMemcpyAsync(p, ps->x, N * 8, hipMemcpyDeviceToHost, ps->st));

@@ -382,13 +382,13 @@ KPressureRows(Geo g, const double* __restrict__ fstar, const double* __restrict_
 }
 
 // Correction (fluid.hpp:1033-1057) for cells, with the convergence
-// indicator max_c |u_curr - u_prev| (solver.hpp:804-813) reduced into *rs
-// (non-negative doubles ordered as unsigned integers).
+// indicator max_c |u_curr - u_prev| (solver.hpp:804-813): per-block maxima,
+// folded by KMaxFold (max is order-independent, so the result is exact).
 __global__ void __launch_bounds__(kBlock)
 KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ ap,
               const double* __restrict__ pprev, double alpha_p, double* __restrict__ pcurr,
               double* __restrict__ u0, double* __restrict__ u1, const double* __restrict__ up0,
-              const double* __restrict__ up1, unsigned long long* __restrict__ rs) {
+              const double* __restrict__ up1, double* __restrict__ part) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.;
   if (c < g.nx * g.ny) {
@@ -416,7 +416,23 @@ KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ a
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, red[w]);
-    atomicMax(rs, (unsigned long long)__double_as_longlong(m));
+    part[blockIdx.x] = m;
+  }
+}
+
+// Folds the per-block maxima of KCorrectCells into rs (one workgroup).
+__global__ void __launch_bounds__(kBlock)
+KMaxFold(const double* __restrict__ part, int n, double* __restrict__ rs) {
+  double m = 0.;
+  for (int b = threadIdx.x; b < n; b += kBlock) m = fmax(m, part[b]);
+  m = WaveMax(m);
+  __shared__ double red[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) t = fmax(t, red[w]);
+    *rs = t;
   }
 }
 
@@ -704,9 +720,11 @@ void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const
 void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
                         const double* pprev, double alpha_p, double* pcurr, double* u0,
                         double* u1, const double* up0, const double* up1,
-                        unsigned long long* rs) {
-  KCorrectCells<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, pc, ap, pprev, alpha_p, pcurr, u0,
-                                                             u1, up0, up1, rs);
+                        double* part, double* rs) {
+  const long n = long(g.nx) * g.ny;
+  KCorrectCells<<<Grid(n), kBlock, 0, st>>>(g, pc, ap, pprev, alpha_p, pcurr, u0, u1, up0, up1,
+                                            part);
+  KMaxFold<<<1, kBlock, 0, st>>>(part, int((n + kBlock - 1) / kBlock), rs);
 }
 void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         const double* pc, double* flux) {

@@ -17,7 +17,8 @@ import numpy as np
 
 from . import capi
 
-FACE_FIELDS = (capi.HG_FIELD_VOLUME_FLUX, capi.HG_FIELD_VOLUME_FLUX_ITER)
+FACE_FIELDS = (capi.HG_FIELD_VOLUME_FLUX, capi.HG_FIELD_VOLUME_FLUX_ITER,
+               capi.HG_FIELD_PRESSURE_FACE_COEF)
 
 
 class HydroError(RuntimeError):

@@ -1,0 +1,85 @@
+"""Synthetic workloads of the benchmark and the tests (host-side numpy).
+
+* Rayleigh-Taylor initial state: heavy fluid (phase 1) above
+  y = 1 + 0.01 cos(2 pi x) + per-column noise U(-1e-4, 1e-4), with the
+  cell-averaged volume fraction from the area above the interface
+  (16-point Gauss-Legendre in x, exact clipping in y).
+* Variable-coefficient Poisson operators: seeded two-phase volume-fraction
+  snapshots at density ratio 1000, face coefficients k_f = A / (d rho_f),
+  and a Neumann-compatible right-hand side (the discrete divergence of a
+  smooth random velocity).
+"""
+import numpy as np
+
+from . import capi
+
+
+def rt_volume_fraction(nx, ny, lx=1.0, ly=2.0, amp=0.01, noise=1e-4, seed=2026):
+    rng = np.random.default_rng(seed)
+    xn = np.arange(nx + 1) / nx * lx
+    yn = np.arange(ny + 1) / ny * ly
+    col_noise = rng.uniform(-noise, noise, nx)
+    gx, gw = np.polynomial.legendre.leggauss(16)
+    x0, x1 = xn[:-1], xn[1:]
+    xq = 0.5 * (x1 - x0)[:, None] * gx[None, :] + 0.5 * (x0 + x1)[:, None]   # (nx, 16)
+    h = 0.5 * ly + amp * np.cos(2 * np.pi * xq) + col_noise[:, None]        # (nx, 16)
+    alpha = np.zeros((ny, nx))
+    dy = yn[1:] - yn[:-1]
+    lo, hi = h.min() - 1e-12, h.max() + 1e-12
+    j_lo = max(int(np.searchsorted(yn, lo)) - 1, 0)
+    j_hi = min(int(np.searchsorted(yn, hi)) + 1, ny)
+    alpha[j_hi:, :] = 1.0
+    for j in range(j_lo, j_hi):
+        above = np.clip(yn[j + 1] - np.maximum(h, yn[j]), 0.0, dy[j]) / dy[j]   # (nx, 16)
+        alpha[j, :] = 0.5 * (above * gw[None, :]).sum(axis=1)
+    return alpha.reshape(-1)
+
+
+def set_rt_state(hydro, rho_light=1.0, rho_heavy=3.0, **kw):
+    """Puts the RT interface into a Hydro (or Oracle) case and refreshes the
+    derived fields (initial flux, properties)."""
+    cfg = hydro.cfg
+    a = rt_volume_fraction(cfg.nx, cfg.ny, lx=cfg.b[0] - cfg.a[0], ly=cfg.b[1] - cfg.a[1], **kw)
+    hydro.set(capi.HG_FIELD_PARTIAL_DENSITY_0 + 1, a * rho_heavy)
+    hydro.set(capi.HG_FIELD_PARTIAL_DENSITY_0, (1.0 - a) * rho_light)
+    hydro.refresh()
+    return a
+
+
+def _smooth_once(a):
+    p = np.pad(a, 1, mode="edge")
+    return (4 * a + p[:-2, 1:-1] + p[2:, 1:-1] + p[1:-1, :-2] + p[1:-1, 2:]) / 8.0
+
+
+def poisson_problem(n=512, seed=0, ratio=1000.0):
+    """(kx, ky, rhs) of a variable-coefficient pressure system on an n x n
+    unit square: rho = 1 + (ratio-1) alpha from random discs and a layer."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(xc, xc)
+    alpha = (Y < rng.uniform(0.2, 0.35)).astype(float)
+    for _ in range(int(rng.integers(8, 17))):
+        cx, cy = rng.uniform(0.1, 0.9, 2)
+        rx, ry = rng.uniform(0.03, 0.12, 2)
+        alpha[((X - cx) / rx) ** 2 + ((Y - cy) / ry) ** 2 < 1.0] = 1.0
+    alpha = _smooth_once(alpha)
+    rho = 1.0 + (ratio - 1.0) * alpha
+    kx = np.zeros((n, n + 1))
+    ky = np.zeros((n + 1, n))
+    kx[:, 1:-1] = h / (h * 0.5 * (rho[:, :-1] + rho[:, 1:]))
+    ky[1:-1, :] = h / (h * 0.5 * (rho[:-1, :] + rho[1:, :]))
+    # rhs: divergence of a smooth random velocity (32 Fourier modes)
+    xf = np.arange(n + 1) * h
+    u = np.zeros((n, n + 1))
+    v = np.zeros((n + 1, n))
+    for _ in range(32):
+        kx_, ky_ = rng.integers(1, 9, 2)
+        a, b = rng.normal(size=2)
+        ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+        u += a * np.sin(np.pi * kx_ * xf[None, :]) * np.cos(np.pi * ky_ * xc[:, None] + ph1)
+        v += b * np.sin(np.pi * ky_ * xf[:, None]) * np.cos(np.pi * kx_ * xc[None, :] + ph2)
+    div = (u[:, 1:] - u[:, :-1]) * h + (v[1:, :] - v[:-1, :]) * h
+    rhs = div.reshape(-1)
+    rhs = rhs - rhs.mean()
+    return kx.reshape(-1), ky.reshape(-1), rhs

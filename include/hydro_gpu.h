@@ -155,7 +155,12 @@ typedef enum hg_field {
   HG_FIELD_FORCE_X = 13,            /* (read only) */
   HG_FIELD_FORCE_Y = 14,            /* (read only) */
   HG_FIELD_VOLUME_FLUX_ITER = 15,   /* faces, iter_curr layer (advection input) */
-  HG_FIELD_COUNT = 16
+  /* The last SIMPLE pressure-correction system in face form (read only):
+   * k_f = A/(d a_f) on inner faces, 0 on walls, and the cell right-hand
+   * side b with A p' = b (fluid.hpp:946-1027). */
+  HG_FIELD_PRESSURE_FACE_COEF = 16, /* faces */
+  HG_FIELD_PRESSURE_RHS = 17,       /* cells */
+  HG_FIELD_COUNT = 18
 } hg_field;
 
 typedef struct hg_iter_stats {

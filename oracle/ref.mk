@@ -20,7 +20,13 @@ REF_SRCS := $(SRC)/common/common.cpp $(SRC)/common/linear_algebra.cpp \
 CXX ?= g++
 COMMON := -std=c++11 -w -I$(SRC) -DMODULE_HYDRO_2D -fopenmp -pthread
 
-all: $(OUT)/hydro_ref $(OUT)/hydro_ref_fast
+all: $(OUT)/hydro_ref $(OUT)/hydro_ref_fast $(OUT)/general.hydroconf
+
+# The reference binary reads every parameter it uses; its defaults file
+# goes next to it (an output of this build, not a committed file).
+$(OUT)/general.hydroconf: $(REF)/examples/general.hydroconf
+	mkdir -p $(OUT)
+	cp $< $@
 
 $(OUT)/hydro_ref: oracle/ref_shim.cpp $(REF_SRCS)
 	mkdir -p $(OUT)
