@@ -17,8 +17,9 @@ STATUS_NAMES = {0: "HG_OK", 1: "HG_ERR_NAN", 2: "HG_ERR_NOT_CONVERGED",
                 6: "HG_ERR_PARSE"}
 
 SOLVERS = {"lu": 0, "lu_relaxed": 1, "gauss_seidel": 2, "jacobi": 3,
-           "lu_rb": 4, "cg": 5, "fcg_learned": 6}
+           "lu_rb": 4, "cg": 5, "fcg_learned": 6, "fcg_mg": 7}
 HG_PREC_FP64, HG_PREC_MIXED = 0, 1
+HG_TRAIN_LINEAR, HG_TRAIN_STANDALONE, HG_TRAIN_TIED = 1, 2, 4
 
 HG_FIELD_VELOCITY_X = 0
 HG_FIELD_VELOCITY_Y = 1
@@ -132,6 +133,20 @@ class HgSolveStats(ctypes.Structure):
                 ("ms", ctypes.c_double)]
 
 
+class HgTrainOpts(ctypes.Structure):
+    _fields_ = [("rhs_count", ctypes.c_int32), ("mg_iters", ctypes.c_int32),
+                ("steps", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("seed", ctypes.c_uint64), ("flags", ctypes.c_int32)]
+
+
+class HgTrainStats(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("loss_first", ctypes.c_double), ("loss_last", ctypes.c_double),
+                ("ms", ctypes.c_double), ("accepted", ctypes.c_int32),
+                ("val_iters", ctypes.c_int32), ("val_iters_baseline", ctypes.c_int32),
+                ("val_ms", ctypes.c_double), ("val_ms_baseline", ctypes.c_double)]
+
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libhydrogpu.so")
 
@@ -145,10 +160,16 @@ EXPORTS = [
     "hg_calc_stats", "hg_step", "hg_last_step_timing",
     "hg_stencil_solve", "hg_poisson_create", "hg_poisson_destroy",
     "hg_poisson_solve",
+    "hg_train_opts_default", "hg_poisson_train", "hg_train_pressure_net",
+    "hg_net_create", "hg_net_destroy", "hg_net_num_weights", "hg_net_configure",
+    "hg_net_set_weights",
+    "hg_net_get_weights", "hg_net_forward", "hg_net_gradient", "hg_net_train_step",
+    "hg_net_time",
 ]
 
 _P = ctypes.POINTER
 _dp = _P(ctypes.c_double)
+_fp = _P(ctypes.c_float)
 
 
 def _declare(lib):
@@ -180,6 +201,19 @@ def _declare(lib):
         "hg_poisson_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, _dp, _P(vp)]),
         "hg_poisson_destroy": (None, [vp]),
         "hg_poisson_solve": (ctypes.c_int, [vp, _dp, _dp, _P(HgSolveOpts), _P(HgSolveStats)]),
+        "hg_train_opts_default": (None, [_P(HgTrainOpts)]),
+        "hg_poisson_train": (ctypes.c_int, [vp, _P(HgTrainOpts), _P(HgTrainStats)]),
+        "hg_train_pressure_net": (ctypes.c_int, [vp, _P(HgTrainOpts), _P(HgTrainStats)]),
+        "hg_net_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _P(vp)]),
+        "hg_net_destroy": (None, [vp]),
+        "hg_net_num_weights": (ctypes.c_int64, [vp]),
+        "hg_net_configure": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "hg_net_set_weights": (ctypes.c_int, [vp, _fp]),
+        "hg_net_get_weights": (ctypes.c_int, [vp, _fp]),
+        "hg_net_forward": (ctypes.c_int, [vp, _fp, _fp]),
+        "hg_net_gradient": (ctypes.c_int, [vp, _fp, _fp, _fp]),
+        "hg_net_train_step": (ctypes.c_int, [vp, _fp, _fp, ctypes.c_float, _dp]),
+        "hg_net_time": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -227,3 +261,8 @@ def parse_config(text, cfg=None, lib=None):
 def dptr(arr):
     """Pointer to a C-contiguous float64 numpy array."""
     return arr.ctypes.data_as(_dp)
+
+
+def fptr(arr):
+    """Pointer to a C-contiguous float32 numpy array."""
+    return arr.ctypes.data_as(_fp)

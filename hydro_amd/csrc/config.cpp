@@ -117,6 +117,7 @@ int32_t hg_solver_from_name(const char* name) {
   if (n == "lu_rb") return HG_SOLVER_LU_RB;
   if (n == "cg") return HG_SOLVER_CG;
   if (n == "fcg_learned") return HG_SOLVER_FCG_LEARNED;
+  if (n == "fcg_mg") return HG_SOLVER_FCG_MG;
   return -1;
 }
 

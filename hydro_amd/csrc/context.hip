@@ -128,6 +128,7 @@ struct hg_ctx {
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
   std::unique_ptr<hg::MgPrecond> mg;
+  bool pressure_system = false;  // cg/mg hold the operator of a pressure solve
   hipEvent_t ev[4];
   double timing[3] = {0, 0, 0};
 
@@ -337,6 +338,7 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
       is->pressure_solve_iters = 0;
       break;
     case HG_SOLVER_CG:
+    case HG_SOLVER_FCG_MG:
     case HG_SOLVER_FCG_LEARNED: {
       if (c->pinned >= 0)
         throw HgError(HG_ERR_UNSUPPORTED, "cg with pressure_fixed_point is not supported");
@@ -344,11 +346,14 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
       KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
       c->cg->SetOperator(c->kface, c->kface + c->geo.nfx);
       hg::Precond* pc = nullptr;
-      if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED) {
+      if (cfg.linear_solver_pressure != HG_SOLVER_CG) {
         c->mg->Build(c->kface, c->kface + c->geo.nfx, c->st);
-        pc = (c->net && c->net->Trained()) ? static_cast<hg::Precond*>(c->net.get())
-                                           : static_cast<hg::Precond*>(c->mg.get());
+        const bool learned = cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED && c->net &&
+                             c->net->Trained();
+        pc = learned ? static_cast<hg::Precond*>(c->net.get())
+                     : static_cast<hg::Precond*>(c->mg.get());
       }
+      c->pressure_system = true;
       hg::CgResult r = c->cg->Solve(c->pb, c->pcorr, cfg.pressure_tolerance_gpu,
                                     cfg.pressure_max_iters_gpu, cfg.precision_gpu, true, pc);
       is->pressure_solve_iters = r.iterations;
@@ -543,10 +548,10 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     HG_CHECK(hipHostMalloc((void**)&c->flag_h, 4 * sizeof(int)));
     HG_CHECK(hipHostMalloc((void**)&c->info_h, 16));
     HG_CHECK(hipHostMalloc((void**)&c->stat_h, 8 * 8));
-    if (cfg.linear_solver_pressure == HG_SOLVER_CG ||
-        cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED)
+    const int32_t lsp = cfg.linear_solver_pressure;
+    if (lsp == HG_SOLVER_CG || lsp == HG_SOLVER_FCG_LEARNED || lsp == HG_SOLVER_FCG_MG)
       c->cg.reset(new hg::CgSolver(cfg.nx, cfg.ny, c->st));
-    if (cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED)
+    if (lsp == HG_SOLVER_FCG_LEARNED || lsp == HG_SOLVER_FCG_MG)
       c->mg.reset(new hg::MgPrecond(cfg.nx, cfg.ny));
     Upload(c.get(), c->xn_d, c->mesh.xn);
     Upload(c.get(), c->yn_d, c->mesh.yn);
@@ -798,6 +803,39 @@ hg_status hg_last_step_timing(const hg_ctx* c, double* f, double* a, double* p) 
   if (a) *a = c->timing[1];
   if (p) *p = c->timing[2];
   return HG_OK;
+}
+
+hg_status hg_train_pressure_net(hg_ctx* c, const hg_train_opts* opts, hg_train_stats* stats) {
+  if (!c || !opts) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    const hg_config& cfg = c->cfg;
+    if (cfg.linear_solver_pressure != HG_SOLVER_FCG_LEARNED)
+      throw HgError(HG_ERR_BAD_ARG, "linear_solver_pressure is not fcg_learned");
+    if (!c->pressure_system)
+      throw HgError(HG_ERR_BAD_ARG, "no pressure system yet: run a fluid iteration first");
+    if (!hg::LearnedNet::Supported(cfg.nx, cfg.ny))
+      throw HgError(HG_ERR_UNSUPPORTED, "learned preconditioner needs Nx, Ny multiples of 8");
+    hg::TrainOpts o;
+    try {
+      o = hg::TrainOptsFromAbi(*opts, c->pinned < 0);
+    } catch (const std::invalid_argument& e) {
+      throw HgError(HG_ERR_BAD_ARG, e.what());
+    }
+    if (!c->net) c->net.reset(new hg::LearnedNet(cfg.nx, cfg.ny, c->mg.get()));
+    hg::TrainReport r = c->net->Train(*c->cg, o);
+    if (stats) {
+      stats->pairs = r.pairs;
+      stats->steps = r.steps;
+      stats->loss_first = r.loss_first;
+      stats->loss_last = r.loss_last;
+      stats->ms = r.ms;
+      stats->accepted = r.accepted ? 1 : 0;
+      stats->val_iters = r.val_iters;
+      stats->val_iters_baseline = r.val_iters_baseline;
+      stats->val_ms = r.val_ms;
+      stats->val_ms_baseline = r.val_ms_baseline;
+    }
+  });
 }
 
 }  // extern "C"

@@ -202,7 +202,8 @@ void hg_poisson_destroy(hg_poisson* ps) { delete ps; }
 hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
                            const hg_solve_opts* opts, hg_solve_stats* stats) {
   if (!ps || !rhs || !p || !opts) return HG_ERR_BAD_ARG;
-  if (opts->solver != HG_SOLVER_CG && opts->solver != HG_SOLVER_FCG_LEARNED)
+  if (opts->solver != HG_SOLVER_CG && opts->solver != HG_SOLVER_FCG_LEARNED &&
+      opts->solver != HG_SOLVER_FCG_MG)
     return HG_ERR_UNSUPPORTED;
   try {
     const size_t N = size_t(ps->nx) * ps->ny;
@@ -211,6 +212,8 @@ hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
     if (opts->solver == HG_SOLVER_FCG_LEARNED)
       pc = (ps->net && ps->net->Trained()) ? static_cast<hg::Precond*>(ps->net.get())
                                            : static_cast<hg::Precond*>(ps->mg.get());
+    else if (opts->solver == HG_SOLVER_FCG_MG)
+      pc = ps->mg.get();
     hg::CgResult r = ps->cg->Solve(ps->b, ps->x, opts->tolerance, opts->max_iters,
                                    opts->precision, true, pc);
     HG_CHECK(hipMemcpyAsync(p, ps->x, N * 8, hipMemcpyDeviceToHost, ps->st));
@@ -225,6 +228,47 @@ hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
     return r.converged ? HG_OK : HG_ERR_NOT_CONVERGED;
   } catch (const std::exception& ex) {
     std::fprintf(stderr, "hg_poisson_solve: %s\n", ex.what());
+    return HG_ERR_HIP;
+  }
+}
+
+void hg_train_opts_default(hg_train_opts* o) {
+  if (!o) return;
+  hg::TrainOpts d;
+  o->rhs_count = d.rhs_count;
+  o->mg_iters = d.mg_iters;
+  o->steps = d.steps;
+  o->lr = d.lr;
+  o->seed = d.seed;
+  o->flags = (d.linear ? HG_TRAIN_LINEAR : 0) | (d.standalone ? HG_TRAIN_STANDALONE : 0) |
+             (d.tied ? HG_TRAIN_TIED : 0);
+}
+
+hg_status hg_poisson_train(hg_poisson* ps, const hg_train_opts* opts, hg_train_stats* stats) {
+  if (!ps || !opts) return HG_ERR_BAD_ARG;
+  if (!hg::LearnedNet::Supported(ps->nx, ps->ny)) return HG_ERR_UNSUPPORTED;
+  try {
+    hg::TrainOpts o = hg::TrainOptsFromAbi(*opts, true);
+    if (!ps->net) ps->net.reset(new hg::LearnedNet(ps->nx, ps->ny, ps->mg.get()));
+    hg::TrainReport r = ps->net->Train(*ps->cg, o);
+    if (stats) {
+      stats->pairs = r.pairs;
+      stats->steps = r.steps;
+      stats->loss_first = r.loss_first;
+      stats->loss_last = r.loss_last;
+      stats->ms = r.ms;
+      stats->accepted = r.accepted ? 1 : 0;
+      stats->val_iters = r.val_iters;
+      stats->val_iters_baseline = r.val_iters_baseline;
+      stats->val_ms = r.val_ms;
+      stats->val_ms_baseline = r.val_ms_baseline;
+    }
+    return HG_OK;
+  } catch (const std::invalid_argument& ex) {
+    std::fprintf(stderr, "hg_poisson_train: %s\n", ex.what());
+    return HG_ERR_BAD_ARG;
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "hg_poisson_train: %s\n", ex.what());
     return HG_ERR_HIP;
   }
 }

@@ -127,6 +127,16 @@ class Hydro:
                "calc_stats")
         return st
 
+    def train_pressure_net(self, rhs_count=None, mg_iters=None, steps=None, lr=None,
+                           seed=None, flags=None):
+        """Trains the learned pressure preconditioner on the operator of the
+        last pressure solve (linear_solver_pressure fcg_learned)."""
+        opts = train_opts(rhs_count, mg_iters, steps, lr, seed, flags)
+        st = capi.HgTrainStats()
+        rc = self.lib.hg_train_pressure_net(self.ctx, ctypes.byref(opts), ctypes.byref(st))
+        _raise(self.lib, self.ctx, rc, "hg_train_pressure_net")
+        return st
+
     def timing(self):
         f, a, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self.lib.hg_last_step_timing(self.ctx, ctypes.byref(f), ctypes.byref(a), ctypes.byref(p))
@@ -175,10 +185,107 @@ class Poisson:
             _raise(self.lib, None, rc, "hg_poisson_solve")
         return p, st
 
+    def train(self, rhs_count=None, mg_iters=None, steps=None, lr=None, seed=None, flags=None):
+        """Trains the learned preconditioner on this operator; returns the
+        hg_train_stats (pairs, steps, loss_first, loss_last, ms)."""
+        opts = train_opts(rhs_count, mg_iters, steps, lr, seed, flags)
+        st = capi.HgTrainStats()
+        rc = self.lib.hg_poisson_train(self.ps, ctypes.byref(opts), ctypes.byref(st))
+        _raise(self.lib, None, rc, "hg_poisson_train")
+        return st
+
     def close(self):
         if self.ps:
             self.lib.hg_poisson_destroy(self.ps)
             self.ps = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def train_opts(rhs_count=None, mg_iters=None, steps=None, lr=None, seed=None, flags=None):
+    """hg_train_opts: library defaults, overridden where given."""
+    lib = capi.load()
+    o = capi.HgTrainOpts()
+    lib.hg_train_opts_default(ctypes.byref(o))
+    for name, v in (("rhs_count", rhs_count), ("mg_iters", mg_iters), ("steps", steps),
+                    ("lr", lr), ("seed", seed), ("flags", flags)):
+        if v is not None:
+            setattr(o, name, v)
+    return o
+
+
+class Net:
+    """The learned preconditioner's network alone, on an nx x ny grid."""
+
+    def __init__(self, nx, ny):
+        self.lib = capi.load()
+        self.nx, self.ny = nx, ny
+        self.h = ctypes.c_void_p()
+        _raise(self.lib, None, self.lib.hg_net_create(nx, ny, ctypes.byref(self.h)),
+               "hg_net_create")
+        self.num_weights = int(self.lib.hg_net_num_weights(self.h))
+
+    def _f32(self, a, n):
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        if a.size != n:
+            raise ValueError(f"expected {n} values, got {a.size}")
+        return a
+
+    def configure(self, linear=False, tied=False):
+        flags = (capi.HG_TRAIN_LINEAR if linear else 0) | (capi.HG_TRAIN_TIED if tied else 0)
+        _raise(self.lib, None, self.lib.hg_net_configure(self.h, flags), "hg_net_configure")
+
+    def set_weights(self, w):
+        w = self._f32(w, self.num_weights)
+        _raise(self.lib, None, self.lib.hg_net_set_weights(self.h, capi.fptr(w)),
+               "hg_net_set_weights")
+
+    def get_weights(self):
+        w = np.zeros(self.num_weights, np.float32)
+        _raise(self.lib, None, self.lib.hg_net_get_weights(self.h, capi.fptr(w)),
+               "hg_net_get_weights")
+        return w
+
+    def forward(self, x):
+        n = self.nx * self.ny
+        x = self._f32(x, n)
+        y = np.zeros(n, np.float32)
+        _raise(self.lib, None, self.lib.hg_net_forward(self.h, capi.fptr(x), capi.fptr(y)),
+               "hg_net_forward")
+        return y.reshape(self.ny, self.nx)
+
+    def gradient(self, x, gy):
+        n = self.nx * self.ny
+        x, gy = self._f32(x, n), self._f32(gy, n)
+        gw = np.zeros(self.num_weights, np.float32)
+        _raise(self.lib, None, self.lib.hg_net_gradient(self.h, capi.fptr(x), capi.fptr(gy),
+                                                        capi.fptr(gw)), "hg_net_gradient")
+        return gw
+
+    def train_step(self, x, t, lr):
+        n = self.nx * self.ny
+        x, t = self._f32(x, n), self._f32(t, n)
+        loss = ctypes.c_double()
+        _raise(self.lib, None, self.lib.hg_net_train_step(self.h, capi.fptr(x), capi.fptr(t),
+                                                          lr, ctypes.byref(loss)),
+               "hg_net_train_step")
+        return loss.value
+
+    def time(self, reps=10):
+        v = [ctypes.c_double() for _ in range(4)]
+        _raise(self.lib, None, self.lib.hg_net_time(self.h, reps, *[ctypes.byref(a) for a in v]),
+               "hg_net_time")
+        return {"forward_ms": v[0].value, "train_ms": v[1].value,
+                "forward_flops": v[2].value, "train_flops": v[3].value}
+
+    def close(self):
+        if self.h:
+            self.lib.hg_net_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -45,7 +45,8 @@ typedef enum hg_solver {
   HG_SOLVER_JACOBI = 3,       /* "jacobi" (linear.hpp:736-783) */
   HG_SOLVER_LU_RB = 4,        /* "lu_rb": red-black ordered symmetric GS sweep (new) */
   HG_SOLVER_CG = 5,           /* "cg": Jacobi-PCG to a relative residual (new) */
-  HG_SOLVER_FCG_LEARNED = 6   /* "fcg_learned": flexible CG + learned preconditioner (new) */
+  HG_SOLVER_FCG_LEARNED = 6,  /* "fcg_learned": flexible CG + learned preconditioner (new) */
+  HG_SOLVER_FCG_MG = 7        /* "fcg_mg": flexible CG + the multigrid V-cycle alone (new) */
 } hg_solver;
 
 typedef enum hg_precision {
@@ -286,6 +287,79 @@ void hg_poisson_destroy(hg_poisson* ps);
  * HG_PREC_MIXED runs FP32 inner solves under FP64 iterative refinement. */
 hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
                            const hg_solve_opts* opts, hg_solve_stats* stats);
+
+/* ---- learned preconditioner (no reference counterpart: the reference's
+ * pressure solvers are the stationary sweeps of linear.hpp) ----------------
+ * HG_SOLVER_FCG_LEARNED preconditions flexible CG with
+ *   M(r) = V(r) + N(D^-1 (r - A V(r)))
+ * (V: multigrid V-cycle, N: a 4-level BF16 U-Net, 16..64 channels) once the
+ * net is trained, and with V alone before.  Training pairs come from the
+ * solver on the operator in use: random multi-scale right-hand sides, a
+ * tight reference solve, then the stationary V-cycle iteration's residuals
+ * and remaining errors.  Requires nx, ny multiples of 8 (>= 16). */
+typedef struct hg_train_opts {
+  int32_t rhs_count;   /* right-hand sides (4) */
+  int32_t mg_iters;    /* V-cycle iterations per right-hand side = pairs each (12) */
+  int32_t steps;       /* Adam steps, one pair each (300) */
+  float lr;            /* peak learning rate, cosine decay to lr/10 (1e-3) */
+  uint64_t seed;       /* (1) */
+  int32_t flags;       /* HG_TRAIN_* bits (HG_TRAIN_TIED) */
+} hg_train_opts;
+
+#define HG_TRAIN_LINEAR 1      /* network without activations: M stays a linear operator */
+#define HG_TRAIN_STANDALONE 2  /* no V-cycle: M(r) = N(D^-1 r) (S N S when tied); judged
+                                  against Jacobi-PCG */
+#define HG_TRAIN_TIED 4        /* symmetric linear net N = sum_l 4^l F_l^T F_l: M = V + S N S,
+                                  S = D^-1/2, symmetric positive definite by construction */
+
+typedef struct hg_train_stats {
+  int32_t pairs;
+  int32_t steps;
+  double loss_first;   /* mean relative energy-norm error (best output multiple),
+                          first 10% of steps */
+  double loss_last;    /* ... last 10% */
+  double ms;           /* wall time of pair generation, training and acceptance */
+  /* Acceptance solve on a held-out right-hand side to a 1e-8 true FP64
+   * relative residual: FCG with the learned preconditioner against the
+   * method without it (FCG + V-cycle; Jacobi-PCG for HG_TRAIN_STANDALONE).
+   * The solvers use the net only when it took fewer iterations. */
+  int32_t accepted;
+  int32_t val_iters;
+  int32_t val_iters_baseline;
+  double val_ms;
+  double val_ms_baseline;
+} hg_train_stats;
+
+void hg_train_opts_default(hg_train_opts* opts);
+/* Trains the preconditioner of a pressure-solve object on its operator. */
+hg_status hg_poisson_train(hg_poisson* ps, const hg_train_opts* opts, hg_train_stats* stats);
+/* Trains the preconditioner of a simulation on the operator of its last
+ * pressure solve (linear_solver_pressure "fcg_learned"; at least one
+ * iteration done). */
+hg_status hg_train_pressure_net(hg_ctx* ctx, const hg_train_opts* opts, hg_train_stats* stats);
+
+/* ---- the network alone (no reference counterpart), for checks and
+ * throughput figures.  Arrays are host arrays of nx*ny floats (cell
+ * i + nx*j); weights are packed per layer as W[tap][ci][co], tap = 3*ky+kx,
+ * layers E0..E3, D3..D0. */
+typedef struct hg_net hg_net;
+hg_status hg_net_create(int32_t nx, int32_t ny, hg_net** out);
+void hg_net_destroy(hg_net* net);
+int64_t hg_net_num_weights(const hg_net* net);
+/* HG_TRAIN_LINEAR / HG_TRAIN_TIED bits: activations off / tied decoder. */
+hg_status hg_net_configure(hg_net* net, int32_t flags);
+hg_status hg_net_set_weights(hg_net* net, const float* w);
+hg_status hg_net_get_weights(hg_net* net, float* w);
+/* y = N(x) */
+hg_status hg_net_forward(hg_net* net, const float* x, float* y);
+/* gw = d/dw sum(gy * N(x)) */
+hg_status hg_net_gradient(hg_net* net, const float* x, const float* gy, float* gw);
+/* One Adam step on mean((N(x) - t)^2); *loss = that mean before the step. */
+hg_status hg_net_train_step(hg_net* net, const float* x, const float* t, float lr, double* loss);
+/* Device time per forward pass and per training step on resident synthetic
+ * input (HIP events over `reps` launches each), and the FLOPs of each. */
+hg_status hg_net_time(hg_net* net, int32_t reps, double* forward_ms, double* train_ms,
+                      double* forward_flops, double* train_flops);
 
 #ifdef __cplusplus
 }
