@@ -4,8 +4,10 @@
 // both dimensions while they stay even: coarse face coefficient = mean of
 // the two fine faces it covers (rediscretisation; in 2D k_f = A/(d a_f) does
 // not scale with h), restriction = sum of the four children's residuals,
-// prolongation = injection, smoother = damped Jacobi (omega 0.8, 2 pre +
-// 2 post sweeps), coarsest grid (<= 1024 cells) relaxed in one workgroup.
+// prolongation = cell-centred bilinear weighted by the coarse face
+// coefficients (injection across walls and strong density jumps), smoother = damped
+// Jacobi (omega 0.8, 2 pre + 2 post sweeps), coarsest grid (<= 1024 cells)
+// relaxed in one workgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +26,15 @@ struct MgLevel {
   float *r = nullptr, *xa = nullptr, *xb = nullptr;
 };
 
+// The coarse levels handed to the one-workgroup cycle kernel.
+constexpr int kMaxInKernelLevels = 12;
+struct CoarseSet {
+  int count, bilinear;
+  int nx[kMaxInKernelLevels], ny[kMaxInKernelLevels];
+  const float *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
+  float *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];
+};
+
 class MgPrecond : public Precond {
  public:
   MgPrecond(int nx, int ny);
@@ -39,9 +50,14 @@ class MgPrecond : public Precond {
   const MgLevel& level(int l) const { return lv_[l]; }
   float omega = 0.8f;
   int coarse_sweeps = 40;
+  bool bilinear = true;  // prolongation: bilinear (true) or injection
+  // levels of <= 8192 cells down to a coarsest grid of <= 64 cells run in
+  // one workgroup (one launch) instead of one launch per sweep
+  bool use_coarse_kernel = true;
 
  private:
   std::vector<MgLevel> lv_;
+  int first_in_kernel_ = -1;
   void Cycle(int l, const double* r64, float* out, hipStream_t st, const int* d_done);
 };
 

@@ -18,6 +18,8 @@ namespace hg {
 namespace {
 
 constexpr int kCoarsestMax = 1024;
+constexpr int kTinyMax = 64;          // coarsest grid solved by one wave
+constexpr int kInKernelMax = 8192;    // levels handled by the one-workgroup cycle
 
 __device__ __forceinline__ bool Done(const int* d) { return d && *d; }
 
@@ -131,19 +133,54 @@ KResRestrict(int nx, int ny, const float* __restrict__ kx, const float* __restri
   rc[I + nxc * J] = s;
 }
 
-// Prolongation by injection fused with a Jacobi sweep:
+// Coarse correction at fine cell (i, j).  Injection: the parent's value.
+// Bilinear (cell-centred) weighted by coupling: the parent P, its neighbours
+// X (in x, on the child's side), Y (in y) and the diagonal D contribute
+//   e = eP + (3 wx (eX - eP) + 3 wy (eY - eP) + wx wy (eD - eP)) / 16
+// with wx = min(1, 4 k(P|X) / max k(P faces)) (coarse face coefficients):
+// coefficients within a factor 4 of each other give the 9/16, 3/16, 3/16,
+// 1/16 bilinear weights; a wall (k = 0) or a weakly coupled neighbour across
+// a strong density jump gives injection in that direction.
+template <bool BILINEAR>
+__device__ __forceinline__ float Interp(const float* __restrict__ e,
+                                        const float* __restrict__ kxc,
+                                        const float* __restrict__ kyc, int nxc, int nyc, int i,
+                                        int j) {
+  const int I = i >> 1, J = j >> 1;
+  const float ec = e[I + nxc * J];
+  if (!BILINEAR) return ec;
+  const int sx = (i & 1) ? 1 : 0, sy = (j & 1) ? 1 : 0;  // face on the child's side
+  const float kw = kxc[I + (nxc + 1) * J], ke = kxc[I + 1 + (nxc + 1) * J];
+  const float ks = kyc[I + nxc * J], kn = kyc[I + nxc * (J + 1)];
+  const float kmax = fmaxf(fmaxf(kw, ke), fmaxf(ks, kn));
+  if (!(kmax > 0.f)) return ec;
+  const float wx = fminf(1.f, 4.f * (sx ? ke : kw) / kmax);
+  const float wy = fminf(1.f, 4.f * (sy ? kn : ks) / kmax);
+  const int Ix = I + (sx ? 1 : -1), Jy = J + (sy ? 1 : -1);
+  // a neighbour outside the grid sits behind a wall face (k = 0, weight 0)
+  const float ex = wx > 0.f ? e[Ix + nxc * J] : ec;
+  const float ey = wy > 0.f ? e[I + nxc * Jy] : ec;
+  const float wd = wx * wy;
+  const float ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
+  return ec + (3.f * wx * (ex - ec) + 3.f * wy * (ey - ec) + wd * (ed - ec)) * 0.0625f;
+}
+
+// Prolongation fused with a Jacobi sweep:
 // y = x + P e, xn = y + omega D^-1 (r - A y).
-template <class RT>
+template <class RT, bool BILINEAR>
 __global__ void __launch_bounds__(kTileThreads)
 KProlongJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx,
                const float* __restrict__ ky, const float* __restrict__ dinv,
                const RT* __restrict__ r, const float* __restrict__ x,
-               const float* __restrict__ e, int nxc, float omega, float* __restrict__ xn,
-               const int* done) {
+               const float* __restrict__ e, const float* __restrict__ kxc,
+               const float* __restrict__ kyc, int nxc, int nyc, float omega,
+               float* __restrict__ xn, const int* done) {
   if (Done(done)) return;
   int i, j;
   if (!TileCell(tg, nx, ny, i, j)) return;
-  auto Y = [&](int ii, int jj) { return x[ii + nx * jj] + e[(ii >> 1) + nxc * (jj >> 1)]; };
+  auto Y = [&](int ii, int jj) {
+    return x[ii + nx * jj] + Interp<BILINEAR>(e, kxc, kyc, nxc, nyc, ii, jj);
+  };
   const int c = i + nx * j;
   const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
   const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
@@ -197,6 +234,131 @@ KCoarsest(int nx, int ny, const float* __restrict__ kx, const float* __restrict_
   if (act) out[c] = xs[cur][c];
 }
 
+__device__ __forceinline__ float WaveSumF(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ void BlockSync() {
+  __threadfence_block();
+  __syncthreads();
+}
+
+// The coarse end of the V-cycle in one workgroup: levels first..last of the
+// hierarchy (each <= kInKernelMax cells, data L2-resident), the same
+// smoothing / restriction / prolongation as the grid kernels, and on the
+// coarsest level (<= 64 cells, one cell per lane of wave 0) Jacobi-PCG to
+// convergence.  Input: r of the first level; output: its xb.
+__global__ void __launch_bounds__(1024)
+KCoarseCycle(CoarseSet cs, float omega, const int* done) {
+  if (Done(done)) return;
+  const int T = blockDim.x, t = threadIdx.x;
+  const int L = cs.count;
+  for (int l = 0; l < L - 1; ++l) {
+    const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l], *r = cs.r[l];
+    float *xa = cs.xa[l], *xb = cs.xb[l];
+    for (int c = t; c < n; c += T) xa[c] = omega * dinv[c] * r[c];
+    BlockSync();
+    for (int c = t; c < n; c += T) {
+      const int i = c % nx, j = c / nx;
+      xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+    }
+    BlockSync();
+    const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
+    float* rc = cs.r[l + 1];
+    for (int C = t; C < nc; C += T) {
+      const int I = C % nxc, J = C / nxc;
+      float sum = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
+        sum += r[i + nx * j] - Ax(nx, ny, kx, ky, xb, i, j);
+      }
+      rc[C] = sum;
+    }
+    BlockSync();
+  }
+  if (t < 64) {
+    const int nx = cs.nx[L - 1], ny = cs.ny[L - 1], n = nx * ny;
+    const float *kx = cs.kx[L - 1], *ky = cs.ky[L - 1];
+    const int lane = t;
+    const bool act = lane < n;
+    const int i = act ? lane % nx : 0, j = act ? lane / nx : 0;
+    float kw = 0.f, ke = 0.f, ks = 0.f, kn = 0.f, di = 0.f, b = 0.f;
+    if (act) {
+      kw = kx[i + (nx + 1) * j];
+      ke = kx[i + 1 + (nx + 1) * j];
+      ks = ky[i + nx * j];
+      kn = ky[i + nx * (j + 1)];
+      di = cs.dinv[L - 1][lane];
+      b = cs.r[L - 1][lane];
+    }
+    const float dg = ((kw + ke) + ks) + kn;
+    // the face-form operator has the constants as null space: solve with the
+    // right-hand side projected to zero mean (consistent system)
+    b -= WaveSumF(b) / float(n);
+    if (!act) b = 0.f;
+    float x = 0.f, rr = b, z = di * rr, p = z;
+    float rz = WaveSumF(rr * z);
+    const float rz0 = rz;
+    for (int it = 0; it < n && rz > 1e-12f * rz0; ++it) {
+      const float pw = __shfl(p, (lane + 63) & 63, 64), pe = __shfl(p, (lane + 1) & 63, 64);
+      const float ps = __shfl(p, (lane + 64 - nx) & 63, 64);
+      const float pn = __shfl(p, (lane + nx) & 63, 64);
+      float ap = dg * p;
+      if (i > 0) ap -= kw * pw;
+      if (i < nx - 1) ap -= ke * pe;
+      if (j > 0) ap -= ks * ps;
+      if (j < ny - 1) ap -= kn * pn;
+      if (!act) ap = 0.f;
+      const float pap = WaveSumF(p * ap);
+      if (!(pap > 0.f)) break;
+      const float alpha = rz / pap;
+      x += alpha * p;
+      rr -= alpha * ap;
+      z = di * rr;
+      const float rzn = WaveSumF(rr * z);
+      p = z + (rzn / rz) * p;
+      rz = rzn;
+    }
+    x -= WaveSumF(act ? x : 0.f) / float(n);
+    if (act) cs.xb[L - 1][lane] = x;
+  }
+  BlockSync();
+  for (int l = L - 2; l >= 0; --l) {
+    const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
+    const int nxc = cs.nx[l + 1], nyc = cs.ny[l + 1];
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l], *r = cs.r[l];
+    const float* e = cs.xb[l + 1];
+    const float *kxc = cs.kx[l + 1], *kyc = cs.ky[l + 1];
+    float *xa = cs.xa[l], *xb = cs.xb[l];
+    for (int c = t; c < n; c += T) {
+      const int i = c % nx, j = c / nx;
+      auto Y = [&](int ii, int jj) {
+        return xb[ii + nx * jj] +
+               (cs.bilinear ? Interp<true>(e, kxc, kyc, nxc, nyc, ii, jj)
+                            : Interp<false>(e, kxc, kyc, nxc, nyc, ii, jj));
+      };
+      const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+      const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+      const float yc = Y(i, j);
+      float ay = (((kw + ke) + ks) + kn) * yc;
+      if (i > 0) ay -= kw * Y(i - 1, j);
+      if (i < nx - 1) ay -= ke * Y(i + 1, j);
+      if (j > 0) ay -= ks * Y(i, j - 1);
+      if (j < ny - 1) ay -= kn * Y(i, j + 1);
+      xa[c] = yc + omega * dinv[c] * (r[c] - ay);
+    }
+    BlockSync();
+    for (int c = t; c < n; c += T) {
+      const int i = c % nx, j = c / nx;
+      xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+    }
+    BlockSync();
+  }
+}
+
 inline int Blocks1D(long n) { return int(std::min<long>((n + 255) / 256, 4096)); }
 
 }  // namespace
@@ -216,12 +378,24 @@ MgPrecond::MgPrecond(int nx, int ny) {
     HG_CHECK(hipMalloc(&L.xa, n * 4));
     HG_CHECK(hipMalloc(&L.xb, n * 4));
     lv_.push_back(L);
-    if ((cx & 1) || (cy & 1) || n <= kCoarsestMax / 2 || cx < 4 || cy < 4) break;
+    if ((cx & 1) || (cy & 1) || n <= kTinyMax || cx < 4 || cy < 4) break;
     cx /= 2;
     cy /= 2;
   }
-  if (long(lv_.back().nx) * lv_.back().ny > kCoarsestMax)
+  const long nlast = long(lv_.back().nx) * lv_.back().ny;
+  if (nlast > kCoarsestMax)
     coarse_sweeps = -1;  // coarsest too large for one workgroup: relax with grid kernels
+  // levels 1.. of at most kInKernelMax cells go to the one-workgroup cycle
+  // when the hierarchy ends on a tiny grid
+  if (nlast <= kTinyMax && lv_.size() > 1) {
+    for (int l = 1; l < int(lv_.size()); ++l)
+      if (long(lv_[l].nx) * lv_[l].ny <= kInKernelMax) {
+        first_in_kernel_ = l;
+        break;
+      }
+  }
+  if (first_in_kernel_ > 0 && int(lv_.size()) - first_in_kernel_ > kMaxInKernelLevels)
+    first_in_kernel_ = int(lv_.size()) - kMaxInKernelLevels;
 }
 
 MgPrecond::~MgPrecond() {
@@ -246,6 +420,26 @@ void MgPrecond::Build(const double* kx, const double* ky, hipStream_t st) {
 
 void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, const int* done) {
   MgLevel& L = lv_[l];
+  if (l > 0 && l == first_in_kernel_ && use_coarse_kernel) {
+    CoarseSet cs{};
+    cs.count = int(lv_.size()) - l;
+    cs.bilinear = bilinear ? 1 : 0;
+    for (int k = 0; k < cs.count; ++k) {
+      const MgLevel& C = lv_[l + k];
+      cs.nx[k] = C.nx;
+      cs.ny[k] = C.ny;
+      cs.kx[k] = C.kx;
+      cs.ky[k] = C.ky;
+      cs.dinv[k] = C.dinv;
+      cs.r[k] = C.r;
+      cs.xa[k] = C.xa;
+      cs.xb[k] = C.xb;
+    }
+    // the caller prolongs from this level's xb
+    (void)out;
+    KCoarseCycle<<<1, 1024, 0, st>>>(cs, omega, done);
+    return;
+  }
   const bool coarsest = (l == int(lv_.size()) - 1);
   const int blocks = L.tg.blocks();
   if (coarsest && coarse_sweeps > 0) {
@@ -299,15 +493,21 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
   }
   Cycle(l + 1, nullptr, C.xb, st, done);
   if (l == 0) {
-    KProlongJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
-                                                            r64, L.xb, C.xb, C.nx, omega, L.xa,
-                                                            done);
+    if (bilinear)
+      KProlongJacobi<double, true><<<blocks, kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
+    else
+      KProlongJacobi<double, false><<<blocks, kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
     KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
                                                      L.xa, omega, out, done);
   } else {
-    KProlongJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
-                                                           L.r, L.xb, C.xb, C.nx, omega, L.xa,
-                                                           done);
+    if (bilinear)
+      KProlongJacobi<float, true><<<blocks, kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
+    else
+      KProlongJacobi<float, false><<<blocks, kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
     KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
                                                     L.xa, omega, out, done);
   }
