@@ -28,8 +28,10 @@ struct MgLevel {
 
 // The coarse levels handed to the one-workgroup cycle kernel.
 constexpr int kMaxInKernelLevels = 12;
+constexpr int kLdsCells = 2752;  // LDS cells per vector for all in-kernel levels
 struct CoarseSet {
   int count, bilinear;
+  int off[kMaxInKernelLevels];  // offset of each level in the LDS vectors
   int nx[kMaxInKernelLevels], ny[kMaxInKernelLevels];
   const float *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
   float *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];
@@ -51,7 +53,7 @@ class MgPrecond : public Precond {
   float omega = 0.8f;
   int coarse_sweeps = 40;
   bool bilinear = true;  // prolongation: bilinear (true) or injection
-  // levels of <= 8192 cells down to a coarsest grid of <= 64 cells run in
+  // levels of <= 2048 cells down to a coarsest grid of <= 64 cells run in
   // one workgroup (one launch) instead of one launch per sweep
   bool use_coarse_kernel = true;
 

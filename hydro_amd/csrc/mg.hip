@@ -19,7 +19,7 @@ namespace {
 
 constexpr int kCoarsestMax = 1024;
 constexpr int kTinyMax = 64;          // coarsest grid solved by one wave
-constexpr int kInKernelMax = 8192;    // levels handled by the one-workgroup cycle
+constexpr int kInKernelMax = 2048;    // levels handled by the one-workgroup cycle
 
 __device__ __forceinline__ bool Done(const int* d) { return d && *d; }
 
@@ -115,24 +115,6 @@ KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* 
   xn[c] = x[c] + omega * dinv[c] * (float(r[c]) - Ax(nx, ny, kx, ky, x, i, j));
 }
 
-// Residual of the fine level summed over each coarse cell's four children.
-template <class RT>
-__global__ void __launch_bounds__(kTileThreads)
-KResRestrict(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
-             const RT* __restrict__ r, const float* __restrict__ x, int nxc, int nyc,
-             TileGrid tgc, float* __restrict__ rc, const int* done) {
-  if (Done(done)) return;
-  int I, J;
-  if (!TileCell(tgc, nxc, nyc, I, J)) return;
-  float s = 0.f;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
-    s += float(r[i + nx * j]) - Ax(nx, ny, kx, ky, x, i, j);
-  }
-  rc[I + nxc * J] = s;
-}
-
 // Coarse correction at fine cell (i, j).  Injection: the parent's value.
 // Bilinear (cell-centred) weighted by coupling: the parent P, its neighbours
 // X (in x, on the child's side), Y (in y) and the diagonal D contribute
@@ -165,32 +147,146 @@ __device__ __forceinline__ float Interp(const float* __restrict__ e,
   return ec + (3.f * wx * (ex - ec) + 3.f * wy * (ey - ec) + wd * (ed - ec)) * 0.0625f;
 }
 
-// Prolongation fused with a Jacobi sweep:
-// y = x + P e, xn = y + omega D^-1 (r - A y).
+// Fused down-leg of one level: two damped Jacobi sweeps from zero
+// (x1 = omega D^-1 r, x2 = x1 + omega D^-1 (r - A x1)) and the restriction of
+// r - A x2 (sum of the four children).  The block works on its 64 x 4 tile
+// with a two-cell halo in LDS (x1 on tile + 2, x2 on tile + 1), so one launch
+// and one pass over the level's arrays replace three.
+template <class RT>
+__global__ void __launch_bounds__(kTileThreads)
+KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
+      const float* __restrict__ dinv, const RT* __restrict__ r, float omega,
+      float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done) {
+  if (Done(done)) return;
+  constexpr int H2X = kTX + 4, H2Y = kTY + 4, H1X = kTX + 2, H1Y = kTY + 2;
+  __shared__ float s1[H2Y][H2X], sr[H2Y][H2X], s2[H1Y][H1X], sres[kTY][kTX];
+  const int b = blockIdx.x;
+  const int tile = (b % kXcds) * tg.per + b / kXcds;
+  if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
+  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * kTY;
+  for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
+    const int hj = q / H2X, hi = q % H2X;
+    const int ii = i0 + hi - 2, jj = j0 + hj - 2;
+    float rv = 0.f, xv = 0.f;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const int c = ii + nx * jj;
+      rv = float(r[c]);
+      xv = omega * dinv[c] * rv;
+    }
+    sr[hj][hi] = rv;
+    s1[hj][hi] = xv;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
+    const int hj = q / H1X, hi = q % H1X;
+    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
+    float xv = 0.f;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const int c = ii + nx * jj;
+      const int a = hj + 1, e = hi + 1;  // position in the halo-2 arrays
+      const float kw = kx[ii + (nx + 1) * jj], ke = kx[ii + 1 + (nx + 1) * jj];
+      const float ks = ky[ii + nx * jj], kn = ky[ii + nx * (jj + 1)];
+      float v = (((kw + ke) + ks) + kn) * s1[a][e];
+      if (ii > 0) v -= kw * s1[a][e - 1];
+      if (ii < nx - 1) v -= ke * s1[a][e + 1];
+      if (jj > 0) v -= ks * s1[a - 1][e];
+      if (jj < ny - 1) v -= kn * s1[a + 1][e];
+      xv = s1[a][e] + omega * dinv[c] * (sr[a][e] - v);
+    }
+    s2[hj][hi] = xv;
+  }
+  __syncthreads();
+  {
+    const int li = threadIdx.x & (kTX - 1), lj = threadIdx.x / kTX;
+    const int i = i0 + li, j = j0 + lj;
+    float res = 0.f;
+    if (i < nx && j < ny) {
+      const int c = i + nx * j;
+      const int a = lj + 1, e = li + 1;
+      const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+      const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+      float v = (((kw + ke) + ks) + kn) * s2[a][e];
+      if (i > 0) v -= kw * s2[a][e - 1];
+      if (i < nx - 1) v -= ke * s2[a][e + 1];
+      if (j > 0) v -= ks * s2[a - 1][e];
+      if (j < ny - 1) v -= kn * s2[a + 1][e];
+      x2out[c] = s2[a][e];
+      res = sr[lj + 2][li + 2] - v;
+    }
+    sres[lj][li] = res;
+  }
+  __syncthreads();
+  if (threadIdx.x < (kTX / 2) * (kTY / 2)) {
+    const int ci = threadIdx.x % (kTX / 2), cj = threadIdx.x / (kTX / 2);
+    const int I = i0 / 2 + ci, J = j0 / 2 + cj;
+    if (I < nxc && J < nyc)
+      rc[I + nxc * J] = ((sres[2 * cj][2 * ci] + sres[2 * cj][2 * ci + 1]) +
+                         sres[2 * cj + 1][2 * ci]) + sres[2 * cj + 1][2 * ci + 1];
+  }
+}
+
+// Fused up-leg of one level: y = x2 + P e, x3 = y + omega D^-1 (r - A y),
+// out = x3 + omega D^-1 (r - A x3), on the block's tile with a two-cell halo
+// in LDS.
 template <class RT, bool BILINEAR>
 __global__ void __launch_bounds__(kTileThreads)
-KProlongJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx,
-               const float* __restrict__ ky, const float* __restrict__ dinv,
-               const RT* __restrict__ r, const float* __restrict__ x,
-               const float* __restrict__ e, const float* __restrict__ kxc,
-               const float* __restrict__ kyc, int nxc, int nyc, float omega,
-               float* __restrict__ xn, const int* done) {
+KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
+    const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
+    const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
+    int nxc, int nyc, float omega, float* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  int i, j;
-  if (!TileCell(tg, nx, ny, i, j)) return;
-  auto Y = [&](int ii, int jj) {
-    return x[ii + nx * jj] + Interp<BILINEAR>(e, kxc, kyc, nxc, nyc, ii, jj);
-  };
+  constexpr int H2X = kTX + 4, H2Y = kTY + 4, H1X = kTX + 2, H1Y = kTY + 2;
+  __shared__ float sy[H2Y][H2X], sr[H1Y][H1X], s3[H1Y][H1X];
+  const int b = blockIdx.x;
+  const int tile = (b % kXcds) * tg.per + b / kXcds;
+  if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
+  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * kTY;
+  for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
+    const int hj = q / H2X, hi = q % H2X;
+    const int ii = i0 + hi - 2, jj = j0 + hj - 2;
+    float yv = 0.f;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
+      yv = x2[ii + nx * jj] + Interp<BILINEAR>(e, kxc, kyc, nxc, nyc, ii, jj);
+    sy[hj][hi] = yv;
+  }
+  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
+    const int hj = q / H1X, hi = q % H1X;
+    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
+    sr[hj][hi] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? float(r[ii + nx * jj]) : 0.f;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
+    const int hj = q / H1X, hi = q % H1X;
+    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
+    float xv = 0.f;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const int c = ii + nx * jj;
+      const int a = hj + 1, f = hi + 1;
+      const float kw = kx[ii + (nx + 1) * jj], ke = kx[ii + 1 + (nx + 1) * jj];
+      const float ks = ky[ii + nx * jj], kn = ky[ii + nx * (jj + 1)];
+      float v = (((kw + ke) + ks) + kn) * sy[a][f];
+      if (ii > 0) v -= kw * sy[a][f - 1];
+      if (ii < nx - 1) v -= ke * sy[a][f + 1];
+      if (jj > 0) v -= ks * sy[a - 1][f];
+      if (jj < ny - 1) v -= kn * sy[a + 1][f];
+      xv = sy[a][f] + omega * dinv[c] * (sr[hj][hi] - v);
+    }
+    s3[hj][hi] = xv;
+  }
+  __syncthreads();
+  const int li = threadIdx.x & (kTX - 1), lj = threadIdx.x / kTX;
+  const int i = i0 + li, j = j0 + lj;
+  if (i >= nx || j >= ny) return;
   const int c = i + nx * j;
+  const int a = lj + 1, f = li + 1;
   const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
   const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-  const float yc = Y(i, j);
-  float ay = (((kw + ke) + ks) + kn) * yc;
-  if (i > 0) ay -= kw * Y(i - 1, j);
-  if (i < nx - 1) ay -= ke * Y(i + 1, j);
-  if (j > 0) ay -= ks * Y(i, j - 1);
-  if (j < ny - 1) ay -= kn * Y(i, j + 1);
-  xn[c] = yc + omega * dinv[c] * (float(r[c]) - ay);
+  float v = (((kw + ke) + ks) + kn) * s3[a][f];
+  if (i > 0) v -= kw * s3[a][f - 1];
+  if (i < nx - 1) v -= ke * s3[a][f + 1];
+  if (j > 0) v -= ks * s3[a - 1][f];
+  if (j < ny - 1) v -= kn * s3[a + 1][f];
+  out[c] = s3[a][f] + omega * dinv[c] * (sr[a][f] - v);
 }
 
 // Coarsest level: Jacobi sweeps in one workgroup, x and r in LDS.
@@ -245,28 +341,34 @@ __device__ __forceinline__ void BlockSync() {
 }
 
 // The coarse end of the V-cycle in one workgroup: levels first..last of the
-// hierarchy (each <= kInKernelMax cells, data L2-resident), the same
-// smoothing / restriction / prolongation as the grid kernels, and on the
-// coarsest level (<= 64 cells, one cell per lane of wave 0) Jacobi-PCG to
-// convergence.  Input: r of the first level; output: its xb.
+// hierarchy (each <= kInKernelMax cells), the same smoothing / restriction /
+// prolongation as the grid kernels with the iterates and residuals in LDS
+// (coefficients read through the caches), and on the coarsest level (<= 64
+// cells, one cell per lane of wave 0) Jacobi-PCG to convergence.  Input: r of
+// the first level (global); output: its xb (global).
 __global__ void __launch_bounds__(1024)
 KCoarseCycle(CoarseSet cs, float omega, const int* done) {
   if (Done(done)) return;
+  __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int T = blockDim.x, t = threadIdx.x;
   const int L = cs.count;
+  const int n0 = cs.nx[0] * cs.ny[0];
+  for (int c = t; c < n0; c += T) sr[c] = cs.r[0][c];
+  __syncthreads();
   for (int l = 0; l < L - 1; ++l) {
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
-    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l], *r = cs.r[l];
-    float *xa = cs.xa[l], *xb = cs.xb[l];
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
+    const float* r = sr + cs.off[l];
+    float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
     for (int c = t; c < n; c += T) xa[c] = omega * dinv[c] * r[c];
-    BlockSync();
+    __syncthreads();
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
       xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
     }
-    BlockSync();
+    __syncthreads();
     const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
-    float* rc = cs.r[l + 1];
+    float* rc = sr + cs.off[l + 1];
     for (int C = t; C < nc; C += T) {
       const int I = C % nxc, J = C / nxc;
       float sum = 0.f;
@@ -277,11 +379,12 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
       }
       rc[C] = sum;
     }
-    BlockSync();
+    __syncthreads();
   }
   if (t < 64) {
     const int nx = cs.nx[L - 1], ny = cs.ny[L - 1], n = nx * ny;
     const float *kx = cs.kx[L - 1], *ky = cs.ky[L - 1];
+    const float* r = sr + cs.off[L - 1];
     const int lane = t;
     const bool act = lane < n;
     const int i = act ? lane % nx : 0, j = act ? lane / nx : 0;
@@ -292,7 +395,7 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
       ks = ky[i + nx * j];
       kn = ky[i + nx * (j + 1)];
       di = cs.dinv[L - 1][lane];
-      b = cs.r[L - 1][lane];
+      b = r[lane];
     }
     const float dg = ((kw + ke) + ks) + kn;
     // the face-form operator has the constants as null space: solve with the
@@ -323,16 +426,17 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
       rz = rzn;
     }
     x -= WaveSumF(act ? x : 0.f) / float(n);
-    if (act) cs.xb[L - 1][lane] = x;
+    if (act) sxb[cs.off[L - 1] + lane] = x;
   }
-  BlockSync();
+  __syncthreads();
   for (int l = L - 2; l >= 0; --l) {
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
     const int nxc = cs.nx[l + 1], nyc = cs.ny[l + 1];
-    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l], *r = cs.r[l];
-    const float* e = cs.xb[l + 1];
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
+    const float* r = sr + cs.off[l];
+    const float* e = sxb + cs.off[l + 1];
     const float *kxc = cs.kx[l + 1], *kyc = cs.ky[l + 1];
-    float *xa = cs.xa[l], *xb = cs.xb[l];
+    float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
       auto Y = [&](int ii, int jj) {
@@ -350,13 +454,14 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
       if (j < ny - 1) ay -= kn * Y(i, j + 1);
       xa[c] = yc + omega * dinv[c] * (r[c] - ay);
     }
-    BlockSync();
+    __syncthreads();
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
       xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
     }
-    BlockSync();
+    __syncthreads();
   }
+  for (int c = t; c < n0; c += T) cs.xb[0][c] = sxb[c];
 }
 
 inline int Blocks1D(long n) { return int(std::min<long>((n + 255) / 256, 4096)); }
@@ -396,6 +501,11 @@ MgPrecond::MgPrecond(int nx, int ny) {
   }
   if (first_in_kernel_ > 0 && int(lv_.size()) - first_in_kernel_ > kMaxInKernelLevels)
     first_in_kernel_ = int(lv_.size()) - kMaxInKernelLevels;
+  if (first_in_kernel_ > 0) {
+    long cells = 0;
+    for (int l = first_in_kernel_; l < int(lv_.size()); ++l) cells += long(lv_[l].nx) * lv_[l].ny;
+    if (cells > kLdsCells) first_in_kernel_ = -1;  // cannot happen: 2048 * 4/3 < kLdsCells
+  }
 }
 
 MgPrecond::~MgPrecond() {
@@ -424,8 +534,11 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     CoarseSet cs{};
     cs.count = int(lv_.size()) - l;
     cs.bilinear = bilinear ? 1 : 0;
+    int off = 0;
     for (int k = 0; k < cs.count; ++k) {
       const MgLevel& C = lv_[l + k];
+      cs.off[k] = off;
+      off += C.nx * C.ny;
       cs.nx[k] = C.nx;
       cs.ny[k] = C.ny;
       cs.kx[k] = C.kx;
@@ -451,13 +564,13 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
       return;
     }
   }
-  if (l == 0)
-    KSmoothFirst<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, r64, L.dinv, omega,
-                                                          L.xa, done);
-  else
-    KSmoothFirst<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.r, L.dinv, omega,
-                                                         L.xa, done);
   if (coarsest) {
+    if (l == 0)
+      KSmoothFirst<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, r64, L.dinv, omega,
+                                                            L.xa, done);
+    else
+      KSmoothFirst<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.r, L.dinv, omega,
+                                                           L.xa, done);
     // no coarser level (odd or tiny grids): seven more Jacobi sweeps ending
     // in `out`, ping-ponging between xa and xb
     float* cur = L.xa;
@@ -478,39 +591,28 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     return;
   }
   MgLevel& C = lv_[l + 1];
-  if (l == 0) {
-    KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
-                                                     L.xa, omega, L.xb, done);
-    KResRestrict<double><<<C.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.kx, L.ky, r64,
-                                                                 L.xb, C.nx, C.ny, C.tg, C.r,
-                                                                 done);
-  } else {
-    KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
-                                                    L.xa, omega, L.xb, done);
-    KResRestrict<float><<<C.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.kx, L.ky, L.r,
-                                                                L.xb, C.nx, C.ny, C.tg, C.r,
-                                                                done);
-  }
+  if (l == 0)
+    KDown<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
+                                                   omega, L.xb, C.nx, C.ny, C.r, done);
+  else
+    KDown<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
+                                                  omega, L.xb, C.nx, C.ny, C.r, done);
   Cycle(l + 1, nullptr, C.xb, st, done);
+#define HG_UP(RT, B, R)                                                                       \
+  KUp<RT, B><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, R, L.xb,  \
+                                              C.xb, C.kx, C.ky, C.nx, C.ny, omega, out, done)
   if (l == 0) {
     if (bilinear)
-      KProlongJacobi<double, true><<<blocks, kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
+      HG_UP(double, true, r64);
     else
-      KProlongJacobi<double, false><<<blocks, kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
-    KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
-                                                     L.xa, omega, out, done);
+      HG_UP(double, false, r64);
   } else {
     if (bilinear)
-      KProlongJacobi<float, true><<<blocks, kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
+      HG_UP(float, true, L.r);
     else
-      KProlongJacobi<float, false><<<blocks, kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega, L.xa, done);
-    KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
-                                                    L.xa, omega, out, done);
+      HG_UP(float, false, L.r);
   }
+#undef HG_UP
 }
 
 void MgPrecond::Apply(const double* r, float* z, hipStream_t st, const int* d_done) {
