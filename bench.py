@@ -7,14 +7,21 @@ noise).  A step is hydro::step (hydro2d.hpp:1531-1621): the SIMPLE loop
 solve to 1e-8 relative residual, correction), 10 TVD advection sub-steps and
 the property update, all on the GPU with fields resident in HBM.
 
-Reported: steps/s of the headline path (mixed precision, learned
-preconditioner when trained) next to the plain FP64 path, and the pressure
-solve to 1e-8 (ms) for both on the same SIMPLE pressure system.
-`roofline` is for the dominant kernels (one CG iteration: SpMV + update),
+Two paths run on the same state:
+  fp64      plain FP64 Jacobi-PCG pressure solve;
+  headline  FP64 flexible CG preconditioned by the FP32 multigrid V-cycle
+            plus the learned correction, which is trained on the simulation's
+            own pressure operator after the warm-up and kept only if it beats
+            the V-cycle alone on a held-out solve (fcg_learned, mixed).
+`value` is the headline steps/s.  The pressure solve to 1e-8 is also timed
+on the last SIMPLE pressure system of the fp64 path for every solver.
+`roofline` is the dominant kernel of the headline step (the finest-level
+up-leg of the V-cycle), its average duration measured with HIP events on
+the solver's stream and its algorithmic bytes from hydro_amd/manifest.py.
 `cpu_baseline` times the reference module built from its sources
-(oracle/_ref/hydro_ref_fast) on one step of the same grid.
+(oracle/_ref/hydro_ref_fast, OpenMP) on one step of the same grid.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 import argparse
 import json
@@ -30,13 +37,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_path(text, steps, warmup, label):
+def run_path(text, steps, warmup, label, train=False):
     from hydro_amd.hydro import Hydro
     from hydro_amd import workloads
     h = Hydro(text)
@@ -45,6 +53,17 @@ def run_path(text, steps, warmup, label):
         st = h.step()
         log(f"{label} warmup step {i + 1}: SIMPLE {st.simple_iterations}, "
             f"pressure iters {st.pressure_solve_iters_total}")
+    train_stats = None
+    if train:
+        ts = h.train_pressure_net()
+        train_stats = {"pairs": ts.pairs, "steps": ts.steps,
+                       "loss_first": round(ts.loss_first, 4), "loss_last": round(ts.loss_last, 4),
+                       "accepted": bool(ts.accepted),
+                       "heldout_iters": ts.val_iters, "heldout_ms": round(ts.val_ms, 3),
+                       "heldout_iters_vcycle_alone": ts.val_iters_baseline,
+                       "heldout_ms_vcycle_alone": round(ts.val_ms_baseline, 3),
+                       "train_ms": round(ts.ms, 1)}
+        log(f"{label} learned correction: {train_stats}")
     t0 = time.perf_counter()
     iters = []
     for i in range(steps):
@@ -52,7 +71,7 @@ def run_path(text, steps, warmup, label):
         iters.append((st.simple_iterations, st.pressure_solve_iters_total))
     dt = time.perf_counter() - t0
     log(f"{label}: {steps} steps in {dt:.3f} s, (SIMPLE, pressure iters) per step {iters}")
-    return h, dt, iters
+    return h, dt, iters, train_stats
 
 
 def cpu_baseline(seconds_cap=150):
@@ -86,10 +105,24 @@ set bool no_output 1
         return None
     t = float(m.group(2))
     sweeps = [int(x) for x in re.findall(r"iter = (\d+), diff", r.stdout)]
-    return {"value": 1.0 / t, "unit": "steps/s", "cores": cores, "kind": "reference",
+    return {"value": round(1.0 / t, 5), "unit": "steps/s", "cores": cores, "kind": "reference",
             "sample": f"1 step of the reference module (gauss_seidel + lu, OpenMP) on 1024x2048 "
                       f"two-phase RT-like state: {m.group(1)} SIMPLE iterations, "
                       f"SOR sweeps {sweeps}, {t:.2f} s"}
+
+
+def net_throughput(nx, ny):
+    """BF16 MFMA network alone at the RT grid: forward and training step."""
+    from hydro_amd.hydro import Net
+    n = Net(nx, ny)
+    t = n.time(reps=10)
+    n.close()
+    fwd = t["forward_flops"] / (t["forward_ms"] * 1e-3) / 1e12
+    trn = t["train_flops"] / (t["train_ms"] * 1e-3) / 1e12
+    return {"forward_ms": round(t["forward_ms"], 4), "train_step_ms": round(t["train_ms"], 4),
+            "forward_tflops": round(fwd, 2), "train_tflops": round(trn, 2),
+            "bf16_peak_tflops": BF16_PEAK_TFLOPS,
+            "forward_frac": round(fwd / BF16_PEAK_TFLOPS, 4)}
 
 
 def main():
@@ -104,49 +137,64 @@ def main():
               f"(--gpus {args.gpus} requested); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from hydro_amd import capi, workloads
+    from hydro_amd import capi, manifest
     from hydro_amd.hydro import Poisson
     from tests import cases
 
     nx, ny = 1024, 2048
-    ncell = nx * ny
     text64 = cases.rt(nx, ny, solver_p="cg", solver_v="lu_rb", precision="fp64")
     textmx = cases.rt(nx, ny, solver_p="fcg_learned", solver_v="lu_rb", precision="mixed")
 
-    h64, t64, it64 = run_path(text64, args.steps, args.warmup, "fp64")
+    h64, t64, it64, _ = run_path(text64, args.steps, args.warmup, "fp64")
     # the SIMPLE pressure system of the last step, for the solve comparison
     kf = h64.get(capi.HG_FIELD_PRESSURE_FACE_COEF)
     rhs = h64.get(capi.HG_FIELD_PRESSURE_RHS)
     h64.close()
-    hmx, tmx, itmx = run_path(textmx, args.steps, args.warmup, "mixed")
+    hmx, tmx, itmx, train = run_path(textmx, args.steps, args.warmup, "headline", train=True)
     hmx.close()
 
     nfx = (nx + 1) * ny
     ps = Poisson(nx, ny, kf[:nfx], kf[nfx:])
     solves = {}
-    for label, solver, prec in (("fp64", "cg", 0), ("cg_mixed", "cg", 1),
-                                ("learned_mixed", "fcg_learned", 1)):
+    for label, solver, prec in (("fp64_jacobi_pcg", "cg", 0), ("mixed_jacobi_pcg", "cg", 1),
+                                ("mixed_fcg_vcycle", "fcg_mg", 1)):
         ms, its, res = [], None, None
         for rep in range(4):
             _, st = ps.solve(rhs, solver=solver, tol=1e-8, precision=prec)
             if rep > 0:
                 ms.append(st.ms)
             its, res = st.iterations, st.rel_residual
-        solves[label] = {"ms": float(np.median(ms)), "iterations": int(its),
+        solves[label] = {"ms": round(float(np.median(ms)), 3), "iterations": int(its),
                          "rel_residual": float(res)}
         log(f"pressure solve {label}: {solves[label]}")
+
+    # kernel timings on the RT operator (the V-cycle's finest level and its
+    # one-workgroup coarse cycle), HIP events around back-to-back launches
+    kernels = {}
+    for name in ("KUp<double, true>", "KDown<double>"):
+        ms = ps.time_kernel(name, reps=50)
+        gbs = manifest.kernel_bytes(name, nx, ny) / (ms * 1e-3) / 1e9
+        kernels[name] = {"avg_us": round(ms * 1e3, 2), "gbs": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    kernels["KCoarseCycle"] = {"avg_us": round(ps.time_kernel("KCoarseCycle", reps=50) * 1e3, 2),
+                               "bound": "latency (one workgroup, levels <= 2048 cells)"}
     ps.close()
+    cg_ms_iter = solves["fp64_jacobi_pcg"]["ms"] / max(1, solves["fp64_jacobi_pcg"]["iterations"])
+    cg_gbs = manifest.cg_iteration_bytes(nx, ny) / (cg_ms_iter * 1e-3) / 1e9
+    kernels["fp64 Jacobi-PCG iteration (KPcgSpmv + KPcgUpdate)"] = {
+        "avg_us": round(cg_ms_iter * 1e3, 2), "gbs": round(cg_gbs, 1),
+        "frac": round(cg_gbs / HBM_PEAK_GBS, 4)}
+    log(f"kernels: {kernels}")
 
-    # roofline of one FP64 CG iteration: SpMV reads z, p_old, kx, ky and
-    # writes p, q (6 words); the update reads p, q, D^-1, x, r and writes
-    # x, r, z (8 words): 112 B per cell per iteration.
-    bytes_iter = 112.0 * ncell
-    ms_iter = solves["fp64"]["ms"] / solves["fp64"]["iterations"]
-    achieved = bytes_iter / (ms_iter * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "FP64 CG iteration (KSpmv + KUpdate), 112 B/cell"}
+    dom = "KUp<double, true>"
+    achieved = kernels[dom]["gbs"]
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"{dom}: finest-level up-leg of the V-cycle "
+                          f"({manifest.kernel_bytes(dom, nx, ny) / 1e6:.1f} MB per launch)"}
 
+    net = net_throughput(nx, ny)
+    log(f"net: {net}")
     cpu = None if args.no_cpu else cpu_baseline()
     steps_mx = args.steps / tmx
     steps_64 = args.steps / t64
@@ -164,14 +212,18 @@ def main():
         "dtype": "f64+f32 (mixed)",
         "data": "synthetic",
         "config": {"workload": "Rayleigh-Taylor 1024x2048, Atwood 0.5, cos amplitude 0.01 + "
-                               "1e-4 seeded noise; lu_rb velocity, cg pressure to 1e-8",
+                               "1e-4 seeded noise; lu_rb velocity, pressure to 1e-8",
                    "grid": [nx, ny],
-                   "path": "fcg_learned: FP64 flexible CG, FP32 multigrid V-cycle preconditioner"
-                           " (+ learned correction when trained)"},
+                   "path": "fcg_learned: FP64 flexible CG, FP32 multigrid V-cycle "
+                           "(+ learned correction when accepted)"},
         "steps_per_s_fp64": round(steps_64, 4),
         "ms_per_step_fp64": round(1e3 * t64 / args.steps, 3),
-        "pressure_solve_ms": dict(solves, speedup_learned_mixed_vs_fp64=round(
-            solves["fp64"]["ms"] / solves["learned_mixed"]["ms"], 3)),
+        "pressure_iters_per_step": {"headline": [p for _, p in itmx], "fp64": [p for _, p in it64]},
+        "learned_correction": train,
+        "pressure_solve_ms": dict(solves, speedup_mixed_fcg_vs_fp64=round(
+            solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2)),
+        "kernels": kernels,
+        "net": net,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

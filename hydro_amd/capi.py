@@ -159,7 +159,7 @@ EXPORTS = [
     "hg_fluid_finish_step", "hg_advect_substep", "hg_update_properties",
     "hg_calc_stats", "hg_step", "hg_last_step_timing",
     "hg_stencil_solve", "hg_poisson_create", "hg_poisson_destroy",
-    "hg_poisson_solve",
+    "hg_poisson_solve", "hg_poisson_time_kernel",
     "hg_train_opts_default", "hg_poisson_train", "hg_train_pressure_net",
     "hg_net_create", "hg_net_destroy", "hg_net_num_weights", "hg_net_configure",
     "hg_net_set_weights",
@@ -201,6 +201,7 @@ def _declare(lib):
         "hg_poisson_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, _dp, _P(vp)]),
         "hg_poisson_destroy": (None, [vp]),
         "hg_poisson_solve": (ctypes.c_int, [vp, _dp, _dp, _P(HgSolveOpts), _P(HgSolveStats)]),
+        "hg_poisson_time_kernel": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int32, _dp]),
         "hg_train_opts_default": (None, [_P(HgTrainOpts)]),
         "hg_poisson_train": (ctypes.c_int, [vp, _P(HgTrainOpts), _P(HgTrainStats)]),
         "hg_train_pressure_net": (ctypes.c_int, [vp, _P(HgTrainOpts), _P(HgTrainStats)]),

@@ -69,7 +69,8 @@ class LearnedNet : public Precond {
   LearnedNet& operator=(const LearnedNet&) = delete;
   static bool Supported(int nx, int ny) { return UNet::Supported(nx, ny); }
   // True once training produced a net that beat the method without it on
-  // the held-out acceptance solve; the solvers use the net only then.
+  // the held-out acceptance solve (fewer iterations and less device time);
+  // the solvers use the net only then.
   bool Trained() const { return trained_; }
   void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;
   // Generates pairs on cg's operator (cg.SetOperator and mg->Build done) and

@@ -438,7 +438,9 @@ TrainReport LearnedNet::Train(CgSolver& cg, const TrainOpts& o) {
     rep.val_iters_baseline = without.iterations;
     rep.val_ms = with.ms;
     rep.val_ms_baseline = without.ms;
-    trained_ = with.converged && !with.used_fallback && with.iterations < without.iterations;
+    // used from now on only when it is both fewer iterations and faster
+    trained_ = with.converged && !with.used_fallback && with.iterations < without.iterations &&
+               with.ms < without.ms;
   }
   rep.accepted = trained_;
   rep.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -49,6 +49,11 @@ class MgPrecond : public Precond {
   // nothing when *d_done is set.
   void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;
   int levels() const { return int(lv_.size()); }
+  // Average device time (ms) of one launch of a V-cycle kernel ("KDown<double>",
+  // "KUp<double, true>" on the finest level, "KCoarseCycle"), launched `reps`
+  // times back to back between two HIP events on `st`, reading r (FP64,
+  // finest level) and the level buffers.  Throws on an unknown name.
+  double TimeKernel(const char* name, int reps, const double* r, hipStream_t st);
   const MgLevel& level(int l) const { return lv_[l]; }
   float omega = 0.8f;
   int coarse_sweeps = 40;

@@ -619,4 +619,39 @@ void MgPrecond::Apply(const double* r, float* z, hipStream_t st, const int* d_do
   Cycle(0, r, z, st, d_done);
 }
 
+double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStream_t st) {
+  if (lv_.size() < 2) throw std::runtime_error("TimeKernel: single-level hierarchy");
+  const std::string k(name);
+  MgLevel& L = lv_[0];
+  MgLevel& C = lv_[1];
+  auto launch = [&]() {
+    if (k == "KDown<double>") {
+      KDown<double><<<L.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+                                                            r, omega, L.xb, C.nx, C.ny, C.r,
+                                                            nullptr);
+    } else if (k == "KUp<double, true>") {
+      KUp<double, true><<<L.tg.blocks(), kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega,
+          L.xa, nullptr);
+    } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
+      Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xb, st, nullptr);
+    } else {
+      throw std::runtime_error("TimeKernel: unknown kernel " + k);
+    }
+  };
+  launch();  // warm-up
+  hipEvent_t e0, e1;
+  HG_CHECK(hipEventCreate(&e0));
+  HG_CHECK(hipEventCreate(&e1));
+  HG_CHECK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) launch();
+  HG_CHECK(hipEventRecord(e1, st));
+  HG_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HG_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return double(ms) / reps;
+}
+
 }  // namespace hg

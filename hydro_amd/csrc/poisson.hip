@@ -232,6 +232,18 @@ hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
   }
 }
 
+hg_status hg_poisson_time_kernel(hg_poisson* ps, const char* kernel, int32_t reps,
+                                 double* avg_ms) {
+  if (!ps || !kernel || reps < 1 || !avg_ms) return HG_ERR_BAD_ARG;
+  try {
+    *avg_ms = ps->mg->TimeKernel(kernel, reps, ps->b, ps->st);
+    return HG_OK;
+  } catch (const std::exception& ex) {
+    std::fprintf(stderr, "hg_poisson_time_kernel: %s\n", ex.what());
+    return HG_ERR_BAD_ARG;
+  }
+}
+
 void hg_train_opts_default(hg_train_opts* o) {
   if (!o) return;
   hg::TrainOpts d;

@@ -185,6 +185,13 @@ class Poisson:
             _raise(self.lib, None, rc, "hg_poisson_solve")
         return p, st
 
+    def time_kernel(self, kernel, reps=20):
+        """Average device ms of one launch of a pressure-solve kernel."""
+        ms = ctypes.c_double()
+        rc = self.lib.hg_poisson_time_kernel(self.ps, kernel.encode(), reps, ctypes.byref(ms))
+        _raise(self.lib, None, rc, "hg_poisson_time_kernel")
+        return ms.value
+
     def train(self, rhs_count=None, mg_iters=None, steps=None, lr=None, seed=None, flags=None):
         """Trains the learned preconditioner on this operator; returns the
         hg_train_stats (pairs, steps, loss_first, loss_last, ms)."""

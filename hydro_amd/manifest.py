@@ -1,0 +1,95 @@
+"""Algorithmic HBM bytes per launch of every kernel on the time-step path.
+
+Counting rule: each array a kernel must read or write between two global
+synchronisations counts once, at its element size (FP64 8 B, FP32 4 B,
+BF16 2 B), whatever the kernel actually re-reads through the caches.  N is
+the number of cells (nx*ny), F the number of faces ((nx+1)*ny + nx*(ny+1)),
+Fx/Fy the x-/y-faces.  Level-l multigrid arrays have N_l = N / 4^l cells.
+
+The same table gives `roofline.achieved` in bench.py (bytes / measured
+duration) and the per-kernel bandwidth column of the profile summaries.
+"""
+
+F64, F32 = 8, 4
+
+
+def sizes(nx, ny):
+    n = nx * ny
+    fx, fy = (nx + 1) * ny, nx * (ny + 1)
+    return n, fx + fy, fx, fy
+
+
+def _b(nx, ny, cells64=0, faces64=0, cells32=0, faces32=0, coarse32=0):
+    n, f, _, _ = sizes(nx, ny)
+    return (cells64 * n + faces64 * f) * F64 + (cells32 * n + faces32 * f + coarse32 * n) * F32
+
+
+# name -> (bytes(nx, ny), what is counted)
+KERNELS = {
+    # ---- SIMPLE iteration (step_kernels.hip) ----
+    "KForceFaces": (lambda nx, ny: _b(nx, ny, cells64=3, faces64=3),
+                    "read f_x, f_y, mu; write face f_x, f_y, mu"),
+    "KRestoredForce": (lambda nx, ny: _b(nx, ny, cells64=2, faces64=1),
+                       "read face f_x (x-faces), f_y (y-faces); write cell restored force x, y"),
+    "KFacesDeriv0Pair": (lambda nx, ny: _b(nx, ny, cells64=2, faces64=2),
+                         "read 2 cell fields; write 2 face fields"),
+    "KCellGrads": (lambda nx, ny: _b(nx, ny, cells64=9),
+                   "read p, u0, u1; write 3 gradients x 2 components"),
+    "KMomentum": (lambda nx, ny: _b(nx, ny, cells64=22, faces64=2),
+                  "read u0, u1, 4 velocity gradients, grad p (2), restored force (2), rho, "
+                  "u at t-1 and t (4), face mu, volume flux; write 5 row coefficients + 2 constants"),
+    "KDiag": (lambda nx, ny: _b(nx, ny, cells64=6), "read 5 row coefficients; write a_P"),
+    "KLuRbBlack": (lambda nx, ny: _b(nx, ny, cells64=8),
+                   "read 5 coefficients, 2 constants; write the black half of 2 iterates"),
+    "KLuRbRed": (lambda nx, ny: _b(nx, ny, cells64=13),
+                 "read 5 coefficients, 2 constants, 2 black iterates, 2 bases; write 2 results"),
+    "KRhieChow": (lambda nx, ny: _b(nx, ny, cells64=6, faces64=4),
+                  "read u0, u1, grad p (2), p, a_P, face restored force and force; "
+                  "write flux* and flux coefficient"),
+    "KPressureRows": (lambda nx, ny: _b(nx, ny, cells64=6, faces64=2),
+                      "read flux*, flux coefficient; write 5 row coefficients + constant"),
+    "KCorrectCells": (lambda nx, ny: _b(nx, ny, cells64=10),
+                      "read p', a_P, p_prev, u0, u1, u_prev (2); write p, u0, u1"),
+    "KCorrectFaces": (lambda nx, ny: _b(nx, ny, cells64=1, faces64=3),
+                      "read flux*, flux coefficient, p'; write flux"),
+    "KNegateFaces": (lambda nx, ny: _b(nx, ny, faces64=2), "read flux coefficient; write k_f"),
+    "KNeg": (lambda nx, ny: _b(nx, ny, cells64=2), "read constant; write right-hand side"),
+    # ---- advection and properties ----
+    "KAdvGrad": (lambda nx, ny: _b(nx, ny, cells64=3), "read field; write gradient (2)"),
+    "KAdvUpdate": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
+                   "read field, gradient (2), volume flux; write field"),
+    "KProperties": (lambda nx, ny: _b(nx, ny, cells64=8),
+                    "2 phases: read 2 partial densities; write 2 fractions, rho, mu, force (2)"),
+    "KSmooth": (lambda nx, ny: _b(nx, ny, cells64=2), "read field; write smoothed field"),
+    # ---- FP64 Jacobi-PCG iteration (cg.hip) ----
+    "KPcgSpmv<double>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
+                         "read z, p_old, k_x, k_y; write p, q"),
+    "KPcgUpdate<double>": (lambda nx, ny: _b(nx, ny, cells64=8),
+                           "read p, q, D^-1, x, r; write x, r, z"),
+    # ---- FCG iteration with the FP32 multigrid preconditioner ----
+    "KFcgCheck": (lambda nx, ny: 0, "one workgroup folds partial sums"),
+    "KFcgDir": (lambda nx, ny: _b(nx, ny, cells64=2, cells32=1), "read z (FP32), q_old, r"),
+    "KFcgSpmv": (lambda nx, ny: _b(nx, ny, cells64=3, faces64=1, cells32=1),
+                 "read z (FP32), p_old, k_x, k_y; write p, q"),
+    "KFcgUpdate": (lambda nx, ny: _b(nx, ny, cells64=6), "read p, q, x, r; write x, r"),
+    # level 0 of the V-cycle (FP32 level arrays, FP64 residual input)
+    "KDown<double>": (lambda nx, ny: _b(nx, ny, cells64=1, cells32=2, faces32=1, coarse32=0.25),
+                      "read r (FP64), D^-1, k_x, k_y; write x2; write the coarse residual (N/4)"),
+    "KUp<double, true>": (lambda nx, ny: _b(nx, ny, cells64=1, cells32=3, faces32=1.25,
+                                            coarse32=0.25),
+                          "read x2, r (FP64), D^-1, k_x, k_y, coarse correction and coarse k; "
+                          "write the level output"),
+}
+
+
+def kernel_bytes(name, nx, ny):
+    """Algorithmic bytes of one launch of `name` on an nx x ny grid (for the
+    multigrid kernels: on that level)."""
+    fn, _ = KERNELS[name]
+    return float(fn(nx, ny))
+
+
+def cg_iteration_bytes(nx, ny):
+    """FP64 Jacobi-PCG iteration: SpMV + update (16 words per cell counting
+    k_x, k_y as one word each)."""
+    return kernel_bytes("KPcgSpmv<double>", nx, ny) + kernel_bytes("KPcgUpdate<double>", nx, ny)

@@ -288,6 +288,15 @@ void hg_poisson_destroy(hg_poisson* ps);
 hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
                            const hg_solve_opts* opts, hg_solve_stats* stats);
 
+/* Average device time (ms) of one launch of a pressure-solve kernel on this
+ * object's operator, launched `reps` times back to back between two HIP
+ * events on its stream (no reference counterpart; for throughput figures).
+ * kernel: "KDown<double>" / "KUp<double, true>" (finest multigrid level),
+ * "KCoarseCycle" (the one-workgroup coarse V-cycle).  The data are whatever
+ * the level buffers hold; call after a solve. */
+hg_status hg_poisson_time_kernel(hg_poisson* ps, const char* kernel, int32_t reps,
+                                 double* avg_ms);
+
 /* ---- learned preconditioner (no reference counterpart: the reference's
  * pressure solvers are the stationary sweeps of linear.hpp) ----------------
  * HG_SOLVER_FCG_LEARNED preconditions flexible CG with
@@ -322,7 +331,8 @@ typedef struct hg_train_stats {
   /* Acceptance solve on a held-out right-hand side to a 1e-8 true FP64
    * relative residual: FCG with the learned preconditioner against the
    * method without it (FCG + V-cycle; Jacobi-PCG for HG_TRAIN_STANDALONE).
-   * The solvers use the net only when it took fewer iterations. */
+   * The solvers use the net only when it took fewer iterations and less
+ * device time. */
   int32_t accepted;
   int32_t val_iters;
   int32_t val_iters_baseline;
