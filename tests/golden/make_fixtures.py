@@ -9,6 +9,9 @@ not needed afterwards).
    (oracle/_ref/hydro_ref, built from the reference sources by oracle/ref.mk
    with -O2 -ffp-contract=off) after given steps, with the SIMPLE iteration
    counts and pressure-solver sweep counts it printed.
+3. ref_*.json: the same for long trajectories, stored compactly: the
+   SHA-256 of each FP64 field's bytes (bit-for-bit identity) and its L2 norm
+   and maximum (to report how far a mismatch is), per dumped step.
 
     python tests/golden/make_fixtures.py
 """
@@ -76,12 +79,51 @@ def reference_dump(name, text, steps, dumps):
         print(f"{name}: {steps} steps, SIMPLE iterations {iters}")
 
 
+LONG = {
+    # C1: lid-driven cavity 64x64, 200 steps; C2: broken dam 256x128, the
+    # first 50 steps (later steps of this case need the 1000-sweep limit of
+    # the pressure solver on every SIMPLE iteration)
+    "ref_cavity64_200": (cases.cavity(64), 200, (200,)),
+    "ref_dam256x128_50": (cases.dam(256, 128), 50, (50,)),
+}
+
+
+def reference_hashes(name, text, steps, dumps):
+    import hashlib
+    import json
+    script = open(GENERAL).read() + "\n" + text + "\nset bool no_output 1\n"
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.conf")
+        open(path, "w").write(script)
+        r = subprocess.run([pyoracle.REF_BIN, path, str(steps), os.path.join(tmp, "d"),
+                            str(min(dumps))], capture_output=True, text=True, cwd=tmp)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        out = {"text": text, "steps": steps,
+               "simple_iters": [int(m.group(1)) for m in
+                                re.finditer(r"STEP s=\d+ iters=(\d+)", r.stdout)],
+               "pressure_sweeps_total": sum(int(m.group(1)) for m in
+                                            re.finditer(r"iter = (\d+), diff", r.stdout)),
+               "dumps": {}}
+        for s in dumps:
+            d = pyoracle.read_ref_dump(os.path.join(tmp, f"d.{s}.bin"))
+            out["dumps"][str(s)] = {
+                k: {"sha256": hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest(),
+                    "l2": float(np.linalg.norm(v)), "max": float(np.abs(v).max())}
+                for k, v in d.items() if isinstance(v, np.ndarray)}
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(out, f, indent=1)
+        print(f"{name}: {steps} steps, SIMPLE iterations total {sum(out['simple_iters'])}")
+
+
 def main():
     if not os.path.exists(pyoracle.REF_BIN):
         raise SystemExit("build oracle/_ref first: make -f oracle/ref.mk")
     sample()
     for name, (text, steps, dumps) in FIXTURES.items():
         reference_dump(name, text, steps, dumps)
+    for name, (text, steps, dumps) in LONG.items():
+        reference_hashes(name, text, steps, dumps)
 
 
 if __name__ == "__main__":

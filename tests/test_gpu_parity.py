@@ -82,3 +82,26 @@ def test_stencil_solvers_bitwise(gpu, solver):
                                     omega=omega)
     assert sg.iterations == so.iterations
     assert np.array_equal(xg, xo)
+
+
+@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50"])
+def test_long_trajectory_matches_reference(gpu, name):
+    """C1 (cavity 64x64, 200 steps) and the first 50 steps of C2 (dam
+    256x128): every dumped field bit for bit against the reference module's
+    own run (SHA-256 digests in tests/golden), same SIMPLE iterations per
+    step and same total pressure sweeps."""
+    import json
+    import os
+    from tests.test_oracle_reference import GOLDEN, check_long_fixture, field_digests
+    fix = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    print(f"case: {name}, {fix['steps']} steps, gauss_seidel + lu", flush=True)
+    g = Hydro(fix["text"])
+    stats, digests = [], {}
+    for s in range(1, fix["steps"] + 1):
+        stats.append(g.step())
+        if s % 50 == 0:
+            print(f"  step {s}", flush=True)
+        if str(s) in fix["dumps"]:
+            digests[s] = field_digests(g.get, g.cfg.num_phases)
+    g.close()
+    check_long_fixture(fix, digests, stats)

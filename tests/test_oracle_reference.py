@@ -63,3 +63,43 @@ def test_oracle_reproduces_cavity_sample_log():
     rel = np.abs(np.array(out) - rs_ref) / rs_ref
     # 6 significant digits in the log: rounding error <= 5e-6 relative
     assert rel.max() < 5e-6, rel.max()
+
+
+def field_digests(get, num_phases):
+    """SHA-256 of each FP64 field's bytes, keyed like the reference dump."""
+    import hashlib
+    out = {}
+    arrays = [(name, get(f)) for name, f in FIELDS]
+    for i in range(num_phases):
+        arrays.append((f"alpha{i}", get(capi.HG_FIELD_VOLUME_FRACTION_0 + i)))
+        arrays.append((f"pd{i}", get(capi.HG_FIELD_PARTIAL_DENSITY_0 + i)))
+    for name, a in arrays:
+        out[name] = (hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+                     .hexdigest(), float(np.linalg.norm(a)))
+    return out
+
+
+def check_long_fixture(fix, digests_at_step, step_stats):
+    """Compares a run with a long-trajectory fixture: SIMPLE iterations per
+    step, total pressure sweeps, and every dumped field bit for bit (digest),
+    reporting the relative L2-norm difference of any mismatch."""
+    assert [s.simple_iterations for s in step_stats] == fix["simple_iters"]
+    assert sum(s.pressure_solve_iters_total for s in step_stats) == fix["pressure_sweeps_total"]
+    for step, fields in fix["dumps"].items():
+        got = digests_at_step[int(step)]
+        bad = {k: abs(got[k][1] - v["l2"]) / max(v["l2"], 1e-300)
+               for k, v in fields.items() if k in got and got[k][0] != v["sha256"]}
+        assert not bad, (step, bad)
+
+
+@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50"])
+def test_oracle_long_trajectory(name):
+    import json
+    fix = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    o = pyoracle.Oracle(capi.parse_config(fix["text"]))
+    stats, digests = [], {}
+    for s in range(1, fix["steps"] + 1):
+        stats.append(o.step())
+        if str(s) in fix["dumps"]:
+            digests[s] = field_digests(o.get, o.cfg.num_phases)
+    check_long_fixture(fix, digests, stats)
