@@ -434,12 +434,13 @@ void FinishStep(hg_ctx* c) {
 }
 
 void AdvectSubstep(hg_ctx* c) {
-  for (int i = 0; i < c->np; ++i) {
-    hg::LaunchAdvGrad(c->st, c->geo, c->pd[i], c->g0x, c->g0y);
-    hg::LaunchAdvUpdate(c->st, c->geo, c->pd[i], c->g0x, c->g0y, c->fl_ic, c->dt_adv,
-                        c->pd_new[i]);
-    std::swap(c->pd[i], c->pd_new[i]);
-  }
+  // all phase fields in one pass (the reference advects them one after the
+  // other with the same flux; the fields do not interact within a sub-step)
+  const double* u[3] = {c->pd[0], c->np > 1 ? c->pd[1] : nullptr, c->np > 2 ? c->pd[2] : nullptr};
+  double* un[3] = {c->pd_new[0], c->np > 1 ? c->pd_new[1] : nullptr,
+                   c->np > 2 ? c->pd_new[2] : nullptr};
+  hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un);
+  for (int i = 0; i < c->np; ++i) std::swap(c->pd[i], c->pd_new[i]);
   c->t_adv += c->dt_adv;
 }
 

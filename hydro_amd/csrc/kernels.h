@@ -59,6 +59,9 @@ void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double*
 void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, double* gy);
 void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
                      const double* gy, const double* flux, double dt, double* un);
+// One advection sub-step of nf (1..3) fields with gradients formed in place.
+void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
+                   const double* flux, double dt, double* const un[3]);
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy);
 void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out);

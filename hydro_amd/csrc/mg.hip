@@ -149,21 +149,22 @@ __device__ __forceinline__ float Interp(const float* __restrict__ e,
 
 // Fused down-leg of one level: two damped Jacobi sweeps from zero
 // (x1 = omega D^-1 r, x2 = x1 + omega D^-1 (r - A x1)) and the restriction of
-// r - A x2 (sum of the four children).  The block works on its 64 x 4 tile
-// with a two-cell halo in LDS (x1 on tile + 2, x2 on tile + 1), so one launch
-// and one pass over the level's arrays replace three.
-template <class RT>
+// r - A x2 (sum of the four children).  A block of 256 threads works on a
+// 64 x TY tile with a two-cell halo in LDS (x1 on tile + 2, x2 on tile + 1),
+// so one launch and one pass over the level's arrays replace three; taller
+// tiles re-read less halo.
+template <class RT, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
       const float* __restrict__ dinv, const RT* __restrict__ r, float omega,
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done) {
   if (Done(done)) return;
-  constexpr int H2X = kTX + 4, H2Y = kTY + 4, H1X = kTX + 2, H1Y = kTY + 2;
-  __shared__ float s1[H2Y][H2X], sr[H2Y][H2X], s2[H1Y][H1X], sres[kTY][kTX];
+  constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
+  __shared__ float s1[H2Y][H2X], sr[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
-  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * kTY;
+  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
     const int hj = q / H2X, hi = q % H2X;
     const int ii = i0 + hi - 2, jj = j0 + hj - 2;
@@ -196,8 +197,8 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     s2[hj][hi] = xv;
   }
   __syncthreads();
-  {
-    const int li = threadIdx.x & (kTX - 1), lj = threadIdx.x / kTX;
+  for (int q = threadIdx.x; q < TY * kTX; q += kTileThreads) {
+    const int li = q % kTX, lj = q / kTX;
     const int i = i0 + li, j = j0 + lj;
     float res = 0.f;
     if (i < nx && j < ny) {
@@ -216,8 +217,8 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     sres[lj][li] = res;
   }
   __syncthreads();
-  if (threadIdx.x < (kTX / 2) * (kTY / 2)) {
-    const int ci = threadIdx.x % (kTX / 2), cj = threadIdx.x / (kTX / 2);
+  for (int q = threadIdx.x; q < (kTX / 2) * (TY / 2); q += kTileThreads) {
+    const int ci = q % (kTX / 2), cj = q / (kTX / 2);
     const int I = i0 / 2 + ci, J = j0 / 2 + cj;
     if (I < nxc && J < nyc)
       rc[I + nxc * J] = ((sres[2 * cj][2 * ci] + sres[2 * cj][2 * ci + 1]) +
@@ -226,21 +227,21 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
 }
 
 // Fused up-leg of one level: y = x2 + P e, x3 = y + omega D^-1 (r - A y),
-// out = x3 + omega D^-1 (r - A x3), on the block's tile with a two-cell halo
-// in LDS.
-template <class RT, bool BILINEAR>
+// out = x3 + omega D^-1 (r - A x3), on the block's 64 x TY tile with a
+// two-cell halo in LDS.
+template <class RT, bool BILINEAR, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
     const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
     int nxc, int nyc, float omega, float* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  constexpr int H2X = kTX + 4, H2Y = kTY + 4, H1X = kTX + 2, H1Y = kTY + 2;
+  constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
   __shared__ float sy[H2Y][H2X], sr[H1Y][H1X], s3[H1Y][H1X];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
-  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * kTY;
+  const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
     const int hj = q / H2X, hi = q % H2X;
     const int ii = i0 + hi - 2, jj = j0 + hj - 2;
@@ -274,19 +275,21 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     s3[hj][hi] = xv;
   }
   __syncthreads();
-  const int li = threadIdx.x & (kTX - 1), lj = threadIdx.x / kTX;
-  const int i = i0 + li, j = j0 + lj;
-  if (i >= nx || j >= ny) return;
-  const int c = i + nx * j;
-  const int a = lj + 1, f = li + 1;
-  const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-  const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-  float v = (((kw + ke) + ks) + kn) * s3[a][f];
-  if (i > 0) v -= kw * s3[a][f - 1];
-  if (i < nx - 1) v -= ke * s3[a][f + 1];
-  if (j > 0) v -= ks * s3[a - 1][f];
-  if (j < ny - 1) v -= kn * s3[a + 1][f];
-  out[c] = s3[a][f] + omega * dinv[c] * (sr[a][f] - v);
+  for (int q = threadIdx.x; q < TY * kTX; q += kTileThreads) {
+    const int li = q % kTX, lj = q / kTX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= nx || j >= ny) continue;
+    const int c = i + nx * j;
+    const int a = lj + 1, f = li + 1;
+    const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+    const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+    float v = (((kw + ke) + ks) + kn) * s3[a][f];
+    if (i > 0) v -= kw * s3[a][f - 1];
+    if (i < nx - 1) v -= ke * s3[a][f + 1];
+    if (j > 0) v -= ks * s3[a - 1][f];
+    if (j < ny - 1) v -= kn * s3[a + 1][f];
+    out[c] = s3[a][f] + omega * dinv[c] * (sr[a][f] - v);
+  }
 }
 
 // Coarsest level: Jacobi sweeps in one workgroup, x and r in LDS.
@@ -475,6 +478,7 @@ MgPrecond::MgPrecond(int nx, int ny) {
     L.nx = cx;
     L.ny = cy;
     L.tg = MakeTileGrid(cx, cy);
+    L.tall = MakeTileGrid(cx, cy, kTallY);
     long n = long(cx) * cy;
     HG_CHECK(hipMalloc(&L.kx, size_t(cx + 1) * cy * 4));
     HG_CHECK(hipMalloc(&L.ky, size_t(cx) * (cy + 1) * 4));
@@ -591,26 +595,28 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     return;
   }
   MgLevel& C = lv_[l + 1];
+  // level 0 (FP64 residual) on 64 x 16 tiles, the FP32 levels on 64 x 4
   if (l == 0)
-    KDown<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
-                                                   omega, L.xb, C.nx, C.ny, C.r, done);
+    KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, omega, L.xb, C.nx, C.ny, C.r, done);
   else
-    KDown<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
-                                                  omega, L.xb, C.nx, C.ny, C.r, done);
+    KDown<float, kTY><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+                                                       L.r, omega, L.xb, C.nx, C.ny, C.r, done);
   Cycle(l + 1, nullptr, C.xb, st, done);
-#define HG_UP(RT, B, R)                                                                       \
-  KUp<RT, B><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, R, L.xb,  \
-                                              C.xb, C.kx, C.ky, C.nx, C.ny, omega, out, done)
+#define HG_UP(RT, B, TY, TG, R)                                                               \
+  KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
+                                                      L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,    \
+                                                      omega, out, done)
   if (l == 0) {
     if (bilinear)
-      HG_UP(double, true, r64);
+      HG_UP(double, true, kTallY, L.tall, r64);
     else
-      HG_UP(double, false, r64);
+      HG_UP(double, false, kTallY, L.tall, r64);
   } else {
     if (bilinear)
-      HG_UP(float, true, L.r);
+      HG_UP(float, true, kTY, L.tg, L.r);
     else
-      HG_UP(float, false, L.r);
+      HG_UP(float, false, kTY, L.tg, L.r);
   }
 #undef HG_UP
 }
@@ -626,12 +632,11 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
   MgLevel& C = lv_[1];
   auto launch = [&]() {
     if (k == "KDown<double>") {
-      KDown<double><<<L.tg.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
-                                                            r, omega, L.xb, C.nx, C.ny, C.r,
-                                                            nullptr);
+      KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, omega, L.xb, C.nx, C.ny, C.r, nullptr);
     } else if (k == "KUp<double, true>") {
-      KUp<double, true><<<L.tg.blocks(), kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega,
+      KUp<double, true, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega,
           L.xa, nullptr);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
       Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xb, st, nullptr);

@@ -547,6 +547,67 @@ KAdvUpdate(Geo g, const double* __restrict__ u, const double* __restrict__ gx,
 
 // ---------------------------------------------------------------------------
 // Fluid properties (hydro2d.hpp:980-997, 1221-1246, 1308-1318).
+// Gradient components of the advected field with the zero-derivative
+// condition, the same operations as CellFaces<1> + GradFromFaces.
+__device__ __forceinline__ double AdvGradX(const Geo& g, const double* __restrict__ u, int i,
+                                           int j) {
+  const int c = i + g.nx * j;
+  const double uc = u[c];
+  const double fw = i > 0 ? Mid(u[c - 1], uc) : uc;
+  const double fe = i < g.nx - 1 ? Mid(uc, u[c + 1]) : uc;
+  double sxv = (-g.sx) * fw;
+  sxv += g.sx * fe;
+  return sxv / g.V;
+}
+__device__ __forceinline__ double AdvGradY(const Geo& g, const double* __restrict__ u, int i,
+                                           int j) {
+  const int c = i + g.nx * j;
+  const double uc = u[c];
+  const double fs = j > 0 ? Mid(u[c - g.nx], uc) : uc;
+  const double fn = j < g.ny - 1 ? Mid(uc, u[c + g.nx]) : uc;
+  double syv = (-g.sy) * fs;
+  syv += g.sy * fn;
+  return syv / g.V;
+}
+
+// One advection sub-step of NF fields sharing the volume flux
+// (advection.hpp:440-539): the cell gradients the Superbee faces need are
+// formed in place (KAdvGrad's values, bit for bit) instead of being written
+// to HBM and read back, and the flux is read once for all fields.
+struct AdvFields {
+  const double* u[3];
+  double* un[3];
+};
+
+template <int NF>
+__global__ void __launch_bounds__(kBlock)
+KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  const double vw = flux[fw], ve = flux[fe], vs = flux[fs], vn = flux[fn];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const double* __restrict__ u = fl.u[q];
+    const double uc = u[c];
+    const double gxc = AdvGradX(g, u, i, j), gyc = AdvGradY(g, u, i, j);
+    double uw = uc, ue = uc, us_ = uc, uN = uc;
+    if (i > 0) uw = SuperbeeFace(u[c - 1], uc, AdvGradX(g, u, i - 1, j), gxc, g.tx0, g.tx1, vw);
+    if (i < nx - 1) ue = SuperbeeFace(uc, u[c + 1], gxc, AdvGradX(g, u, i + 1, j), g.tx0, g.tx1, ve);
+    if (j > 0) us_ = SuperbeeFace(u[c - nx], uc, AdvGradY(g, u, i, j - 1), gyc, g.ty0, g.ty1, vs);
+    if (j < ny - 1) uN = SuperbeeFace(uc, u[c + nx], gyc, AdvGradY(g, u, i, j + 1), g.ty0, g.ty1, vn);
+    double fsum = 0.;
+    fsum += uw * vw * -1.;
+    fsum += ue * ve * 1.;
+    fsum += us_ * vs * -1.;
+    fsum += uN * vn * 1.;
+    fl.un[q][c] = uc + -dt / g.V * fsum;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 KProperties(Geo g, Phases ph, const double* __restrict__ pd0, const double* __restrict__ pd1,
             const double* __restrict__ pd2, double* __restrict__ a0, double* __restrict__ a1,
@@ -742,6 +803,21 @@ void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, do
 void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
                      const double* gy, const double* flux, double dt, double* un) {
   KAdvUpdate<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, gx, gy, flux, dt, un);
+}
+void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
+                   const double* flux, double dt, double* const un[3]) {
+  AdvFields fl{};
+  for (int q = 0; q < nf; ++q) {
+    fl.u[q] = u[q];
+    fl.un[q] = un[q];
+  }
+  const dim3 grid = Grid(long(g.nx) * g.ny);
+  if (nf == 1)
+    KAdvStep<1><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+  else if (nf == 2)
+    KAdvStep<2><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+  else
+    KAdvStep<3><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
 }
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy) {

@@ -21,10 +21,10 @@ struct TileGrid {
   int blocks() const { return per * kXcds; }
 };
 
-inline TileGrid MakeTileGrid(int nx, int ny) {
+inline TileGrid MakeTileGrid(int nx, int ny, int ty = kTY) {
   TileGrid t;
   t.ntx = (nx + kTX - 1) / kTX;
-  t.nty = (ny + kTY - 1) / kTY;
+  t.nty = (ny + ty - 1) / ty;
   int nt = t.ntx * t.nty;
   t.per = (nt + kXcds - 1) / kXcds;
   return t;
