@@ -56,9 +56,6 @@ void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const
                         const double* pc, double* flux);
 void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                        const double* u1, double mvx, double mvy, double* flux);
-void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, double* gy);
-void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
-                     const double* gy, const double* flux, double dt, double* un);
 // One advection sub-step of nf (1..3) fields with gradients formed in place.
 void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
                    const double* flux, double dt, double* const un[3]);

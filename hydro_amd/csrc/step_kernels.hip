@@ -498,15 +498,6 @@ KInitialFlux(Geo g, Bc bc, const double* __restrict__ u0, const double* __restri
 // Advection (advection.hpp:440-478): gradient of the zero-derivative face
 // interpolation, then Superbee face values and the explicit update.
 
-__global__ void __launch_bounds__(kBlock)
-KAdvGrad(Geo g, const double* __restrict__ u, double* __restrict__ gx, double* __restrict__ gy) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.nx * g.ny) return;
-  double f[4];
-  CellFaces<1>(g, u, c % g.nx, c / g.nx, nullptr, f);
-  GradFromFaces(g, f, gx[c], gy[c]);
-}
-
 __device__ __forceinline__ double Superbee(double p, double q) {
   if (p > 0. && q > 0.) return fmax(fmin(2 * p, q), fmin(p, 2 * q));
   if (p < 0. && q < 0.) return -fmax(fmin(-2 * p, -q), fmin(-p, -2 * q));
@@ -521,90 +512,94 @@ __device__ __forceinline__ double SuperbeeFace(double uP, double uE, double gPn,
   return 0.5 * (uP + uE);
 }
 
-__global__ void __launch_bounds__(kBlock)
-KAdvUpdate(Geo g, const double* __restrict__ u, const double* __restrict__ gx,
-           const double* __restrict__ gy, const double* __restrict__ flux, double dt,
-           double* __restrict__ un) {
-  const int nx = g.nx, ny = g.ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny) return;
-  const int i = c % nx, j = c / nx;
-  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
-  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
-  const double uc = u[c];
-  double vw = flux[fw], ve = flux[fe], vs = flux[fs], vn = flux[fn];
-  double uw = i > 0 ? SuperbeeFace(u[c - 1], uc, gx[c - 1], gx[c], g.tx0, g.tx1, vw) : uc;
-  double ue = i < nx - 1 ? SuperbeeFace(uc, u[c + 1], gx[c], gx[c + 1], g.tx0, g.tx1, ve) : uc;
-  double us = j > 0 ? SuperbeeFace(u[c - nx], uc, gy[c - nx], gy[c], g.ty0, g.ty1, vs) : uc;
-  double uN = j < ny - 1 ? SuperbeeFace(uc, u[c + nx], gy[c], gy[c + nx], g.ty0, g.ty1, vn) : uc;
-  double fsum = 0.;
-  fsum += uw * vw * -1.;
-  fsum += ue * ve * 1.;
-  fsum += us * vs * -1.;
-  fsum += uN * vn * 1.;
-  un[c] = uc + -dt / g.V * fsum;
-}
-
-// ---------------------------------------------------------------------------
-// Fluid properties (hydro2d.hpp:980-997, 1221-1246, 1308-1318).
-// Gradient components of the advected field with the zero-derivative
-// condition, the same operations as CellFaces<1> + GradFromFaces.
-__device__ __forceinline__ double AdvGradX(const Geo& g, const double* __restrict__ u, int i,
-                                           int j) {
-  const int c = i + g.nx * j;
-  const double uc = u[c];
-  const double fw = i > 0 ? Mid(u[c - 1], uc) : uc;
-  const double fe = i < g.nx - 1 ? Mid(uc, u[c + 1]) : uc;
-  double sxv = (-g.sx) * fw;
-  sxv += g.sx * fe;
-  return sxv / g.V;
-}
-__device__ __forceinline__ double AdvGradY(const Geo& g, const double* __restrict__ u, int i,
-                                           int j) {
-  const int c = i + g.nx * j;
-  const double uc = u[c];
-  const double fs = j > 0 ? Mid(u[c - g.nx], uc) : uc;
-  const double fn = j < g.ny - 1 ? Mid(uc, u[c + g.nx]) : uc;
-  double syv = (-g.sy) * fs;
-  syv += g.sy * fn;
-  return syv / g.V;
-}
-
-// One advection sub-step of NF fields sharing the volume flux
-// (advection.hpp:440-539): the cell gradients the Superbee faces need are
-// formed in place (KAdvGrad's values, bit for bit) instead of being written
-// to HBM and read back, and the flux is read once for all fields.
 struct AdvFields {
   const double* u[3];
   double* un[3];
 };
 
+// One advection sub-step (advection.hpp:440-539) of NF fields sharing the
+// volume flux.  The cell gradients with the zero-derivative condition
+// (CellFaces<1> + GradFromFaces) are formed in LDS instead of being written
+// to HBM and read back: the block loads u on its 64 x kAdvTY tile with a
+// two-cell halo, forms the x-gradients on the tile plus one column each side
+// and the y-gradients on the tile plus one row each side, then updates the
+// tile with the Superbee faces.  Every value is computed with the same
+// operations as the unfused form, so the result is bit for bit the same.
+constexpr int kAdvTX = 64, kAdvTY = 8;
+
 template <int NF>
 __global__ void __launch_bounds__(kBlock)
 KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
+  constexpr int UX = kAdvTX + 4, UY = kAdvTY + 4;
+  __shared__ double su[UY][UX];
+  __shared__ double sgx[kAdvTY][kAdvTX + 2];
+  __shared__ double sgy[kAdvTY + 2][kAdvTX];
   const int nx = g.nx, ny = g.ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nx * ny) return;
-  const int i = c % nx, j = c / nx;
-  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
-  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
-  const double vw = flux[fw], ve = flux[fe], vs = flux[fs], vn = flux[fn];
-#pragma unroll
+  const int ntx = (nx + kAdvTX - 1) / kAdvTX;
+  const int i0 = (blockIdx.x % ntx) * kAdvTX, j0 = (blockIdx.x / ntx) * kAdvTY;
   for (int q = 0; q < NF; ++q) {
     const double* __restrict__ u = fl.u[q];
-    const double uc = u[c];
-    const double gxc = AdvGradX(g, u, i, j), gyc = AdvGradY(g, u, i, j);
-    double uw = uc, ue = uc, us_ = uc, uN = uc;
-    if (i > 0) uw = SuperbeeFace(u[c - 1], uc, AdvGradX(g, u, i - 1, j), gxc, g.tx0, g.tx1, vw);
-    if (i < nx - 1) ue = SuperbeeFace(uc, u[c + 1], gxc, AdvGradX(g, u, i + 1, j), g.tx0, g.tx1, ve);
-    if (j > 0) us_ = SuperbeeFace(u[c - nx], uc, AdvGradY(g, u, i, j - 1), gyc, g.ty0, g.ty1, vs);
-    if (j < ny - 1) uN = SuperbeeFace(uc, u[c + nx], gyc, AdvGradY(g, u, i, j + 1), g.ty0, g.ty1, vn);
-    double fsum = 0.;
-    fsum += uw * vw * -1.;
-    fsum += ue * ve * 1.;
-    fsum += us_ * vs * -1.;
-    fsum += uN * vn * 1.;
-    fl.un[q][c] = uc + -dt / g.V * fsum;
+    if (q > 0) __syncthreads();  // the previous field's LDS is still being read
+    for (int t = threadIdx.x; t < UX * UY; t += kBlock) {
+      const int hj = t / UX, hi = t % UX;
+      const int ii = i0 + hi - 2, jj = j0 + hj - 2;
+      su[hj][hi] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? u[ii + nx * jj] : 0.;
+    }
+    __syncthreads();
+    // x-gradients on columns i0-1 .. i0+64 of the tile rows
+    for (int t = threadIdx.x; t < kAdvTY * (kAdvTX + 2); t += kBlock) {
+      const int lj = t / (kAdvTX + 2), hi = t % (kAdvTX + 2);
+      const int ii = i0 + hi - 1, jj = j0 + lj;
+      double gv = 0.;
+      if (ii >= 0 && ii < nx && jj < ny) {
+        const double uc = su[lj + 2][hi + 1];
+        const double fw = ii > 0 ? Mid(su[lj + 2][hi], uc) : uc;
+        const double fe = ii < nx - 1 ? Mid(uc, su[lj + 2][hi + 2]) : uc;
+        double sxv = (-g.sx) * fw;
+        sxv += g.sx * fe;
+        gv = sxv / g.V;
+      }
+      sgx[lj][hi] = gv;
+    }
+    // y-gradients on rows j0-1 .. j0+kAdvTY of the tile columns
+    for (int t = threadIdx.x; t < (kAdvTY + 2) * kAdvTX; t += kBlock) {
+      const int hj = t / kAdvTX, li = t % kAdvTX;
+      const int ii = i0 + li, jj = j0 + hj - 1;
+      double gv = 0.;
+      if (ii < nx && jj >= 0 && jj < ny) {
+        const double uc = su[hj + 1][li + 2];
+        const double fs = jj > 0 ? Mid(su[hj][li + 2], uc) : uc;
+        const double fn = jj < ny - 1 ? Mid(uc, su[hj + 2][li + 2]) : uc;
+        double syv = (-g.sy) * fs;
+        syv += g.sy * fn;
+        gv = syv / g.V;
+      }
+      sgy[hj][li] = gv;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kAdvTY * kAdvTX; t += kBlock) {
+      const int lj = t / kAdvTX, li = t % kAdvTX;
+      const int i = i0 + li, j = j0 + lj;
+      if (i >= nx || j >= ny) continue;
+      const int c = i + nx * j;
+      const double vw = flux[XFace(g, i, j)], ve = flux[XFace(g, i + 1, j)];
+      const double vs = flux[YFace(g, i, j)], vn = flux[YFace(g, i, j + 1)];
+      const double uc = su[lj + 2][li + 2];
+      const double gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
+      double uw = uc, ue = uc, us_ = uc, uN = uc;
+      if (i > 0) uw = SuperbeeFace(su[lj + 2][li + 1], uc, sgx[lj][li], gxc, g.tx0, g.tx1, vw);
+      if (i < nx - 1)
+        ue = SuperbeeFace(uc, su[lj + 2][li + 3], gxc, sgx[lj][li + 2], g.tx0, g.tx1, ve);
+      if (j > 0) us_ = SuperbeeFace(su[lj + 1][li + 2], uc, sgy[lj][li], gyc, g.ty0, g.ty1, vs);
+      if (j < ny - 1)
+        uN = SuperbeeFace(uc, su[lj + 3][li + 2], gyc, sgy[lj + 2][li], g.ty0, g.ty1, vn);
+      double fsum = 0.;
+      fsum += uw * vw * -1.;
+      fsum += ue * ve * 1.;
+      fsum += us_ * vs * -1.;
+      fsum += uN * vn * 1.;
+      fl.un[q][c] = uc + -dt / g.V * fsum;
+    }
   }
 }
 
@@ -797,13 +792,6 @@ void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double*
   long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
   KInitialFlux<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, mvx, mvy, flux);
 }
-void LaunchAdvGrad(hipStream_t st, const Geo& g, const double* u, double* gx, double* gy) {
-  KAdvGrad<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, gx, gy);
-}
-void LaunchAdvUpdate(hipStream_t st, const Geo& g, const double* u, const double* gx,
-                     const double* gy, const double* flux, double dt, double* un) {
-  KAdvUpdate<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, gx, gy, flux, dt, un);
-}
 void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
                    const double* flux, double dt, double* const un[3]) {
   AdvFields fl{};
@@ -811,7 +799,7 @@ void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3
     fl.u[q] = u[q];
     fl.un[q] = un[q];
   }
-  const dim3 grid = Grid(long(g.nx) * g.ny);
+  const unsigned grid = unsigned(((g.nx + kAdvTX - 1) / kAdvTX) * ((g.ny + kAdvTY - 1) / kAdvTY));
   if (nf == 1)
     KAdvStep<1><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
   else if (nf == 2)
