@@ -65,6 +65,7 @@ class CgSolver {
   void* scal_ = nullptr;     // device scalars (CgScalars)
   void* host_scal_ = nullptr;  // pinned mirror
   int blocks_;
+  int last_fcg_iters_ = 5;  // iterations of the last flexible solve
   hipEvent_t ev0_, ev1_;
 
   CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project);

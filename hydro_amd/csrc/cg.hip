@@ -618,7 +618,11 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
   double* pold = pa_;
   double* pnew = pb_;
   int k = 0;
-  const int batch = 4;
+  // iterations are queued in batches between host checks of the stop flag;
+  // the first batch is sized from the previous solve (solves of one
+  // simulation take similar counts), so few kernels are queued past
+  // convergence (they return at once) and the host waits once or twice
+  int batch = std::max(2, std::min(64, last_fcg_iters_ - 1));
   CgResult res;
   while (true) {
     for (int it = 0; it < batch; ++it, ++k) {
@@ -631,8 +635,10 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
     }
     ReadScalars();
     if (hs->done || k >= max_iters + batch) break;
+    batch = 2;
   }
   res.iterations = hs->iter;
+  last_fcg_iters_ = hs->fail ? 4 : std::max(2, hs->iter);
   if (hs->fail) {
     // the preconditioner broke the descent property: finish with
     // Jacobi-PCG from the current iterate (same FP64 tolerance; sc->bb

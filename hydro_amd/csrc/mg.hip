@@ -148,7 +148,7 @@ __device__ __forceinline__ float Interp(const float* __restrict__ e,
 }
 
 // Fused down-leg of one level: two damped Jacobi sweeps from zero
-// (x1 = omega D^-1 r, x2 = x1 + omega D^-1 (r - A x1)) and the restriction of
+// (x1 = w.a D^-1 r, x2 = x1 + w.b D^-1 (r - A x1)) and the restriction of
 // r - A x2 (sum of the four children).  A block of 256 threads works on a
 // 64 x TY tile with a two-cell halo in LDS (x1 on tile + 2, x2 on tile + 1),
 // so one launch and one pass over the level's arrays replace three; taller
@@ -156,7 +156,7 @@ __device__ __forceinline__ float Interp(const float* __restrict__ e,
 template <class RT, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
-      const float* __restrict__ dinv, const RT* __restrict__ r, float omega,
+      const float* __restrict__ dinv, const RT* __restrict__ r, Omega2 w,
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done) {
   if (Done(done)) return;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
@@ -172,7 +172,7 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
       const int c = ii + nx * jj;
       rv = float(r[c]);
-      xv = omega * dinv[c] * rv;
+      xv = w.a * dinv[c] * rv;
     }
     sr[hj][hi] = rv;
     s1[hj][hi] = xv;
@@ -192,7 +192,7 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
       if (ii < nx - 1) v -= ke * s1[a][e + 1];
       if (jj > 0) v -= ks * s1[a - 1][e];
       if (jj < ny - 1) v -= kn * s1[a + 1][e];
-      xv = s1[a][e] + omega * dinv[c] * (sr[a][e] - v);
+      xv = s1[a][e] + w.b * dinv[c] * (sr[a][e] - v);
     }
     s2[hj][hi] = xv;
   }
@@ -226,15 +226,15 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   }
 }
 
-// Fused up-leg of one level: y = x2 + P e, x3 = y + omega D^-1 (r - A y),
-// out = x3 + omega D^-1 (r - A x3), on the block's 64 x TY tile with a
+// Fused up-leg of one level: y = x2 + P e, x3 = y + w.a D^-1 (r - A y),
+// out = x3 + w.b D^-1 (r - A x3), on the block's 64 x TY tile with a
 // two-cell halo in LDS.
 template <class RT, bool BILINEAR, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
     const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
-    int nxc, int nyc, float omega, float* __restrict__ out, const int* done) {
+    int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done) {
   if (Done(done)) return;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
   __shared__ float sy[H2Y][H2X], sr[H1Y][H1X], s3[H1Y][H1X];
@@ -270,7 +270,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
       if (ii < nx - 1) v -= ke * sy[a][f + 1];
       if (jj > 0) v -= ks * sy[a - 1][f];
       if (jj < ny - 1) v -= kn * sy[a + 1][f];
-      xv = sy[a][f] + omega * dinv[c] * (sr[hj][hi] - v);
+      xv = sy[a][f] + w.a * dinv[c] * (sr[hj][hi] - v);
     }
     s3[hj][hi] = xv;
   }
@@ -288,7 +288,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     if (i < nx - 1) v -= ke * s3[a][f + 1];
     if (j > 0) v -= ks * s3[a - 1][f];
     if (j < ny - 1) v -= kn * s3[a + 1][f];
-    out[c] = s3[a][f] + omega * dinv[c] * (sr[a][f] - v);
+    out[c] = s3[a][f] + w.b * dinv[c] * (sr[a][f] - v);
   }
 }
 
@@ -350,7 +350,7 @@ __device__ __forceinline__ void BlockSync() {
 // cells, one cell per lane of wave 0) Jacobi-PCG to convergence.  Input: r of
 // the first level (global); output: its xb (global).
 __global__ void __launch_bounds__(1024)
-KCoarseCycle(CoarseSet cs, float omega, const int* done) {
+KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
   if (Done(done)) return;
   __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int T = blockDim.x, t = threadIdx.x;
@@ -363,11 +363,11 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
     const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
     const float* r = sr + cs.off[l];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
-    for (int c = t; c < n; c += T) xa[c] = omega * dinv[c] * r[c];
+    for (int c = t; c < n; c += T) xa[c] = w.a * dinv[c] * r[c];
     __syncthreads();
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
-      xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+      xb[c] = xa[c] + w.b * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
     }
     __syncthreads();
     const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
@@ -455,12 +455,12 @@ KCoarseCycle(CoarseSet cs, float omega, const int* done) {
       if (i < nx - 1) ay -= ke * Y(i + 1, j);
       if (j > 0) ay -= ks * Y(i, j - 1);
       if (j < ny - 1) ay -= kn * Y(i, j + 1);
-      xa[c] = yc + omega * dinv[c] * (r[c] - ay);
+      xa[c] = yc + w.a * dinv[c] * (r[c] - ay);
     }
     __syncthreads();
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
-      xb[c] = xa[c] + omega * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+      xb[c] = xa[c] + w.b * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
     }
     __syncthreads();
   }
@@ -554,7 +554,7 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     }
     // the caller prolongs from this level's xb
     (void)out;
-    KCoarseCycle<<<1, 1024, 0, st>>>(cs, omega, done);
+    KCoarseCycle<<<1, 1024, 0, st>>>(cs, Omega2{omega1, omega2}, done);
     return;
   }
   const bool coarsest = (l == int(lv_.size()) - 1);
@@ -598,15 +598,17 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
   // level 0 (FP64 residual) on 64 x 16 tiles, the FP32 levels on 64 x 4
   if (l == 0)
     KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, omega, L.xb, C.nx, C.ny, C.r, done);
+        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+        C.r, done);
   else
     KDown<float, kTY><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
-                                                       L.r, omega, L.xb, C.nx, C.ny, C.r, done);
+                                                       L.r, Omega2{omega1, omega2}, L.xb, C.nx,
+                                                       C.ny, C.r, done);
   Cycle(l + 1, nullptr, C.xb, st, done);
 #define HG_UP(RT, B, TY, TG, R)                                                               \
   KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
                                                       L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,    \
-                                                      omega, out, done)
+                                                      Omega2{omega1, omega2}, out, done)
   if (l == 0) {
     if (bilinear)
       HG_UP(double, true, kTallY, L.tall, r64);
@@ -633,11 +635,12 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
   auto launch = [&]() {
     if (k == "KDown<double>") {
       KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, omega, L.xb, C.nx, C.ny, C.r, nullptr);
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+          C.r, nullptr);
     } else if (k == "KUp<double, true>") {
       KUp<double, true, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny, omega,
-          L.xa, nullptr);
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,
+          Omega2{omega1, omega2}, L.xa, nullptr);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
       Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xb, st, nullptr);
     } else {
