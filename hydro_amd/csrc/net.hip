@@ -37,55 +37,96 @@ __device__ __forceinline__ bf F2B(float f) {
 }
 __device__ __forceinline__ float B2F(bf b) { return __uint_as_float(unsigned(b) << 16); }
 
+__device__ __forceinline__ void Unpack8(const uint4 u, float f[8]) {
+  const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = __uint_as_float(w[k] << 16);
+    f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ uint4 Pack8(const float f[8]) {
+  unsigned w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = unsigned(F2B(f[2 * k])) | (unsigned(F2B(f[2 * k + 1])) << 16);
+  return uint4{w[0], w[1], w[2], w[3]};
+}
+__device__ __forceinline__ uint4 Ld8(const bf* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ __forceinline__ void St8(bf* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+
+
 // ---------------------------------------------------------------------------
-// 3x3 'same' convolution, CIN, COUT multiples of 16, as an implicit GEMM:
-// M = 16 pixels of a row segment (one wave), N = COUT, K = 9*CIN in chunks
-// of (tap, 16 channels).  MFMA 16x16x16 BF16 lane layout: A[i][k] in lane
-// i + 16*(k/4), element k%4; B[k][j] in lane j + 16*(k/4); D[i][j] in lane
-// j + 16*(i/4), element i%4.  Weights wt[tap][co][ci] (BF16).  Used for
-// the forward pass and, with tap-flipped weights, for dgrad.
+// 3x3 'same' convolution, CIN, COUT multiples of 16, as an implicit GEMM on
+// MFMA 16x16x16 BF16 (FP32 accumulate).  Per (tap, 16 input channels) step a
+// wave multiplies a weight block (M = 16 output channels, K = 16 input
+// channels) by SEG input blocks (K x N = 16 pixels each): the weight
+// fragment is loaded once and used SEG times.  Operand lane layouts: A[i][k]
+// in lane i + 16*(k/4), element k%4 (weights wt[tap][co][ci], 8 contiguous
+// bytes per lane); B[k][j] in lane j + 16*(k/4) (pixel-major NHWC input, 8
+// contiguous bytes per lane); D[i][j] in lane j + 16*(i/4), element i%4, so
+// each lane ends with 4 consecutive output channels of one pixel and stores
+// them as one 8-byte word.  One wave covers 16*SEG pixels of a row.  Used
+// for the forward pass and, with tap-flipped weights, for dgrad.
+constexpr int kSeg = 4;
+
 template <int CIN, int COUT, bool RELU>
 __global__ void __launch_bounds__(256)
 KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
           bf* __restrict__ out, const int* done) {
   if (Done(done)) return;
+  constexpr int SEG = kSeg;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tiles_x = (W + 15) >> 4;
+  const int tiles_x = (W + 16 * SEG - 1) / (16 * SEG);
   const int tile = blockIdx.x * 4 + wave;
   const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
   if (ty >= H) return;
-  const int x0 = tx << 4;
-  const int pi = lane & 15, kq = lane >> 4;
-  f4 acc[COUT / 16];
+  const int x0 = tx * 16 * SEG;
+  const int pj = lane & 15, kq = lane >> 4;
+  f4 acc[SEG][COUT / 16];
 #pragma unroll
-  for (int n = 0; n < COUT / 16; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < SEG; ++s)
+#pragma unroll
+    for (int n = 0; n < COUT / 16; ++n) acc[s][n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
-    const int yy = ty + t / 3 - 1, xx = x0 + pi + t % 3 - 1;
-    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-    const bf* ip = in + (size_t(yy) * W + xx) * CIN + kq * 4;
+    const int yy = ty + t / 3 - 1;
+    const bool rowok = yy >= 0 && yy < H;
 #pragma unroll
     for (int cc = 0; cc < CIN / 16; ++cc) {
-      s4 a = ok ? *reinterpret_cast<const s4*>(ip + cc * 16) : s4{0, 0, 0, 0};
+      s4 bw[COUT / 16];
 #pragma unroll
-      for (int n = 0; n < COUT / 16; ++n) {
-        const s4 b = *reinterpret_cast<const s4*>(
-            wt + (size_t(t * COUT + n * 16 + pi) * CIN + cc * 16 + kq * 4));
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+      for (int n = 0; n < COUT / 16; ++n)
+        bw[n] = *reinterpret_cast<const s4*>(
+            wt + (size_t(t * COUT + n * 16 + pj) * CIN + cc * 16 + kq * 4));
+#pragma unroll
+      for (int s = 0; s < SEG; ++s) {
+        const int xx = x0 + 16 * s + pj + t % 3 - 1;
+        const bool ok = rowok && xx >= 0 && xx < W;
+        const s4 xin = ok ? *reinterpret_cast<const s4*>(in + (size_t(yy) * W + xx) * CIN +
+                                                         cc * 16 + kq * 4)
+                          : s4{0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < COUT / 16; ++n)
+          acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[n], xin, acc[s][n], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int n = 0; n < COUT / 16; ++n)
+  for (int s = 0; s < SEG; ++s) {
+    const int px = x0 + 16 * s + pj;
+    if (px >= W) continue;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int px = x0 + kq * 4 + v;
-      if (px < W) {
-        float r = acc[n][v];
+    for (int n = 0; n < COUT / 16; ++n) {
+      s4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float r = acc[s][n][v];
         if (RELU) r = fmaxf(r, 0.f);
-        out[(size_t(ty) * W + px) * COUT + n * 16 + pi] = F2B(r);
+        o[v] = short(F2B(r));
       }
+      *reinterpret_cast<s4*>(out + (size_t(ty) * W + px) * COUT + n * 16 + kq * 4) = o;
     }
+  }
 }
 
 // Weight gradient dW[tap][ci][co] += sum_p in[p + off(tap)][ci] g[p][co]:
@@ -165,8 +206,11 @@ KConvE0(int W, int H, const float* __restrict__ x, const float* __restrict__ w,
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] += v * w[t * 16 + c];
   }
+  if (relu)
 #pragma unroll
-  for (int c = 0; c < 16; ++c) out[size_t(p) * 16 + c] = F2B(relu ? fmaxf(acc[c], 0.f) : acc[c]);
+    for (int c = 0; c < 16; ++c) acc[c] = fmaxf(acc[c], 0.f);
+  St8(out + size_t(p) * 16, Pack8(acc));
+  St8(out + size_t(p) * 16 + 8, Pack8(acc + 8));
 }
 
 // D0: 16 -> 1 channel, FP32 output y = base + conv(in); w[tap][ci].  y may
@@ -184,8 +228,11 @@ KConvD0(int W, int H, const bf* __restrict__ in, const float* __restrict__ w,
     const int xx = px + t % 3 - 1, yy = py + t / 3 - 1;
     if (xx < 0 || xx >= W || yy < 0 || yy >= H) continue;
     const bf* q = in + (size_t(yy) * W + xx) * 16;
+    float f[16];
+    Unpack8(Ld8(q), f);
+    Unpack8(Ld8(q + 8), f + 8);
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc += B2F(q[c]) * w[t * 16 + c];
+    for (int c = 0; c < 16; ++c) acc += f[c] * w[t * 16 + c];
   }
   y[p] = base ? base[p] + acc : acc;
 }
@@ -208,8 +255,8 @@ KConvD0Dgrad(int W, int H, const float* __restrict__ gy, const float* __restrict
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] += g * w[t * 16 + c];
   }
-#pragma unroll
-  for (int c = 0; c < 16; ++c) gin[size_t(p) * 16 + c] = F2B(acc[c]);
+  St8(gin + size_t(p) * 16, Pack8(acc));
+  St8(gin + size_t(p) * 16 + 8, Pack8(acc + 8));
 }
 
 // Weight gradients of the 1-channel ends, reduced per block then atomics:
@@ -254,61 +301,96 @@ KEndWgrad(int W, int H, const float* __restrict__ xs, const bf* __restrict__ gb,
 }
 
 // ---- elementwise ops on BF16 NHWC tensors ----------------------------------
+// Channel counts are multiples of 16: every thread moves groups of 8 channels
+// (16 bytes).  Sums are formed in FP32 in a fixed order and rounded once.
 __global__ void KPool(int W, int H, int C, const bf* __restrict__ in, bf* __restrict__ out,
                       const int* done) {
   if (Done(done)) return;
-  const int Wc = W / 2, Hc = H / 2;
-  const long n = long(Wc) * Hc * C;
+  const int Wc = W / 2, Hc = H / 2, G = C / 8;
+  const long n = long(Wc) * Hc * G;
   for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += gridDim.x * 256L) {
-    const int c = int(q % C);
-    const long p = q / C;
+    const int c = int(q % G) * 8;
+    const long p = q / G;
     const int X = int(p % Wc), Y = int(p / Wc);
     const bf* b = in + (size_t(2 * Y) * W + 2 * X) * C + c;
-    const float s = B2F(b[0]) + B2F(b[C]) + B2F(b[size_t(W) * C]) + B2F(b[size_t(W) * C + C]);
-    out[q] = F2B(0.25f * s);
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    Unpack8(Ld8(b), f0);
+    Unpack8(Ld8(b + C), f1);
+    Unpack8(Ld8(b + size_t(W) * C), f2);
+    Unpack8(Ld8(b + size_t(W) * C + C), f3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = 0.25f * (f0[k] + f1[k] + f2[k] + f3[k]);
+    St8(out + p * C + c, Pack8(o));
   }
 }
 // out = up(d) + e (fine grid W x H, C channels)
 __global__ void KUpAdd(int W, int H, int C, const bf* __restrict__ d, const bf* __restrict__ e,
                        bf* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  const long n = long(W) * H * C;
-  const int Wc = W / 2;
+  const int Wc = W / 2, G = C / 8;
+  const long n = long(W) * H * G;
   for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += gridDim.x * 256L) {
-    const int c = int(q % C);
-    const long p = q / C;
+    const int c = int(q % G) * 8;
+    const long p = q / G;
     const int x = int(p % W), y = int(p / W);
-    out[q] = F2B(B2F(d[(size_t(y / 2) * Wc + x / 2) * C + c]) + B2F(e[q]));
+    float fd[8], fe[8], o[8];
+    Unpack8(Ld8(d + (size_t(y / 2) * Wc + x / 2) * C + c), fd);
+    Unpack8(Ld8(e + p * C + c), fe);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = fd[k] + fe[k];
+    St8(out + p * C + c, Pack8(o));
   }
 }
 // g_coarse = sum of the four children of g_fine
 __global__ void KUpBwd(int Wc, int Hc, int C, const bf* __restrict__ gf, bf* __restrict__ gc) {
-  const long n = long(Wc) * Hc * C;
-  const int W = 2 * Wc;
+  const int W = 2 * Wc, G = C / 8;
+  const long n = long(Wc) * Hc * G;
   for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += gridDim.x * 256L) {
-    const int c = int(q % C);
-    const long p = q / C;
+    const int c = int(q % G) * 8;
+    const long p = q / G;
     const int X = int(p % Wc), Y = int(p / Wc);
     const bf* b = gf + (size_t(2 * Y) * W + 2 * X) * C + c;
-    gc[q] = F2B(B2F(b[0]) + B2F(b[C]) + B2F(b[size_t(W) * C]) + B2F(b[size_t(W) * C + C]));
+    float f0[8], f1[8], f2[8], f3[8], o[8];
+    Unpack8(Ld8(b), f0);
+    Unpack8(Ld8(b + C), f1);
+    Unpack8(Ld8(b + size_t(W) * C), f2);
+    Unpack8(Ld8(b + size_t(W) * C + C), f3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = f0[k] + f1[k] + f2[k] + f3[k];
+    St8(gc + p * C + c, Pack8(o));
   }
 }
 // g_fine (+)= 0.25 * g_coarse at each child
 __global__ void KPoolBwdAdd(int W, int H, int C, const bf* __restrict__ gc, bf* __restrict__ gf) {
-  const long n = long(W) * H * C;
-  const int Wc = W / 2;
+  const int Wc = W / 2, G = C / 8;
+  const long n = long(W) * H * G;
   for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += gridDim.x * 256L) {
-    const int c = int(q % C);
-    const long p = q / C;
+    const int c = int(q % G) * 8;
+    const long p = q / G;
     const int x = int(p % W), y = int(p / W);
-    gf[q] = F2B(B2F(gf[q]) + 0.25f * B2F(gc[(size_t(y / 2) * Wc + x / 2) * C + c]));
+    float ff[8], fc[8], o[8];
+    Unpack8(Ld8(gf + p * C + c), ff);
+    Unpack8(Ld8(gc + (size_t(y / 2) * Wc + x / 2) * C + c), fc);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = ff[k] + 0.25f * fc[k];
+    St8(gf + p * C + c, Pack8(o));
   }
 }
-// out = g * (act > 0)
+// out = g * (act > 0); n is a multiple of 8
 __global__ void KMask(long n, const bf* __restrict__ g, const bf* __restrict__ act,
                       bf* __restrict__ out) {
-  for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += gridDim.x * 256L)
-    out[q] = B2F(act[q]) > 0.f ? g[q] : bf(0);
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < n / 8; q += gridDim.x * 256L) {
+    const uint4 gv = Ld8(g + q * 8), av = Ld8(act + q * 8);
+    const unsigned gw[4] = {gv.x, gv.y, gv.z, gv.w}, aw[4] = {av.x, av.y, av.z, av.w};
+    unsigned ow[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned lo = __uint_as_float(aw[k] << 16) > 0.f ? (gw[k] & 0xffffu) : 0u;
+      const unsigned hi = __uint_as_float(aw[k] & 0xffff0000u) > 0.f ? (gw[k] & 0xffff0000u) : 0u;
+      ow[k] = lo | hi;
+    }
+    St8(out + q * 8, uint4{ow[0], ow[1], ow[2], ow[3]});
+  }
 }
 
 // loss = mean((y - t)^2) (partials), gy = 2 (y - t) / N
@@ -487,7 +569,7 @@ inline unsigned Blk(long n) { return unsigned(std::min<long>((n + 255) / 256, 65
 // Dispatch of the MFMA conv for the (cin, cout) pairs of the network.
 void ConvFwd(int cin, int cout, bool relu, int W, int H, const bf* in, const bf* wt, bf* out,
              hipStream_t st, const int* done) {
-  const int tiles = ((W + 15) / 16) * H;
+  const int tiles = ((W + 16 * kSeg - 1) / (16 * kSeg)) * H;
   const unsigned grid = unsigned((tiles + 3) / 4);
 #define HG_CONV(CI, CO)                                                                  \
   if (cin == CI && cout == CO) {                                                         \
