@@ -57,40 +57,61 @@ __device__ __forceinline__ void St8(bf* p, uint4 v) { *reinterpret_cast<uint4*>(
 
 // ---------------------------------------------------------------------------
 // 3x3 'same' convolution, CIN, COUT multiples of 16, as an implicit GEMM on
-// MFMA 16x16x16 BF16 (FP32 accumulate).  Per (tap, 16 input channels) step a
-// wave multiplies a weight block (M = 16 output channels, K = 16 input
-// channels) by SEG input blocks (K x N = 16 pixels each): the weight
-// fragment is loaded once and used SEG times.  Operand lane layouts: A[i][k]
-// in lane i + 16*(k/4), element k%4 (weights wt[tap][co][ci], 8 contiguous
-// bytes per lane); B[k][j] in lane j + 16*(k/4) (pixel-major NHWC input, 8
-// contiguous bytes per lane); D[i][j] in lane j + 16*(i/4), element i%4, so
-// each lane ends with 4 consecutive output channels of one pixel and stores
-// them as one 8-byte word.  One wave covers 16*SEG pixels of a row.  Used
-// for the forward pass and, with tap-flipped weights, for dgrad.
-constexpr int kSeg = 4;
+// MFMA 16x16x16 BF16 (FP32 accumulate).  A block of 4 waves computes a
+// 64 x 4 pixel tile (one row per wave) for all output channels.  The input
+// tile with its one-pixel halo is staged once in LDS (16-byte global loads,
+// rows padded by 4 channels against bank conflicts); per (tap, 16 input
+// channels) step a wave multiplies each weight block (M = 16 output
+// channels, K = 16 input channels, read once and used kSeg times) by kSeg
+// input blocks from LDS (K x N = 16 pixels each).  Operand lane layouts:
+// A[i][k] in lane i + 16*(k/4), element k%4 (weights wt[tap][co][ci], 8
+// contiguous bytes per lane); B[k][j] in lane j + 16*(k/4); D[i][j] in lane
+// j + 16*(i/4), element i%4, so each lane ends with 4 consecutive output
+// channels of one pixel and stores them as one 8-byte word.  Used for the
+// forward pass and, with tap-flipped weights, for dgrad.
+constexpr int kSeg = 4;           // 16-pixel blocks per wave: 64-pixel rows
+constexpr int kCTX = 16 * kSeg;   // tile width
+constexpr int kCTY = 4;           // tile height (one row per wave)
 
 template <int CIN, int COUT, bool RELU>
 __global__ void __launch_bounds__(256)
 KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
           bf* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  constexpr int SEG = kSeg;
+  constexpr int CP = CIN + 4;  // padded channel stride in LDS
+  constexpr int HX = kCTX + 2, HY = kCTY + 2;
+  __shared__ __attribute__((aligned(16))) bf sin_[HY * HX * CP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tiles_x = (W + 16 * SEG - 1) / (16 * SEG);
-  const int tile = blockIdx.x * 4 + wave;
-  const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  const int tiles_x = (W + kCTX - 1) / kCTX;
+  const int bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  const int x0 = bx * kCTX, y0 = by * kCTY;
+  // stage the input tile + halo: 8 channels (16 bytes) per load
+  constexpr int G = CIN / 8;
+  for (int q = threadIdx.x; q < HX * HY * G; q += 256) {
+    const int g = q % G, pix = q / G;
+    const int hx = pix % HX, hy = pix / HX;
+    const int xx = x0 + hx - 1, yy = y0 + hy - 1;
+    uint4 v = uint4{0u, 0u, 0u, 0u};
+    if (xx >= 0 && xx < W && yy >= 0 && yy < H) v = Ld8(in + (size_t(yy) * W + xx) * CIN + g * 8);
+    // CP*2 bytes per pixel is a multiple of 8: two 8-byte stores
+    unsigned* dst = reinterpret_cast<unsigned*>(sin_ + pix * CP + g * 8);
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+    dst[3] = v.w;
+  }
+  __syncthreads();
+  const int ty = y0 + wave;
   if (ty >= H) return;
-  const int x0 = tx * 16 * SEG;
   const int pj = lane & 15, kq = lane >> 4;
-  f4 acc[SEG][COUT / 16];
+  f4 acc[kSeg][COUT / 16];
 #pragma unroll
-  for (int s = 0; s < SEG; ++s)
+  for (int s = 0; s < kSeg; ++s)
 #pragma unroll
     for (int n = 0; n < COUT / 16; ++n) acc[s][n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
-    const int yy = ty + t / 3 - 1;
-    const bool rowok = yy >= 0 && yy < H;
+    const int hy = wave + t / 3;
 #pragma unroll
     for (int cc = 0; cc < CIN / 16; ++cc) {
       s4 bw[COUT / 16];
@@ -99,12 +120,10 @@ KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
         bw[n] = *reinterpret_cast<const s4*>(
             wt + (size_t(t * COUT + n * 16 + pj) * CIN + cc * 16 + kq * 4));
 #pragma unroll
-      for (int s = 0; s < SEG; ++s) {
-        const int xx = x0 + 16 * s + pj + t % 3 - 1;
-        const bool ok = rowok && xx >= 0 && xx < W;
-        const s4 xin = ok ? *reinterpret_cast<const s4*>(in + (size_t(yy) * W + xx) * CIN +
-                                                         cc * 16 + kq * 4)
-                          : s4{0, 0, 0, 0};
+      for (int s = 0; s < kSeg; ++s) {
+        const int hx = 16 * s + pj + t % 3;
+        const s4 xin =
+            *reinterpret_cast<const s4*>(sin_ + (hy * HX + hx) * CP + cc * 16 + kq * 4);
 #pragma unroll
         for (int n = 0; n < COUT / 16; ++n)
           acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[n], xin, acc[s][n], 0, 0, 0);
@@ -112,7 +131,7 @@ KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
     }
   }
 #pragma unroll
-  for (int s = 0; s < SEG; ++s) {
+  for (int s = 0; s < kSeg; ++s) {
     const int px = x0 + 16 * s + pj;
     if (px >= W) continue;
 #pragma unroll
@@ -569,8 +588,7 @@ inline unsigned Blk(long n) { return unsigned(std::min<long>((n + 255) / 256, 65
 // Dispatch of the MFMA conv for the (cin, cout) pairs of the network.
 void ConvFwd(int cin, int cout, bool relu, int W, int H, const bf* in, const bf* wt, bf* out,
              hipStream_t st, const int* done) {
-  const int tiles = ((W + 16 * kSeg - 1) / (16 * kSeg)) * H;
-  const unsigned grid = unsigned((tiles + 3) / 4);
+  const unsigned grid = unsigned(((W + kCTX - 1) / kCTX) * ((H + kCTY - 1) / kCTY));
 #define HG_CONV(CI, CO)                                                                  \
   if (cin == CI && cout == CO) {                                                         \
     if (relu)                                                                            \
