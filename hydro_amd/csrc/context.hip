@@ -17,6 +17,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <thread>
+#include <atomic>
+#include <chrono>
 
 #include "../../include/hydro_gpu.h"
 #include "cg.h"
@@ -837,6 +840,34 @@ hg_status hg_train_pressure_net(hg_ctx* c, const hg_train_opts* opts, hg_train_s
       stats->val_ms_baseline = r.val_ms_baseline;
     }
   });
+}
+
+hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t threads,
+                       hg_step_stats* stats, double* wall_ms) {
+  if (!ctxs || n < 0 || steps < 0 || threads < 0) return HG_ERR_BAD_ARG;
+  for (int32_t i = 0; i < n; ++i)
+    if (!ctxs[i]) return HG_ERR_BAD_ARG;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<hg_status> rc(size_t(n), HG_OK);
+  std::atomic<int32_t> next{0};
+  auto worker = [&]() {
+    for (int32_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+      hg_step_stats last{};
+      for (int32_t s = 0; s < steps && rc[i] == HG_OK; ++s) rc[i] = hg_step(ctxs[i], &last);
+      if (stats) stats[i] = last;
+    }
+  };
+  const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(threads > 0 ? threads : n, n));
+  std::vector<std::thread> pool;
+  for (int32_t t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  if (wall_ms)
+    *wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                   .count();
+  for (int32_t i = 0; i < n; ++i)
+    if (rc[i] != HG_OK) return rc[i];
+  return HG_OK;
 }
 
 }  // extern "C"

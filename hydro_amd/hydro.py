@@ -299,3 +299,17 @@ class Net:
             self.close()
         except Exception:
             pass
+
+
+def batch_run(cases, steps, threads=0):
+    """Steps every Hydro in `cases` `steps` times concurrently (hg_batch_run);
+    returns (last step stats per case, wall ms)."""
+    lib = capi.load()
+    n = len(cases)
+    arr = (ctypes.c_void_p * n)(*[h.ctx for h in cases])
+    stats = (capi.HgStepStats * n)()
+    ms = ctypes.c_double()
+    rc = lib.hg_batch_run(ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), n, steps, threads,
+                          stats, ctypes.byref(ms))
+    _raise(lib, None, rc, "hg_batch_run")
+    return list(stats), ms.value

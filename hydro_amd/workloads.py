@@ -83,3 +83,31 @@ def poisson_problem(n=512, seed=0, ratio=1000.0):
     rhs = div.reshape(-1)
     rhs = rhs - rhs.mean()
     return kx.reshape(-1), ky.reshape(-1), rhs
+
+
+def c5_cases(count=64, seed=2026, solver_p="fcg_mg", solver_v="lu_rb", precision="mixed",
+             sides=tuple(range(64, 513, 32))):
+    """The batch workload: `count` independent cases, side n drawn from
+    `sides` (64, 96, ..., 512), either a lid-driven cavity on the unit square
+    (n x n) or a broken dam on the 2 x 1 box (n x n/2), physical parameters
+    jittered by +-10 %; all seeded.  Returns .hydroconf texts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice(sides))
+        j = lambda v: v * (1.0 + rng.uniform(-0.1, 0.1))  # noqa: E731
+        solvers = (f"set string linear_solver_pressure {solver_p}\n"
+                   f"set string linear_solver_velocity {solver_v}\n"
+                   f"set string precision_gpu {precision}\n")
+        if rng.uniform() < 0.5:
+            out.append(f"set int Nx {n}\nset int Ny {n}\nset double dt 1e-3\n"
+                       f"set double viscosity_0 {j(0.01):.6g}\n"
+                       f"set string condition_top \"wall {j(1.0):.6g} 0\"\n" + solvers)
+        else:
+            out.append(f"set vect A (0, 0)\nset vect B (2, 1)\nset int Nx {n}\nset int Ny {n // 2}\n"
+                       f"set int num_phases 2\nset double density_0 {j(1.0):.6g}\n"
+                       f"set double density_1 {j(1000.0):.6g}\n"
+                       f"set double viscosity_0 {j(1e-5):.6g}\nset double viscosity_1 {j(1e-3):.6g}\n"
+                       f"set vect gravity (0, {j(-9.81):.6g})\nset vect A1 (0, 0)\n"
+                       f"set vect B1 (0.5, 0.5)\nset double dt 1e-4\n" + solvers)
+    return out

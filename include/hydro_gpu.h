@@ -242,6 +242,19 @@ hg_status hg_step(hg_ctx* ctx, hg_step_stats* stats);
 hg_status hg_last_step_timing(const hg_ctx* ctx, double* fluid_ms, double* advection_ms,
                               double* properties_ms);
 
+/* ---- many cases at once (replaces the console's experiment threads,
+ * console.cpp:162-178 and :923, which run each experiment's step loop on its
+ * own std::thread, at most max_threads_number at a time) ---------------------
+ * Steps every context `steps` times.  Up to `threads` host threads (0: one
+ * per context) each take whole contexts from a shared queue; every context
+ * runs on its own HIP stream, so the cases overlap on the GPU.  Results are
+ * those of stepping each context alone (the contexts share nothing).
+ * stats[i] (may be NULL) receives the last step's stats of context i;
+ * *wall_ms the wall time of the whole batch.  Returns the first error met
+ * (by context order); the other contexts still run. */
+hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t threads,
+                       hg_step_stats* stats, double* wall_ms);
+
 /* ---- solve level (replaces LinearSolver<Scal,IdxCell,Expr>::Solve,
  * linear.hpp:302-310) ------------------------------------------------------
  * A 5-point system on an nx*ny grid: row r means
