@@ -100,6 +100,8 @@ struct hg_ctx {
   double *u_tc[2], *u_tp[2], *u_ic[2], *u_ip[2];
   double *p_tc, *p_tp, *p_ic, *p_ip;
   double *fl_tc, *fl_tp, *fl_ic, *fl_ip;
+  // buffer freed by the last FinishStep's rotation (time layers)
+  double *u_spare[2] = {nullptr, nullptr}, *p_spare = nullptr, *fl_spare = nullptr;
   // properties
   double *rho, *rho_s, *mu, *mu_s, *fx, *fy, *tmp1, *tmp2;
   double *pd[3], *pd_new[3], *alpha[3];
@@ -175,12 +177,14 @@ void Smooth(hg_ctx* c, const double* in, double* out, int times) {
   }
   const double* src = in;
   for (int r = 0; r < times; ++r) {
-    double* dst = (r == times - 1) ? c->tmp2 : ((r & 1) ? c->tmp2 : c->tmp1);
+    double* dst = (r & 1) ? c->tmp2 : c->tmp1;
+    if (r == times - 1 && out != src) dst = out;  // the last pass writes out directly
     if (dst == src) dst = (src == c->tmp1) ? c->tmp2 : c->tmp1;
     hg::LaunchSmooth(c->st, c->geo, src, dst);
     src = dst;
   }
-  HG_CHECK(hipMemcpyAsync(out, src, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+  if (src != out)
+    HG_CHECK(hipMemcpyAsync(out, src, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
 }
 
 void UpdateProperties(hg_ctx* c) {
@@ -232,6 +236,11 @@ void StartStep(hg_ctx* c) {
   c->cd_iter = 0;
   if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
     throw HgError(HG_ERR_NAN, "NaN initial pressure");
+  // iter_curr aliases time_curr after FinishStep: give it its own buffer
+  if (c->p_ic == c->p_tc) c->p_ic = c->p_spare;
+  if (c->fl_ic == c->fl_tc) c->fl_ic = c->fl_spare;
+  for (int d = 0; d < 2; ++d)
+    if (c->u_ic[d] == c->u_tc[d]) c->u_ic[d] = c->u_spare[d];
   const double ge = c->cfg.guess_extrapolation;
   for (int d = 0; d < 2; ++d)
     KExtrapolate<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->u_tc[d], c->u_tp[d], ge, c->u_ic[d]);
@@ -392,16 +401,20 @@ void Iterate(hg_ctx* c, hg_iter_stats* is) {
   m.bdf0 = c->bdf[0]; m.bdf1 = c->bdf[1]; m.bdf2 = c->bdf[2];
   m.relax = cfg.velocity_relaxation_factor;
   m.s = c->As; m.w = c->Aw; m.c = c->Ac; m.e = c->Ae; m.n = c->An; m.k0 = c->k0; m.k1 = c->k1;
+  m.ap = c->ap;  // the diagonal of fluid.hpp:876-883, from the same rows
   hg::LaunchMomentum(c->st, c->geo, c->bc, m);
   hg::Rows vrows{c->As, c->Aw, c->Ac, c->Ae, c->An};
   SolveVelocity(c, vrows, is);
   c->cd_iter += 1;
-  hg::LaunchDiag(c->st, c->geo, c->As, c->Aw, c->Ac, c->Ae, c->An, c->ap);
   hg::LaunchRhieChow(c->st, c->geo, c->bc, c->u_ic[0], c->u_ic[1], c->gpx, c->gpy, c->ffrx,
                      c->ffry, c->ffex, c->ffey, c->p_ip, c->ap, cfg.rhie_chow_factor,
                      cfg.meshvel[0], cfg.meshvel[1], c->fstar, c->fcoef);
-  hg::LaunchPressureRows(c->st, c->geo, c->fstar, c->fcoef, c->pinned, cfg.pressure_fixed_value,
-                         c->Ps, c->Pw, c->Pc, c->Pe, c->Pn, c->Pk);
+  const int32_t lsp = cfg.linear_solver_pressure;
+  if (lsp == HG_SOLVER_CG || lsp == HG_SOLVER_FCG_MG || lsp == HG_SOLVER_FCG_LEARNED)
+    hg::LaunchPressureRhs(c->st, c->geo, c->fstar, c->Pk);  // operator from the faces
+  else
+    hg::LaunchPressureRows(c->st, c->geo, c->fstar, c->fcoef, c->pinned,
+                           cfg.pressure_fixed_value, c->Ps, c->Pw, c->Pc, c->Pe, c->Pn, c->Pk);
   SolvePressure(c, is);
   hg::LaunchCorrectCells(c->st, c->geo, c->pcorr, c->ap, c->p_ip,
                          cfg.pressure_relaxation_factor, c->p_ic, c->u_ic[0], c->u_ic[1],
@@ -423,15 +436,20 @@ bool IsConverged(const hg_ctx* c) {
   return c->fluid_iter >= c->cfg.num_iterations_limit || ind < c->cfg.convergence_tolerance;
 }
 
+// Layer shift time_prev <- time_curr <- iter_curr (fluid.hpp:1159-1163) by
+// pointer rotation: iter_curr's buffer becomes time_curr, and iter_curr
+// keeps pointing at it (the two layers hold the same values until the next
+// StartStep, which gives iter_curr the freed time_prev buffer back).
+void Rotate(double*& tp, double*& tc, double*& ic, double*& spare) {
+  spare = tp;
+  tp = tc;
+  tc = ic;
+}
+
 void FinishStep(hg_ctx* c) {
-  std::swap(c->p_tp, c->p_tc);
-  HG_CHECK(hipMemcpyAsync(c->p_tc, c->p_ic, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
-  std::swap(c->fl_tp, c->fl_tc);
-  HG_CHECK(hipMemcpyAsync(c->fl_tc, c->fl_ic, c->nf * 8, hipMemcpyDeviceToDevice, c->st));
-  for (int d = 0; d < 2; ++d) {
-    std::swap(c->u_tp[d], c->u_tc[d]);
-    HG_CHECK(hipMemcpyAsync(c->u_tc[d], c->u_ic[d], c->nc * 8, hipMemcpyDeviceToDevice, c->st));
-  }
+  Rotate(c->p_tp, c->p_tc, c->p_ic, c->p_spare);
+  Rotate(c->fl_tp, c->fl_tc, c->fl_ic, c->fl_spare);
+  for (int d = 0; d < 2; ++d) Rotate(c->u_tp[d], c->u_tc[d], c->u_ic[d], c->u_spare[d]);
   c->t_fluid += c->dt;
   if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
 }

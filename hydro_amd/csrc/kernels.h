@@ -25,6 +25,7 @@ struct MomentumArgs {
   const double *ffmu, *flux, *rho, *utp0, *utp1, *utc0, *utc1;
   double bdf0, bdf1, bdf2, relax;
   double *s, *w, *c, *e, *n, *k0, *k1;
+  double* ap = nullptr;  // when set, also a_P of the relaxed rows (fluid.hpp:876-883)
 };
 
 // ---- step kernels (step_kernels.hip) ----
@@ -38,8 +39,6 @@ void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p
                      const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
                      double* g0y, double* g1x, double* g1y);
 void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a);
-void LaunchDiag(hipStream_t st, const Geo& g, const double* s, const double* w, const double* c,
-                const double* e, const double* n, double* ap);
 void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                     const double* u1, const double* gpx, const double* gpy, const double* ffrx,
                     const double* ffry, const double* ffex, const double* ffey, const double* p,
@@ -48,6 +47,8 @@ void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0
 void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         int pinned, double pval, double* s, double* w, double* c, double* e,
                         double* n, double* k);
+// k of the pressure rows only (no pinned cell).
+void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k);
 void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
                         const double* pprev, double alpha_p, double* pcurr, double* u0,
                         double* u1, const double* up0, const double* up1, double* part,

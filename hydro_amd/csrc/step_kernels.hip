@@ -123,6 +123,7 @@ struct MomIn {
 };
 struct MomOut {
   double *s, *w, *c, *e, *n, *k0, *k1;
+  double* ap;  // optional: a_P of the relaxed rows
 };
 
 // Face-normal gradient component of a cell gradient interpolated to a face
@@ -257,37 +258,32 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
   k1 += u1[c] * cen;
   if (he) { k0 += u0[ce] * nb[1]; k1 += u1[ce] * nb[1]; }
   if (hn) { k0 += u0[cn] * nb[3]; k1 += u1[cn] * nb[3]; }
-  out.s[c] = hs ? nb[2] : 0.;
-  out.w[c] = hw ? nb[0] : 0.;
-  out.c[c] = cen / in.relax;
-  out.e[c] = he ? nb[1] : 0.;
-  out.n[c] = hn ? nb[3] : 0.;
+  const double rs = hs ? nb[2] : 0., rw = hw ? nb[0] : 0., rc = cen / in.relax;
+  const double re = he ? nb[1] : 0., rn = hn ? nb[3] : 0.;
+  out.s[c] = rs;
+  out.w[c] = rw;
+  out.c[c] = rc;
+  out.e[c] = re;
+  out.n[c] = rn;
   out.k0[c] = k0;
   out.k1[c] = k1;
+  if (out.ap) {
+    // a_P = CoeffSum of the relaxed row, averaged over the two velocity
+    // components (identical rows): fluid.hpp:876-883
+    double cs = 0.;
+    if (hs) cs += rs;
+    if (hw) cs += rw;
+    cs += rc;
+    if (he) cs += re;
+    if (hn) cs += rn;
+    double sum = 0.;
+    sum += cs;
+    sum += cs;
+    out.ap[c] = sum / 2.;
+  }
 }
 
 // Diagonal coefficient a_P = mean of the rows' CoeffSum (fluid.hpp:876-884).
-__global__ void __launch_bounds__(kBlock)
-KDiag(Geo g, const double* __restrict__ s, const double* __restrict__ w,
-      const double* __restrict__ cc, const double* __restrict__ e, const double* __restrict__ n,
-      double* __restrict__ ap) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.nx * g.ny) return;
-  const int i = c % g.nx, j = c / g.nx;
-  double cs = 0.;
-  if (j > 0) cs += s[c];
-  if (i > 0) cs += w[c];
-  cs += cc[c];
-  if (i < g.nx - 1) cs += e[c];
-  if (j < g.ny - 1) cs += n[c];
-  double sum = 0.;
-  sum += cs;
-  sum += cs;
-  ap[c] = sum / 2.;
-}
-
-// Momentum interpolation (Rhie-Chow) and volume-flux rows
-// (fluid.hpp:886-970): flux* and the face coefficient -A/(d a_f).
 __global__ void __launch_bounds__(kBlock)
 KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict__ u1,
           const double* __restrict__ gpx, const double* __restrict__ gpy,
@@ -384,6 +380,24 @@ KPressureRows(Geo g, const double* __restrict__ fstar, const double* __restrict_
 // Correction (fluid.hpp:1033-1057) for cells, with the convergence
 // indicator max_c |u_curr - u_prev| (solver.hpp:804-813): per-block maxima,
 // folded by KMaxFold (max is order-independent, so the result is exact).
+// The constant of the pressure rows alone (KPressureRows' k, no pinned
+// cell): what the CG solvers need, which take the operator from the face
+// coefficients.
+__global__ void __launch_bounds__(kBlock)
+KPressureRhs(Geo g, const double* __restrict__ fstar, double* __restrict__ k) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  double kon = 0.;
+  kon += fstar[XFace(g, i, j)] * -1.;
+  kon += fstar[XFace(g, i + 1, j)] * 1.;
+  kon += fstar[YFace(g, i, j)] * -1.;
+  kon += fstar[YFace(g, i, j + 1)] * 1.;
+  kon += -(0. * g.V);
+  k[c] = kon;
+}
+
 __global__ void __launch_bounds__(kBlock)
 KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ ap,
               const double* __restrict__ pprev, double alpha_p, double* __restrict__ pcurr,
@@ -751,12 +765,8 @@ void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumAr
   MomIn in{a.u0,   a.u1,   a.g0x,  a.g0y,  a.g1x,  a.g1y,  a.gpx,  a.gpy,  a.frx, a.fry,
            a.ffmu, a.flux, a.rho,  a.utp0, a.utp1, a.utc0, a.utc1, a.bdf0, a.bdf1, a.bdf2,
            a.relax};
-  MomOut out{a.s, a.w, a.c, a.e, a.n, a.k0, a.k1};
+  MomOut out{a.s, a.w, a.c, a.e, a.n, a.k0, a.k1, a.ap};
   KMomentum<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, in, out);
-}
-void LaunchDiag(hipStream_t st, const Geo& g, const double* s, const double* w, const double* c,
-                const double* e, const double* n, double* ap) {
-  KDiag<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, s, w, c, e, n, ap);
 }
 void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                     const double* u1, const double* gpx, const double* gpy, const double* ffrx,
@@ -766,6 +776,9 @@ void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0
   long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
   KRhieChow<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap,
                                          rc, mvx, mvy, fstar, fcoef);
+}
+void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k) {
+  KPressureRhs<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, fstar, k);
 }
 void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         int pinned, double pval, double* s, double* w, double* c, double* e,

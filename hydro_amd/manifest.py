@@ -35,10 +35,12 @@ KERNELS = {
                          "read 2 cell fields; write 2 face fields"),
     "KCellGrads": (lambda nx, ny: _b(nx, ny, cells64=9),
                    "read p, u0, u1; write 3 gradients x 2 components"),
-    "KMomentum": (lambda nx, ny: _b(nx, ny, cells64=22, faces64=2),
+    "KMomentum": (lambda nx, ny: _b(nx, ny, cells64=23, faces64=2),
                   "read u0, u1, 4 velocity gradients, grad p (2), restored force (2), rho, "
-                  "u at t-1 and t (4), face mu, volume flux; write 5 row coefficients + 2 constants"),
-    "KDiag": (lambda nx, ny: _b(nx, ny, cells64=6), "read 5 row coefficients; write a_P"),
+                  "u at t-1 and t (4), face mu, volume flux; write 5 row coefficients, "
+                  "2 constants and a_P"),
+    "KPressureRhs": (lambda nx, ny: _b(nx, ny, cells64=1, faces64=1),
+                     "read flux*; write the pressure rows' constant"),
     "KLuRbBlack": (lambda nx, ny: _b(nx, ny, cells64=8),
                    "read 5 coefficients, 2 constants; write the black half of 2 iterates"),
     "KLuRbRed": (lambda nx, ny: _b(nx, ny, cells64=13),
