@@ -15,9 +15,12 @@ Two paths run on the same state:
             the V-cycle alone on a held-out solve (fcg_learned, mixed).
 `value` is the headline steps/s.  The pressure solve to 1e-8 is also timed
 on the last SIMPLE pressure system of the fp64 path for every solver.
-`roofline` is the dominant kernel of the headline step (the finest-level
-up-leg of the V-cycle), its average duration measured with HIP events on
-the solver's stream and its algorithmic bytes from hydro_amd/manifest.py.
+`roofline` is the kernel with the largest share of the headline step's time
+(launches per step x average duration, each duration measured with HIP
+events around back-to-back launches on the kernel's own stream), with its
+algorithmic bytes from hydro_amd/manifest.py and, where measured, its HBM
+traffic from rocprof counters.  `batch_c5` steps 64 independent cases at
+once.
 `cpu_baseline` times the reference module built from its sources
 (oracle/_ref/hydro_ref_fast, OpenMP) on one step of the same grid.
 
@@ -42,6 +45,12 @@ BF16_PEAK_TFLOPS = 2500.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on the RT 1024x2048 state, each scaled by what the counter reads
+# for a 256 MiB device-to-device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
+MEASURED_TRAFFIC = {"KUp<double, true>": 65.66e6, "KDown<double>": 54.07e6}
 
 
 def run_path(text, steps, warmup, label, train=False):
@@ -71,6 +80,10 @@ def run_path(text, steps, warmup, label, train=False):
         iters.append((st.simple_iterations, st.pressure_solve_iters_total))
     dt = time.perf_counter() - t0
     log(f"{label}: {steps} steps in {dt:.3f} s, (SIMPLE, pressure iters) per step {iters}")
+    if train:
+        # step kernels on the final state, HIP events around back-to-back launches
+        train_stats = dict(train_stats or {})
+        h.kernel_ms = {k: h.time_kernel(k, reps=20) for k in ("KAdvStep", "KMomentum")}
     return h, dt, iters, train_stats
 
 
@@ -125,6 +138,29 @@ def net_throughput(nx, ny):
             "forward_frac": round(fwd / BF16_PEAK_TFLOPS, 4)}
 
 
+def batch_c5(steps=3):
+    """64 independent cavity/dam cases (sides 64..512, seeded, parameters
+    jittered), FCG + V-cycle pressure, stepped concurrently from 8 host
+    threads (one HIP stream per case) and, for comparison, one after the
+    other."""
+    import re as _re
+    from hydro_amd import workloads
+    from hydro_amd.hydro import Hydro, batch_run
+    texts = workloads.c5_cases(count=64)
+    cells = sum(int(_re.search(r"Nx (\d+)", t).group(1)) * int(_re.search(r"Ny (\d+)", t).group(1))
+                for t in texts)
+    hs = [Hydro(t) for t in texts]
+    batch_run(hs, 1, threads=8)  # warm-up
+    _, ms_conc = batch_run(hs, steps, threads=8)
+    _, ms_seq = batch_run(hs, steps, threads=1)
+    for h in hs:
+        h.close()
+    return {"cases": 64, "cells": cells, "steps": steps,
+            "case_steps_per_s": round(64 * steps / (ms_conc * 1e-3), 1),
+            "cell_steps_per_s": round(cells * steps / (ms_conc * 1e-3), 0),
+            "ms_concurrent_8_threads": round(ms_conc, 2), "ms_one_after_another": round(ms_seq, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +187,8 @@ def main():
     rhs = h64.get(capi.HG_FIELD_PRESSURE_RHS)
     h64.close()
     hmx, tmx, itmx, train = run_path(textmx, args.steps, args.warmup, "headline", train=True)
+    step_kernel_ms = hmx.kernel_ms
+    substeps = 10
     hmx.close()
 
     nfx = (nx + 1) * ny
@@ -186,12 +224,28 @@ def main():
         "frac": round(cg_gbs / HBM_PEAK_GBS, 4)}
     log(f"kernels: {kernels}")
 
-    dom = "KUp<double, true>"
+    for name, man in (("KAdvStep", "KAdvStep<2>"), ("KMomentum", "KMomentum")):
+        ms = step_kernel_ms[name]
+        gbs = manifest.kernel_bytes(man, nx, ny) / (ms * 1e-3) / 1e9
+        kernels[man] = {"avg_us": round(ms * 1e3, 2), "gbs": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # the dominant kernel of the headline step: launches per step x duration
+    simple_per_step = float(np.mean([s_ for s_, _ in itmx]))
+    fcg_per_step = float(np.mean([p for _, p in itmx]))
+    per_step = {"KAdvStep<2>": substeps, "KMomentum": simple_per_step,
+                "KUp<double, true>": fcg_per_step, "KDown<double>": fcg_per_step}
+    dom = max(per_step, key=lambda k: per_step[k] * kernels[k]["avg_us"])
     achieved = kernels[dom]["gbs"]
+    traffic = MEASURED_TRAFFIC.get(dom)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"{dom}: finest-level up-leg of the V-cycle "
-                          f"({manifest.kernel_bytes(dom, nx, ny) / 1e6:.1f} MB per launch)"}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic else None,
+                "kernel": f"{dom}: {per_step[dom]:.1f} launches per step, "
+                          f"{manifest.kernel_bytes(dom, nx, ny) / 1e6:.1f} MB algorithmic per launch"}
+    log(f"dominant kernel {dom}: {roofline}")
+
+    batch = batch_c5()
+    log(f"batch: {batch}")
 
     net = net_throughput(nx, ny)
     log(f"net: {net}")
@@ -223,6 +277,7 @@ def main():
         "pressure_solve_ms": dict(solves, speedup_mixed_fcg_vs_fp64=round(
             solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2)),
         "kernels": kernels,
+        "batch_c5": batch,
         "net": net,
         "roofline": roofline,
         "cpu_baseline": cpu,

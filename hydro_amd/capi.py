@@ -157,7 +157,7 @@ EXPORTS = [
     "hg_set_field", "hg_get_field", "hg_refresh",
     "hg_fluid_start_step", "hg_fluid_iterate", "hg_fluid_is_converged",
     "hg_fluid_finish_step", "hg_advect_substep", "hg_update_properties",
-    "hg_calc_stats", "hg_step", "hg_last_step_timing", "hg_batch_run",
+    "hg_calc_stats", "hg_step", "hg_last_step_timing", "hg_time_kernel", "hg_batch_run",
     "hg_stencil_solve", "hg_poisson_create", "hg_poisson_destroy",
     "hg_poisson_solve", "hg_poisson_time_kernel",
     "hg_train_opts_default", "hg_poisson_train", "hg_train_pressure_net",
@@ -196,6 +196,7 @@ def _declare(lib):
         "hg_calc_stats": (ctypes.c_int, [vp, _P(HgStats)]),
         "hg_step": (ctypes.c_int, [vp, _P(HgStepStats)]),
         "hg_last_step_timing": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+        "hg_time_kernel": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int32, _dp]),
         "hg_batch_run": (ctypes.c_int, [_P(vp), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         _P(HgStepStats), _dp]),
         "hg_stencil_solve": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32] + [_dp] * 7 +

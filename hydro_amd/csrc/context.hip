@@ -888,4 +888,46 @@ hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t th
   return HG_OK;
 }
 
+hg_status hg_time_kernel(hg_ctx* c, const char* kernel, int32_t reps, double* avg_ms) {
+  if (!c || !kernel || reps < 1 || !avg_ms) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    const std::string k(kernel);
+    std::function<void()> launch;
+    if (k == "KAdvStep") {
+      // reads the phase fields and the flux, writes the scratch layer pd_new
+      const double* u[3] = {c->pd[0], c->np > 1 ? c->pd[1] : nullptr,
+                            c->np > 2 ? c->pd[2] : nullptr};
+      double* un[3] = {c->pd_new[0], c->np > 1 ? c->pd_new[1] : nullptr,
+                       c->np > 2 ? c->pd_new[2] : nullptr};
+      launch = [=]() { hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un); };
+    } else if (k == "KMomentum") {
+      // the rows of the last SIMPLE iteration's inputs, rewritten in place
+      hg::MomentumArgs m{};
+      m.u0 = c->u_ip[0];
+      m.u1 = c->u_ip[1];
+      m.g0x = c->g0x; m.g0y = c->g0y; m.g1x = c->g1x; m.g1y = c->g1y;
+      m.gpx = c->gpx; m.gpy = c->gpy; m.frx = c->frx; m.fry = c->fry;
+      m.ffmu = c->ffmu;
+      m.flux = c->fl_ip;
+      m.rho = c->rho_s;
+      m.utp0 = c->u_tp[0]; m.utp1 = c->u_tp[1]; m.utc0 = c->u_tc[0]; m.utc1 = c->u_tc[1];
+      m.bdf0 = c->bdf[0]; m.bdf1 = c->bdf[1]; m.bdf2 = c->bdf[2];
+      m.relax = c->cfg.velocity_relaxation_factor;
+      m.s = c->As; m.w = c->Aw; m.c = c->Ac; m.e = c->Ae; m.n = c->An; m.k0 = c->k0;
+      m.k1 = c->k1; m.ap = c->ap;
+      launch = [=]() { hg::LaunchMomentum(c->st, c->geo, c->bc, m); };
+    } else {
+      throw HgError(HG_ERR_BAD_ARG, "hg_time_kernel: unknown kernel " + k);
+    }
+    launch();
+    HG_CHECK(hipEventRecord(c->ev[0], c->st));
+    for (int32_t i = 0; i < reps; ++i) launch();
+    HG_CHECK(hipEventRecord(c->ev[1], c->st));
+    HG_CHECK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HG_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    *avg_ms = double(ms) / reps;
+  });
+}
+
 }  // extern "C"

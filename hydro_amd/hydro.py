@@ -137,6 +137,13 @@ class Hydro:
         _raise(self.lib, self.ctx, rc, "hg_train_pressure_net")
         return st
 
+    def time_kernel(self, kernel, reps=20):
+        """Average device ms of one launch of a step kernel on the current state."""
+        ms = ctypes.c_double()
+        rc = self.lib.hg_time_kernel(self.ctx, kernel.encode(), reps, ctypes.byref(ms))
+        _raise(self.lib, self.ctx, rc, "hg_time_kernel")
+        return ms.value
+
     def timing(self):
         f, a, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self.lib.hg_last_step_timing(self.ctx, ctypes.byref(f), ctypes.byref(a), ctypes.byref(p))

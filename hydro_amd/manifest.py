@@ -57,6 +57,8 @@ KERNELS = {
     "KNegateFaces": (lambda nx, ny: _b(nx, ny, faces64=2), "read flux coefficient; write k_f"),
     "KNeg": (lambda nx, ny: _b(nx, ny, cells64=2), "read constant; write right-hand side"),
     # ---- advection and properties ----
+    "KAdvStep<2>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
+                    "two phase fields: read both fields and the volume flux; write both fields"),
     "KAdvGrad": (lambda nx, ny: _b(nx, ny, cells64=3), "read field; write gradient (2)"),
     "KAdvUpdate": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                    "read field, gradient (2), volume flux; write field"),

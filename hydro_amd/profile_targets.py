@@ -1,0 +1,76 @@
+"""Short, fixed workloads to run under rocprofv3 (kernel traces and counters).
+
+    python -m hydro_amd.profile_targets step      # RT 1024x2048 headline steps
+    python -m hydro_amd.profile_targets kernels   # a 256 MiB device copy (counter
+                                                  # calibration), then the step's
+                                                  # dominant kernels
+    python -m hydro_amd.profile_targets net       # the BF16 MFMA network alone
+
+Each prints what it ran; none writes files.
+"""
+import ctypes
+import sys
+import time
+
+
+def _rt(solver_p="fcg_mg"):
+    from hydro_amd import workloads
+    from hydro_amd.hydro import Hydro
+    from tests import cases
+    h = Hydro(cases.rt(1024, 2048, solver_p=solver_p, solver_v="lu_rb", precision="mixed"))
+    workloads.set_rt_state(h)
+    return h
+
+
+def step(n=5):
+    h = _rt()
+    h.step()
+    t0 = time.perf_counter()
+    for i in range(n):
+        st = h.step()
+        print(f"step {i}: SIMPLE {st.simple_iterations}, pressure iterations "
+              f"{st.pressure_solve_iters_total}, {h.timing()}", flush=True)
+    print(f"ms/step {1e3 * (time.perf_counter() - t0) / n:.3f}", flush=True)
+    h.close()
+
+
+def kernels(reps=5):
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("calibration copy: 3 x 256 MiB", flush=True)
+    from hydro_amd.hydro import Poisson
+    h = _rt()
+    h.step()
+    for k in ("KAdvStep", "KMomentum"):
+        print(f"{k}: {h.time_kernel(k, reps=reps) * 1e3:.2f} us", flush=True)
+    kf, rhs = h.get(16), h.get(17)
+    h.close()
+    nx, ny = 1024, 2048
+    ps = Poisson(nx, ny, kf[:(nx + 1) * ny], kf[(nx + 1) * ny:])
+    ps.solve(rhs, solver="fcg_mg", tol=1e-8, precision=1)
+    for k in ("KUp<double, true>", "KDown<double>", "KCoarseCycle"):
+        print(f"{k}: {ps.time_kernel(k, reps=reps) * 1e3:.2f} us", flush=True)
+    ps.close()
+
+
+def net(reps=5):
+    from hydro_amd.hydro import Net
+    n = Net(1024, 2048)
+    print(n.time(reps=reps), flush=True)
+    n.close()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "step"
+    {"step": step, "kernels": kernels, "net": net}[what]()

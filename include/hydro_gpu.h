@@ -242,6 +242,14 @@ hg_status hg_step(hg_ctx* ctx, hg_step_stats* stats);
 hg_status hg_last_step_timing(const hg_ctx* ctx, double* fluid_ms, double* advection_ms,
                               double* properties_ms);
 
+/* Average device time (ms) of one launch of a step kernel on this context's
+ * current state, launched `reps` times back to back between two HIP events
+ * on its stream (no reference counterpart; for throughput figures).  The
+ * launches write scratch outputs only, so the state is unchanged.
+ * kernel: "KAdvStep" (one advection sub-step of all phase fields),
+ * "KMomentum" (momentum rows of one SIMPLE iteration). */
+hg_status hg_time_kernel(hg_ctx* ctx, const char* kernel, int32_t reps, double* avg_ms);
+
 /* ---- many cases at once (replaces the console's experiment threads,
  * console.cpp:162-178 and :923, which run each experiment's step loop on its
  * own std::thread, at most max_threads_number at a time) ---------------------
