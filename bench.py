@@ -50,7 +50,8 @@ def log(msg):
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on the RT 1024x2048 state, each scaled by what the counter reads
 # for a 256 MiB device-to-device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
-MEASURED_TRAFFIC = {"KUp<double, true>": 65.66e6, "KDown<double>": 54.07e6}
+MEASURED_TRAFFIC = {"KUp<double, true>": 65.65e6, "KDown<double>": 54.11e6,
+                    "KAdvStep<2>": 134.86e6, "KMomentum": 562.25e6}
 
 
 def run_path(text, steps, warmup, label, train=False):
@@ -244,6 +245,14 @@ def main():
                           f"{manifest.kernel_bytes(dom, nx, ny) / 1e6:.1f} MB algorithmic per launch"}
     log(f"dominant kernel {dom}: {roofline}")
 
+    # the whole headline step against the HBM roofline: the algorithmic bytes
+    # of all its kernels (launch counts of the timed steps) over its time
+    step_bytes = manifest.headline_step_bytes(nx, ny, simple_per_step, fcg_per_step)
+    step_gbs = step_bytes / (tmx / args.steps) / 1e9
+    step_roofline = {"algorithmic_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+    log(f"step roofline: {step_roofline}")
+
     batch = batch_c5()
     log(f"batch: {batch}")
 
@@ -276,6 +285,7 @@ def main():
         "learned_correction": train,
         "pressure_solve_ms": dict(solves, speedup_mixed_fcg_vs_fp64=round(
             solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2)),
+        "step_roofline": step_roofline,
         "kernels": kernels,
         "batch_c5": batch,
         "net": net,

@@ -37,12 +37,9 @@ struct Omega2 {
 
 constexpr int kMaxInKernelLevels = 12;
 constexpr int kLdsCells = 2752;  // LDS cells per vector for all in-kernel levels
-constexpr int kLdsFaces = 3072;  // LDS x- (or y-) faces for all in-kernel levels
 struct CoarseSet {
   int count, bilinear;
-  int off[kMaxInKernelLevels];   // offset of each level in the LDS cell vectors
-  int offx[kMaxInKernelLevels];  // ... in the LDS x-face coefficients
-  int offy[kMaxInKernelLevels];  // ... in the LDS y-face coefficients
+  int off[kMaxInKernelLevels];  // offset of each level in the LDS vectors
   int nx[kMaxInKernelLevels], ny[kMaxInKernelLevels];
   const float *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
   float *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];

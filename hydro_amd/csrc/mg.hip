@@ -353,23 +353,14 @@ __global__ void __launch_bounds__(1024)
 KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
   if (Done(done)) return;
   __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
-  // the coefficients of every level, copied once: every phase below then
-  // reads LDS only
-  __shared__ float skx[kLdsFaces], sky[kLdsFaces], sdinv[kLdsCells];
   const int T = blockDim.x, t = threadIdx.x;
   const int L = cs.count;
   const int n0 = cs.nx[0] * cs.ny[0];
   for (int c = t; c < n0; c += T) sr[c] = cs.r[0][c];
-  for (int l = 0; l < L; ++l) {
-    const int nx = cs.nx[l], ny = cs.ny[l];
-    for (int f = t; f < (nx + 1) * ny; f += T) skx[cs.offx[l] + f] = cs.kx[l][f];
-    for (int f = t; f < nx * (ny + 1); f += T) sky[cs.offy[l] + f] = cs.ky[l][f];
-    for (int c = t; c < nx * ny; c += T) sdinv[cs.off[l] + c] = cs.dinv[l][c];
-  }
   __syncthreads();
   for (int l = 0; l < L - 1; ++l) {
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
-    const float *kx = skx + cs.offx[l], *ky = sky + cs.offy[l], *dinv = sdinv + cs.off[l];
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
     const float* r = sr + cs.off[l];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
     for (int c = t; c < n; c += T) xa[c] = w.a * dinv[c] * r[c];
@@ -395,7 +386,7 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
   }
   if (t < 64) {
     const int nx = cs.nx[L - 1], ny = cs.ny[L - 1], n = nx * ny;
-    const float *kx = skx + cs.offx[L - 1], *ky = sky + cs.offy[L - 1];
+    const float *kx = cs.kx[L - 1], *ky = cs.ky[L - 1];
     const float* r = sr + cs.off[L - 1];
     const int lane = t;
     const bool act = lane < n;
@@ -406,7 +397,7 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
       ke = kx[i + 1 + (nx + 1) * j];
       ks = ky[i + nx * j];
       kn = ky[i + nx * (j + 1)];
-      di = sdinv[cs.off[L - 1] + lane];
+      di = cs.dinv[L - 1][lane];
       b = r[lane];
     }
     const float dg = ((kw + ke) + ks) + kn;
@@ -444,10 +435,10 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
   for (int l = L - 2; l >= 0; --l) {
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
     const int nxc = cs.nx[l + 1], nyc = cs.ny[l + 1];
-    const float *kx = skx + cs.offx[l], *ky = sky + cs.offy[l], *dinv = sdinv + cs.off[l];
+    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
     const float* r = sr + cs.off[l];
     const float* e = sxb + cs.off[l + 1];
-    const float *kxc = skx + cs.offx[l + 1], *kyc = sky + cs.offy[l + 1];
+    const float *kxc = cs.kx[l + 1], *kyc = cs.ky[l + 1];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
     for (int c = t; c < n; c += T) {
       const int i = c % nx, j = c / nx;
@@ -515,15 +506,9 @@ MgPrecond::MgPrecond(int nx, int ny) {
   if (first_in_kernel_ > 0 && int(lv_.size()) - first_in_kernel_ > kMaxInKernelLevels)
     first_in_kernel_ = int(lv_.size()) - kMaxInKernelLevels;
   if (first_in_kernel_ > 0) {
-    long cells = 0, fx = 0, fy = 0;
-    for (int l = first_in_kernel_; l < int(lv_.size()); ++l) {
-      cells += long(lv_[l].nx) * lv_[l].ny;
-      fx += long(lv_[l].nx + 1) * lv_[l].ny;
-      fy += long(lv_[l].nx) * (lv_[l].ny + 1);
-    }
-    // (2048 cells * 4/3 < kLdsCells; the face counts exceed the cells by
-    // one row or column per level, which only degenerate strips overflow)
-    if (cells > kLdsCells || fx > kLdsFaces || fy > kLdsFaces) first_in_kernel_ = -1;
+    long cells = 0;
+    for (int l = first_in_kernel_; l < int(lv_.size()); ++l) cells += long(lv_[l].nx) * lv_[l].ny;
+    if (cells > kLdsCells) first_in_kernel_ = -1;  // cannot happen: 2048 * 4/3 < kLdsCells
   }
 }
 
@@ -553,15 +538,11 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     CoarseSet cs{};
     cs.count = int(lv_.size()) - l;
     cs.bilinear = bilinear ? 1 : 0;
-    int off = 0, offx = 0, offy = 0;
+    int off = 0;
     for (int k = 0; k < cs.count; ++k) {
       const MgLevel& C = lv_[l + k];
       cs.off[k] = off;
-      cs.offx[k] = offx;
-      cs.offy[k] = offy;
       off += C.nx * C.ny;
-      offx += (C.nx + 1) * C.ny;
-      offy += C.nx * (C.ny + 1);
       cs.nx[k] = C.nx;
       cs.ny[k] = C.ny;
       cs.kx[k] = C.kx;

@@ -97,3 +97,42 @@ def cg_iteration_bytes(nx, ny):
     """FP64 Jacobi-PCG iteration: SpMV + update (16 words per cell counting
     k_x, k_y as one word each)."""
     return kernel_bytes("KPcgSpmv<double>", nx, ny) + kernel_bytes("KPcgUpdate<double>", nx, ny)
+
+
+def _vcycle_bytes(nx, ny):
+    """One V-cycle: the finest level's fused kernels plus the FP32 levels
+    below it (each level quarters the cells; down-leg reads r, D^-1, k and
+    writes x2 and the coarse residual; up-leg reads x2, r, D^-1, k and the
+    coarse correction and writes the output)."""
+    b = kernel_bytes("KDown<double>", nx, ny) + kernel_bytes("KUp<double, true>", nx, ny)
+    cx, cy = nx // 2, ny // 2
+    while cx * cy > 2048 and cx % 2 == 0 and cy % 2 == 0:
+        b += _b(cx, cy, cells32=3, faces32=1, coarse32=0.25)      # down
+        b += _b(cx, cy, cells32=4, faces32=1.25, coarse32=0.25)   # up
+        cx, cy = cx // 2, cy // 2
+    return b
+
+
+def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
+    """Algorithmic bytes of one step of the headline path (two phases,
+    lu_rb velocity, FCG + V-cycle pressure in mixed precision): the sum over
+    the step's kernels, launches per step times bytes per launch."""
+    n, f, _, _ = sizes(nx, ny)
+    w = F64
+    start = w * (3 * n + 9 * n + 3 * f) \
+        + kernel_bytes("KForceFaces", nx, ny) + kernel_bytes("KRestoredForce", nx, ny) \
+        + kernel_bytes("KFacesDeriv0Pair", nx, ny)                 # NaN check, guesses, faces
+    per_iter = sum(kernel_bytes(k, nx, ny) for k in (
+        "KCellGrads", "KMomentum", "KLuRbBlack", "KLuRbRed", "KRhieChow", "KPressureRhs",
+        "KNegateFaces", "KNeg", "KCorrectCells", "KCorrectFaces"))
+    # operator set-up: FP64 face copy, FP32 faces and D^-1 for CG and for the
+    # multigrid's finest level
+    per_iter += w * (2 * f) + (f * w + f * F32 + n * F32) * 2
+    # FCG start (projection, residual) and final true residual
+    per_iter += w * (4 * n) + 2 * w * (3 * n + f)
+    per_fcg = sum(kernel_bytes(k, nx, ny) for k in ("KFcgDir", "KFcgSpmv", "KFcgUpdate")) \
+        + _vcycle_bytes(nx, ny)
+    finish = w * 3 * n                                             # NaN check
+    advection = substeps * kernel_bytes("KAdvStep<2>", nx, ny)
+    properties = kernel_bytes("KProperties", nx, ny) + 4 * kernel_bytes("KSmooth", nx, ny)
+    return start + simple_iters * per_iter + fcg_iters * per_fcg + finish + advection + properties
