@@ -23,6 +23,8 @@ struct CgResult {
   bool converged = false;
   double ms = 0.;
   bool used_fallback = false;  // learned preconditioner rejected
+  double tol = 0.;             // the solve's tolerance (for a deferred check)
+  bool residual_pending = false;  // rel_residual and converged not read back yet
 };
 
 class CgSolver {
@@ -32,15 +34,27 @@ class CgSolver {
   CgSolver(const CgSolver&) = delete;
   CgSolver& operator=(const CgSolver&) = delete;
 
-  // Face coefficients already in device memory (FP64), copied internally;
-  // also builds the FP32 copies and the inverse diagonal.
-  void SetOperator(const double* d_kx, const double* d_ky);
+  // Face coefficients already in device memory (FP64), multiplied by `sign`
+  // (+1 or -1: exact) on the way in.  One pass writes the FP64 operator, its
+  // FP32 copy and the FP64 and FP32 inverse diagonals.  The inputs must not
+  // alias this solver's own arrays.
+  void SetOperator(const double* d_kx, const double* d_ky, double sign = 1.);
+  // The FP32 face coefficients and inverse diagonal, rewritten by every
+  // SetOperator; the multigrid's finest level reads them in place.
+  float* kx32() const { return kx32_; }
+  float* ky32() const { return ky32_; }
+  float* dinv32() const { return dinv32_; }
   // Solves A x = b (device pointers, FP64) from x = 0.
   // precision: 0 FP64 Jacobi-PCG; 1 FP32 inner PCG under FP64 refinement.
   // With `pc` set: flexible CG with FP64 Krylov vectors preconditioned by
   // pc (FP32 output); falls back to Jacobi-PCG when pc breaks descent.
+  // With defer_final the closing true-residual check and the solve timer are
+  // queued without a wait: `iterations` is final at return, rel_residual,
+  // converged and ms come from FinishDeferred once the caller has
+  // synchronised the stream (before the next Solve).
   CgResult Solve(const double* d_b, double* d_x, double tol, int max_iters, int precision,
-                 bool project, Precond* pc = nullptr);
+                 bool project, Precond* pc = nullptr, bool defer_final = false);
+  void FinishDeferred(CgResult& r);
   // y = A x (FP64), for tests and residual checks.
   void Apply(const double* d_x, double* d_y);
   const double* kx() const { return kx_; }
@@ -77,6 +91,9 @@ class CgSolver {
   int JacobiPcg64(double* x, int max_iters);
   int InnerFp32(double tol_inner, int max_iters, Precond* pc, bool* pc_failed);
   double ResidualFp64(const double* b, const double* x, double* r);  // returns ||r||^2
+  // r = b - A x, (r, r) into the device scalars, scalars copied to the host
+  // mirror: queued, no wait.
+  void QueueFinalResidual(const double* b, const double* x, double* r);
   double Dot64(const double* a, const double* b);
   void ReadScalars();
 };

@@ -238,15 +238,16 @@ KFcgCheck(int k, Parts pt, CgScalars* sc) {
 }
 
 // (z, q_old), (r, z).
+// Iteration 0 has no previous direction: (z, q_old) = 0 without reading q_old.
 __global__ void __launch_bounds__(kB)
-KFcgDir(Walk w, const float* __restrict__ z, const double* __restrict__ qold,
+KFcgDir(Walk w, int k, const float* __restrict__ z, const double* __restrict__ qold,
         const double* __restrict__ r, Parts pt, CgScalars* sc) {
   if (sc->done) return;
   double zq = 0., rz = 0.;
   ForTiles(w, [&](int i, int j) {
     const int c = i + w.nx * j;
     const double zv = double(z[c]);
-    zq += zv * qold[c];
+    if (k > 0) zq += zv * qold[c];
     rz += r[c] * zv;
   });
   double v[2] = {zq, rz};
@@ -254,6 +255,7 @@ KFcgDir(Walk w, const float* __restrict__ z, const double* __restrict__ qold,
 }
 
 // p = z + beta p_old with beta = -(z, q_old) / (p_old, q_old); q = A p.
+// Iteration 0: p = z (p_old is not read).
 __global__ void __launch_bounds__(kB)
 KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict__ ky,
          const float* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
@@ -274,7 +276,7 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
   ForTiles(w, [&](int i, int j) {
     auto P = [&](int ii, int jj) {
       const int cc = ii + w.nx * jj;
-      return double(z[cc]) + beta * pold[cc];
+      return k > 0 ? double(z[cc]) + beta * pold[cc] : double(z[cc]);
     };
     const double pc = P(i, j);
     const double qv = ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
@@ -350,11 +352,30 @@ KResidual(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
   PutPartials<1>(v, pt.misc);
 }
 
-// Folds misc into sc->rr (and sc->bb when set_bb); also copies it to the
-// update slot 1 (r, r) so iteration 0 of FCG sees it.
+// Start of a solve from x = 0: b_ = b - mean(b) (mean folded from the misc
+// partials) or b_ = b; r = b_; x = 0; partial (r, r) into update slot 1, the
+// (r, r) iteration 0 of FCG reads.
 __global__ void __launch_bounds__(kB)
-KTakeRR(Parts pt, CgScalars* sc, int set_bb) {
-  const double rr = Fold(pt.misc, pt.nb);
+KStart(Walk w, const double* __restrict__ b, int project, double* __restrict__ bp,
+       double* __restrict__ r, double* __restrict__ x, Parts pt) {
+  const double mean = project ? Fold(pt.misc, pt.nb) / (double(w.nx) * double(w.ny)) : 0.;
+  double acc = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    const double v = project ? b[c] - mean : b[c];
+    bp[c] = v;
+    r[c] = v;
+    x[c] = 0.;
+    acc += v * v;
+  });
+  double v[1] = {acc};
+  PutPartials<1>(v, pt.upd[1] + pt.nb);
+}
+
+// Folds `nb` partials into sc->rr (and sc->bb when set_bb).
+__global__ void __launch_bounds__(kB)
+KTakeRR(const double* part, int nb, CgScalars* sc, int set_bb) {
+  const double rr = Fold(part, nb);
   if (threadIdx.x == 0) {
     sc->rr = rr;
     if (set_bb) sc->bb = rr;
@@ -370,27 +391,35 @@ KApply(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
   });
 }
 
+// The operator from input faces times `sign`: FP64 and FP32 faces, FP64 and
+// FP32 inverse diagonal.  Each cell writes its west and south faces, and the
+// east / north face on the last column / row.
 __global__ void __launch_bounds__(kB)
-KDinv(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
-      double* __restrict__ dinv, float* __restrict__ dinv32) {
+KSetOperator(Walk w, const double* __restrict__ ikx, const double* __restrict__ iky, double sign,
+             double* __restrict__ kx, double* __restrict__ ky, float* __restrict__ kx32,
+             float* __restrict__ ky32, double* __restrict__ dinv, float* __restrict__ dinv32) {
   ForTiles(w, [&](int i, int j) {
     const int nx = w.nx;
-    const double d = ((kx[i + (nx + 1) * j] + kx[i + 1 + (nx + 1) * j]) + ky[i + nx * j]) +
-                     ky[i + nx * (j + 1)];
+    const int fw = i + (nx + 1) * j, fs = i + nx * j;
+    const double kw = sign * ikx[fw], ke = sign * ikx[fw + 1];
+    const double ks = sign * iky[fs], kn = sign * iky[fs + nx];
+    kx[fw] = kw;
+    kx32[fw] = float(kw);
+    ky[fs] = ks;
+    ky32[fs] = float(ks);
+    if (i == nx - 1) {
+      kx[fw + 1] = ke;
+      kx32[fw + 1] = float(ke);
+    }
+    if (j == w.ny - 1) {
+      ky[fs + nx] = kn;
+      ky32[fs + nx] = float(kn);
+    }
+    const double d = ((kw + ke) + ks) + kn;
     const double di = d > 0. ? 1. / d : 0.;
     dinv[i + nx * j] = di;
     dinv32[i + nx * j] = float(di);
   });
-}
-
-__global__ void KCastFaces(long n, const double* __restrict__ a, float* __restrict__ b) {
-  long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
-  if (f < n) b[f] = float(a[f]);
-}
-
-__global__ void __launch_bounds__(kB)
-KZero(Walk w, double* __restrict__ a) {
-  ForTiles(w, [&](int i, int j) { a[i + w.nx * j] = 0.; });
 }
 
 __global__ void __launch_bounds__(kB)
@@ -494,15 +523,11 @@ static Parts MakeParts(double* base, int nb) {
   return p;
 }
 
-void CgSolver::SetOperator(const double* d_kx, const double* d_ky) {
-  long nfx = long(nx_ + 1) * ny_, nfy = long(nx_) * (ny_ + 1);
+void CgSolver::SetOperator(const double* d_kx, const double* d_ky, double sign) {
   int nb;
   Walk w = MakeWalk(nx_, ny_, &nb);
-  HG_CHECK(hipMemcpyAsync(kx_, d_kx, nfx * 8, hipMemcpyDeviceToDevice, stream_));
-  HG_CHECK(hipMemcpyAsync(ky_, d_ky, nfy * 8, hipMemcpyDeviceToDevice, stream_));
-  KDinv<<<nb, kB, 0, stream_>>>(w, kx_, ky_, dinv_, dinv32_);
-  KCastFaces<<<unsigned((nfx + 255) / 256), 256, 0, stream_>>>(nfx, kx_, kx32_);
-  KCastFaces<<<unsigned((nfy + 255) / 256), 256, 0, stream_>>>(nfy, ky_, ky32_);
+  KSetOperator<<<nb, kB, 0, stream_>>>(w, d_kx, d_ky, sign, kx_, ky_, kx32_, ky32_, dinv_,
+                                       dinv32_);
 }
 
 void CgSolver::ReadScalars() {
@@ -516,18 +541,35 @@ void CgSolver::Apply(const double* x, double* y) {
   KApply<<<nb, kB, 0, stream_>>>(w, kx_, ky_, x, y);
 }
 
-double CgSolver::ResidualFp64(const double* b, const double* x, double* r) {
+void CgSolver::QueueFinalResidual(const double* b, const double* x, double* r) {
   int nb;
   Walk w = MakeWalk(nx_, ny_, &nb);
   Parts pt = MakeParts(part_, nb);
   KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b, x, r, pt);
-  KTakeRR<<<1, kB, 0, stream_>>>(pt, (CgScalars*)scal_, 0);
-  ReadScalars();
+  KTakeRR<<<1, kB, 0, stream_>>>(pt.misc, nb, (CgScalars*)scal_, 0);
+  HG_CHECK(hipMemcpyAsync(host_scal_, scal_, sizeof(CgScalars), hipMemcpyDeviceToHost, stream_));
+}
+
+double CgSolver::ResidualFp64(const double* b, const double* x, double* r) {
+  QueueFinalResidual(b, x, r);
+  HG_CHECK(hipStreamSynchronize(stream_));
   return ((CgScalars*)host_scal_)->rr;
 }
 
+void CgSolver::FinishDeferred(CgResult& res) {
+  if (res.residual_pending) {
+    const CgScalars* hs = (const CgScalars*)host_scal_;
+    res.rel_residual = hs->bb > 0 ? std::sqrt(hs->rr / hs->bb) : 0.;
+    res.converged = res.rel_residual <= res.tol;
+    res.residual_pending = false;
+  }
+  float ms = 0.f;
+  HG_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  res.ms = ms;
+}
+
 CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, int precision,
-                         bool project, Precond* pc) {
+                         bool project, Precond* pc, bool defer_final) {
   HG_CHECK(hipEventRecord(ev0_, stream_));
   CgResult res;
   if (pc)
@@ -537,31 +579,22 @@ CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, 
   else
     res = SolveMixed(b, x, tol, max_iters, project, nullptr);
   HG_CHECK(hipEventRecord(ev1_, stream_));
+  if (defer_final) return res;
   HG_CHECK(hipEventSynchronize(ev1_));
-  float ms = 0.f;
-  HG_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  res.ms = ms;
+  FinishDeferred(res);
   return res;
 }
 
-// b_ <- b (projected to zero mean when singular); x <- 0; r_ <- b_;
-// sc->bb = ||b_||^2; (r, r) also placed in update slot 1.
+// b_ <- b (projected to zero mean when singular); x <- 0; r_ <- b_ (the
+// residual of x = 0); sc->bb = ||b_||^2, its partials in update slot 1.
 void CgSolver::Prepare(const double* b, double* x, bool project) {
   int nb;
   Walk w = MakeWalk(nx_, ny_, &nb);
   Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
-  if (project) {
-    KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
-    KProject<<<nb, kB, 0, stream_>>>(w, b, b_, pt);
-  } else {
-    HG_CHECK(hipMemcpyAsync(b_, b, n_ * 8, hipMemcpyDeviceToDevice, stream_));
-  }
-  KZero<<<nb, kB, 0, stream_>>>(w, x);
-  KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b_, x, r_, pt);
-  KTakeRR<<<1, kB, 0, stream_>>>(pt, sc, 1);
-  HG_CHECK(hipMemcpyAsync(pt.upd[1] + nb, pt.misc, size_t(nb) * 8, hipMemcpyDeviceToDevice,
-                          stream_));
+  if (project) KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
+  KStart<<<nb, kB, 0, stream_>>>(w, b, project ? 1 : 0, b_, r_, x, pt);
+  KTakeRR<<<1, kB, 0, stream_>>>(pt.upd[1] + nb, nb, sc, 1);
 }
 
 // Jacobi-PCG on A x = b_ continuing from the current x and r_ (FP64).
@@ -591,15 +624,13 @@ int CgSolver::JacobiPcg64(double* x, int max_iters) {
 CgResult CgSolver::SolveFp64(const double* b, double* x, double tol, int max_iters,
                              bool project) {
   CgScalars* sc = (CgScalars*)scal_;
-  CgScalars* hs = (CgScalars*)host_scal_;
   KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
   Prepare(b, x, project);
   CgResult res;
   res.iterations = JacobiPcg64(x, max_iters);
-  double bb = hs->bb;
-  double rr = ResidualFp64(b_, x, r_);
-  res.rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.;
-  res.converged = res.rel_residual <= tol;
+  QueueFinalResidual(b_, x, r_);
+  res.tol = tol;
+  res.residual_pending = true;
   res.outer = 1;
   return res;
 }
@@ -613,8 +644,6 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
   CgScalars* hs = (CgScalars*)host_scal_;
   KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
   Prepare(b, x, project);
-  HG_CHECK(hipMemsetAsync(q_, 0, n_ * 8, stream_));
-  HG_CHECK(hipMemsetAsync(pa_, 0, n_ * 8, stream_));
   double* pold = pa_;
   double* pnew = pb_;
   int k = 0;
@@ -628,7 +657,7 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
     for (int it = 0; it < batch; ++it, ++k) {
       KFcgCheck<<<1, kB, 0, stream_>>>(k, pt, sc);
       pc->Apply(r_, z32_, stream_, &sc->done);
-      KFcgDir<<<nb, kB, 0, stream_>>>(w, z32_, q_, r_, pt, sc);
+      KFcgDir<<<nb, kB, 0, stream_>>>(w, k, z32_, q_, r_, pt, sc);
       KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pt, sc);
       KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pt, sc);
       std::swap(pold, pnew);
@@ -649,10 +678,9 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
     KResidual<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b_, x, r_, pt);
     res.iterations += JacobiPcg64(x, left);
   }
-  double bb = hs->bb;
-  double rr = ResidualFp64(b_, x, r_);
-  res.rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.;
-  res.converged = res.rel_residual <= tol;
+  QueueFinalResidual(b_, x, r_);
+  res.tol = tol;
+  res.residual_pending = true;
   res.outer = 1;
   return res;
 }

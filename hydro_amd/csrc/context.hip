@@ -68,6 +68,12 @@ __global__ void KExtrapolate(long n, const double* __restrict__ tc, const double
   long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
   if (i < n) out[i] = tc[i] + (tc[i] - tp[i]) * ge;
 }
+
+// Pressure solvers that take the operator in face form (CgSolver).
+inline bool FaceSolver(int32_t s) {
+  return s == HG_SOLVER_CG || s == HG_SOLVER_FCG_MG || s == HG_SOLVER_FCG_LEARNED;
+}
+
 __global__ void KNegateFaces(long nf, const double* __restrict__ fcoef, double* __restrict__ k) {
   long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
   if (f < nf) k[f] = -fcoef[f];
@@ -157,7 +163,34 @@ struct hg_ctx {
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
-  void Sync() { HG_CHECK(hipStreamSynchronize(st)); }
+  // Readbacks that wait for the next synchronisation of the stream instead
+  // of one of their own (hg_step queues them): the pressure solve's closing
+  // residual check and timer, the convergence indicator.  Sync() settles
+  // them into last_iter / last_rs.
+  bool pending_cg = false, pending_rs = false;
+  hg::CgResult cg_res;
+  // p and u of time_curr were checked for NaN by FinishStep and not changed
+  // since (StartStep need not check them again)
+  bool fields_checked = false;
+  void Sync() {
+    HG_CHECK(hipStreamSynchronize(st));
+    Settle();
+  }
+  void Settle() {
+    if (pending_cg) {
+      pending_cg = false;
+      cg->FinishDeferred(cg_res);
+      last_iter.pressure_solve_diff = cg_res.rel_residual;
+      last_iter.pressure_converged = cg_res.converged ? 1 : 0;
+    }
+    if (pending_rs) {
+      pending_rs = false;
+      double rs;
+      std::memcpy(&rs, rs_h, 8);
+      last_rs = rs;
+      last_iter.convergence_indicator = rs;
+    }
+  }
 };
 
 namespace {
@@ -234,8 +267,9 @@ bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
 void StartStep(hg_ctx* c) {
   c->fluid_iter = 0;
   c->cd_iter = 0;
-  if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
+  if (!c->fields_checked && NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
     throw HgError(HG_ERR_NAN, "NaN initial pressure");
+  c->fields_checked = false;
   // iter_curr aliases time_curr after FinishStep: give it its own buffer
   if (c->p_ic == c->p_tc) c->p_ic = c->p_spare;
   if (c->fl_ic == c->fl_tc) c->fl_ic = c->fl_spare;
@@ -354,23 +388,23 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
     case HG_SOLVER_FCG_LEARNED: {
       if (c->pinned >= 0)
         throw HgError(HG_ERR_UNSUPPORTED, "cg with pressure_fixed_point is not supported");
-      KNegateFaces<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fcoef, c->kface);
-      KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
-      c->cg->SetOperator(c->kface, c->kface + c->geo.nfx);
+      // operator k_f = -fcoef (negated on the way in); the right-hand side
+      // pb = -(row constant) was written by the pressure right-hand side kernel
+      c->cg->SetOperator(c->fcoef, c->fcoef + c->geo.nfx, -1.);
       hg::Precond* pc = nullptr;
       if (cfg.linear_solver_pressure != HG_SOLVER_CG) {
-        c->mg->Build(c->kface, c->kface + c->geo.nfx, c->st);
+        c->mg->Build(c->st);
         const bool learned = cfg.linear_solver_pressure == HG_SOLVER_FCG_LEARNED && c->net &&
                              c->net->Trained();
         pc = learned ? static_cast<hg::Precond*>(c->net.get())
                      : static_cast<hg::Precond*>(c->mg.get());
       }
       c->pressure_system = true;
-      hg::CgResult r = c->cg->Solve(c->pb, c->pcorr, cfg.pressure_tolerance_gpu,
-                                    cfg.pressure_max_iters_gpu, cfg.precision_gpu, true, pc);
-      is->pressure_solve_iters = r.iterations;
-      is->pressure_solve_diff = r.rel_residual;
-      is->pressure_converged = r.converged ? 1 : 0;
+      // the closing residual check is read back at the next synchronisation
+      c->cg_res = c->cg->Solve(c->pb, c->pcorr, cfg.pressure_tolerance_gpu,
+                               cfg.pressure_max_iters_gpu, cfg.precision_gpu, true, pc, true);
+      c->pending_cg = true;
+      is->pressure_solve_iters = c->cg_res.iterations;
       break;
     }
     default:
@@ -378,7 +412,12 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
   }
 }
 
-void Iterate(hg_ctx* c, hg_iter_stats* is) {
+// One SIMPLE iteration; its statistics are built in c->last_iter and copied
+// to *out.  With defer_rs the convergence indicator (and the pressure
+// solve's closing check) are left for the next Sync(): for callers whose
+// loop ends on the iteration limit without looking at the indicator.
+void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
+  hg_iter_stats* is = &c->last_iter;
   std::memset(is, 0, sizeof(*is));
   const hg_config& cfg = c->cfg;
   // iter_prev <- iter_curr (fluid.hpp:815-818, conv_diff.hpp:132-134); the
@@ -409,9 +448,8 @@ void Iterate(hg_ctx* c, hg_iter_stats* is) {
   hg::LaunchRhieChow(c->st, c->geo, c->bc, c->u_ic[0], c->u_ic[1], c->gpx, c->gpy, c->ffrx,
                      c->ffry, c->ffex, c->ffey, c->p_ip, c->ap, cfg.rhie_chow_factor,
                      cfg.meshvel[0], cfg.meshvel[1], c->fstar, c->fcoef);
-  const int32_t lsp = cfg.linear_solver_pressure;
-  if (lsp == HG_SOLVER_CG || lsp == HG_SOLVER_FCG_MG || lsp == HG_SOLVER_FCG_LEARNED)
-    hg::LaunchPressureRhs(c->st, c->geo, c->fstar, c->Pk);  // operator from the faces
+  if (FaceSolver(cfg.linear_solver_pressure))
+    hg::LaunchPressureRhs(c->st, c->geo, c->fstar, c->pb, true);  // operator from the faces
   else
     hg::LaunchPressureRows(c->st, c->geo, c->fstar, c->fcoef, c->pinned,
                            cfg.pressure_fixed_value, c->Ps, c->Pw, c->Pc, c->Pe, c->Pn, c->Pk);
@@ -421,14 +459,11 @@ void Iterate(hg_ctx* c, hg_iter_stats* is) {
                          c->u_ip[0], c->u_ip[1], c->rs_part, (double*)c->rs_d);
   hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic);
   HG_CHECK(hipMemcpyAsync(c->rs_h, c->rs_d, 8, hipMemcpyDeviceToHost, c->st));
-  c->Sync();
-  double rs;
-  std::memcpy(&rs, c->rs_h, 8);
-  c->last_rs = rs;
+  c->pending_rs = true;
   c->fluid_iter += 1;
   is->iteration = c->fluid_iter;
-  is->convergence_indicator = rs;
-  c->last_iter = *is;
+  if (!defer_rs) c->Sync();  // settles the indicator into last_rs / last_iter
+  *out = *is;
 }
 
 bool IsConverged(const hg_ctx* c) {
@@ -452,6 +487,7 @@ void FinishStep(hg_ctx* c) {
   for (int d = 0; d < 2; ++d) Rotate(c->u_tp[d], c->u_tc[d], c->u_ic[d], c->u_spare[d]);
   c->t_fluid += c->dt;
   if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
+  c->fields_checked = true;
 }
 
 void AdvectSubstep(hg_ctx* c) {
@@ -470,10 +506,16 @@ hg_status Guard(hg_ctx* ctx, const std::function<void()>& fn) {
     fn();
     return HG_OK;
   } catch (const HgError& e) {
-    if (ctx) ctx->err = e.what();
+    if (ctx) {
+      ctx->err = e.what();
+      ctx->pending_cg = ctx->pending_rs = false;  // a failed call settles nothing
+    }
     return e.code;
   } catch (const std::exception& e) {
-    if (ctx) ctx->err = e.what();
+    if (ctx) {
+      ctx->err = e.what();
+      ctx->pending_cg = ctx->pending_rs = false;
+    }
     return HG_ERR_HIP;
   }
 }
@@ -574,7 +616,8 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     if (lsp == HG_SOLVER_CG || lsp == HG_SOLVER_FCG_LEARNED || lsp == HG_SOLVER_FCG_MG)
       c->cg.reset(new hg::CgSolver(cfg.nx, cfg.ny, c->st));
     if (lsp == HG_SOLVER_FCG_LEARNED || lsp == HG_SOLVER_FCG_MG)
-      c->mg.reset(new hg::MgPrecond(cfg.nx, cfg.ny));
+      c->mg.reset(new hg::MgPrecond(cfg.nx, cfg.ny, c->cg->kx32(), c->cg->ky32(),
+                                    c->cg->dinv32()));
     Upload(c.get(), c->xn_d, c->mesh.xn);
     Upload(c.get(), c->yn_d, c->mesh.yn);
 
@@ -692,7 +735,9 @@ static double* FieldPtr(hg_ctx* c, int f, bool write, long* n) {
       KNegateFaces<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fcoef, c->kface);
       return c->kface;
     case HG_FIELD_PRESSURE_RHS:
-      KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
+      // the face-form solvers' right-hand side is written negated already
+      if (!FaceSolver(c->cfg.linear_solver_pressure))
+        KNeg<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->Pk, c->pb);
       return c->pb;
     default: return nullptr;
   }
@@ -707,6 +752,7 @@ hg_status hg_set_field(hg_ctx* c, int32_t field, const double* host, size_t n) {
     return HG_ERR_BAD_ARG;
   }
   return Guard(c, [&]() {
+    c->fields_checked = false;
     HG_CHECK(hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->st));
     double* prev = nullptr;
     if (field == HG_FIELD_VELOCITY_X) prev = c->u_tp[0];
@@ -735,6 +781,7 @@ hg_status hg_get_field(hg_ctx* c, int32_t field, double* host, size_t n) {
 hg_status hg_refresh(hg_ctx* c) {
   if (!c) return HG_ERR_BAD_ARG;
   return Guard(c, [&]() {
+    c->fields_checked = false;
     InitialFlux(c);
     UpdateProperties(c);
     c->Sync();
@@ -786,12 +833,15 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     hg_step_stats st{};
     HG_CHECK(hipEventRecord(c->ev[0], c->st));
     StartStep(c);
+    bool iterated = false;
     if (c->cfg.fluid_enable) {
       while (!IsConverged(c)) {
         hg_iter_stats is;
-        Iterate(c, &is);
+        // when the iteration limit ends the loop, the indicator is not
+        // needed to decide and is read back at a later synchronisation
+        Iterate(c, &is, c->fluid_iter + 1 >= c->cfg.num_iterations_limit);
         st.pressure_solve_iters_total += is.pressure_solve_iters;
-        st.convergence_indicator = is.convergence_indicator;
+        iterated = true;
       }
     }
     st.simple_iterations = c->fluid_iter;
@@ -814,6 +864,7 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     c->timing[0] = a;
     c->timing[1] = b;
     c->timing[2] = d;
+    if (iterated) st.convergence_indicator = c->last_iter.convergence_indicator;
     st.time = c->t_fluid;
     if (s) *s = st;
   });

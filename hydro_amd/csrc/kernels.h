@@ -47,8 +47,9 @@ void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0
 void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         int pinned, double pval, double* s, double* w, double* c, double* e,
                         double* n, double* k);
-// k of the pressure rows only (no pinned cell).
-void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k);
+// k of the pressure rows only (no pinned cell); -k with `negate`.
+void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k,
+                       bool negate = false);
 void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const double* ap,
                         const double* pprev, double alpha_p, double* pcurr, double* u0,
                         double* u1, const double* up0, const double* up1, double* part,

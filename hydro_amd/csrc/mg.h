@@ -47,12 +47,15 @@ struct CoarseSet {
 
 class MgPrecond : public Precond {
  public:
-  MgPrecond(int nx, int ny);
+  // kx0, ky0, dinv0: the finest level's FP32 face coefficients and inverse
+  // diagonal, owned by the caller (CgSolver::kx32() ...) and rewritten by it
+  // before each Build.
+  MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0);
   ~MgPrecond();
   MgPrecond(const MgPrecond&) = delete;
   MgPrecond& operator=(const MgPrecond&) = delete;
-  // FP64 face coefficients of the finest level (device pointers).
-  void Build(const double* kx, const double* ky, hipStream_t st);
+  // Coarse-level coefficients and inverse diagonals from the finest level's.
+  void Build(hipStream_t st);
   // z = V(r): r is the FP64 residual of the finest level; z FP32.  Does
   // nothing when *d_done is set.
   void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;

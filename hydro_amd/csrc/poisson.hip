@@ -184,8 +184,8 @@ hg_status hg_poisson_create(int32_t nx, int32_t ny, const double* kx, const doub
     HG_CHECK(hipMemcpyAsync(dkx.p, kx, nfx * 8, hipMemcpyHostToDevice, ps->st));
     HG_CHECK(hipMemcpyAsync(dky.p, ky, nfy * 8, hipMemcpyHostToDevice, ps->st));
     ps->cg->SetOperator(dkx.p, dky.p);
-    ps->mg.reset(new hg::MgPrecond(nx, ny));
-    ps->mg->Build(dkx.p, dky.p, ps->st);
+    ps->mg.reset(new hg::MgPrecond(nx, ny, ps->cg->kx32(), ps->cg->ky32(), ps->cg->dinv32()));
+    ps->mg->Build(ps->st);
     HG_CHECK(hipMalloc(&ps->b, N * 8));
     HG_CHECK(hipMalloc(&ps->x, N * 8));
     HG_CHECK(hipStreamSynchronize(ps->st));

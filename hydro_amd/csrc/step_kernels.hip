@@ -384,7 +384,7 @@ KPressureRows(Geo g, const double* __restrict__ fstar, const double* __restrict_
 // cell): what the CG solvers need, which take the operator from the face
 // coefficients.
 __global__ void __launch_bounds__(kBlock)
-KPressureRhs(Geo g, const double* __restrict__ fstar, double* __restrict__ k) {
+KPressureRhs(Geo g, const double* __restrict__ fstar, double* __restrict__ k, int negate) {
   const int nx = g.nx, ny = g.ny;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
@@ -395,7 +395,7 @@ KPressureRhs(Geo g, const double* __restrict__ fstar, double* __restrict__ k) {
   kon += fstar[YFace(g, i, j)] * -1.;
   kon += fstar[YFace(g, i, j + 1)] * 1.;
   kon += -(0. * g.V);
-  k[c] = kon;
+  k[c] = negate ? -kon : kon;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -777,8 +777,9 @@ void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0
   KRhieChow<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap,
                                          rc, mvx, mvy, fstar, fcoef);
 }
-void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k) {
-  KPressureRhs<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, fstar, k);
+void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k,
+                       bool negate) {
+  KPressureRhs<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, fstar, k, negate ? 1 : 0);
 }
 void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         int pinned, double pval, double* s, double* w, double* c, double* e,
