@@ -351,128 +351,208 @@ __device__ __forceinline__ void BlockSync() {
   __syncthreads();
 }
 
-// The coarse end of the V-cycle in one workgroup: levels first..last of the
-// hierarchy (each <= kInKernelMax cells), the same smoothing / restriction /
-// prolongation as the grid kernels with the iterates and residuals in LDS
-// (coefficients read through the caches), and on the coarsest level (<= 64
-// cells, one cell per lane of wave 0) at most kTinyIters Jacobi-PCG iterations.  Input: r of
-// the first level (global); output: its xb (global).
-__global__ void __launch_bounds__(1024)
+// The coarse end of the V-cycle in one workgroup of kCcThreads threads:
+// levels first..last of the hierarchy (the first <= kInKernelMax cells, each
+// next one a quarter of it), the same smoothing / restriction / bilinear
+// prolongation as the grid kernels, and on the coarsest level (<= 64 cells,
+// one cell per lane of wave 0) at most kTinyIters Jacobi-PCG iterations.
+// Thread t owns cells t, t + kCcThreads, ... of every level; their face
+// coefficients, inverse diagonals and prolongation weights are loaded into
+// registers once, so the phases of the cycle touch only registers and LDS
+// (iterates, residuals) between barriers.  Input: r of the first level
+// (global); output: its correction (global xb of the first level).
+constexpr int kCcThreads = 1024;
+constexpr int CcCells(int l) { return kInKernelMax >> (2 * l); }
+constexpr int CcSlots(int l) { return CcCells(l) > kCcThreads ? CcCells(l) / kCcThreads : 1; }
+constexpr int CcLevels() {
+  int l = 0;
+  while (CcCells(l) > kTinyMax) ++l;
+  return l;
+}
+constexpr int kCcLevels = CcLevels();  // smoothed (non-coarsest) levels at most
+constexpr int kCcSlots = CcSlots(0);
+static_assert(kInKernelMax % kCcThreads == 0 || kInKernelMax < kCcThreads, "slots");
+
+__global__ void __launch_bounds__(kCcThreads)
 KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
   if (Done(done)) return;
   __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
-  const int T = blockDim.x, t = threadIdx.x;
-  const int L = cs.count;
+  const int t = threadIdx.x;
+  const int nlev = cs.count - 1;
+  // per owned cell: faces w e s n, inverse diagonal, prolongation weights
+  float kw[kCcLevels][kCcSlots], ke[kCcLevels][kCcSlots], ks[kCcLevels][kCcSlots],
+      kn[kCcLevels][kCcSlots], di[kCcLevels][kCcSlots], wx[kCcLevels][kCcSlots],
+      wy[kCcLevels][kCcSlots];
+#pragma unroll
+  for (int l = 0; l < kCcLevels; ++l) {
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      kw[l][m] = ke[l][m] = ks[l][m] = kn[l][m] = di[l][m] = wx[l][m] = wy[l][m] = 0.f;
+      if (l >= nlev) continue;
+      const int nx = cs.nx[l], c = t + m * kCcThreads;
+      if (c >= nx * cs.ny[l]) continue;
+      const int i = c % nx, j = c / nx;
+      kw[l][m] = cs.kx[l][i + (nx + 1) * j];
+      ke[l][m] = cs.kx[l][i + 1 + (nx + 1) * j];
+      ks[l][m] = cs.ky[l][i + nx * j];
+      kn[l][m] = cs.ky[l][i + nx * (j + 1)];
+      di[l][m] = cs.dinv[l][c];
+      if (cs.bilinear) {
+        // weights of the coarse neighbours on this child's side (zero across
+        // walls, where the coarse face coefficient is zero)
+        const int nxc = cs.nx[l + 1], I = i >> 1, J = j >> 1;
+        const float cw = cs.kx[l + 1][I + (nxc + 1) * J], ce = cs.kx[l + 1][I + 1 + (nxc + 1) * J];
+        const float cs_ = cs.ky[l + 1][I + nxc * J], cn = cs.ky[l + 1][I + nxc * (J + 1)];
+        const float kmax = fmaxf(fmaxf(cw, ce), fmaxf(cs_, cn));
+        if (kmax > 0.f) {
+          wx[l][m] = fminf(1.f, 4.f * ((i & 1) ? ce : cw) / kmax);
+          wy[l][m] = fminf(1.f, 4.f * ((j & 1) ? cn : cs_) / kmax);
+        }
+      }
+    }
+  }
+  // the coarsest level's operator, one cell per lane of wave 0
+  const int nxT = cs.nx[nlev], nyT = cs.ny[nlev], nT = nxT * nyT;
+  const bool actT = t < nT;
+  const int iT = actT ? t % nxT : 0, jT = actT ? t / nxT : 0;
+  float tw = 0.f, te = 0.f, ts = 0.f, tn = 0.f, tdi = 0.f;
+  if (actT) {
+    tw = cs.kx[nlev][iT + (nxT + 1) * jT];
+    te = cs.kx[nlev][iT + 1 + (nxT + 1) * jT];
+    ts = cs.ky[nlev][iT + nxT * jT];
+    tn = cs.ky[nlev][iT + nxT * (jT + 1)];
+    tdi = cs.dinv[nlev][t];
+  }
   const int n0 = cs.nx[0] * cs.ny[0];
-  for (int c = t; c < n0; c += T) sr[c] = cs.r[0][c];
+#pragma unroll
+  for (int m = 0; m < kCcSlots; ++m) {
+    const int c = t + m * kCcThreads;
+    if (c < n0) sr[c] = cs.r[0][c];
+  }
   __syncthreads();
-  for (int l = 0; l < L - 1; ++l) {
+  // (A x) of owned cell (l, m) at (i, j) from LDS values x (level-local)
+  auto Apply = [&](const float* x, int l, int m, int nx, int ny, int c, int i, int j) {
+    float v = (((kw[l][m] + ke[l][m]) + ks[l][m]) + kn[l][m]) * x[c];
+    if (i > 0) v -= kw[l][m] * x[c - 1];
+    if (i < nx - 1) v -= ke[l][m] * x[c + 1];
+    if (j > 0) v -= ks[l][m] * x[c - nx];
+    if (j < ny - 1) v -= kn[l][m] * x[c + nx];
+    return v;
+  };
+  // ---- down: two sweeps from zero, residual, restriction (sum of children)
+#pragma unroll
+  for (int l = 0; l < kCcLevels; ++l) {
+    if (l >= nlev) break;
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
-    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
     const float* r = sr + cs.off[l];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
-    for (int c = t; c < n; c += T) xa[c] = w.a * dinv[c] * r[c];
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c < n) xa[c] = w.a * di[l][m] * r[c];
+    }
     __syncthreads();
-    for (int c = t; c < n; c += T) {
-      const int i = c % nx, j = c / nx;
-      xb[c] = xa[c] + w.b * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c < n) xb[c] = xa[c] + w.b * di[l][m] * (r[c] - Apply(xa, l, m, nx, ny, c, c % nx, c / nx));
+    }
+    __syncthreads();
+    // residual r - A xb into xa (free until the up-leg)
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c < n) xa[c] = r[c] - Apply(xb, l, m, nx, ny, c, c % nx, c / nx);
     }
     __syncthreads();
     const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
     float* rc = sr + cs.off[l + 1];
-    for (int C = t; C < nc; C += T) {
+    for (int C = t; C < nc; C += kCcThreads) {
       const int I = C % nxc, J = C / nxc;
       float sum = 0.f;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
-        sum += r[i + nx * j] - Ax(nx, ny, kx, ky, xb, i, j);
-      }
+      for (int q = 0; q < 4; ++q) sum += xa[2 * I + (q & 1) + nx * (2 * J + (q >> 1))];
       rc[C] = sum;
     }
     __syncthreads();
   }
+  // ---- coarsest: Jacobi-PCG in wave 0, result into sxa
   if (t < 64) {
-    const int nx = cs.nx[L - 1], ny = cs.ny[L - 1], n = nx * ny;
-    const float *kx = cs.kx[L - 1], *ky = cs.ky[L - 1];
-    const float* r = sr + cs.off[L - 1];
-    const int lane = t;
-    const bool act = lane < n;
-    const int i = act ? lane % nx : 0, j = act ? lane / nx : 0;
-    float kw = 0.f, ke = 0.f, ks = 0.f, kn = 0.f, di = 0.f, b = 0.f;
-    if (act) {
-      kw = kx[i + (nx + 1) * j];
-      ke = kx[i + 1 + (nx + 1) * j];
-      ks = ky[i + nx * j];
-      kn = ky[i + nx * (j + 1)];
-      di = cs.dinv[L - 1][lane];
-      b = r[lane];
-    }
-    const float dg = ((kw + ke) + ks) + kn;
+    const float* r = sr + cs.off[nlev];
+    float b = actT ? r[t] : 0.f;
+    const float dg = ((tw + te) + ts) + tn;
     // the face-form operator has the constants as null space: solve with the
     // right-hand side projected to zero mean (consistent system)
-    b -= WaveSumF(b) / float(n);
-    if (!act) b = 0.f;
-    float x = 0.f, rr = b, z = di * rr, p = z;
+    b -= WaveSumF(b) / float(nT);
+    if (!actT) b = 0.f;
+    float x = 0.f, rr = b, z = tdi * rr, p = z;
     float rz = WaveSumF(rr * z);
     const float rz0 = rz;
     for (int it = 0; it < kTinyIters && rz > 1e-12f * rz0; ++it) {
-      const float pw = __shfl(p, (lane + 63) & 63, 64), pe = __shfl(p, (lane + 1) & 63, 64);
-      const float ps = __shfl(p, (lane + 64 - nx) & 63, 64);
-      const float pn = __shfl(p, (lane + nx) & 63, 64);
+      const float pw = __shfl(p, (t + 63) & 63, 64), pe = __shfl(p, (t + 1) & 63, 64);
+      const float ps = __shfl(p, (t + 64 - nxT) & 63, 64);
+      const float pn = __shfl(p, (t + nxT) & 63, 64);
       float ap = dg * p;
-      if (i > 0) ap -= kw * pw;
-      if (i < nx - 1) ap -= ke * pe;
-      if (j > 0) ap -= ks * ps;
-      if (j < ny - 1) ap -= kn * pn;
-      if (!act) ap = 0.f;
+      if (iT > 0) ap -= tw * pw;
+      if (iT < nxT - 1) ap -= te * pe;
+      if (jT > 0) ap -= ts * ps;
+      if (jT < nyT - 1) ap -= tn * pn;
+      if (!actT) ap = 0.f;
       const float pap = WaveSumF(p * ap);
       if (!(pap > 0.f)) break;
       const float alpha = rz / pap;
       x += alpha * p;
       rr -= alpha * ap;
-      z = di * rr;
+      z = tdi * rr;
       const float rzn = WaveSumF(rr * z);
       p = z + (rzn / rz) * p;
       rz = rzn;
     }
-    x -= WaveSumF(act ? x : 0.f) / float(n);
-    if (act) sxb[cs.off[L - 1] + lane] = x;
+    x -= WaveSumF(actT ? x : 0.f) / float(nT);
+    if (actT) sxa[cs.off[nlev] + t] = x;
   }
   __syncthreads();
-  for (int l = L - 2; l >= 0; --l) {
-    const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
-    const int nxc = cs.nx[l + 1], nyc = cs.ny[l + 1];
-    const float *kx = cs.kx[l], *ky = cs.ky[l], *dinv = cs.dinv[l];
+  // ---- up: y = xb + P e (into xa), two sweeps; the level's result in xa
+#pragma unroll
+  for (int l = kCcLevels - 1; l >= 0; --l) {
+    if (l >= nlev) continue;
+    const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny, nxc = cs.nx[l + 1];
     const float* r = sr + cs.off[l];
-    const float* e = sxb + cs.off[l + 1];
-    const float *kxc = cs.kx[l + 1], *kyc = cs.ky[l + 1];
+    const float* e = sxa + cs.off[l + 1];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
-    for (int c = t; c < n; c += T) {
-      const int i = c % nx, j = c / nx;
-      auto Y = [&](int ii, int jj) {
-        return xb[ii + nx * jj] +
-               (cs.bilinear ? Interp<true>(e, kxc, kyc, nxc, nyc, ii, jj)
-                            : Interp<false>(e, kxc, kyc, nxc, nyc, ii, jj));
-      };
-      const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-      const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-      const float yc = Y(i, j);
-      float ay = (((kw + ke) + ks) + kn) * yc;
-      if (i > 0) ay -= kw * Y(i - 1, j);
-      if (i < nx - 1) ay -= ke * Y(i + 1, j);
-      if (j > 0) ay -= ks * Y(i, j - 1);
-      if (j < ny - 1) ay -= kn * Y(i, j + 1);
-      xa[c] = yc + w.a * dinv[c] * (r[c] - ay);
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c >= n) continue;
+      const int i = c % nx, j = c / nx, I = i >> 1, J = j >> 1;
+      const int Ix = I + ((i & 1) ? 1 : -1), Jy = J + ((j & 1) ? 1 : -1);
+      const float ec = e[I + nxc * J];
+      const float ex = wx[l][m] > 0.f ? e[Ix + nxc * J] : ec;
+      const float ey = wy[l][m] > 0.f ? e[I + nxc * Jy] : ec;
+      const float wd = wx[l][m] * wy[l][m];
+      const float ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
+      xa[c] = xb[c] + (ec + (3.f * wx[l][m] * (ex - ec) + 3.f * wy[l][m] * (ey - ec) +
+                             wd * (ed - ec)) * 0.0625f);
     }
     __syncthreads();
-    for (int c = t; c < n; c += T) {
-      const int i = c % nx, j = c / nx;
-      xb[c] = xa[c] + w.b * dinv[c] * (r[c] - Ax(nx, ny, kx, ky, xa, i, j));
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c < n) xb[c] = xa[c] + w.a * di[l][m] * (r[c] - Apply(xa, l, m, nx, ny, c, c % nx, c / nx));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < CcSlots(l); ++m) {
+      const int c = t + m * kCcThreads;
+      if (c < n) xa[c] = xb[c] + w.b * di[l][m] * (r[c] - Apply(xb, l, m, nx, ny, c, c % nx, c / nx));
     }
     __syncthreads();
   }
-  for (int c = t; c < n0; c += T) cs.xb[0][c] = sxb[c];
+#pragma unroll
+  for (int m = 0; m < kCcSlots; ++m) {
+    const int c = t + m * kCcThreads;
+    if (c < n0) cs.xb[0][c] = sxa[c];
+  }
 }
 
 inline int Blocks1D(long n) { return int(std::min<long>((n + 255) / 256, 4096)); }
@@ -517,8 +597,10 @@ MgPrecond::MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0) {
         break;
       }
   }
-  if (first_in_kernel_ > 0 && int(lv_.size()) - first_in_kernel_ > kMaxInKernelLevels)
-    first_in_kernel_ = int(lv_.size()) - kMaxInKernelLevels;
+  // (a chain from <= kInKernelMax cells down to <= kTinyMax has at most
+  // kCcLevels + 1 levels; anything else stays on the grid kernels)
+  if (first_in_kernel_ > 0 && int(lv_.size()) - first_in_kernel_ > kCcLevels + 1)
+    first_in_kernel_ = -1;
   if (first_in_kernel_ > 0) {
     long cells = 0;
     for (int l = first_in_kernel_; l < int(lv_.size()); ++l) cells += long(lv_[l].nx) * lv_[l].ny;
