@@ -12,7 +12,7 @@
 //                  continues with Jacobi-PCG from the current iterate, to the
 //                  same tolerance.
 //
-// Work distribution: a persistent grid of at most 1024 workgroups walks
+// Work distribution: a persistent grid of at most 2048 workgroups walks
 // 64x4 tiles (tile.h), each XCD a contiguous band.  Reductions use no
 // atomics: each workgroup writes its partial sums, and the kernel that needs
 // the scalar folds all partials itself, in block order, in every workgroup
@@ -37,7 +37,10 @@ namespace hg {
 namespace {
 
 constexpr int kB = kTileThreads;
-constexpr int kMaxPersist = 1024;
+// persistent grid: up to 2048 workgroups (at 1024x2048, 4 tiles each); more
+// workgroups overlap more memory traffic, fewer make the per-workgroup
+// folds of the block partials cheaper -- 2048 measured best
+constexpr int kMaxPersist = 2048;
 
 struct CgScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, zq;
@@ -112,17 +115,51 @@ __device__ __forceinline__ double Fold(const double* part, int n) {
   return r;
 }
 
-// (A p)_c = d_c p_c - sum_nb k_f p_nb on the face form.
+// Sums of NQ partial arrays of n entries each in one pass: the same values,
+// in the same order, as NQ calls of Fold, with one pair of barriers.
+template <int NQ>
+__device__ __forceinline__ void FoldN(const double* const (&part)[NQ], int n,
+                                      double (&out)[NQ]) {
+  __shared__ double red[NQ][kB / 64];
+  double s[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) s[q] = 0.;
+  for (int b = threadIdx.x; b < n; b += kB)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) s[q] += part[q][b];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double v = WaveSum(s[q]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double t = 0.;
+    for (int w = 0; w < kB / 64; ++w) t += red[q][w];
+    out[q] = t;
+  }
+  __syncthreads();
+}
+
+// (A p)_c = d_c p_c - sum_nb k_f p_nb on the face form.  The five values
+// are loaded unconditionally (at clamped indices on the edges, where the
+// face term is then dropped by a select) so that their loads issue together
+// instead of one memory round trip per branch.
 template <class T, class F>
 __device__ __forceinline__ T ApplyFn(int nx, int ny, const T* __restrict__ kx,
                                      const T* __restrict__ ky, int i, int j, F&& P) {
   const T kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
   const T ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-  T v = (((kw + ke) + ks) + kn) * P(i, j);
-  if (i > 0) v -= kw * P(i - 1, j);
-  if (i < nx - 1) v -= ke * P(i + 1, j);
-  if (j > 0) v -= ks * P(i, j - 1);
-  if (j < ny - 1) v -= kn * P(i, j + 1);
+  const bool w = i > 0, e = i < nx - 1, s = j > 0, n = j < ny - 1;
+  const T pc = P(i, j);
+  const T pw = P(w ? i - 1 : i, j), pe = P(e ? i + 1 : i, j);
+  const T ps = P(i, s ? j - 1 : j), pn = P(i, n ? j + 1 : j);
+  T v = (((kw + ke) + ks) + kn) * pc;
+  v = w ? v - kw * pw : v;
+  v = e ? v - ke * pe : v;
+  v = s ? v - ks * ps : v;
+  v = n ? v - kn * pn : v;
   return v;
 }
 
@@ -148,8 +185,10 @@ KPcgSpmv(Walk w, int k, const T* __restrict__ kx, const T* __restrict__ ky,
   if (sc->done) return;
   const int nb = pt.nb;
   // stop test and beta from the previous update (or the start partials)
-  const double rz_new = Fold(pt.upd[(k + 1) & 1], nb);
-  const double rr = Fold(pt.upd[(k + 1) & 1] + nb, nb);
+  double f[3];
+  FoldN<3>({pt.upd[(k + 1) & 1], pt.upd[(k + 1) & 1] + nb, pt.upd[k & 1]}, nb, f);
+  const double rz_new = f[0];
+  const double rr = f[1];
   const bool stop = !(rr > sc->tol2 * sc->bb) || k >= sc->max_iter || !(rr == rr);
   if (stop) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -160,7 +199,7 @@ KPcgSpmv(Walk w, int k, const T* __restrict__ kx, const T* __restrict__ ky,
     return;
   }
   double beta = 0.;
-  if (k > 0) beta = rz_new / Fold(pt.upd[k & 1], nb);
+  if (k > 0) beta = rz_new / f[2];
   const T b = T(beta);
   double acc = 0.;
   ForTiles(w, [&](int i, int j) {
@@ -186,8 +225,10 @@ KPcgUpdate(Walk w, int k, const T* __restrict__ p, const T* __restrict__ q,
            Parts pt, CgScalars* sc) {
   if (sc->done) return;
   const int nb = pt.nb;
-  const double rz = Fold(pt.upd[(k + 1) & 1], nb);
-  const double pq = Fold(pt.spmv[k & 1], nb);
+  double f[2];
+  FoldN<2>({pt.upd[(k + 1) & 1], pt.spmv[k & 1]}, nb, f);
+  const double rz = f[0];
+  const double pq = f[1];
   const T a = T(rz / pq);
   double srz = 0., srr = 0.;
   ForTiles(w, [&](int i, int j) {
@@ -262,7 +303,9 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
          double* __restrict__ q, Parts pt, CgScalars* sc) {
   if (sc->done) return;
   const int nb = pt.nb;
-  const double rz = Fold(pt.dir + nb, nb);
+  double f[3];
+  FoldN<3>({pt.dir + nb, pt.dir, pt.spmv[(k + 1) & 1]}, nb, f);
+  const double rz = f[0];
   if (!(rz > 0.)) {  // preconditioner not positive on r
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->fail = 1;
@@ -271,7 +314,7 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
     return;
   }
   double beta = 0.;
-  if (k > 0) beta = -Fold(pt.dir, nb) / Fold(pt.spmv[(k + 1) & 1], nb);
+  if (k > 0) beta = -f[1] / f[2];
   double acc = 0.;
   ForTiles(w, [&](int i, int j) {
     auto P = [&](int ii, int jj) {
@@ -295,8 +338,10 @@ KFcgUpdate(Walk w, int k, const double* __restrict__ p, const double* __restrict
            double* __restrict__ x, double* __restrict__ r, Parts pt, CgScalars* sc) {
   if (sc->done) return;
   const int nb = pt.nb;
-  const double rz = Fold(pt.dir + nb, nb);
-  const double pq = Fold(pt.spmv[k & 1], nb);
+  double f[2];
+  FoldN<2>({pt.dir + nb, pt.spmv[k & 1]}, nb, f);
+  const double rz = f[0];
+  const double pq = f[1];
   if (!(pq > 0.)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->fail = 1;

@@ -123,7 +123,7 @@ KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* 
   xn[c] = x[c] + omega * dinv[c] * (float(r[c]) - Ax(nx, ny, kx, ky, x, i, j));
 }
 
-// Coarse correction at fine cell (i, j).  Injection: the parent's value.
+// Coarse correction at a fine cell.  Injection: the parent's value.
 // Bilinear (cell-centred) weighted by coupling: the parent P, its neighbours
 // X (in x, on the child's side), Y (in y) and the diagonal D contribute
 //   e = eP + (3 wx (eX - eP) + 3 wy (eY - eP) + wx wy (eD - eP)) / 16
@@ -131,98 +131,156 @@ KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* 
 // coefficients within a factor 4 of each other give the 9/16, 3/16, 3/16,
 // 1/16 bilinear weights; a wall (k = 0) or a weakly coupled neighbour across
 // a strong density jump gives injection in that direction.
-template <bool BILINEAR>
-__device__ __forceinline__ float Interp(const float* __restrict__ e,
-                                        const float* __restrict__ kxc,
-                                        const float* __restrict__ kyc, int nxc, int nyc, int i,
-                                        int j) {
-  const int I = i >> 1, J = j >> 1;
-  const float ec = e[I + nxc * J];
-  if (!BILINEAR) return ec;
-  const int sx = (i & 1) ? 1 : 0, sy = (j & 1) ? 1 : 0;  // face on the child's side
-  const float kw = kxc[I + (nxc + 1) * J], ke = kxc[I + 1 + (nxc + 1) * J];
-  const float ks = kyc[I + nxc * J], kn = kyc[I + nxc * (J + 1)];
+__device__ __forceinline__ void SideWeights(float kw, float ke, float ks, float kn,
+                                            float (&wt)[4]) {
   const float kmax = fmaxf(fmaxf(kw, ke), fmaxf(ks, kn));
-  if (!(kmax > 0.f)) return ec;
-  const float wx = fminf(1.f, 4.f * (sx ? ke : kw) / kmax);
-  const float wy = fminf(1.f, 4.f * (sy ? kn : ks) / kmax);
-  const int Ix = I + (sx ? 1 : -1), Jy = J + (sy ? 1 : -1);
-  // a neighbour outside the grid sits behind a wall face (k = 0, weight 0)
-  const float ex = wx > 0.f ? e[Ix + nxc * J] : ec;
-  const float ey = wy > 0.f ? e[I + nxc * Jy] : ec;
+  const bool any = kmax > 0.f;
+  wt[0] = any ? fminf(1.f, 4.f * kw / kmax) : 0.f;
+  wt[1] = any ? fminf(1.f, 4.f * ke / kmax) : 0.f;
+  wt[2] = any ? fminf(1.f, 4.f * ks / kmax) : 0.f;
+  wt[3] = any ? fminf(1.f, 4.f * kn / kmax) : 0.f;
+}
+__device__ __forceinline__ float Bilinear(float ec, float ex, float ey, float ed, float wx,
+                                          float wy) {
+  ex = wx > 0.f ? ex : ec;
+  ey = wy > 0.f ? ey : ec;
   const float wd = wx * wy;
-  const float ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
+  ed = wd > 0.f ? ed : ec;
   return ec + (3.f * wx * (ex - ec) + 3.f * wy * (ey - ec) + wd * (ed - ec)) * 0.0625f;
+}
+
+// The halo-2 region of a 64 x TY tile split into per-thread slots: slots
+// 0..NT-1 are tile cells (tile position threadIdx.x + 256 s), slot NT a cell
+// of the ring at distance 1, slot NT + 1 a cell of the ring at distance 2
+// (threads beyond a ring's length have none).  A thread keeps its cells
+// through the phases of a fused kernel, so the cells' coefficients and
+// right-hand side are loaded once, together, into registers; LDS carries
+// only the iterates that neighbours exchange.
+template <int TY>
+struct TileSlots {
+  static constexpr int NT = TY * kTX / kTileThreads;
+  static constexpr int N = NT + 2;
+  static_assert(2 * (kTX + 4) + 2 * (TY + 2) <= kTileThreads, "ring longer than the block");
+  // ring at distance d: rows -d and TY-1+d (with corners), columns -d and
+  // kTX-1+d (without); position relative to the tile origin
+  __device__ static bool Ring(int r, int d, int& li, int& lj) {
+    const int W = kTX + 2 * d, H = TY + 2 * d - 2;
+    if (r < W) { li = r - d; lj = -d; return true; }
+    r -= W;
+    if (r < W) { li = r - d; lj = TY - 1 + d; return true; }
+    r -= W;
+    if (r < H) { li = -d; lj = r - d + 1; return true; }
+    r -= H;
+    if (r < H) { li = kTX - 1 + d; lj = r - d + 1; return true; }
+    return false;
+  }
+  __device__ static bool Pos(int s, int& li, int& lj) {
+    if (s < NT) {
+      const int q = threadIdx.x + kTileThreads * s;
+      li = q % kTX;
+      lj = q / kTX;
+      return true;
+    }
+    return Ring(threadIdx.x, s - NT + 1, li, lj);
+  }
+};
+
+// One slot's cell: position, grid validity and (for the tile and the first
+// ring) its faces and inverse diagonal.
+struct SlotCell {
+  int li, lj, i, j;
+  bool mine;  // this thread has a cell in the slot
+  bool in;    // ... and it lies inside the grid
+  float kw, ke, ks, kn, di;
+};
+
+template <int TY, class RT>
+__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], float (&rv)[TileSlots<TY>::N],
+                                          int nx, int ny, int i0, int j0,
+                                          const float* __restrict__ kx,
+                                          const float* __restrict__ ky,
+                                          const float* __restrict__ dinv,
+                                          const RT* __restrict__ r, bool r_on_ring2) {
+  using S = TileSlots<TY>;
+#pragma unroll
+  for (int s = 0; s < S::N; ++s) {
+    SlotCell& c = sc[s];
+    c.li = c.lj = 0;
+    c.mine = S::Pos(s, c.li, c.lj);
+    c.i = i0 + c.li;
+    c.j = j0 + c.lj;
+    c.in = c.mine && c.i >= 0 && c.i < nx && c.j >= 0 && c.j < ny;
+    const int k = c.i + nx * c.j;
+    const bool coef = c.in && s <= S::NT;  // tile and first ring
+    c.kw = coef ? kx[c.i + (nx + 1) * c.j] : 0.f;
+    c.ke = coef ? kx[c.i + 1 + (nx + 1) * c.j] : 0.f;
+    c.ks = coef ? ky[k] : 0.f;
+    c.kn = coef ? ky[k + nx] : 0.f;
+    c.di = c.in ? dinv[k] : 0.f;
+    rv[s] = (c.in && (s <= S::NT || r_on_ring2)) ? float(r[k]) : 0.f;
+  }
+}
+
+// (A x) at slot cell c from LDS values x, the cell at x[lj + O][li + O].
+template <int O, int XW>
+__device__ __forceinline__ float SlotApply(const SlotCell& c, const float (*x)[XW], int nx,
+                                           int ny) {
+  const int a = c.lj + O, f = c.li + O;
+  float v = (((c.kw + c.ke) + c.ks) + c.kn) * x[a][f];
+  if (c.i > 0) v -= c.kw * x[a][f - 1];
+  if (c.i < nx - 1) v -= c.ke * x[a][f + 1];
+  if (c.j > 0) v -= c.ks * x[a - 1][f];
+  if (c.j < ny - 1) v -= c.kn * x[a + 1][f];
+  return v;
 }
 
 // Fused down-leg of one level: two damped Jacobi sweeps from zero
 // (x1 = w.a D^-1 r, x2 = x1 + w.b D^-1 (r - A x1)) and the restriction of
 // r - A x2 (sum of the four children).  A block of 256 threads works on a
-// 64 x TY tile with a two-cell halo in LDS (x1 on tile + 2, x2 on tile + 1),
-// so one launch and one pass over the level's arrays replace three; taller
-// tiles re-read less halo.
+// 64 x TY tile with a two-cell halo (x1 on tile + 2, x2 on tile + 1 in LDS):
+// one launch and one pass over the level's arrays replace three, and all
+// global loads are issued at the start (TileSlots), so a workgroup waits on
+// memory once.
 template <class RT, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
       const float* __restrict__ dinv, const RT* __restrict__ r, Omega2 w,
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done) {
   if (Done(done)) return;
+  using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
-  __shared__ float s1[H2Y][H2X], sr[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
+  __shared__ float s1[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
-  for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
-    const int hj = q / H2X, hi = q % H2X;
-    const int ii = i0 + hi - 2, jj = j0 + hj - 2;
-    float rv = 0.f, xv = 0.f;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
-      const int c = ii + nx * jj;
-      rv = float(r[c]);
-      xv = w.a * dinv[c] * rv;
-    }
-    sr[hj][hi] = rv;
-    s1[hj][hi] = xv;
-  }
+  SlotCell sc[S::N];
+  float rv[S::N];
+  LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
+#pragma unroll
+  for (int s = 0; s < S::N; ++s)
+    if (sc[s].mine) s1[sc[s].lj + 2][sc[s].li + 2] = sc[s].in ? w.a * sc[s].di * rv[s] : 0.f;
   __syncthreads();
-  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
-    const int hj = q / H1X, hi = q % H1X;
-    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
+#pragma unroll
+  for (int s = 0; s <= S::NT; ++s) {
+    const SlotCell& c = sc[s];
+    if (!c.mine) continue;
     float xv = 0.f;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
-      const int c = ii + nx * jj;
-      const int a = hj + 1, e = hi + 1;  // position in the halo-2 arrays
-      const float kw = kx[ii + (nx + 1) * jj], ke = kx[ii + 1 + (nx + 1) * jj];
-      const float ks = ky[ii + nx * jj], kn = ky[ii + nx * (jj + 1)];
-      float v = (((kw + ke) + ks) + kn) * s1[a][e];
-      if (ii > 0) v -= kw * s1[a][e - 1];
-      if (ii < nx - 1) v -= ke * s1[a][e + 1];
-      if (jj > 0) v -= ks * s1[a - 1][e];
-      if (jj < ny - 1) v -= kn * s1[a + 1][e];
-      xv = s1[a][e] + w.b * dinv[c] * (sr[a][e] - v);
-    }
-    s2[hj][hi] = xv;
+    if (c.in)
+      xv = s1[c.lj + 2][c.li + 2] +
+           w.b * c.di * (rv[s] - SlotApply<2, H2X>(c, s1, nx, ny));
+    s2[c.lj + 1][c.li + 1] = xv;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < TY * kTX; q += kTileThreads) {
-    const int li = q % kTX, lj = q / kTX;
-    const int i = i0 + li, j = j0 + lj;
+#pragma unroll
+  for (int s = 0; s < S::NT; ++s) {
+    const SlotCell& c = sc[s];
     float res = 0.f;
-    if (i < nx && j < ny) {
-      const int c = i + nx * j;
-      const int a = lj + 1, e = li + 1;
-      const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-      const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-      float v = (((kw + ke) + ks) + kn) * s2[a][e];
-      if (i > 0) v -= kw * s2[a][e - 1];
-      if (i < nx - 1) v -= ke * s2[a][e + 1];
-      if (j > 0) v -= ks * s2[a - 1][e];
-      if (j < ny - 1) v -= kn * s2[a + 1][e];
-      x2out[c] = s2[a][e];
-      res = sr[lj + 2][li + 2] - v;
+    if (c.in) {
+      x2out[c.i + nx * c.j] = s2[c.lj + 1][c.li + 1];
+      res = rv[s] - SlotApply<1, H1X>(c, s2, nx, ny);
     }
-    sres[lj][li] = res;
+    sres[c.lj][c.li] = res;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < (kTX / 2) * (TY / 2); q += kTileThreads) {
@@ -236,7 +294,10 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
 
 // Fused up-leg of one level: y = x2 + P e, x3 = y + w.a D^-1 (r - A y),
 // out = x3 + w.b D^-1 (r - A x3), on the block's 64 x TY tile with a
-// two-cell halo in LDS.
+// two-cell halo (y and x3 in LDS).  The coarse corrections and the coarse
+// cells' side weights of the covering coarse region are staged in LDS; x2,
+// r and the fine coefficients of the thread's cells (TileSlots) are loaded
+// at the start with them.
 template <class RT, bool BILINEAR, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
@@ -244,59 +305,90 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
     int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done) {
   if (Done(done)) return;
+  using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
-  __shared__ float sy[H2Y][H2X], sr[H1Y][H1X], s3[H1Y][H1X];
+  // coarse cells i0/2 - 2 .. i0/2 + kTX/2 + 1 (parents of the halo-2 region
+  // and their neighbours), likewise in y
+  constexpr int CX = kTX / 2 + 4, CY = TY / 2 + 4, NC = CX * CY;
+  constexpr int KC = (NC + kTileThreads - 1) / kTileThreads;
+  __shared__ float sy[H2Y][H2X], s3[H1Y][H1X];
+  __shared__ float se[CY][CX], sw[BILINEAR ? 4 : 1][CY][CX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
-  for (int q = threadIdx.x; q < H2X * H2Y; q += kTileThreads) {
-    const int hj = q / H2X, hi = q % H2X;
-    const int ii = i0 + hi - 2, jj = j0 + hj - 2;
-    float yv = 0.f;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
-      yv = x2[ii + nx * jj] + Interp<BILINEAR>(e, kxc, kyc, nxc, nyc, ii, jj);
-    sy[hj][hi] = yv;
-  }
-  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
-    const int hj = q / H1X, hi = q % H1X;
-    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
-    sr[hj][hi] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? float(r[ii + nx * jj]) : 0.f;
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < H1X * H1Y; q += kTileThreads) {
-    const int hj = q / H1X, hi = q % H1X;
-    const int ii = i0 + hi - 1, jj = j0 + hj - 1;
-    float xv = 0.f;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
-      const int c = ii + nx * jj;
-      const int a = hj + 1, f = hi + 1;
-      const float kw = kx[ii + (nx + 1) * jj], ke = kx[ii + 1 + (nx + 1) * jj];
-      const float ks = ky[ii + nx * jj], kn = ky[ii + nx * (jj + 1)];
-      float v = (((kw + ke) + ks) + kn) * sy[a][f];
-      if (ii > 0) v -= kw * sy[a][f - 1];
-      if (ii < nx - 1) v -= ke * sy[a][f + 1];
-      if (jj > 0) v -= ks * sy[a - 1][f];
-      if (jj < ny - 1) v -= kn * sy[a + 1][f];
-      xv = sy[a][f] + w.a * dinv[c] * (sr[hj][hi] - v);
+  const int ci0 = i0 / 2 - 2, cj0 = j0 / 2 - 2;
+  // coarse loads, clamped into the coarse grid: cells outside are read only
+  // behind a zero weight (walls), where the value is not used
+  float ce[KC], cf[KC][4];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int q = threadIdx.x + k * kTileThreads;
+    const int I = min(max(ci0 + q % CX, 0), nxc - 1), J = min(max(cj0 + q / CX, 0), nyc - 1);
+    const bool ok = q < NC;
+    ce[k] = ok ? e[I + nxc * J] : 0.f;
+    if (BILINEAR) {
+      cf[k][0] = ok ? kxc[I + (nxc + 1) * J] : 0.f;
+      cf[k][1] = ok ? kxc[I + 1 + (nxc + 1) * J] : 0.f;
+      cf[k][2] = ok ? kyc[I + nxc * J] : 0.f;
+      cf[k][3] = ok ? kyc[I + nxc * (J + 1)] : 0.f;
     }
-    s3[hj][hi] = xv;
+  }
+  SlotCell sc[S::N];
+  float rv[S::N], xv2[S::N];
+  LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
+#pragma unroll
+  for (int s = 0; s < S::N; ++s) xv2[s] = sc[s].in ? x2[sc[s].i + nx * sc[s].j] : 0.f;
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int q = threadIdx.x + k * kTileThreads;
+    if (q >= NC) continue;
+    se[q / CX][q % CX] = ce[k];
+    if (BILINEAR) {
+      float wt[4];
+      SideWeights(cf[k][0], cf[k][1], cf[k][2], cf[k][3], wt);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) sw[m][q / CX][q % CX] = wt[m];
+    }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < TY * kTX; q += kTileThreads) {
-    const int li = q % kTX, lj = q / kTX;
-    const int i = i0 + li, j = j0 + lj;
-    if (i >= nx || j >= ny) continue;
-    const int c = i + nx * j;
-    const int a = lj + 1, f = li + 1;
-    const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-    const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-    float v = (((kw + ke) + ks) + kn) * s3[a][f];
-    if (i > 0) v -= kw * s3[a][f - 1];
-    if (i < nx - 1) v -= ke * s3[a][f + 1];
-    if (j > 0) v -= ks * s3[a - 1][f];
-    if (j < ny - 1) v -= kn * s3[a + 1][f];
-    out[c] = s3[a][f] + w.b * dinv[c] * (sr[a][f] - v);
+  // y = x2 + P e on the halo-2 region (zero outside the grid)
+#pragma unroll
+  for (int s = 0; s < S::N; ++s) {
+    const SlotCell& c = sc[s];
+    if (!c.mine) continue;
+    float yv = 0.f;
+    if (c.in) {
+      const int pi = (c.i >> 1) - ci0, pj = (c.j >> 1) - cj0;  // parent, local
+      const float ec = se[pj][pi];
+      float corr = ec;
+      if (BILINEAR) {
+        const int sx = c.i & 1, sy_ = c.j & 1;
+        const float wx = sw[sx ? 1 : 0][pj][pi], wy = sw[sy_ ? 3 : 2][pj][pi];
+        const int xi = pi + (sx ? 1 : -1), yj = pj + (sy_ ? 1 : -1);
+        corr = Bilinear(ec, se[pj][xi], se[yj][pi], se[yj][xi], wx, wy);
+      }
+      yv = xv2[s] + corr;
+    }
+    sy[c.lj + 2][c.li + 2] = yv;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s <= S::NT; ++s) {
+    const SlotCell& c = sc[s];
+    if (!c.mine) continue;
+    float xv = 0.f;
+    if (c.in)
+      xv = sy[c.lj + 2][c.li + 2] + w.a * c.di * (rv[s] - SlotApply<2, H2X>(c, sy, nx, ny));
+    s3[c.lj + 1][c.li + 1] = xv;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < S::NT; ++s) {
+    const SlotCell& c = sc[s];
+    if (!c.in) continue;
+    out[c.i + nx * c.j] =
+        s3[c.lj + 1][c.li + 1] + w.b * c.di * (rv[s] - SlotApply<1, H1X>(c, s3, nx, ny));
   }
 }
 
