@@ -551,13 +551,39 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
   const int nx = g.nx, ny = g.ny;
   const int ntx = (nx + kAdvTX - 1) / kAdvTX;
   const int i0 = (blockIdx.x % ntx) * kAdvTX, j0 = (blockIdx.x / ntx) * kAdvTY;
+  // every global load of the block is issued here, before any value is used:
+  // each field on the halo-2 tile and the four face fluxes of the thread's
+  // tile cells (so the block waits on memory once, not once per element)
+  constexpr int NU = UX * UY, KU = (NU + kBlock - 1) / kBlock;
+  constexpr int NT = kAdvTX * kAdvTY, KT = (NT + kBlock - 1) / kBlock;
+  double ureg[NF][KU];
+#pragma unroll
+  for (int q = 0; q < NF; ++q)
+#pragma unroll
+    for (int k = 0; k < KU; ++k) {
+      const int t = threadIdx.x + k * kBlock;
+      const int ii = i0 + t % UX - 2, jj = j0 + t / UX - 2;
+      ureg[q][k] = (t < NU && ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? fl.u[q][ii + nx * jj]
+                                                                         : 0.;
+    }
+  double vw_[KT], ve_[KT], vs_[KT], vn_[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    const int i = i0 + t % kAdvTX, j = j0 + t / kAdvTX;
+    const bool in = t < NT && i < nx && j < ny;
+    vw_[k] = in ? flux[XFace(g, i, j)] : 0.;
+    ve_[k] = in ? flux[XFace(g, i + 1, j)] : 0.;
+    vs_[k] = in ? flux[YFace(g, i, j)] : 0.;
+    vn_[k] = in ? flux[YFace(g, i, j + 1)] : 0.;
+  }
+#pragma unroll
   for (int q = 0; q < NF; ++q) {
-    const double* __restrict__ u = fl.u[q];
     if (q > 0) __syncthreads();  // the previous field's LDS is still being read
-    for (int t = threadIdx.x; t < UX * UY; t += kBlock) {
-      const int hj = t / UX, hi = t % UX;
-      const int ii = i0 + hi - 2, jj = j0 + hj - 2;
-      su[hj][hi] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? u[ii + nx * jj] : 0.;
+#pragma unroll
+    for (int k = 0; k < KU; ++k) {
+      const int t = threadIdx.x + k * kBlock;
+      if (t < NU) su[t / UX][t % UX] = ureg[q][k];
     }
     __syncthreads();
     // x-gradients on columns i0-1 .. i0+64 of the tile rows
@@ -591,13 +617,14 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       sgy[hj][li] = gv;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < kAdvTY * kAdvTX; t += kBlock) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int t = threadIdx.x + k * kBlock;
       const int lj = t / kAdvTX, li = t % kAdvTX;
       const int i = i0 + li, j = j0 + lj;
-      if (i >= nx || j >= ny) continue;
+      if (t >= NT || i >= nx || j >= ny) continue;
       const int c = i + nx * j;
-      const double vw = flux[XFace(g, i, j)], ve = flux[XFace(g, i + 1, j)];
-      const double vs = flux[YFace(g, i, j)], vn = flux[YFace(g, i, j + 1)];
+      const double vw = vw_[k], ve = ve_[k], vs = vs_[k], vn = vn_[k];
       const double uc = su[lj + 2][li + 2];
       const double gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
       double uw = uc, ue = uc, us_ = uc, uN = uc;
