@@ -80,6 +80,8 @@ class CgSolver {
   void* host_scal_ = nullptr;  // pinned mirror
   int blocks_;
   int last_fcg_iters_ = 5;  // iterations of the last flexible solve
+  double* dirx_ = nullptr;  // direction-dot partials written by the preconditioner
+  int dirx_cap_ = 0;        // ... partials per dot it holds
   hipEvent_t ev0_, ev1_;
 
   CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project);

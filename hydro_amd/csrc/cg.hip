@@ -118,15 +118,20 @@ __device__ __forceinline__ double Fold(const double* part, int n) {
 // Sums of NQ partial arrays of n entries each in one pass: the same values,
 // in the same order, as NQ calls of Fold, with one pair of barriers.
 template <int NQ>
-__device__ __forceinline__ void FoldN(const double* const (&part)[NQ], int n,
+__device__ __forceinline__ void FoldN(const double* const (&part)[NQ], const int (&n)[NQ],
                                       double (&out)[NQ]) {
   __shared__ double red[NQ][kB / 64];
   double s[NQ];
+  int nmax = 0;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) s[q] = 0.;
-  for (int b = threadIdx.x; b < n; b += kB)
+  for (int q = 0; q < NQ; ++q) {
+    s[q] = 0.;
+    nmax = n[q] > nmax ? n[q] : nmax;
+  }
+  for (int b = threadIdx.x; b < nmax; b += kB)
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) s[q] += part[q][b];
+    for (int q = 0; q < NQ; ++q)
+      if (b < n[q]) s[q] += part[q][b];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const double v = WaveSum(s[q]);
@@ -165,13 +170,15 @@ __device__ __forceinline__ T ApplyFn(int nx, int ny, const T* __restrict__ kx,
 
 // Partial-buffer layout: upd[2][2 * nb] holds (rz, rr) of the update of
 // iteration k in slot k & 1; spmv[2][nb] holds (p, q) of iteration k in
-// slot k & 1; dir[2 * nb] holds (z, q_old), (r, z) of FCG.
+// slot k & 1; dir[2 * ndir] holds (z, q_old), (r, z) of FCG (ndir = nb when
+// KFcgDir writes them, the preconditioner's grid size when its last kernel
+// does).
 struct Parts {
   double* upd[2];
   double* spmv[2];
   double* dir;
   double* misc;
-  int nb;
+  int nb, ndir;
 };
 
 // ---- Jacobi-PCG (T = double or float vectors) ------------------------------
@@ -186,7 +193,7 @@ KPcgSpmv(Walk w, int k, const T* __restrict__ kx, const T* __restrict__ ky,
   const int nb = pt.nb;
   // stop test and beta from the previous update (or the start partials)
   double f[3];
-  FoldN<3>({pt.upd[(k + 1) & 1], pt.upd[(k + 1) & 1] + nb, pt.upd[k & 1]}, nb, f);
+  FoldN<3>({pt.upd[(k + 1) & 1], pt.upd[(k + 1) & 1] + nb, pt.upd[k & 1]}, {nb, nb, nb}, f);
   const double rz_new = f[0];
   const double rr = f[1];
   const bool stop = !(rr > sc->tol2 * sc->bb) || k >= sc->max_iter || !(rr == rr);
@@ -226,7 +233,7 @@ KPcgUpdate(Walk w, int k, const T* __restrict__ p, const T* __restrict__ q,
   if (sc->done) return;
   const int nb = pt.nb;
   double f[2];
-  FoldN<2>({pt.upd[(k + 1) & 1], pt.spmv[k & 1]}, nb, f);
+  FoldN<2>({pt.upd[(k + 1) & 1], pt.spmv[k & 1]}, {nb, nb}, f);
   const double rz = f[0];
   const double pq = f[1];
   const T a = T(rz / pq);
@@ -304,7 +311,7 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
   if (sc->done) return;
   const int nb = pt.nb;
   double f[3];
-  FoldN<3>({pt.dir + nb, pt.dir, pt.spmv[(k + 1) & 1]}, nb, f);
+  FoldN<3>({pt.dir + pt.ndir, pt.dir, pt.spmv[(k + 1) & 1]}, {pt.ndir, pt.ndir, nb}, f);
   const double rz = f[0];
   if (!(rz > 0.)) {  // preconditioner not positive on r
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -339,7 +346,7 @@ KFcgUpdate(Walk w, int k, const double* __restrict__ p, const double* __restrict
   if (sc->done) return;
   const int nb = pt.nb;
   double f[2];
-  FoldN<2>({pt.dir + nb, pt.spmv[k & 1]}, nb, f);
+  FoldN<2>({pt.dir + pt.ndir, pt.spmv[k & 1]}, {pt.ndir, nb}, f);
   const double rz = f[0];
   const double pq = f[1];
   if (!(pq > 0.)) {
@@ -552,6 +559,7 @@ CgSolver::~CgSolver() {
                   (void*)q32_, (void*)e32_, (void*)zprev32_, (void*)part_, scal_})
     if (p) (void)hipFree(p);
   if (host_scal_) (void)hipHostFree(host_scal_);
+  if (dirx_) (void)hipFree(dirx_);
   (void)hipEventDestroy(ev0_);
   (void)hipEventDestroy(ev1_);
 }
@@ -565,6 +573,7 @@ static Parts MakeParts(double* base, int nb) {
   p.spmv[1] = base + 5 * nb;
   p.dir = base + 6 * nb;
   p.misc = base + 8 * nb;
+  p.ndir = nb;
   return p;
 }
 
@@ -697,14 +706,32 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
   // simulation take similar counts), so few kernels are queued past
   // convergence (they return at once) and the host waits once or twice
   int batch = std::max(2, std::min(64, last_fcg_iters_ - 1));
+  // a preconditioner that takes over the stop test and the direction dots
+  // (FcgFusion) writes the dots to dirx_, sized for its grid
+  const int nfp = pc->FcgPartials();
+  if (nfp > dirx_cap_) {
+    if (dirx_) HG_CHECK(hipFree(dirx_));
+    dirx_ = nullptr;
+    HG_CHECK(hipMalloc(&dirx_, size_t(2) * nfp * 8));
+    dirx_cap_ = nfp;
+  }
   CgResult res;
   while (true) {
     for (int it = 0; it < batch; ++it, ++k) {
-      KFcgCheck<<<1, kB, 0, stream_>>>(k, pt, sc);
-      pc->Apply(r_, z32_, stream_, &sc->done);
-      KFcgDir<<<nb, kB, 0, stream_>>>(w, k, z32_, q_, r_, pt, sc);
-      KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pt, sc);
-      KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pt, sc);
+      Parts pk = pt;
+      FcgFusion f{pt.upd[(k + 1) & 1] + nb, nb, k, &sc->rr, &sc->iter, &sc->done,
+                  &sc->tol2, &sc->bb, &sc->max_iter, &sc->fail, k > 0 ? q_ : nullptr, dirx_};
+      int ndir = 0;
+      if (nfp > 0 && pc->ApplyFcg(r_, z32_, stream_, &sc->done, f, &ndir)) {
+        pk.dir = dirx_;
+        pk.ndir = ndir;
+      } else {
+        KFcgCheck<<<1, kB, 0, stream_>>>(k, pt, sc);
+        pc->Apply(r_, z32_, stream_, &sc->done);
+        KFcgDir<<<nb, kB, 0, stream_>>>(w, k, z32_, q_, r_, pt, sc);
+      }
+      KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pk, sc);
+      KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
       std::swap(pold, pnew);
     }
     ReadScalars();

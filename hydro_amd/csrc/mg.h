@@ -59,6 +59,11 @@ class MgPrecond : public Precond {
   // z = V(r): r is the FP64 residual of the finest level; z FP32.  Does
   // nothing when *d_done is set.
   void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override;
+  // Apply with flexible CG's stop test in the finest level's first kernel
+  // and its direction dots in the finest level's last kernel.
+  bool ApplyFcg(const double* r, float* z, hipStream_t st, const int* d_done,
+                const FcgFusion& f, int* ndir) override;
+  int FcgPartials() const override { return lv_.size() > 1 ? lv_[0].tall.blocks() : 0; }
   int levels() const { return int(lv_.size()); }
   // Average device time (ms) of one launch of a V-cycle kernel ("KDown<double>",
   // "KUp<double, true>" on the finest level, "KCoarseCycle"), launched `reps`
@@ -82,7 +87,8 @@ class MgPrecond : public Precond {
  private:
   std::vector<MgLevel> lv_;
   int first_in_kernel_ = -1;
-  void Cycle(int l, const double* r64, float* out, hipStream_t st, const int* d_done);
+  void Cycle(int l, const double* r64, float* out, hipStream_t st, const int* d_done,
+             const FcgFusion* f = nullptr);
 };
 
 }  // namespace hg

@@ -195,7 +195,7 @@ struct SlotCell {
 };
 
 template <int TY, class RT>
-__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], float (&rv)[TileSlots<TY>::N],
+__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], RT (&rv)[TileSlots<TY>::N],
                                           int nx, int ny, int i0, int j0,
                                           const float* __restrict__ kx,
                                           const float* __restrict__ ky,
@@ -217,7 +217,7 @@ __device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], floa
     c.ks = coef ? ky[k] : 0.f;
     c.kn = coef ? ky[k + nx] : 0.f;
     c.di = c.in ? dinv[k] : 0.f;
-    rv[s] = (c.in && (s <= S::NT || r_on_ring2)) ? float(r[k]) : 0.f;
+    rv[s] = (c.in && (s <= S::NT || r_on_ring2)) ? r[k] : RT(0);
   }
 }
 
@@ -245,8 +245,10 @@ template <class RT, int TY>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
       const float* __restrict__ dinv, const RT* __restrict__ r, Omega2 w,
-      float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done) {
+      float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done,
+      FcgFusion fcg, int stop_test) {
   if (Done(done)) return;
+  if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
   using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
   __shared__ float s1[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
@@ -255,11 +257,11 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   SlotCell sc[S::N];
-  float rv[S::N];
+  RT rv[S::N];
   LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
 #pragma unroll
   for (int s = 0; s < S::N; ++s)
-    if (sc[s].mine) s1[sc[s].lj + 2][sc[s].li + 2] = sc[s].in ? w.a * sc[s].di * rv[s] : 0.f;
+    if (sc[s].mine) s1[sc[s].lj + 2][sc[s].li + 2] = sc[s].in ? w.a * sc[s].di * float(rv[s]) : 0.f;
   __syncthreads();
 #pragma unroll
   for (int s = 0; s <= S::NT; ++s) {
@@ -268,7 +270,7 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     float xv = 0.f;
     if (c.in)
       xv = s1[c.lj + 2][c.li + 2] +
-           w.b * c.di * (rv[s] - SlotApply<2, H2X>(c, s1, nx, ny));
+           w.b * c.di * (float(rv[s]) - SlotApply<2, H2X>(c, s1, nx, ny));
     s2[c.lj + 1][c.li + 1] = xv;
   }
   __syncthreads();
@@ -278,7 +280,7 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     float res = 0.f;
     if (c.in) {
       x2out[c.i + nx * c.j] = s2[c.lj + 1][c.li + 1];
-      res = rv[s] - SlotApply<1, H1X>(c, s2, nx, ny);
+      res = float(rv[s]) - SlotApply<1, H1X>(c, s2, nx, ny);
     }
     sres[c.lj][c.li] = res;
   }
@@ -303,7 +305,8 @@ __global__ void __launch_bounds__(kTileThreads)
 KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
     const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
-    int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done) {
+    int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done, FcgFusion fcg,
+    int dots) {
   if (Done(done)) return;
   using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
@@ -315,7 +318,10 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   __shared__ float se[CY][CX], sw[BILINEAR ? 4 : 1][CY][CX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
-  if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
+  if (tile >= tg.ntx * tg.nty) {  // uniform over the block
+    if (dots && threadIdx.x == 0) fcg.dir[b] = fcg.dir[gridDim.x + b] = 0.;
+    return;
+  }
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   const int ci0 = i0 / 2 - 2, cj0 = j0 / 2 - 2;
   // coarse loads, clamped into the coarse grid: cells outside are read only
@@ -335,10 +341,15 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     }
   }
   SlotCell sc[S::N];
-  float rv[S::N], xv2[S::N];
+  RT rv[S::N];
+  float xv2[S::N];
+  double qo[S::NT];
   LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
 #pragma unroll
   for (int s = 0; s < S::N; ++s) xv2[s] = sc[s].in ? x2[sc[s].i + nx * sc[s].j] : 0.f;
+#pragma unroll
+  for (int s = 0; s < S::NT; ++s)
+    qo[s] = (dots && fcg.qold && sc[s].in) ? fcg.qold[sc[s].i + nx * sc[s].j] : 0.;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int q = threadIdx.x + k * kTileThreads;
@@ -379,16 +390,41 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     if (!c.mine) continue;
     float xv = 0.f;
     if (c.in)
-      xv = sy[c.lj + 2][c.li + 2] + w.a * c.di * (rv[s] - SlotApply<2, H2X>(c, sy, nx, ny));
+      xv = sy[c.lj + 2][c.li + 2] +
+           w.a * c.di * (float(rv[s]) - SlotApply<2, H2X>(c, sy, nx, ny));
     s3[c.lj + 1][c.li + 1] = xv;
   }
   __syncthreads();
+  double zq = 0., rz = 0.;
 #pragma unroll
   for (int s = 0; s < S::NT; ++s) {
     const SlotCell& c = sc[s];
     if (!c.in) continue;
-    out[c.i + nx * c.j] =
-        s3[c.lj + 1][c.li + 1] + w.b * c.di * (rv[s] - SlotApply<1, H1X>(c, s3, nx, ny));
+    const float z = s3[c.lj + 1][c.li + 1] +
+                    w.b * c.di * (float(rv[s]) - SlotApply<1, H1X>(c, s3, nx, ny));
+    out[c.i + nx * c.j] = z;
+    const double zd = double(z);
+    zq += zd * qo[s];
+    rz += double(rv[s]) * zd;
+  }
+  if (dots) {  // block partials of (z, q_old) and (r, z)
+    __shared__ double red[2][kTileThreads / 64];
+    zq = WaveSum(zq);
+    rz = WaveSum(rz);
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = zq;
+      red[1][threadIdx.x >> 6] = rz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = 0., c = 0.;
+      for (int q = 0; q < kTileThreads / 64; ++q) {
+        a += red[0][q];
+        c += red[1][q];
+      }
+      fcg.dir[b] = a;
+      fcg.dir[gridDim.x + b] = c;
+    }
   }
 }
 
@@ -738,7 +774,10 @@ void MgPrecond::Build(hipStream_t st) {
   }
 }
 
-void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, const int* done) {
+void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, const int* done,
+                      const FcgFusion* f) {
+  const FcgFusion fz = f ? *f : FcgFusion{};
+  const int fused = (f && l == 0) ? 1 : 0;
   MgLevel& L = lv_[l];
   if (l > 0 && l == first_in_kernel_ && use_coarse_kernel) {
     CoarseSet cs{};
@@ -805,16 +844,17 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
   if (l == 0)
     KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
         L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
-        C.r, done);
+        C.r, done, fz, fused);
   else
     KDown<float, kTY><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
                                                        L.r, Omega2{omega1, omega2}, L.xb, C.nx,
-                                                       C.ny, C.r, done);
+                                                       C.ny, C.r, done, fz, 0);
   Cycle(l + 1, nullptr, C.xb, st, done);
 #define HG_UP(RT, B, TY, TG, R)                                                               \
   KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
                                                       L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,    \
-                                                      Omega2{omega1, omega2}, out, done)
+                                                      Omega2{omega1, omega2}, out, done, fz,  \
+                                                      fused)
   if (l == 0) {
     if (bilinear)
       HG_UP(double, true, kTallY, L.tall, r64);
@@ -833,6 +873,14 @@ void MgPrecond::Apply(const double* r, float* z, hipStream_t st, const int* d_do
   Cycle(0, r, z, st, d_done);
 }
 
+bool MgPrecond::ApplyFcg(const double* r, float* z, hipStream_t st, const int* d_done,
+                         const FcgFusion& f, int* ndir) {
+  if (lv_.size() < 2) return false;  // no KDown / KUp pair on the finest level
+  Cycle(0, r, z, st, d_done, &f);
+  *ndir = lv_[0].tall.blocks();
+  return true;
+}
+
 double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStream_t st) {
   if (lv_.size() < 2) throw std::runtime_error("TimeKernel: single-level hierarchy");
   const std::string k(name);
@@ -842,11 +890,11 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
     if (k == "KDown<double>") {
       KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
-          C.r, nullptr);
+          C.r, nullptr, FcgFusion{}, 0);
     } else if (k == "KUp<double, true>") {
       KUp<double, true, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,
-          Omega2{omega1, omega2}, L.xa, nullptr);
+          Omega2{omega1, omega2}, L.xa, nullptr, FcgFusion{}, 0);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
       Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xb, st, nullptr);
     } else {
