@@ -126,13 +126,6 @@ struct MomOut {
   double* ap;  // optional: a_P of the relaxed rows
 };
 
-// Face-normal gradient component of a cell gradient interpolated to a face
-// with the zero-derivative condition (fluid.hpp:842).
-__device__ __forceinline__ double FaceGrad(const double* __restrict__ gv, int c, int nb,
-                                           bool inner) {
-  return inner ? Mid(gv[nb < c ? nb : c], gv[nb < c ? c : nb]) : gv[c];
-}
-
 // Explicit viscosity + force assembly + momentum rows (fluid.hpp:835-874,
 // conv_diff.hpp:130-243).  Row coefficients are shared by both velocity
 // components (their BCs, fluxes and densities coincide); constants differ.
@@ -147,15 +140,40 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
   const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
   const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
   const double V = g.V;
+  // every input of the cell is loaded here, neighbours at clamped indices
+  // (a neighbour value is used only where the neighbour exists), so the
+  // thread waits on memory once; below, values are selected, not loaded
+  const int cwc = hw ? cw : c, cec = he ? ce : c, csc = hs ? cs : c, cnc = hn ? cn : c;
   const double muw = in.ffmu[fw], mue = in.ffmu[fe], mus = in.ffmu[fs], mun = in.ffmu[fn];
+  const double Fw = in.flux[fw], Fe = in.flux[fe], Fs = in.flux[fs], Fn = in.flux[fn];
+  // gradients: g0x at C, W, E; g0y and g1x at C, W, E, S, N; g1y at C, S, N
+  const double g0xC = in.g0x[c], g0xW = in.g0x[cwc], g0xE = in.g0x[cec];
+  const double g0yC = in.g0y[c], g0yW = in.g0y[cwc], g0yE = in.g0y[cec], g0yS = in.g0y[csc],
+               g0yN = in.g0y[cnc];
+  const double g1xC = in.g1x[c], g1xW = in.g1x[cwc], g1xE = in.g1x[cec], g1xS = in.g1x[csc],
+               g1xN = in.g1x[cnc];
+  const double g1yC = in.g1y[c], g1yS = in.g1y[csc], g1yN = in.g1y[cnc];
+  const double u0C = in.u0[c], u0W = in.u0[cwc], u0E = in.u0[cec], u0S = in.u0[csc],
+               u0N = in.u0[cnc];
+  const double u1C = in.u1[c], u1W = in.u1[cwc], u1E = in.u1[cec], u1S = in.u1[csc],
+               u1N = in.u1[cnc];
+  const double gpxc = in.gpx[c], gpyc = in.gpy[c], frxc = in.frx[c], fryc = in.fry[c];
+  const double rhoc = in.rho[c];
+  const double utp0c = in.utp0[c], utp1c = in.utp1[c], utc0c = in.utc0[c], utc1c = in.utc1[c];
+  // face value of a cell gradient with the zero-derivative condition
+  // (fluid.hpp:842): Mid(lower-index cell, higher-index cell) on inner faces,
+  // the cell's own value on boundary faces
+  auto FG = [](double vc, double vnb, bool nb_lower, bool inner) {
+    return inner ? Mid(nb_lower ? vnb : vc, nb_lower ? vc : vnb) : vc;
+  };
 
   // --- explicit viscosity (transpose stress), component by component
   double fox = 0., foy = 0.;
   {
     // d = 0: weights mu_f * S_out[0] on the W/E faces.
     double ww = muw * (-g.sx), we = mue * g.sx;
-    double gwx = FaceGrad(in.g0x, c, cw, hw), gex = FaceGrad(in.g0x, c, ce, he);
-    double gwy = FaceGrad(in.g0y, c, cw, hw), gey = FaceGrad(in.g0y, c, ce, he);
+    double gwx = FG(g0xC, g0xW, true, hw), gex = FG(g0xC, g0xE, false, he);
+    double gwy = FG(g0yC, g0yW, true, hw), gey = FG(g0yC, g0yE, false, he);
     double sx = gwx * ww;
     sx += gex * we;
     double sy = gwy * ww;
@@ -164,8 +182,8 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
     foy += sy / V;
     // d = 1: weights mu_f * S_out[1] on the S/N faces.
     double ws = mus * (-g.sy), wn = mun * g.sy;
-    double gsx = FaceGrad(in.g1x, c, cs, hs), gnx = FaceGrad(in.g1x, c, cn, hn);
-    double gsy = FaceGrad(in.g1y, c, cs, hs), gny = FaceGrad(in.g1y, c, cn, hn);
+    double gsx = FG(g1xC, g1xS, true, hs), gnx = FG(g1xC, g1xN, false, hn);
+    double gsy = FG(g1yC, g1yS, true, hs), gny = FG(g1yC, g1yN, false, hn);
     sx = gsx * ws;
     sx += gnx * wn;
     sy = gsy * ws;
@@ -175,14 +193,16 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
   }
   // --- force assembly: += -grad p + restored force (+ zero surface tension
   //     and zero volume-source compensation on this path)
-  fox += in.gpx[c] * (-1.) + in.frx[c];
-  foy += in.gpy[c] * (-1.) + in.fry[c];
+  fox += gpxc * (-1.) + frxc;
+  foy += gpyc * (-1.) + fryc;
 
   // --- face expressions.  Convective (upwind deferred, threshold 1e-10,
   //     solver.hpp:201-243) and diffusive (solver.hpp:283-341).
-  const int nbi[4] = {cw, ce, cs, cn};
-  const int fid[4] = {fw, fe, fs, fn};
   const bool inner[4] = {hw, he, hs, hn};
+  const double Ff[4] = {Fw, Fe, Fs, Fn};
+  // the cell's and the neighbour's values per face (W, E: x-faces; S, N)
+  const double a0nb[4] = {g0xW, g0xE, g0yS, g0yN}, a0c[4] = {g0xC, g0xC, g0yC, g0yC};
+  const double a1nb[4] = {g1xW, g1xE, g1yS, g1yN}, a1c[4] = {g1xC, g1xC, g1yC, g1yC};
   const double mu[4] = {muw, mue, mus, mun};
   const double outf[4] = {-1., 1., -1., 1.};
   const int side[4] = {kLeft, kRight, kBottom, kTop};
@@ -192,23 +212,24 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
   for (int k = 0; k < 4; ++k) {
     const bool xface = k < 2;
     const double A = xface ? g.ax : g.ay;
-    const double F = in.flux[fid[k]];
+    const double F = Ff[k];
     const double o = outf[k];
     const bool self_m = (k == 1 || k == 3);
     if (inner[k]) {
-      const int cm = self_m ? c : nbi[k];
-      const int cp = self_m ? nbi[k] : c;
+      // gradient values at cell m (minus side) and p (plus side) of the face
+      const double g0m = self_m ? a0c[k] : a0nb[k], g0p = self_m ? a0nb[k] : a0c[k];
+      const double g1m = self_m ? a1c[k] : a1nb[k], g1p = self_m ? a1nb[k] : a1c[k];
       double am, ap, ak0, ak1;
       if (F > 1e-10) {
         am = 1.;
         ap = 0.;
-        ak0 = -(xface ? in.g0x[cm] * g.tx0 : in.g0y[cm] * g.ty0);
-        ak1 = -(xface ? in.g1x[cm] * g.tx0 : in.g1y[cm] * g.ty0);
+        ak0 = -(xface ? g0m * g.tx0 : g0m * g.ty0);
+        ak1 = -(xface ? g1m * g.tx0 : g1m * g.ty0);
       } else if (F < -1e-10) {
         am = 0.;
         ap = 1.;
-        ak0 = -(xface ? in.g0x[cp] * g.tx1 : in.g0y[cp] * g.ty1);
-        ak1 = -(xface ? in.g1x[cp] * g.tx1 : in.g1y[cp] * g.ty1);
+        ak0 = -(xface ? g0p * g.tx1 : g0p * g.ty1);
+        ak1 = -(xface ? g1p * g.tx1 : g1p * g.ty1);
       } else {
         am = 0.5;
         ap = 0.5;
@@ -239,25 +260,23 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
       Dk1 += (((al * v1) * (-mu[k])) * A) * o;
     }
   }
-  const double rho = in.rho[c];
+  const double rho = rhoc;
   const double cen = (Dc / V) + ((Cc / V + in.bdf2) * rho);
   double nb[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) nb[k] = (Dnb[k] / V) + ((Cnb[k] / V) * rho);
   // constants, then the delta form (row evaluated at the previous iterate in
   // the order S, W, C, E, N) and under-relaxation of the diagonal
-  const double un0 = in.bdf0 * in.utp0[c] + in.bdf1 * in.utc0[c];
-  const double un1 = in.bdf0 * in.utp1[c] + in.bdf1 * in.utc1[c];
+  const double un0 = in.bdf0 * utp0c + in.bdf1 * utc0c;
+  const double un1 = in.bdf0 * utp1c + in.bdf1 * utc1c;
   double k0 = ((Dk0 / V) + ((Ck0 / V + un0) * rho)) + (-fox);
   double k1 = ((Dk1 / V) + ((Ck1 / V + un1) * rho)) + (-foy);
-  const double* u0 = in.u0;
-  const double* u1 = in.u1;
-  if (hs) { k0 += u0[cs] * nb[2]; k1 += u1[cs] * nb[2]; }
-  if (hw) { k0 += u0[cw] * nb[0]; k1 += u1[cw] * nb[0]; }
-  k0 += u0[c] * cen;
-  k1 += u1[c] * cen;
-  if (he) { k0 += u0[ce] * nb[1]; k1 += u1[ce] * nb[1]; }
-  if (hn) { k0 += u0[cn] * nb[3]; k1 += u1[cn] * nb[3]; }
+  if (hs) { k0 += u0S * nb[2]; k1 += u1S * nb[2]; }
+  if (hw) { k0 += u0W * nb[0]; k1 += u1W * nb[0]; }
+  k0 += u0C * cen;
+  k1 += u1C * cen;
+  if (he) { k0 += u0E * nb[1]; k1 += u1E * nb[1]; }
+  if (hn) { k0 += u0N * nb[3]; k1 += u1N * nb[3]; }
   const double rs = hs ? nb[2] : 0., rw = hw ? nb[0] : 0., rc = cen / in.relax;
   const double re = he ? nb[1] : 0., rn = hn ? nb[3] : 0.;
   out.s[c] = rs;
