@@ -51,17 +51,34 @@ __device__ __forceinline__ double Extrap(const Geo& g, int s, double uc, double 
 // Face values of a cell field at the four faces (W, E, S, N) of cell (i,j).
 // kind: 0 = value condition (bv per side), 1 = zero derivative,
 // 2 = extrapolation.
-template <int kind>
-__device__ __forceinline__ void CellFaces(const Geo& g, const double* __restrict__ u, int i,
-                                          int j, const double* bv, double f[4]) {
-  const int nx = g.nx, ny = g.ny;
+// The cell's value and its four neighbours' (W, E, S, N), loaded together;
+// a missing neighbour reads the cell itself (its value is then not used).
+struct Cross {
+  double c, w, e, s, n;
+};
+__device__ __forceinline__ Cross LoadCross(const double* __restrict__ u, int nx, int ny, int i,
+                                           int j) {
   const int c = i + nx * j;
-  const double uc = u[c];
+  Cross x;
+  x.c = u[c];
+  x.w = u[i > 0 ? c - 1 : c];
+  x.e = u[i < nx - 1 ? c + 1 : c];
+  x.s = u[j > 0 ? c - nx : c];
+  x.n = u[j < ny - 1 ? c + nx : c];
+  return x;
+}
+
+// CellFaces (below) on values already loaded.
+template <int kind>
+__device__ __forceinline__ void CellFacesV(const Geo& g, const Cross& x, int i, int j,
+                                           const double* bv, double f[4]) {
+  const int nx = g.nx, ny = g.ny;
+  const double uc = x.c;
   double fw = 0, fe = 0, fs = 0, fn = 0;
-  if (i > 0) fw = Mid(u[c - 1], uc);
-  if (i < nx - 1) fe = Mid(uc, u[c + 1]);
-  if (j > 0) fs = Mid(u[c - nx], uc);
-  if (j < ny - 1) fn = Mid(uc, u[c + nx]);
+  if (i > 0) fw = Mid(x.w, uc);
+  if (i < nx - 1) fe = Mid(uc, x.e);
+  if (j > 0) fs = Mid(x.s, uc);
+  if (j < ny - 1) fn = Mid(uc, x.n);
   if (kind == 0) {
     if (i == 0) fw = bv[kLeft];
     if (i == nx - 1) fe = bv[kRight];
@@ -82,6 +99,11 @@ __device__ __forceinline__ void CellFaces(const Geo& g, const double* __restrict
   f[1] = fe;
   f[2] = fs;
   f[3] = fn;
+}
+template <int kind>
+__device__ __forceinline__ void CellFaces(const Geo& g, const double* __restrict__ u, int i,
+                                          int j, const double* bv, double f[4]) {
+  CellFacesV<kind>(g, LoadCross(u, g.nx, g.ny, i, j), i, j, bv, f);
 }
 
 // Gradient from the four face values (solver.hpp:658-677).

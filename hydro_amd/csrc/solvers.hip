@@ -226,29 +226,35 @@ KLuRbBlack(int nx, int ny, Rows a, const double* __restrict__ k0, const double* 
   if (r >= nx * ny) return;
   const int i = r % nx, j = r / nx;
   if (((i + j) & 1) == 0) return;
+  // neighbours at clamped indices, all loaded before use (a missing
+  // neighbour's values are not used)
+  const int qs = j > 0 ? r - nx : r, qw = i > 0 ? r - 1 : r;
+  const int qe = i < nx - 1 ? r + 1 : r, qn = j < ny - 1 ? r + nx : r;
+  const double as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
+  const double k0r = k0[r], k1r = k1[r];
+  const double k0s = k0[qs], k1s = k1[qs], cs = a.c[qs];
+  const double k0w = k0[qw], k1w = k1[qw], cw = a.c[qw];
+  const double k0e = k0[qe], k1e = k1[qe], ce = a.c[qe];
+  const double k0n = k0[qn], k1n = k1[qn], cn = a.c[qn];
   double s0 = 0., s1 = 0.;
   if (j > 0) {
-    const int q = r - nx;
-    s0 += a.s[r] * ((-k0[q] - 0.) / a.c[q]);
-    s1 += a.s[r] * ((-k1[q] - 0.) / a.c[q]);
+    s0 += as * ((-k0s - 0.) / cs);
+    s1 += as * ((-k1s - 0.) / cs);
   }
   if (i > 0) {
-    const int q = r - 1;
-    s0 += a.w[r] * ((-k0[q] - 0.) / a.c[q]);
-    s1 += a.w[r] * ((-k1[q] - 0.) / a.c[q]);
+    s0 += aw * ((-k0w - 0.) / cw);
+    s1 += aw * ((-k1w - 0.) / cw);
   }
   if (i < nx - 1) {
-    const int q = r + 1;
-    s0 += a.e[r] * ((-k0[q] - 0.) / a.c[q]);
-    s1 += a.e[r] * ((-k1[q] - 0.) / a.c[q]);
+    s0 += ae * ((-k0e - 0.) / ce);
+    s1 += ae * ((-k1e - 0.) / ce);
   }
   if (j < ny - 1) {
-    const int q = r + nx;
-    s0 += a.n[r] * ((-k0[q] - 0.) / a.c[q]);
-    s1 += a.n[r] * ((-k1[q] - 0.) / a.c[q]);
+    s0 += an * ((-k0n - 0.) / cn);
+    s1 += an * ((-k1n - 0.) / cn);
   }
-  xb0[r] = (-k0[r] - s0) / a.c[r];
-  xb1[r] = (-k1[r] - s1) / a.c[r];
+  xb0[r] = (-k0r - s0) / ac;
+  xb1[r] = (-k1r - s1) / ac;
 }
 
 // "lu_rb", second kernel: backward update of red cells from black
@@ -266,17 +272,24 @@ KLuRbRed(int nx, int ny, Rows a, const double* __restrict__ k0, const double* __
     out1[r] = base1[r] + xb1[r];
     return;
   }
-  double x0 = (-k0[r] - 0.) / a.c[r];
-  double x1 = (-k1[r] - 0.) / a.c[r];
+  // neighbours at clamped indices, all loaded before use
+  const int qs = j > 0 ? r - nx : r, qw = i > 0 ? r - 1 : r;
+  const int qe = i < nx - 1 ? r + 1 : r, qn = j < ny - 1 ? r + nx : r;
+  const double as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
+  const double b0 = base0[r], b1 = base1[r];
+  const double x0n = xb0[qn], x1n = xb1[qn], x0e = xb0[qe], x1e = xb1[qe];
+  const double x0w = xb0[qw], x1w = xb1[qw], x0s = xb0[qs], x1s = xb1[qs];
+  double x0 = (-k0[r] - 0.) / ac;
+  double x1 = (-k1[r] - 0.) / ac;
   double s0 = 0., s1 = 0.;
-  if (j < ny - 1) { s0 += a.n[r] * xb0[r + nx]; s1 += a.n[r] * xb1[r + nx]; }
-  if (i < nx - 1) { s0 += a.e[r] * xb0[r + 1]; s1 += a.e[r] * xb1[r + 1]; }
-  if (i > 0) { s0 += a.w[r] * xb0[r - 1]; s1 += a.w[r] * xb1[r - 1]; }
-  if (j > 0) { s0 += a.s[r] * xb0[r - nx]; s1 += a.s[r] * xb1[r - nx]; }
-  x0 -= s0 / a.c[r];
-  x1 -= s1 / a.c[r];
-  out0[r] = base0[r] + x0;
-  out1[r] = base1[r] + x1;
+  if (j < ny - 1) { s0 += an * x0n; s1 += an * x1n; }
+  if (i < nx - 1) { s0 += ae * x0e; s1 += ae * x1e; }
+  if (i > 0) { s0 += aw * x0w; s1 += aw * x1w; }
+  if (j > 0) { s0 += as * x0s; s1 += as * x1s; }
+  x0 -= s0 / ac;
+  x1 -= s1 / ac;
+  out0[r] = b0 + x0;
+  out1[r] = b1 + x1;
 }
 
 }  // namespace

@@ -107,12 +107,16 @@ KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict_
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nx * g.ny) return;
   const int i = c % g.nx, j = c / g.nx;
+  // all three fields' neighbourhoods are loaded before any is used
+  const Cross xp = LoadCross(p, g.nx, g.ny, i, j);
+  const Cross x0 = LoadCross(u0, g.nx, g.ny, i, j);
+  const Cross x1 = LoadCross(u1, g.nx, g.ny, i, j);
   double f[4];
-  CellFaces<2>(g, p, i, j, nullptr, f);
+  CellFacesV<2>(g, xp, i, j, nullptr, f);
   GradFromFaces(g, f, gpx[c], gpy[c]);
-  CellFaces<0>(g, u0, i, j, bc.vel[0], f);
+  CellFacesV<0>(g, x0, i, j, bc.vel[0], f);
   GradFromFaces(g, f, g0x[c], g0y[c]);
-  CellFaces<0>(g, u1, i, j, bc.vel[1], f);
+  CellFacesV<0>(g, x1, i, j, bc.vel[1], f);
   GradFromFaces(g, f, g1x[c], g1y[c]);
 }
 
