@@ -50,8 +50,8 @@ def log(msg):
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on the RT 1024x2048 state, each scaled by what the counter reads
 # for a 256 MiB device-to-device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
-MEASURED_TRAFFIC = {"KUp<double, true>": 65.65e6, "KDown<double>": 54.11e6,
-                    "KAdvStep<2>": 134.86e6, "KMomentum": 562.25e6}
+MEASURED_TRAFFIC = {"KUp<double, true>": 82.79e6, "KDown<double>": 58.70e6,
+                    "KAdvStep<2>": 120.91e6, "KMomentum": 579.68e6}
 
 
 def run_path(text, steps, warmup, label, train=False):
@@ -165,8 +165,8 @@ def batch_c5(steps=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -54,8 +54,12 @@ KERNELS = {
                       "read p', a_P, p_prev, u0, u1, u_prev (2); write p, u0, u1"),
     "KCorrectFaces": (lambda nx, ny: _b(nx, ny, cells64=1, faces64=3),
                       "read flux*, flux coefficient, p'; write flux"),
-    "KNegateFaces": (lambda nx, ny: _b(nx, ny, faces64=2), "read flux coefficient; write k_f"),
-    "KNeg": (lambda nx, ny: _b(nx, ny, cells64=2), "read constant; write right-hand side"),
+    # ---- pressure-solve set-up and closing check (cg.hip) ----
+    "KSetOperator": (lambda nx, ny: _b(nx, ny, faces64=2, faces32=1, cells64=1, cells32=1),
+                     "read the flux coefficients; write k_f (FP64 and FP32), D^-1 (FP64, FP32)"),
+    "KSumPart": (lambda nx, ny: _b(nx, ny, cells64=1), "read b (partial sums of the mean)"),
+    "KStart": (lambda nx, ny: _b(nx, ny, cells64=4), "read b; write projected b, r, x = 0"),
+    "KResidual": (lambda nx, ny: _b(nx, ny, cells64=3, faces64=1), "read b, x, k_x, k_y; write r"),
     # ---- advection and properties ----
     "KAdvStep<2>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                     "two phase fields: read both fields and the volume flux; write both fields"),
@@ -71,8 +75,10 @@ KERNELS = {
     "KPcgUpdate<double>": (lambda nx, ny: _b(nx, ny, cells64=8),
                            "read p, q, D^-1, x, r; write x, r, z"),
     # ---- FCG iteration with the FP32 multigrid preconditioner ----
-    "KFcgCheck": (lambda nx, ny: 0, "one workgroup folds partial sums"),
-    "KFcgDir": (lambda nx, ny: _b(nx, ny, cells64=2, cells32=1), "read z (FP32), q_old, r"),
+    "KFcgCheck": (lambda nx, ny: 0, "one workgroup folds partial sums (preconditioners "
+                  "without the fused stop test)"),
+    "KFcgDir": (lambda nx, ny: _b(nx, ny, cells64=2, cells32=1),
+                "read z (FP32), q_old, r (preconditioners without the fused dots)"),
     "KFcgSpmv": (lambda nx, ny: _b(nx, ny, cells64=3, faces64=1, cells32=1),
                  "read z (FP32), p_old, k_x, k_y; write p, q"),
     "KFcgUpdate": (lambda nx, ny: _b(nx, ny, cells64=6), "read p, q, x, r; write x, r"),
@@ -83,6 +89,10 @@ KERNELS = {
                                             coarse32=0.25),
                           "read x2, r (FP64), D^-1, k_x, k_y, coarse correction and coarse k; "
                           "write the level output"),
+    # in an FCG iteration it also reads q_old for the direction dots
+    "KUp<double, true> + FCG dots": (lambda nx, ny: _b(nx, ny, cells64=2, cells32=3,
+                                                       faces32=1.25, coarse32=0.25),
+                                     "as KUp<double, true>, plus q_old"),
 }
 
 
@@ -99,12 +109,14 @@ def cg_iteration_bytes(nx, ny):
     return kernel_bytes("KPcgSpmv<double>", nx, ny) + kernel_bytes("KPcgUpdate<double>", nx, ny)
 
 
-def _vcycle_bytes(nx, ny):
+def _vcycle_bytes(nx, ny, fcg_dots=False):
     """One V-cycle: the finest level's fused kernels plus the FP32 levels
     below it (each level quarters the cells; down-leg reads r, D^-1, k and
     writes x2 and the coarse residual; up-leg reads x2, r, D^-1, k and the
-    coarse correction and writes the output)."""
-    b = kernel_bytes("KDown<double>", nx, ny) + kernel_bytes("KUp<double, true>", nx, ny)
+    coarse correction and writes the output).  fcg_dots: the finest up-leg
+    also reads q_old (flexible CG's direction dots)."""
+    up = "KUp<double, true> + FCG dots" if fcg_dots else "KUp<double, true>"
+    b = kernel_bytes("KDown<double>", nx, ny) + kernel_bytes(up, nx, ny)
     cx, cy = nx // 2, ny // 2
     while cx * cy > 2048 and cx % 2 == 0 and cy % 2 == 0:
         b += _b(cx, cy, cells32=3, faces32=1, coarse32=0.25)      # down
@@ -124,14 +136,14 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
         + kernel_bytes("KFacesDeriv0Pair", nx, ny)                 # NaN check, guesses, faces
     per_iter = sum(kernel_bytes(k, nx, ny) for k in (
         "KCellGrads", "KMomentum", "KLuRbBlack", "KLuRbRed", "KRhieChow", "KPressureRhs",
-        "KNegateFaces", "KNeg", "KCorrectCells", "KCorrectFaces"))
-    # operator set-up: FP64 face copy, FP32 faces and D^-1 for CG and for the
-    # multigrid's finest level
-    per_iter += w * (2 * f) + (f * w + f * F32 + n * F32) * 2
-    # FCG start (projection, residual) and final true residual
-    per_iter += w * (4 * n) + 2 * w * (3 * n + f)
-    per_fcg = sum(kernel_bytes(k, nx, ny) for k in ("KFcgDir", "KFcgSpmv", "KFcgUpdate")) \
-        + _vcycle_bytes(nx, ny)
+        "KCorrectCells", "KCorrectFaces"))
+    # operator set-up (the multigrid's finest level shares the FP32 arrays;
+    # coarse levels are negligible), FCG start and the closing true residual
+    per_iter += sum(kernel_bytes(k, nx, ny) for k in (
+        "KSetOperator", "KSumPart", "KStart", "KResidual"))
+    # FCG iteration: stop test and direction dots inside the V-cycle kernels
+    per_fcg = kernel_bytes("KFcgSpmv", nx, ny) + kernel_bytes("KFcgUpdate", nx, ny) \
+        + _vcycle_bytes(nx, ny, fcg_dots=True)
     finish = w * 3 * n                                             # NaN check
     advection = substeps * kernel_bytes("KAdvStep<2>", nx, ny)
     properties = kernel_bytes("KProperties", nx, ny) + 4 * kernel_bytes("KSmooth", nx, ny)
