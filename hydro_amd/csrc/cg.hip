@@ -322,19 +322,58 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
   }
   double beta = 0.;
   if (k > 0) beta = -f[1] / f[2];
+  // p is formed once per cell of the tile and its one-cell halo, in LDS, and
+  // the stencil reads it there; each thread's loads (two halo-region cells,
+  // its own cell's faces) are issued together
+  constexpr int HX = kTX + 2, HY = kTY + 2, NH = HX * HY;
+  static_assert(NH <= 2 * kB, "two staging slots per thread");
+  __shared__ double sp[HY][HX];
+  const int nx = w.nx, ny = w.ny;
+  const int nt = w.tg.ntx * w.tg.nty;
+  const int bpx = gridDim.x / kXcds, xcd = blockIdx.x % kXcds;
   double acc = 0.;
-  ForTiles(w, [&](int i, int j) {
-    auto P = [&](int ii, int jj) {
-      const int cc = ii + w.nx * jj;
-      return k > 0 ? double(z[cc]) + beta * pold[cc] : double(z[cc]);
-    };
-    const double pc = P(i, j);
-    const double qv = ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
-    const int c = i + w.nx * j;
-    pnew[c] = pc;
-    q[c] = qv;
-    acc += pc * qv;
-  });
+  for (int t = blockIdx.x / kXcds; t < w.tg.per; t += bpx) {
+    const int tile = xcd * w.tg.per + t;
+    if (tile >= nt) break;  // uniform over the block
+    const int i0 = (tile % w.tg.ntx) * kTX, j0 = (tile / w.tg.ntx) * kTY;
+    double zs[2], ps[2];
+    bool ins[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int h = threadIdx.x + s * kB;
+      const int ii = i0 - 1 + h % HX, jj = j0 - 1 + h / HX;
+      ins[s] = h < NH && ii >= 0 && ii < nx && jj >= 0 && jj < ny;
+      const int cc = ins[s] ? ii + nx * jj : 0;
+      zs[s] = ins[s] ? double(z[cc]) : 0.;
+      ps[s] = (ins[s] && k > 0) ? pold[cc] : 0.;
+    }
+    const int li = threadIdx.x & (kTX - 1), lj = threadIdx.x >> 6;
+    const int i = i0 + li, j = j0 + lj;
+    const bool in = i < nx && j < ny;
+    const int fw = in ? i + (nx + 1) * j : 0, fs = in ? i + nx * j : 0;
+    const double kw = in ? kx[fw] : 0., ke = in ? kx[fw + 1] : 0.;
+    const double ks = in ? ky[fs] : 0., kn = in ? ky[fs + nx] : 0.;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int h = threadIdx.x + s * kB;
+      if (h < NH) sp[h / HX][h % HX] = ins[s] ? (k > 0 ? zs[s] + beta * ps[s] : zs[s]) : 0.;
+    }
+    __syncthreads();
+    if (in) {
+      const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+      const double pc = sp[lj + 1][li + 1];
+      double qv = (((kw + ke) + ks) + kn) * pc;
+      qv = hw ? qv - kw * sp[lj + 1][li] : qv;
+      qv = he ? qv - ke * sp[lj + 1][li + 2] : qv;
+      qv = hs ? qv - ks * sp[lj][li + 1] : qv;
+      qv = hn ? qv - kn * sp[lj + 2][li + 1] : qv;
+      const int c = i + nx * j;
+      pnew[c] = pc;
+      q[c] = qv;
+      acc += pc * qv;
+    }
+    __syncthreads();  // sp is rewritten by the next tile
+  }
   double v[1] = {acc};
   PutPartials<1>(v, pt.spmv[k & 1]);
 }
@@ -703,9 +742,10 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
   int k = 0;
   // iterations are queued in batches between host checks of the stop flag;
   // the first batch is sized from the previous solve (solves of one
-  // simulation take similar counts), so few kernels are queued past
-  // convergence (they return at once) and the host waits once or twice
-  int batch = std::max(2, std::min(64, last_fcg_iters_ - 1));
+  // simulation take similar counts): one more than its count, so that the
+  // iteration whose stop test ends the solve is usually in it and the host
+  // waits once; iterations queued past convergence return at once
+  int batch = std::max(2, std::min(64, last_fcg_iters_ + 1));
   // a preconditioner that takes over the stop test and the direction dots
   // (FcgFusion) writes the dots to dirx_, sized for its grid
   const int nfp = pc->FcgPartials();
