@@ -44,7 +44,9 @@ class CgSolver {
   float* kx32() const { return kx32_; }
   float* ky32() const { return ky32_; }
   float* dinv32() const { return dinv32_; }
-  // Solves A x = b (device pointers, FP64) from x = 0.
+  // Solves A x = b (device pointers, FP64) from x = 0, or, with warm, from
+  // the x the caller left in d_x (flexible and FP64 CG; the mixed
+  // refinement always starts from 0).  Convergence is relative to ||b||.
   // precision: 0 FP64 Jacobi-PCG; 1 FP32 inner PCG under FP64 refinement.
   // With `pc` set: flexible CG with FP64 Krylov vectors preconditioned by
   // pc (FP32 output); falls back to Jacobi-PCG when pc breaks descent.
@@ -53,7 +55,8 @@ class CgSolver {
   // converged and ms come from FinishDeferred once the caller has
   // synchronised the stream (before the next Solve).
   CgResult Solve(const double* d_b, double* d_x, double tol, int max_iters, int precision,
-                 bool project, Precond* pc = nullptr, bool defer_final = false);
+                 bool project, Precond* pc = nullptr, bool defer_final = false,
+                 bool warm = false);
   void FinishDeferred(CgResult& r);
   // y = A x (FP64), for tests and residual checks.
   void Apply(const double* d_x, double* d_y);
@@ -84,12 +87,13 @@ class CgSolver {
   int dirx_cap_ = 0;        // ... partials per dot it holds
   hipEvent_t ev0_, ev1_;
 
-  CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project);
+  CgResult SolveFp64(const double* b, double* x, double tol, int max_iters, bool project,
+                     bool warm);
   CgResult SolveMixed(const double* b, double* x, double tol, int max_iters, bool project,
                       Precond* pc);
   CgResult SolveFlexible(const double* b, double* x, double tol, int max_iters, bool project,
-                         Precond* pc);
-  void Prepare(const double* b, double* x, bool project);
+                         Precond* pc, bool warm);
+  void Prepare(const double* b, double* x, bool project, bool warm);
   int JacobiPcg64(double* x, int max_iters);
   int InnerFp32(double tol_inner, int max_iters, Precond* pc, bool* pc_failed);
   double ResidualFp64(const double* b, const double* x, double* r);  // returns ||r||^2

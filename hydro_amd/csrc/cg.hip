@@ -463,6 +463,55 @@ KStart(Walk w, const double* __restrict__ b, int project, double* __restrict__ b
   PutPartials<1>(v, pt.upd[1] + pt.nb);
 }
 
+// Start from a multiple s x of the x the caller left in place, s minimising
+// the residual ||b_ - s A x|| (so the start is never worse than x = 0, and
+// s is about 1 when x is a good guess).  Pass 1: b_ = b - mean(b) (or b),
+// r = A x (held in r until pass 2), partials (b_, A x) and (A x, A x) in
+// update slot 0 (free until the first update of the solve) and (b_, b_) in
+// spmv slot 0 (folded into sc->bb before the first SpMV writes it).
+__global__ void __launch_bounds__(kB)
+KStartWarm(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
+           const double* __restrict__ b, int project, double* __restrict__ bp,
+           double* __restrict__ r, const double* __restrict__ x, Parts pt) {
+  const double mean = project ? Fold(pt.misc, pt.nb) / (double(w.nx) * double(w.ny)) : 0.;
+  double ba = 0., aa = 0., bb = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    const double v = project ? b[c] - mean : b[c];
+    auto P = [&](int ii, int jj) { return x[ii + w.nx * jj]; };
+    const double ax = ApplyFn<double>(w.nx, w.ny, kx, ky, i, j, P);
+    bp[c] = v;
+    r[c] = ax;
+    ba += v * ax;
+    aa += ax * ax;
+    bb += v * v;
+  });
+  double v2[2] = {ba, aa}, v1[1] = {bb};
+  PutPartials<2>(v2, pt.upd[0]);
+  __syncthreads();  // the partial sums' LDS is reused
+  PutPartials<1>(v1, pt.spmv[0]);
+}
+
+// Pass 2: s = (b_, A x) / (A x, A x) (0 when A x = 0); x <- s x,
+// r <- b_ - s A x; partial (r, r) into update slot 1.
+__global__ void __launch_bounds__(kB)
+KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
+                double* __restrict__ x, Parts pt) {
+  double f[2];
+  FoldN<2>({pt.upd[0], pt.upd[0] + pt.nb}, {pt.nb, pt.nb}, f);
+  const double s = f[1] > 0. ? f[0] / f[1] : 0.;
+  double rr = 0.;
+  ForTiles(w, [&](int i, int j) {
+    const int c = i + w.nx * j;
+    const double rv = bp[c] - s * r[c];
+    x[c] = s * x[c];
+    r[c] = rv;
+    rr += rv * rv;
+  });
+  double v[1] = {rr};
+  PutPartials<1>(v, pt.upd[1] + pt.nb);
+}
+
 // Folds `nb` partials into sc->rr (and sc->bb when set_bb).
 __global__ void __launch_bounds__(kB)
 KTakeRR(const double* part, int nb, CgScalars* sc, int set_bb) {
@@ -662,13 +711,13 @@ void CgSolver::FinishDeferred(CgResult& res) {
 }
 
 CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, int precision,
-                         bool project, Precond* pc, bool defer_final) {
+                         bool project, Precond* pc, bool defer_final, bool warm) {
   HG_CHECK(hipEventRecord(ev0_, stream_));
   CgResult res;
   if (pc)
-    res = SolveFlexible(b, x, tol, max_iters, project, pc);
+    res = SolveFlexible(b, x, tol, max_iters, project, pc, warm);
   else if (precision == 0)
-    res = SolveFp64(b, x, tol, max_iters, project);
+    res = SolveFp64(b, x, tol, max_iters, project, warm);
   else
     res = SolveMixed(b, x, tol, max_iters, project, nullptr);
   HG_CHECK(hipEventRecord(ev1_, stream_));
@@ -678,16 +727,23 @@ CgResult CgSolver::Solve(const double* b, double* x, double tol, int max_iters, 
   return res;
 }
 
-// b_ <- b (projected to zero mean when singular); x <- 0; r_ <- b_ (the
-// residual of x = 0); sc->bb = ||b_||^2, its partials in update slot 1.
-void CgSolver::Prepare(const double* b, double* x, bool project) {
+// b_ <- b (projected to zero mean when singular); sc->bb = ||b_||^2.  Cold:
+// x <- 0, r_ <- b_ (the residual of x = 0).  Warm: x as the caller left it,
+// r_ <- b_ - A x.  (r, r) partials in update slot 1.
+void CgSolver::Prepare(const double* b, double* x, bool project, bool warm) {
   int nb;
   Walk w = MakeWalk(nx_, ny_, &nb);
   Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
   if (project) KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
-  KStart<<<nb, kB, 0, stream_>>>(w, b, project ? 1 : 0, b_, r_, x, pt);
-  KTakeRR<<<1, kB, 0, stream_>>>(pt.upd[1] + nb, nb, sc, 1);
+  if (warm) {
+    KStartWarm<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b, project ? 1 : 0, b_, r_, x, pt);
+    KStartWarmScale<<<nb, kB, 0, stream_>>>(w, b_, r_, x, pt);
+    KTakeRR<<<1, kB, 0, stream_>>>(pt.spmv[0], nb, sc, 1);
+  } else {
+    KStart<<<nb, kB, 0, stream_>>>(w, b, project ? 1 : 0, b_, r_, x, pt);
+    KTakeRR<<<1, kB, 0, stream_>>>(pt.upd[1] + nb, nb, sc, 1);
+  }
 }
 
 // Jacobi-PCG on A x = b_ continuing from the current x and r_ (FP64).
@@ -715,10 +771,10 @@ int CgSolver::JacobiPcg64(double* x, int max_iters) {
 }
 
 CgResult CgSolver::SolveFp64(const double* b, double* x, double tol, int max_iters,
-                             bool project) {
+                             bool project, bool warm) {
   CgScalars* sc = (CgScalars*)scal_;
   KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
-  Prepare(b, x, project);
+  Prepare(b, x, project, warm);
   CgResult res;
   res.iterations = JacobiPcg64(x, max_iters);
   QueueFinalResidual(b_, x, r_);
@@ -729,14 +785,14 @@ CgResult CgSolver::SolveFp64(const double* b, double* x, double tol, int max_ite
 }
 
 CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max_iters,
-                                 bool project, Precond* pc) {
+                                 bool project, Precond* pc, bool warm) {
   int nb;
   Walk w = MakeWalk(nx_, ny_, &nb);
   Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
   CgScalars* hs = (CgScalars*)host_scal_;
   KSetup<<<1, 64, 0, stream_>>>(sc, tol * tol, max_iters);
-  Prepare(b, x, project);
+  Prepare(b, x, project, warm);
   double* pold = pa_;
   double* pnew = pb_;
   int k = 0;
@@ -832,7 +888,7 @@ CgResult CgSolver::SolveMixed(const double* b, double* x, double tol, int max_it
   CgScalars* sc = (CgScalars*)scal_;
   CgResult res;
   KSetup<<<1, 64, 0, stream_>>>(sc, 0., 1);
-  Prepare(b, x, project);
+  Prepare(b, x, project, false);
   ReadScalars();
   double bb = ((CgScalars*)host_scal_)->bb;
   if (bb == 0.) {

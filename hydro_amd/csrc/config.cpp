@@ -106,6 +106,7 @@ void hg_config_default(hg_config* c) {
   c->precision_gpu = HG_PREC_FP64;
   c->learned_levels_gpu = 4;
   c->learned_channels_gpu = 16;
+  c->pressure_warm_start_gpu = 1;
 }
 
 int32_t hg_solver_from_name(const char* name) {
@@ -163,6 +164,7 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "pressure_max_iters_gpu") c->pressure_max_iters_gpu = iv;
       else if (name == "learned_levels_gpu") c->learned_levels_gpu = iv;
       else if (name == "learned_channels_gpu") c->learned_channels_gpu = iv;
+      else if (name == "pressure_warm_start_gpu") c->pressure_warm_start_gpu = iv;
     } else if (type == "double") {
       double v;
       if (!(in >> v)) return fail(line_no, "bad double for " + name);

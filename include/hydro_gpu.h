@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define HG_ABI_VERSION 1
+#define HG_ABI_VERSION 2
 #define HG_MAX_PHASES 3
 
 typedef enum hg_status {
@@ -123,6 +123,8 @@ typedef struct hg_config {
   int32_t precision_gpu;               /* hg_precision */
   int32_t learned_levels_gpu;          /* levels of the learned preconditioner (4) */
   int32_t learned_channels_gpu;        /* channels at the finest level (16) */
+  int32_t pressure_warm_start_gpu;     /* 1: a step's first cg / fcg solve starts from the
+                                          previous step's correction (1) */
   /* Bits of hg_unsupported: features the script switched on that this path
    * does not run (hg_create refuses them with HG_ERR_UNSUPPORTED). */
   int32_t unsupported_flags;
