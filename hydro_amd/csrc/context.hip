@@ -173,11 +173,11 @@ struct hg_ctx {
   // since (StartStep need not check them again)
   bool fields_checked = false;
   // Warm start of the face-form pressure solve: pcorr holds the converged
-  // correction of the previous step's only SIMPLE iteration, which the next
-  // step's first solve starts from (successive steps' corrections are close:
-  // the pressure changes smoothly in time; at 1024x2048 RT this takes FCG
-  // from 8.1 to 7.6 iterations per step, and a linear extrapolation from
-  // the last two corrections gains nothing more).
+  // correction of the previous step's only SIMPLE iteration, a multiple of
+  // which the next step's first solve starts from (successive steps'
+  // corrections are close: the pressure changes smoothly in time; at
+  // 1024x2048 RT this takes FCG from 8.1 to 7.35 iterations per step, and a
+  // linear extrapolation from the last two corrections gains nothing more).
   bool pcorr_converged = false, pcorr_warm = false;
   void Sync() {
     HG_CHECK(hipStreamSynchronize(st));

@@ -21,10 +21,12 @@
 namespace hg {
 
 constexpr int kTallY = 16;  // tile height of the finest level's fused kernels
+constexpr int kMidY = 4;    // tile height of the coarser levels' fused kernels
 
 struct MgLevel {
   int nx, ny;
   TileGrid tg;    // 64 x 4 tiles
+  TileGrid mid;   // 64 x kMidY tiles
   TileGrid tall;  // 64 x kTallY tiles
   float *kx = nullptr, *ky = nullptr, *dinv = nullptr;
   float *r = nullptr, *xa = nullptr, *xb = nullptr;

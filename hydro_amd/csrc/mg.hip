@@ -694,6 +694,7 @@ MgPrecond::MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0) {
     L.nx = cx;
     L.ny = cy;
     L.tg = MakeTileGrid(cx, cy);
+    L.mid = MakeTileGrid(cx, cy, kMidY);
     L.tall = MakeTileGrid(cx, cy, kTallY);
     long n = long(cx) * cy;
     if (lv_.empty()) {
@@ -840,13 +841,13 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     return;
   }
   MgLevel& C = lv_[l + 1];
-  // level 0 (FP64 residual) on 64 x 16 tiles, the FP32 levels on 64 x 4
+  // level 0 (FP64 residual) on 64 x kTallY tiles, the FP32 levels on 64 x kMidY
   if (l == 0)
     KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
         L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
         C.r, done, fz, fused);
   else
-    KDown<float, kTY><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+    KDown<float, kMidY><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
                                                        L.r, Omega2{omega1, omega2}, L.xb, C.nx,
                                                        C.ny, C.r, done, fz, 0);
   Cycle(l + 1, nullptr, C.xb, st, done);
@@ -862,9 +863,9 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
       HG_UP(double, false, kTallY, L.tall, r64);
   } else {
     if (bilinear)
-      HG_UP(float, true, kTY, L.tg, L.r);
+      HG_UP(float, true, kMidY, L.mid, L.r);
     else
-      HG_UP(float, false, kTY, L.tg, L.r);
+      HG_UP(float, false, kMidY, L.mid, L.r);
   }
 #undef HG_UP
 }
