@@ -25,7 +25,7 @@ ARCH = "gfx950"
 
 # Parity-path sources evaluate every expression in the reference's order:
 # no contraction into FMAs.  The throughput solvers and the network allow it.
-EXACT = {"step_kernels.hip", "solvers.hip", "context.hip", "config.cpp"}
+EXACT = {"step_kernels.hip", "solvers.hip", "context.hip", "config.cpp", "output.cpp"}
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]

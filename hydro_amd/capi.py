@@ -1,4 +1,5 @@
-"""ctypes mirror of include/hydro_gpu.h (the C-ABI of libhydrogpu.so).
+"""ctypes mirror of include/hydro_gpu.h and include/hydro_output.h (the C-ABI
+of libhydrogpu.so).
 
 The structures below follow the header field by field; `HG_ABI_VERSION`
 guards against a stale library.  Everything here is plumbing: the compute
@@ -7,14 +8,14 @@ lives in the HIP kernels behind the C-ABI.
 import ctypes
 import os
 
-HG_ABI_VERSION = 2
+HG_ABI_VERSION = 3
 HG_MAX_PHASES = 3
 
 HG_OK, HG_ERR_NAN, HG_ERR_NOT_CONVERGED, HG_ERR_BAD_ARG, HG_ERR_HIP, \
-    HG_ERR_UNSUPPORTED, HG_ERR_PARSE = range(7)
+    HG_ERR_UNSUPPORTED, HG_ERR_PARSE, HG_ERR_IO = range(8)
 STATUS_NAMES = {0: "HG_OK", 1: "HG_ERR_NAN", 2: "HG_ERR_NOT_CONVERGED",
                 3: "HG_ERR_BAD_ARG", 4: "HG_ERR_HIP", 5: "HG_ERR_UNSUPPORTED",
-                6: "HG_ERR_PARSE"}
+                6: "HG_ERR_PARSE", 7: "HG_ERR_IO"}
 
 SOLVERS = {"lu": 0, "lu_relaxed": 1, "gauss_seidel": 2, "jacobi": 3,
            "lu_rb": 4, "cg": 5, "fcg_learned": 6, "fcg_mg": 7}
@@ -95,6 +96,7 @@ class HgConfig(ctypes.Structure):
         ("learned_levels_gpu", ctypes.c_int32),
         ("learned_channels_gpu", ctypes.c_int32),
         ("pressure_warm_start_gpu", ctypes.c_int32),
+        ("meshvel_output", ctypes.c_int32),
         ("unsupported_flags", ctypes.c_int32),
     ]
 
@@ -118,7 +120,40 @@ class HgStepStats(ctypes.Structure):
 
 class HgStats(ctypes.Structure):
     _fields_ = [(n, _d3) for n in ("volume", "mass", "pd_min", "pd_max",
-                                   "cx", "cy", "vx", "vy")]
+                                   "cx", "cy", "vx", "vy", "volume_in", "volume_out",
+                                   "mass_in", "mass_out")]
+
+
+class HgProgress(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("iteration", ctypes.c_int32),
+                ("convergence_indicator", ctypes.c_double), ("meshpos", _d2)]
+
+
+# hg_output_entry: the module's output entries in writing order
+OUTPUT_ENTRIES = ["x", "y", "z", "velocity_x", "velocity_y", "velocity_z", "pressure",
+                  "density", "viscosity", "radiation", "volume_source", "temperature",
+                  "excluded", "divergence"] + [
+    f"{base}{i}" for base in ("mass_source_", "mass_fraction_", "density_",
+                              "target_density_", "partial_density_", "volume_fraction_")
+    for i in range(HG_MAX_PHASES)]
+HG_OUT_COUNT = 32
+
+
+class HgOutputConfig(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_int32),
+                ("name", ctypes.c_char * 256), ("plt_title", ctypes.c_char * 256),
+                ("filename_field", ctypes.c_char * 512),
+                ("filename_scalar", ctypes.c_char * 512),
+                ("filename_iter_history", ctypes.c_char * 512),
+                ("field_output_format", ctypes.c_char * 32),
+                ("no_output", ctypes.c_int32), ("no_mesh_output", ctypes.c_int32),
+                ("max_frame_index", ctypes.c_int32),
+                ("max_frame_scalar_index", ctypes.c_int32),
+                ("output_factor", ctypes.c_int32 * 2), ("T", ctypes.c_double),
+                ("iter_history_enable", ctypes.c_int32), ("iter_history_n", ctypes.c_int32),
+                ("iter_history_sfixed", ctypes.c_int32),
+                ("output", ctypes.c_int32 * HG_OUT_COUNT)]
 
 
 class HgSolveOpts(ctypes.Structure):
@@ -151,8 +186,11 @@ class HgTrainStats(ctypes.Structure):
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libhydrogpu.so")
 
-# Every entry point include/hydro_gpu.h declares.
+# Every entry point include/hydro_gpu.h and include/hydro_output.h declare.
 EXPORTS = [
+    "hg_get_config", "hg_get_progress", "hg_get_iteration_history",
+    "hg_output_config_default", "hg_output_config_parse", "hg_output_open",
+    "hg_output_write_results", "hg_output_close", "hg_output_last_error", "hg_vts_write",
     "hg_config_default", "hg_config_parse", "hg_solver_from_name",
     "hg_create", "hg_destroy", "hg_last_error", "hg_num_cells", "hg_num_faces",
     "hg_set_field", "hg_get_field", "hg_refresh",
@@ -188,6 +226,20 @@ def _declare(lib):
         "hg_set_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
         "hg_get_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
         "hg_refresh": (ctypes.c_int, [vp]),
+        "hg_get_config": (ctypes.c_int, [vp, _P(HgConfig)]),
+        "hg_get_progress": (ctypes.c_int, [vp, _P(HgProgress)]),
+        "hg_get_iteration_history": (ctypes.c_int, [vp, _dp, ctypes.c_int32,
+                                                    _P(ctypes.c_int32)]),
+        "hg_output_config_default": (None, [_P(HgOutputConfig)]),
+        "hg_output_config_parse": (ctypes.c_int, [_P(HgOutputConfig), ctypes.c_char_p,
+                                                  ctypes.c_char_p, ctypes.c_size_t]),
+        "hg_output_open": (ctypes.c_int, [vp, _P(HgOutputConfig), _P(vp)]),
+        "hg_output_write_results": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "hg_output_close": (ctypes.c_int, [vp]),
+        "hg_output_last_error": (ctypes.c_char_p, [vp]),
+        "hg_vts_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
+                                        ctypes.c_int32, _P(ctypes.c_char_p), _P(_dp),
+                                        ctypes.c_int32, _P(ctypes.c_char_p), _P(_dp)]),
         "hg_fluid_start_step": (ctypes.c_int, [vp]),
         "hg_fluid_iterate": (ctypes.c_int, [vp, _P(HgIterStats)]),
         "hg_fluid_is_converged": (ctypes.c_int32, [vp]),
@@ -261,6 +313,49 @@ def parse_config(text, cfg=None, lib=None):
     if rc != HG_OK:
         raise ValueError(err.value.decode())
     return cfg
+
+
+def default_output_config(lib=None):
+    lib = lib or load()
+    oc = HgOutputConfig()
+    lib.hg_output_config_default(ctypes.byref(oc))
+    return oc
+
+
+def parse_output_config(text, oc=None, lib=None):
+    """Applies the output keys of .hydroconf `set` lines to oc (defaults
+    first if oc is None)."""
+    lib = lib or load()
+    oc = oc if oc is not None else default_output_config(lib)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.hg_output_config_parse(ctypes.byref(oc), text.encode(), err, 512)
+    if rc != HG_OK:
+        raise ValueError(err.value.decode())
+    return oc
+
+
+def vts_write(path, xn, yn, node_arrays, cell_arrays, lib=None):
+    """hg_vts_write: node_arrays / cell_arrays are lists of (name, float64
+    array) on the (len(xn)-1) x (len(yn)-1) grid."""
+    import numpy as np
+    lib = lib or load()
+    xn = np.ascontiguousarray(xn, dtype=np.float64)
+    yn = np.ascontiguousarray(yn, dtype=np.float64)
+    keep = []
+
+    def pack(arrays):
+        names = (ctypes.c_char_p * max(1, len(arrays)))(*[n.encode() for n, _ in arrays])
+        vals = [np.ascontiguousarray(a, dtype=np.float64) for _, a in arrays]
+        keep.extend(vals)
+        ptrs = (_dp * max(1, len(arrays)))(*[dptr(v) for v in vals])
+        return names, ptrs
+
+    nn, npp = pack(node_arrays)
+    cn, cp = pack(cell_arrays)
+    rc = lib.hg_vts_write(path.encode(), len(xn) - 1, len(yn) - 1, dptr(xn), dptr(yn),
+                          len(node_arrays), nn, npp, len(cell_arrays), cn, cp)
+    if rc != HG_OK:
+        raise RuntimeError(f"hg_vts_write: {STATUS_NAMES.get(rc, rc)}")
 
 
 def dptr(arr):

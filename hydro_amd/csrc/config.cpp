@@ -107,6 +107,7 @@ void hg_config_default(hg_config* c) {
   c->learned_levels_gpu = 4;
   c->learned_channels_gpu = 16;
   c->pressure_warm_start_gpu = 1;
+  c->meshvel_output = 1;
 }
 
 int32_t hg_solver_from_name(const char* name) {
@@ -208,6 +209,7 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "simpler") c->simpler = v;
       else if (name == "force_geometric_average") c->force_geometric_average = v;
       else if (name == "tvd_split") c->tvd_split = v;
+      else if (name == "meshvel_output") c->meshvel_output = v;
       else if (name == "dt_auto") flag(HG_UNSUP_DT_AUTO);
       else if (name == "compressible_enable") flag(HG_UNSUP_COMPRESSIBLE);
       else if (name == "heat_enable") flag(HG_UNSUP_HEAT);

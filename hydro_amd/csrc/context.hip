@@ -135,6 +135,16 @@ struct hg_ctx {
   int fluid_iter = 0, cd_iter = 0;
   double last_rs = 1.;
   hg_iter_stats last_iter{};
+  // the module state the output reads: "s" (the last iteration count), the
+  // steps finished, the Rs of the current step's iterations, and the running
+  // part of CalcStat (hydro2d.hpp:1474-1528): boundary in/out integrals per
+  // phase on the device (volume_in, volume_out, mass_in, mass_out), the
+  // output offset meshpos and the value of it the last statistics used
+  int last_s = 0;
+  long steps = 0;
+  std::vector<double> iter_rs;
+  double* bflux_acc = nullptr;
+  double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
   std::string err;
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
@@ -197,6 +207,7 @@ struct hg_ctx {
       std::memcpy(&rs, rs_h, 8);
       last_rs = rs;
       last_iter.convergence_indicator = rs;
+      if (!iter_rs.empty()) iter_rs.back() = rs;  // the iteration that deferred it
     }
   }
 };
@@ -238,8 +249,32 @@ void UpdateProperties(hg_ctx* c) {
   Smooth(c, c->fy, c->fy, c->cfg.force_smooth_times);
 }
 
+// The running part of CalcStat, run after every property update as the
+// reference's step runs CalcStat after UpdateFluidProperties
+// (hydro2d.hpp:1615-1617): the boundary in/out integrals over the
+// time_curr volume flux (hydro2d.hpp:1474-1508) and the output offset
+// meshpos (hydro2d.hpp:1526-1528).
+void AccumulateStats(hg_ctx* c) {
+  const double* al[3] = {c->alpha[0], c->alpha[1], c->alpha[2]};
+  hg::LaunchBoundaryFlux(c->st, c->geo, c->fl_tc, al, c->np, c->cfg.density, c->dt,
+                         c->bflux_acc);
+  for (int d = 0; d < 2; ++d) c->meshpos_stat[d] = c->meshpos[d];
+  if (c->cfg.meshvel_output)
+    for (int d = 0; d < 2; ++d) c->meshpos[d] += c->cfg.meshvel[d] * c->cfg.dt;
+}
+
+// Constructor state of the running statistics, then the constructor's
+// CalcStat (hydro2d.hpp:966-967).
+void InitStats(hg_ctx* c) {
+  HG_CHECK(hipMemsetAsync(c->bflux_acc, 0, 4 * HG_MAX_PHASES * sizeof(double), c->st));
+  c->meshpos[0] = c->meshpos[1] = 0.;
+  AccumulateStats(c);
+}
+
 void CalcStats(hg_ctx* c, hg_stats* s) {
   std::memset(s, 0, sizeof(*s));
+  HG_CHECK(hipMemcpyAsync(c->stat_h + 8, c->bflux_acc, 4 * HG_MAX_PHASES * 8,
+                          hipMemcpyDeviceToHost, c->st));
   for (int i = 0; i < c->np; ++i) {
     hg::LaunchStats(c->st, c->geo, c->xn_d, c->yn_d, c->alpha[i], c->pd[i], c->u_tc[0],
                     c->u_tc[1], c->stat_part, c->stat_out);
@@ -249,12 +284,17 @@ void CalcStats(hg_ctx* c, hg_stats* s) {
     double volume = v[0];
     s->volume[i] = volume;
     s->mass[i] = volume * c->cfg.density[i];
-    s->cx[i] = v[1] / volume;
-    s->cy[i] = v[2] / volume;
+    s->cx[i] = v[1] / volume + c->meshpos_stat[0];
+    s->cy[i] = v[2] / volume + c->meshpos_stat[1];
     s->vx[i] = v[3] / volume;
     s->vy[i] = v[4] / volume;
     s->pd_min[i] = v[5];
     s->pd_max[i] = v[6];
+    const double* acc = c->stat_h + 8 + 4 * i;
+    s->volume_in[i] = acc[0];
+    s->volume_out[i] = acc[1];
+    s->mass_in[i] = acc[2];
+    s->mass_out[i] = acc[3];
   }
 }
 
@@ -275,6 +315,7 @@ bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
 void StartStep(hg_ctx* c) {
   c->fluid_iter = 0;
   c->cd_iter = 0;
+  c->iter_rs.clear();
   if (!c->fields_checked && NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
     throw HgError(HG_ERR_NAN, "NaN initial pressure");
   c->fields_checked = false;
@@ -474,7 +515,9 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
   HG_CHECK(hipMemcpyAsync(c->rs_h, c->rs_d, 8, hipMemcpyDeviceToHost, c->st));
   c->pending_rs = true;
   c->fluid_iter += 1;
+  c->last_s = c->fluid_iter;
   is->iteration = c->fluid_iter;
+  c->iter_rs.push_back(0.);  // set when the indicator settles
   if (!defer_rs) c->Sync();  // settles the indicator into last_rs / last_iter
   *out = *is;
 }
@@ -499,6 +542,7 @@ void FinishStep(hg_ctx* c) {
   Rotate(c->fl_tp, c->fl_tc, c->fl_ic, c->fl_spare);
   for (int d = 0; d < 2; ++d) Rotate(c->u_tp[d], c->u_tc[d], c->u_ic[d], c->u_spare[d]);
   c->t_fluid += c->dt;
+  c->steps += 1;
   if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
   c->fields_checked = true;
   // (NanCheck synchronised: the step's last pressure solve is settled)
@@ -615,6 +659,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     c->yn_d = c->Alloc(cfg.ny + 1);
     c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
     c->stat_out = c->Alloc(8);
+    c->bflux_acc = c->Alloc(4 * HG_MAX_PHASES);
     c->rs_part = c->Alloc((nc + 255) / 256);
     c->rs_d = (unsigned long long*)c->Alloc(1);
     c->nan_d = (int*)c->Alloc(1);
@@ -626,7 +671,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     HG_CHECK(hipHostMalloc((void**)&c->rs_h, 8));
     HG_CHECK(hipHostMalloc((void**)&c->flag_h, 4 * sizeof(int)));
     HG_CHECK(hipHostMalloc((void**)&c->info_h, 16));
-    HG_CHECK(hipHostMalloc((void**)&c->stat_h, 8 * 8));
+    HG_CHECK(hipHostMalloc((void**)&c->stat_h, (8 + 4 * HG_MAX_PHASES) * 8));
     const int32_t lsp = cfg.linear_solver_pressure;
     if (lsp == HG_SOLVER_CG || lsp == HG_SOLVER_FCG_LEARNED || lsp == HG_SOLVER_FCG_MG)
       c->cg.reset(new hg::CgSolver(cfg.nx, cfg.ny, c->st));
@@ -705,6 +750,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     hg::LaunchPhase0Init(c->st, c->geo, c->ph, c->pd[1], c->pd[2], c->pd[0]);
     InitialFlux(c.get());
     UpdateProperties(c.get());
+    InitStats(c.get());
     c->Sync();
   });
   if (rc != HG_OK) {
@@ -793,12 +839,41 @@ hg_status hg_get_field(hg_ctx* c, int32_t field, double* host, size_t n) {
   });
 }
 
+hg_status hg_get_config(const hg_ctx* c, hg_config* cfg) {
+  if (!c || !cfg) return HG_ERR_BAD_ARG;
+  *cfg = c->cfg;
+  return HG_OK;
+}
+
+hg_status hg_get_progress(hg_ctx* c, hg_progress* out) {
+  if (!c || !out) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    c->Sync();  // settles a deferred indicator
+    out->time = c->t_fluid;
+    out->steps = c->steps;
+    out->iteration = c->last_s;
+    out->convergence_indicator = (c->cd_iter == 0) ? 1. : c->last_rs;
+    out->meshpos[0] = c->meshpos[0];
+    out->meshpos[1] = c->meshpos[1];
+  });
+}
+
+hg_status hg_get_iteration_history(hg_ctx* c, double* rs, int32_t cap, int32_t* n) {
+  if (!c || !n || cap < 0 || (cap > 0 && !rs)) return HG_ERR_BAD_ARG;
+  return Guard(c, [&]() {
+    c->Sync();
+    *n = int32_t(c->iter_rs.size());
+    for (int32_t k = 0; k < std::min(*n, cap); ++k) rs[k] = c->iter_rs[k];
+  });
+}
+
 hg_status hg_refresh(hg_ctx* c) {
   if (!c) return HG_ERR_BAD_ARG;
   return Guard(c, [&]() {
     c->fields_checked = false;
     InitialFlux(c);
     UpdateProperties(c);
+    InitStats(c);
     c->Sync();
   });
 }
@@ -833,6 +908,7 @@ hg_status hg_update_properties(hg_ctx* c) {
   if (!c) return HG_ERR_BAD_ARG;
   return Guard(c, [&]() {
     UpdateProperties(c);
+    AccumulateStats(c);
     c->Sync();
   });
 }
@@ -870,6 +946,7 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     }
     HG_CHECK(hipEventRecord(c->ev[2], c->st));
     UpdateProperties(c);
+    AccumulateStats(c);
     HG_CHECK(hipEventRecord(c->ev[3], c->st));
     c->Sync();
     float a = 0, b = 0, d = 0;

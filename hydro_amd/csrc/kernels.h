@@ -68,6 +68,11 @@ void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const doub
                       const double* pd2, double* pd0);
 void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
                     int* flag);
+// CalcStat's boundary in/out integrals: acc[4p..4p+3] (volume in, out, mass
+// in, out of phase p) += dt * the boundary fluxes of the time_curr flux.
+void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
+                        const double* const alpha[3], int np, const double density[3], double dt,
+                        double* acc);
 int StatsBlocks(const Geo& g);
 void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,
                  const double* alpha, const double* pd, const double* u0, const double* u1,

@@ -787,6 +787,69 @@ __global__ void KStatsFinal(const double* __restrict__ part, int nblocks, double
   out[q] = x;
 }
 
+// Boundary in/out integrals of CalcStat (hydro2d.hpp:1474-1508), one
+// workgroup: per phase, the volume flux into the domain through boundary
+// faces times the cell's volume fraction (and out of it), then the four
+// running totals += flux * dt (volume and mass in and out).  The boundary
+// faces are taken left, right (x-faces i = 0, nx), bottom, top (y-faces
+// j = 0, ny); the flux toward the inner cell is +flux on left and bottom
+// faces, -flux on right and top faces (GetValidNeighbourCellId 1 / 0).
+struct PhaseCells {
+  const double* alpha[3];
+  double density[3];
+};
+
+__global__ void __launch_bounds__(kBlock)
+KBoundaryFlux(Geo g, const double* __restrict__ flux, PhaseCells pc, int np, double dt,
+              double* __restrict__ acc) {
+  const int nb = 2 * (g.nx + g.ny);
+  double in[3] = {0., 0., 0.}, out[3] = {0., 0., 0.};
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+    int face, cell;
+    double factor;
+    if (b < 2 * g.ny) {
+      const int j = b < g.ny ? b : b - g.ny;
+      const bool left = b < g.ny;
+      face = XFace(g, left ? 0 : g.nx, j);
+      cell = (left ? 0 : g.nx - 1) + g.nx * j;
+      factor = left ? 1. : -1.;
+    } else {
+      const int r = b - 2 * g.ny;
+      const int i = r < g.nx ? r : r - g.nx;
+      const bool bottom = r < g.nx;
+      face = YFace(g, i, bottom ? 0 : g.ny);
+      cell = i + g.nx * (bottom ? 0 : g.ny - 1);
+      factor = bottom ? 1. : -1.;
+    }
+    const double to_cell = flux[face] * factor;
+    for (int p = 0; p < 3; ++p) {
+      if (p >= np) break;
+      const double vf = pc.alpha[p][cell];
+      if (to_cell >= 0) in[p] += to_cell * vf;
+      else out[p] -= to_cell * vf;
+    }
+  }
+  __shared__ double red[6][kBlock / 64];
+  for (int q = 0; q < 6; ++q) {
+    const double x = WaveSum(q < 3 ? in[q] : out[q - 3]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < np) {
+    const int p = threadIdx.x;
+    double vin = red[p][0], vout = red[p + 3][0];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      vin += red[p][w];
+      vout += red[p + 3][w];
+    }
+    const double min = vin * pc.density[p], mout = vout * pc.density[p];
+    acc[4 * p + 0] += vin * dt;
+    acc[4 * p + 1] += vout * dt;
+    acc[4 * p + 2] += min * dt;
+    acc[4 * p + 3] += mout * dt;
+  }
+}
+
 }  // namespace
 
 // ---- launchers --------------------------------------------------------------
@@ -886,6 +949,16 @@ void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const doub
 void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
                     int* flag) {
   KNanCheck<<<Grid(n), kBlock, 0, st>>>(a, b, c, n, flag);
+}
+void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
+                        const double* const alpha[3], int np, const double density[3], double dt,
+                        double* acc) {
+  PhaseCells pc{};
+  for (int p = 0; p < np; ++p) {
+    pc.alpha[p] = alpha[p];
+    pc.density[p] = density[p];
+  }
+  KBoundaryFlux<<<1, kBlock, 0, st>>>(g, flux, pc, np, dt, acc);
 }
 int StatsBlocks(const Geo& g) { return int((long(g.nx) * g.ny + kBlock - 1) / kBlock); }
 void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,

@@ -127,6 +127,31 @@ class Hydro:
                "calc_stats")
         return st
 
+    def progress(self):
+        """Fluid time, steps finished, "s", convergence indicator, meshpos."""
+        pr = capi.HgProgress()
+        _raise(self.lib, self.ctx, self.lib.hg_get_progress(self.ctx, ctypes.byref(pr)),
+               "hg_get_progress")
+        return pr
+
+    def iteration_history(self):
+        """Rs of the SIMPLE iterations of the current (or last) step."""
+        n = ctypes.c_int32()
+        _raise(self.lib, self.ctx,
+               self.lib.hg_get_iteration_history(self.ctx, None, 0, ctypes.byref(n)),
+               "hg_get_iteration_history")
+        rs = np.zeros(max(n.value, 1))
+        _raise(self.lib, self.ctx,
+               self.lib.hg_get_iteration_history(self.ctx, capi.dptr(rs), n.value,
+                                                 ctypes.byref(n)),
+               "hg_get_iteration_history")
+        return rs[:n.value]
+
+    def open_output(self, config):
+        """An output session on this case (hydro::InitOutput): `config` is
+        .hydroconf text or an HgOutputConfig."""
+        return Output(self, config)
+
     def train_pressure_net(self, rhs_count=None, mg_iters=None, steps=None, lr=None,
                            seed=None, flags=None):
         """Trains the learned pressure preconditioner on the operator of the
@@ -149,6 +174,55 @@ class Hydro:
         self.lib.hg_last_step_timing(self.ctx, ctypes.byref(f), ctypes.byref(a), ctypes.byref(p))
         return {"step.fluid": f.value, "step.advection": a.value,
                 "step.fluid_properties": p.value}
+
+
+class Output:
+    """The module's output sessions over a Hydro (hg_output_*): VTS/PVD
+    frames, the scalar .dat table and the iteration history, written by
+    `write_results(force)` as hydro::write_results (hydro2d.hpp:1623-1652)."""
+
+    def __init__(self, hydro, config):
+        self.lib = hydro.lib
+        self.hydro = hydro
+        if isinstance(config, str):
+            config = capi.parse_output_config(config, lib=self.lib)
+        self.oc = config
+        self.h = ctypes.c_void_p()
+        rc = self.lib.hg_output_open(hydro.ctx, ctypes.byref(config), ctypes.byref(self.h))
+        if rc != capi.HG_OK:
+            self.h = ctypes.c_void_p()
+            self._fail(rc, "hg_output_open", None)
+
+    def _fail(self, rc, what, h):
+        msg = self.lib.hg_output_last_error(h).decode()
+        text = f"{what}: {capi.STATUS_NAMES.get(rc, rc)} {msg}".strip()
+        if rc == capi.HG_ERR_UNSUPPORTED:
+            raise Unsupported(text)
+        raise HydroError(text)
+
+    def write_results(self, force=False):
+        rc = self.lib.hg_output_write_results(self.h, 1 if force else 0)
+        if rc != capi.HG_OK:
+            self._fail(rc, "hg_output_write_results", self.h)
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, ctypes.c_void_p()
+            rc = self.lib.hg_output_close(h)
+            if rc != capi.HG_OK:
+                self._fail(rc, "hg_output_close", None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def stencil_solve(nx, ny, s, w, c, e, n, k, solver, tol=0.0, max_iters=100, omega=1.0,

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define HG_ABI_VERSION 2
+#define HG_ABI_VERSION 3
 #define HG_MAX_PHASES 3
 
 typedef enum hg_status {
@@ -32,7 +32,8 @@ typedef enum hg_status {
   HG_ERR_BAD_ARG = 3,
   HG_ERR_HIP = 4,
   HG_ERR_UNSUPPORTED = 5,    /* a configuration this build does not run */
-  HG_ERR_PARSE = 6           /* .hydroconf text could not be read */
+  HG_ERR_PARSE = 6,          /* .hydroconf text could not be read */
+  HG_ERR_IO = 7              /* an output file could not be written */
 } hg_status;
 
 /* Linear solver names: the values of linear_solver_velocity /
@@ -125,6 +126,9 @@ typedef struct hg_config {
   int32_t learned_channels_gpu;        /* channels at the finest level (16) */
   int32_t pressure_warm_start_gpu;     /* 1: a step's first cg / fcg solve starts from the
                                           previous step's correction (1) */
+  /* output side of the module state */
+  int32_t meshvel_output;              /* meshvel_output: the output mesh moves with meshvel
+                                          (hydro2d.hpp:1526-1528) (1) */
   /* Bits of hg_unsupported: features the script switched on that this path
    * does not run (hg_create refuses them with HG_ERR_UNSUPPORTED). */
   int32_t unsupported_flags;
@@ -188,9 +192,26 @@ typedef struct hg_stats {           /* CalcStat (hydro2d.hpp:1432-1529), per pha
   double mass[HG_MAX_PHASES];
   double pd_min[HG_MAX_PHASES];
   double pd_max[HG_MAX_PHASES];
-  double cx[HG_MAX_PHASES], cy[HG_MAX_PHASES];
+  double cx[HG_MAX_PHASES], cy[HG_MAX_PHASES];   /* + the output offset meshpos */
   double vx[HG_MAX_PHASES], vy[HG_MAX_PHASES];
+  /* volume and mass through the boundary, in and out, integrated in time:
+   * every property update adds dt * sum over boundary faces of the flux
+   * into (out of) the domain times the cell's volume fraction
+   * (hydro2d.hpp:1474-1508); zero on walls at rest */
+  double volume_in[HG_MAX_PHASES], volume_out[HG_MAX_PHASES];
+  double mass_in[HG_MAX_PHASES], mass_out[HG_MAX_PHASES];
 } hg_stats;
+
+typedef struct hg_progress {        /* what the module's output reads besides fields */
+  double time;                      /* fluid time */
+  int64_t steps;                    /* steps finished (FinishStep calls) */
+  int32_t iteration;                /* "s": SIMPLE iteration count of the last iteration
+                                       made (hydro2d.hpp:1580-1581); 0 before any */
+  double convergence_indicator;     /* FluidSimple::GetConvergenceIndicator: 1 before the
+                                       first SIMPLE iteration of a step, else the last Rs
+                                       (fluid.hpp:174-179,1169-1171) */
+  double meshpos[2];                /* offset of the output mesh (hydro2d.hpp:1526-1528) */
+} hg_progress;
 
 /* ---- configuration (the .hydroconf interface) ------------------------- */
 
@@ -221,6 +242,15 @@ int32_t hg_num_faces(const hg_ctx* ctx);
 hg_status hg_set_field(hg_ctx* ctx, int32_t field, const double* host, size_t n);
 hg_status hg_get_field(hg_ctx* ctx, int32_t field, double* host, size_t n);
 hg_status hg_refresh(hg_ctx* ctx);
+/* The configuration the context was created with. */
+hg_status hg_get_config(const hg_ctx* ctx, hg_config* cfg);
+/* Time, step and iteration counters, convergence indicator, mesh offset. */
+hg_status hg_get_progress(hg_ctx* ctx, hg_progress* out);
+/* Convergence indicators Rs of the SIMPLE iterations of the current step (of
+ * the last one once it finished), in order: *n receives their number, and
+ * the first min(*n, cap) go to rs (rs may be NULL with cap 0).  The
+ * reference logs them per iteration (hydro2d.hpp:1583-1590). */
+hg_status hg_get_iteration_history(hg_ctx* ctx, double* rs, int32_t cap, int32_t* n);
 
 /* ---- stepping at the reference's granularity --------------------------- */
 
@@ -232,9 +262,11 @@ int32_t hg_fluid_is_converged(const hg_ctx* ctx);
 hg_status hg_fluid_finish_step(hg_ctx* ctx);
 /* One AdvectionSolverMultiExplicit sub-step (advection.hpp:417-545). */
 hg_status hg_advect_substep(hg_ctx* ctx);
-/* hydro::UpdateFluidProperties (hydro2d.hpp:1404-1430). */
+/* hydro::UpdateFluidProperties (hydro2d.hpp:1404-1430) and the running part
+ * of hydro::CalcStat that follows it in the step (the boundary in/out
+ * integrals and meshpos, hydro2d.hpp:1474-1528). */
 hg_status hg_update_properties(hg_ctx* ctx);
-/* hydro::CalcStat reductions (hydro2d.hpp:1432-1529). */
+/* hydro::CalcStat reductions (hydro2d.hpp:1432-1529) of the current state. */
 hg_status hg_calc_stats(hg_ctx* ctx, hg_stats* stats);
 /* hydro::step (hydro2d.hpp:1531-1621): SIMPLE loop, advection sub-steps,
  * property update, stats. */

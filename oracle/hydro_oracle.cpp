@@ -597,10 +597,10 @@ struct or_ctx {
   Sys p_sys;
   Vec a_p;  // diagonal coefficient
   Timers timers;
-  hg_stats stats_prev;
-  bool have_cx = false;
-  double cx_prev[HG_MAX_PHASES];
-  double stat_in[HG_MAX_PHASES], stat_out[HG_MAX_PHASES];
+  // running part of CalcStat: boundary volume / mass in and out per phase,
+  // the output offset meshpos and the value the last statistics used
+  double bacc[HG_MAX_PHASES][4] = {};
+  double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
 };
 
 namespace {
@@ -706,6 +706,37 @@ void UpdateProperties(or_ctx* o) {
 
 extern "C" {
 
+// The running part of hydro::CalcStat (hydro2d.hpp:1474-1528), after every
+// property update: per phase, the boundary faces in face order (inner faces
+// skipped), the time_curr flux toward the valid neighbour cell (factor +1
+// when it is cell 1, -1 when cell 0) times that cell's volume fraction,
+// summed into "in" when >= 0 and subtracted into "out" otherwise; then the
+// totals += flux * dt.  Last, meshpos += meshvel * dt (meshvel_output).
+static void RunningStats(or_ctx* o) {
+  const Mesh& m = o->m;
+  for (int i = 0; i < o->np; ++i) {
+    double vin = 0., vout = 0.;
+    for (size_t f = 0; f < m.nf; ++f) {
+      if (m.Inner(f)) continue;
+      const int id = m.ValidId(f);
+      const size_t c = size_t(id == 1 ? m.fcp[f] : m.fcm[f]);
+      const double factor = id == 1 ? 1. : -1.;
+      const double to_cell = o->fl_tc[f] * factor;
+      const double vf = o->alpha[i][c];
+      if (to_cell >= 0) vin += to_cell * vf;
+      else vout -= to_cell * vf;
+    }
+    const double min = vin * o->cfg.density[i], mout = vout * o->cfg.density[i];
+    o->bacc[i][0] += vin * o->dt;
+    o->bacc[i][1] += vout * o->dt;
+    o->bacc[i][2] += min * o->dt;
+    o->bacc[i][3] += mout * o->dt;
+  }
+  for (int d = 0; d < 2; ++d) o->meshpos_stat[d] = o->meshpos[d];
+  if (o->cfg.meshvel_output)
+    for (int d = 0; d < 2; ++d) o->meshpos[d] += o->cfg.meshvel[d] * o->cfg.dt;
+}
+
 int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
   auto fail = [&](const char* msg) {
     if (err && errlen) std::snprintf(err, errlen, "%s", msg);
@@ -782,8 +813,7 @@ int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
     o->alpha[i].assign(nc, 0.);
   }
   UpdateProperties(o);
-  hg_stats st;
-  or_calc_stats(o, &st);
+  RunningStats(o);
   *out = o;
   return HG_OK;
 }
@@ -834,9 +864,10 @@ int or_get_field(or_ctx* o, int field, double* host, size_t n) {
 int or_refresh(or_ctx* o) {
   InitialFlux(o);
   UpdateProperties(o);
-  o->have_cx = false;
-  hg_stats st;
-  return or_calc_stats(o, &st);
+  std::memset(o->bacc, 0, sizeof(o->bacc));
+  o->meshpos[0] = o->meshpos[1] = 0.;
+  RunningStats(o);
+  return HG_OK;
 }
 
 // FluidSimple::StartStep (fluid.hpp:793-812) with the component solvers'
@@ -1354,6 +1385,7 @@ int or_advect_substep(or_ctx* o) {
 
 int or_update_properties(or_ctx* o) {
   UpdateProperties(o);
+  RunningStats(o);
   return HG_OK;
 }
 
@@ -1380,8 +1412,12 @@ int or_calc_stats(or_ctx* o, hg_stats* st) {
     st->mass[i] = volume * o->cfg.density[i];
     st->pd_min[i] = pdmin;
     st->pd_max[i] = pdmax;
-    st->cx[i] = cen.x / volume;
-    st->cy[i] = cen.y / volume;
+    st->cx[i] = cen.x / volume + o->meshpos_stat[0];
+    st->cy[i] = cen.y / volume + o->meshpos_stat[1];
+    st->volume_in[i] = o->bacc[i][0];
+    st->volume_out[i] = o->bacc[i][1];
+    st->mass_in[i] = o->bacc[i][2];
+    st->mass_out[i] = o->bacc[i][3];
     st->vx[i] = vel.x / volume;
     st->vy[i] = vel.y / volume;
   }
@@ -1422,8 +1458,7 @@ int or_step(or_ctx* o, hg_step_stats* stats) {
   {
     ScopedTimer t(&o->timers, "step.fluid_properties");
     UpdateProperties(o);
-    hg_stats st;
-    or_calc_stats(o, &st);
+    RunningStats(o);
   }
   s.time = o->t_fluid;
   if (stats) *stats = s;

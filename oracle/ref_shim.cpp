@@ -16,6 +16,10 @@
 //     header: int32 nx, ny, num_phases, step; then float64 arrays
 //     u[N], v[N], p[N], flux[F], then alpha_i[N], pd_i[N] per phase
 //   and prints one line per step: "STEP s=<n> iters=<SIMPLE iters> t=<seconds>"
+//   dump-every < 0: output mode, no dumps; after each step the module time
+//   advances and hydro::write_results runs as in TModule::cycle, so the
+//   module's own writers produce their files in the working directory (the
+//   run stops early once the time reaches T, like the reference).
 
 #include <chrono>
 #include <cstdio>
@@ -59,6 +63,14 @@ using AdvPtr = std::shared_ptr<solver::AdvectionSolverMulti<Mesh, geom::FieldFac
 HR_MEMBER(TagAdv, AdvPtr, advection_solver)
 HR_MEMBER(TagFrac, std::vector<geom::FieldCell<double>>, v_fc_volume_fraction)
 HR_MEMBER(TagNp, const size_t, num_phases)
+HR_MEMBER(TagSession, std::shared_ptr<output::Session>, session)
+HR_MEMBER(TagSessionScalar, std::shared_ptr<output::Session>, session_scalar)
+// TModule::increase_time (protected; experiment.hpp:109)
+struct TagIncTime {
+  typedef void (TModule::*type)();
+  friend type Get(TagIncTime);
+};
+template struct Grant<TagIncTime, &TModule::increase_time>;
 
 static bool ApplySet(TExperiment& ex, const std::string& line) {
   std::stringstream buf(line);
@@ -147,6 +159,20 @@ int main(int argc, char** argv) {
       std::fflush(stdout);
       if (every > 0 && prefix != "-" && (s % every == 0 || s == nsteps))
         Dump(h, prefix + "." + std::to_string(s) + ".bin", s);
+      if (every < 0) {
+        // output mode: the rest of TModule::cycle (module.cpp:46-73) --
+        // advance the module time and let the module write its results
+        (h.*Get(TagIncTime()))();
+        bool finished = h.tn >= ex.P_double["T"];
+        h.write_results(finished);
+        if (finished) break;
+      }
+    }
+    if (every < 0) {
+      // close the sessions: the ParaView collection's footer is written by
+      // its destructor (output_paraview.hpp:158-160)
+      (h.*Get(TagSession())).reset();
+      (h.*Get(TagSessionScalar())).reset();
     }
     // TExperiment's destructor expects the console that owns it; this
     // driver has none, so leave without running destructors.

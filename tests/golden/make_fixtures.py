@@ -12,11 +12,19 @@ not needed afterwards).
 3. ref_*.json: the same for long trajectories, stored compactly: the
    SHA-256 of each FP64 field's bytes (bit-for-bit identity) and its L2 norm
    and maximum (to report how far a mismatch is), per dumped step.
+4. ref_output_*.npz: every file the reference module's own writers produce
+   (ParaView .vts frames and .pvd collection, scalar .dat table, Tecplot
+   iteration history) for a case run as the reference console runs it
+   (step, advance the time, write_results), with the case and output keys.
+5. sample_*.gz: the field frame and iteration history the reference ships
+   with its cavity sample run (examples/cavity/sample), gzip-compressed:
+   data only.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [output]   (output: 4 and 5 only)
 """
 import os
 import re
+import gzip
 import subprocess
 import sys
 import tempfile
@@ -116,9 +124,87 @@ def reference_hashes(name, text, steps, dumps):
         print(f"{name}: {steps} steps, SIMPLE iterations total {sum(out['simple_iters'])}")
 
 
+# Output keys of the output fixtures: several frames and scalar rows, every
+# entry the GPU path can supply, an iteration history.
+DAM_OUTPUT = """
+set string name exp
+set string plt_title Dam_32x16
+set double T 0.0008
+set int max_frame_index 3
+set int max_frame_scalar_index 5
+set int output_factor_x 2
+set bool output_z 1
+set bool output_velocity_z 1
+set bool output_density 1
+set bool output_viscosity 1
+set bool output_radiation 1
+set bool output_excluded 1
+set bool output_divergence 1
+set bool output_mass_fraction_0 1
+set bool output_mass_fraction_1 1
+set bool output_density_0 1
+set bool output_density_1 1
+set bool output_target_density_0 1
+set bool output_target_density_1 1
+set bool output_partial_density_0 1
+set bool output_partial_density_1 1
+set bool iter_history_enable 1
+set int iter_history_n 2
+"""
+CAVITY_MESHVEL_OUTPUT = """
+set string name cav
+set string plt_title Cavity_16_meshvel
+set double T 0.03
+set int max_frame_index 2
+set int max_frame_scalar_index 10
+set int output_factor_y 2
+set bool output_divergence 1
+set bool iter_history_enable 1
+set int iter_history_n 0
+"""
+OUTPUT = {
+    # name: (case text, output keys, step limit; the run ends at T)
+    "ref_output_dam32x16": (cases.dam(32, 16), DAM_OUTPUT, 100),
+    "ref_output_cavity16_meshvel": (
+        cases.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
+}
+
+
+def reference_output(name, text, out_text, steps):
+    script = open(GENERAL).read() + "\n" + text + "\n" + out_text
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.conf")
+        open(path, "w").write(script)
+        r = subprocess.run([pyoracle.REF_BIN, path, str(steps), "-", "-1"],
+                           capture_output=True, text=True, cwd=tmp)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        files = sorted(f for f in os.listdir(tmp) if f != "case.conf")
+        contents = [open(os.path.join(tmp, f)).read() for f in files]
+        nsteps = len(re.findall(r"STEP s=", r.stdout))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), text=np.array(text),
+                            output=np.array(out_text), steps=np.array(nsteps),
+                            files=np.array(files), contents=np.array(contents))
+        print(f"{name}: {nsteps} steps, files {files}")
+
+
+def sample_files():
+    for f in ("exp.field.0.vts", "exp.iter_history.plt"):
+        data = open(os.path.join(os.path.dirname(SAMPLE_LOG), f), "rb").read()
+        with open(os.path.join(HERE, "sample_" + f + ".gz"), "wb") as raw, \
+                gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as g:
+            g.write(data)
+        print(f"sample_{f}.gz: {len(data)} bytes")
+
+
 def main():
     if not os.path.exists(pyoracle.REF_BIN):
         raise SystemExit("build oracle/_ref first: make -f oracle/ref.mk")
+    sample_files()
+    for name, (text, out_text, steps) in OUTPUT.items():
+        reference_output(name, text, out_text, steps)
+    if sys.argv[1:] == ["output"]:
+        return
     sample()
     for name, (text, steps, dumps) in FIXTURES.items():
         reference_dump(name, text, steps, dumps)

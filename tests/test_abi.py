@@ -1,7 +1,7 @@
 """The C-ABI library and its host logic, without a GPU.
 
-* libhydrogpu.so loads and exports every function include/hydro_gpu.h
-  declares (no compute calls here).
+* libhydrogpu.so loads and exports every function include/hydro_gpu.h and
+  include/hydro_output.h declare (no compute calls here).
 * The .hydroconf reader keeps the reference's defaults and semantics.
 * Calls that need a device fail loudly when none is visible.
 """
@@ -15,11 +15,11 @@ from hydro_amd import capi
 from tests.conftest import gpu_available
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "hydro_gpu.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("hydro_gpu.h", "hydro_output.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(hg_[a-z0-9_]+)\s*\(", text)))
 

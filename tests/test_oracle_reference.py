@@ -6,7 +6,9 @@
   and the same number of SOR/Jacobi sweeps per pressure solve.
 * The reference's shipped sample run (examples/cavity/sample/exp.log): the
   1000 per-iteration convergence indicators Rs, printed with 6 significant
-  digits, must be reproduced to that precision.
+  digits, must be reproduced to that precision, and the final fields written
+  through the library's .vts writer must reproduce the sample's field file
+  (exp.field.0.vts) byte for byte.
 """
 import os
 
@@ -51,7 +53,7 @@ def test_oracle_matches_reference_fixture(name):
     assert pos == len(ref_sweeps)
 
 
-def test_oracle_reproduces_cavity_sample_log():
+def test_oracle_reproduces_cavity_sample_log(tmp_path):
     text = open(os.path.join(GOLDEN, "cavity_sample.hydroconf")).read()
     rs_ref = np.loadtxt(os.path.join(GOLDEN, "cavity_sample_rs.txt"))
     o = pyoracle.Oracle(capi.parse_config(text))
@@ -63,6 +65,30 @@ def test_oracle_reproduces_cavity_sample_log():
     rel = np.abs(np.array(out) - rs_ref) / rs_ref
     # 6 significant digits in the log: rounding error <= 5e-6 relative
     assert rel.max() < 5e-6, rel.max()
+    # ... and as printed, every one of them
+    assert ["%g" % v for v in out] == ["%g" % v for v in rs_ref]
+    o.finish_step()
+    check_sample_frame(o, tmp_path)
+
+
+def check_sample_frame(o, tmp_path):
+    """The sample's field frame (examples/cavity/sample/exp.field.0.vts,
+    written by the reference at the end of the run) byte for byte from the
+    oracle's fields through hg_vts_write: node x, y; cell velocity, pressure,
+    volume fraction (the sample's output_* flags)."""
+    import gzip
+    want = gzip.open(os.path.join(GOLDEN, "sample_exp.field.0.vts.gz"), "rt").read()
+    n = o.cfg.nx
+    xn = np.array([o.cfg.a[0] + (i / n) * (o.cfg.b[0] - o.cfg.a[0]) for i in range(n + 1)])
+    yn = np.array([o.cfg.a[1] + (j / n) * (o.cfg.b[1] - o.cfg.a[1]) for j in range(n + 1)])
+    gx, gy = np.meshgrid(xn, yn)
+    path = str(tmp_path / "exp.field.0.vts")
+    capi.vts_write(path, xn, yn, [("x", gx.ravel()), ("y", gy.ravel())],
+                   [("velocity_x", o.get(capi.HG_FIELD_VELOCITY_X)),
+                    ("velocity_y", o.get(capi.HG_FIELD_VELOCITY_Y)),
+                    ("pressure", o.get(capi.HG_FIELD_PRESSURE)),
+                    ("volume_fraction_0", o.get(capi.HG_FIELD_VOLUME_FRACTION_0))])
+    assert open(path).read() == want
 
 
 def field_digests(get, num_phases):
