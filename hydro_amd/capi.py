@@ -69,6 +69,8 @@ class HgConfig(ctypes.Structure):
         ("force_smooth_times", ctypes.c_int32),
         ("fluid_enable", ctypes.c_int32),
         ("advection_enable", ctypes.c_int32),
+        ("dt_auto", ctypes.c_int32), ("cfl", ctypes.c_double),
+        ("cfl_advection", ctypes.c_double),
         ("velocity_relaxation_factor", ctypes.c_double),
         ("pressure_relaxation_factor", ctypes.c_double),
         ("rhie_chow_factor", ctypes.c_double),
@@ -115,7 +117,7 @@ class HgStepStats(ctypes.Structure):
                 ("convergence_indicator", ctypes.c_double),
                 ("advection_substeps", ctypes.c_int32),
                 ("pressure_solve_iters_total", ctypes.c_int64),
-                ("time", ctypes.c_double)]
+                ("time", ctypes.c_double), ("dt", ctypes.c_double)]
 
 
 class HgStats(ctypes.Structure):
@@ -125,7 +127,9 @@ class HgStats(ctypes.Structure):
 
 
 class HgProgress(ctypes.Structure):
-    _fields_ = [("time", ctypes.c_double), ("steps", ctypes.c_int64),
+    _fields_ = [("time", ctypes.c_double), ("module_time", ctypes.c_double),
+                ("dt", ctypes.c_double), ("dt_advection", ctypes.c_double),
+                ("steps", ctypes.c_int64),
                 ("iteration", ctypes.c_int32),
                 ("convergence_indicator", ctypes.c_double), ("meshpos", _d2)]
 

@@ -84,6 +84,9 @@ void hg_config_default(hg_config* c) {
   c->force_smooth_times = 0;
   c->fluid_enable = 1;
   c->advection_enable = 1;
+  c->dt_auto = 0;
+  c->cfl = 0.5;
+  c->cfl_advection = 0.5;
   c->velocity_relaxation_factor = 0.8;
   c->pressure_relaxation_factor = 0.9;
   c->rhie_chow_factor = 1;
@@ -189,6 +192,8 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "lu_relaxed_relaxation_factor") c->lu_relaxed_relaxation_factor = v;
       else if (name == "advection_dt_factor") c->advection_dt_factor = v;
       else if (name == "sharp") c->sharp = v;
+      else if (name == "cfl") c->cfl = v;
+      else if (name == "cfl_advection") c->cfl_advection = v;
       else if (name == "initial_sin_lambda") c->initial_sin_lambda = v;
       else if (name == "initial_sin_phase") c->initial_sin_phase = v;
       else if (name == "pressure_tolerance_gpu") c->pressure_tolerance_gpu = v;
@@ -210,7 +215,7 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "force_geometric_average") c->force_geometric_average = v;
       else if (name == "tvd_split") c->tvd_split = v;
       else if (name == "meshvel_output") c->meshvel_output = v;
-      else if (name == "dt_auto") flag(HG_UNSUP_DT_AUTO);
+      else if (name == "dt_auto") c->dt_auto = v;
       else if (name == "compressible_enable") flag(HG_UNSUP_COMPRESSIBLE);
       else if (name == "heat_enable") flag(HG_UNSUP_HEAT);
       else if (name == "radiation_enable") flag(HG_UNSUP_RADIATION);

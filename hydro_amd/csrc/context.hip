@@ -63,10 +63,29 @@ std::vector<double> DerivCoeffs(double target, const std::vector<double>& args) 
 }
 
 // ---- small elementwise kernels -------------------------------------------
-__global__ void KExtrapolate(long n, const double* __restrict__ tc, const double* __restrict__ tp,
-                             double ge, double* __restrict__ out) {
-  long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
-  if (i < n) out[i] = tc[i] + (tc[i] - tp[i]) * ge;
+// Initial guess of a step (fluid.hpp:800-811, conv_diff.hpp:122-128):
+// iter_curr = time_curr + (time_curr - time_prev) * guess_extrapolation, for
+// u0, u1, p (threads < nc) and the volume flux (the next nf threads) in one
+// launch.
+struct ExtrapLayers {
+  const double *tc[4], *tp[4];
+  double* out[4];
+};
+
+__global__ void KExtrapolate(long nc, long nf, ExtrapLayers l, double ge) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < nc) {
+    double a[3], b[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = l.tc[k][i];
+      b[k] = l.tp[k][i];
+    }
+    for (int k = 0; k < 3; ++k) l.out[k][i] = a[k] + (a[k] - b[k]) * ge;
+  } else if (i < nc + nf) {
+    const long f = i - nc;
+    const double a = l.tc[3][f], b = l.tp[3][f];
+    l.out[3][f] = a + (a - b) * ge;
+  }
 }
 
 // Pressure solvers that take the operator in face form (CgSolver).
@@ -132,6 +151,9 @@ struct hg_ctx {
   int pinned = -1;
   double bdf[3] = {0, 0, 0};
   double t_fluid = 0., t_adv = 0., dt = 0., dt_adv = 0.;
+  // the module's own clock (TModule::tn / tnp, module.cpp:74-81), advanced
+  // once per step with the dt in force after it
+  double mod_t = 0., mod_tnp = 0.;
   int fluid_iter = 0, cd_iter = 0;
   double last_rs = 1.;
   hg_iter_stats last_iter{};
@@ -260,7 +282,7 @@ void AccumulateStats(hg_ctx* c) {
                          c->bflux_acc);
   for (int d = 0; d < 2; ++d) c->meshpos_stat[d] = c->meshpos[d];
   if (c->cfg.meshvel_output)
-    for (int d = 0; d < 2; ++d) c->meshpos[d] += c->cfg.meshvel[d] * c->cfg.dt;
+    for (int d = 0; d < 2; ++d) c->meshpos[d] += c->cfg.meshvel[d] * c->dt;
 }
 
 // Constructor state of the running statistics, then the constructor's
@@ -298,6 +320,30 @@ void CalcStats(hg_ctx* c, hg_stats* s) {
   }
 }
 
+// Time steps and the unsteady coefficients that follow from them
+// (GetDerivativeApproxCoeffs at {-2dt, -dt, 0}, or {-dt, 0} for first order;
+// conv_diff.hpp:201-203).
+void SetTimeSteps(hg_ctx* c, double dt, double dt_adv) {
+  c->dt = dt;
+  c->dt_adv = dt_adv;
+  std::vector<double> co = DerivCoeffs(0., {-2. * dt, -dt, 0.});
+  if (!c->cfg.time_second_order) {
+    std::vector<double> c2 = DerivCoeffs(0., {-dt, 0.});
+    co = {0., c2[0], c2[1]};
+  }
+  for (int i = 0; i < 3; ++i) c->bdf[i] = co[i];
+}
+
+// dt_auto (hydro2d.hpp:1536-1544): dtm = FluidSimple::GetAutoTimeStep on the
+// time_curr flux; fluid dt = dtm * cfl, advection dt = dtm * cfl_advection.
+void AutoTimeStep(hg_ctx* c) {
+  hg::LaunchAutoDt(c->st, c->geo, c->fl_tc, c->stat_part, c->stat_out);
+  HG_CHECK(hipMemcpyAsync(c->stat_h, c->stat_out, 8, hipMemcpyDeviceToHost, c->st));
+  c->Sync();
+  const double dtm = c->stat_h[0];
+  SetTimeSteps(c, dtm * c->cfg.cfl, dtm * c->cfg.cfl_advection);
+}
+
 void InitialFlux(hg_ctx* c) {
   hg::LaunchInitialFlux(c->st, c->geo, c->bc, c->u_tc[0], c->u_tc[1], c->cfg.meshvel[0],
                         c->cfg.meshvel[1], c->fl_tc);
@@ -324,11 +370,11 @@ void StartStep(hg_ctx* c) {
   if (c->fl_ic == c->fl_tc) c->fl_ic = c->fl_spare;
   for (int d = 0; d < 2; ++d)
     if (c->u_ic[d] == c->u_tc[d]) c->u_ic[d] = c->u_spare[d];
-  const double ge = c->cfg.guess_extrapolation;
-  for (int d = 0; d < 2; ++d)
-    KExtrapolate<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->u_tc[d], c->u_tp[d], ge, c->u_ic[d]);
-  KExtrapolate<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->p_tc, c->p_tp, ge, c->p_ic);
-  KExtrapolate<<<G(c->nf), 256, 0, c->st>>>(c->nf, c->fl_tc, c->fl_tp, ge, c->fl_ic);
+  ExtrapLayers l{{c->u_tc[0], c->u_tc[1], c->p_tc, c->fl_tc},
+                 {c->u_tp[0], c->u_tp[1], c->p_tp, c->fl_tp},
+                 {c->u_ic[0], c->u_ic[1], c->p_ic, c->fl_ic}};
+  KExtrapolate<<<G(c->nc + c->nf), 256, 0, c->st>>>(c->nc, c->nf, l,
+                                                     c->cfg.guess_extrapolation);
   // inputs that stay fixed through the SIMPLE loop (fluid.hpp:602-641)
   hg::LaunchForceFaces(c->st, c->geo, c->fx, c->fy, c->mu_s, c->ffex, c->ffey, c->ffmu);
   hg::LaunchRestoredForce(c->st, c->geo, c->ffex, c->ffey, c->frx, c->fry);
@@ -543,6 +589,8 @@ void FinishStep(hg_ctx* c) {
   for (int d = 0; d < 2; ++d) Rotate(c->u_tp[d], c->u_tc[d], c->u_ic[d], c->u_spare[d]);
   c->t_fluid += c->dt;
   c->steps += 1;
+  c->mod_t = c->mod_tnp;
+  c->mod_tnp = c->mod_t + c->dt;
   if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
   c->fields_checked = true;
   // (NanCheck synchronised: the step's last pressure solve is settled)
@@ -622,14 +670,8 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
       c->ph.grav[d] = cfg.gravity[d];
       c->ph.force[d] = cfg.force[d];
     }
-    c->dt = cfg.dt;
-    c->dt_adv = cfg.dt * cfg.advection_dt_factor;
-    std::vector<double> co = DerivCoeffs(0., {-2. * c->dt, -c->dt, 0.});
-    if (!cfg.time_second_order) {
-      std::vector<double> c2 = DerivCoeffs(0., {-c->dt, 0.});
-      co = {0., c2[0], c2[1]};
-    }
-    for (int i = 0; i < 3; ++i) c->bdf[i] = co[i];
+    SetTimeSteps(c.get(), cfg.dt, cfg.dt * cfg.advection_dt_factor);
+    c->mod_tnp = 0. + cfg.dt;
     const long nc = long(cfg.nx) * cfg.ny;
     const long nf = long(cfg.nx + 1) * cfg.ny + long(cfg.nx) * (cfg.ny + 1);
     c->nc = nc;
@@ -850,6 +892,9 @@ hg_status hg_get_progress(hg_ctx* c, hg_progress* out) {
   return Guard(c, [&]() {
     c->Sync();  // settles a deferred indicator
     out->time = c->t_fluid;
+    out->module_time = c->mod_t;
+    out->dt = c->dt;
+    out->dt_advection = c->dt_adv;
     out->steps = c->steps;
     out->iteration = c->last_s;
     out->convergence_indicator = (c->cd_iter == 0) ? 1. : c->last_rs;
@@ -923,6 +968,7 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
   return Guard(c, [&]() {
     hg_step_stats st{};
     HG_CHECK(hipEventRecord(c->ev[0], c->st));
+    if (c->cfg.dt_auto) AutoTimeStep(c);
     StartStep(c);
     bool iterated = false;
     if (c->cfg.fluid_enable) {
@@ -958,6 +1004,7 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     c->timing[2] = d;
     if (iterated) st.convergence_indicator = c->last_iter.convergence_indicator;
     st.time = c->t_fluid;
+    st.dt = c->dt;
     if (s) *s = st;
   });
 }

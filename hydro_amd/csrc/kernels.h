@@ -73,6 +73,8 @@ void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const doub
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
                         const double* const alpha[3], int np, const double density[3], double dt,
                         double* acc);
+// FluidSimple::GetAutoTimeStep into out[0]; part holds StatsBlocks(g) values.
+void LaunchAutoDt(hipStream_t st, const Geo& g, const double* flux, double* part, double* out);
 int StatsBlocks(const Geo& g);
 void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,
                  const double* alpha, const double* pd, const double* u0, const double* u1,

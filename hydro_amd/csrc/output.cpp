@@ -326,7 +326,7 @@ struct hg_output {
       s += Fmt(v);
       s += ' ';
     };
-    put(pr.time);
+    put(pr.module_time);  // P_double["t"], the module's clock
     put(double(pr.iteration));
     put(pr.convergence_indicator);
     const int np = cfg.num_phases;
@@ -455,7 +455,7 @@ struct hg_output {
       WriteHistory();
       history_step = long(pr.steps);
     }
-    const double time = pr.time;
+    const double time = pr.time;  // fluid_solver->GetTime()
     const double total = oc.T;
     // the frame counts are size_t in the reference: total / count in double
     const double frame_duration = total / double(size_t(long(oc.max_frame_index)));
@@ -503,6 +503,7 @@ void hg_output_config_default(hg_output_config* oc) {
   std::memset(oc, 0, sizeof(*oc));
   oc->abi_version = HG_ABI_VERSION;
   SetString(oc->name, sizeof oc->name, "exp");
+  SetString(oc->plt_title, sizeof oc->plt_title, "Template");
   SetString(oc->field_output_format, sizeof oc->field_output_format, "paraview");
   oc->max_frame_index = 100;
   oc->max_frame_scalar_index = 10000;

@@ -850,6 +850,49 @@ KBoundaryFlux(Geo g, const double* __restrict__ flux, PhaseCells pc, int np, dou
   }
 }
 
+// FluidSimple::GetAutoTimeStep (fluid.hpp:1191-1206): the smallest
+// |V / flux| over every cell's four faces with a nonzero time_curr flux,
+// 1e10 if there is none.  A minimum is exact in any order.
+__global__ void __launch_bounds__(kBlock)
+KAutoDtPartial(Geo g, const double* __restrict__ flux, double* __restrict__ part) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = 1e10;
+  if (c < g.nx * g.ny) {
+    const int i = c % g.nx, j = c / g.nx;
+    const double f[4] = {flux[XFace(g, i, j)], flux[XFace(g, i + 1, j)], flux[YFace(g, i, j)],
+                         flux[YFace(g, i, j + 1)]};
+    for (int k = 0; k < 4; ++k)
+      if (f[k] != 0.) {
+        const double x = fabs(g.V / f[k]);
+        dt = x < dt ? x : dt;
+      }
+  }
+  __shared__ double red[kBlock / 64];
+  const double w = WaveMin(dt);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int q = 1; q < kBlock / 64; ++q) x = fmin(x, red[q]);
+    part[blockIdx.x] = x;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+KAutoDtFinal(const double* __restrict__ part, int nblocks, double* __restrict__ out) {
+  double x = 1e10;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) x = fmin(x, part[b]);
+  __shared__ double red[kBlock / 64];
+  const double w = WaveMin(x);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double y = red[0];
+    for (int q = 1; q < kBlock / 64; ++q) y = fmin(y, red[q]);
+    out[0] = y;
+  }
+}
+
 }  // namespace
 
 // ---- launchers --------------------------------------------------------------
@@ -959,6 +1002,11 @@ void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
     pc.density[p] = density[p];
   }
   KBoundaryFlux<<<1, kBlock, 0, st>>>(g, flux, pc, np, dt, acc);
+}
+void LaunchAutoDt(hipStream_t st, const Geo& g, const double* flux, double* part, double* out) {
+  const int nb = StatsBlocks(g);
+  KAutoDtPartial<<<nb, kBlock, 0, st>>>(g, flux, part);
+  KAutoDtFinal<<<1, kBlock, 0, st>>>(part, nb, out);
 }
 int StatsBlocks(const Geo& g) { return int((long(g.nx) * g.ny + kBlock - 1) / kBlock); }
 void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* yn,

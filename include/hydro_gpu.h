@@ -95,6 +95,10 @@ typedef struct hg_config {
   /* fluid solver */
   int32_t fluid_enable;                /* fluid_enable */
   int32_t advection_enable;            /* advection_enable */
+  int32_t dt_auto;                     /* dt_auto: each step's dt from the volume flux
+                                          (hydro2d.hpp:1536-1544) (0) */
+  double cfl;                          /* cfl: fluid dt = cfl * min |V / flux| (0.5) */
+  double cfl_advection;                /* cfl_advection: advection dt likewise (0.5) */
   double velocity_relaxation_factor;
   double pressure_relaxation_factor;
   double rhie_chow_factor;
@@ -135,7 +139,7 @@ typedef struct hg_config {
 } hg_config;
 
 typedef enum hg_unsupported {
-  HG_UNSUP_DT_AUTO = 1 << 0,        /* dt_auto 1 (hydro2d.hpp:1536-1544) */
+  HG_UNSUP_DT_AUTO = 1 << 0,        /* (no longer set: dt_auto runs) */
   HG_UNSUP_COMPRESSIBLE = 1 << 1,   /* compressible_enable 1 */
   HG_UNSUP_HEAT = 1 << 2,           /* heat_enable 1 */
   HG_UNSUP_RADIATION = 1 << 3,      /* radiation_enable 1 */
@@ -185,6 +189,7 @@ typedef struct hg_step_stats {
   int32_t advection_substeps;
   int64_t pressure_solve_iters_total;
   double time;                      /* fluid time after the step */
+  double dt;                        /* the step's time step (dt_auto: chosen by the step) */
 } hg_step_stats;
 
 typedef struct hg_stats {           /* CalcStat (hydro2d.hpp:1432-1529), per phase */
@@ -204,6 +209,11 @@ typedef struct hg_stats {           /* CalcStat (hydro2d.hpp:1432-1529), per pha
 
 typedef struct hg_progress {        /* what the module's output reads besides fields */
   double time;                      /* fluid time */
+  double module_time;               /* the module's "t" after the console's time advance
+                                       (module.cpp:74-81): 0, then t + the dt in force
+                                       before the step, so with dt_auto it trails the fluid
+                                       time by one step's change of dt */
+  double dt, dt_advection;          /* time steps in force (dt_auto changes them) */
   int64_t steps;                    /* steps finished (FinishStep calls) */
   int32_t iteration;                /* "s": SIMPLE iteration count of the last iteration
                                        made (hydro2d.hpp:1580-1581); 0 before any */
@@ -268,8 +278,9 @@ hg_status hg_advect_substep(hg_ctx* ctx);
 hg_status hg_update_properties(hg_ctx* ctx);
 /* hydro::CalcStat reductions (hydro2d.hpp:1432-1529) of the current state. */
 hg_status hg_calc_stats(hg_ctx* ctx, hg_stats* stats);
-/* hydro::step (hydro2d.hpp:1531-1621): SIMPLE loop, advection sub-steps,
- * property update, stats. */
+/* hydro::step (hydro2d.hpp:1531-1621): with dt_auto the step's dt first
+ * (FluidSimple::GetAutoTimeStep, fluid.hpp:1191-1206), then the SIMPLE loop,
+ * advection sub-steps, property update, stats. */
 hg_status hg_step(hg_ctx* ctx, hg_step_stats* stats);
 /* Wall-clock of the last hg_step split by the reference's timer regions
  * (step.fluid, step.advection, step.fluid_properties), in ms, HIP events. */

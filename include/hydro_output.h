@@ -56,7 +56,7 @@ typedef enum hg_output_entry {
 typedef struct hg_output_config {
   int32_t abi_version;             /* = HG_ABI_VERSION */
   char name[256];                  /* name: experiment name ("exp") */
-  char plt_title[256];             /* plt_title ("" -> name) */
+  char plt_title[256];             /* plt_title ("Template"; "" -> name) */
   char filename_field[512];        /* filename_field ("" -> <name>.field) */
   char filename_scalar[512];       /* filename_scalar ("" -> <name>.scalar) */
   char filename_iter_history[512]; /* filename_iter_history ("" -> <name>.iter_history.plt) */

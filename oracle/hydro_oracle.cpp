@@ -734,7 +734,7 @@ static void RunningStats(or_ctx* o) {
   }
   for (int d = 0; d < 2; ++d) o->meshpos_stat[d] = o->meshpos[d];
   if (o->cfg.meshvel_output)
-    for (int d = 0; d < 2; ++d) o->meshpos[d] += o->cfg.meshvel[d] * o->cfg.dt;
+    for (int d = 0; d < 2; ++d) o->meshpos[d] += o->cfg.meshvel[d] * o->dt;
 }
 
 int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
@@ -1425,11 +1425,33 @@ int or_calc_stats(or_ctx* o, hg_stats* st) {
 }
 
 // hydro::step (hydro2d.hpp:1531-1621), without output.
+// FluidSimple::GetAutoTimeStep (fluid.hpp:1191-1206): over cells in order
+// and their faces in neighbour order, the smallest |V / flux| of a nonzero
+// time_curr flux, from 1e10.
+static double AutoTimeStep(const or_ctx* o) {
+  const Mesh& m = o->m;
+  double dt = 1e10;
+  for (size_t c = 0; c < m.nc; ++c)
+    for (int k = 0; k < 4; ++k) {
+      const double f = o->fl_tc[m.Face(c, k)];
+      if (f != 0.) dt = std::min(dt, std::abs(m.vol[c] / f));
+    }
+  return dt;
+}
+
 int or_step(or_ctx* o, hg_step_stats* stats) {
   ScopedTimer t_step(&o->timers, "step");
   hg_step_stats s;
   std::memset(&s, 0, sizeof(s));
   int rc;
+  if (o->cfg.dt_auto) {
+    // hydro::step (hydro2d.hpp:1536-1544): fluid dt = dtm * cfl, advection
+    // dt = dtm * cfl_advection; the unsteady coefficients follow the new dt
+    const double dtm = AutoTimeStep(o);
+    o->dt = dtm * o->cfg.cfl;
+    o->dt_adv = dtm * o->cfg.cfl_advection;
+    o->bdf = DerivCoeffs(0., {-2. * o->dt, -o->dt, 0.}, o->cfg.time_second_order ? 0 : 1);
+  }
   {
     ScopedTimer t(&o->timers, "step.fluid");
     rc = or_fluid_start_step(o);
@@ -1461,6 +1483,7 @@ int or_step(or_ctx* o, hg_step_stats* stats) {
     RunningStats(o);
   }
   s.time = o->t_fluid;
+  s.dt = o->dt;
   if (stats) *stats = s;
   return HG_OK;
 }

@@ -39,6 +39,16 @@ set string linear_solver_velocity {solver_v}
 """
 
 
+# dt_auto from a moving start (at rest every flux is 0 and dtm = 1e10); at
+# cfl 0.4 the dam case blows up in the reference itself within two steps
+DT_AUTO = """
+set bool dt_auto 1
+set double cfl 0.02
+set double cfl_advection 0.01
+set vect initial_velocity (0.2, 0.1)
+"""
+
+
 def rt(nx=1024, ny=2048, dt=2e-4, solver_p="cg", solver_v="lu_rb", precision="fp64", extra=""):
     """Rayleigh-Taylor, [0,1]x[0,2], Atwood 0.5 (heavy 3 on top of light 1),
     g = (0,-1).  The interface is set through the partial-density fields."""

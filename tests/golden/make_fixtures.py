@@ -20,7 +20,7 @@ not needed afterwards).
    with its cavity sample run (examples/cavity/sample), gzip-compressed:
    data only.
 
-    python tests/golden/make_fixtures.py [output]   (output: 4 and 5 only)
+    python tests/golden/make_fixtures.py [output | only NAME...]
 """
 import os
 import re
@@ -42,12 +42,15 @@ REFERENCE = "/root/reference"
 GENERAL = os.path.join(REFERENCE, "examples", "general.hydroconf")
 SAMPLE_LOG = os.path.join(REFERENCE, "examples", "cavity", "sample", "exp.log")
 
+DT_AUTO = cases.DT_AUTO
+
 FIXTURES = {
     # name: (case text, steps, dump steps)
     "ref_cavity16": (cases.cavity(16), 5, (1, 5)),
     "ref_cavity64": (cases.cavity(64), 20, (20,)),
     "ref_dam64x32": (cases.dam(64, 32), 5, (1, 5)),
     "ref_cavity32_jacobi": (cases.cavity(32, solver_p="jacobi", extra="set double lu_relaxed_relaxation_factor 0.9"), 3, (3,)),
+    "ref_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), 6, (1, 6)),
 }
 
 
@@ -162,11 +165,20 @@ set bool output_divergence 1
 set bool iter_history_enable 1
 set int iter_history_n 0
 """
+DTAUTO_OUTPUT = """
+set string name dta
+set double T 0.02
+set int max_frame_index 2
+set int max_frame_scalar_index 40
+set bool iter_history_enable 1
+set int iter_history_n 1
+"""
 OUTPUT = {
     # name: (case text, output keys, step limit; the run ends at T)
     "ref_output_dam32x16": (cases.dam(32, 16), DAM_OUTPUT, 100),
     "ref_output_cavity16_meshvel": (
         cases.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
+    "ref_output_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), DTAUTO_OUTPUT, 100),
 }
 
 
@@ -198,12 +210,26 @@ def sample_files():
 
 
 def main():
+    """No argument: every fixture.  `output`: the output files and samples;
+    `only NAME...`: the named fixtures."""
     if not os.path.exists(pyoracle.REF_BIN):
         raise SystemExit("build oracle/_ref first: make -f oracle/ref.mk")
+    args = sys.argv[1:]
+    if args[:1] == ["only"]:
+        for name in args[1:]:
+            if name in FIXTURES:
+                reference_dump(name, *FIXTURES[name])
+            elif name in OUTPUT:
+                reference_output(name, *OUTPUT[name])
+            elif name in LONG:
+                reference_hashes(name, *LONG[name])
+            else:
+                raise SystemExit(f"unknown fixture {name}")
+        return
     sample_files()
     for name, (text, out_text, steps) in OUTPUT.items():
         reference_output(name, text, out_text, steps)
-    if sys.argv[1:] == ["output"]:
+    if args == ["output"]:
         return
     sample()
     for name, (text, steps, dumps) in FIXTURES.items():

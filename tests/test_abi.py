@@ -61,6 +61,10 @@ def test_parse_reference_sample_parameter_dump():
     assert cfg.unsupported_flags == 0
 
 
+def capi_unsup(name):
+    return {"condition": 1 << 9}[name]
+
+
 def capi_side(name):
     return {"left": 0, "right": 1, "bottom": 2, "top": 3}[name]
 
@@ -93,10 +97,13 @@ def test_parse_errors_like_the_reference():
 
 
 def test_unsupported_features_are_flagged():
-    cfg = capi.parse_config("set bool dt_auto 1\nset string condition_right outlet\n")
+    cfg = capi.parse_config("set bool heat_enable 1\nset string condition_right outlet\n")
     assert cfg.unsupported_flags != 0
-    cfg = capi.parse_config("set bool dt_auto 0\n")
-    assert cfg.unsupported_flags == 0
+    cfg = capi.parse_config("set bool heat_enable 0\n", cfg)
+    assert cfg.unsupported_flags == capi_unsup("condition")
+    cfg = capi.parse_config("set bool dt_auto 1\nset double cfl 0.25\n")
+    assert cfg.unsupported_flags == 0 and cfg.dt_auto == 1 and cfg.cfl == 0.25
+    assert capi.default_config().cfl_advection == 0.5
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device path")

@@ -32,7 +32,7 @@ def run_cycle(h, out, t_end, max_steps):
     reference's module loop; stops once the time reaches T."""
     for _ in range(max_steps):
         h.step()
-        finished = h.progress().time >= t_end
+        finished = h.progress().module_time >= t_end
         out.write_results(finished)
         if finished:
             break

@@ -60,6 +60,26 @@ def test_dam64_two_phase(gpu):
         assert np.array_equal(g.get(f), o.get(f))
 
 
+def test_dam32_dt_auto(gpu):
+    """dt_auto (hydro2d.hpp:1536-1544): every step's dt from the flux, the
+    advection dt with it; same dt and bits as the oracle, which matches the
+    reference module (fixture ref_dam32x16_dtauto)."""
+    print("case: dam 32x16 dt_auto, 6 steps", flush=True)
+    text = cases.dam(32, 16, extra=cases.DT_AUTO)
+    g = Hydro(text)
+    o = pyoracle.Oracle(capi.parse_config(text))
+    for s in range(6):
+        gs, os_ = g.step(), o.step()
+        assert gs.dt == os_.dt and gs.time == os_.time, (s, gs.dt, os_.dt)
+        assert gs.advection_substeps == os_.advection_substeps
+        assert gs.simple_iterations == os_.simple_iterations
+    for name, f in FIELDS + [("pd1", capi.HG_FIELD_PARTIAL_DENSITY_0 + 1)]:
+        assert np.array_equal(g.get(f), o.get(f)), name
+    pr = g.progress()
+    assert pr.dt == gs.dt and pr.steps == 6 and pr.module_time < pr.time
+    g.close()
+
+
 @pytest.mark.parametrize("solver", ["lu", "gauss_seidel", "jacobi", "lu_rb"])
 def test_stencil_solvers_bitwise(gpu, solver):
     print(f"case: stencil solve {solver} 48x40", flush=True)

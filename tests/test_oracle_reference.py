@@ -19,7 +19,8 @@ from hydro_amd import capi
 from oracle import pyoracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ["ref_cavity16", "ref_cavity64", "ref_dam64x32", "ref_cavity32_jacobi"]
+NAMES = ["ref_cavity16", "ref_cavity64", "ref_dam64x32", "ref_cavity32_jacobi",
+         "ref_dam32x16_dtauto"]
 
 FIELDS = [("u", capi.HG_FIELD_VELOCITY_X), ("v", capi.HG_FIELD_VELOCITY_Y),
           ("p", capi.HG_FIELD_PRESSURE), ("flux", capi.HG_FIELD_VOLUME_FLUX)]
