@@ -36,6 +36,7 @@ HG_FIELD_FORCE_Y = 14
 HG_FIELD_VOLUME_FLUX_ITER = 15
 HG_FIELD_PRESSURE_FACE_COEF = 16
 HG_FIELD_PRESSURE_RHS = 17
+HG_FIELD_TEMPERATURE = 18
 
 _d2 = ctypes.c_double * 2
 _d3 = ctypes.c_double * HG_MAX_PHASES
@@ -92,6 +93,11 @@ class HgConfig(ctypes.Structure):
         ("advection_dt_factor", ctypes.c_double),
         ("tvd_split", ctypes.c_int32),
         ("sharp", ctypes.c_double),
+        ("heat_enable", ctypes.c_int32), ("linear_solver_heat", ctypes.c_int32),
+        ("heat_box_lb", _d2), ("heat_box_rt", _d2), ("conductivity", _d3),
+        ("temperature_initial", ctypes.c_double), ("heat_box_temperature", ctypes.c_double),
+        ("heat_relaxation_factor", ctypes.c_double),
+        ("time_second_order_heat", ctypes.c_int32),
         ("pressure_tolerance_gpu", ctypes.c_double),
         ("pressure_max_iters_gpu", ctypes.c_int32),
         ("precision_gpu", ctypes.c_int32),

@@ -104,6 +104,13 @@ void hg_config_default(hg_config* c) {
   c->advection_dt_factor = 0.1;
   c->tvd_split = 0;
   c->sharp = 0.;
+  c->heat_enable = 0;
+  c->linear_solver_heat = HG_SOLVER_LU;
+  for (int i = 0; i < HG_MAX_PHASES; ++i) c->conductivity[i] = 1;
+  c->temperature_initial = 0;
+  c->heat_box_temperature = 0;
+  c->heat_relaxation_factor = 1.;
+  c->time_second_order_heat = 1;
   c->pressure_tolerance_gpu = 1e-8;
   c->pressure_max_iters_gpu = 20000;
   c->precision_gpu = HG_PREC_FP64;
@@ -193,6 +200,11 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "advection_dt_factor") c->advection_dt_factor = v;
       else if (name == "sharp") c->sharp = v;
       else if (name == "cfl") c->cfl = v;
+      else if (name.rfind("conductivity_", 0) == 0 && name.size() == 14 && isdigit(name[13]) && name[13] - '0' < HG_MAX_PHASES)
+        c->conductivity[name[13] - '0'] = v;
+      else if (name == "temperature_initial") c->temperature_initial = v;
+      else if (name == "heat_box_temperature") c->heat_box_temperature = v;
+      else if (name == "heat_relaxation_factor") c->heat_relaxation_factor = v;
       else if (name == "cfl_advection") c->cfl_advection = v;
       else if (name == "initial_sin_lambda") c->initial_sin_lambda = v;
       else if (name == "initial_sin_phase") c->initial_sin_phase = v;
@@ -217,7 +229,8 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "meshvel_output") c->meshvel_output = v;
       else if (name == "dt_auto") c->dt_auto = v;
       else if (name == "compressible_enable") flag(HG_UNSUP_COMPRESSIBLE);
-      else if (name == "heat_enable") flag(HG_UNSUP_HEAT);
+      else if (name == "heat_enable") c->heat_enable = v;
+      else if (name == "time_second_order_heat") c->time_second_order_heat = v;
       else if (name == "radiation_enable") flag(HG_UNSUP_RADIATION);
       else if (name.rfind("enable_settling_", 0) == 0) flag(HG_UNSUP_SETTLING);
       else if (name == "initial_pois" || name == "deforming_velocity") flag(HG_UNSUP_OTHER);
@@ -244,10 +257,12 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
         } else {
           return fail(line_no, "unknown boundary condition '" + kind + "'");
         }
-      } else if (name == "linear_solver_velocity" || name == "linear_solver_pressure") {
+      } else if (name == "linear_solver_velocity" || name == "linear_solver_pressure" ||
+                 name == "linear_solver_heat") {
         int32_t s = hg_solver_from_name(v.c_str());
         if (s < 0) return fail(line_no, "Unknown linear solver '" + v + "'");
         if (name == "linear_solver_velocity") c->linear_solver_velocity = s;
+        else if (name == "linear_solver_heat") c->linear_solver_heat = s;
         else c->linear_solver_pressure = s;
       } else if (name == "advection_solver") {
         if (v == "tvd") c->unsupported_flags &= ~HG_UNSUP_ADVECTION;
@@ -277,6 +292,8 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "force") ok = Set2(c->force, v);
       else if (name == "initial_velocity") ok = Set2(c->initial_velocity, v);
       else if (name == "meshvel") ok = Set2(c->meshvel, v);
+      else if (name == "heat_box_lb") ok = Set2(c->heat_box_lb, v);
+      else if (name == "heat_box_rt") ok = Set2(c->heat_box_rt, v);
       else if (name == "pressure_fixed_point") {
         ok = Set2(c->pressure_fixed_point, v);
         c->has_pressure_fixed_point = 1;

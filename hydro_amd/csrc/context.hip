@@ -88,6 +88,16 @@ __global__ void KExtrapolate(long nc, long nf, ExtrapLayers l, double ge) {
   }
 }
 
+// One field's initial guess: out = tc + (tc - tp) * ge.
+__global__ void KGuess(long n, const double* __restrict__ tc, const double* __restrict__ tp,
+                       double ge, double* __restrict__ out) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    const double a = tc[i], b = tp[i];
+    out[i] = a + (a - b) * ge;
+  }
+}
+
 // Pressure solvers that take the operator in face form (CgSolver).
 inline bool FaceSolver(int32_t s) {
   return s == HG_SOLVER_CG || s == HG_SOLVER_FCG_MG || s == HG_SOLVER_FCG_LEARNED;
@@ -164,6 +174,14 @@ struct hg_ctx {
   // output offset meshpos and the value of it the last statistics used
   int last_s = 0;
   long steps = 0;
+  // heat (heat.hpp:18-94): temperature time layers, two scratch layers, the
+  // per-boundary-face conditions, cell conductivity and temperature
+  // gradient, the unsteady coefficients at the heat solver's own dt
+  double *t_tc = nullptr, *t_tp = nullptr, *t_a = nullptr, *t_b = nullptr;
+  int* hb_kind = nullptr;
+  double* hb_val = nullptr;
+  double *kcond = nullptr, *tgx = nullptr, *tgy = nullptr;
+  double hbdf[3] = {0., 0., 0.};
   std::vector<double> iter_rs;
   double* bflux_acc = nullptr;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
@@ -269,6 +287,100 @@ void UpdateProperties(hg_ctx* c) {
   Smooth(c, c->mu, c->mu_s, c->cfg.viscosity_smooth_times);
   Smooth(c, c->fx, c->fx, c->cfg.force_smooth_times);
   Smooth(c, c->fy, c->fy, c->cfg.force_smooth_times);
+}
+
+bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d);
+
+// InitHeatSolver (hydro2d.hpp:652-688): a uniform temperature_initial; a
+// boundary face whose centre lies in [heat_box_lb, heat_box_rt] (bounds
+// included, vect.hpp:239-246) holds heat_box_temperature, every other one a
+// zero normal derivative.  The heat solver keeps the dt it is built with.
+void InitHeat(hg_ctx* c) {
+  const hg_config& cfg = c->cfg;
+  const long nc = c->nc;
+  const int nx = cfg.nx, ny = cfg.ny, nb = 2 * (nx + ny);
+  for (double** p : {&c->t_tc, &c->t_tp, &c->t_a, &c->t_b, &c->kcond, &c->tgx, &c->tgy})
+    *p = c->Alloc(nc);
+  c->hb_kind = (int*)c->Alloc(nb / 2 + 1);
+  c->hb_val = c->Alloc(nb);
+  std::vector<double> t0(size_t(nc), cfg.temperature_initial);
+  Upload(c, c->t_tc, t0);
+  Upload(c, c->t_tp, t0);
+  std::vector<int> kind(static_cast<size_t>(nb));
+  std::vector<double> val(static_cast<size_t>(nb));
+  for (int b = 0; b < nb; ++b) {
+    hg::HostMesh::Face f;
+    if (b < ny) f = c->mesh.XFaceGeo(0, b);
+    else if (b < 2 * ny) f = c->mesh.XFaceGeo(nx, b - ny);
+    else if (b < 2 * ny + nx) f = c->mesh.YFaceGeo(b - 2 * ny, 0);
+    else f = c->mesh.YFaceGeo(b - 2 * ny - nx, ny);
+    const bool inside = !(f.cx < cfg.heat_box_lb[0] || cfg.heat_box_rt[0] < f.cx) &&
+                        !(f.cy < cfg.heat_box_lb[1] || cfg.heat_box_rt[1] < f.cy);
+    kind[size_t(b)] = inside ? 0 : 1;
+    val[size_t(b)] = inside ? cfg.heat_box_temperature : 0.;
+  }
+  HG_CHECK(hipMemcpyAsync(c->hb_kind, kind.data(), size_t(nb) * sizeof(int),
+                          hipMemcpyHostToDevice, c->st));
+  Upload(c, c->hb_val, val);
+  const double dt = cfg.dt;
+  std::vector<double> co = DerivCoeffs(0., {-2. * dt, -dt, 0.});
+  if (!cfg.time_second_order_heat) {
+    std::vector<double> c2 = DerivCoeffs(0., {-dt, 0.});
+    co = {0., c2[0], c2[1]};
+  }
+  for (int i = 0; i < 3; ++i) c->hbdf[i] = co[i];
+  c->Sync();  // the host vectors above go out of scope
+}
+
+// One heat step (hydro2d.hpp:1607-1612): StartStep (guess = time_curr, the
+// heat solver's extrapolation is 0), one MakeIteration (its iteration limit
+// is 1), FinishStep with the "NaN field" check (conv_diff.hpp:117-128,
+// 130-243, 244-251; heat.hpp:62-80).
+void HeatStep(hg_ctx* c) {
+  const hg_config& cfg = c->cfg;
+  const int nx = c->geo.nx, ny = c->geo.ny;
+  KGuess<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->t_tc, c->t_tp, 0., c->t_a);
+  hg::HeatFaces hb{c->hb_kind, c->hb_val};
+  hg::HeatIn in{};
+  in.t = c->t_a;
+  in.ttp = c->t_tp;
+  in.ttc = c->t_tc;
+  in.flux = c->fl_ic;  // the fluid's iter_curr flux (hydro2d.hpp:685)
+  in.bdf0 = c->hbdf[0];
+  in.bdf1 = c->hbdf[1];
+  in.bdf2 = c->hbdf[2];
+  in.relax = cfg.heat_relaxation_factor;
+  const double* al[3] = {c->alpha[0], c->alpha[1], c->alpha[2]};
+  hg::LaunchHeatRows(c->st, c->geo, hb, al, c->np, cfg.conductivity, in, c->kcond, c->tgx,
+                     c->tgy, c->As, c->Aw, c->Ac, c->Ae, c->An, c->k0);
+  hg::Rows rows{c->As, c->Aw, c->Ac, c->Ae, c->An};
+  switch (cfg.linear_solver_heat) {
+    case HG_SOLVER_LU:
+      hg::LaunchLuWavefront(c->st, nx, ny, rows, c->k0, nullptr, c->x0, nullptr, c->t_a,
+                            nullptr, c->t_b, nullptr);
+      break;
+    case HG_SOLVER_LU_RB:
+      // the red-black sweep takes two right-hand sides: the second is a copy
+      hg::LaunchLuRb(c->st, nx, ny, rows, c->k0, c->k0, c->x0, c->x1, c->t_a, c->t_a, c->t_b,
+                     c->tmp1);
+      break;
+    case HG_SOLVER_GAUSS_SEIDEL:
+      hg::LaunchGsWavefront(c->st, nx, ny, rows, c->k0, c->x0, cfg.lu_relaxed_tolerance,
+                            cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
+                            c->gs_info);
+      KAddTo<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->t_a, c->x0, c->t_b);
+      break;
+    default:
+      throw HgError(HG_ERR_UNSUPPORTED,
+                    "linear_solver_heat: supported values are lu, lu_rb, gauss_seidel");
+  }
+  // time_prev <- time_curr <- iter_curr
+  double* old_tp = c->t_tp;
+  c->t_tp = c->t_tc;
+  c->t_tc = c->t_b;
+  c->t_b = c->t_a;
+  c->t_a = old_tp;
+  if (NanCheck(c, c->t_tc, c->t_tc, c->t_tc)) throw HgError(HG_ERR_NAN, "NaN field");
 }
 
 // The running part of CalcStat, run after every property update as the
@@ -702,6 +814,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
     c->stat_out = c->Alloc(8);
     c->bflux_acc = c->Alloc(4 * HG_MAX_PHASES);
+    if (cfg.heat_enable) InitHeat(c.get());
     c->rs_part = c->Alloc((nc + 255) / 256);
     c->rs_d = (unsigned long long*)c->Alloc(1);
     c->nan_d = (int*)c->Alloc(1);
@@ -819,6 +932,7 @@ static double* FieldPtr(hg_ctx* c, int f, bool write, long* n) {
     case HG_FIELD_VELOCITY_Y: return c->u_tc[1];
     case HG_FIELD_PRESSURE: return c->p_tc;
     case HG_FIELD_VOLUME_FLUX: *n = c->nf; return c->fl_tc;
+    case HG_FIELD_TEMPERATURE: return c->t_tc;  // nullptr unless heat_enable
     default: break;
   }
   if (f >= HG_FIELD_PARTIAL_DENSITY_0 && f < HG_FIELD_PARTIAL_DENSITY_0 + c->np)
@@ -862,6 +976,7 @@ hg_status hg_set_field(hg_ctx* c, int32_t field, const double* host, size_t n) {
     if (field == HG_FIELD_VELOCITY_Y) prev = c->u_tp[1];
     if (field == HG_FIELD_PRESSURE) prev = c->p_tp;
     if (field == HG_FIELD_VOLUME_FLUX) prev = c->fl_tp;
+    if (field == HG_FIELD_TEMPERATURE) prev = c->t_tp;
     if (prev) HG_CHECK(hipMemcpyAsync(prev, host, n * 8, hipMemcpyHostToDevice, c->st));
     c->Sync();
   });
@@ -990,6 +1105,7 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
         st.advection_substeps += 1;
       }
     }
+    if (c->cfg.heat_enable) HeatStep(c);
     HG_CHECK(hipEventRecord(c->ev[2], c->st));
     UpdateProperties(c);
     AccumulateStats(c);

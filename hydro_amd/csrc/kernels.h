@@ -28,6 +28,21 @@ struct MomentumArgs {
   double* ap = nullptr;  // when set, also a_P of the relaxed rows (fluid.hpp:876-883)
 };
 
+// Boundary conditions of the temperature, per boundary face (left j,
+// right ny + j, bottom 2 ny + i, top 2 ny + nx + i): kind 0 value fixed,
+// kind 1 normal derivative fixed, to val.
+struct HeatFaces {
+  const int* kind;
+  const double* val;
+};
+struct HeatIn {
+  const double *t;          // previous iterate
+  const double *ttp, *ttc;  // time layers
+  const double* flux;       // volume flux
+  const double *cond, *gx, *gy;  // filled by LaunchHeatRows
+  double bdf0, bdf1, bdf2, relax;
+};
+
 // ---- step kernels (step_kernels.hip) ----
 void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
                       const double* mu, double* ffex, double* ffey, double* ffmu);
@@ -68,6 +83,14 @@ void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const doub
                       const double* pd2, double* pd0);
 void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
                     int* flag);
+// The temperature's rows of one heat iteration (heat.hpp:66-75,
+// conv_diff.hpp:130-243) in delta form: cell conductivity (from the volume
+// fractions and conductivity_i) and the temperature gradient into cond, gx,
+// gy, then the 5-point rows into out / k.
+void LaunchHeatRows(hipStream_t st, const Geo& g, const HeatFaces& hb, const double* const alpha[3],
+                    int np, const double cond_i[3], const HeatIn& in, double* cond, double* gx,
+                    double* gy, double* s, double* w, double* c, double* e, double* n,
+                    double* k);
 // CalcStat's boundary in/out integrals: acc[4p..4p+3] (volume in, out, mass
 // in, out of phase p) += dt * the boundary fluxes of the time_curr flux.
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,

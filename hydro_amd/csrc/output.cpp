@@ -271,6 +271,11 @@ struct hg_output {
       else if (e == HG_OUT_VISCOSITY) v = Get(HG_FIELD_VISCOSITY_SMOOTH, nc);
       else if (e == HG_OUT_VELOCITY_Z || e == HG_OUT_RADIATION || e == HG_OUT_EXCLUDED) {
         v.assign(size_t(nc), 0.);
+      } else if (e == HG_OUT_TEMPERATURE) {
+        // the heat solver exists in every run; without heat_enable its
+        // temperature stays temperature_initial (hydro2d.hpp:652-688)
+        if (cfg.heat_enable) v = Get(HG_FIELD_TEMPERATURE, nc);
+        else v.assign(size_t(nc), cfg.temperature_initial);
       } else if (e == HG_OUT_DIVERGENCE) {
         // sum over the cell's faces (left, right, bottom, top; outward
         // factors -1, 1, -1, 1) of the volume flux, over the cell volume
@@ -391,9 +396,8 @@ struct hg_output {
     if (oc.iter_history_enable && oc.iter_history_sfixed > 0)
       throw OutError(HG_ERR_UNSUPPORTED,
                      "iter_history_sfixed > 0 (iterating past convergence) is not supported");
-    for (int e : {int(HG_OUT_VOLUME_SOURCE), int(HG_OUT_TEMPERATURE)})
-      if (oc.output[e]) throw OutError(HG_ERR_UNSUPPORTED, "output_" + EntryName(e) +
-                                                               " is not available");
+    if (oc.output[HG_OUT_VOLUME_SOURCE])
+      throw OutError(HG_ERR_UNSUPPORTED, "output_volume_source is not available");
     for (int i = 0; i < cfg.num_phases; ++i)
       if (oc.output[HG_OUT_MASS_SOURCE_0 + i])
         throw OutError(HG_ERR_UNSUPPORTED,

@@ -787,6 +787,165 @@ __global__ void KStatsFinal(const double* __restrict__ part, int nblocks, double
   out[q] = x;
 }
 
+// Per-phase cell fields and a per-phase constant (density, conductivity).
+struct PhaseCells {
+  const double* alpha[3];
+  double density[3];
+};
+
+// ---- heat: the temperature's implicit convection-diffusion iteration ---------
+// (HeatSolver::MakeIteration, heat.hpp:66-75, on ConvectionDiffusionScalar-
+// Implicit::MakeIteration, conv_diff.hpp:130-243) with the scaling 1, zero
+// source, the step's volume flux and the face conductivity interpolated with
+// zero normal derivative on the boundary.  Boundary face b (left j, right
+// ny + j, bottom 2 ny + i, top 2 ny + nx + i) holds either a fixed value
+// (kind 0) or a fixed normal derivative (kind 1).
+
+// Index of the boundary face on side s at position q along it.
+__device__ __forceinline__ int BoundaryIndex(const Geo& g, int s, int q) {
+  return s == kLeft ? q : s == kRight ? g.ny + q : s == kBottom ? 2 * g.ny + q
+                                                                : 2 * g.ny + g.nx + q;
+}
+// Signed distance factor of the zero-index side's valid cell: the valid
+// neighbour of a left / bottom face is cell 1 (factor -1), of a right / top
+// face cell 0 (factor +1).
+__device__ __forceinline__ double SideFactor(int s) {
+  return (s == kLeft || s == kBottom) ? -1. : 1.;
+}
+
+// Cell conductivity (GetVolumeAveraged of conductivity_i, hydro2d.hpp:1236-1246)
+// and the gradient of the temperature's face values (Interpolate with the
+// heat conditions, solver.hpp:392-470, then Gradient, solver.hpp:658-677).
+__global__ void __launch_bounds__(kBlock)
+KHeatGrad(Geo g, HeatFaces hb, PhaseCells pc, int np, const double* __restrict__ t,
+          double* __restrict__ cond, double* __restrict__ gx, double* __restrict__ gy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nx * g.ny) return;
+  const int i = c % g.nx, j = c / g.nx;
+  const Cross x = LoadCross(t, g.nx, g.ny, i, j);
+  double a[3] = {0., 0., 0.};
+  for (int p = 0; p < 3; ++p)
+    if (p < np) a[p] = pc.alpha[p][c];
+  double k = 0.;
+  for (int p = 0; p < 3; ++p)
+    if (p < np) k += pc.density[p] * a[p];
+  cond[c] = k;
+  double f[4];
+  CellFacesV<1>(g, x, i, j, nullptr, f);
+  const int sides[4] = {kLeft, kRight, kBottom, kTop};
+  const bool bnd[4] = {i == 0, i == g.nx - 1, j == 0, j == g.ny - 1};
+  for (int q = 0; q < 4; ++q) {
+    if (!bnd[q]) continue;
+    const int s = sides[q];
+    const int b = BoundaryIndex(g, s, s < 2 ? j : i);
+    const double v = hb.val[b];
+    f[q] = hb.kind[b] == 0 ? v : x.c + v * (g.ex_dist[s] * SideFactor(s));
+  }
+  GradFromFaces(g, f, gx[c], gy[c]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+KHeatRows(Geo g, HeatFaces hb, HeatIn in, double* __restrict__ rs_, double* __restrict__ rw_,
+          double* __restrict__ rc_, double* __restrict__ re_, double* __restrict__ rn_,
+          double* __restrict__ rk_) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+  const int fw = XFace(g, i, j), fe = XFace(g, i + 1, j);
+  const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
+  const double V = g.V;
+  // inputs first (neighbours at clamped indices), selected below
+  const Cross t = LoadCross(in.t, nx, ny, i, j);
+  const Cross kc = LoadCross(in.cond, nx, ny, i, j);
+  const Cross gxc = LoadCross(in.gx, nx, ny, i, j);
+  const Cross gyc = LoadCross(in.gy, nx, ny, i, j);
+  const double Ff[4] = {in.flux[fw], in.flux[fe], in.flux[fs], in.flux[fn]};
+  const double ttp = in.ttp[c], ttc = in.ttc[c];
+  const bool inner[4] = {hw, he, hs, hn};
+  const double knb[4] = {kc.w, kc.e, kc.s, kc.n};
+  // the face-normal gradient component at the neighbour and at the cell
+  const double gnb[4] = {gxc.w, gxc.e, gyc.s, gyc.n};
+  const double gcc[4] = {gxc.c, gxc.c, gyc.c, gyc.c};
+  const double outf[4] = {-1., 1., -1., 1.};
+  const int side[4] = {kLeft, kRight, kBottom, kTop};
+  double Cc = 0., Dc = 0., Ck = 0., Dk = 0., Cnb[4] = {0, 0, 0, 0}, Dnb[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool xface = q < 2;
+    const double A = xface ? g.ax : g.ay;
+    const double F = Ff[q];
+    const double o = outf[q];
+    const bool self_m = (q == 1 || q == 3);
+    if (inner[q]) {
+      // face conductivity: the mean of the two cells (lower index first)
+      const double D = Mid(self_m ? kc.c : knb[q], self_m ? knb[q] : kc.c);
+      const double gm = self_m ? gcc[q] : gnb[q], gp = self_m ? gnb[q] : gcc[q];
+      double am, ap, ak;
+      if (F > 1e-10) {
+        am = 1.;
+        ap = 0.;
+        ak = -(xface ? gm * g.tx0 : gm * g.ty0);
+      } else if (F < -1e-10) {
+        am = 0.;
+        ap = 1.;
+        ak = -(xface ? gp * g.tx1 : gp * g.ty1);
+      } else {
+        am = 0.5;
+        ap = 0.5;
+        ak = 0.;
+      }
+      const double cam = am * F, cap = ap * F;
+      const double al = 1. / (xface ? g.dnx : g.dny);
+      const double dcm = ((-al) * (-D)) * A;
+      const double dcp = (al * (-D)) * A;
+      const double dck = (0. * (-D)) * A;
+      Cc += (self_m ? cam : cap) * o;
+      Cnb[q] = (self_m ? cap : cam) * o;
+      Dc += (self_m ? dcm : dcp) * o;
+      Dnb[q] = (self_m ? dcp : dcm) * o;
+      Ck += (ak * F) * o;
+      Dk += dck * o;
+    } else {
+      const int s = side[q];
+      const int b = BoundaryIndex(g, s, s < 2 ? j : i);
+      const double v = hb.val[b];
+      // the boundary face's conductivity: the cell's, zero normal derivative
+      const double D = kc.c + 0. * (g.ex_dist[s] * SideFactor(s));
+      if (hb.kind[b] == 0) {
+        // value fixed: convective constant v F; diffusive -al on the cell
+        const double al = g.bal[s];
+        Ck += (v * F) * o;
+        Dc += (((-al) * (-D)) * A) * o;
+        Dk += (((al * v) * (-D)) * A) * o;
+      } else {
+        // derivative fixed: the value is the cell's + v * signed distance
+        Cc += (1. * F) * o;
+        Ck += (((g.ex_dist[s] * SideFactor(s)) * v) * F) * o;
+        Dk += ((v * (-D)) * A) * o;
+      }
+    }
+  }
+  const double cen = (Dc / V) + ((Cc / V + in.bdf2) * 1.);
+  double nb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) nb[q] = (Dnb[q] / V) + ((Cnb[q] / V) * 1.);
+  const double un = in.bdf0 * ttp + in.bdf1 * ttc;
+  double k = ((Dk / V) + ((Ck / V + un) * 1.)) + (-0.);
+  if (hs) k += t.s * nb[2];
+  if (hw) k += t.w * nb[0];
+  k += t.c * cen;
+  if (he) k += t.e * nb[1];
+  if (hn) k += t.n * nb[3];
+  rs_[c] = hs ? nb[2] : 0.;
+  rw_[c] = hw ? nb[0] : 0.;
+  rc_[c] = cen / in.relax;
+  re_[c] = he ? nb[1] : 0.;
+  rn_[c] = hn ? nb[3] : 0.;
+  rk_[c] = k;
+}
+
 // Boundary in/out integrals of CalcStat (hydro2d.hpp:1474-1508), one
 // workgroup: per phase, the volume flux into the domain through boundary
 // faces times the cell's volume fraction (and out of it), then the four
@@ -794,11 +953,6 @@ __global__ void KStatsFinal(const double* __restrict__ part, int nblocks, double
 // faces are taken left, right (x-faces i = 0, nx), bottom, top (y-faces
 // j = 0, ny); the flux toward the inner cell is +flux on left and bottom
 // faces, -flux on right and top faces (GetValidNeighbourCellId 1 / 0).
-struct PhaseCells {
-  const double* alpha[3];
-  double density[3];
-};
-
 __global__ void __launch_bounds__(kBlock)
 KBoundaryFlux(Geo g, const double* __restrict__ flux, PhaseCells pc, int np, double dt,
               double* __restrict__ acc) {
@@ -992,6 +1146,23 @@ void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const doub
 void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,
                     int* flag) {
   KNanCheck<<<Grid(n), kBlock, 0, st>>>(a, b, c, n, flag);
+}
+void LaunchHeatRows(hipStream_t st, const Geo& g, const HeatFaces& hb, const double* const alpha[3],
+                    int np, const double cond_i[3], const HeatIn& in_, double* cond, double* gx,
+                    double* gy, double* s, double* w, double* c, double* e, double* n,
+                    double* k) {
+  PhaseCells pc{};
+  for (int p = 0; p < np; ++p) {
+    pc.alpha[p] = alpha[p];
+    pc.density[p] = cond_i[p];
+  }
+  const dim3 grid = Grid(long(g.nx) * g.ny);
+  KHeatGrad<<<grid, kBlock, 0, st>>>(g, hb, pc, np, in_.t, cond, gx, gy);
+  HeatIn in = in_;
+  in.cond = cond;
+  in.gx = gx;
+  in.gy = gy;
+  KHeatRows<<<grid, kBlock, 0, st>>>(g, hb, in, s, w, c, e, n, k);
 }
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
                         const double* const alpha[3], int np, const double density[3], double dt,

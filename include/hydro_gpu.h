@@ -122,6 +122,18 @@ typedef struct hg_config {
   double advection_dt_factor;
   int32_t tvd_split;
   double sharp;
+  /* heat: a passive temperature advected by the step's flux and diffused
+   * (hydro2d.hpp:652-688,1607-1612; heat.hpp:18-94) */
+  int32_t heat_enable;                 /* heat_enable (0) */
+  int32_t linear_solver_heat;          /* linear_solver_heat: lu, lu_rb, gauss_seidel (lu) */
+  double heat_box_lb[2], heat_box_rt[2]; /* heat_box_lb/_rt: boundary faces whose centre
+                                          lies in this box hold heat_box_temperature; the
+                                          others have zero normal derivative ((0,0)) */
+  double conductivity[HG_MAX_PHASES];  /* conductivity_i (1) */
+  double temperature_initial;          /* temperature_initial (0) */
+  double heat_box_temperature;         /* heat_box_temperature (0) */
+  double heat_relaxation_factor;       /* heat_relaxation_factor (1) */
+  int32_t time_second_order_heat;      /* time_second_order_heat (1) */
   /* new keys of the GPU path (no reference counterpart) */
   double pressure_tolerance_gpu;       /* relative residual for cg / fcg_learned (1e-8) */
   int32_t pressure_max_iters_gpu;      /* iteration cap for cg / fcg_learned */
@@ -141,7 +153,7 @@ typedef struct hg_config {
 typedef enum hg_unsupported {
   HG_UNSUP_DT_AUTO = 1 << 0,        /* (no longer set: dt_auto runs) */
   HG_UNSUP_COMPRESSIBLE = 1 << 1,   /* compressible_enable 1 */
-  HG_UNSUP_HEAT = 1 << 2,           /* heat_enable 1 */
+  HG_UNSUP_HEAT = 1 << 2,           /* (no longer set: the heat solver runs) */
   HG_UNSUP_RADIATION = 1 << 3,      /* radiation_enable 1 */
   HG_UNSUP_CHEMISTRY = 1 << 4,      /* chem_intensity != 0 */
   HG_UNSUP_ANTIDIFFUSION = 1 << 5,  /* antidiffusion_factor set */
@@ -171,7 +183,8 @@ typedef enum hg_field {
    * side b with A p' = b (fluid.hpp:946-1027). */
   HG_FIELD_PRESSURE_FACE_COEF = 16, /* faces */
   HG_FIELD_PRESSURE_RHS = 17,       /* cells */
-  HG_FIELD_COUNT = 18
+  HG_FIELD_TEMPERATURE = 18,        /* cells, time_curr layer (heat_enable) */
+  HG_FIELD_COUNT = 19
 } hg_field;
 
 typedef struct hg_iter_stats {

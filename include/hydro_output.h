@@ -40,7 +40,7 @@ typedef enum hg_output_entry {
   HG_OUT_VISCOSITY = 8,        /* smoothed mixture viscosity */
   HG_OUT_RADIATION = 9,        /* 0: radiation is not run */
   HG_OUT_VOLUME_SOURCE = 10,   /* not available (HG_ERR_UNSUPPORTED) */
-  HG_OUT_TEMPERATURE = 11,     /* not available (HG_ERR_UNSUPPORTED) */
+  HG_OUT_TEMPERATURE = 11,     /* temperature (temperature_initial without heat_enable) */
   HG_OUT_EXCLUDED = 12,        /* 0: no excluded cells */
   HG_OUT_DIVERGENCE = 13,      /* sum of outward volume fluxes / cell volume */
   HG_OUT_MASS_SOURCE_0 = 14,   /* +i; not available (HG_ERR_UNSUPPORTED) */

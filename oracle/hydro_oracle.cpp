@@ -600,6 +600,12 @@ struct or_ctx {
   // running part of CalcStat: boundary volume / mass in and out per phase,
   // the output offset meshpos and the value the last statistics used
   double bacc[HG_MAX_PHASES][4] = {};
+  // heat: temperature time layers, per-face conditions (boundary faces:
+  // kind 0 value fixed, 1 derivative fixed), unsteady coefficients
+  Vec t_tc, t_tp;
+  std::vector<int> hkind;
+  Vec hval;
+  std::vector<double> hbdf;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
 };
 
@@ -737,6 +743,8 @@ static void RunningStats(or_ctx* o) {
     for (int d = 0; d < 2; ++d) o->meshpos[d] += o->cfg.meshvel[d] * o->dt;
 }
 
+static void InitHeat(or_ctx* o);
+
 int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
   auto fail = [&](const char* msg) {
     if (err && errlen) std::snprintf(err, errlen, "%s", msg);
@@ -756,6 +764,7 @@ int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
   o->dt_adv = cfg.dt * cfg.advection_dt_factor;
   o->bdf = DerivCoeffs(0., {-2. * o->dt, -o->dt, 0.}, cfg.time_second_order ? 0 : 1);
   const size_t nc = m.nc;
+  if (cfg.heat_enable) InitHeat(o);
   // Initial velocity (hydro2d.hpp:316-367)
   for (int d = 0; d < 2; ++d) o->u_tc[d].assign(nc, cfg.initial_velocity[d]);
   if (cfg.has_initial_sin) {
@@ -827,6 +836,7 @@ static Vec* FieldPtr(or_ctx* o, int field, bool write) {
   if (field == HG_FIELD_VELOCITY_Y) return &o->u_tc[1];
   if (field == HG_FIELD_PRESSURE) return &o->p_tc;
   if (field == HG_FIELD_VOLUME_FLUX) return &o->fl_tc;
+  if (field == HG_FIELD_TEMPERATURE && o->cfg.heat_enable) return &o->t_tc;
   if (field >= HG_FIELD_PARTIAL_DENSITY_0 && field < HG_FIELD_PARTIAL_DENSITY_0 + o->np)
     return &o->pd_tc[field - HG_FIELD_PARTIAL_DENSITY_0];
   if (write) return nullptr;
@@ -849,6 +859,7 @@ int or_set_field(or_ctx* o, int field, const double* host, size_t n) {
   if (field == HG_FIELD_VELOCITY_Y) o->u_tp[1] = *v;
   if (field == HG_FIELD_PRESSURE) o->p_tp = *v;
   if (field == HG_FIELD_VOLUME_FLUX) o->fl_tp = *v;
+  if (field == HG_FIELD_TEMPERATURE) o->t_tp = *v;
   if (field >= HG_FIELD_PARTIAL_DENSITY_0 && field < HG_FIELD_PARTIAL_DENSITY_0 + o->np)
     o->pd_tp[field - HG_FIELD_PARTIAL_DENSITY_0] = *v;
   return HG_OK;
@@ -1424,6 +1435,177 @@ int or_calc_stats(or_ctx* o, hg_stats* st) {
   return HG_OK;
 }
 
+// InitHeatSolver (hydro2d.hpp:652-688): uniform temperature_initial; a
+// boundary face whose centre lies in the closed box [heat_box_lb,
+// heat_box_rt] (vect.hpp:239-246) holds heat_box_temperature, any other a
+// zero derivative; the heat solver keeps the dt it is built with.
+static void InitHeat(or_ctx* o) {
+  const hg_config& cfg = o->cfg;
+  const Mesh& m = o->m;
+  o->t_tc.assign(m.nc, cfg.temperature_initial);
+  o->t_tp = o->t_tc;
+  o->hkind.assign(m.nf, 1);
+  o->hval.assign(m.nf, 0.);
+  for (size_t f = 0; f < m.nf; ++f) {
+    if (m.Inner(f)) continue;
+    const V2 x = m.fc[f];
+    bool inside = true;
+    if (x.x < cfg.heat_box_lb[0] || cfg.heat_box_rt[0] < x.x) inside = false;
+    if (x.y < cfg.heat_box_lb[1] || cfg.heat_box_rt[1] < x.y) inside = false;
+    if (inside) {
+      o->hkind[f] = 0;
+      o->hval[f] = cfg.heat_box_temperature;
+    }
+  }
+  o->hbdf = DerivCoeffs(0., {-2. * cfg.dt, -cfg.dt, 0.}, cfg.time_second_order_heat ? 0 : 1);
+}
+
+// One heat step (hydro2d.hpp:1607-1612): HeatSolver StartStep, one
+// MakeIteration (limit 1), FinishStep (heat.hpp:62-80) on
+// ConvectionDiffusionScalarImplicit (conv_diff.hpp:117-251) with scaling 1,
+// zero source, the fluid's iter_curr flux and the face conductivity
+// Interpolate(GetVolumeAveraged(conductivity_i), zero derivative).
+static int HeatStep(or_ctx* o) {
+  const hg_config& cfg = o->cfg;
+  const Mesh& m = o->m;
+  const size_t nc = m.nc, nf = m.nf;
+  Vec prev(nc);
+  for (size_t c = 0; c < nc; ++c) prev[c] = o->t_tc[c] + (o->t_tc[c] - o->t_tp[c]) * 0.;
+  // conductivity: cells (hydro2d.hpp:1236-1246, 1428), faces
+  Vec kc(nc, 0.), kf(nf);
+  for (int i = 0; i < o->np; ++i)
+    for (size_t c = 0; c < nc; ++c) kc[c] += cfg.conductivity[i] * o->alpha[i][c];
+  Interpolate(m, kc.data(), kf.data(), MakeBc(BC_DERIV, 0.));
+  // gradient of the previous iterate with the heat conditions
+  Vec tf(nf), gx(nc), gy(nc);
+  for (size_t f = 0; f < nf; ++f) {
+    if (m.Inner(f)) {
+      tf[f] = prev[m.fcm[f]] * (1. - 0.5) + prev[m.fcp[f]] * 0.5;
+    } else {
+      const int id = m.ValidId(f);
+      const size_t c = size_t(id == 0 ? m.fcm[f] : m.fcp[f]);
+      const double factor = (id == 0 ? 1. : -1.);
+      if (o->hkind[f] == 0) tf[f] = o->hval[f];
+      else tf[f] = prev[c] + o->hval[f] * (Norm(m.ToCell(f, id)) * factor);
+    }
+  }
+  Gradient(m, tf.data(), gx.data(), gy.data());
+  // face expressions (solver.hpp:201-243, 246-341): convective cam, cap, cak
+  // and diffusive dcm, dcp, dck on the two sides of each face
+  const Vec& flux = o->fl_ic;
+  Vec cam(nf), cap(nf), cak(nf), dcm(nf), dcp(nf), dck(nf);
+  for (size_t f = 0; f < nf; ++f) {
+    const double F = flux[f], D = kf[f], A_ = m.area[f];
+    if (m.Inner(f)) {
+      const long cm = m.fcm[f], cp = m.fcp[f];
+      double am, ap, ak;
+      if (F > 1e-10) {
+        am = 1.;
+        ap = 0.;
+        ak = -Dot(V2{gx[cm], gy[cm]}, m.to0[f]);
+      } else if (F < -1e-10) {
+        am = 0.;
+        ap = 1.;
+        ak = -Dot(V2{gx[cp], gy[cp]}, m.to1[f]);
+      } else {
+        am = 0.5;
+        ap = 0.5;
+        ak = 0.;
+      }
+      cam[f] = am * F;
+      cap[f] = ap * F;
+      cak[f] = ak * F;
+      V2 dd{m.to1[f].x - m.to0[f].x, m.to1[f].y - m.to0[f].y};
+      const double al = 1. / Norm(dd);
+      dcm[f] = ((-al) * (-D)) * A_;
+      dcp[f] = (al * (-D)) * A_;
+      dck[f] = (0. * (-D)) * A_;
+    } else {
+      const int id = m.ValidId(f);
+      const double factor = (id == 0 ? 1. : -1.);
+      const double v = o->hval[f];
+      double ccell, dcell;
+      if (o->hkind[f] == 0) {
+        // value: convective constant v; diffusive -al on the cell, al v
+        const double al = 1. / Norm(m.ToCell(f, id)) * factor;
+        ccell = 0.;
+        cak[f] = v * F;
+        dcell = ((-al) * (-D)) * A_;
+        dck[f] = ((al * v) * (-D)) * A_;
+      } else {
+        // derivative: the cell's value + alpha v; diffusive constant v
+        const double al = Norm(m.ToCell(f, id)) * factor;
+        ccell = 1. * F;
+        cak[f] = (al * v) * F;
+        dcell = 0.;
+        dck[f] = (v * (-D)) * A_;
+      }
+      cam[f] = id == 0 ? ccell : 0.;
+      cap[f] = id == 0 ? 0. : ccell;
+      dcm[f] = id == 0 ? dcell : 0.;
+      dcp[f] = id == 0 ? 0. : dcell;
+    }
+  }
+  // cell rows (conv_diff.hpp:183-236), merged face sums in W, E, S, N order
+  Sys A;
+  A.Resize(m.nx, m.ny);
+  Vec k(nc);
+  const std::vector<double>& co = o->hbdf;
+  for (size_t c = 0; c < nc; ++c) {
+    double Cc = 0., Ck = 0., Dc = 0., Dk = 0.;
+    double Cnb[4] = {0, 0, 0, 0}, Dnb[4] = {0, 0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+      const size_t f = m.Face(c, q);
+      const double o_ = Mesh::Out(q);
+      const bool self_is_m = (q == 1 || q == 3);
+      const double cself = self_is_m ? cam[f] : cap[f];
+      const double dself = self_is_m ? dcm[f] : dcp[f];
+      if (m.Inner(f)) {
+        Cc += cself * o_;
+        Cnb[q] = (self_is_m ? cap[f] : cam[f]) * o_;
+        Dc += dself * o_;
+        Dnb[q] = (self_is_m ? dcp[f] : dcm[f]) * o_;
+      } else {
+        // a boundary face's terms sit on the cell (none for a value's
+        // convection or a derivative's diffusion)
+        if (o->hkind[f] == 1) Cc += cself * o_;
+        if (o->hkind[f] == 0) Dc += dself * o_;
+      }
+      Ck += cak[f] * o_;
+      Dk += dck[f] * o_;
+    }
+    const double V = m.vol[c];
+    const double cen = (Dc / V) + ((Cc / V + co[2]) * 1.);
+    double nb[4];
+    for (int q = 0; q < 4; ++q) nb[q] = (Dnb[q] / V) + ((Cnb[q] / V) * 1.);
+    const double unst = co[0] * o->t_tp[c] + co[1] * o->t_tc[c];
+    double kon = ((Dk / V) + ((Ck / V + unst) * 1.)) - 0.;
+    const int i = int(c % m.nx), j = int(c / m.nx);
+    double ev = kon;
+    if (j > 0) ev += prev[c - m.nx] * nb[2];
+    if (i > 0) ev += prev[c - 1] * nb[0];
+    ev += prev[c] * cen;
+    if (i < m.nx - 1) ev += prev[c + 1] * nb[1];
+    if (j < m.ny - 1) ev += prev[c + m.nx] * nb[3];
+    k[c] = ev;
+    A.w[c] = i > 0 ? nb[0] : 0.;
+    A.e[c] = i < m.nx - 1 ? nb[1] : 0.;
+    A.s[c] = j > 0 ? nb[2] : 0.;
+    A.n[c] = j < m.ny - 1 ? nb[3] : 0.;
+    A.c[c] = cen / cfg.heat_relaxation_factor;
+  }
+  Vec corr;
+  RunSolver(o, cfg.linear_solver_heat, A, k, corr, false);
+  Vec cur(nc);
+  for (size_t c = 0; c < nc; ++c) cur[c] = prev[c] + corr[c];
+  // FinishStep: time_prev <- time_curr <- iter_curr, "NaN field"
+  o->t_tp = o->t_tc;
+  o->t_tc = cur;
+  for (double v : o->t_tc)
+    if (!(v * 0. == 0.)) return HG_ERR_NAN;
+  return HG_OK;
+}
+
 // hydro::step (hydro2d.hpp:1531-1621), without output.
 // FluidSimple::GetAutoTimeStep (fluid.hpp:1191-1206): over cells in order
 // and their faces in neighbour order, the smallest |V / flux| of a nonzero
@@ -1476,6 +1658,11 @@ int or_step(or_ctx* o, hg_step_stats* stats) {
       if (rc) return rc;
       s.advection_substeps += 1;
     }
+  }
+  if (o->cfg.heat_enable) {
+    ScopedTimer t(&o->timers, "step.heat");
+    rc = HeatStep(o);
+    if (rc) return rc;
   }
   {
     ScopedTimer t(&o->timers, "step.fluid_properties");

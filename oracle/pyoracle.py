@@ -208,4 +208,6 @@ def read_ref_dump(path):
         off += nc
         out[f"pd{i}"] = data[off:off + nc].copy()
         off += nc
+    if data.size >= off + nc:  # dumps written since the temperature was added
+        out["T"] = data[off:off + nc].copy()
     return out

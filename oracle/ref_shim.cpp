@@ -14,7 +14,8 @@
 //   script: a file of `set <type> <name> <value>` lines
 //   writes <prefix>.<step>.bin for each dumped step:
 //     header: int32 nx, ny, num_phases, step; then float64 arrays
-//     u[N], v[N], p[N], flux[F], then alpha_i[N], pd_i[N] per phase
+//     u[N], v[N], p[N], flux[F], then alpha_i[N], pd_i[N] per phase,
+//     then the temperature T[N]
 //   and prints one line per step: "STEP s=<n> iters=<SIMPLE iters> t=<seconds>"
 //   dump-every < 0: output mode, no dumps; after each step the module time
 //   advances and hydro::write_results runs as in TModule::cycle, so the
@@ -63,6 +64,7 @@ using AdvPtr = std::shared_ptr<solver::AdvectionSolverMulti<Mesh, geom::FieldFac
 HR_MEMBER(TagAdv, AdvPtr, advection_solver)
 HR_MEMBER(TagFrac, std::vector<geom::FieldCell<double>>, v_fc_volume_fraction)
 HR_MEMBER(TagNp, const size_t, num_phases)
+HR_MEMBER(TagHeat, std::shared_ptr<solver::HeatSolver<Mesh>>, heat_solver)
 HR_MEMBER(TagSession, std::shared_ptr<output::Session>, session)
 HR_MEMBER(TagSessionScalar, std::shared_ptr<output::Session>, session_scalar)
 // TModule::increase_time (protected; experiment.hpp:109)
@@ -123,6 +125,12 @@ static void Dump(Module& h, const std::string& path, int step) {
     for (auto c : mesh.Cells()) buf.push_back(pd[c]);
     f.write(reinterpret_cast<const char*>(buf.data()), buf.size() * 8);
   }
+  // the temperature last (the heat solver always exists; it changes only
+  // with heat_enable)
+  buf.clear();
+  const auto& t = (h.*Get(TagHeat()))->GetTemperature();
+  for (auto c : mesh.Cells()) buf.push_back(t[c]);
+  f.write(reinterpret_cast<const char*>(buf.data()), buf.size() * 8);
 }
 
 int main(int argc, char** argv) {

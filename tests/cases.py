@@ -49,6 +49,32 @@ set vect initial_velocity (0.2, 0.1)
 """
 
 
+# heat: the bottom wall held at temperature 1 (the box holds the y = 0
+# faces' centres), every other wall adiabatic; lu for the temperature
+HEAT_BOTTOM = """
+set bool heat_enable 1
+set vect heat_box_lb (-1, -1)
+set vect heat_box_rt (2, 0)
+set double heat_box_temperature 1
+set double temperature_initial 0.25
+set double conductivity_0 0.02
+set double conductivity_1 0.002
+"""
+# heat: the left wall held at temperature 2, first order in time, SOR for
+# the temperature
+HEAT_LEFT = """
+set bool heat_enable 1
+set string linear_solver_heat gauss_seidel
+set vect heat_box_lb (0, 0)
+set vect heat_box_rt (0, 1)
+set double heat_box_temperature 2
+set double conductivity_0 0.05
+set double conductivity_1 0.001
+set bool time_second_order_heat 0
+set double heat_relaxation_factor 0.9
+"""
+
+
 def rt(nx=1024, ny=2048, dt=2e-4, solver_p="cg", solver_v="lu_rb", precision="fp64", extra=""):
     """Rayleigh-Taylor, [0,1]x[0,2], Atwood 0.5 (heavy 3 on top of light 1),
     g = (0,-1).  The interface is set through the partial-density fields."""

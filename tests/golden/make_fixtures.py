@@ -51,6 +51,8 @@ FIXTURES = {
     "ref_dam64x32": (cases.dam(64, 32), 5, (1, 5)),
     "ref_cavity32_jacobi": (cases.cavity(32, solver_p="jacobi", extra="set double lu_relaxed_relaxation_factor 0.9"), 3, (3,)),
     "ref_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), 6, (1, 6)),
+    "ref_cavity32_heat": (cases.cavity(32, extra=cases.HEAT_BOTTOM), 5, (1, 5)),
+    "ref_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_LEFT), 4, (4,)),
 }
 
 
@@ -173,12 +175,20 @@ set int max_frame_scalar_index 40
 set bool iter_history_enable 1
 set int iter_history_n 1
 """
+HEAT_OUTPUT = """
+set string name heat
+set double T 0.0006
+set int max_frame_index 2
+set int max_frame_scalar_index 3
+set bool output_temperature 1
+"""
 OUTPUT = {
     # name: (case text, output keys, step limit; the run ends at T)
     "ref_output_dam32x16": (cases.dam(32, 16), DAM_OUTPUT, 100),
     "ref_output_cavity16_meshvel": (
         cases.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
     "ref_output_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), DTAUTO_OUTPUT, 100),
+    "ref_output_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_BOTTOM), HEAT_OUTPUT, 100),
 }
 
 

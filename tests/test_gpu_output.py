@@ -118,7 +118,7 @@ def test_running_stats_match_oracle(gpu):
 def test_output_refuses_entries_it_cannot_supply(gpu, tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     with Hydro("set int Nx 16\nset int Ny 16\n") as h:
-        for key in ("output_temperature", "output_volume_source", "output_mass_source_0"):
+        for key in ("output_volume_source", "output_mass_source_0"):
             with pytest.raises(Unsupported, match=key):
                 h.open_output(f"set bool {key} 1\n")
         out = h.open_output("set bool no_output 1\n")

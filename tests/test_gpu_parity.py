@@ -80,6 +80,27 @@ def test_dam32_dt_auto(gpu):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["cavity32_heat", "dam32x16_heat", "dam32x16_heat_lu_rb"])
+def test_heat_matches_oracle(gpu, case):
+    """The heat solver (heat.hpp:18-94): the temperature bit for bit with the
+    oracle, which matches the reference module (fixtures ref_*_heat); lu_rb
+    has no reference counterpart and is checked against the oracle only."""
+    print(f"case: {case}, 4 steps", flush=True)
+    if case == "cavity32_heat":
+        text = cases.cavity(32, extra=cases.HEAT_BOTTOM)
+    elif case == "dam32x16_heat":
+        text = cases.dam(32, 16, extra=cases.HEAT_LEFT)
+    else:
+        text = cases.dam(32, 16, extra=cases.HEAT_BOTTOM + "set string linear_solver_heat lu_rb\n")
+    g, o, worst = run_pair(text, 4)
+    t = g.get(capi.HG_FIELD_TEMPERATURE)
+    assert np.array_equal(t, o.get(capi.HG_FIELD_TEMPERATURE))
+    assert t.max() > t.min()  # the held wall has heated the fluid
+    for name, f in FIELDS:
+        assert np.array_equal(g.get(f), o.get(f)), name
+    g.close()
+
+
 @pytest.mark.parametrize("solver", ["lu", "gauss_seidel", "jacobi", "lu_rb"])
 def test_stencil_solvers_bitwise(gpu, solver):
     print(f"case: stencil solve {solver} 48x40", flush=True)

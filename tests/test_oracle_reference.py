@@ -20,7 +20,7 @@ from oracle import pyoracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = ["ref_cavity16", "ref_cavity64", "ref_dam64x32", "ref_cavity32_jacobi",
-         "ref_dam32x16_dtauto"]
+         "ref_dam32x16_dtauto", "ref_cavity32_heat", "ref_dam32x16_heat"]
 
 FIELDS = [("u", capi.HG_FIELD_VELOCITY_X), ("v", capi.HG_FIELD_VELOCITY_Y),
           ("p", capi.HG_FIELD_PRESSURE), ("flux", capi.HG_FIELD_VOLUME_FLUX)]
@@ -43,6 +43,8 @@ def test_oracle_matches_reference_fixture(name):
         n = st.simple_iterations
         assert st.pressure_solve_iters_total == sum(ref_sweeps[pos:pos + n]), (name, s)
         pos += n
+        if o.cfg.heat_enable and o.cfg.linear_solver_heat != capi.SOLVERS["lu"]:
+            pos += 1  # the heat solve's own sweep count follows the step's
         if s in dumps:
             for fname, f in FIELDS:
                 assert np.array_equal(o.get(f), z[f"s{s}_{fname}"]), (name, s, fname)
@@ -51,6 +53,8 @@ def test_oracle_matches_reference_fixture(name):
                                       z[f"s{s}_alpha{i}"]), (name, s, i)
                 assert np.array_equal(o.get(capi.HG_FIELD_PARTIAL_DENSITY_0 + i),
                                       z[f"s{s}_pd{i}"]), (name, s, i)
+            if o.cfg.heat_enable:
+                assert np.array_equal(o.get(capi.HG_FIELD_TEMPERATURE), z[f"s{s}_T"]), (name, s)
     assert pos == len(ref_sweeps)
 
 

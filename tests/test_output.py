@@ -20,7 +20,7 @@ from oracle import pyoracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 OUTPUT_FIXTURES = ["ref_output_dam32x16", "ref_output_cavity16_meshvel",
-                   "ref_output_dam32x16_dtauto"]
+                   "ref_output_dam32x16_dtauto", "ref_output_dam32x16_heat"]
 
 
 def load_output_fixture(name):
