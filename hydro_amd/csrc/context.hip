@@ -183,7 +183,7 @@ struct hg_ctx {
   double *kcond = nullptr, *tgx = nullptr, *tgy = nullptr;
   double hbdf[3] = {0., 0., 0.};
   std::vector<double> iter_rs;
-  double* bflux_acc = nullptr;
+  double *bflux_acc = nullptr, *bflux_part = nullptr;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
   std::string err;
   std::unique_ptr<hg::CgSolver> cg;
@@ -391,7 +391,7 @@ void HeatStep(hg_ctx* c) {
 void AccumulateStats(hg_ctx* c) {
   const double* al[3] = {c->alpha[0], c->alpha[1], c->alpha[2]};
   hg::LaunchBoundaryFlux(c->st, c->geo, c->fl_tc, al, c->np, c->cfg.density, c->dt,
-                         c->bflux_acc);
+                         c->bflux_part, c->bflux_acc);
   for (int d = 0; d < 2; ++d) c->meshpos_stat[d] = c->meshpos[d];
   if (c->cfg.meshvel_output)
     for (int d = 0; d < 2; ++d) c->meshpos[d] += c->cfg.meshvel[d] * c->dt;
@@ -814,6 +814,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
     c->stat_out = c->Alloc(8);
     c->bflux_acc = c->Alloc(4 * HG_MAX_PHASES);
+    c->bflux_part = c->Alloc(6 * long(hg::BoundaryFluxBlocks(c->geo)));
     if (cfg.heat_enable) InitHeat(c.get());
     c->rs_part = c->Alloc((nc + 255) / 256);
     c->rs_d = (unsigned long long*)c->Alloc(1);

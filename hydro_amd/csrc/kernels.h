@@ -92,10 +92,12 @@ void LaunchHeatRows(hipStream_t st, const Geo& g, const HeatFaces& hb, const dou
                     double* gy, double* s, double* w, double* c, double* e, double* n,
                     double* k);
 // CalcStat's boundary in/out integrals: acc[4p..4p+3] (volume in, out, mass
-// in, out of phase p) += dt * the boundary fluxes of the time_curr flux.
+// in, out of phase p) += dt * the boundary fluxes of the time_curr flux;
+// part holds 6 * BoundaryFluxBlocks(g) partial sums.
+int BoundaryFluxBlocks(const Geo& g);
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
                         const double* const alpha[3], int np, const double density[3], double dt,
-                        double* acc);
+                        double* part, double* acc);
 // FluidSimple::GetAutoTimeStep into out[0]; part holds StatsBlocks(g) values.
 void LaunchAutoDt(hipStream_t st, const Geo& g, const double* flux, double* part, double* out);
 int StatsBlocks(const Geo& g);

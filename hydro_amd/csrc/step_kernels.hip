@@ -946,41 +946,44 @@ KHeatRows(Geo g, HeatFaces hb, HeatIn in, double* __restrict__ rs_, double* __re
   rk_[c] = k;
 }
 
-// Boundary in/out integrals of CalcStat (hydro2d.hpp:1474-1508), one
-// workgroup: per phase, the volume flux into the domain through boundary
-// faces times the cell's volume fraction (and out of it), then the four
-// running totals += flux * dt (volume and mass in and out).  The boundary
-// faces are taken left, right (x-faces i = 0, nx), bottom, top (y-faces
-// j = 0, ny); the flux toward the inner cell is +flux on left and bottom
-// faces, -flux on right and top faces (GetValidNeighbourCellId 1 / 0).
+// Boundary in/out integrals of CalcStat (hydro2d.hpp:1474-1508): per
+// phase, the volume flux into the domain through boundary faces times the
+// cell's volume fraction (and out of it), then the four running totals +=
+// flux * dt (volume and mass in and out).  The boundary faces are taken
+// left, right (x-faces i = 0, nx), bottom, top (y-faces j = 0, ny); the flux
+// toward the inner cell is +flux on left and bottom faces, -flux on right
+// and top faces (GetValidNeighbourCellId 1 / 0).  One thread per boundary
+// face; per-workgroup partial sums, folded by one workgroup.
 __global__ void __launch_bounds__(kBlock)
-KBoundaryFlux(Geo g, const double* __restrict__ flux, PhaseCells pc, int np, double dt,
-              double* __restrict__ acc) {
+KBoundaryFluxPartial(Geo g, const double* __restrict__ flux, PhaseCells pc, int np,
+                     double* __restrict__ part) {
   const int nb = 2 * (g.nx + g.ny);
+  const int b = blockIdx.x * kBlock + threadIdx.x;
   double in[3] = {0., 0., 0.}, out[3] = {0., 0., 0.};
-  for (int b = threadIdx.x; b < nb; b += kBlock) {
+  if (b < nb) {
     int face, cell;
     double factor;
     if (b < 2 * g.ny) {
-      const int j = b < g.ny ? b : b - g.ny;
       const bool left = b < g.ny;
+      const int j = left ? b : b - g.ny;
       face = XFace(g, left ? 0 : g.nx, j);
       cell = (left ? 0 : g.nx - 1) + g.nx * j;
       factor = left ? 1. : -1.;
     } else {
       const int r = b - 2 * g.ny;
-      const int i = r < g.nx ? r : r - g.nx;
       const bool bottom = r < g.nx;
+      const int i = bottom ? r : r - g.nx;
       face = YFace(g, i, bottom ? 0 : g.ny);
       cell = i + g.nx * (bottom ? 0 : g.ny - 1);
       factor = bottom ? 1. : -1.;
     }
+    double vf[3] = {0., 0., 0.};
+    for (int p = 0; p < 3; ++p)
+      if (p < np) vf[p] = pc.alpha[p][cell];
     const double to_cell = flux[face] * factor;
     for (int p = 0; p < 3; ++p) {
-      if (p >= np) break;
-      const double vf = pc.alpha[p][cell];
-      if (to_cell >= 0) in[p] += to_cell * vf;
-      else out[p] -= to_cell * vf;
+      if (to_cell >= 0) in[p] += to_cell * vf[p];
+      else out[p] -= to_cell * vf[p];
     }
   }
   __shared__ double red[6][kBlock / 64];
@@ -989,19 +992,27 @@ KBoundaryFlux(Geo g, const double* __restrict__ flux, PhaseCells pc, int np, dou
     if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
   }
   __syncthreads();
-  if (threadIdx.x < np) {
-    const int p = threadIdx.x;
-    double vin = red[p][0], vout = red[p + 3][0];
-    for (int w = 1; w < kBlock / 64; ++w) {
-      vin += red[p][w];
-      vout += red[p + 3][w];
-    }
-    const double min = vin * pc.density[p], mout = vout * pc.density[p];
-    acc[4 * p + 0] += vin * dt;
-    acc[4 * p + 1] += vout * dt;
-    acc[4 * p + 2] += min * dt;
-    acc[4 * p + 3] += mout * dt;
+  if (threadIdx.x < 6) {
+    double x = red[threadIdx.x][0];
+    for (int w = 1; w < kBlock / 64; ++w) x += red[threadIdx.x][w];
+    part[blockIdx.x * 6 + threadIdx.x] = x;
   }
+}
+
+__global__ void KBoundaryFluxFinal(const double* __restrict__ part, int nblocks, PhaseCells pc,
+                                   int np, double dt, double* __restrict__ acc) {
+  const int p = threadIdx.x;
+  if (p >= np) return;
+  double vin = 0., vout = 0.;
+  for (int k = 0; k < nblocks; ++k) {
+    vin += part[k * 6 + p];
+    vout += part[k * 6 + 3 + p];
+  }
+  const double min = vin * pc.density[p], mout = vout * pc.density[p];
+  acc[4 * p + 0] += vin * dt;
+  acc[4 * p + 1] += vout * dt;
+  acc[4 * p + 2] += min * dt;
+  acc[4 * p + 3] += mout * dt;
 }
 
 // FluidSimple::GetAutoTimeStep (fluid.hpp:1191-1206): the smallest
@@ -1164,15 +1175,18 @@ void LaunchHeatRows(hipStream_t st, const Geo& g, const HeatFaces& hb, const dou
   in.gy = gy;
   KHeatRows<<<grid, kBlock, 0, st>>>(g, hb, in, s, w, c, e, n, k);
 }
+int BoundaryFluxBlocks(const Geo& g) { return (2 * (g.nx + g.ny) + kBlock - 1) / kBlock; }
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,
                         const double* const alpha[3], int np, const double density[3], double dt,
-                        double* acc) {
+                        double* part, double* acc) {
   PhaseCells pc{};
   for (int p = 0; p < np; ++p) {
     pc.alpha[p] = alpha[p];
     pc.density[p] = density[p];
   }
-  KBoundaryFlux<<<1, kBlock, 0, st>>>(g, flux, pc, np, dt, acc);
+  const int nb = BoundaryFluxBlocks(g);
+  KBoundaryFluxPartial<<<nb, kBlock, 0, st>>>(g, flux, pc, np, part);
+  KBoundaryFluxFinal<<<1, 64, 0, st>>>(part, nb, pc, np, dt, acc);
 }
 void LaunchAutoDt(hipStream_t st, const Geo& g, const double* flux, double* part, double* out) {
   const int nb = StatsBlocks(g);
