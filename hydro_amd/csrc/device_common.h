@@ -32,7 +32,17 @@ struct Geo {
   double ex_dot[4];   // extrapolation: S_out(opposite face) . n_outward / per side
   double ex_den[4];   // extrapolation: 1/dist - S_out(self) . n_outward / V
   double rdist_x, rdist_y;  // |x_cell - x_face| for x- / y-faces
+  // 1 / V, and whether V is a power of two: then x / V and x * (1 / V) are
+  // the same real number, so both round to the same double (a division
+  // costs about ten FP64 instructions, a product one)
+  double invV;
+  int pow2V;
 };
+
+// x / V, as a product when that is exact.
+__device__ __forceinline__ double OverV(const Geo& g, double x) {
+  return g.pow2V ? x * g.invV : x / g.V;
+}
 
 __device__ __forceinline__ int XFace(const Geo& g, int i, int j) { return i + (g.nx + 1) * j; }
 __device__ __forceinline__ int YFace(const Geo& g, int i, int j) { return g.nfx + i + g.nx * j; }
@@ -113,8 +123,8 @@ __device__ __forceinline__ void GradFromFaces(const Geo& g, const double f[4], d
   sxv += g.sx * f[1];
   double syv = (-g.sy) * f[2];
   syv += g.sy * f[3];
-  gx = sxv / g.V;
-  gy = syv / g.V;
+  gx = OverV(g, sxv);
+  gy = OverV(g, syv);
 }
 
 // Wave-level reductions (64 lanes).

@@ -109,6 +109,9 @@ struct HostMesh {
     for (int i = 0; i <= nx; ++i) xn[i] = a[0] + (double(i) / double(nx)) * dx;
     for (int j = 0; j <= ny; ++j) yn[j] = a[1] + (double(j) / double(ny)) * dy;
     geo = Derive(1, 1);
+    geo.invV = 1. / geo.V;
+    int e = 0;
+    geo.pow2V = std::frexp(geo.V, &e) == 0.5 ? 1 : 0;
     // Boundary sides.  The valid neighbour of a left/bottom face is cell 1
     // (factor -1), of a right/top face cell 0 (factor +1).
     for (int s = 0; s < 4; ++s) {

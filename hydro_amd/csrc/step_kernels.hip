@@ -65,8 +65,8 @@ KRestoredForce(Geo g, const double* __restrict__ ffex, const double* __restrict_
   sx += g.sx * ffex[fe] * g.rdist_x;
   sy += g.sy * ffey[fs] * g.rdist_y;
   sy += g.sy * ffey[fn] * g.rdist_y;
-  frx[c] = sx / g.V;
-  fry[c] = sy / g.V;
+  frx[c] = OverV(g, sx);
+  fry[c] = OverV(g, sy);
 }
 
 // Face interpolation with zero-derivative condition for a pair of cell fields.
@@ -182,8 +182,8 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
     sx += gex * we;
     double sy = gwy * ww;
     sy += gey * we;
-    fox += sx / V;
-    foy += sy / V;
+    fox += OverV(g, sx);
+    foy += OverV(g, sy);
     // d = 1: weights mu_f * S_out[1] on the S/N faces.
     double ws = mus * (-g.sy), wn = mun * g.sy;
     double gsx = FG(g1xC, g1xS, true, hs), gnx = FG(g1xC, g1xN, false, hn);
@@ -192,8 +192,8 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
     sx += gnx * wn;
     sy = gsy * ws;
     sy += gny * wn;
-    fox += sx / V;
-    foy += sy / V;
+    fox += OverV(g, sx);
+    foy += OverV(g, sy);
   }
   // --- force assembly: += -grad p + restored force (+ zero surface tension
   //     and zero volume-source compensation on this path)
@@ -265,16 +265,16 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
     }
   }
   const double rho = rhoc;
-  const double cen = (Dc / V) + ((Cc / V + in.bdf2) * rho);
+  const double cen = OverV(g, Dc) + ((OverV(g, Cc) + in.bdf2) * rho);
   double nb[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) nb[k] = (Dnb[k] / V) + ((Cnb[k] / V) * rho);
+  for (int k = 0; k < 4; ++k) nb[k] = OverV(g, Dnb[k]) + (OverV(g, Cnb[k]) * rho);
   // constants, then the delta form (row evaluated at the previous iterate in
   // the order S, W, C, E, N) and under-relaxation of the diagonal
   const double un0 = in.bdf0 * utp0c + in.bdf1 * utc0c;
   const double un1 = in.bdf0 * utp1c + in.bdf1 * utc1c;
-  double k0 = ((Dk0 / V) + ((Ck0 / V + un0) * rho)) + (-fox);
-  double k1 = ((Dk1 / V) + ((Ck1 / V + un1) * rho)) + (-foy);
+  double k0 = (OverV(g, Dk0) + ((OverV(g, Ck0) + un0) * rho)) + (-fox);
+  double k1 = (OverV(g, Dk1) + ((OverV(g, Ck1) + un1) * rho)) + (-foy);
   if (hs) { k0 += u0S * nb[2]; k1 += u1S * nb[2]; }
   if (hw) { k0 += u0W * nb[0]; k1 += u1W * nb[0]; }
   k0 += u0C * cen;
@@ -620,7 +620,7 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
         const double fe = ii < nx - 1 ? Mid(uc, su[lj + 2][hi + 2]) : uc;
         double sxv = (-g.sx) * fw;
         sxv += g.sx * fe;
-        gv = sxv / g.V;
+        gv = OverV(g, sxv);
       }
       sgx[lj][hi] = gv;
     }
@@ -635,7 +635,7 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
         const double fn = jj < ny - 1 ? Mid(uc, su[hj + 2][li + 2]) : uc;
         double syv = (-g.sy) * fs;
         syv += g.sy * fn;
-        gv = syv / g.V;
+        gv = OverV(g, syv);
       }
       sgy[hj][li] = gv;
     }
@@ -662,7 +662,7 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       fsum += ue * ve * 1.;
       fsum += us_ * vs * -1.;
       fsum += uN * vn * 1.;
-      fl.un[q][c] = uc + -dt / g.V * fsum;
+      fl.un[q][c] = uc + OverV(g, -dt) * fsum;
     }
   }
 }
@@ -927,12 +927,12 @@ KHeatRows(Geo g, HeatFaces hb, HeatIn in, double* __restrict__ rs_, double* __re
       }
     }
   }
-  const double cen = (Dc / V) + ((Cc / V + in.bdf2) * 1.);
+  const double cen = OverV(g, Dc) + ((OverV(g, Cc) + in.bdf2) * 1.);
   double nb[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) nb[q] = (Dnb[q] / V) + ((Cnb[q] / V) * 1.);
+  for (int q = 0; q < 4; ++q) nb[q] = OverV(g, Dnb[q]) + (OverV(g, Cnb[q]) * 1.);
   const double un = in.bdf0 * ttp + in.bdf1 * ttc;
-  double k = ((Dk / V) + ((Ck / V + un) * 1.)) + (-0.);
+  double k = (OverV(g, Dk) + ((OverV(g, Ck) + un) * 1.)) + (-0.);
   if (hs) k += t.s * nb[2];
   if (hw) k += t.w * nb[0];
   k += t.c * cen;
