@@ -88,6 +88,12 @@ __global__ void KExtrapolate(long nc, long nf, ExtrapLayers l, double ge) {
   }
 }
 
+// inout += x.
+__global__ void KAddInPlace(long n, const double* __restrict__ x, double* __restrict__ inout) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) inout[i] = inout[i] + x[i];
+}
+
 // One field's initial guess: out = tc + (tc - tp) * ge.
 __global__ void KGuess(long n, const double* __restrict__ tc, const double* __restrict__ tp,
                        double ge, double* __restrict__ out) {
@@ -670,6 +676,23 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
                          cfg.pressure_relaxation_factor, c->p_ic, c->u_ic[0], c->u_ic[1],
                          c->u_ip[0], c->u_ip[1], c->rs_part, (double*)c->rs_d);
   hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic);
+  if (cfg.simpler) {
+    // SIMPLER (fluid.hpp:1059-1151): a second solve on the same pressure
+    // rows with the right-hand side of KSimplerRhs, added to p; the
+    // iteration's velocity gradients are dead by now and hold the scratch
+    hg::Rows vrows2{c->As, c->Aw, c->Ac, c->Ae, c->An};
+    hg::Rows prow{c->Ps, c->Pw, c->Pc, c->Pe, c->Pn};
+    hg::LaunchSimpler(c->st, c->geo, vrows2, c->k0, c->k1, c->u_ic[0], c->u_ic[1], c->u_ip[0],
+                      c->u_ip[1], c->frx, c->fry, c->gpx, c->gpy, c->ffex, c->ffey, c->ap,
+                      c->fl_ic, cfg.rhie_chow_factor, prow, c->p_ic, c->pinned,
+                      cfg.pressure_fixed_value, c->g0x, c->g0y, c->Pk);
+    const hg_iter_stats first = *is;
+    SolvePressure(c, is);
+    KAddInPlace<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->pcorr, c->p_ic);
+    is->pressure_solve_iters += first.pressure_solve_iters;
+    is->pressure_solve_diff = first.pressure_solve_diff;
+    is->pressure_converged = first.pressure_converged && is->pressure_converged;
+  }
   HG_CHECK(hipMemcpyAsync(c->rs_h, c->rs_d, 8, hipMemcpyDeviceToHost, c->st));
   c->pending_rs = true;
   c->fluid_iter += 1;
@@ -744,6 +767,9 @@ hg_status Guard(hg_ctx* ctx, const std::function<void()>& fn) {
 // ===========================================================================
 extern "C" {
 
+// Message of the last failed hg_create on this thread.
+static thread_local std::string g_create_error;
+
 hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
   if (!cfg_in || !out) return HG_ERR_BAD_ARG;
   *out = nullptr;
@@ -759,8 +785,11 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     if (cfg.unsupported_flags)
       throw HgError(HG_ERR_UNSUPPORTED, "script enables features outside this path (flags " +
                                             std::to_string(cfg.unsupported_flags) + ")");
-    if (cfg.simpler || cfg.force_geometric_average)
-      throw HgError(HG_ERR_UNSUPPORTED, "simpler / force_geometric_average are not supported");
+    if (cfg.force_geometric_average)
+      throw HgError(HG_ERR_UNSUPPORTED, "force_geometric_average is not supported");
+    if (cfg.simpler && FaceSolver(cfg.linear_solver_pressure))
+      throw HgError(HG_ERR_UNSUPPORTED,
+                    "simpler runs with the row solvers (gauss_seidel, jacobi, lu), not cg / fcg");
     if (cfg.sigma != 0. && cfg.num_phases >= 2)
       throw HgError(HG_ERR_UNSUPPORTED, "sigma != 0 is not supported");
     if (std::abs(cfg.sharp) > 1e-10) throw HgError(HG_ERR_UNSUPPORTED, "sharp != 0 is not supported");
@@ -911,9 +940,8 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
   });
   if (rc != HG_OK) {
     // keep the message reachable: hand back the context only on success
-    static thread_local std::string last;
-    last = c->err;
-    std::fprintf(stderr, "hg_create: %s\n", last.c_str());
+    g_create_error = c->err;
+    std::fprintf(stderr, "hg_create: %s\n", g_create_error.c_str());
     return rc;
   }
   *out = c.release();
@@ -922,7 +950,9 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
 
 void hg_destroy(hg_ctx* ctx) { delete ctx; }
 
-const char* hg_last_error(const hg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* hg_last_error(const hg_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
 int32_t hg_num_cells(const hg_ctx* ctx) { return ctx ? int32_t(ctx->nc) : 0; }
 int32_t hg_num_faces(const hg_ctx* ctx) { return ctx ? int32_t(ctx->nf) : 0; }
 

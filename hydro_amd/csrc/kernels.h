@@ -136,6 +136,18 @@ void LaunchLuRb(hipStream_t st, int nx, int ny, Rows a, const double* k0, const 
                 double* xb0, double* xb1, const double* base0, const double* base1,
                 double* out0, double* out1);
 
+// SIMPLER's second pressure right-hand side (fluid.hpp:1059-1151) into k,
+// from the velocity rows and constants of the iteration, the corrected and
+// previous velocity, restored force, pressure gradient, face force, a_P, the
+// corrected flux and the pressure rows at the current pressure p; ev0 / ev1
+// are scratch cell arrays.
+void LaunchSimpler(hipStream_t st, const Geo& g, Rows vel, const double* k0, const double* k1,
+                   const double* uc0, const double* uc1, const double* up0, const double* up1,
+                   const double* frx, const double* fry, const double* gpx, const double* gpy,
+                   const double* ffex, const double* ffey, const double* ap, const double* flux,
+                   double rc, Rows pres, const double* p, int pinned, double pval, double* ev0,
+                   double* ev1, double* k);
+
 // ---- Poisson / CG (cg.hip) ----
 struct CgSystem;  // opaque, defined in cg.hip
 

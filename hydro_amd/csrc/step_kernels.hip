@@ -787,6 +787,109 @@ __global__ void KStatsFinal(const double* __restrict__ part, int nblocks, double
   out[q] = x;
 }
 
+// ---- SIMPLER (fluid.hpp:1059-1151) -------------------------------------------
+// The momentum rows of the iteration evaluated on the velocity change
+// u_curr - u_prev (Expression::Evaluate, terms S, W, C, E, N), plus the
+// restored force minus the pressure gradient, per component.
+__global__ void __launch_bounds__(kBlock)
+KSimplerCells(Geo g, Rows a, const double* __restrict__ k0, const double* __restrict__ k1,
+              const double* __restrict__ uc0, const double* __restrict__ uc1,
+              const double* __restrict__ up0, const double* __restrict__ up1,
+              const double* __restrict__ frx, const double* __restrict__ fry,
+              const double* __restrict__ gpx, const double* __restrict__ gpy,
+              double* __restrict__ ev0, double* __restrict__ ev1) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const Cross c0 = LoadCross(uc0, nx, ny, i, j), p0 = LoadCross(up0, nx, ny, i, j);
+  const Cross c1 = LoadCross(uc1, nx, ny, i, j), p1 = LoadCross(up1, nx, ny, i, j);
+  const double as = a.s[c], aw = a.w[c], ac = a.c[c], ae = a.e[c], an = a.n[c];
+  const double kk0 = k0[c], kk1 = k1[c];
+  const double f0 = frx[c], f1 = fry[c], g0 = gpx[c], g1 = gpy[c];
+  double r0 = kk0, r1 = kk1;
+  if (j > 0) {
+    r0 += (c0.s - p0.s) * as;
+    r1 += (c1.s - p1.s) * as;
+  }
+  if (i > 0) {
+    r0 += (c0.w - p0.w) * aw;
+    r1 += (c1.w - p1.w) * aw;
+  }
+  r0 += (c0.c - p0.c) * ac;
+  r1 += (c1.c - p1.c) * ac;
+  if (i < nx - 1) {
+    r0 += (c0.e - p0.e) * ae;
+    r1 += (c1.e - p1.e) * ae;
+  }
+  if (j < ny - 1) {
+    r0 += (c0.n - p0.n) * an;
+    r1 += (c1.n - p1.n) * an;
+  }
+  ev0[c] = r0 + (f0 - g0);
+  ev1[c] = r1 + (f1 - g1);
+}
+
+// The second pressure right-hand side: per inner face
+//   ((ev_f - f_f) . S) / a_P,f + (flux - u_f . S) / rhie_chow_factor
+// (face means of the cell values), the outward sum over the cell's faces,
+// negated, then the pressure row evaluated at the current pressure; a fixed
+// cell keeps -value and its neighbours' rows stay without its term.
+__global__ void __launch_bounds__(kBlock)
+KSimplerRhs(Geo g, const double* __restrict__ ev0, const double* __restrict__ ev1,
+            const double* __restrict__ ffex, const double* __restrict__ ffey,
+            const double* __restrict__ ap, const double* __restrict__ flux,
+            const double* __restrict__ u0, const double* __restrict__ u1, double rc, Rows pr,
+            const double* __restrict__ p, int pinned, double pval, double* __restrict__ k) {
+  const int nx = g.nx, ny = g.ny;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nx * ny) return;
+  const int i = c % nx, j = c / nx;
+  const bool inner[4] = {i > 0, i < nx - 1, j > 0, j < ny - 1};
+  const int face[4] = {XFace(g, i, j), XFace(g, i + 1, j), YFace(g, i, j), YFace(g, i, j + 1)};
+  const int nb[4] = {c - 1, c + 1, c - nx, c + nx};
+  const double outf[4] = {-1., 1., -1., 1.};
+  const Cross e0 = LoadCross(ev0, nx, ny, i, j), e1 = LoadCross(ev1, nx, ny, i, j);
+  const Cross a = LoadCross(ap, nx, ny, i, j);
+  const Cross v0 = LoadCross(u0, nx, ny, i, j), v1 = LoadCross(u1, nx, ny, i, j);
+  const Cross pp = LoadCross(p, nx, ny, i, j);
+  double fl[4], fe[4];
+  for (int q = 0; q < 4; ++q) {
+    fl[q] = flux[face[q]];
+    fe[q] = q < 2 ? ffex[face[q]] : ffey[face[q]];
+  }
+  const double enb[4] = {e0.w, e0.e, e1.s, e1.n}, ec[4] = {e0.c, e0.c, e1.c, e1.c};
+  const double anb[4] = {a.w, a.e, a.s, a.n};
+  const double unb[4] = {v0.w, v0.e, v1.s, v1.n}, uc[4] = {v0.c, v0.c, v1.c, v1.c};
+  double sum = 0.;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double r = 0.;
+    if (inner[q]) {
+      const bool self_m = (q == 1 || q == 3);  // the cell is the face's lower-index side
+      const double S = q < 2 ? g.sx : g.sy;
+      const double em = self_m ? ec[q] : enb[q], ep = self_m ? enb[q] : ec[q];
+      const double am = self_m ? a.c : anb[q], apl = self_m ? anb[q] : a.c;
+      const double um = self_m ? uc[q] : unb[q], up = self_m ? unb[q] : uc[q];
+      const double evf = Mid(em, ep) - fe[q];
+      r = (evf * S) / Mid(am, apl) + (fl[q] - Mid(um, up) * S) / rc;
+    }
+    sum += r * outf[q];
+  }
+  double kk = -sum;
+  if (c == pinned) {
+    kk = -pval;
+    kk += pp.c * pr.c[c];
+  } else {
+    if (inner[2] && nb[2] != pinned) kk += pp.s * pr.s[c];
+    if (inner[0] && nb[0] != pinned) kk += pp.w * pr.w[c];
+    kk += pp.c * pr.c[c];
+    if (inner[1] && nb[1] != pinned) kk += pp.e * pr.e[c];
+    if (inner[3] && nb[3] != pinned) kk += pp.n * pr.n[c];
+  }
+  k[c] = kk;
+}
+
 // Per-phase cell fields and a per-phase constant (density, conductivity).
 struct PhaseCells {
   const double* alpha[3];
@@ -1174,6 +1277,18 @@ void LaunchHeatRows(hipStream_t st, const Geo& g, const HeatFaces& hb, const dou
   in.gx = gx;
   in.gy = gy;
   KHeatRows<<<grid, kBlock, 0, st>>>(g, hb, in, s, w, c, e, n, k);
+}
+void LaunchSimpler(hipStream_t st, const Geo& g, Rows vel, const double* k0, const double* k1,
+                   const double* uc0, const double* uc1, const double* up0, const double* up1,
+                   const double* frx, const double* fry, const double* gpx, const double* gpy,
+                   const double* ffex, const double* ffey, const double* ap, const double* flux,
+                   double rc, Rows pres, const double* p, int pinned, double pval, double* ev0,
+                   double* ev1, double* k) {
+  const dim3 grid = Grid(long(g.nx) * g.ny);
+  KSimplerCells<<<grid, kBlock, 0, st>>>(g, vel, k0, k1, uc0, uc1, up0, up1, frx, fry, gpx, gpy,
+                                         ev0, ev1);
+  KSimplerRhs<<<grid, kBlock, 0, st>>>(g, ev0, ev1, ffex, ffey, ap, flux, uc0, uc1, rc, pres, p,
+                                       pinned, pval, k);
 }
 int BoundaryFluxBlocks(const Geo& g) { return (2 * (g.nx + g.ny) + kBlock - 1) / kBlock; }
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,

@@ -33,10 +33,13 @@ class Unsupported(HydroError):
     pass
 
 
-def _raise(lib, ctx, rc, what):
+def _raise(lib, ctx, rc, what, create=False):
     if rc == capi.HG_OK:
         return
-    msg = lib.hg_last_error(ctx).decode() if ctx else ""
+    if ctx:
+        msg = lib.hg_last_error(ctx).decode()
+    else:  # hg_last_error(NULL): the last failed hg_create's message
+        msg = lib.hg_last_error(None).decode() if create else ""
     text = f"{what}: {capi.STATUS_NAMES.get(rc, rc)} {msg}".strip()
     if rc == capi.HG_ERR_NAN:
         raise HydroNaN(msg or "NaN")
@@ -57,7 +60,7 @@ class Hydro:
         rc = self.lib.hg_create(ctypes.byref(config), ctypes.byref(self.ctx))
         if rc != capi.HG_OK:
             self.ctx = ctypes.c_void_p()
-            _raise(self.lib, None, rc, "hg_create")
+            _raise(self.lib, None, rc, "hg_create", create=True)
         self.nc = self.lib.hg_num_cells(self.ctx)
         self.nf = self.lib.hg_num_faces(self.ctx)
 

@@ -255,6 +255,8 @@ typedef struct hg_ctx hg_ctx;
 
 hg_status hg_create(const hg_config* cfg, hg_ctx** out);
 void hg_destroy(hg_ctx* ctx);
+/* The context's last error; with NULL, that of the last failed hg_create
+ * on this thread. */
 const char* hg_last_error(const hg_ctx* ctx);
 int32_t hg_num_cells(const hg_ctx* ctx);
 int32_t hg_num_faces(const hg_ctx* ctx);

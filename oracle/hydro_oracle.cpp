@@ -753,7 +753,6 @@ int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
   const hg_config& cfg = *cfg_in;
   if (cfg.nx < 2 || cfg.ny < 2) return fail("Nx, Ny must be >= 2");
   if (cfg.num_phases < 1 || cfg.num_phases > HG_MAX_PHASES) return fail("num_phases 1..3");
-  if (cfg.simpler) return fail("simpler=1 is not part of this path");
   if (cfg.force_geometric_average) return fail("force_geometric_average=1 unsupported");
   or_ctx* o = new or_ctx();
   o->cfg = cfg;
@@ -1269,6 +1268,79 @@ int or_fluid_iterate(or_ctx* o, hg_iter_stats* stats) {
       }
       o->fl_ic[f] = v;
     }
+  }
+  if (cfg.simpler) {
+    // SIMPLER pressure correction (fluid.hpp:1059-1151): the momentum rows
+    // of this iteration evaluated on the velocity change, plus restored
+    // force minus the pressure gradient, give face values whose Rhie-Chow
+    // balance with the corrected flux is the right-hand side of a second
+    // pressure solve on the same rows; its result is added to p.
+    ScopedTimer t(&o->timers, "fluid.8.simpler");
+    const double rc = cfg.rhie_chow_factor;
+    Vec fux(nf), fuy(nf);
+    Interpolate(m, o->u_ic[0].data(), fux.data(), VelBc(o, 0));
+    Interpolate(m, o->u_ic[1].data(), fuy.data(), VelBc(o, 1));
+    Vec dl[2], ev[2];
+    for (int d = 0; d < 2; ++d) {
+      dl[d].resize(nc);
+      for (size_t c = 0; c < nc; ++c) dl[d][c] = o->u_ic[d][c] - o->u_ip[d][c];
+    }
+    for (int d = 0; d < 2; ++d) {
+      ev[d].resize(nc);
+      const Vec& x = dl[d];
+      for (size_t c = 0; c < nc; ++c) {
+        // Expression::Evaluate (linear.hpp:129-135), terms S, W, C, E, N
+        const int i = int(c % m.nx), j = int(c / m.nx);
+        double r = o->vel_k[d][c];
+        if (j > 0) r += x[c - m.nx] * A.s[c];
+        if (i > 0) r += x[c - 1] * A.w[c];
+        r += x[c] * A.c[c];
+        if (i < m.nx - 1) r += x[c + 1] * A.e[c];
+        if (j < m.ny - 1) r += x[c + m.nx] * A.n[c];
+        ev[d][c] = r;
+      }
+    }
+    for (size_t c = 0; c < nc; ++c) {
+      ev[0][c] += frx[c] - gpx[c];
+      ev[1][c] += fry[c] - gpy[c];
+    }
+    Vec fe0(nf), fe1(nf), rhs(nf, 0.);
+    Interpolate(m, ev[0].data(), fe0.data(), MakeBc(BC_NONE));
+    Interpolate(m, ev[1].data(), fe1.data(), MakeBc(BC_NONE));
+    for (size_t f = 0; f < nf; ++f) {
+      if (!m.Inner(f)) continue;
+      const V2 e{fe0[f] - ffex[f], fe1[f] - ffey[f]};
+      rhs[f] = Dot(e, m.surf[f]) / ffd[f] +
+               (o->fl_ic[f] - Dot(V2{fux[f], fuy[f]}, m.surf[f])) / rc;
+    }
+    const long pin = o->pinned;
+    for (size_t c = 0; c < nc; ++c) {
+      double sum = 0.;
+      for (int q = 0; q < 4; ++q) sum += rhs[m.Face(c, q)] * Mesh::Out(q);
+      P.k[c] = -sum;
+    }
+    if (pin >= 0) P.k[size_t(pin)] = -cfg.pressure_fixed_value;
+    // constant = the row evaluated at the current pressure (terms removed
+    // for a fixed cell stay removed)
+    for (size_t c = 0; c < nc; ++c) {
+      const int i = int(c % m.nx), j = int(c / m.nx);
+      const long lc = long(c);
+      double r = P.k[c];
+      if (lc == pin) {
+        r += o->p_ic[c] * P.c[c];
+      } else {
+        if (j > 0 && lc - m.nx != pin) r += o->p_ic[c - m.nx] * P.s[c];
+        if (i > 0 && lc - 1 != pin) r += o->p_ic[c - 1] * P.w[c];
+        r += o->p_ic[c] * P.c[c];
+        if (i < m.nx - 1 && lc + 1 != pin) r += o->p_ic[c + 1] * P.e[c];
+        if (j < m.ny - 1 && lc + m.nx != pin) r += o->p_ic[c + m.nx] * P.n[c];
+      }
+      P.k[c] = r;
+    }
+    Vec pc2;
+    SolveResult pr2 = RunSolver(o, cfg.linear_solver_pressure, P, P.k, pc2, pin < 0);
+    for (size_t c = 0; c < nc; ++c) o->p_ic[c] += pc2[c];
+    pr.iters += pr2.iters;
   }
   o->fluid_iter += 1;
   if (stats) {

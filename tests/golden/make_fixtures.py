@@ -53,6 +53,12 @@ FIXTURES = {
     "ref_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), 6, (1, 6)),
     "ref_cavity32_heat": (cases.cavity(32, extra=cases.HEAT_BOTTOM), 5, (1, 5)),
     "ref_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_LEFT), 4, (4,)),
+    "ref_cavity32_simpler": (cases.cavity(32, extra="set bool simpler 1"), 5, (1, 5)),
+    "ref_dam32x16_simpler": (cases.dam(32, 16, extra="set bool simpler 1"), 4, (4,)),
+    "ref_cavity32_simpler_pinned": (cases.cavity(32, solver_p="jacobi", extra=(
+        "set bool simpler 1\nset double lu_relaxed_relaxation_factor 0.9\n"
+        "set vect pressure_fixed_point (0.3, 0.6)\nset double pressure_fixed_value 0.1\n")),
+        4, (4,)),
 }
 
 

@@ -101,6 +101,32 @@ def test_heat_matches_oracle(gpu, case):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["cavity32", "dam32x16", "cavity32_jacobi_pinned"])
+def test_simpler_matches_oracle(gpu, case):
+    """SIMPLER's second pressure solve (fluid.hpp:1059-1151) bit for bit with
+    the oracle, which matches the reference module (fixtures ref_*_simpler,
+    the Jacobi case with a fixed pressure cell included)."""
+    print(f"case: simpler {case}, 4 steps", flush=True)
+    if case == "cavity32":
+        text = cases.cavity(32, extra="set bool simpler 1")
+    elif case == "dam32x16":
+        text = cases.dam(32, 16, extra="set bool simpler 1")
+    else:
+        text = cases.cavity(32, solver_p="jacobi", extra=(
+            "set bool simpler 1\nset double lu_relaxed_relaxation_factor 0.9\n"
+            "set vect pressure_fixed_point (0.3, 0.6)\nset double pressure_fixed_value 0.1\n"))
+    g, o, worst = run_pair(text, 4)
+    for name, f in FIELDS:
+        assert np.array_equal(g.get(f), o.get(f)), name
+    g.close()
+
+
+def test_simpler_refuses_face_solvers(gpu):
+    from hydro_amd.hydro import Unsupported
+    with pytest.raises(Unsupported, match="simpler"):
+        Hydro(cases.cavity(16, solver_p="cg", extra="set bool simpler 1"))
+
+
 @pytest.mark.parametrize("solver", ["lu", "gauss_seidel", "jacobi", "lu_rb"])
 def test_stencil_solvers_bitwise(gpu, solver):
     print(f"case: stencil solve {solver} 48x40", flush=True)
