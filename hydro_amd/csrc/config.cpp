@@ -252,8 +252,16 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
           c->bc_kind[side] = HG_BC_WALL;
           c->bc_velocity[side][0] = vx;
           c->bc_velocity[side][1] = vy;
-        } else if (kind == "inlet" || kind == "outlet") {
-          c->unsupported_flags |= HG_UNSUP_CONDITION;
+        } else if (kind == "inlet") {
+          double vx = 0, vy = 0;
+          cs >> vx >> vy;
+          c->bc_kind[side] = HG_BC_INLET;
+          c->bc_velocity[side][0] = vx;
+          c->bc_velocity[side][1] = vy;
+        } else if (kind == "outlet") {
+          c->bc_kind[side] = HG_BC_OUTLET;
+          c->bc_velocity[side][0] = 0.;
+          c->bc_velocity[side][1] = 0.;
         } else {
           return fail(line_no, "unknown boundary condition '" + kind + "'");
         }

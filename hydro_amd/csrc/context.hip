@@ -188,6 +188,12 @@ struct hg_ctx {
   double* hb_val = nullptr;
   double *kcond = nullptr, *tgx = nullptr, *tgy = nullptr;
   double hbdf[3] = {0., 0., 0.};
+  // inlets and outlets: per-boundary-face velocities (bc.fvel points into
+  // bvel_d; outlets are updated every iteration), the inlet partial
+  // densities, scratch of the outlet update
+  bool open_bc = false, any_inlet = false, any_outlet = false;
+  int side_kind[4] = {0, 0, 0, 0};
+  double *bvel_d = nullptr, *pdb_d = nullptr, *obc_scratch = nullptr;
   std::vector<double> iter_rs;
   double *bflux_acc = nullptr, *bflux_part = nullptr;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
@@ -462,6 +468,27 @@ void AutoTimeStep(hg_ctx* c) {
   SetTimeSteps(c, dtm * c->cfg.cfl, dtm * c->cfg.cfl_advection);
 }
 
+// Fluid conditions per boundary face (hydro2d.hpp:370-392): walls and
+// inlets hold their side's velocity, outlets start at zero
+// (fluid.hpp:318-327); the inlet partial densities are the adjacent cells'
+// (hydro2d.hpp:565-583).
+void InitBoundary(hg_ctx* c) {
+  const int nx = c->geo.nx, ny = c->geo.ny;
+  const long nb = 2L * (nx + ny);
+  std::vector<double> v(static_cast<size_t>(2 * nb), 0.);
+  for (long b = 0; b < nb; ++b) {
+    const int s = b < ny ? HG_SIDE_LEFT : b < 2 * ny ? HG_SIDE_RIGHT
+                : b < 2 * ny + nx ? HG_SIDE_BOTTOM : HG_SIDE_TOP;
+    if (c->side_kind[s] == HG_BC_OUTLET) continue;
+    v[size_t(b)] = c->cfg.bc_velocity[s][0];
+    v[size_t(nb + b)] = c->cfg.bc_velocity[s][1];
+  }
+  Upload(c, c->bvel_d, v);
+  const double* pdv[3] = {c->pd[0], c->pd[1], c->pd[2]};
+  hg::LaunchGatherInlet(c->st, c->geo, c->side_kind, pdv, c->np, c->pdb_d);
+  c->Sync();  // v goes out of scope
+}
+
 void InitialFlux(hg_ctx* c) {
   hg::LaunchInitialFlux(c->st, c->geo, c->bc, c->u_tc[0], c->u_tc[1], c->cfg.meshvel[0],
                         c->cfg.meshvel[1], c->fl_tc);
@@ -638,6 +665,12 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
   hg_iter_stats* is = &c->last_iter;
   std::memset(is, 0, sizeof(*is));
   const hg_config& cfg = c->cfg;
+  // outlet velocities from the current iterate (fluid.hpp:820-821)
+  if (c->any_outlet) {
+    const long nb = 2L * (c->geo.nx + c->geo.ny);
+    hg::LaunchOutletUpdate(c->st, c->geo, c->side_kind, c->u_ic[0], c->u_ic[1], c->bvel_d,
+                           c->bvel_d + nb, c->obc_scratch);
+  }
   // iter_prev <- iter_curr (fluid.hpp:815-818, conv_diff.hpp:132-134); the
   // iter_curr buffers are rewritten in full below.
   std::swap(c->p_ip, c->p_ic);
@@ -738,7 +771,14 @@ void AdvectSubstep(hg_ctx* c) {
   const double* u[3] = {c->pd[0], c->np > 1 ? c->pd[1] : nullptr, c->np > 2 ? c->pd[2] : nullptr};
   double* un[3] = {c->pd_new[0], c->np > 1 ? c->pd_new[1] : nullptr,
                    c->np > 2 ? c->pd_new[2] : nullptr};
-  hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un);
+  const long nb = 2L * (c->geo.nx + c->geo.ny);
+  const double* bval[3] = {nullptr, nullptr, nullptr};
+  if (c->any_inlet)
+    for (int i = 0; i < c->np; ++i) bval[i] = c->pdb_d + i * nb;
+  int inlet[4];
+  for (int s = 0; s < 4; ++s) inlet[s] = c->side_kind[s] == HG_BC_INLET ? 1 : 0;
+  hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un,
+                    c->any_inlet ? bval : nullptr, inlet);
   for (int i = 0; i < c->np; ++i) std::swap(c->pd[i], c->pd_new[i]);
   c->t_adv += c->dt_adv;
 }
@@ -802,6 +842,12 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     c->geo = c->mesh.geo;
     for (int d = 0; d < 2; ++d)
       for (int s = 0; s < 4; ++s) c->bc.vel[d][s] = cfg.bc_velocity[s][d];
+    for (int s = 0; s < 4; ++s) {
+      c->side_kind[s] = cfg.bc_kind[s];
+      c->any_inlet = c->any_inlet || cfg.bc_kind[s] == HG_BC_INLET;
+      c->any_outlet = c->any_outlet || cfg.bc_kind[s] == HG_BC_OUTLET;
+    }
+    c->open_bc = c->any_inlet || c->any_outlet;
     c->ph.n = c->np;
     for (int i = 0; i < 3; ++i) {
       c->ph.rho[i] = cfg.density[i];
@@ -933,6 +979,15 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     for (int i = 1; i < c->np; ++i)
       Smooth(c.get(), c->pd[i], c->pd[i], cfg.initial_volume_fraction_smooth_times);
     hg::LaunchPhase0Init(c->st, c->geo, c->ph, c->pd[1], c->pd[2], c->pd[0]);
+    if (c->open_bc) {
+      const long nb = 2L * (cfg.nx + cfg.ny);
+      c->bvel_d = c->Alloc(2 * nb);
+      c->pdb_d = c->Alloc(c->np * nb);
+      c->obc_scratch = c->Alloc(2 * nb + 1);
+      c->bc.fvel[0] = c->bvel_d;
+      c->bc.fvel[1] = c->bvel_d + nb;
+      InitBoundary(c.get());
+    }
     InitialFlux(c.get());
     UpdateProperties(c.get());
     InitStats(c.get());
@@ -1062,6 +1117,7 @@ hg_status hg_refresh(hg_ctx* c) {
   if (!c) return HG_ERR_BAD_ARG;
   return Guard(c, [&]() {
     c->fields_checked = false;
+    if (c->open_bc) InitBoundary(c);
     InitialFlux(c);
     UpdateProperties(c);
     InitStats(c);

@@ -61,6 +61,18 @@ __device__ __forceinline__ double Extrap(const Geo& g, int s, double uc, double 
 // Face values of a cell field at the four faces (W, E, S, N) of cell (i,j).
 // kind: 0 = value condition (bv per side), 1 = zero derivative,
 // 2 = extrapolation.
+// Index of the boundary face on side s at position q along it.
+__device__ __forceinline__ int BoundaryIndex(const Geo& g, int s, int q) {
+  return s == kLeft ? q : s == kRight ? g.ny + q : s == kBottom ? 2 * g.ny + q
+                                                                : 2 * g.ny + g.nx + q;
+}
+// Signed distance factor of the zero-index side's valid cell: the valid
+// neighbour of a left / bottom face is cell 1 (factor -1), of a right / top
+// face cell 0 (factor +1).
+__device__ __forceinline__ double SideFactor(int s) {
+  return (s == kLeft || s == kBottom) ? -1. : 1.;
+}
+
 // The cell's value and its four neighbours' (W, E, S, N), loaded together;
 // a missing neighbour reads the cell itself (its value is then not used).
 struct Cross {

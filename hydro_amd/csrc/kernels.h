@@ -10,6 +10,10 @@ namespace hg {
 // Velocity boundary values per side (wall velocity), per component.
 struct Bc {
   double vel[2][4];
+  // per boundary face (left j, right ny + j, bottom 2 ny + i, top
+  // 2 ny + nx + i) when the values vary along a side (outlets); nullptr:
+  // the per-side values above
+  const double* fvel[2] = {nullptr, nullptr};
 };
 
 struct Phases {
@@ -73,9 +77,20 @@ void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const
                         const double* pc, double* flux);
 void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                        const double* u1, double mvx, double mvy, double* flux);
-// One advection sub-step of nf (1..3) fields with gradients formed in place.
+// One advection sub-step of nf (1..3) fields with gradients formed in place;
+// bval (may be NULL): per field, the inlet value of each boundary face, on
+// the sides with inlet[s] != 0.
 void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
-                   const double* flux, double dt, double* const un[3]);
+                   const double* flux, double dt, double* const un[3],
+                   const double* const bval[3] = nullptr, const int* inlet = nullptr);
+// Inlets and outlets (kind[s]: hg_bc_kind per side).  Inlet values of the
+// phases into pdb[p * nb + b] (nb = 2 (nx + ny) boundary faces); the outlet
+// update of the boundary velocities bv0 / bv1 from the cells' u0 / u1
+// (FluidSimple::UpdateOutletBaseConditions); scratch holds 2 nb + 1 values.
+void LaunchGatherInlet(hipStream_t st, const Geo& g, const int kind[4], const double* const pd[3],
+                       int np, double* pdb);
+void LaunchOutletUpdate(hipStream_t st, const Geo& g, const int kind[4], const double* u0,
+                        const double* u1, double* bv0, double* bv1, double* scratch);
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy);
 void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out);

@@ -18,6 +18,12 @@ constexpr int kBlock = 256;
 
 inline dim3 Grid(long n) { return dim3(unsigned((n + kBlock - 1) / kBlock)); }
 
+// Velocity component `comp` of the fluid condition on side s at position q
+// (j on the left / right, i on the bottom / top).
+__device__ __forceinline__ double BVel(const Bc& bc, const Geo& g, int comp, int s, int q) {
+  return bc.fvel[comp] ? bc.fvel[comp][BoundaryIndex(g, s, q)] : bc.vel[comp][s];
+}
+
 // ---------------------------------------------------------------------------
 // Face interpolation of the per-step inputs (fluid.hpp:602-641).
 __global__ void __launch_bounds__(kBlock)
@@ -114,9 +120,15 @@ KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict_
   double f[4];
   CellFacesV<2>(g, xp, i, j, nullptr, f);
   GradFromFaces(g, f, gpx[c], gpy[c]);
-  CellFacesV<0>(g, x0, i, j, bc.vel[0], f);
+  double bv0[4], bv1[4];
+  const int qs[4] = {j, j, i, i};
+  for (int s = 0; s < 4; ++s) {
+    bv0[s] = BVel(bc, g, 0, s, qs[s]);
+    bv1[s] = BVel(bc, g, 1, s, qs[s]);
+  }
+  CellFacesV<0>(g, x0, i, j, bv0, f);
   GradFromFaces(g, f, g0x[c], g0y[c]);
-  CellFacesV<0>(g, x1, i, j, bc.vel[1], f);
+  CellFacesV<0>(g, x1, i, j, bv1, f);
   GradFromFaces(g, f, g1x[c], g1y[c]);
 }
 
@@ -256,7 +268,7 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
     } else {
       const int sd = side[k];
       const double al = g.bal[sd];
-      const double v0 = bc.vel[0][sd], v1 = bc.vel[1][sd];
+      const double v0 = BVel(bc, g, 0, sd, sd < 2 ? j : i), v1 = BVel(bc, g, 1, sd, sd < 2 ? j : i);
       Ck0 += (v0 * F) * o;
       Ck1 += (v1 * F) * o;
       Dc += (((-al) * (-mu[k])) * A) * o;
@@ -318,18 +330,20 @@ KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict_
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   const bool xface = f < g.nfx;
-  int cm, cp, sd;
+  int cm, cp, sd, q;
   if (xface) {
     int i = f % (g.nx + 1), j = f / (g.nx + 1);
     cm = (i > 0) ? (i - 1) + g.nx * j : -1;
     cp = (i < g.nx) ? i + g.nx * j : -1;
     sd = (i == 0) ? kLeft : kRight;
+    q = j;
   } else {
     int r = f - g.nfx;
     int i = r % g.nx, j = r / g.nx;
     cm = (j > 0) ? i + g.nx * (j - 1) : -1;
     cp = (j < g.ny) ? i + g.nx * j : -1;
     sd = (j == 0) ? kBottom : kTop;
+    q = i;
   }
   const double S = xface ? g.sx : g.sy;
   double fs, fc = 0.;
@@ -348,7 +362,7 @@ KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict_
     fs = vfi + rc * (wide - compact) / af + 0;
     fc = -A / (dn * af);
   } else {
-    const double vb = xface ? bc.vel[0][sd] : bc.vel[1][sd];
+    const double vb = BVel(bc, g, xface ? 0 : 1, sd, q);
     fs = vb * S;
   }
   fs -= (xface ? mvx : mvy) * S;
@@ -507,25 +521,27 @@ KInitialFlux(Geo g, Bc bc, const double* __restrict__ u0, const double* __restri
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   const bool xface = f < g.nfx;
-  int cm, cp, sd;
+  int cm, cp, sd, q;
   if (xface) {
     int i = f % (g.nx + 1), j = f / (g.nx + 1);
     cm = (i > 0) ? (i - 1) + g.nx * j : -1;
     cp = (i < g.nx) ? i + g.nx * j : -1;
     sd = (i == 0) ? kLeft : kRight;
+    q = j;
   } else {
     int r = f - g.nfx;
     int i = r % g.nx, j = r / g.nx;
     cm = (j > 0) ? i + g.nx * (j - 1) : -1;
     cp = (j < g.ny) ? i + g.nx * j : -1;
     sd = (j == 0) ? kBottom : kTop;
+    q = i;
   }
   const double S = xface ? g.sx : g.sy;
   double un;
   if (cm >= 0 && cp >= 0)
     un = xface ? Mid(u0[cm], u0[cp]) : Mid(u1[cm], u1[cp]);
   else
-    un = xface ? bc.vel[0][sd] : bc.vel[1][sd];
+    un = BVel(bc, g, xface ? 0 : 1, sd, q);
   double v = un * S;
   v -= (xface ? mvx : mvy) * S;
   flux[f] = v;
@@ -552,6 +568,10 @@ __device__ __forceinline__ double SuperbeeFace(double uP, double uE, double gPn,
 struct AdvFields {
   const double* u[3];
   double* un[3];
+  // inlet values per boundary face of each field (nullptr: no inlet) and
+  // which sides are inlets; other boundary faces have zero derivative
+  const double* bval[3];
+  int inlet[4];
 };
 
 // One advection sub-step (advection.hpp:440-539) of NF fields sharing the
@@ -609,6 +629,11 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       if (t < NU) su[t / UX][t % UX] = ureg[q][k];
     }
     __syncthreads();
+    // a boundary face's value: the inlet's, or the cell's (zero derivative)
+    const double* bv = fl.bval[q];
+    auto bface = [&](int s, int pos, double uc) {
+      return (bv && fl.inlet[s]) ? bv[BoundaryIndex(g, s, pos)] : uc;
+    };
     // x-gradients on columns i0-1 .. i0+64 of the tile rows
     for (int t = threadIdx.x; t < kAdvTY * (kAdvTX + 2); t += kBlock) {
       const int lj = t / (kAdvTX + 2), hi = t % (kAdvTX + 2);
@@ -616,8 +641,8 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       double gv = 0.;
       if (ii >= 0 && ii < nx && jj < ny) {
         const double uc = su[lj + 2][hi + 1];
-        const double fw = ii > 0 ? Mid(su[lj + 2][hi], uc) : uc;
-        const double fe = ii < nx - 1 ? Mid(uc, su[lj + 2][hi + 2]) : uc;
+        const double fw = ii > 0 ? Mid(su[lj + 2][hi], uc) : bface(kLeft, jj, uc);
+        const double fe = ii < nx - 1 ? Mid(uc, su[lj + 2][hi + 2]) : bface(kRight, jj, uc);
         double sxv = (-g.sx) * fw;
         sxv += g.sx * fe;
         gv = OverV(g, sxv);
@@ -631,8 +656,8 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       double gv = 0.;
       if (ii < nx && jj >= 0 && jj < ny) {
         const double uc = su[hj + 1][li + 2];
-        const double fs = jj > 0 ? Mid(su[hj][li + 2], uc) : uc;
-        const double fn = jj < ny - 1 ? Mid(uc, su[hj + 2][li + 2]) : uc;
+        const double fs = jj > 0 ? Mid(su[hj][li + 2], uc) : bface(kBottom, ii, uc);
+        const double fn = jj < ny - 1 ? Mid(uc, su[hj + 2][li + 2]) : bface(kTop, ii, uc);
         double syv = (-g.sy) * fs;
         syv += g.sy * fn;
         gv = OverV(g, syv);
@@ -650,7 +675,8 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       const double vw = vw_[k], ve = ve_[k], vs = vs_[k], vn = vn_[k];
       const double uc = su[lj + 2][li + 2];
       const double gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
-      double uw = uc, ue = uc, us_ = uc, uN = uc;
+      double uw = i > 0 ? uc : bface(kLeft, j, uc), ue = i < nx - 1 ? uc : bface(kRight, j, uc);
+      double us_ = j > 0 ? uc : bface(kBottom, i, uc), uN = j < ny - 1 ? uc : bface(kTop, i, uc);
       if (i > 0) uw = SuperbeeFace(su[lj + 2][li + 1], uc, sgx[lj][li], gxc, g.tx0, g.tx1, vw);
       if (i < nx - 1)
         ue = SuperbeeFace(uc, su[lj + 2][li + 3], gxc, sgx[lj][li + 2], g.tx0, g.tx1, ve);
@@ -896,6 +922,132 @@ struct PhaseCells {
   double density[3];
 };
 
+// ---- inlets and outlets (fluid.hpp:542-600, hydro2d.hpp:565-583) ------------
+struct SideKinds {
+  int kind[4];  // hg_bc_kind per side
+};
+
+// The geometry of boundary face b: side, position along it, face, cell.
+__device__ __forceinline__ void BoundaryFace(const Geo& g, int b, int& s, int& q, int& face,
+                                             int& cell) {
+  if (b < 2 * g.ny) {
+    s = b < g.ny ? kLeft : kRight;
+    q = b < g.ny ? b : b - g.ny;
+    face = XFace(g, s == kLeft ? 0 : g.nx, q);
+    cell = (s == kLeft ? 0 : g.nx - 1) + g.nx * q;
+  } else {
+    const int r = b - 2 * g.ny;
+    s = r < g.nx ? kBottom : kTop;
+    q = r < g.nx ? r : r - g.nx;
+    face = YFace(g, q, s == kBottom ? 0 : g.ny);
+    cell = q + g.nx * (s == kBottom ? 0 : g.ny - 1);
+  }
+}
+
+// u . S of a boundary face (x-faces S = (sx, -0), y-faces S = (+0, sy)), in
+// Vect::dot's order.
+__device__ __forceinline__ double BoundaryDot(const Geo& g, int s, double ux, double uy) {
+  const bool xf = s < 2;
+  const double Sx = xf ? g.sx : 0., Sy = xf ? -0. : g.sy;
+  double d = 0.;
+  d += ux * Sx;
+  d += uy * Sy;
+  return d;
+}
+
+// Inlet values of the phases: the adjacent cells' partial densities.
+__global__ void KGatherInlet(Geo g, SideKinds sk, PhaseCells pd, int np, double* __restrict__ pdb) {
+  const int nb = 2 * (g.nx + g.ny);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int s, q, face, cell;
+  BoundaryFace(g, b, s, q, face, cell);
+  for (int p = 0; p < np; ++p) pdb[p * nb + b] = sk.kind[s] == 1 ? pd.alpha[p][cell] : 0.;
+}
+
+// Outlet faces take the adjacent cell's velocity; per face, the terms of
+// the outflow (u . S toward the outside) and area, or of the inflow.
+__global__ void KOutletTerms(Geo g, SideKinds sk, const double* __restrict__ u0,
+                             const double* __restrict__ u1, double* __restrict__ bv0,
+                             double* __restrict__ bv1, double* __restrict__ term,
+                             double* __restrict__ area) {
+  const int nb = 2 * (g.nx + g.ny);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int s, q, face, cell;
+  BoundaryFace(g, b, s, q, face, cell);
+  (void)face;
+  double t = 0., a = 0.;
+  if (sk.kind[s] == 2) {
+    const double ux = u0[cell], uy = u1[cell];
+    bv0[b] = ux;
+    bv1[b] = uy;
+    t = BoundaryDot(g, s, ux, uy) * SideFactor(s);
+    a = s < 2 ? g.ax : g.ay;
+  } else if (sk.kind[s] == 1) {
+    t = BoundaryDot(g, s, bv0[b], bv1[b]) * (-SideFactor(s));
+  }
+  term[b] = t;
+  area[b] = a;
+}
+
+// The sums in the reference's face order (rows of left / right x-faces,
+// then the bottom and top y-faces), one thread, staged through LDS; then
+// corr = (inflow - outflow) / outlet area.
+__global__ void __launch_bounds__(kBlock)
+KOutletSum(Geo g, SideKinds sk, const double* __restrict__ term,
+           const double* __restrict__ area, double* __restrict__ corr) {
+  const int nb = 2 * (g.nx + g.ny);
+  __shared__ double st[kBlock], sa[kBlock];
+  __shared__ int sk_[kBlock];
+  double in = 0., out = 0., ar = 0.;
+  for (int k0 = 0; k0 < nb; k0 += kBlock) {
+    const int k = k0 + threadIdx.x;
+    if (k < nb) {
+      const int b = k < 2 * g.ny ? (k & 1) * g.ny + (k >> 1) : k;
+      int s, q, face, cell;
+      BoundaryFace(g, b, s, q, face, cell);
+      st[threadIdx.x] = term[b];
+      sa[threadIdx.x] = area[b];
+      sk_[threadIdx.x] = sk.kind[s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int n = nb - k0 < kBlock ? nb - k0 : kBlock;
+      for (int t = 0; t < n; ++t) {
+        if (sk_[t] == 2) {
+          out += st[t];
+          ar += sa[t];
+        } else if (sk_[t] == 1) {
+          in += st[t];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    in += 0.;  // the volume sources (zero on this path) times the cell volumes
+    corr[0] = (in - out) / ar;
+  }
+}
+
+// Outlet velocity += normal * corr * factor (fluid.hpp:583-599).
+__global__ void KOutletApply(Geo g, SideKinds sk, const double* __restrict__ corr,
+                             double* __restrict__ bv0, double* __restrict__ bv1) {
+  const int nb = 2 * (g.nx + g.ny);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  int s, q, face, cell;
+  BoundaryFace(g, b, s, q, face, cell);
+  if (sk.kind[s] != 2) return;
+  const bool xf = s < 2;
+  const double A = xf ? g.ax : g.ay;
+  const double nx_ = (xf ? g.sx : 0.) / A, ny_ = (xf ? -0. : g.sy) / A;
+  const double c = corr[0], factor = SideFactor(s);
+  bv0[b] = bv0[b] + (nx_ * c) * factor;
+  bv1[b] = bv1[b] + (ny_ * c) * factor;
+}
+
 // ---- heat: the temperature's implicit convection-diffusion iteration ---------
 // (HeatSolver::MakeIteration, heat.hpp:66-75, on ConvectionDiffusionScalar-
 // Implicit::MakeIteration, conv_diff.hpp:130-243) with the scaling 1, zero
@@ -903,18 +1055,6 @@ struct PhaseCells {
 // zero normal derivative on the boundary.  Boundary face b (left j, right
 // ny + j, bottom 2 ny + i, top 2 ny + nx + i) holds either a fixed value
 // (kind 0) or a fixed normal derivative (kind 1).
-
-// Index of the boundary face on side s at position q along it.
-__device__ __forceinline__ int BoundaryIndex(const Geo& g, int s, int q) {
-  return s == kLeft ? q : s == kRight ? g.ny + q : s == kBottom ? 2 * g.ny + q
-                                                                : 2 * g.ny + g.nx + q;
-}
-// Signed distance factor of the zero-index side's valid cell: the valid
-// neighbour of a left / bottom face is cell 1 (factor -1), of a right / top
-// face cell 0 (factor +1).
-__device__ __forceinline__ double SideFactor(int s) {
-  return (s == kLeft || s == kBottom) ? -1. : 1.;
-}
 
 // Cell conductivity (GetVolumeAveraged of conductivity_i, hydro2d.hpp:1236-1246)
 // and the gradient of the temperature's face values (Interpolate with the
@@ -1231,12 +1371,15 @@ void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double*
   KInitialFlux<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, mvx, mvy, flux);
 }
 void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
-                   const double* flux, double dt, double* const un[3]) {
+                   const double* flux, double dt, double* const un[3],
+                   const double* const bval[3], const int* inlet) {
   AdvFields fl{};
   for (int q = 0; q < nf; ++q) {
     fl.u[q] = u[q];
     fl.un[q] = un[q];
+    fl.bval[q] = bval ? bval[q] : nullptr;
   }
+  for (int s = 0; s < 4; ++s) fl.inlet[s] = inlet ? inlet[s] : 0;
   const unsigned grid = unsigned(((g.nx + kAdvTX - 1) / kAdvTX) * ((g.ny + kAdvTY - 1) / kAdvTY));
   if (nf == 1)
     KAdvStep<1><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
@@ -1289,6 +1432,24 @@ void LaunchSimpler(hipStream_t st, const Geo& g, Rows vel, const double* k0, con
                                          ev0, ev1);
   KSimplerRhs<<<grid, kBlock, 0, st>>>(g, ev0, ev1, ffex, ffey, ap, flux, uc0, uc1, rc, pres, p,
                                        pinned, pval, k);
+}
+void LaunchGatherInlet(hipStream_t st, const Geo& g, const int kind[4], const double* const pd[3],
+                       int np, double* pdb) {
+  SideKinds sk{{kind[0], kind[1], kind[2], kind[3]}};
+  PhaseCells pc{};
+  for (int p = 0; p < np; ++p) pc.alpha[p] = pd[p];
+  const int nb = 2 * (g.nx + g.ny);
+  KGatherInlet<<<(nb + kBlock - 1) / kBlock, kBlock, 0, st>>>(g, sk, pc, np, pdb);
+}
+void LaunchOutletUpdate(hipStream_t st, const Geo& g, const int kind[4], const double* u0,
+                        const double* u1, double* bv0, double* bv1, double* scratch) {
+  SideKinds sk{{kind[0], kind[1], kind[2], kind[3]}};
+  const int nb = 2 * (g.nx + g.ny);
+  const int grid = (nb + kBlock - 1) / kBlock;
+  double *term = scratch, *area = scratch + nb, *corr = scratch + 2 * nb;
+  KOutletTerms<<<grid, kBlock, 0, st>>>(g, sk, u0, u1, bv0, bv1, term, area);
+  KOutletSum<<<1, kBlock, 0, st>>>(g, sk, term, area, corr);
+  KOutletApply<<<grid, kBlock, 0, st>>>(g, sk, corr, bv0, bv1);
 }
 int BoundaryFluxBlocks(const Geo& g) { return (2 * (g.nx + g.ny) + kBlock - 1) / kBlock; }
 void LaunchBoundaryFlux(hipStream_t st, const Geo& g, const double* flux,

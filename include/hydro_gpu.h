@@ -56,7 +56,13 @@ typedef enum hg_precision {
 } hg_precision;
 
 typedef enum hg_bc_kind {
-  HG_BC_WALL = 0              /* "wall vx vy": no-slip wall moving with (vx,vy) */
+  HG_BC_WALL = 0,             /* "wall vx vy": no-slip wall moving with (vx,vy) */
+  HG_BC_INLET = 1,            /* "inlet vx vy": fixed inflow velocity; the phases enter
+                                 with the partial densities of the adjacent cells at
+                                 construction (hydro2d.hpp:565-583) */
+  HG_BC_OUTLET = 2            /* "outlet": the adjacent cell's velocity plus a uniform
+                                 normal correction that balances the inflow
+                                 (fluid.hpp:542-600) */
 } hg_bc_kind;
 
 enum { HG_SIDE_LEFT = 0, HG_SIDE_RIGHT = 1, HG_SIDE_BOTTOM = 2, HG_SIDE_TOP = 3 };
@@ -160,7 +166,7 @@ typedef enum hg_unsupported {
   HG_UNSUP_SETTLING = 1 << 6,       /* enable_settling_i 1 */
   HG_UNSUP_ADVECTION = 1 << 7,      /* advection_solver other than tvd */
   HG_UNSUP_RIGID_BOX = 1 << 8,      /* box_A/box_B holding cells */
-  HG_UNSUP_CONDITION = 1 << 9,      /* inlet/outlet boundary conditions */
+  HG_UNSUP_CONDITION = 1 << 9,      /* (no longer set: inlets and outlets run) */
   HG_UNSUP_IMAGE_INIT = 1 << 10,    /* img_init / imgu_init / imgv_init */
   HG_UNSUP_OTHER = 1 << 11          /* initial_pois, deforming_velocity, meshvel_auto */
 } hg_unsupported;

@@ -50,6 +50,9 @@ enum BcKind { BC_NONE = 0, BC_VALUE, BC_DERIV, BC_EXTRAP };
 struct SideBc {
   BcKind kind[4];
   double val[4];
+  // per-face values of BC_VALUE sides (indexed by face), when they vary
+  // along a side (outlets, inlet partial densities)
+  const double* fval = nullptr;
 };
 
 SideBc MakeBc(BcKind k, double v = 0.) {
@@ -194,7 +197,7 @@ void Interpolate(const Mesh& m, const double* u, double* res, const SideBc& bc) 
     size_t c = size_t(id == 0 ? m.fcm[f] : m.fcp[f]);
     double factor = (id == 0 ? 1. : -1.);
     if (kind == BC_VALUE) {
-      res[f] = bc.val[side];
+      res[f] = bc.fval ? bc.fval[f] : bc.val[side];
     } else if (kind == BC_DERIV) {
       double alpha = Norm(m.ToCell(f, id)) * factor;
       res[f] = u[c] + bc.val[side] * alpha;
@@ -607,6 +610,11 @@ struct or_ctx {
   Vec hval;
   std::vector<double> hbdf;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
+  // fluid conditions per boundary face: the velocity (walls and inlets
+  // fixed, outlets updated every iteration) and, at inlets, the phases'
+  // partial densities
+  Vec bvel[2];
+  Vec pdb[HG_MAX_PHASES];
 };
 
 namespace {
@@ -617,7 +625,81 @@ SideBc VelBc(const or_ctx* o, int comp) {
     b.kind[s] = BC_VALUE;
     b.val[s] = o->cfg.bc_velocity[s][comp];
   }
+  b.fval = o->bvel[comp].data();
   return b;
+}
+
+// Partial density conditions of phase i (hydro2d.hpp:565-583): the inlet
+// value on inlet faces, zero derivative elsewhere.
+SideBc PdBc(const or_ctx* o, int i) {
+  SideBc b = MakeBc(BC_DERIV, 0.);
+  for (int s = 0; s < 4; ++s)
+    if (o->cfg.bc_kind[s] == HG_BC_INLET) b.kind[s] = BC_VALUE;
+  b.fval = o->pdb[i].data();
+  return b;
+}
+
+// Fluid conditions per face (hydro2d.hpp:370-392, fluid.hpp:394-415): walls
+// and inlets hold their velocity, outlets start at zero.
+void InitBoundaryVelocity(or_ctx* o) {
+  const Mesh& m = o->m;
+  for (int d = 0; d < 2; ++d) o->bvel[d].assign(m.nf, 0.);
+  for (size_t f = 0; f < m.nf; ++f) {
+    if (m.Inner(f)) continue;
+    const int s = m.Side(f);
+    if (o->cfg.bc_kind[s] != HG_BC_OUTLET)
+      for (int d = 0; d < 2; ++d) o->bvel[d][f] = o->cfg.bc_velocity[s][d];
+  }
+}
+
+// The inlet partial densities: those of the adjacent cells now
+// (hydro2d.hpp:565-583).
+void InitBoundaryPd(or_ctx* o) {
+  const Mesh& m = o->m;
+  for (int i = 0; i < HG_MAX_PHASES; ++i) o->pdb[i].assign(m.nf, 0.);
+  for (size_t f = 0; f < m.nf; ++f) {
+    if (m.Inner(f) || o->cfg.bc_kind[m.Side(f)] != HG_BC_INLET) continue;
+    const size_t c = size_t(m.ValidId(f) == 0 ? m.fcm[f] : m.fcp[f]);
+    for (int i = 0; i < o->np; ++i) o->pdb[i][f] = o->pd_tc[i][c];
+  }
+}
+
+// FluidSimple::UpdateOutletBaseConditions (fluid.hpp:542-600), boundary
+// faces in face order: outlet faces take the adjacent cell's iter_curr
+// velocity; the inflow (inlet faces' u . S toward the domain, plus the
+// volume sources) minus the outflow over the outlet area is added along
+// each outlet face's normal.
+void UpdateOutlets(or_ctx* o) {
+  const Mesh& m = o->m;
+  bool any = false;
+  for (int s = 0; s < 4; ++s) any = any || o->cfg.bc_kind[s] == HG_BC_OUTLET;
+  if (!any) return;
+  double inflow = 0., outflow = 0., area = 0.;
+  for (size_t f = 0; f < m.nf; ++f) {
+    if (m.Inner(f)) continue;
+    const int kind = o->cfg.bc_kind[m.Side(f)];
+    const int id = m.ValidId(f);
+    if (kind == HG_BC_OUTLET) {
+      const size_t c = size_t(id == 0 ? m.fcm[f] : m.fcp[f]);
+      const double factor = (id == 0 ? 1. : -1.);
+      o->bvel[0][f] = o->u_ic[0][c];
+      o->bvel[1][f] = o->u_ic[1][c];
+      outflow += Dot(V2{o->bvel[0][f], o->bvel[1][f]}, m.surf[f]) * factor;
+      area += m.area[f];
+    } else if (kind == HG_BC_INLET) {
+      const double factor = (id == 0 ? -1. : 1.);
+      inflow += Dot(V2{o->bvel[0][f], o->bvel[1][f]}, m.surf[f]) * factor;
+    }
+  }
+  for (size_t c = 0; c < m.nc; ++c) inflow += o->vsrc[c] * m.vol[c];
+  const double corr = (inflow - outflow) / area;
+  for (size_t f = 0; f < m.nf; ++f) {
+    if (m.Inner(f) || o->cfg.bc_kind[m.Side(f)] != HG_BC_OUTLET) continue;
+    const double factor = (m.ValidId(f) == 0 ? 1. : -1.);
+    const V2 nrm{m.surf[f].x / m.area[f], m.surf[f].y / m.area[f]};
+    o->bvel[0][f] = o->bvel[0][f] + (nrm.x * corr) * factor;
+    o->bvel[1][f] = o->bvel[1][f] + (nrm.y * corr) * factor;
+  }
 }
 
 double DistV(V2 a, V2 b) { return Norm(V2{b.x - a.x, b.y - a.y}); }
@@ -787,6 +869,7 @@ int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
   }
   o->p_tc.assign(nc, 0.);
   o->p_tp = o->p_tc;
+  InitBoundaryVelocity(o);
   InitialFlux(o);
   // Initial partial densities (hydro2d.hpp:494-566)
   for (int i = 0; i < o->np; ++i)
@@ -820,6 +903,7 @@ int or_create(const hg_config* cfg_in, or_ctx** out, char* err, size_t errlen) {
     o->pd_tp[i] = o->pd_tc[i];
     o->alpha[i].assign(nc, 0.);
   }
+  InitBoundaryPd(o);
   UpdateProperties(o);
   RunningStats(o);
   *out = o;
@@ -872,6 +956,8 @@ int or_get_field(or_ctx* o, int field, double* host, size_t n) {
 }
 
 int or_refresh(or_ctx* o) {
+  InitBoundaryVelocity(o);
+  InitBoundaryPd(o);
   InitialFlux(o);
   UpdateProperties(o);
   std::memset(o->bacc, 0, sizeof(o->bacc));
@@ -956,6 +1042,7 @@ int or_fluid_iterate(or_ctx* o, hg_iter_stats* stats) {
   const long lnc = long(nc), lnf = long(nf);
   o->p_ip = o->p_ic;
   o->fl_ip = o->fl_ic;
+  UpdateOutlets(o);
 
   Vec ffex(nf), ffey(nf), frx(nc), fry(nc), ffrx(nf), ffry(nf), ffmu(nf);
   {
@@ -1076,7 +1163,7 @@ int or_fluid_iterate(or_ctx* o, hg_iter_stats* stats) {
           dck[f] = (0. * (-mu)) * A_;
         } else {
           int id = m.ValidId(f);
-          double val = vbc.val[m.Side(f)];
+          double val = vbc.fval ? vbc.fval[f] : vbc.val[m.Side(f)];
           cam[f] = cap[f] = 0.;
           cak[f] = val * F;
           double factor = (id == 0 ? 1. : -1.);
@@ -1385,8 +1472,8 @@ int or_advect_substep(or_ctx* o) {
     o->pd_ic[i] = o->pd_tp[i];
   }
   const Vec& flux = o->fl_ic;  // mixture flux (iter_curr layer), slip = 0
-  SideBc d0 = MakeBc(BC_DERIV, 0.);
   for (int i = 0; i < o->np; ++i) {
+    const SideBc d0 = PdBc(o, i);
     o->pd_ip[i] = o->pd_ic[i];
     Vec& curr = o->pd_ic[i];
     Vec vf(nf);
@@ -1411,6 +1498,8 @@ int or_advect_substep(or_ctx* o) {
           } else {
             fu[f] = 0.5 * (u[P] + u[E]);
           }
+        } else if (d0.kind[m.Side(f)] == BC_VALUE) {
+          fu[f] = o->pdb[i][f];  // inlet
         } else {
           int id = m.ValidId(f);
           long cc = id == 0 ? m.fcm[f] : m.fcp[f];
@@ -1435,10 +1524,10 @@ int or_advect_substep(or_ctx* o) {
     Vec ff(nf, 0.);
     if (std::abs(cfg.sharp) > 1e-10) {
       Vec af(nf), gx(nc), gy(nc), gfx(nf), gfy(nf);
-      Interpolate(m, curr.data(), af.data(), d0);
+      Interpolate(m, curr.data(), af.data(), d0);  // the phase's conditions
       Gradient(m, af.data(), gx.data(), gy.data());
-      Interpolate(m, gx.data(), gfx.data(), d0);
-      Interpolate(m, gy.data(), gfy.data(), d0);
+      Interpolate(m, gx.data(), gfx.data(), MakeBc(BC_DERIV, 0.));  // zero derivative
+      Interpolate(m, gy.data(), gfy.data(), MakeBc(BC_DERIV, 0.));
       double am = cfg.density[i];
       for (size_t f = 0; f < nf; ++f) {
         V2 n{gfx[f], gfy[f]};

@@ -75,6 +75,30 @@ set double heat_relaxation_factor 0.9
 """
 
 
+def channel(nx=32, ny=16, dt=2e-3, extra=""):
+    """Channel [0,2]x[0,1]: inflow 1 through the left side, outflow on the
+    right, walls at the bottom and the top; two phases, a heavy block in the
+    lower left so that both cross the inlet's and the outlet's faces."""
+    return f"""
+set vect A (0, 0)
+set vect B (2, 1)
+set int Nx {nx}
+set int Ny {ny}
+set int num_phases 2
+set double density_0 1
+set double density_1 5
+set double viscosity_0 0.01
+set double viscosity_1 0.02
+set vect gravity (0, -1)
+set vect A1 (0, 0)
+set vect B1 (0.4, 0.3)
+set double dt {dt}
+set string condition_left "inlet 1 0"
+set string condition_right "outlet"
+{extra}
+"""
+
+
 def rt(nx=1024, ny=2048, dt=2e-4, solver_p="cg", solver_v="lu_rb", precision="fp64", extra=""):
     """Rayleigh-Taylor, [0,1]x[0,2], Atwood 0.5 (heavy 3 on top of light 1),
     g = (0,-1).  The interface is set through the partial-density fields."""

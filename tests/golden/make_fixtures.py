@@ -55,6 +55,10 @@ FIXTURES = {
     "ref_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_LEFT), 4, (4,)),
     "ref_cavity32_simpler": (cases.cavity(32, extra="set bool simpler 1"), 5, (1, 5)),
     "ref_dam32x16_simpler": (cases.dam(32, 16, extra="set bool simpler 1"), 4, (4,)),
+    "ref_channel32x16": (cases.channel(32, 16), 6, (1, 6)),
+    "ref_channel32x16_jacobi": (cases.channel(32, 16, extra=(
+        "set string linear_solver_pressure jacobi\n"
+        "set double lu_relaxed_relaxation_factor 0.9\n")), 4, (4,)),
     "ref_cavity32_simpler_pinned": (cases.cavity(32, solver_p="jacobi", extra=(
         "set bool simpler 1\nset double lu_relaxed_relaxation_factor 0.9\n"
         "set vect pressure_fixed_point (0.3, 0.6)\nset double pressure_fixed_value 0.1\n")),
@@ -188,6 +192,13 @@ set int max_frame_index 2
 set int max_frame_scalar_index 3
 set bool output_temperature 1
 """
+CHANNEL_OUTPUT = """
+set string name chan
+set double T 0.02
+set int max_frame_index 2
+set int max_frame_scalar_index 5
+set bool output_divergence 1
+"""
 OUTPUT = {
     # name: (case text, output keys, step limit; the run ends at T)
     "ref_output_dam32x16": (cases.dam(32, 16), DAM_OUTPUT, 100),
@@ -195,6 +206,7 @@ OUTPUT = {
         cases.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
     "ref_output_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), DTAUTO_OUTPUT, 100),
     "ref_output_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_BOTTOM), HEAT_OUTPUT, 100),
+    "ref_output_channel32x16": (cases.channel(32, 16), CHANNEL_OUTPUT, 100),
 }
 
 

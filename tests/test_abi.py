@@ -62,7 +62,7 @@ def test_parse_reference_sample_parameter_dump():
 
 
 def capi_unsup(name):
-    return {"condition": 1 << 9}[name]
+    return {"radiation": 1 << 3}[name]
 
 
 def capi_side(name):
@@ -97,12 +97,17 @@ def test_parse_errors_like_the_reference():
 
 
 def test_unsupported_features_are_flagged():
-    cfg = capi.parse_config("set bool heat_enable 1\nset string condition_right outlet\n")
+    cfg = capi.parse_config("set bool compressible_enable 1\nset bool radiation_enable 1\n")
     assert cfg.unsupported_flags != 0
-    cfg = capi.parse_config("set bool heat_enable 0\n", cfg)
-    assert cfg.unsupported_flags == capi_unsup("condition")
-    cfg = capi.parse_config("set bool dt_auto 1\nset double cfl 0.25\n")
+    cfg = capi.parse_config("set bool compressible_enable 0\n", cfg)
+    assert cfg.unsupported_flags == capi_unsup("radiation")
+    # variants that run: dt_auto, heat, inlets / outlets, SIMPLER
+    cfg = capi.parse_config("set bool dt_auto 1\nset double cfl 0.25\nset bool heat_enable 1\n"
+                            "set string condition_left \"inlet 1 0\"\n"
+                            "set string condition_right outlet\nset bool simpler 1\n")
     assert cfg.unsupported_flags == 0 and cfg.dt_auto == 1 and cfg.cfl == 0.25
+    assert cfg.heat_enable == 1 and cfg.simpler == 1
+    assert tuple(cfg.bc_kind) == (1, 2, 0, 0) and tuple(cfg.bc_velocity[0]) == (1.0, 0.0)
     assert capi.default_config().cfl_advection == 0.5
 
 

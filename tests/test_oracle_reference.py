@@ -21,7 +21,8 @@ from oracle import pyoracle
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = ["ref_cavity16", "ref_cavity64", "ref_dam64x32", "ref_cavity32_jacobi",
          "ref_dam32x16_dtauto", "ref_cavity32_heat", "ref_dam32x16_heat",
-         "ref_cavity32_simpler", "ref_dam32x16_simpler", "ref_cavity32_simpler_pinned"]
+         "ref_cavity32_simpler", "ref_dam32x16_simpler", "ref_cavity32_simpler_pinned",
+         "ref_channel32x16", "ref_channel32x16_jacobi"]
 
 FIELDS = [("u", capi.HG_FIELD_VELOCITY_X), ("v", capi.HG_FIELD_VELOCITY_Y),
           ("p", capi.HG_FIELD_PRESSURE), ("flux", capi.HG_FIELD_VOLUME_FLUX)]

@@ -20,7 +20,8 @@ from oracle import pyoracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 OUTPUT_FIXTURES = ["ref_output_dam32x16", "ref_output_cavity16_meshvel",
-                   "ref_output_dam32x16_dtauto", "ref_output_dam32x16_heat"]
+                   "ref_output_dam32x16_dtauto", "ref_output_dam32x16_heat",
+                   "ref_output_channel32x16"]
 
 
 def load_output_fixture(name):
