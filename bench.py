@@ -51,7 +51,7 @@ def log(msg):
 # WRITE_SIZE on the RT 1024x2048 state, each scaled by what the counter reads
 # for a 256 MiB device-to-device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
 MEASURED_TRAFFIC = {"KUp<double, true>": 82.79e6, "KDown<double>": 58.70e6,
-                    "KAdvStep<2>": 120.91e6, "KMomentum": 579.68e6}
+                    "KAdvStep<2, false>": 120.91e6, "KMomentum": 579.68e6}
 
 
 def run_path(text, steps, warmup, label, train=False):
@@ -225,7 +225,7 @@ def main():
         "frac": round(cg_gbs / HBM_PEAK_GBS, 4)}
     log(f"kernels: {kernels}")
 
-    for name, man in (("KAdvStep", "KAdvStep<2>"), ("KMomentum", "KMomentum")):
+    for name, man in (("KAdvStep", "KAdvStep<2, false>"), ("KMomentum", "KMomentum")):
         ms = step_kernel_ms[name]
         gbs = manifest.kernel_bytes(man, nx, ny) / (ms * 1e-3) / 1e9
         kernels[man] = {"avg_us": round(ms * 1e3, 2), "gbs": round(gbs, 1),
@@ -233,7 +233,7 @@ def main():
     # the dominant kernel of the headline step: launches per step x duration
     simple_per_step = float(np.mean([s_ for s_, _ in itmx]))
     fcg_per_step = float(np.mean([p for _, p in itmx]))
-    per_step = {"KAdvStep<2>": substeps, "KMomentum": simple_per_step,
+    per_step = {"KAdvStep<2, false>": substeps, "KMomentum": simple_per_step,
                 "KUp<double, true>": fcg_per_step, "KDown<double>": fcg_per_step}
     dom = max(per_step, key=lambda k: per_step[k] * kernels[k]["avg_us"])
     achieved = kernels[dom]["gbs"]

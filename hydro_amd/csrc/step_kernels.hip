@@ -105,6 +105,9 @@ KFacesDeriv0Pair(Geo g, const double* __restrict__ ax, const double* __restrict_
 
 // Pressure gradient (extrapolation condition) and velocity gradients (value
 // condition) at cells (fluid.hpp:827-832, 840-842; conv_diff.hpp:135).
+// FV: the condition has per-face velocities (inlets or outlets); without
+// them the side's constant velocity needs no per-cell load.
+template <bool FV>
 __global__ void __launch_bounds__(kBlock)
 KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict__ u0,
            const double* __restrict__ u1, double* __restrict__ gpx, double* __restrict__ gpy,
@@ -123,8 +126,8 @@ KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict_
   double bv0[4], bv1[4];
   const int qs[4] = {j, j, i, i};
   for (int s = 0; s < 4; ++s) {
-    bv0[s] = BVel(bc, g, 0, s, qs[s]);
-    bv1[s] = BVel(bc, g, 1, s, qs[s]);
+    bv0[s] = FV ? BVel(bc, g, 0, s, qs[s]) : bc.vel[0][s];
+    bv1[s] = FV ? BVel(bc, g, 1, s, qs[s]) : bc.vel[1][s];
   }
   CellFacesV<0>(g, x0, i, j, bv0, f);
   GradFromFaces(g, f, g0x[c], g0y[c]);
@@ -584,7 +587,7 @@ struct AdvFields {
 // operations as the unfused form, so the result is bit for bit the same.
 constexpr int kAdvTX = 64, kAdvTY = 8;
 
-template <int NF>
+template <int NF, bool IN>
 __global__ void __launch_bounds__(kBlock)
 KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
   constexpr int UX = kAdvTX + 4, UY = kAdvTY + 4;
@@ -632,7 +635,7 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
     // a boundary face's value: the inlet's, or the cell's (zero derivative)
     const double* bv = fl.bval[q];
     auto bface = [&](int s, int pos, double uc) {
-      return (bv && fl.inlet[s]) ? bv[BoundaryIndex(g, s, pos)] : uc;
+      return (IN && bv && fl.inlet[s]) ? bv[BoundaryIndex(g, s, pos)] : uc;
     };
     // x-gradients on columns i0-1 .. i0+64 of the tile rows
     for (int t = threadIdx.x; t < kAdvTY * (kAdvTX + 2); t += kBlock) {
@@ -1322,8 +1325,12 @@ void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const double* ax, const
 void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
                      const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
                      double* g0y, double* g1x, double* g1y) {
-  KCellGrads<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x, g0y,
-                                                          g1x, g1y);
+  if (bc.fvel[0] || bc.fvel[1])
+    KCellGrads<true><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x,
+                                                                g0y, g1x, g1y);
+  else
+    KCellGrads<false><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x,
+                                                                 g0y, g1x, g1y);
 }
 void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a) {
   MomIn in{a.u0,   a.u1,   a.g0x,  a.g0y,  a.g1x,  a.g1y,  a.gpx,  a.gpy,  a.frx, a.fry,
@@ -1380,13 +1387,27 @@ void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3
     fl.bval[q] = bval ? bval[q] : nullptr;
   }
   for (int s = 0; s < 4; ++s) fl.inlet[s] = inlet ? inlet[s] : 0;
+  // IN: some field has inlet values on some inlet side (else every
+  // boundary face takes the cell's value)
+  bool any_in = false;
+  for (int q = 0; q < nf; ++q)
+    for (int s = 0; s < 4; ++s) any_in = any_in || (fl.bval[q] && fl.inlet[s]);
   const unsigned grid = unsigned(((g.nx + kAdvTX - 1) / kAdvTX) * ((g.ny + kAdvTY - 1) / kAdvTY));
-  if (nf == 1)
-    KAdvStep<1><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
-  else if (nf == 2)
-    KAdvStep<2><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
-  else
-    KAdvStep<3><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+  if (any_in) {
+    if (nf == 1)
+      KAdvStep<1, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+    else if (nf == 2)
+      KAdvStep<2, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+    else
+      KAdvStep<3, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+  } else {
+    if (nf == 1)
+      KAdvStep<1, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+    else if (nf == 2)
+      KAdvStep<2, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+    else
+      KAdvStep<3, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+  }
 }
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy) {

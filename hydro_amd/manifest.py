@@ -33,7 +33,7 @@ KERNELS = {
                        "read face f_x (x-faces), f_y (y-faces); write cell restored force x, y"),
     "KFacesDeriv0Pair": (lambda nx, ny: _b(nx, ny, cells64=2, faces64=2),
                          "read 2 cell fields; write 2 face fields"),
-    "KCellGrads": (lambda nx, ny: _b(nx, ny, cells64=9),
+    "KCellGrads<false>": (lambda nx, ny: _b(nx, ny, cells64=9),
                    "read p, u0, u1; write 3 gradients x 2 components"),
     "KMomentum": (lambda nx, ny: _b(nx, ny, cells64=23, faces64=2),
                   "read u0, u1, 4 velocity gradients, grad p (2), restored force (2), rho, "
@@ -61,7 +61,7 @@ KERNELS = {
     "KStart": (lambda nx, ny: _b(nx, ny, cells64=4), "read b; write projected b, r, x = 0"),
     "KResidual": (lambda nx, ny: _b(nx, ny, cells64=3, faces64=1), "read b, x, k_x, k_y; write r"),
     # ---- advection and properties ----
-    "KAdvStep<2>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
+    "KAdvStep<2, false>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                     "two phase fields: read both fields and the volume flux; write both fields"),
     "KAdvGrad": (lambda nx, ny: _b(nx, ny, cells64=3), "read field; write gradient (2)"),
     "KAdvUpdate": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
@@ -135,7 +135,7 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
         + kernel_bytes("KForceFaces", nx, ny) + kernel_bytes("KRestoredForce", nx, ny) \
         + kernel_bytes("KFacesDeriv0Pair", nx, ny)                 # NaN check, guesses, faces
     per_iter = sum(kernel_bytes(k, nx, ny) for k in (
-        "KCellGrads", "KMomentum", "KLuRbBlack", "KLuRbRed", "KRhieChow", "KPressureRhs",
+        "KCellGrads<false>", "KMomentum", "KLuRbBlack", "KLuRbRed", "KRhieChow", "KPressureRhs",
         "KCorrectCells", "KCorrectFaces"))
     # operator set-up (the multigrid's finest level shares the FP32 arrays;
     # coarse levels are negligible), FCG start and the closing true residual
@@ -145,6 +145,6 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
     per_fcg = kernel_bytes("KFcgSpmv", nx, ny) + kernel_bytes("KFcgUpdate", nx, ny) \
         + _vcycle_bytes(nx, ny, fcg_dots=True)
     finish = w * 3 * n                                             # NaN check
-    advection = substeps * kernel_bytes("KAdvStep<2>", nx, ny)
+    advection = substeps * kernel_bytes("KAdvStep<2, false>", nx, ny)
     properties = kernel_bytes("KProperties", nx, ny) + 4 * kernel_bytes("KSmooth", nx, ny)
     return start + simple_iters * per_iter + fcg_iters * per_fcg + finish + advection + properties
