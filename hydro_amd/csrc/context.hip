@@ -74,6 +74,22 @@ struct ExtrapLayers {
 
 __global__ void KExtrapolate(long nc, long nf, ExtrapLayers l, double ge) {
   const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (ge == 0.) {
+    // a + (a - b) * 0 is a itself, except for a = -0 (the sign of the sum
+    // follows b) and a non-finite: only there is time_prev read
+    auto Keep = [](double a, const double* tp, long k) {
+      return (a == 0. || !isfinite(a)) ? a + (a - tp[k]) * 0. : a;
+    };
+    if (i < nc) {
+      double a[3];
+      for (int k = 0; k < 3; ++k) a[k] = l.tc[k][i];
+      for (int k = 0; k < 3; ++k) l.out[k][i] = Keep(a[k], l.tp[k], i);
+    } else if (i < nc + nf) {
+      const long f = i - nc;
+      l.out[3][f] = Keep(l.tc[3][f], l.tp[3], f);
+    }
+    return;
+  }
   if (i < nc) {
     double a[3], b[3];
     for (int k = 0; k < 3; ++k) {
@@ -279,13 +295,19 @@ void Smooth(hg_ctx* c, const double* in, double* out, int times) {
       HG_CHECK(hipMemcpyAsync(out, in, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
     return;
   }
+  // passes two at a time in one launch, an odd last pass alone; the last
+  // launch writes out directly unless out is its input
   const double* src = in;
-  for (int r = 0; r < times; ++r) {
-    double* dst = (r & 1) ? c->tmp2 : c->tmp1;
-    if (r == times - 1 && out != src) dst = out;  // the last pass writes out directly
-    if (dst == src) dst = (src == c->tmp1) ? c->tmp2 : c->tmp1;
-    hg::LaunchSmooth(c->st, c->geo, src, dst);
+  for (int r = 0; r < times;) {
+    const int k = times - r >= 2 ? 2 : 1;
+    double* dst = src == c->tmp1 ? c->tmp2 : c->tmp1;
+    if (r + k == times && out != src) dst = out;
+    if (k == 2)
+      hg::LaunchSmooth2(c->st, c->geo, src, dst);
+    else
+      hg::LaunchSmooth(c->st, c->geo, src, dst);
     src = dst;
+    r += k;
   }
   if (src != out)
     HG_CHECK(hipMemcpyAsync(out, src, c->nc * 8, hipMemcpyDeviceToDevice, c->st));

@@ -94,6 +94,8 @@ void LaunchOutletUpdate(hipStream_t st, const Geo& g, const int kind[4], const d
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy);
 void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out);
+// Two smoothing passes in one launch (out must not alias u).
+void LaunchSmooth2(hipStream_t st, const Geo& g, const double* u, double* out);
 void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const double* pd1,
                       const double* pd2, double* pd0);
 void LaunchNanCheck(hipStream_t st, const double* a, const double* b, const double* c, int n,

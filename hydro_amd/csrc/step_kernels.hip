@@ -739,6 +739,69 @@ KSmooth(Geo g, const double* __restrict__ u, double* __restrict__ out) {
   out[c] = s / 4.;
 }
 
+// Two passes of GetSmoothField in one launch.  The block loads its 64 x 8
+// tile with a two-cell halo into LDS, forms the first pass on the tile plus
+// a one-cell halo, then the second pass on the tile.  Each value is formed
+// with the operations of KSmooth, so the result is bit for bit that of two
+// KSmooth launches.  `out` must not alias `u`.
+constexpr int kSmTX = 64, kSmTY = 8;
+
+__device__ __forceinline__ double SmoothAt(int nx, int ny, int i, int j, double uc, double uw,
+                                           double ue, double us, double un) {
+  const double fw = i > 0 ? Mid(uw, uc) : uc;
+  const double fe = i < nx - 1 ? Mid(uc, ue) : uc;
+  const double fs = j > 0 ? Mid(us, uc) : uc;
+  const double fn = j < ny - 1 ? Mid(uc, un) : uc;
+  double s = 0.;
+  s += fw;
+  s += fe;
+  s += fs;
+  s += fn;
+  return s / 4.;
+}
+
+__global__ void __launch_bounds__(kBlock)
+KSmooth2(Geo g, const double* __restrict__ u, double* __restrict__ out) {
+  constexpr int UX = kSmTX + 4, UY = kSmTY + 4, PX = kSmTX + 2, PY = kSmTY + 2;
+  __shared__ double su[UY][UX];
+  __shared__ double sp[PY][PX];
+  const int nx = g.nx, ny = g.ny;
+  const int ntx = (nx + kSmTX - 1) / kSmTX;
+  const int i0 = (blockIdx.x % ntx) * kSmTX, j0 = (blockIdx.x / ntx) * kSmTY;
+  constexpr int NU = UX * UY, KU = (NU + kBlock - 1) / kBlock;
+  double ureg[KU];
+#pragma unroll
+  for (int k = 0; k < KU; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    const int ii = i0 + t % UX - 2, jj = j0 + t / UX - 2;
+    ureg[k] = (t < NU && ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? u[ii + nx * jj] : 0.;
+  }
+#pragma unroll
+  for (int k = 0; k < KU; ++k) {
+    const int t = threadIdx.x + k * kBlock;
+    if (t < NU) su[t / UX][t % UX] = ureg[k];
+  }
+  __syncthreads();
+  // first pass on cells i0-1 .. i0+64, j0-1 .. j0+8 inside the grid
+  for (int t = threadIdx.x; t < PX * PY; t += kBlock) {
+    const int pj = t / PX, pi = t % PX;
+    const int ii = i0 + pi - 1, jj = j0 + pj - 1;
+    double v = 0.;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny)
+      v = SmoothAt(nx, ny, ii, jj, su[pj + 1][pi + 1], su[pj + 1][pi], su[pj + 1][pi + 2],
+                   su[pj][pi + 1], su[pj + 2][pi + 1]);
+    sp[pj][pi] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSmTX * kSmTY; t += kBlock) {
+    const int lj = t / kSmTX, li = t % kSmTX;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= nx || j >= ny) continue;
+    out[i + nx * j] = SmoothAt(nx, ny, i, j, sp[lj + 1][li + 1], sp[lj + 1][li],
+                               sp[lj + 1][li + 2], sp[lj][li + 1], sp[lj + 2][li + 1]);
+  }
+}
+
 // Initial partial density of phase 0 compensating the others (hydro2d.hpp:556-565).
 __global__ void __launch_bounds__(kBlock)
 KPhase0Init(Geo g, Phases ph, const double* __restrict__ pd1, const double* __restrict__ pd2,
@@ -1416,6 +1479,10 @@ void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const doub
 }
 void LaunchSmooth(hipStream_t st, const Geo& g, const double* u, double* out) {
   KSmooth<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, u, out);
+}
+void LaunchSmooth2(hipStream_t st, const Geo& g, const double* u, double* out) {
+  const unsigned grid = unsigned(((g.nx + kSmTX - 1) / kSmTX) * ((g.ny + kSmTY - 1) / kSmTY));
+  KSmooth2<<<grid, kBlock, 0, st>>>(g, u, out);
 }
 void LaunchPhase0Init(hipStream_t st, const Geo& g, const Phases& ph, const double* pd1,
                       const double* pd2, double* pd0) {

@@ -69,6 +69,11 @@ KERNELS = {
     "KProperties": (lambda nx, ny: _b(nx, ny, cells64=8),
                     "2 phases: read 2 partial densities; write 2 fractions, rho, mu, force (2)"),
     "KSmooth": (lambda nx, ny: _b(nx, ny, cells64=2), "read field; write smoothed field"),
+    "KSmooth2": (lambda nx, ny: _b(nx, ny, cells64=2),
+                 "two smoothing passes: read field; write twice-smoothed field"),
+    "KExtrapolate": (lambda nx, ny: _b(nx, ny, cells64=6, faces64=2),
+                     "guess_extrapolation 0: read time_curr u_x, u_y, p, flux; write "
+                     "iter_curr (time_prev only where time_curr is zero or non-finite)"),
     # ---- FP64 Jacobi-PCG iteration (cg.hip) ----
     "KPcgSpmv<double>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                          "read z, p_old, k_x, k_y; write p, q"),
@@ -131,7 +136,7 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
     the step's kernels, launches per step times bytes per launch."""
     n, f, _, _ = sizes(nx, ny)
     w = F64
-    start = w * (3 * n + 9 * n + 3 * f) \
+    start = w * 3 * n + kernel_bytes("KExtrapolate", nx, ny) \
         + kernel_bytes("KForceFaces", nx, ny) + kernel_bytes("KRestoredForce", nx, ny) \
         + kernel_bytes("KFacesDeriv0Pair", nx, ny)                 # NaN check, guesses, faces
     per_iter = sum(kernel_bytes(k, nx, ny) for k in (
@@ -146,5 +151,6 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
         + _vcycle_bytes(nx, ny, fcg_dots=True)
     finish = w * 3 * n                                             # NaN check
     advection = substeps * kernel_bytes("KAdvStep<2, false>", nx, ny)
-    properties = kernel_bytes("KProperties", nx, ny) + 4 * kernel_bytes("KSmooth", nx, ny)
+    # density and viscosity, two smoothing passes each in one launch
+    properties = kernel_bytes("KProperties", nx, ny) + 2 * kernel_bytes("KSmooth2", nx, ny)
     return start + simple_iters * per_iter + fcg_iters * per_fcg + finish + advection + properties
