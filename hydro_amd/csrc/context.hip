@@ -219,7 +219,7 @@ struct hg_ctx {
   std::unique_ptr<hg::MgPrecond> mg;
   bool pressure_system = false;  // cg/mg hold the operator of a pressure solve
   hipEvent_t ev[4];
-  double timing[3] = {0, 0, 0};
+  bool timed = false;  // ev[0..3] hold the regions of the last hg_step
 
   double* Alloc(long n) {
     void* p = nullptr;
@@ -518,11 +518,15 @@ void InitialFlux(hg_ctx* c) {
 }
 
 bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
-  HG_CHECK(hipMemsetAsync(c->nan_d, 0, sizeof(int), c->st));
+  // the flag is zero between checks (zeroed at allocation, and again after
+  // a check that set it), so no clearing pass precedes the kernel
   hg::LaunchNanCheck(c->st, a, b, d, int(c->nc), c->nan_d);
   HG_CHECK(hipMemcpyAsync(c->flag_h, c->nan_d, sizeof(int), hipMemcpyDeviceToHost, c->st));
   c->Sync();
-  return *c->flag_h != 0;
+  if (*c->flag_h == 0) return false;
+  HG_CHECK(hipMemsetAsync(c->nan_d, 0, sizeof(int), c->st));
+  c->Sync();
+  return true;
 }
 
 void StartStep(hg_ctx* c) {
@@ -1219,14 +1223,12 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     UpdateProperties(c);
     AccumulateStats(c);
     HG_CHECK(hipEventRecord(c->ev[3], c->st));
-    c->Sync();
-    float a = 0, b = 0, d = 0;
-    HG_CHECK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HG_CHECK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-    HG_CHECK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
-    c->timing[0] = a;
-    c->timing[1] = b;
-    c->timing[2] = d;
+    // no synchronisation at the end of the step: FinishStep's NaN check
+    // waited for the SIMPLE loop (settling its indicator), and the work
+    // queued since (advection, properties, running stats) overlaps with the
+    // host's return and the next call; every readback waits for it, and the
+    // region timings are read from the events when asked for
+    c->timed = true;
     if (iterated) st.convergence_indicator = c->last_iter.convergence_indicator;
     st.time = c->t_fluid;
     st.dt = c->dt;
@@ -1236,9 +1238,15 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
 
 hg_status hg_last_step_timing(const hg_ctx* c, double* f, double* a, double* p) {
   if (!c) return HG_ERR_BAD_ARG;
-  if (f) *f = c->timing[0];
-  if (a) *a = c->timing[1];
-  if (p) *p = c->timing[2];
+  float t[3] = {0.f, 0.f, 0.f};
+  if (c->timed) {
+    if (hipEventSynchronize(c->ev[3]) != hipSuccess) return HG_ERR_HIP;
+    for (int k = 0; k < 3; ++k)
+      if (hipEventElapsedTime(&t[k], c->ev[k], c->ev[k + 1]) != hipSuccess) return HG_ERR_HIP;
+  }
+  if (f) *f = t[0];
+  if (a) *a = t[1];
+  if (p) *p = t[2];
   return HG_OK;
 }
 
@@ -1287,6 +1295,9 @@ hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t th
     for (int32_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
       hg_step_stats last{};
       for (int32_t s = 0; s < steps && rc[i] == HG_OK; ++s) rc[i] = hg_step(ctxs[i], &last);
+      // hg_step returns with its last kernels queued: the case is done when
+      // its stream is
+      if (rc[i] == HG_OK) rc[i] = Guard(ctxs[i], [&]() { ctxs[i]->Sync(); });
       if (stats) stats[i] = last;
     }
   };

@@ -301,10 +301,14 @@ hg_status hg_update_properties(hg_ctx* ctx);
 hg_status hg_calc_stats(hg_ctx* ctx, hg_stats* stats);
 /* hydro::step (hydro2d.hpp:1531-1621): with dt_auto the step's dt first
  * (FluidSimple::GetAutoTimeStep, fluid.hpp:1191-1206), then the SIMPLE loop,
- * advection sub-steps, property update, stats. */
+ * advection sub-steps, property update, stats.  Returns once the SIMPLE loop
+ * has finished on the GPU (its NaN check and *stats are settled); the
+ * advection, property and stats kernels may still be running on the
+ * context's stream.  Every call that reads device state waits for them. */
 hg_status hg_step(hg_ctx* ctx, hg_step_stats* stats);
 /* Wall-clock of the last hg_step split by the reference's timer regions
- * (step.fluid, step.advection, step.fluid_properties), in ms, HIP events. */
+ * (step.fluid, step.advection, step.fluid_properties), in ms, HIP events;
+ * waits for the step's last kernels.  Zeros before the first hg_step. */
 hg_status hg_last_step_timing(const hg_ctx* ctx, double* fluid_ms, double* advection_ms,
                               double* properties_ms);
 
