@@ -512,6 +512,23 @@ KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
   PutPartials<1>(v, pt.upd[1] + pt.nb);
 }
 
+// alpha = (r, z) / (p, q) of iteration k into sc->alpha, for an update
+// that a later kernel applies; (p, q) <= 0 sets fail and done instead.
+__global__ void __launch_bounds__(kB)
+KFcgAlpha(int k, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  double f[2];
+  FoldN<2>({pt.dir + pt.ndir, pt.spmv[k & 1]}, {pt.ndir, pt.nb}, f);
+  if (threadIdx.x == 0) {
+    if (f[1] > 0.) {
+      sc->alpha = f[0] / f[1];
+    } else {
+      sc->fail = 1;
+      sc->done = 1;
+    }
+  }
+}
+
 // Folds `nb` partials into sc->rr (and sc->bb when set_bb).
 __global__ void __launch_bounds__(kB)
 KTakeRR(const double* part, int nb, CgScalars* sc, int set_bb) {
@@ -804,30 +821,57 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
   int batch = std::max(2, std::min(64, last_fcg_iters_ + 1));
   // a preconditioner that takes over the stop test and the direction dots
   // (FcgFusion) writes the dots to dirx_, sized for its grid
+  // (and, when it takes the previous iteration's update, the update's
+  // (r, r) partials at dirx_ + 2 nfp)
   const int nfp = pc->FcgPartials();
   if (nfp > dirx_cap_) {
     if (dirx_) HG_CHECK(hipFree(dirx_));
     dirx_ = nullptr;
-    HG_CHECK(hipMalloc(&dirx_, size_t(2) * nfp * 8));
+    HG_CHECK(hipMalloc(&dirx_, size_t(3) * nfp * 8));
     dirx_cap_ = nfp;
   }
+  // from the second iteration on, the preconditioner's first kernel runs the
+  // previous iteration's update with the alpha that KFcgAlpha folded after
+  // that iteration's SpMV, and no KFcgUpdate is queued
+  const bool fuse = nfp > 0 && pc->FusesUpdate();
+  // the folded update reads the residual on its tiles' halos while other
+  // workgroups update theirs, so it writes the new residual to the other
+  // buffer of a pair (z_, unused by FCG, is the second), which the up-leg
+  // then reads
+  double* rcur = r_;
+  double* ralt = z_;
   CgResult res;
   while (true) {
     for (int it = 0; it < batch; ++it, ++k) {
       Parts pk = pt;
       FcgFusion f{pt.upd[(k + 1) & 1] + nb, nb, k, &sc->rr, &sc->iter, &sc->done,
                   &sc->tol2, &sc->bb, &sc->max_iter, &sc->fail, k > 0 ? q_ : nullptr, dirx_};
+      if (fuse && k > 0) {
+        f.rr_part = dirx_ + 2 * nfp;
+        f.nb = nfp;
+        f.up_on = 1;
+        f.up_alpha = &sc->alpha;
+        f.up_p = pold;
+        f.up_q = q_;
+        f.up_x = x;
+        f.up_r = ralt;
+        f.up_rr = dirx_ + 2 * nfp;
+      }
       int ndir = 0;
-      if (nfp > 0 && pc->ApplyFcg(r_, z32_, stream_, &sc->done, f, &ndir)) {
+      if (nfp > 0 && pc->ApplyFcg(rcur, z32_, stream_, &sc->done, f, &ndir)) {
         pk.dir = dirx_;
         pk.ndir = ndir;
+        if (f.up_on) std::swap(rcur, ralt);
       } else {
         KFcgCheck<<<1, kB, 0, stream_>>>(k, pt, sc);
         pc->Apply(r_, z32_, stream_, &sc->done);
         KFcgDir<<<nb, kB, 0, stream_>>>(w, k, z32_, q_, r_, pt, sc);
       }
       KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pk, sc);
-      KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
+      if (fuse)
+        KFcgAlpha<<<1, kB, 0, stream_>>>(k, pk, sc);
+      else
+        KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
       std::swap(pold, pnew);
     }
     ReadScalars();

@@ -27,6 +27,16 @@ struct FcgFusion {
   // is then 0).
   const double* qold;
   double* dir;
+  // The previous iteration's update, folded into the finest down-leg
+  // (up_on): with alpha = *up_alpha, x += alpha p and r -= alpha q on the
+  // kernel's tile cells (the new r written to up_r, not over its input),
+  // and r - alpha q on its halo as the down-leg's input; (r, r) of the tile
+  // cells at up_rr[b].  The stop test then runs in the next level's first
+  // kernel.
+  int up_on;
+  const double* up_alpha;
+  const double *up_p, *up_q;
+  double *up_x, *up_r, *up_rr;
 };
 
 // The stop test above, by one workgroup of 256 threads.
@@ -60,6 +70,8 @@ class Precond {
   }
   // Partials per dot that ApplyFcg writes (0: not supported).
   virtual int FcgPartials() const { return 0; }
+  // ApplyFcg takes the previous iteration's update (FcgFusion::up_on).
+  virtual bool FusesUpdate() const { return false; }
 };
 
 }  // namespace hg

@@ -66,6 +66,10 @@ class MgPrecond : public Precond {
   bool ApplyFcg(const double* r, float* z, hipStream_t st, const int* d_done,
                 const FcgFusion& f, int* ndir) override;
   int FcgPartials() const override { return lv_.size() > 1 ? lv_[0].tall.blocks() : 0; }
+  // level 1 runs a tiled down-leg (its first kernel takes the stop test)
+  bool FusesUpdate() const override {
+    return lv_.size() >= 3 && !(use_coarse_kernel && first_in_kernel_ == 1);
+  }
   int levels() const { return int(lv_.size()); }
   // Average device time (ms) of one launch of a V-cycle kernel ("KDown<double>",
   // "KUp<double, true>" on the finest level, "KCoarseCycle"), launched `reps`

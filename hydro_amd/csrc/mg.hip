@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "mg.h"
 
@@ -234,19 +235,58 @@ __device__ __forceinline__ float SlotApply(const SlotCell& c, const float (*x)[X
   return v;
 }
 
+// The previous FCG iteration's update inside the finest down-leg
+// (FcgFusion::up_on), alpha given: r - alpha q replaces the slots' residual
+// (halo included, so the down-leg smooths the updated residual), and the
+// tile cells write r, x += alpha p and contribute (r, r).
+template <int TY>
+__device__ __forceinline__ void FoldedUpdate(const FcgFusion& f,
+                                             const SlotCell (&sc)[TileSlots<TY>::N],
+                                             double (&rv)[TileSlots<TY>::N], int nx) {
+  using S = TileSlots<TY>;
+  double qv[S::N];
+#pragma unroll
+  for (int s = 0; s < S::N; ++s) qv[s] = sc[s].in ? f.up_q[sc[s].i + nx * sc[s].j] : 0.;
+  const double a = *f.up_alpha;
+  double rr = 0.;
+#pragma unroll
+  for (int s = 0; s < S::N; ++s) {
+    if (!sc[s].in) continue;
+    rv[s] = rv[s] - a * qv[s];
+    if (s < S::NT) {
+      const int k = sc[s].i + nx * sc[s].j;
+      f.up_r[k] = rv[s];
+      f.up_x[k] += a * f.up_p[k];
+      rr += rv[s] * rv[s];
+    }
+  }
+  __shared__ double red[kTileThreads / 64];
+  rr = WaveSum(rr);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.;
+#pragma unroll
+    for (int q = 0; q < kTileThreads / 64; ++q) t += red[q];
+    f.up_rr[blockIdx.x] = t;
+  }
+}
+
 // Fused down-leg of one level: two damped Jacobi sweeps from zero
 // (x1 = w.a D^-1 r, x2 = x1 + w.b D^-1 (r - A x1)) and the restriction of
 // r - A x2 (sum of the four children).  A block of 256 threads works on a
 // 64 x TY tile with a two-cell halo (x1 on tile + 2, x2 on tile + 1 in LDS):
 // one launch and one pass over the level's arrays replace three, and all
 // global loads are issued at the start (TileSlots), so a workgroup waits on
-// memory once.
-template <class RT, int TY>
+// memory once.  UP (FP64 residual only): the previous FCG update first
+// (FoldedUpdate).
+template <class RT, int TY, bool UP = false>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
       const float* __restrict__ dinv, const RT* __restrict__ r, Omega2 w,
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done,
       FcgFusion fcg, int stop_test) {
+  static_assert(!UP || std::is_same<RT, double>::value, "the update is on the FP64 residual");
   if (Done(done)) return;
   if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
   using S = TileSlots<TY>;
@@ -254,11 +294,15 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   __shared__ float s1[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
-  if (tile >= tg.ntx * tg.nty) return;  // uniform over the block
+  if (tile >= tg.ntx * tg.nty) {  // uniform over the block
+    if (UP && threadIdx.x == 0) fcg.up_rr[b] = 0.;
+    return;
+  }
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   SlotCell sc[S::N];
   RT rv[S::N];
   LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
+  if constexpr (UP) FoldedUpdate<TY>(fcg, sc, rv, nx);
 #pragma unroll
   for (int s = 0; s < S::N; ++s)
     if (sc[s].mine) s1[sc[s].lj + 2][sc[s].li + 2] = sc[s].in ? w.a * sc[s].di * float(rv[s]) : 0.f;
@@ -841,26 +885,35 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     return;
   }
   MgLevel& C = lv_[l + 1];
-  // level 0 (FP64 residual) on 64 x kTallY tiles, the FP32 levels on 64 x kMidY
-  if (l == 0)
+  // level 0 (FP64 residual) on 64 x kTallY tiles, the FP32 levels on 64 x kMidY;
+  // with the FCG update folded into level 0 its stop test moves to level 1
+  const bool up = fz.up_on != 0;
+  if (l == 0 && up)
+    KDown<double, kTallY, true><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+        C.r, done, fz, 0);
+  else if (l == 0)
     KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
         L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
         C.r, done, fz, fused);
   else
     KDown<float, kMidY><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
                                                        L.r, Omega2{omega1, omega2}, L.xb, C.nx,
-                                                       C.ny, C.r, done, fz, 0);
-  Cycle(l + 1, nullptr, C.xb, st, done);
+                                                       C.ny, C.r, done, fz,
+                                                       (l == 1 && up) ? 1 : 0);
+  Cycle(l + 1, nullptr, C.xb, st, done, (l == 0 && up) ? f : nullptr);
 #define HG_UP(RT, B, TY, TG, R)                                                               \
   KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
                                                       L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,    \
                                                       Omega2{omega1, omega2}, out, done, fz,  \
                                                       fused)
   if (l == 0) {
+    // after the folded update the level's residual is the one it wrote
+    const double* r0 = up ? fz.up_r : r64;
     if (bilinear)
-      HG_UP(double, true, kTallY, L.tall, r64);
+      HG_UP(double, true, kTallY, L.tall, r0);
     else
-      HG_UP(double, false, kTallY, L.tall, r64);
+      HG_UP(double, false, kTallY, L.tall, r0);
   } else {
     if (bilinear)
       HG_UP(float, true, kMidY, L.mid, L.r);

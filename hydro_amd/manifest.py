@@ -98,6 +98,10 @@ KERNELS = {
     "KUp<double, true> + FCG dots": (lambda nx, ny: _b(nx, ny, cells64=2, cells32=3,
                                                        faces32=1.25, coarse32=0.25),
                                      "as KUp<double, true>, plus q_old"),
+    "KDown<double, 16, true>": (lambda nx, ny: _b(nx, ny, cells64=6, cells32=2, faces32=1,
+                                                  coarse32=0.25),
+                                "KDown<double> with the previous FCG update: plus read q, "
+                                "p, x; write x and the new r"),
 }
 
 
@@ -118,10 +122,12 @@ def _vcycle_bytes(nx, ny, fcg_dots=False):
     """One V-cycle: the finest level's fused kernels plus the FP32 levels
     below it (each level quarters the cells; down-leg reads r, D^-1, k and
     writes x2 and the coarse residual; up-leg reads x2, r, D^-1, k and the
-    coarse correction and writes the output).  fcg_dots: the finest up-leg
-    also reads q_old (flexible CG's direction dots)."""
+    coarse correction and writes the output).  fcg_dots: the V-cycle of a
+    flexible CG iteration: the finest up-leg also reads q_old (the direction
+    dots) and the finest down-leg carries the previous iteration's update."""
     up = "KUp<double, true> + FCG dots" if fcg_dots else "KUp<double, true>"
-    b = kernel_bytes("KDown<double>", nx, ny) + kernel_bytes(up, nx, ny)
+    down = "KDown<double, 16, true>" if fcg_dots else "KDown<double>"
+    b = kernel_bytes(down, nx, ny) + kernel_bytes(up, nx, ny)
     cx, cy = nx // 2, ny // 2
     while cx * cy > 2048 and cx % 2 == 0 and cy % 2 == 0:
         b += _b(cx, cy, cells32=3, faces32=1, coarse32=0.25)      # down
@@ -147,8 +153,8 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
     per_iter += sum(kernel_bytes(k, nx, ny) for k in (
         "KSetOperator", "KSumPart", "KStart", "KResidual"))
     # FCG iteration: stop test and direction dots inside the V-cycle kernels
-    per_fcg = kernel_bytes("KFcgSpmv", nx, ny) + kernel_bytes("KFcgUpdate", nx, ny) \
-        + _vcycle_bytes(nx, ny, fcg_dots=True)
+    # (the update of each iteration runs in the next V-cycle's first kernel)
+    per_fcg = kernel_bytes("KFcgSpmv", nx, ny) + _vcycle_bytes(nx, ny, fcg_dots=True)
     finish = w * 3 * n                                             # NaN check
     advection = substeps * kernel_bytes("KAdvStep<2, false>", nx, ny)
     # density and viscosity, two smoothing passes each in one launch
