@@ -513,15 +513,32 @@ KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
 }
 
 // alpha = (r, z) / (p, q) of iteration k into sc->alpha, for an update
-// that a later kernel applies; (p, q) <= 0 sets fail and done instead.
-__global__ void __launch_bounds__(kB)
+// that a later kernel applies; (p, q) <= 0 sets fail and done instead.  One
+// workgroup of 1024 threads: the fold is a few thousand partials, and wider
+// is faster for it than the 256 of the other folds.
+constexpr int kAlphaThreads = 1024;
+__global__ void __launch_bounds__(kAlphaThreads)
 KFcgAlpha(int k, Parts pt, CgScalars* sc) {
   if (sc->done) return;
-  double f[2];
-  FoldN<2>({pt.dir + pt.ndir, pt.spmv[k & 1]}, {pt.ndir, pt.nb}, f);
+  __shared__ double red[2][kAlphaThreads / 64];
+  double rz = 0., pq = 0.;
+  for (int b = threadIdx.x; b < pt.ndir; b += kAlphaThreads) rz += pt.dir[pt.ndir + b];
+  for (int b = threadIdx.x; b < pt.nb; b += kAlphaThreads) pq += pt.spmv[k & 1][b];
+  rz = WaveSum(rz);
+  pq = WaveSum(pq);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = rz;
+    red[1][threadIdx.x >> 6] = pq;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (f[1] > 0.) {
-      sc->alpha = f[0] / f[1];
+    double RZ = 0., PQ = 0.;
+    for (int w = 0; w < kAlphaThreads / 64; ++w) {
+      RZ += red[0][w];
+      PQ += red[1][w];
+    }
+    if (PQ > 0.) {
+      sc->alpha = RZ / PQ;
     } else {
       sc->fail = 1;
       sc->done = 1;
@@ -869,7 +886,7 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
       }
       KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pk, sc);
       if (fuse)
-        KFcgAlpha<<<1, kB, 0, stream_>>>(k, pk, sc);
+        KFcgAlpha<<<1, kAlphaThreads, 0, stream_>>>(k, pk, sc);
       else
         KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
       std::swap(pold, pnew);
