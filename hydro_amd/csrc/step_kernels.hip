@@ -474,18 +474,21 @@ KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ a
   }
 }
 
-// Folds the per-block maxima of KCorrectCells into rs (one workgroup).
-__global__ void __launch_bounds__(kBlock)
+// Folds the per-block maxima of KCorrectCells into rs (one workgroup of
+// kFoldThreads: the partials number N / 256, and a wider workgroup folds
+// them sooner; a maximum does not depend on the order).
+constexpr int kFoldThreads = 1024;
+__global__ void __launch_bounds__(kFoldThreads)
 KMaxFold(const double* __restrict__ part, int n, double* __restrict__ rs) {
   double m = 0.;
-  for (int b = threadIdx.x; b < n; b += kBlock) m = fmax(m, part[b]);
+  for (int b = threadIdx.x; b < n; b += kFoldThreads) m = fmax(m, part[b]);
   m = WaveMax(m);
-  __shared__ double red[kBlock / 64];
+  __shared__ double red[kFoldThreads / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = red[0];
-    for (int w = 1; w < kBlock / 64; ++w) t = fmax(t, red[w]);
+    for (int w = 1; w < kFoldThreads / 64; ++w) t = fmax(t, red[w]);
     *rs = t;
   }
 }
@@ -1428,7 +1431,7 @@ void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const do
   const long n = long(g.nx) * g.ny;
   KCorrectCells<<<Grid(n), kBlock, 0, st>>>(g, pc, ap, pprev, alpha_p, pcurr, u0, u1, up0, up1,
                                             part);
-  KMaxFold<<<1, kBlock, 0, st>>>(part, int((n + kBlock - 1) / kBlock), rs);
+  KMaxFold<<<1, kFoldThreads, 0, st>>>(part, int((n + kBlock - 1) / kBlock), rs);
 }
 void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
                         const double* pc, double* flux) {
