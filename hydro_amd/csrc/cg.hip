@@ -302,26 +302,14 @@ KFcgDir(Walk w, int k, const float* __restrict__ z, const double* __restrict__ q
   PutPartials<2>(v, pt.dir);
 }
 
-// p = z + beta p_old with beta = -(z, q_old) / (p_old, q_old); q = A p.
-// Iteration 0: p = z (p_old is not read).
+// p = z + beta p_old with beta = -(z, q_old) / (p_old, q_old) (sc->beta,
+// from KFcgBeta); q = A p.  Iteration 0: p = z (p_old is not read).
 __global__ void __launch_bounds__(kB)
 KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict__ ky,
          const float* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
          double* __restrict__ q, Parts pt, CgScalars* sc) {
   if (sc->done) return;
-  const int nb = pt.nb;
-  double f[3];
-  FoldN<3>({pt.dir + pt.ndir, pt.dir, pt.spmv[(k + 1) & 1]}, {pt.ndir, pt.ndir, nb}, f);
-  const double rz = f[0];
-  if (!(rz > 0.)) {  // preconditioner not positive on r
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->fail = 1;
-      sc->done = 1;
-    }
-    return;
-  }
-  double beta = 0.;
-  if (k > 0) beta = -f[1] / f[2];
+  const double beta = sc->beta;  // KFcgBeta
   // p is formed once per cell of the tile and its one-cell halo, in LDS, and
   // the stencil reads it there; each thread's loads (two halo-region cells,
   // its own cell's faces) are issued together
@@ -512,33 +500,60 @@ KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
   PutPartials<1>(v, pt.upd[1] + pt.nb);
 }
 
-// alpha = (r, z) / (p, q) of iteration k into sc->alpha, for an update
-// that a later kernel applies; (p, q) <= 0 sets fail and done instead.  One
-// workgroup of 1024 threads: the fold is a few thousand partials, and wider
-// is faster for it than the 256 of the other folds.
-constexpr int kAlphaThreads = 1024;
-__global__ void __launch_bounds__(kAlphaThreads)
-KFcgAlpha(int k, Parts pt, CgScalars* sc) {
-  if (sc->done) return;
-  __shared__ double red[2][kAlphaThreads / 64];
-  double rz = 0., pq = 0.;
-  for (int b = threadIdx.x; b < pt.ndir; b += kAlphaThreads) rz += pt.dir[pt.ndir + b];
-  for (int b = threadIdx.x; b < pt.nb; b += kAlphaThreads) pq += pt.spmv[k & 1][b];
-  rz = WaveSum(rz);
-  pq = WaveSum(pq);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = rz;
-    red[1][threadIdx.x >> 6] = pq;
+// Sums of NQ partial arrays by one workgroup of kWide threads, the same in
+// every thread.  The FCG scalars are folded once per iteration this way
+// rather than in every workgroup of the kernel that uses them: a few
+// thousand partials, which a wide workgroup folds sooner than 256 threads.
+constexpr int kWide = 1024;
+template <int NQ>
+__device__ __forceinline__ void WideFold(const double* const (&part)[NQ], const int (&n)[NQ],
+                                         double (&out)[NQ]) {
+  __shared__ double red[NQ][kWide / 64];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < n[q]; b += kWide) s += part[q][b];
+    s = WaveSum(s);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s;
   }
   __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double t = 0.;
+    for (int w = 0; w < kWide / 64; ++w) t += red[q][w];
+    out[q] = t;
+  }
+}
+
+// FCG(1) direction scalars of iteration k from the preconditioner's dots:
+// sc->rz = (r, z) and sc->beta = -(z, q_old) / (p_old, q_old) (0 at k = 0);
+// (r, z) <= 0 (preconditioner not positive on r) sets fail and done.
+__global__ void __launch_bounds__(kWide)
+KFcgBeta(int k, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  double f[3];
+  WideFold<3>({pt.dir + pt.ndir, pt.dir, pt.spmv[(k + 1) & 1]}, {pt.ndir, pt.ndir, pt.nb}, f);
   if (threadIdx.x == 0) {
-    double RZ = 0., PQ = 0.;
-    for (int w = 0; w < kAlphaThreads / 64; ++w) {
-      RZ += red[0][w];
-      PQ += red[1][w];
+    if (f[0] > 0.) {
+      sc->rz = f[0];
+      sc->beta = k > 0 ? -f[1] / f[2] : 0.;
+    } else {
+      sc->fail = 1;
+      sc->done = 1;
     }
-    if (PQ > 0.) {
-      sc->alpha = RZ / PQ;
+  }
+}
+
+// alpha = (r, z) / (p, q) of iteration k into sc->alpha, for an update that
+// a later kernel applies; (p, q) <= 0 sets fail and done instead.
+__global__ void __launch_bounds__(kWide)
+KFcgAlpha(int k, Parts pt, CgScalars* sc) {
+  if (sc->done) return;
+  double f[1];
+  WideFold<1>({pt.spmv[k & 1]}, {pt.nb}, f);
+  if (threadIdx.x == 0) {
+    if (f[0] > 0.) {
+      sc->alpha = sc->rz / f[0];
     } else {
       sc->fail = 1;
       sc->done = 1;
@@ -884,9 +899,10 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
         pc->Apply(r_, z32_, stream_, &sc->done);
         KFcgDir<<<nb, kB, 0, stream_>>>(w, k, z32_, q_, r_, pt, sc);
       }
+      KFcgBeta<<<1, kWide, 0, stream_>>>(k, pk, sc);
       KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pk, sc);
       if (fuse)
-        KFcgAlpha<<<1, kAlphaThreads, 0, stream_>>>(k, pk, sc);
+        KFcgAlpha<<<1, kWide, 0, stream_>>>(k, pk, sc);
       else
         KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
       std::swap(pold, pnew);
