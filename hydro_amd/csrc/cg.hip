@@ -46,6 +46,7 @@ struct CgScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, zq;
   int iter, max_iter, done, fail;
   double mean;
+  double warm_s;  // scale of a warm start's x
 };
 
 #define HG_CHECK(x)                                                                  \
@@ -431,13 +432,13 @@ KResidual(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
   PutPartials<1>(v, pt.misc);
 }
 
-// Start of a solve from x = 0: b_ = b - mean(b) (mean folded from the misc
-// partials) or b_ = b; r = b_; x = 0; partial (r, r) into update slot 1, the
+// Start of a solve from x = 0: b_ = b - mean(b) (sc->mean, which KFoldMean
+// folded from the misc partials) or b_ = b; r = b_; x = 0; partial (r, r) into update slot 1, the
 // (r, r) iteration 0 of FCG reads.
 __global__ void __launch_bounds__(kB)
 KStart(Walk w, const double* __restrict__ b, int project, double* __restrict__ bp,
-       double* __restrict__ r, double* __restrict__ x, Parts pt) {
-  const double mean = project ? Fold(pt.misc, pt.nb) / (double(w.nx) * double(w.ny)) : 0.;
+       double* __restrict__ r, double* __restrict__ x, Parts pt, const CgScalars* sc) {
+  const double mean = project ? sc->mean : 0.;  // KFoldMean
   double acc = 0.;
   ForTiles(w, [&](int i, int j) {
     const int c = i + w.nx * j;
@@ -460,8 +461,9 @@ KStart(Walk w, const double* __restrict__ b, int project, double* __restrict__ b
 __global__ void __launch_bounds__(kB)
 KStartWarm(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
            const double* __restrict__ b, int project, double* __restrict__ bp,
-           double* __restrict__ r, const double* __restrict__ x, Parts pt) {
-  const double mean = project ? Fold(pt.misc, pt.nb) / (double(w.nx) * double(w.ny)) : 0.;
+           double* __restrict__ r, const double* __restrict__ x, Parts pt,
+           const CgScalars* sc) {
+  const double mean = project ? sc->mean : 0.;  // KFoldMean
   double ba = 0., aa = 0., bb = 0.;
   ForTiles(w, [&](int i, int j) {
     const int c = i + w.nx * j;
@@ -480,14 +482,13 @@ KStartWarm(Walk w, const double* __restrict__ kx, const double* __restrict__ ky,
   PutPartials<1>(v1, pt.spmv[0]);
 }
 
-// Pass 2: s = (b_, A x) / (A x, A x) (0 when A x = 0); x <- s x,
+// Pass 2: s = (b_, A x) / (A x, A x) (0 when A x = 0; sc->warm_s, folded
+// by KWarmScale); x <- s x,
 // r <- b_ - s A x; partial (r, r) into update slot 1.
 __global__ void __launch_bounds__(kB)
 KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
-                double* __restrict__ x, Parts pt) {
-  double f[2];
-  FoldN<2>({pt.upd[0], pt.upd[0] + pt.nb}, {pt.nb, pt.nb}, f);
-  const double s = f[1] > 0. ? f[0] / f[1] : 0.;
+                double* __restrict__ x, Parts pt, const CgScalars* sc) {
+  const double s = sc->warm_s;  // KWarmScale
   double rr = 0.;
   ForTiles(w, [&](int i, int j) {
     const int c = i + w.nx * j;
@@ -559,6 +560,24 @@ KFcgAlpha(int k, Parts pt, CgScalars* sc) {
       sc->done = 1;
     }
   }
+}
+
+// sc->mean = (sum of the KSumPart partials) / n, the mean a singular
+// system's right-hand side is projected by.
+__global__ void __launch_bounds__(kWide)
+KFoldMean(Parts pt, double n, CgScalars* sc) {
+  double f[1];
+  WideFold<1>({pt.misc}, {pt.nb}, f);
+  if (threadIdx.x == 0) sc->mean = f[0] / n;
+}
+
+// sc->warm_s = (b_, A x) / (A x, A x) from KStartWarm's partials (0 when
+// A x = 0).
+__global__ void __launch_bounds__(kWide)
+KWarmScale(Parts pt, CgScalars* sc) {
+  double f[2];
+  WideFold<2>({pt.upd[0], pt.upd[0] + pt.nb}, {pt.nb, pt.nb}, f);
+  if (threadIdx.x == 0) sc->warm_s = f[1] > 0. ? f[0] / f[1] : 0.;
 }
 
 // Folds `nb` partials into sc->rr (and sc->bb when set_bb).
@@ -784,13 +803,17 @@ void CgSolver::Prepare(const double* b, double* x, bool project, bool warm) {
   Walk w = MakeWalk(nx_, ny_, &nb);
   Parts pt = MakeParts(part_, nb);
   CgScalars* sc = (CgScalars*)scal_;
-  if (project) KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
+  if (project) {
+    KSumPart<<<nb, kB, 0, stream_>>>(w, b, pt);
+    KFoldMean<<<1, kWide, 0, stream_>>>(pt, double(nx_) * double(ny_), sc);
+  }
   if (warm) {
-    KStartWarm<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b, project ? 1 : 0, b_, r_, x, pt);
-    KStartWarmScale<<<nb, kB, 0, stream_>>>(w, b_, r_, x, pt);
+    KStartWarm<<<nb, kB, 0, stream_>>>(w, kx_, ky_, b, project ? 1 : 0, b_, r_, x, pt, sc);
+    KWarmScale<<<1, kWide, 0, stream_>>>(pt, sc);
+    KStartWarmScale<<<nb, kB, 0, stream_>>>(w, b_, r_, x, pt, sc);
     KTakeRR<<<1, kB, 0, stream_>>>(pt.spmv[0], nb, sc, 1);
   } else {
-    KStart<<<nb, kB, 0, stream_>>>(w, b, project ? 1 : 0, b_, r_, x, pt);
+    KStart<<<nb, kB, 0, stream_>>>(w, b, project ? 1 : 0, b_, r_, x, pt, sc);
     KTakeRR<<<1, kB, 0, stream_>>>(pt.upd[1] + nb, nb, sc, 1);
   }
 }
