@@ -74,11 +74,15 @@ def run_path(text, steps, warmup, label, train=False):
                        "heldout_ms_vcycle_alone": round(ts.val_ms_baseline, 3),
                        "train_ms": round(ts.ms, 1)}
         log(f"{label} learned correction: {train_stats}")
+    h.timing()  # waits for the warm-up's last kernels
     t0 = time.perf_counter()
     iters = []
     for i in range(steps):
         st = h.step()
         iters.append((st.simple_iterations, st.pressure_solve_iters_total))
+    # hg_step returns with the step's advection and property kernels still
+    # queued: the clock stops once they have finished
+    h.timing()
     dt = time.perf_counter() - t0
     log(f"{label}: {steps} steps in {dt:.3f} s, (SIMPLE, pressure iters) per step {iters}")
     if train:
@@ -96,9 +100,9 @@ def cpu_baseline(seconds_cap=150):
     exe = pyoracle.REF_BIN_FAST
     if not os.path.exists(exe):
         return None
-    from tests import cases
+    from hydro_amd import workloads
     general = os.path.join(ROOT, "oracle", "_ref", "general.hydroconf")
-    text = open(general).read() + cases.rt(solver_p="gauss_seidel", solver_v="lu") + """
+    text = open(general).read() + workloads.rt(solver_p="gauss_seidel", solver_v="lu") + """
 set vect A1 (0, 1)
 set vect B1 (1, 2)
 set bool no_output 1
@@ -176,11 +180,11 @@ def main():
 
     from hydro_amd import capi, manifest
     from hydro_amd.hydro import Poisson
-    from tests import cases
+    from hydro_amd import workloads
 
     nx, ny = 1024, 2048
-    text64 = cases.rt(nx, ny, solver_p="cg", solver_v="lu_rb", precision="fp64")
-    textmx = cases.rt(nx, ny, solver_p="fcg_learned", solver_v="lu_rb", precision="mixed")
+    text64 = workloads.rt(nx, ny, solver_p="cg", solver_v="lu_rb", precision="fp64")
+    textmx = workloads.rt(nx, ny, solver_p="fcg_learned", solver_v="lu_rb", precision="mixed")
 
     h64, t64, it64, _ = run_path(text64, args.steps, args.warmup, "fp64")
     # the SIMPLE pressure system of the last step, for the solve comparison

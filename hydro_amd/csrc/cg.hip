@@ -14,12 +14,15 @@
 //
 // Work distribution: a persistent grid of at most 2048 workgroups walks
 // 64x4 tiles (tile.h), each XCD a contiguous band.  Reductions use no
-// atomics: each workgroup writes its partial sums, and the kernel that needs
-// the scalar folds all partials itself, in block order, in every workgroup
-// (so every workgroup derives bit-identical alpha / beta / stop decisions).
-// Partial buffers alternate by iteration parity so a fold never races with
-// the next producer.  Workgroup 0 records the scalars for the host, which
-// queues batches of iterations and polls rarely.
+// atomics: each workgroup writes its partial sums.  The scalars of an FCG
+// iteration -- beta with (r, z), alpha, the projection mean, the warm-start
+// multiple -- are folded once, in a fixed order, by one 1024-thread
+// workgroup (KFcgBeta, KFcgAlpha, KFoldMean, KWarmScale) into CgScalars,
+// which every later kernel reads; the Jacobi-PCG kernels and the stop test
+// fold the few partials they need themselves, in block order, so every
+// workgroup derives the same decisions.  Partial buffers alternate by
+// iteration parity so a fold never races with the next producer.  The host
+// queues batches of iterations and reads the scalars rarely.
 
 #include <algorithm>
 #include <cmath>
@@ -367,24 +370,13 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
   PutPartials<1>(v, pt.spmv[k & 1]);
 }
 
-// x += alpha p, r -= alpha q with alpha = (r, z) / (p, q); partial (r, r).
+// x += alpha p, r -= alpha q with alpha = (r, z) / (p, q) as KFcgAlpha folded
+// it once (from the (r, z) that KFcgBeta checked); partial (r, r).
 __global__ void __launch_bounds__(kB)
 KFcgUpdate(Walk w, int k, const double* __restrict__ p, const double* __restrict__ q,
            double* __restrict__ x, double* __restrict__ r, Parts pt, CgScalars* sc) {
-  if (sc->done) return;
-  const int nb = pt.nb;
-  double f[2];
-  FoldN<2>({pt.dir + pt.ndir, pt.spmv[k & 1]}, {pt.ndir, nb}, f);
-  const double rz = f[0];
-  const double pq = f[1];
-  if (!(pq > 0.)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->fail = 1;
-      sc->done = 1;
-    }
-    return;
-  }
-  const double a = rz / pq;
+  if (sc->done) return;  // also when KFcgAlpha found (p, q) <= 0
+  const double a = sc->alpha;
   double rr = 0.;
   ForTiles(w, [&](int i, int j) {
     const int c = i + w.nx * j;
@@ -924,10 +916,8 @@ CgResult CgSolver::SolveFlexible(const double* b, double* x, double tol, int max
       }
       KFcgBeta<<<1, kWide, 0, stream_>>>(k, pk, sc);
       KFcgSpmv<<<nb, kB, 0, stream_>>>(w, k, kx_, ky_, z32_, pold, pnew, q_, pk, sc);
-      if (fuse)
-        KFcgAlpha<<<1, kWide, 0, stream_>>>(k, pk, sc);
-      else
-        KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
+      KFcgAlpha<<<1, kWide, 0, stream_>>>(k, pk, sc);
+      if (!fuse) KFcgUpdate<<<nb, kB, 0, stream_>>>(w, k, pnew, q_, x, r_, pk, sc);
       std::swap(pold, pnew);
     }
     ReadScalars();

@@ -220,6 +220,7 @@ struct hg_ctx {
   bool pressure_system = false;  // cg/mg hold the operator of a pressure solve
   hipEvent_t ev[4];
   bool timed = false;  // ev[0..3] hold the regions of the last hg_step
+  hipEvent_t kev[2];   // hg_time_kernel's own pair
 
   double* Alloc(long n) {
     void* p = nullptr;
@@ -239,6 +240,7 @@ struct hg_ctx {
     net.reset();
     mg.reset();
     for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : kev) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
   // Readbacks that wait for the next synchronisation of the stream instead
@@ -323,7 +325,9 @@ void UpdateProperties(hg_ctx* c) {
   Smooth(c, c->fy, c->fy, c->cfg.force_smooth_times);
 }
 
-bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d);
+int NanCheck(hg_ctx* c, const double* a, const double* b, const double* d);
+void SolveRowsIterative(hg_ctx* c, int32_t solver, hg::Rows rows, const double* k, double* x,
+                        int* iters, double* diff);
 
 // InitHeatSolver (hydro2d.hpp:652-688): a uniform temperature_initial; a
 // boundary face whose centre lies in [heat_box_lb, heat_box_rt] (bounds
@@ -399,14 +403,18 @@ void HeatStep(hg_ctx* c) {
                      c->tmp1);
       break;
     case HG_SOLVER_GAUSS_SEIDEL:
-      hg::LaunchGsWavefront(c->st, nx, ny, rows, c->k0, c->x0, cfg.lu_relaxed_tolerance,
-                            cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
-                            c->gs_info);
+    case HG_SOLVER_JACOBI:
+    case HG_SOLVER_LU_RELAXED: {
+      int iters;
+      double diff;
+      SolveRowsIterative(c, cfg.linear_solver_heat, rows, c->k0, c->x0, &iters, &diff);
       KAddTo<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->t_a, c->x0, c->t_b);
       break;
+    }
     default:
       throw HgError(HG_ERR_UNSUPPORTED,
-                    "linear_solver_heat: supported values are lu, lu_rb, gauss_seidel");
+                    "linear_solver_heat: supported values are lu, lu_rb, gauss_seidel, jacobi, "
+                    "lu_relaxed");
   }
   // time_prev <- time_curr <- iter_curr
   double* old_tp = c->t_tp;
@@ -517,24 +525,32 @@ void InitialFlux(hg_ctx* c) {
   HG_CHECK(hipMemcpyAsync(c->fl_tp, c->fl_tc, c->nf * 8, hipMemcpyDeviceToDevice, c->st));
 }
 
-bool NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
+// Returns 0 when a, b and d are free of NaN / inf; else bit 1 for a, bit 2
+// for b or d.
+int NanCheck(hg_ctx* c, const double* a, const double* b, const double* d) {
   // the flag is zero between checks (zeroed at allocation, and again after
   // a check that set it), so no clearing pass precedes the kernel
   hg::LaunchNanCheck(c->st, a, b, d, int(c->nc), c->nan_d);
   HG_CHECK(hipMemcpyAsync(c->flag_h, c->nan_d, sizeof(int), hipMemcpyDeviceToHost, c->st));
   c->Sync();
-  if (*c->flag_h == 0) return false;
+  const int bits = *c->flag_h;
+  if (bits == 0) return 0;
   HG_CHECK(hipMemsetAsync(c->nan_d, 0, sizeof(int), c->st));
   c->Sync();
-  return true;
+  return bits;
 }
 
 void StartStep(hg_ctx* c) {
   c->fluid_iter = 0;
   c->cd_iter = 0;
   c->iter_rs.clear();
-  if (!c->fields_checked && NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]))
-    throw HgError(HG_ERR_NAN, "NaN initial pressure");
+  // FluidSimple::StartStep checks the pressure, then the velocity solver's
+  // StartStep the velocity (fluid.hpp:795-797, conv_diff.hpp:119-121)
+  if (!c->fields_checked) {
+    const int bits = NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]);
+    if (bits & 1) throw HgError(HG_ERR_NAN, "NaN initial pressure");
+    if (bits & 2) throw HgError(HG_ERR_NAN, "NaN initial field");
+  }
   c->fields_checked = false;
   // iter_curr aliases time_curr after FinishStep: give it its own buffer
   if (c->p_ic == c->p_tc) c->p_ic = c->p_spare;
@@ -552,6 +568,61 @@ void StartStep(hg_ctx* c) {
   hg::LaunchFacesDeriv0Pair(c->st, c->geo, c->frx, c->fry, c->ffrx, c->ffry);
 }
 
+// The reference's iterative row solvers from a zero guess into x:
+// gauss_seidel (linear.hpp:685-715), jacobi (:750-783) and lu_relaxed
+// (:592-648), all configured by the lu_relaxed_* keys (hydro2d.hpp:195-248).
+// *iters and *diff receive the iteration count and final max|corr| the
+// reference prints (linear.hpp:647,712,779).  Uses the Jacobi work arrays as
+// scratch.
+void SolveRowsIterative(hg_ctx* c, int32_t solver, hg::Rows rows, const double* k, double* x,
+                        int* iters, double* diff) {
+  const int nx = c->geo.nx, ny = c->geo.ny;
+  const hg_config& cfg = c->cfg;
+  const double tol = cfg.lu_relaxed_tolerance, omega = cfg.lu_relaxed_relaxation_factor;
+  const int limit = cfg.lu_relaxed_num_iters_limit;
+  hg::JacobiWork& w = c->jw;
+  if (solver == HG_SOLVER_JACOBI) {
+    HG_CHECK(hipMemsetAsync(w.xa, 0, c->nc * 8, c->st));
+    HG_CHECK(hipMemsetAsync(w.diff, 0, size_t(limit + 2) * 8, c->st));
+    HG_CHECK(hipMemsetAsync(w.state, 0, 4 * sizeof(int), c->st));
+    int first = 0;
+    while (first <= limit) {
+      int n = std::min(64, limit + 1 - first);
+      hg::LaunchJacobiSweeps(c->st, nx, ny, rows, k, w, first, n, tol, limit, omega);
+      first += n;
+      HG_CHECK(hipMemcpyAsync(c->flag_h, w.state, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+      c->Sync();
+      if (c->flag_h[0]) break;
+    }
+    const int res_buf = c->flag_h[3];
+    const int it = c->flag_h[2];
+    HG_CHECK(hipMemcpyAsync(x, res_buf ? w.xb : w.xa, c->nc * 8, hipMemcpyDeviceToDevice, c->st));
+    unsigned long long bits;
+    const int last_sweep = std::min(first, limit + 1) - 1;
+    // the final diff is that of the last executed sweep
+    HG_CHECK(hipMemcpyAsync(&bits, w.diff + std::max(0, std::min(it, last_sweep)), 8,
+                            hipMemcpyDeviceToHost, c->st));
+    c->Sync();
+    std::memcpy(diff, &bits, 8);
+    *iters = it;
+    return;
+  }
+  if (solver == HG_SOLVER_GAUSS_SEIDEL)
+    hg::LaunchGsWavefront(c->st, nx, ny, rows, k, x, tol, limit, omega, c->gs_info);
+  else if (solver == HG_SOLVER_LU_RELAXED)
+    hg::LaunchLuRelaxed(c->st, nx, ny, rows, k, x, w.xa, w.xb, tol, limit, omega, c->gs_info);
+  else
+    throw HgError(HG_ERR_UNSUPPORTED, "not an iterative row solver");
+  HG_CHECK(hipMemcpyAsync(c->info_h, c->gs_info, 16, hipMemcpyDeviceToHost, c->st));
+  c->Sync();
+  *iters = int(c->info_h[0]);
+  std::memcpy(diff, &c->info_h[1], 8);
+}
+
+inline bool IterativeRowSolver(int32_t s) {
+  return s == HG_SOLVER_GAUSS_SEIDEL || s == HG_SOLVER_JACOBI || s == HG_SOLVER_LU_RELAXED;
+}
+
 void SolveVelocity(hg_ctx* c, hg::Rows rows, hg_iter_stats* is) {
   const int nx = c->geo.nx, ny = c->geo.ny;
   const hg_config& cfg = c->cfg;
@@ -566,23 +637,25 @@ void SolveVelocity(hg_ctx* c, hg::Rows rows, hg_iter_stats* is) {
                      c->u_ic[0], c->u_ic[1]);
       is->velocity_solve_iters[0] = is->velocity_solve_iters[1] = 0;
       break;
-    case HG_SOLVER_GAUSS_SEIDEL: {
+    case HG_SOLVER_GAUSS_SEIDEL:
+    case HG_SOLVER_JACOBI:
+    case HG_SOLVER_LU_RELAXED: {
+      // one component after the other, as the reference solves them
+      // (conv_diff.hpp:245, fluid.hpp:157-165)
       const double* kk[2] = {c->k0, c->k1};
       double* xx[2] = {c->x0, c->x1};
       for (int d = 0; d < 2; ++d) {
-        hg::LaunchGsWavefront(c->st, nx, ny, rows, kk[d], xx[d], cfg.lu_relaxed_tolerance,
-                              cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
-                              c->gs_info);
-        HG_CHECK(hipMemcpyAsync(c->info_h, c->gs_info, 16, hipMemcpyDeviceToHost, c->st));
-        c->Sync();
-        is->velocity_solve_iters[d] = int(c->info_h[0]);
+        double diff;
+        SolveRowsIterative(c, cfg.linear_solver_velocity, rows, kk[d], xx[d],
+                           &is->velocity_solve_iters[d], &diff);
         KAddTo<<<G(c->nc), 256, 0, c->st>>>(c->nc, c->u_ip[d], xx[d], c->u_ic[d]);
       }
       break;
     }
     default:
       throw HgError(HG_ERR_UNSUPPORTED,
-                    "linear_solver_velocity: supported values are lu, lu_rb, gauss_seidel");
+                    "linear_solver_velocity: supported values are lu, lu_rb, gauss_seidel, "
+                    "jacobi, lu_relaxed");
   }
 }
 
@@ -592,48 +665,12 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
   hg::Rows rows{c->Ps, c->Pw, c->Pc, c->Pe, c->Pn};
   is->pressure_converged = 1;
   switch (cfg.linear_solver_pressure) {
-    case HG_SOLVER_GAUSS_SEIDEL: {
-      hg::LaunchGsWavefront(c->st, nx, ny, rows, c->Pk, c->pcorr, cfg.lu_relaxed_tolerance,
-                            cfg.lu_relaxed_num_iters_limit, cfg.lu_relaxed_relaxation_factor,
-                            c->gs_info);
-      HG_CHECK(hipMemcpyAsync(c->info_h, c->gs_info, 16, hipMemcpyDeviceToHost, c->st));
-      c->Sync();
-      is->pressure_solve_iters = int(c->info_h[0]);
+    case HG_SOLVER_GAUSS_SEIDEL:
+    case HG_SOLVER_JACOBI:
+    case HG_SOLVER_LU_RELAXED: {
       double d;
-      std::memcpy(&d, &c->info_h[1], 8);
-      is->pressure_solve_diff = d;
-      is->pressure_converged = !(d > cfg.lu_relaxed_tolerance);
-      break;
-    }
-    case HG_SOLVER_JACOBI: {
-      hg::JacobiWork& w = c->jw;
-      const int limit = cfg.lu_relaxed_num_iters_limit;
-      HG_CHECK(hipMemsetAsync(w.xa, 0, c->nc * 8, c->st));
-      HG_CHECK(hipMemsetAsync(w.diff, 0, size_t(limit + 2) * 8, c->st));
-      HG_CHECK(hipMemsetAsync(w.state, 0, 4 * sizeof(int), c->st));
-      int first = 0;
-      while (first <= limit) {
-        int n = std::min(64, limit + 1 - first);
-        hg::LaunchJacobiSweeps(c->st, nx, ny, rows, c->Pk, w, first, n, cfg.lu_relaxed_tolerance,
-                               limit, cfg.lu_relaxed_relaxation_factor);
-        first += n;
-        HG_CHECK(hipMemcpyAsync(c->flag_h, w.state, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-        c->Sync();
-        if (c->flag_h[0]) break;
-      }
-      int res_buf = c->flag_h[3];
-      int iters = c->flag_h[2];
-      HG_CHECK(hipMemcpyAsync(c->pcorr, res_buf ? w.xb : w.xa, c->nc * 8,
-                              hipMemcpyDeviceToDevice, c->st));
-      unsigned long long bits;
-      int last_sweep = std::min(first, limit + 1) - 1;
-      // the final diff is that of the last executed sweep
-      HG_CHECK(hipMemcpyAsync(&bits, w.diff + std::max(0, std::min(iters, last_sweep)), 8,
-                              hipMemcpyDeviceToHost, c->st));
-      c->Sync();
-      double d;
-      std::memcpy(&d, &bits, 8);
-      is->pressure_solve_iters = iters;
+      SolveRowsIterative(c, cfg.linear_solver_pressure, rows, c->Pk, c->pcorr,
+                         &is->pressure_solve_iters, &d);
       is->pressure_solve_diff = d;
       is->pressure_converged = !(d > cfg.lu_relaxed_tolerance);
       break;
@@ -652,8 +689,6 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
     case HG_SOLVER_CG:
     case HG_SOLVER_FCG_MG:
     case HG_SOLVER_FCG_LEARNED: {
-      if (c->pinned >= 0)
-        throw HgError(HG_ERR_UNSUPPORTED, "cg with pressure_fixed_point is not supported");
       // operator k_f = -fcoef (negated on the way in); the right-hand side
       // pb = -(row constant) was written by the pressure right-hand side kernel
       c->cg->SetOperator(c->fcoef, c->fcoef + c->geo.nfx, -1.);
@@ -781,11 +816,15 @@ void FinishStep(hg_ctx* c) {
   Rotate(c->p_tp, c->p_tc, c->p_ic, c->p_spare);
   Rotate(c->fl_tp, c->fl_tc, c->fl_ic, c->fl_spare);
   for (int d = 0; d < 2; ++d) Rotate(c->u_tp[d], c->u_tc[d], c->u_ic[d], c->u_spare[d]);
+  // the checks come before the time advances, as FinishStep throws before
+  // IncTime (fluid.hpp:1164-1168, conv_diff.hpp:255-258): pressure first
+  const int bits = NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1]);
+  if (bits & 1) throw HgError(HG_ERR_NAN, "NaN pressure");
+  if (bits & 2) throw HgError(HG_ERR_NAN, "NaN field");
   c->t_fluid += c->dt;
   c->steps += 1;
   c->mod_t = c->mod_tnp;
   c->mod_tnp = c->mod_t + c->dt;
-  if (NanCheck(c, c->p_tc, c->u_tc[0], c->u_tc[1])) throw HgError(HG_ERR_NAN, "NaN pressure");
   c->fields_checked = true;
   // (NanCheck synchronised: the step's last pressure solve is settled)
   c->pcorr_warm = c->fluid_iter == 1 && c->pcorr_converged;
@@ -830,10 +869,16 @@ hg_status Guard(hg_ctx* ctx, const std::function<void()>& fn) {
 
 }  // namespace
 
+namespace hg {
+// hg_last_error(NULL) reports the message of the last failed context-free
+// call on this thread.
+void SetThreadError(const std::string& msg);
+}
+
 // ===========================================================================
 extern "C" {
 
-// Message of the last failed hg_create on this thread.
+// Message of the last failed hg_create (or context-free call) on this thread.
 static thread_local std::string g_create_error;
 
 hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
@@ -853,6 +898,31 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
                                             std::to_string(cfg.unsupported_flags) + ")");
     if (cfg.force_geometric_average)
       throw HgError(HG_ERR_UNSUPPORTED, "force_geometric_average is not supported");
+    // solver names are checked here, before any step runs (the reference's
+    // factory throws at construction, hydro2d.hpp:247, 267-276)
+    {
+      const int32_t v = cfg.linear_solver_velocity, p = cfg.linear_solver_pressure,
+                    h = cfg.linear_solver_heat;
+      const bool row_solver_v = v == HG_SOLVER_LU || v == HG_SOLVER_LU_RB ||
+                                v == HG_SOLVER_GAUSS_SEIDEL || v == HG_SOLVER_JACOBI ||
+                                v == HG_SOLVER_LU_RELAXED;
+      if (!row_solver_v)
+        throw HgError(HG_ERR_UNSUPPORTED,
+                      "linear_solver_velocity: supported values are lu, lu_rb, gauss_seidel, "
+                      "jacobi, lu_relaxed");
+      if (p < HG_SOLVER_LU || p > HG_SOLVER_FCG_MG)
+        throw HgError(HG_ERR_BAD_ARG, "linear_solver_pressure: unknown solver");
+      if (cfg.heat_enable && !(h == HG_SOLVER_LU || h == HG_SOLVER_LU_RB ||
+                               h == HG_SOLVER_GAUSS_SEIDEL || h == HG_SOLVER_JACOBI ||
+                               h == HG_SOLVER_LU_RELAXED))
+        throw HgError(HG_ERR_UNSUPPORTED,
+                      "linear_solver_heat: supported values are lu, lu_rb, gauss_seidel, "
+                      "jacobi, lu_relaxed");
+      if (FaceSolver(p) && cfg.has_pressure_fixed_point)
+        throw HgError(HG_ERR_UNSUPPORTED,
+                      "cg / fcg solve the pressure up to a constant: pressure_fixed_point is not "
+                      "supported with them");
+    }
     if (cfg.simpler && FaceSolver(cfg.linear_solver_pressure))
       throw HgError(HG_ERR_UNSUPPORTED,
                     "simpler runs with the row solvers (gauss_seidel, jacobi, lu), not cg / fcg");
@@ -863,6 +933,7 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     if (long(cfg.nx) * cfg.ny > (1L << 30)) throw HgError(HG_ERR_BAD_ARG, "mesh too large");
     HG_CHECK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     for (auto& e : c->ev) HG_CHECK(hipEventCreate(&e));
+    for (auto& e : c->kev) HG_CHECK(hipEventCreate(&e));
     c->np = cfg.num_phases;
     c->mesh.Init(cfg.nx, cfg.ny, cfg.a, cfg.b);
     c->geo = c->mesh.geo;
@@ -1030,6 +1101,12 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
 }
 
 void hg_destroy(hg_ctx* ctx) { delete ctx; }
+
+}  // extern "C"
+
+void hg::SetThreadError(const std::string& msg) { g_create_error = msg; }
+
+extern "C" {
 
 const char* hg_last_error(const hg_ctx* ctx) {
   return ctx ? ctx->err.c_str() : g_create_error.c_str();
@@ -1288,6 +1365,12 @@ hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t th
   if (!ctxs || n < 0 || steps < 0 || threads < 0) return HG_ERR_BAD_ARG;
   for (int32_t i = 0; i < n; ++i)
     if (!ctxs[i]) return HG_ERR_BAD_ARG;
+  {
+    // a context listed twice would be stepped by two threads at once
+    std::vector<hg_ctx*> sorted(ctxs, ctxs + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return HG_ERR_BAD_ARG;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<hg_status> rc(size_t(n), HG_OK);
   std::atomic<int32_t> next{0};
@@ -1346,12 +1429,12 @@ hg_status hg_time_kernel(hg_ctx* c, const char* kernel, int32_t reps, double* av
       throw HgError(HG_ERR_BAD_ARG, "hg_time_kernel: unknown kernel " + k);
     }
     launch();
-    HG_CHECK(hipEventRecord(c->ev[0], c->st));
+    HG_CHECK(hipEventRecord(c->kev[0], c->st));
     for (int32_t i = 0; i < reps; ++i) launch();
-    HG_CHECK(hipEventRecord(c->ev[1], c->st));
-    HG_CHECK(hipEventSynchronize(c->ev[1]));
+    HG_CHECK(hipEventRecord(c->kev[1], c->st));
+    HG_CHECK(hipEventSynchronize(c->kev[1]));
     float ms = 0.f;
-    HG_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    HG_CHECK(hipEventElapsedTime(&ms, c->kev[0], c->kev[1]));
     *avg_ms = double(ms) / reps;
   });
 }

@@ -3,9 +3,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "device_common.h"
 
 namespace hg {
+
+// The message hg_last_error(NULL) returns on this thread (context.hip).
+void SetThreadError(const std::string& msg);
+
 
 // Velocity boundary values per side (wall velocity), per component.
 struct Bc {
@@ -139,6 +145,12 @@ void LaunchLuWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k0,
 // info[1] = final max|corr| (as double bits).
 void LaunchGsWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k, double* x,
                        double tol, int limit, double omega, unsigned long long* info);
+// LuDecompositionRelaxed (linear.hpp:578-651) from zero, every iteration in
+// one persistent workgroup: res receives the solution; corr and f are
+// scratch cell arrays; info as for LaunchGsWavefront.
+void LaunchLuRelaxed(hipStream_t st, int nx, int ny, Rows a, const double* k, double* res,
+                     double* corr, double* f, double tol, int limit, double relax,
+                     unsigned long long* info);
 // Jacobi (linear.hpp:750-783), batched sweeps with a device-side stop test.
 struct JacobiWork {
   double* xa;

@@ -105,6 +105,16 @@ hg_status hg_stencil_solve(int32_t nx, int32_t ny, const double* s, const double
         std::memcpy(&diff, &h[1], 8);
         break;
       }
+      case HG_SOLVER_LU_RELAXED: {
+        hg::LaunchLuRelaxed(st, nx, ny, rows, dk.p, dx.p, dx2.p, dtmp.p, opts->tolerance,
+                            opts->max_iters, opts->relaxation_factor, (unsigned long long*)info.p);
+        unsigned long long h[2];
+        HG_CHECK(hipMemcpyAsync(h, info.p, 16, hipMemcpyDeviceToHost, st));
+        HG_CHECK(hipStreamSynchronize(st));
+        iters = int(h[0]);
+        std::memcpy(&diff, &h[1], 8);
+        break;
+      }
       case HG_SOLVER_JACOBI: {
         hg::JacobiWork jw{dx.p, dx2.p, (unsigned long long*)jdiff.p, (int*)jstate.p};
         HG_CHECK(hipMemsetAsync(dx.p, 0, N * 8, st));

@@ -168,6 +168,102 @@ KGsWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k,
   }
 }
 
+// LuDecompositionRelaxed::Solve (linear.hpp:592-648), every iteration in
+// this workgroup.  An iteration is a forward sweep on the correction
+//   corr = (-f - s corr_S - w corr_W) / (c + relax)        (raw order: wavefront)
+// then a backward step that reads the PREVIOUS iterate res, not corr, as the
+// reference does, so it has no recurrence:
+//   corr -= (n res_N + e res_E) / (c + relax)
+// then res += corr with diff = max|corr|, and f = the rows evaluated at res
+// (Expression::Evaluate: constant, then S, W, C, E, N).  f starts as the
+// constants and res at zero.  Stops as do { ... } while (diff > tol &&
+// iter++ < limit).
+__global__ void __launch_bounds__(kMaxThreads)
+KLuRelaxed(int nx, int ny, int cw, Rows a, const double* __restrict__ k,
+           double* __restrict__ res, double* __restrict__ corr, double* __restrict__ f,
+           double tol, int limit, double relax, unsigned long long* __restrict__ info) {
+  extern __shared__ double lds[];  // [2][T] west values, then reduction scratch
+  double* red = lds + 2 * blockDim.x;
+  const int T = blockDim.x;
+  const int t = threadIdx.x;
+  const int c0 = t * cw;
+  const int c1 = min(nx, c0 + cw);
+  const bool active = c0 < nx;
+  const int N = nx * ny;
+  for (int r = t; r < N; r += T) {
+    res[r] = 0.;
+    f[r] = k[r];
+  }
+  __syncthreads();
+  double south[kMaxCw];
+  long iter = 0;
+  double diff = 0.;
+  while (true) {
+    // ---- forward (wavefront over i + j)
+    for (int step = 0; step < ny + T - 1; ++step) {
+      const int j = step - t;
+      const int par = step & 1;
+      if (active && j >= 0 && j < ny) {
+        double west = (t > 0) ? lds[(par ^ 1) * T + t - 1] : 0.;
+        for (int i = c0; i < c1; ++i) {
+          const int r = i + nx * j;
+          double sum = 0;
+          if (j > 0) sum += a.s[r] * south[i - c0];
+          if (i > 0) sum += a.w[r] * west;
+          const double v = (-f[r] - sum) / (a.c[r] + relax);
+          corr[r] = v;
+          south[i - c0] = v;
+          west = v;
+        }
+        lds[par * T + t] = west;
+      }
+      __syncthreads();
+    }
+    // ---- backward, from the previous iterate (no recurrence)
+    for (int r = t; r < N; r += T) {
+      const int i = r % nx, j = r / nx;
+      double sum = 0;
+      if (j < ny - 1) sum += a.n[r] * res[r + nx];
+      if (i < nx - 1) sum += a.e[r] * res[r + 1];
+      corr[r] -= sum / (a.c[r] + relax);
+    }
+    __syncthreads();
+    double dmax = 0.;
+    for (int r = t; r < N; r += T) {
+      const double cr = corr[r];
+      res[r] += cr;
+      const double ac = fabs(cr);
+      dmax = (dmax < ac) ? ac : dmax;  // std::max(diff, |corr|)
+    }
+    __syncthreads();
+    for (int r = t; r < N; r += T) {
+      const int i = r % nx, j = r / nx;
+      double v = k[r];
+      if (j > 0) v += res[r - nx] * a.s[r];
+      if (i > 0) v += res[r - 1] * a.w[r];
+      v += res[r] * a.c[r];
+      if (i < nx - 1) v += res[r + 1] * a.e[r];
+      if (j < ny - 1) v += res[r + nx] * a.n[r];
+      f[r] = v;
+    }
+    // workgroup max of |corr| (its barriers also order f before the next
+    // forward sweep reads it)
+    double m = WaveMax(dmax);
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    m = red[0];
+    for (int q = 1; q < T / 64; ++q) m = fmax(m, red[q]);
+    __syncthreads();
+    diff = m;
+    if (!(diff > tol)) break;
+    if (!(iter++ < limit)) break;
+  }
+  if (t == 0) {
+    info[0] = (unsigned long long)iter;
+    info[1] = (unsigned long long)__double_as_longlong(diff);
+  }
+}
+
 // One Jacobi sweep (linear.hpp:757-776) with a device-side stop test run by
 // the last block to finish; later launches see `done` and return.
 __global__ void __launch_bounds__(256)
@@ -314,6 +410,15 @@ void LaunchGsWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k, 
   int T = ThreadsFor(nx, &cw);
   size_t lds = size_t(2) * T * sizeof(double) + 16 * sizeof(double);
   KGsWavefront<<<1, T, lds, st>>>(nx, ny, cw, a, k, x, tol, limit, omega, info);
+}
+
+void LaunchLuRelaxed(hipStream_t st, int nx, int ny, Rows a, const double* k, double* res,
+                     double* corr, double* f, double tol, int limit, double relax,
+                     unsigned long long* info) {
+  int cw;
+  int T = ThreadsFor(nx, &cw);
+  size_t lds = size_t(2) * T * sizeof(double) + 16 * sizeof(double);
+  KLuRelaxed<<<1, T, lds, st>>>(nx, ny, cw, a, k, res, corr, f, tol, limit, relax, info);
 }
 
 void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k, JacobiWork w,

@@ -817,14 +817,17 @@ KPhase0Init(Geo g, Phases ph, const double* __restrict__ pd1, const double* __re
   pd0[c] = (1. - vs) * ph.rho[0];
 }
 
-// NaN test of FinishStep (fluid.hpp:1164-1166, conv_diff.hpp:255-257).
+// NaN test of FinishStep (fluid.hpp:1164-1166, conv_diff.hpp:255-257):
+// IsNan(v) is !(v * 0 == 0) (solver.hpp:18-20).  Bit 1 of the flag: the
+// first field (pressure); bit 2: one of the other two (velocity).
 __global__ void __launch_bounds__(kBlock)
 KNanCheck(const double* __restrict__ a, const double* __restrict__ b,
           const double* __restrict__ c3, int n, int* __restrict__ flag) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  double v = a[c] * 0. + b[c] * 0. + c3[c] * 0.;
-  if (!(v == 0.)) atomicOr(flag, 1);
+  const bool na = !(a[c] * 0. == 0.);
+  const bool nbc = !(b[c] * 0. + c3[c] * 0. == 0.);
+  if (na || nbc) atomicOr(flag, (na ? 1 : 0) | (nbc ? 2 : 0));
 }
 
 // CalcStat partial sums (hydro2d.hpp:1432-1467) for one phase: per block,

@@ -16,8 +16,7 @@ import time
 def _rt(solver_p="fcg_mg"):
     from hydro_amd import workloads
     from hydro_amd.hydro import Hydro
-    from tests import cases
-    h = Hydro(cases.rt(1024, 2048, solver_p=solver_p, solver_v="lu_rb", precision="mixed"))
+    h = Hydro(workloads.rt(1024, 2048, solver_p=solver_p, solver_v="lu_rb", precision="mixed"))
     workloads.set_rt_state(h)
     return h
 

@@ -111,7 +111,9 @@ typedef struct hg_config {
   double convergence_tolerance;
   int32_t num_iterations_limit;
   int32_t time_second_order;
-  int32_t simpler;                     /* must be 0 */
+  int32_t simpler;                     /* simpler: SIMPLER's second pressure solve per
+                                          SIMPLE iteration (fluid.hpp:1059-1151); row
+                                          solvers only (0) */
   int32_t force_geometric_average;     /* must be 0 */
   double guess_extrapolation;
   double meshvel[2];
@@ -131,7 +133,8 @@ typedef struct hg_config {
   /* heat: a passive temperature advected by the step's flux and diffused
    * (hydro2d.hpp:652-688,1607-1612; heat.hpp:18-94) */
   int32_t heat_enable;                 /* heat_enable (0) */
-  int32_t linear_solver_heat;          /* linear_solver_heat: lu, lu_rb, gauss_seidel (lu) */
+  int32_t linear_solver_heat;          /* linear_solver_heat: lu, lu_rb, gauss_seidel, jacobi,
+                                          lu_relaxed (lu) */
   double heat_box_lb[2], heat_box_rt[2]; /* heat_box_lb/_rt: boundary faces whose centre
                                           lies in this box hold heat_box_temperature; the
                                           others have zero normal derivative ((0,0)) */
@@ -329,7 +332,8 @@ hg_status hg_time_kernel(hg_ctx* ctx, const char* kernel, int32_t reps, double* 
  * those of stepping each context alone (the contexts share nothing).
  * stats[i] (may be NULL) receives the last step's stats of context i;
  * *wall_ms the wall time of the whole batch.  Returns the first error met
- * (by context order); the other contexts still run. */
+ * (by context order); the other contexts still run.  Every context may
+ * appear only once in ctxs (HG_ERR_BAD_ARG otherwise). */
 hg_status hg_batch_run(hg_ctx* const* ctxs, int32_t n, int32_t steps, int32_t threads,
                        hg_step_stats* stats, double* wall_ms);
 

@@ -36,33 +36,47 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle import pyoracle  # noqa: E402
-from tests import cases  # noqa: E402
+from hydro_amd import workloads  # noqa: E402
 
 REFERENCE = "/root/reference"
 GENERAL = os.path.join(REFERENCE, "examples", "general.hydroconf")
 SAMPLE_LOG = os.path.join(REFERENCE, "examples", "cavity", "sample", "exp.log")
 
-DT_AUTO = cases.DT_AUTO
+DT_AUTO = workloads.DT_AUTO
 
 FIXTURES = {
     # name: (case text, steps, dump steps)
-    "ref_cavity16": (cases.cavity(16), 5, (1, 5)),
-    "ref_cavity64": (cases.cavity(64), 20, (20,)),
-    "ref_dam64x32": (cases.dam(64, 32), 5, (1, 5)),
-    "ref_cavity32_jacobi": (cases.cavity(32, solver_p="jacobi", extra="set double lu_relaxed_relaxation_factor 0.9"), 3, (3,)),
-    "ref_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), 6, (1, 6)),
-    "ref_cavity32_heat": (cases.cavity(32, extra=cases.HEAT_BOTTOM), 5, (1, 5)),
-    "ref_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_LEFT), 4, (4,)),
-    "ref_cavity32_simpler": (cases.cavity(32, extra="set bool simpler 1"), 5, (1, 5)),
-    "ref_dam32x16_simpler": (cases.dam(32, 16, extra="set bool simpler 1"), 4, (4,)),
-    "ref_channel32x16": (cases.channel(32, 16), 6, (1, 6)),
-    "ref_channel32x16_jacobi": (cases.channel(32, 16, extra=(
+    "ref_cavity16": (workloads.cavity(16), 5, (1, 5)),
+    "ref_cavity64": (workloads.cavity(64), 20, (20,)),
+    "ref_dam64x32": (workloads.dam(64, 32), 5, (1, 5)),
+    "ref_cavity32_jacobi": (workloads.cavity(32, solver_p="jacobi", extra="set double lu_relaxed_relaxation_factor 0.9"), 3, (3,)),
+    "ref_dam32x16_dtauto": (workloads.dam(32, 16, extra=DT_AUTO), 6, (1, 6)),
+    "ref_cavity32_heat": (workloads.cavity(32, extra=workloads.HEAT_BOTTOM), 5, (1, 5)),
+    "ref_dam32x16_heat": (workloads.dam(32, 16, extra=workloads.HEAT_LEFT), 4, (4,)),
+    "ref_cavity32_simpler": (workloads.cavity(32, extra="set bool simpler 1"), 5, (1, 5)),
+    "ref_dam32x16_simpler": (workloads.dam(32, 16, extra="set bool simpler 1"), 4, (4,)),
+    "ref_channel32x16": (workloads.channel(32, 16), 6, (1, 6)),
+    "ref_channel32x16_jacobi": (workloads.channel(32, 16, extra=(
         "set string linear_solver_pressure jacobi\n"
         "set double lu_relaxed_relaxation_factor 0.9\n")), 4, (4,)),
-    "ref_cavity32_simpler_pinned": (cases.cavity(32, solver_p="jacobi", extra=(
+    "ref_cavity32_simpler_pinned": (workloads.cavity(32, solver_p="jacobi", extra=(
         "set bool simpler 1\nset double lu_relaxed_relaxation_factor 0.9\n"
         "set vect pressure_fixed_point (0.3, 0.6)\nset double pressure_fixed_value 0.1\n")),
         4, (4,)),
+    # LuDecompositionRelaxed (linear.hpp:578-651) for the pressure: at the
+    # default tolerance one pass per solve; at 1e-5 every solve runs to the
+    # 1000-iteration limit (the backward step reads the previous iterate)
+    "ref_cavity32_lu_relaxed": (workloads.cavity(32, solver_p="lu_relaxed"), 3, (1, 3)),
+    "ref_cavity32_lu_relaxed_tight": (workloads.cavity(32, solver_p="lu_relaxed", extra=(
+        "set double lu_relaxed_tolerance 1e-5\n")), 2, (2,)),
+    # lu_relaxed and jacobi as the velocity solver (conv_diff.hpp:245): each
+    # SIMPLE iteration prints the two velocity solves before the pressure's
+    "ref_cavity32_vel_lu_relaxed": (workloads.cavity(32, solver_p="jacobi", solver_v="lu_relaxed", extra=(
+        "set double lu_relaxed_tolerance 1e-6\nset double lu_relaxed_relaxation_factor 0.5\n")),
+        3, (1, 3)),
+    "ref_dam32x16_vel_jacobi": (workloads.dam(32, 16, solver_p="lu_relaxed", solver_v="jacobi", extra=(
+        "set double lu_relaxed_tolerance 1e-5\nset double lu_relaxed_relaxation_factor 0.9\n")),
+        3, (1, 3)),
 }
 
 
@@ -106,8 +120,8 @@ LONG = {
     # C1: lid-driven cavity 64x64, 200 steps; C2: broken dam 256x128, the
     # first 50 steps (later steps of this case need the 1000-sweep limit of
     # the pressure solver on every SIMPLE iteration)
-    "ref_cavity64_200": (cases.cavity(64), 200, (200,)),
-    "ref_dam256x128_50": (cases.dam(256, 128), 50, (50,)),
+    "ref_cavity64_200": (workloads.cavity(64), 200, (200,)),
+    "ref_dam256x128_50": (workloads.dam(256, 128), 50, (50,)),
 }
 
 
@@ -201,12 +215,12 @@ set bool output_divergence 1
 """
 OUTPUT = {
     # name: (case text, output keys, step limit; the run ends at T)
-    "ref_output_dam32x16": (cases.dam(32, 16), DAM_OUTPUT, 100),
+    "ref_output_dam32x16": (workloads.dam(32, 16), DAM_OUTPUT, 100),
     "ref_output_cavity16_meshvel": (
-        cases.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
-    "ref_output_dam32x16_dtauto": (cases.dam(32, 16, extra=DT_AUTO), DTAUTO_OUTPUT, 100),
-    "ref_output_dam32x16_heat": (cases.dam(32, 16, extra=cases.HEAT_BOTTOM), HEAT_OUTPUT, 100),
-    "ref_output_channel32x16": (cases.channel(32, 16), CHANNEL_OUTPUT, 100),
+        workloads.cavity(16, extra="set vect meshvel (0.3, -0.2)"), CAVITY_MESHVEL_OUTPUT, 100),
+    "ref_output_dam32x16_dtauto": (workloads.dam(32, 16, extra=DT_AUTO), DTAUTO_OUTPUT, 100),
+    "ref_output_dam32x16_heat": (workloads.dam(32, 16, extra=workloads.HEAT_BOTTOM), HEAT_OUTPUT, 100),
+    "ref_output_channel32x16": (workloads.channel(32, 16), CHANNEL_OUTPUT, 100),
 }
 
 

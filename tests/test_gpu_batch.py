@@ -6,7 +6,6 @@ import pytest
 
 from hydro_amd import capi, workloads
 from hydro_amd.hydro import Hydro, batch_run
-from tests import cases
 
 pytestmark = pytest.mark.gpu
 
@@ -15,9 +14,9 @@ FIELDS = [capi.HG_FIELD_VELOCITY_X, capi.HG_FIELD_VELOCITY_Y, capi.HG_FIELD_PRES
 
 
 def case_texts():
-    return [cases.cavity(32), cases.dam(64, 32), cases.cavity(48, solver_p="jacobi",
+    return [workloads.cavity(32), workloads.dam(64, 32), workloads.cavity(48, solver_p="jacobi",
                                                                 extra="set double lu_relaxed_relaxation_factor 0.9"),
-            cases.dam(32, 16, solver_p="fcg_mg", solver_v="lu_rb", extra="set string precision_gpu mixed"),
+            workloads.dam(32, 16, solver_p="fcg_mg", solver_v="lu_rb", extra="set string precision_gpu mixed"),
             ] + workloads.c5_cases(count=4, seed=7, sides=(64, 96))
 
 

@@ -6,7 +6,6 @@ import pytest
 
 from hydro_amd import capi, workloads
 from hydro_amd.hydro import Hydro, Poisson
-from tests import cases
 from tests.test_gpu_poisson import apply
 
 pytestmark = pytest.mark.gpu
@@ -69,13 +68,13 @@ def test_learned_correction_is_gated(gpu, problem):
 def test_train_pressure_net_in_simulation(gpu):
     """Training on the simulation's own pressure operator, then stepping on."""
     print("case: rt 64x128 fcg_learned training", flush=True)
-    with Hydro(cases.rt(64, 128, solver_p="fcg_learned", precision="mixed")) as h:
+    with Hydro(workloads.rt(64, 128, solver_p="fcg_learned", precision="mixed")) as h:
         h.step()
         st = h.train_pressure_net(steps=60, rhs_count=1, mg_iters=4)
         assert st.pairs == 4 and st.steps == 60
         s = h.step()
         assert s.simple_iterations >= 1
         assert np.all(np.isfinite(h.get(capi.HG_FIELD_PRESSURE)))
-    with Hydro(cases.rt(64, 128, solver_p="cg", precision="fp64")) as h:
+    with Hydro(workloads.rt(64, 128, solver_p="cg", precision="fp64")) as h:
         with pytest.raises(Exception):
             h.train_pressure_net(steps=1)

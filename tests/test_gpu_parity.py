@@ -12,7 +12,7 @@ import pytest
 from hydro_amd import capi
 from hydro_amd.hydro import Hydro, stencil_solve
 from oracle import pyoracle
-from tests import cases
+from hydro_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -47,14 +47,14 @@ def run_pair(text, steps, check_every=1, tol=1e-10):
 
 def test_cavity32_reference_solvers(gpu):
     print("case: cavity 32x32, gauss_seidel + lu, 5 steps", flush=True)
-    g, o, worst = run_pair(cases.cavity(32), 5)
+    g, o, worst = run_pair(workloads.cavity(32), 5)
     for name, f in FIELDS:
         assert np.array_equal(g.get(f), o.get(f)), name  # bit for bit
 
 
 def test_dam64_two_phase(gpu):
     print("case: dam 64x32, gauss_seidel + lu, 5 steps", flush=True)
-    g, o, worst = run_pair(cases.dam(64, 32), 5)
+    g, o, worst = run_pair(workloads.dam(64, 32), 5)
     for i in range(2):
         f = capi.HG_FIELD_PARTIAL_DENSITY_0 + i
         assert np.array_equal(g.get(f), o.get(f))
@@ -65,7 +65,7 @@ def test_dam32_dt_auto(gpu):
     advection dt with it; same dt and bits as the oracle, which matches the
     reference module (fixture ref_dam32x16_dtauto)."""
     print("case: dam 32x16 dt_auto, 6 steps", flush=True)
-    text = cases.dam(32, 16, extra=cases.DT_AUTO)
+    text = workloads.dam(32, 16, extra=workloads.DT_AUTO)
     g = Hydro(text)
     o = pyoracle.Oracle(capi.parse_config(text))
     for s in range(6):
@@ -87,11 +87,11 @@ def test_heat_matches_oracle(gpu, case):
     has no reference counterpart and is checked against the oracle only."""
     print(f"case: {case}, 4 steps", flush=True)
     if case == "cavity32_heat":
-        text = cases.cavity(32, extra=cases.HEAT_BOTTOM)
+        text = workloads.cavity(32, extra=workloads.HEAT_BOTTOM)
     elif case == "dam32x16_heat":
-        text = cases.dam(32, 16, extra=cases.HEAT_LEFT)
+        text = workloads.dam(32, 16, extra=workloads.HEAT_LEFT)
     else:
-        text = cases.dam(32, 16, extra=cases.HEAT_BOTTOM + "set string linear_solver_heat lu_rb\n")
+        text = workloads.dam(32, 16, extra=workloads.HEAT_BOTTOM + "set string linear_solver_heat lu_rb\n")
     g, o, worst = run_pair(text, 4)
     t = g.get(capi.HG_FIELD_TEMPERATURE)
     assert np.array_equal(t, o.get(capi.HG_FIELD_TEMPERATURE))
@@ -108,11 +108,11 @@ def test_simpler_matches_oracle(gpu, case):
     the Jacobi case with a fixed pressure cell included)."""
     print(f"case: simpler {case}, 4 steps", flush=True)
     if case == "cavity32":
-        text = cases.cavity(32, extra="set bool simpler 1")
+        text = workloads.cavity(32, extra="set bool simpler 1")
     elif case == "dam32x16":
-        text = cases.dam(32, 16, extra="set bool simpler 1")
+        text = workloads.dam(32, 16, extra="set bool simpler 1")
     else:
-        text = cases.cavity(32, solver_p="jacobi", extra=(
+        text = workloads.cavity(32, solver_p="jacobi", extra=(
             "set bool simpler 1\nset double lu_relaxed_relaxation_factor 0.9\n"
             "set vect pressure_fixed_point (0.3, 0.6)\nset double pressure_fixed_value 0.1\n"))
     g, o, worst = run_pair(text, 4)
@@ -129,13 +129,13 @@ def test_channel_inlet_outlet(gpu, solver):
     the fields agree with the oracle's CG to the solver tolerance."""
     print(f"case: channel 32x16 inlet/outlet, {solver}, 6 steps", flush=True)
     extra = "set double lu_relaxed_relaxation_factor 0.9\n" if solver == "jacobi" else ""
-    text = cases.channel(32, 16, extra=extra + f"set string linear_solver_pressure {solver}\n")
+    text = workloads.channel(32, 16, extra=extra + f"set string linear_solver_pressure {solver}\n")
     if solver == "fcg_mg":
         # against the oracle's CG to the same tight tolerance
         tight = "set double pressure_tolerance_gpu 1e-12\n"
         g = Hydro(text + tight)
         o = pyoracle.Oracle(capi.parse_config(
-            cases.channel(32, 16, extra="set string linear_solver_pressure cg\n" + tight)))
+            workloads.channel(32, 16, extra="set string linear_solver_pressure cg\n" + tight)))
         for _ in range(6):
             g.step()
             o.step()
@@ -159,10 +159,10 @@ def test_channel_inlet_outlet(gpu, solver):
 def test_simpler_refuses_face_solvers(gpu):
     from hydro_amd.hydro import Unsupported
     with pytest.raises(Unsupported, match="simpler"):
-        Hydro(cases.cavity(16, solver_p="cg", extra="set bool simpler 1"))
+        Hydro(workloads.cavity(16, solver_p="cg", extra="set bool simpler 1"))
 
 
-@pytest.mark.parametrize("solver", ["lu", "gauss_seidel", "jacobi", "lu_rb"])
+@pytest.mark.parametrize("solver", ["lu", "gauss_seidel", "jacobi", "lu_rb", "lu_relaxed"])
 def test_stencil_solvers_bitwise(gpu, solver):
     print(f"case: stencil solve {solver} 48x40", flush=True)
     rng = np.random.default_rng(7)
@@ -178,7 +178,7 @@ def test_stencil_solvers_bitwise(gpu, solver):
     c = -(s + w + e + nn) + rng.uniform(0.1, 0.5, n)
     k = rng.normal(size=n)
     sid = capi.SOLVERS[solver]
-    omega = 0.8 if solver == "jacobi" else 1.5
+    omega = {"jacobi": 0.8, "lu_relaxed": 0.5}.get(solver, 1.5)
     xg, sg = stencil_solve(nx, ny, s, w, c, e, nn, k, sid, tol=1e-6, max_iters=300, omega=omega)
     xo, so = pyoracle.stencil_solve(nx, ny, s, w, c, e, nn, k, sid, tol=1e-6, max_iters=300,
                                     omega=omega)
@@ -207,3 +207,50 @@ def test_long_trajectory_matches_reference(gpu, name):
             digests[s] = field_digests(g.get, g.cfg.num_phases)
     g.close()
     check_long_fixture(fix, digests, stats)
+
+
+@pytest.mark.parametrize("name", ["ref_cavity32_lu_relaxed", "ref_cavity32_lu_relaxed_tight",
+                                  "ref_cavity32_vel_lu_relaxed", "ref_dam32x16_vel_jacobi"])
+def test_reference_solver_fixture(gpu, name):
+    """lu_relaxed (linear.hpp:578-651) as the pressure and the velocity
+    solver, jacobi as the velocity solver: the GPU against the reference
+    module's own dumps, bit for bit, with the same SIMPLE iterations and the
+    same printed iteration count of every solve (velocity components, then
+    the pressure, per SIMPLE iteration)."""
+    import os
+    from tests.test_oracle_reference import GOLDEN, PRINTING
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    text, steps = str(z["text"]), int(z["steps"])
+    print(f"case: {name}, {steps} steps", flush=True)
+    vp = 2 if capi.parse_config(text).linear_solver_velocity in PRINTING else 0
+    sweeps = list(z["pressure_sweeps"])
+    # the first step's solves one by one
+    with Hydro(text) as g:
+        g.start_step()
+        q = 0
+        while not g.is_converged():
+            st = g.iterate()
+            got = ([st.velocity_solve_iters[0], st.velocity_solve_iters[1]] if vp else []) + \
+                [st.pressure_solve_iters]
+            assert got == sweeps[q:q + vp + 1], (q, got)
+            q += vp + 1
+        assert q == (vp + 1) * int(z["simple_iters"][0])
+    # every step, fields at the dumped steps
+    dumps = sorted({int(k[1:k.index("_")]) for k in z.files
+                    if k.startswith("s") and k[1:2].isdigit()})
+    pos = 0
+    with Hydro(text) as g:
+        for s in range(1, steps + 1):
+            st = g.step()
+            n = st.simple_iterations
+            assert n == int(z["simple_iters"][s - 1]), s
+            chunk = sweeps[pos:pos + n * (vp + 1)]
+            assert st.pressure_solve_iters_total == sum(chunk[vp::vp + 1]), s
+            pos += n * (vp + 1)
+            if s in dumps:
+                for fname, f in FIELDS:
+                    assert np.array_equal(g.get(f), z[f"s{s}_{fname}"]), (s, fname)
+                for i in range(g.cfg.num_phases):
+                    assert np.array_equal(g.get(capi.HG_FIELD_VOLUME_FRACTION_0 + i),
+                                          z[f"s{s}_alpha{i}"]), (s, i)
+    assert pos == len(sweeps)

@@ -20,7 +20,7 @@ import pytest
 
 from hydro_amd import capi
 from hydro_amd.hydro import Hydro, HydroNaN
-from tests import cases
+from hydro_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ FIELDS = (capi.HG_FIELD_VELOCITY_X, capi.HG_FIELD_VELOCITY_Y, capi.HG_FIELD_PRES
 
 
 def test_step_readbacks_match_fine_grained_steps(gpu):
-    text = cases.dam(64, 32, dt=1e-4, solver_p="fcg_mg", solver_v="lu_rb",
+    text = workloads.dam(64, 32, dt=1e-4, solver_p="fcg_mg", solver_v="lu_rb",
                      extra="set string precision_gpu mixed\n")
     print("case: dam 64x32, 4 steps, hg_step vs fine-grained entry points", flush=True)
     with Hydro(text) as a, Hydro(text) as b:
@@ -51,7 +51,7 @@ def test_step_readbacks_match_fine_grained_steps(gpu):
 
 def test_nan_raises_and_the_flag_clears(gpu):
     print("case: cavity 16x16 with a NaN pressure cell", flush=True)
-    with Hydro(cases.cavity(16)) as h:
+    with Hydro(workloads.cavity(16)) as h:
         good = h.get(capi.HG_FIELD_PRESSURE)
         bad = good.copy()
         bad[37] = math.nan
@@ -62,3 +62,38 @@ def test_nan_raises_and_the_flag_clears(gpu):
         st = h.step()
         assert st.simple_iterations >= 1
         assert np.isfinite(h.get(capi.HG_FIELD_PRESSURE)).all()
+
+
+def test_nan_velocity_is_named_and_time_does_not_advance(gpu):
+    """A NaN velocity is "NaN initial field" (the velocity solver's
+    StartStep, conv_diff.hpp:119-121), after the pressure passed; a failed
+    step leaves the time and the step count where they were (the checks
+    precede IncTime, fluid.hpp:1164-1168)."""
+    print("case: cavity 16x16 with a NaN velocity cell", flush=True)
+    with Hydro(workloads.cavity(16)) as h:
+        h.step()
+        before = h.progress()
+        good = h.get(capi.HG_FIELD_VELOCITY_X)
+        bad = good.copy()
+        bad[11] = math.nan
+        h.set(capi.HG_FIELD_VELOCITY_X, bad)
+        with pytest.raises(HydroNaN, match="NaN initial field"):
+            h.step()
+        after = h.progress()
+        assert (after.time, after.steps) == (before.time, before.steps)
+        h.set(capi.HG_FIELD_VELOCITY_X, good)
+        h.step()
+        assert h.progress().steps == before.steps + 1
+
+
+def test_time_kernel_leaves_step_timing_intact(gpu):
+    """hg_time_kernel times on events of its own: the step regions that
+    hg_last_step_timing reports afterwards are still the last step's."""
+    print("case: dam 32x16, hg_time_kernel between hg_step and its timing", flush=True)
+    with Hydro(workloads.dam(32, 16)) as h:
+        h.step()
+        t0 = h.timing()
+        h.time_kernel("KAdvStep", reps=3)
+        t1 = h.timing()
+        assert t0 == t1
+        assert all(v > 0 for v in t1.values())

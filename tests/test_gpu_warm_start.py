@@ -9,7 +9,6 @@ import pytest
 
 from hydro_amd import workloads
 from hydro_amd.hydro import Hydro
-from tests import cases
 
 pytestmark = pytest.mark.gpu
 
@@ -21,7 +20,7 @@ def run_steps(warm, n=8, substeps=10):
     residuals of the solves."""
     extra = f"set int pressure_warm_start_gpu {1 if warm else 0}"
     its, res = [], []
-    with Hydro(cases.rt(128, 256, solver_p="fcg_mg", precision="mixed", extra=extra)) as h:
+    with Hydro(workloads.rt(128, 256, solver_p="fcg_mg", precision="mixed", extra=extra)) as h:
         workloads.set_rt_state(h)
         for _ in range(n):
             h.start_step()

@@ -10,7 +10,7 @@ import pytest
 
 from hydro_amd import capi
 from oracle import pyoracle
-from tests import cases
+from hydro_amd import workloads
 
 
 def test_bdf_coefficients_closed_form():
@@ -22,7 +22,7 @@ def test_bdf_coefficients_closed_form():
 
 
 def test_uniform_geometry():
-    o = pyoracle.Oracle(capi.parse_config(cases.dam(16, 8)))  # [0,2]x[0,1], h = 1/8
+    o = pyoracle.Oracle(capi.parse_config(workloads.dam(16, 8)))  # [0,2]x[0,1], h = 1/8
     (cx, cy, vol), (fx, fy, fa, sx, sy) = o.geometry()
     h = 1.0 / 8
     np.testing.assert_array_equal(vol, np.full(16 * 8, h * h))
@@ -164,7 +164,7 @@ def test_poisson_cg_singular_neumann():
 
 
 def test_pressure_system_symmetric_zero_row_sums():
-    o = pyoracle.Oracle(capi.parse_config(cases.dam(16, 8)))
+    o = pyoracle.Oracle(capi.parse_config(workloads.dam(16, 8)))
     o.start_step()
     o.iterate()
     s, w, c, e, nn, k, _ = o.system(1)
@@ -177,7 +177,7 @@ def test_pressure_system_symmetric_zero_row_sums():
 
 
 def test_quiescent_fluid_stays_at_rest():
-    o = pyoracle.Oracle(capi.parse_config(cases.cavity(16).replace('"wall 1 0"', '"wall 0 0"')))
+    o = pyoracle.Oracle(capi.parse_config(workloads.cavity(16).replace('"wall 1 0"', '"wall 0 0"')))
     for _ in range(3):
         o.step()
     assert np.all(o.get(capi.HG_FIELD_VELOCITY_X) == 0)
@@ -185,7 +185,7 @@ def test_quiescent_fluid_stays_at_rest():
 
 
 def test_advection_conserves_volume_and_fractions_sum_to_one():
-    o = pyoracle.Oracle(capi.parse_config(cases.dam(32, 16)))
+    o = pyoracle.Oracle(capi.parse_config(workloads.dam(32, 16)))
     v0 = o.stats().volume[1]
     for _ in range(5):
         o.step()
@@ -197,7 +197,7 @@ def test_advection_conserves_volume_and_fractions_sum_to_one():
 
 
 def test_corrected_flux_is_divergence_free_when_solved_tightly():
-    text = cases.cavity(16, solver_p="cg", extra="set double pressure_tolerance_gpu 1e-13")
+    text = workloads.cavity(16, solver_p="cg", extra="set double pressure_tolerance_gpu 1e-13")
     o = pyoracle.Oracle(capi.parse_config(text))
     o.start_step()
     o.iterate()

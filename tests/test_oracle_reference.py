@@ -22,7 +22,12 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 NAMES = ["ref_cavity16", "ref_cavity64", "ref_dam64x32", "ref_cavity32_jacobi",
          "ref_dam32x16_dtauto", "ref_cavity32_heat", "ref_dam32x16_heat",
          "ref_cavity32_simpler", "ref_dam32x16_simpler", "ref_cavity32_simpler_pinned",
-         "ref_channel32x16", "ref_channel32x16_jacobi"]
+         "ref_channel32x16", "ref_channel32x16_jacobi", "ref_cavity32_lu_relaxed",
+         "ref_cavity32_lu_relaxed_tight", "ref_cavity32_vel_lu_relaxed",
+         "ref_dam32x16_vel_jacobi"]
+
+# Solvers that print "iter = , diff = " after each solve (linear.hpp:647,712,779)
+PRINTING = {capi.SOLVERS[n] for n in ("gauss_seidel", "jacobi", "lu_relaxed")}
 
 FIELDS = [("u", capi.HG_FIELD_VELOCITY_X), ("v", capi.HG_FIELD_VELOCITY_Y),
           ("p", capi.HG_FIELD_PRESSURE), ("flux", capi.HG_FIELD_VOLUME_FLUX)]
@@ -38,14 +43,19 @@ def test_oracle_matches_reference_fixture(name):
                     if k.startswith("s") and k[1:2].isdigit()})
     ref_iters = list(z["simple_iters"])
     ref_sweeps = list(z["pressure_sweeps"])
+    # per SIMPLE iteration the velocity solves (two components) print before
+    # the pressure solve(s); SIMPLER solves the pressure twice
+    vp = 2 if o.cfg.linear_solver_velocity in PRINTING else 0
+    npr = 2 if o.cfg.simpler else 1
     pos = 0
     for s in range(1, steps + 1):
         st = o.step()
         assert st.simple_iterations == ref_iters[s - 1], (name, s)
-        # SIMPLER solves the pressure twice per SIMPLE iteration
-        n = st.simple_iterations * (2 if o.cfg.simpler else 1)
-        assert st.pressure_solve_iters_total == sum(ref_sweeps[pos:pos + n]), (name, s)
-        pos += n
+        n = st.simple_iterations
+        chunk = ref_sweeps[pos:pos + n * (vp + npr)]
+        pressure = [chunk[q * (vp + npr) + vp + r] for q in range(n) for r in range(npr)]
+        assert st.pressure_solve_iters_total == sum(pressure), (name, s)
+        pos += n * (vp + npr)
         if o.cfg.heat_enable and o.cfg.linear_solver_heat != capi.SOLVERS["lu"]:
             pos += 1  # the heat solve's own sweep count follows the step's
         if s in dumps:
@@ -59,6 +69,23 @@ def test_oracle_matches_reference_fixture(name):
             if o.cfg.heat_enable:
                 assert np.array_equal(o.get(capi.HG_FIELD_TEMPERATURE), z[f"s{s}_T"]), (name, s)
     assert pos == len(ref_sweeps)
+
+
+@pytest.mark.parametrize("name", ["ref_cavity32_vel_lu_relaxed", "ref_dam32x16_vel_jacobi"])
+def test_oracle_velocity_sweeps_match_reference(name):
+    """The velocity solves' own sweep counts (the two prints before each
+    pressure solve) over the first step's SIMPLE iterations."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    o = pyoracle.Oracle(capi.parse_config(str(z["text"])))
+    sweeps = list(z["pressure_sweeps"])
+    o.start_step()
+    q = 0
+    while not o.is_converged():
+        st = o.iterate()
+        assert [st.velocity_solve_iters[0], st.velocity_solve_iters[1],
+                st.pressure_solve_iters] == sweeps[q:q + 3], (name, q)
+        q += 3
+    assert q == 3 * int(z["simple_iters"][0])
 
 
 def test_oracle_reproduces_cavity_sample_log(tmp_path):
