@@ -56,9 +56,10 @@ MEASURED_TRAFFIC = {"KUp<double, true>": 82.79e6, "KDown<double>": 58.70e6,
 
 def run_path(text, steps, warmup, label, train=False):
     from hydro_amd.hydro import Hydro
-    from hydro_amd import workloads
+    from hydro_amd import compare, workloads
     h = Hydro(text)
     workloads.set_rt_state(h)
+    h.volumes0 = compare.phase_volumes(h)
     for i in range(warmup):
         st = h.step()
         log(f"{label} warmup step {i + 1}: SIMPLE {st.simple_iterations}, "
@@ -92,6 +93,60 @@ def run_path(text, steps, warmup, label, train=False):
     return h, dt, iters, train_stats
 
 
+def cpu_info():
+    """CPU model and socket count of this host (for the baselines)."""
+    model, sockets = None, set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"cpu_model": model, "sockets": max(1, len(sockets)),
+            "threads_used": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+
+
+def cpu_pressure_solves(nx, ny, kx, ky, rhs, sor_sweeps=600, reps=3):
+    """The pressure solve to 1e-8 on the CPU, same system, zero start, same
+    stopping test (oracle/hydro_oracle.cpp or_poisson_baseline, -O2 -fopenmp):
+    OpenMP Jacobi-PCG to the end (median of `reps`), and the reference's
+    serial lexicographic SOR (omega 1.9) as a bounded sample of `sor_sweeps`
+    sweeps, with the sweeps to 1e-8 extrapolated from its residual decay
+    between the sample's midpoint and its end."""
+    from oracle import pyoracle
+    nfx = (nx + 1) * ny
+    out = {}
+    secs, its, rel = [], None, None
+    for _ in range(reps):
+        _, sec, its, rel = pyoracle.poisson_baseline(nx, ny, kx[:nfx], kx[nfx:] if ky is None
+                                                     else ky, rhs, "pcg", tol=1e-8)
+        secs.append(sec)
+        if sec > 10.0:  # keep the CPU leg bounded: one run of a long solve
+            break
+    out["omp_jacobi_pcg"] = {"ms": round(1e3 * float(np.median(secs)), 1), "iterations": its,
+                             "rel_residual": rel, "runs": len(secs)}
+    half = sor_sweeps // 2
+    _, sec_h, _, rel_h = pyoracle.poisson_baseline(nx, ny, kx[:nfx], kx[nfx:] if ky is None
+                                                   else ky, rhs, "sor", tol=1e-8,
+                                                   max_iters=half, check_every=half)
+    _, sec, it, rel = pyoracle.poisson_baseline(nx, ny, kx[:nfx], kx[nfx:] if ky is None else ky,
+                                                rhs, "sor", tol=1e-8, max_iters=sor_sweeps,
+                                                check_every=10)
+    ms_sweep = 1e3 * sec / max(it, 1)
+    est = None
+    if rel > 1e-8 and 0 < rel < rel_h:
+        per_sweep = (np.log(rel) - np.log(rel_h)) / (it - half)
+        est = int(it + (np.log(1e-8) - np.log(rel)) / per_sweep)
+    out["serial_sor_sample"] = {
+        "sweeps": it, "ms": round(1e3 * sec, 1), "ms_per_sweep": round(ms_sweep, 3),
+        "rel_residual": rel, "rel_residual_at_half": rel_h,
+        "sweeps_to_1e-8_extrapolated": est,
+        "ms_to_1e-8_extrapolated": round(est * ms_sweep, 0) if est else None}
+    return out
+
+
 def cpu_baseline(seconds_cap=150):
     """One step of the reference module (its default solvers, OpenMP) on the
     same 1024x2048 grid with the RT densities; flat interface at y = 1
@@ -123,10 +178,13 @@ set bool no_output 1
         return None
     t = float(m.group(2))
     sweeps = [int(x) for x in re.findall(r"iter = (\d+), diff", r.stdout)]
+    info = cpu_info()
     return {"value": round(1.0 / t, 5), "unit": "steps/s", "cores": cores, "kind": "reference",
-            "sample": f"1 step of the reference module (gauss_seidel + lu, OpenMP) on 1024x2048 "
-                      f"two-phase RT-like state: {m.group(1)} SIMPLE iterations, "
-                      f"SOR sweeps {sweeps}, {t:.2f} s"}
+            "cpu_model": info["cpu_model"], "sockets": info["sockets"],
+            "sample": f"1 step of the reference module (its default gauss_seidel + lu, OpenMP, "
+                      f"-O3) on the 1024x2048 two-phase RT densities with a flat interface: "
+                      f"{m.group(1)} SIMPLE iterations, SOR sweeps {sweeps} (the reference's "
+                      f"SOR stops at max|update| <= 1e-3, not at a 1e-8 residual), {t:.2f} s"}
 
 
 def net_throughput(nx, ny):
@@ -186,14 +244,25 @@ def main():
     text64 = workloads.rt(nx, ny, solver_p="cg", solver_v="lu_rb", precision="fp64")
     textmx = workloads.rt(nx, ny, solver_p="fcg_learned", solver_v="lu_rb", precision="mixed")
 
+    from hydro_amd import compare
     h64, t64, it64, _ = run_path(text64, args.steps, args.warmup, "fp64")
     # the SIMPLE pressure system of the last step, for the solve comparison
     kf = h64.get(capi.HG_FIELD_PRESSURE_FACE_COEF)
     rhs = h64.get(capi.HG_FIELD_PRESSURE_RHS)
+    snap64, vol64 = compare.snapshot(h64), (h64.volumes0, compare.phase_volumes(h64))
     h64.close()
     hmx, tmx, itmx, train = run_path(textmx, args.steps, args.warmup, "headline", train=True)
     step_kernel_ms = hmx.kernel_ms
     substeps = 10
+    # the headline path's fields against the FP64 path's after the same
+    # warmup + steps from the same state
+    field_err = compare.field_errors(compare.snapshot(hmx), snap64)
+    field_err = {k: float(f"{v:.3e}") for k, v in field_err.items()}
+    field_err["volume_drift_headline"] = [float(f"{d:.2e}") for d in compare.volume_drift(
+        hmx.volumes0, compare.phase_volumes(hmx))]
+    field_err["volume_drift_fp64"] = [float(f"{d:.2e}") for d in compare.volume_drift(*vol64)]
+    field_err["after_steps"] = args.warmup + args.steps
+    log(f"field error of the headline path vs the fp64 path: {field_err}")
     hmx.close()
 
     nfx = (nx + 1) * ny
@@ -210,6 +279,37 @@ def main():
         solves[label] = {"ms": round(float(np.median(ms)), 3), "iterations": int(its),
                          "rel_residual": float(res)}
         log(f"pressure solve {label}: {solves[label]}")
+
+    cpu_solves = None
+    if not args.no_cpu:
+        cpu_solves = cpu_pressure_solves(nx, ny, kf, None, rhs)
+        cpu_solves.update(cpu_info())
+        log(f"CPU pressure solves (RT system): {cpu_solves}")
+
+    # C4: the 512x512 variable-coefficient Poisson solve (density ratio 1000)
+    kx4, ky4, rhs4 = workloads.poisson_problem(512, seed=0)
+    ps4 = Poisson(512, 512, kx4, ky4)
+    c4 = {}
+    for label, solver, prec in (("fp64_jacobi_pcg", "cg", 0), ("mixed_fcg_vcycle", "fcg_mg", 1)):
+        ms = []
+        for rep_ in range(4):
+            _, st = ps4.solve(rhs4, solver=solver, tol=1e-8, precision=prec)
+            if rep_ > 0:
+                ms.append(st.ms)
+        c4[label] = {"ms": round(float(np.median(ms)), 3), "iterations": int(st.iterations),
+                     "rel_residual": float(st.rel_residual)}
+    ps4.close()
+    c4["speedup_mixed_fcg_vs_fp64"] = round(c4["fp64_jacobi_pcg"]["ms"] /
+                                            c4["mixed_fcg_vcycle"]["ms"], 2)
+    if not args.no_cpu:
+        from oracle import pyoracle
+        secs = []
+        for _ in range(3):
+            _, sec, its, rel = pyoracle.poisson_baseline(512, 512, kx4, ky4, rhs4, "pcg", tol=1e-8)
+            secs.append(sec)
+        c4["cpu_omp_jacobi_pcg"] = {"ms": round(1e3 * float(np.median(secs)), 1),
+                                    "iterations": its, "rel_residual": rel}
+    log(f"C4 Poisson 512x512: {c4}")
 
     # kernel timings on the RT operator (the V-cycle's finest level and its
     # one-workgroup coarse cycle), HIP events around back-to-back launches
@@ -229,7 +329,7 @@ def main():
         "frac": round(cg_gbs / HBM_PEAK_GBS, 4)}
     log(f"kernels: {kernels}")
 
-    for name, man in (("KAdvStep", "KAdvStep<2, false>"), ("KMomentum", "KMomentum")):
+    for name, man in (("KAdvStep", "KAdvStep<2, false> mixed"), ("KMomentum", "KMomentum mixed")):
         ms = step_kernel_ms[name]
         gbs = manifest.kernel_bytes(man, nx, ny) / (ms * 1e-3) / 1e9
         kernels[man] = {"avg_us": round(ms * 1e3, 2), "gbs": round(gbs, 1),
@@ -237,7 +337,7 @@ def main():
     # the dominant kernel of the headline step: launches per step x duration
     simple_per_step = float(np.mean([s_ for s_, _ in itmx]))
     fcg_per_step = float(np.mean([p for _, p in itmx]))
-    per_step = {"KAdvStep<2, false>": substeps, "KMomentum": simple_per_step,
+    per_step = {"KAdvStep<2, false> mixed": substeps, "KMomentum mixed": simple_per_step,
                 "KUp<double, true>": fcg_per_step, "KDown<double>": fcg_per_step}
     dom = max(per_step, key=lambda k: per_step[k] * kernels[k]["avg_us"])
     achieved = kernels[dom]["gbs"]
@@ -281,14 +381,28 @@ def main():
         "config": {"workload": "Rayleigh-Taylor 1024x2048, Atwood 0.5, cos amplitude 0.01 + "
                                "1e-4 seeded noise; lu_rb velocity, pressure to 1e-8",
                    "grid": [nx, ny],
-                   "path": "fcg_learned: FP64 flexible CG, FP32 multigrid V-cycle "
-                           "(+ learned correction when accepted)"},
+                   "path": ("fcg_learned, mixed precision: FP64 flexible CG preconditioned by "
+                            "the FP32 multigrid V-cycle " +
+                            ("plus the accepted learned correction" if train and
+                             train.get("accepted") else
+                             "alone (the learned correction was trained and rejected by its "
+                             "held-out acceptance solve)") +
+                            "; FP32 face fields, gradients, momentum rows, velocity sweep and "
+                            "advection limiter; FP64 state, residuals and corrections"),
+                   "timed": f"steps {args.warmup + 1}..{args.warmup + args.steps} of RT "
+                            f"started from rest at dt 2e-4 (t <= "
+                            f"{2e-4 * (args.warmup + args.steps):.4f}), where one SIMPLE "
+                            f"iteration converges per step; clock stopped after the last "
+                            f"step's kernels finished"},
         "steps_per_s_fp64": round(steps_64, 4),
         "ms_per_step_fp64": round(1e3 * t64 / args.steps, 3),
         "pressure_iters_per_step": {"headline": [p for _, p in itmx], "fp64": [p for _, p in it64]},
         "learned_correction": train,
         "pressure_solve_ms": dict(solves, speedup_mixed_fcg_vs_fp64=round(
-            solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2)),
+            solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2),
+            cpu=cpu_solves),
+        "poisson_c4_512": c4,
+        "field_error_vs_fp64": field_err,
         "step_roofline": step_roofline,
         "kernels": kernels,
         "batch_c5": batch,

@@ -73,7 +73,9 @@ def build_reference():
     """Compiles the reference module (oracle/ref.mk) when its sources exist."""
     if not os.path.isdir(os.path.join(REFERENCE, "source")):
         return None
-    _run(["make", "-s", "-f", "oracle/ref.mk", "-j4", f"REF={REFERENCE}"])
+    _run(["make", "-s", "-f", "oracle/ref.mk", "-j4", f"REF={REFERENCE}", "all"])
+    if os.path.exists(LIB):
+        _run(["make", "-s", "-f", "oracle/ref.mk", f"REF={REFERENCE}", "gpu"])
     return os.path.join(ORACLE_DIR, "_ref")
 
 

@@ -8,7 +8,7 @@ lives in the HIP kernels behind the C-ABI.
 import ctypes
 import os
 
-HG_ABI_VERSION = 3
+HG_ABI_VERSION = 4
 HG_MAX_PHASES = 3
 
 HG_OK, HG_ERR_NAN, HG_ERR_NOT_CONVERGED, HG_ERR_BAD_ARG, HG_ERR_HIP, \
@@ -104,6 +104,7 @@ class HgConfig(ctypes.Structure):
         ("learned_levels_gpu", ctypes.c_int32),
         ("learned_channels_gpu", ctypes.c_int32),
         ("pressure_warm_start_gpu", ctypes.c_int32),
+        ("fcg_negate_precond_gpu", ctypes.c_int32),
         ("meshvel_output", ctypes.c_int32),
         ("unsupported_flags", ctypes.c_int32),
     ]

@@ -176,6 +176,7 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "learned_levels_gpu") c->learned_levels_gpu = iv;
       else if (name == "learned_channels_gpu") c->learned_channels_gpu = iv;
       else if (name == "pressure_warm_start_gpu") c->pressure_warm_start_gpu = iv;
+      else if (name == "fcg_negate_precond_gpu") c->fcg_negate_precond_gpu = iv;
     } else if (type == "double") {
       double v;
       if (!(in >> v)) return fail(line_no, "bad double for " + name);
@@ -224,6 +225,7 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       else if (name == "advection_enable") c->advection_enable = v;
       else if (name == "time_second_order") c->time_second_order = v;
       else if (name == "simpler") c->simpler = v;
+      else if (name == "fcg_negate_precond_gpu") c->fcg_negate_precond_gpu = v;
       else if (name == "force_geometric_average") c->force_geometric_average = v;
       else if (name == "tvd_split") c->tvd_split = v;
       else if (name == "meshvel_output") c->meshvel_output = v;

@@ -141,6 +141,27 @@ __global__ void KAddTo(long n, const double* __restrict__ base, const double* __
 
 inline dim3 G(long n) { return dim3(unsigned((n + 255) / 256)); }
 
+__global__ void KNegateF(long n, float* __restrict__ z) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) z[i] = -z[i];
+}
+
+// The check knob fcg_negate_precond_gpu: minus the wrapped preconditioner,
+// applied without the fused FCG kernels, so that FCG's descent check fails
+// on its first iteration and the solve takes the Jacobi-PCG fallback.
+class NegatedPrecond : public hg::Precond {
+ public:
+  NegatedPrecond(hg::Precond* inner, long n) : inner_(inner), n_(n) {}
+  void Apply(const double* r, float* z, hipStream_t st, const int* d_done) override {
+    inner_->Apply(r, z, st, d_done);
+    KNegateF<<<G(n_), 256, 0, st>>>(n_, z);
+  }
+
+ private:
+  hg::Precond* inner_;
+  long n_;
+};
+
 }  // namespace
 
 struct hg_ctx {
@@ -214,9 +235,22 @@ struct hg_ctx {
   double *bflux_acc = nullptr, *bflux_part = nullptr;
   double meshpos[2] = {0., 0.}, meshpos_stat[2] = {0., 0.};
   std::string err;
+  // The mixed path (precision_gpu mixed): the per-step face fields, the cell
+  // gradients, the momentum rows and the velocity correction in FP32, and an
+  // FP32 copy of the corrected flux for the advection; the state, the row
+  // constants' arithmetic, the pressure system and the corrections stay FP64.
+  bool mixed = false;
+  struct F32 {
+    float *ffex, *ffey, *ffmu, *frx, *fry, *ffrx, *ffry;
+    float *gpx, *gpy, *g0x, *g0y, *g1x, *g1y;
+    float *s, *w, *c, *e, *n, *k0, *k1, *xb0, *xb1;
+    float* flux;
+  } f32{};
+  bool flux32_valid = false;  // f32.flux holds float(fl_ic)
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
   std::unique_ptr<hg::MgPrecond> mg;
+  std::unique_ptr<hg::Precond> negated;  // fcg_negate_precond_gpu
   bool pressure_system = false;  // cg/mg hold the operator of a pressure solve
   hipEvent_t ev[4];
   bool timed = false;  // ev[0..3] hold the regions of the last hg_step
@@ -228,6 +262,13 @@ struct hg_ctx {
     HG_CHECK(hipMemsetAsync(p, 0, size_t(std::max<long>(n, 1)) * sizeof(double), st));
     allocs.push_back(p);
     return (double*)p;
+  }
+  float* AllocF(long n) {
+    void* p = nullptr;
+    HG_CHECK(hipMalloc(&p, size_t(std::max<long>(n, 1)) * sizeof(float)));
+    HG_CHECK(hipMemsetAsync(p, 0, size_t(std::max<long>(n, 1)) * sizeof(float), st));
+    allocs.push_back(p);
+    return (float*)p;
   }
   ~hg_ctx() {
     if (st) (void)hipStreamSynchronize(st);
@@ -562,7 +603,15 @@ void StartStep(hg_ctx* c) {
                  {c->u_ic[0], c->u_ic[1], c->p_ic, c->fl_ic}};
   KExtrapolate<<<G(c->nc + c->nf), 256, 0, c->st>>>(c->nc, c->nf, l,
                                                      c->cfg.guess_extrapolation);
+  c->flux32_valid = false;  // iter_curr's flux is the new guess
   // inputs that stay fixed through the SIMPLE loop (fluid.hpp:602-641)
+  if (c->mixed) {
+    auto& f = c->f32;
+    hg::LaunchForceFaces(c->st, c->geo, c->fx, c->fy, c->mu_s, f.ffex, f.ffey, f.ffmu);
+    hg::LaunchRestoredForce(c->st, c->geo, f.ffex, f.ffey, f.frx, f.fry);
+    hg::LaunchFacesDeriv0Pair(c->st, c->geo, f.frx, f.fry, f.ffrx, f.ffry);
+    return;
+  }
   hg::LaunchForceFaces(c->st, c->geo, c->fx, c->fy, c->mu_s, c->ffex, c->ffey, c->ffmu);
   hg::LaunchRestoredForce(c->st, c->geo, c->ffex, c->ffey, c->frx, c->fry);
   hg::LaunchFacesDeriv0Pair(c->st, c->geo, c->frx, c->fry, c->ffrx, c->ffry);
@@ -699,6 +748,10 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
                              c->net->Trained();
         pc = learned ? static_cast<hg::Precond*>(c->net.get())
                      : static_cast<hg::Precond*>(c->mg.get());
+        if (cfg.fcg_negate_precond_gpu) {
+          c->negated.reset(new NegatedPrecond(pc, c->nc));
+          pc = c->negated.get();
+        }
       }
       c->pressure_system = true;
       // the closing residual check is read back at the next synchronisation;
@@ -718,26 +771,11 @@ void SolvePressure(hg_ctx* c, hg_iter_stats* is) {
   }
 }
 
-// One SIMPLE iteration; its statistics are built in c->last_iter and copied
-// to *out.  With defer_rs the convergence indicator (and the pressure
-// solve's closing check) are left for the next Sync(): for callers whose
-// loop ends on the iteration limit without looking at the indicator.
-void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
-  hg_iter_stats* is = &c->last_iter;
-  std::memset(is, 0, sizeof(*is));
+// The FP64 path's part of a SIMPLE iteration up to the fluxes: cell
+// gradients, momentum rows (fluid.hpp:827-874, conv_diff.hpp:130-243), the
+// velocity solve and Rhie-Chow (fluid.hpp:876-970).
+void Fp64Momentum(hg_ctx* c, hg_iter_stats* is) {
   const hg_config& cfg = c->cfg;
-  // outlet velocities from the current iterate (fluid.hpp:820-821)
-  if (c->any_outlet) {
-    const long nb = 2L * (c->geo.nx + c->geo.ny);
-    hg::LaunchOutletUpdate(c->st, c->geo, c->side_kind, c->u_ic[0], c->u_ic[1], c->bvel_d,
-                           c->bvel_d + nb, c->obc_scratch);
-  }
-  // iter_prev <- iter_curr (fluid.hpp:815-818, conv_diff.hpp:132-134); the
-  // iter_curr buffers are rewritten in full below.
-  std::swap(c->p_ip, c->p_ic);
-  std::swap(c->fl_ip, c->fl_ic);
-  std::swap(c->u_ip[0], c->u_ic[0]);
-  std::swap(c->u_ip[1], c->u_ic[1]);
   hg::LaunchCellGrads(c->st, c->geo, c->bc, c->p_ip, c->u_ip[0], c->u_ip[1], c->gpx, c->gpy,
                       c->g0x, c->g0y, c->g1x, c->g1y);
   hg::MomentumArgs m{};
@@ -760,6 +798,65 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
   hg::LaunchRhieChow(c->st, c->geo, c->bc, c->u_ic[0], c->u_ic[1], c->gpx, c->gpy, c->ffrx,
                      c->ffry, c->ffex, c->ffey, c->p_ip, c->ap, cfg.rhie_chow_factor,
                      cfg.meshvel[0], cfg.meshvel[1], c->fstar, c->fcoef);
+}
+
+// The mixed path's part of a SIMPLE iteration up to the fluxes: FP32 cell
+// gradients (differences taken in FP64), momentum rows with FP64 residual
+// constants stored as FP32, the red-black velocity correction in FP32 added
+// to the FP64 iterate, and Rhie-Chow in FP64 on the FP32 intermediates.
+void MixedMomentum(hg_ctx* c, hg_iter_stats* is) {
+  const hg_config& cfg = c->cfg;
+  auto& f = c->f32;
+  hg::LaunchCellGrads(c->st, c->geo, c->bc, c->p_ip, c->u_ip[0], c->u_ip[1], f.gpx, f.gpy,
+                      f.g0x, f.g0y, f.g1x, f.g1y);
+  hg::MomentumArgsF m{};
+  m.u0 = c->u_ip[0];
+  m.u1 = c->u_ip[1];
+  m.g0x = f.g0x; m.g0y = f.g0y; m.g1x = f.g1x; m.g1y = f.g1y;
+  m.gpx = f.gpx; m.gpy = f.gpy; m.frx = f.frx; m.fry = f.fry;
+  m.ffmu = f.ffmu;
+  m.flux = c->fl_ip;
+  m.rho = c->rho_s;
+  m.utp0 = c->u_tp[0]; m.utp1 = c->u_tp[1]; m.utc0 = c->u_tc[0]; m.utc1 = c->u_tc[1];
+  m.bdf0 = c->bdf[0]; m.bdf1 = c->bdf[1]; m.bdf2 = c->bdf[2];
+  m.relax = cfg.velocity_relaxation_factor;
+  m.s = f.s; m.w = f.w; m.c = f.c; m.e = f.e; m.n = f.n; m.k0 = f.k0; m.k1 = f.k1;
+  m.ap = c->ap;
+  hg::LaunchMomentum(c->st, c->geo, c->bc, m);
+  hg::RowsF rows{f.s, f.w, f.c, f.e, f.n};
+  hg::LaunchLuRb(c->st, c->geo.nx, c->geo.ny, rows, f.k0, f.k1, f.xb0, f.xb1, c->u_ip[0],
+                 c->u_ip[1], c->u_ic[0], c->u_ic[1]);
+  is->velocity_solve_iters[0] = is->velocity_solve_iters[1] = 0;
+  c->cd_iter += 1;
+  hg::LaunchRhieChow(c->st, c->geo, c->bc, c->u_ic[0], c->u_ic[1], f.gpx, f.gpy, f.ffrx, f.ffry,
+                     f.ffex, f.ffey, c->p_ip, c->ap, cfg.rhie_chow_factor, cfg.meshvel[0],
+                     cfg.meshvel[1], c->fstar, c->fcoef);
+}
+
+// One SIMPLE iteration; its statistics are built in c->last_iter and copied
+// to *out.  With defer_rs the convergence indicator (and the pressure
+// solve's closing check) are left for the next Sync(): for callers whose
+// loop ends on the iteration limit without looking at the indicator.
+void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
+  hg_iter_stats* is = &c->last_iter;
+  std::memset(is, 0, sizeof(*is));
+  const hg_config& cfg = c->cfg;
+  // outlet velocities from the current iterate (fluid.hpp:820-821)
+  if (c->any_outlet) {
+    const long nb = 2L * (c->geo.nx + c->geo.ny);
+    hg::LaunchOutletUpdate(c->st, c->geo, c->side_kind, c->u_ic[0], c->u_ic[1], c->bvel_d,
+                           c->bvel_d + nb, c->obc_scratch);
+  }
+  // iter_prev <- iter_curr (fluid.hpp:815-818, conv_diff.hpp:132-134); the
+  // iter_curr buffers are rewritten in full below.
+  std::swap(c->p_ip, c->p_ic);
+  std::swap(c->fl_ip, c->fl_ic);
+  std::swap(c->u_ip[0], c->u_ic[0]);
+  std::swap(c->u_ip[1], c->u_ic[1]);
+  if (c->mixed)
+    MixedMomentum(c, is);
+  else
+    Fp64Momentum(c, is);
   if (FaceSolver(cfg.linear_solver_pressure))
     hg::LaunchPressureRhs(c->st, c->geo, c->fstar, c->pb, true);  // operator from the faces
   else
@@ -769,7 +866,9 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
   hg::LaunchCorrectCells(c->st, c->geo, c->pcorr, c->ap, c->p_ip,
                          cfg.pressure_relaxation_factor, c->p_ic, c->u_ic[0], c->u_ic[1],
                          c->u_ip[0], c->u_ip[1], c->rs_part, (double*)c->rs_d);
-  hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic);
+  hg::LaunchCorrectFaces(c->st, c->geo, c->fstar, c->fcoef, c->pcorr, c->fl_ic,
+                         c->mixed ? c->f32.flux : nullptr);
+  c->flux32_valid = c->mixed;
   if (cfg.simpler) {
     // SIMPLER (fluid.hpp:1059-1151): a second solve on the same pressure
     // rows with the right-hand side of KSimplerRhs, added to p; the
@@ -842,8 +941,17 @@ void AdvectSubstep(hg_ctx* c) {
     for (int i = 0; i < c->np; ++i) bval[i] = c->pdb_d + i * nb;
   int inlet[4];
   for (int s = 0; s < 4; ++s) inlet[s] = c->side_kind[s] == HG_BC_INLET ? 1 : 0;
-  hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un,
-                    c->any_inlet ? bval : nullptr, inlet);
+  if (c->mixed) {
+    if (!c->flux32_valid) {
+      hg::LaunchToFloat(c->st, c->nf, c->fl_ic, c->f32.flux);
+      c->flux32_valid = true;
+    }
+    hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->f32.flux, c->dt_adv, un,
+                      c->any_inlet ? bval : nullptr, inlet);
+  } else {
+    hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un,
+                      c->any_inlet ? bval : nullptr, inlet);
+  }
   for (int i = 0; i < c->np; ++i) std::swap(c->pd[i], c->pd_new[i]);
   c->t_adv += c->dt_adv;
 }
@@ -923,6 +1031,14 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
                       "cg / fcg solve the pressure up to a constant: pressure_fixed_point is not "
                       "supported with them");
     }
+    if (cfg.precision_gpu == HG_PREC_MIXED) {
+      if (cfg.linear_solver_velocity != HG_SOLVER_LU_RB)
+        throw HgError(HG_ERR_UNSUPPORTED,
+                      "precision_gpu mixed solves the velocity correction in FP32 with lu_rb: "
+                      "set linear_solver_velocity lu_rb");
+      if (cfg.simpler)
+        throw HgError(HG_ERR_UNSUPPORTED, "simpler runs in the FP64 path only");
+    }
     if (cfg.simpler && FaceSolver(cfg.linear_solver_pressure))
       throw HgError(HG_ERR_UNSUPPORTED,
                     "simpler runs with the row solvers (gauss_seidel, jacobi, lu), not cg / fcg");
@@ -981,6 +1097,14 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
                        &c->ap, &c->Ps, &c->Pw, &c->Pc, &c->Pe, &c->Pn, &c->Pk, &c->pcorr, &c->pb,
                        &c->zeros})
       *p = c->Alloc(nc);
+    if (cfg.precision_gpu == HG_PREC_MIXED) {
+      c->mixed = true;
+      auto& f = c->f32;
+      for (float** p : {&f.ffex, &f.ffey, &f.ffmu, &f.ffrx, &f.ffry, &f.flux}) *p = c->AllocF(nf);
+      for (float** p : {&f.frx, &f.fry, &f.gpx, &f.gpy, &f.g0x, &f.g0y, &f.g1x, &f.g1y, &f.s,
+                        &f.w, &f.c, &f.e, &f.n, &f.k0, &f.k1, &f.xb0, &f.xb1})
+        *p = c->AllocF(nc);
+    }
     c->xn_d = c->Alloc(cfg.nx + 1);
     c->yn_d = c->Alloc(cfg.ny + 1);
     c->stat_part = c->Alloc(long(hg::StatsBlocks(c->geo)) * 7);
@@ -1159,6 +1283,7 @@ hg_status hg_set_field(hg_ctx* c, int32_t field, const double* host, size_t n) {
   }
   return Guard(c, [&]() {
     c->fields_checked = false;
+    c->flux32_valid = false;
     HG_CHECK(hipMemcpyAsync(d, host, n * 8, hipMemcpyHostToDevice, c->st));
     double* prev = nullptr;
     if (field == HG_FIELD_VELOCITY_X) prev = c->u_tp[0];
@@ -1220,6 +1345,7 @@ hg_status hg_refresh(hg_ctx* c) {
   if (!c) return HG_ERR_BAD_ARG;
   return Guard(c, [&]() {
     c->fields_checked = false;
+    c->flux32_valid = false;
     if (c->open_bc) InitBoundary(c);
     InitialFlux(c);
     UpdateProperties(c);
@@ -1408,7 +1534,33 @@ hg_status hg_time_kernel(hg_ctx* c, const char* kernel, int32_t reps, double* av
                             c->np > 2 ? c->pd[2] : nullptr};
       double* un[3] = {c->pd_new[0], c->np > 1 ? c->pd_new[1] : nullptr,
                        c->np > 2 ? c->pd_new[2] : nullptr};
-      launch = [=]() { hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un); };
+      if (c->mixed) {
+        if (!c->flux32_valid) {
+          hg::LaunchToFloat(c->st, c->nf, c->fl_ic, c->f32.flux);
+          c->flux32_valid = true;
+        }
+        launch = [=]() {
+          hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->f32.flux, c->dt_adv, un);
+        };
+      } else {
+        launch = [=]() { hg::LaunchAdvStep(c->st, c->geo, c->np, u, c->fl_ic, c->dt_adv, un); };
+      }
+    } else if (k == "KMomentum" && c->mixed) {
+      hg::MomentumArgsF m{};
+      auto& f = c->f32;
+      m.u0 = c->u_ip[0];
+      m.u1 = c->u_ip[1];
+      m.g0x = f.g0x; m.g0y = f.g0y; m.g1x = f.g1x; m.g1y = f.g1y;
+      m.gpx = f.gpx; m.gpy = f.gpy; m.frx = f.frx; m.fry = f.fry;
+      m.ffmu = f.ffmu;
+      m.flux = c->fl_ip;
+      m.rho = c->rho_s;
+      m.utp0 = c->u_tp[0]; m.utp1 = c->u_tp[1]; m.utc0 = c->u_tc[0]; m.utc1 = c->u_tc[1];
+      m.bdf0 = c->bdf[0]; m.bdf1 = c->bdf[1]; m.bdf2 = c->bdf[2];
+      m.relax = c->cfg.velocity_relaxation_factor;
+      m.s = f.s; m.w = f.w; m.c = f.c; m.e = f.e; m.n = f.n; m.k0 = f.k0; m.k1 = f.k1;
+      m.ap = c->ap;
+      launch = [=]() { hg::LaunchMomentum(c->st, c->geo, c->bc, m); };
     } else if (k == "KMomentum") {
       // the rows of the last SIMPLE iteration's inputs, rewritten in place
       hg::MomentumArgs m{};

@@ -30,13 +30,19 @@ struct Phases {
   double force[2];
 };
 
-struct MomentumArgs {
-  const double *u0, *u1, *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry;
-  const double *ffmu, *flux, *rho, *utp0, *utp1, *utc0, *utc1;
+// T: storage type of the gradients, the per-step face fields and the rows
+// (double: FP64 path; float: mixed path).  The state arrays are FP64.
+template <class T>
+struct MomentumArgsT {
+  const double *u0, *u1;
+  const T *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry, *ffmu;
+  const double *flux, *rho, *utp0, *utp1, *utc0, *utc1;
   double bdf0, bdf1, bdf2, relax;
-  double *s, *w, *c, *e, *n, *k0, *k1;
+  T *s, *w, *c, *e, *n, *k0, *k1;
   double* ap = nullptr;  // when set, also a_P of the relaxed rows (fluid.hpp:876-883)
 };
+using MomentumArgs = MomentumArgsT<double>;
+using MomentumArgsF = MomentumArgsT<float>;
 
 // Boundary conditions of the temperature, per boundary face (left j,
 // right ny + j, bottom 2 ny + i, top 2 ny + nx + i): kind 0 value fixed,
@@ -54,19 +60,36 @@ struct HeatIn {
 };
 
 // ---- step kernels (step_kernels.hip) ----
+// Each step kernel with intermediates has an FP64 form and a mixed form
+// (float intermediates; FP64 state and residuals).
 void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
                       const double* mu, double* ffex, double* ffey, double* ffmu);
+void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
+                      const double* mu, float* ffex, float* ffey, float* ffmu);
 void LaunchRestoredForce(hipStream_t st, const Geo& g, const double* ffex, const double* ffey,
                          double* frx, double* fry);
+void LaunchRestoredForce(hipStream_t st, const Geo& g, const float* ffex, const float* ffey,
+                         float* frx, float* fry);
 void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const double* ax, const double* ay,
                            double* fx, double* fy);
+void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const float* ax, const float* ay,
+                           float* fx, float* fy);
 void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
                      const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
                      double* g0y, double* g1x, double* g1y);
+void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
+                     const double* u0, const double* u1, float* gpx, float* gpy, float* g0x,
+                     float* g0y, float* g1x, float* g1y);
 void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a);
+void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgsF& a);
 void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                     const double* u1, const double* gpx, const double* gpy, const double* ffrx,
                     const double* ffry, const double* ffex, const double* ffey, const double* p,
+                    const double* ap, double rc, double mvx, double mvy, double* fstar,
+                    double* fcoef);
+void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                    const double* u1, const float* gpx, const float* gpy, const float* ffrx,
+                    const float* ffry, const float* ffex, const float* ffey, const double* p,
                     const double* ap, double rc, double mvx, double mvy, double* fstar,
                     double* fcoef);
 void LaunchPressureRows(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
@@ -79,8 +102,11 @@ void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const do
                         const double* pprev, double alpha_p, double* pcurr, double* u0,
                         double* u1, const double* up0, const double* up1, double* part,
                         double* rs);
+// flux32 (may be NULL): an FP32 copy of the corrected flux (mixed path).
 void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
-                        const double* pc, double* flux);
+                        const double* pc, double* flux, float* flux32 = nullptr);
+// b = float(a), n values.
+void LaunchToFloat(hipStream_t st, long n, const double* a, float* b);
 void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                        const double* u1, double mvx, double mvy, double* flux);
 // One advection sub-step of nf (1..3) fields with gradients formed in place;
@@ -88,6 +114,11 @@ void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double*
 // the sides with inlet[s] != 0.
 void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
                    const double* flux, double dt, double* const un[3],
+                   const double* const bval[3] = nullptr, const int* inlet = nullptr);
+// The mixed form: FP32 flux, gradients and Superbee face values; FP64
+// fields and FP64 sums of the face terms.
+void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
+                   const float* flux, double dt, double* const un[3],
                    const double* const bval[3] = nullptr, const int* inlet = nullptr);
 // Inlets and outlets (kind[s]: hg_bc_kind per side).  Inlet values of the
 // phases into pdb[p * nb + b] (nb = 2 (nx + ny) boundary faces); the outlet
@@ -130,9 +161,12 @@ void LaunchStats(hipStream_t st, const Geo& g, const double* xn, const double* y
 
 // ---- 5-point solvers (solvers.hip) ----
 // Rows: s x[r-nx] + w x[r-1] + c x[r] + e x[r+1] + n x[r+nx] + k = 0.
-struct Rows {
-  const double *s, *w, *c, *e, *n;
+template <class T>
+struct RowsT {
+  const T *s, *w, *c, *e, *n;
 };
+using Rows = RowsT<double>;
+using RowsF = RowsT<float>;
 
 // Exact-order symmetric Gauss-Seidel sweep from zero (linear.hpp:527-567) for
 // up to two right-hand sides sharing the coefficients, in one workgroup.
@@ -160,9 +194,13 @@ struct JacobiWork {
 };
 void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k, JacobiWork w,
                         int first_sweep, int nsweeps, double tol, int limit, double omega);
-// Red-black ordered symmetric GS sweep ("lu_rb"), two right-hand sides.
+// Red-black ordered symmetric GS sweep ("lu_rb"), two right-hand sides,
+// out = base + x; the float form solves the correction in FP32 (mixed path).
 void LaunchLuRb(hipStream_t st, int nx, int ny, Rows a, const double* k0, const double* k1,
                 double* xb0, double* xb1, const double* base0, const double* base1,
+                double* out0, double* out1);
+void LaunchLuRb(hipStream_t st, int nx, int ny, RowsF a, const float* k0, const float* k1,
+                float* xb0, float* xb1, const double* base0, const double* base1,
                 double* out0, double* out1);
 
 // SIMPLER's second pressure right-hand side (fluid.hpp:1059-1151) into k,

@@ -314,10 +314,13 @@ KJacobiSweep(int nx, int ny, Rows a, const double* __restrict__ k, JacobiWork w,
 }
 
 // "lu_rb", first kernel: black cells ((i+j) odd).  Red neighbours are the
-// forward red values -k/c computed on the fly.
+// forward red values -k/c computed on the fly.  T: the rows' storage and
+// arithmetic type (float in the mixed path: the correction is solved in FP32
+// from a residual formed in FP64).
+template <class T>
 __global__ void __launch_bounds__(256)
-KLuRbBlack(int nx, int ny, Rows a, const double* __restrict__ k0, const double* __restrict__ k1,
-           double* __restrict__ xb0, double* __restrict__ xb1) {
+KLuRbBlack(int nx, int ny, RowsT<T> a, const T* __restrict__ k0, const T* __restrict__ k1,
+           T* __restrict__ xb0, T* __restrict__ xb1) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nx * ny) return;
   const int i = r % nx, j = r / nx;
@@ -326,66 +329,70 @@ KLuRbBlack(int nx, int ny, Rows a, const double* __restrict__ k0, const double* 
   // neighbour's values are not used)
   const int qs = j > 0 ? r - nx : r, qw = i > 0 ? r - 1 : r;
   const int qe = i < nx - 1 ? r + 1 : r, qn = j < ny - 1 ? r + nx : r;
-  const double as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
-  const double k0r = k0[r], k1r = k1[r];
-  const double k0s = k0[qs], k1s = k1[qs], cs = a.c[qs];
-  const double k0w = k0[qw], k1w = k1[qw], cw = a.c[qw];
-  const double k0e = k0[qe], k1e = k1[qe], ce = a.c[qe];
-  const double k0n = k0[qn], k1n = k1[qn], cn = a.c[qn];
-  double s0 = 0., s1 = 0.;
+  const T as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
+  const T k0r = k0[r], k1r = k1[r];
+  const T k0s = k0[qs], k1s = k1[qs], cs = a.c[qs];
+  const T k0w = k0[qw], k1w = k1[qw], cw = a.c[qw];
+  const T k0e = k0[qe], k1e = k1[qe], ce = a.c[qe];
+  const T k0n = k0[qn], k1n = k1[qn], cn = a.c[qn];
+  const T z = T(0);
+  T s0 = z, s1 = z;
   if (j > 0) {
-    s0 += as * ((-k0s - 0.) / cs);
-    s1 += as * ((-k1s - 0.) / cs);
+    s0 += as * ((-k0s - z) / cs);
+    s1 += as * ((-k1s - z) / cs);
   }
   if (i > 0) {
-    s0 += aw * ((-k0w - 0.) / cw);
-    s1 += aw * ((-k1w - 0.) / cw);
+    s0 += aw * ((-k0w - z) / cw);
+    s1 += aw * ((-k1w - z) / cw);
   }
   if (i < nx - 1) {
-    s0 += ae * ((-k0e - 0.) / ce);
-    s1 += ae * ((-k1e - 0.) / ce);
+    s0 += ae * ((-k0e - z) / ce);
+    s1 += ae * ((-k1e - z) / ce);
   }
   if (j < ny - 1) {
-    s0 += an * ((-k0n - 0.) / cn);
-    s1 += an * ((-k1n - 0.) / cn);
+    s0 += an * ((-k0n - z) / cn);
+    s1 += an * ((-k1n - z) / cn);
   }
   xb0[r] = (-k0r - s0) / ac;
   xb1[r] = (-k1r - s1) / ac;
 }
 
 // "lu_rb", second kernel: backward update of red cells from black
-// neighbours (N, E, W, S order) and the delta-form update for all cells.
+// neighbours (N, E, W, S order) and the delta-form update for all cells,
+// out = base + x in FP64.
+template <class T>
 __global__ void __launch_bounds__(256)
-KLuRbRed(int nx, int ny, Rows a, const double* __restrict__ k0, const double* __restrict__ k1,
-         const double* __restrict__ xb0, const double* __restrict__ xb1,
+KLuRbRed(int nx, int ny, RowsT<T> a, const T* __restrict__ k0, const T* __restrict__ k1,
+         const T* __restrict__ xb0, const T* __restrict__ xb1,
          const double* __restrict__ base0, const double* __restrict__ base1,
          double* __restrict__ out0, double* __restrict__ out1) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nx * ny) return;
   const int i = r % nx, j = r / nx;
   if ((i + j) & 1) {
-    out0[r] = base0[r] + xb0[r];
-    out1[r] = base1[r] + xb1[r];
+    out0[r] = base0[r] + double(xb0[r]);
+    out1[r] = base1[r] + double(xb1[r]);
     return;
   }
   // neighbours at clamped indices, all loaded before use
   const int qs = j > 0 ? r - nx : r, qw = i > 0 ? r - 1 : r;
   const int qe = i < nx - 1 ? r + 1 : r, qn = j < ny - 1 ? r + nx : r;
-  const double as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
+  const T as = a.s[r], aw = a.w[r], ae = a.e[r], an = a.n[r], ac = a.c[r];
   const double b0 = base0[r], b1 = base1[r];
-  const double x0n = xb0[qn], x1n = xb1[qn], x0e = xb0[qe], x1e = xb1[qe];
-  const double x0w = xb0[qw], x1w = xb1[qw], x0s = xb0[qs], x1s = xb1[qs];
-  double x0 = (-k0[r] - 0.) / ac;
-  double x1 = (-k1[r] - 0.) / ac;
-  double s0 = 0., s1 = 0.;
+  const T x0n = xb0[qn], x1n = xb1[qn], x0e = xb0[qe], x1e = xb1[qe];
+  const T x0w = xb0[qw], x1w = xb1[qw], x0s = xb0[qs], x1s = xb1[qs];
+  const T z = T(0);
+  T x0 = (-k0[r] - z) / ac;
+  T x1 = (-k1[r] - z) / ac;
+  T s0 = z, s1 = z;
   if (j < ny - 1) { s0 += an * x0n; s1 += an * x1n; }
   if (i < nx - 1) { s0 += ae * x0e; s1 += ae * x1e; }
   if (i > 0) { s0 += aw * x0w; s1 += aw * x1w; }
   if (j > 0) { s0 += as * x0s; s1 += as * x1s; }
   x0 -= s0 / ac;
   x1 -= s1 / ac;
-  out0[r] = b0 + x0;
-  out1[r] = b1 + x1;
+  out0[r] = b0 + double(x0);
+  out1[r] = b1 + double(x1);
 }
 
 }  // namespace
@@ -429,13 +436,25 @@ void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k,
     KJacobiSweep<<<blocks, 256, 0, st>>>(nx, ny, a, k, w, s, tol, limit, omega);
 }
 
+template <class T>
+void LaunchLuRbT(hipStream_t st, int nx, int ny, RowsT<T> a, const T* k0, const T* k1, T* xb0,
+                 T* xb1, const double* base0, const double* base1, double* out0, double* out1) {
+  long n = long(nx) * ny;
+  dim3 grid(unsigned((n + 255) / 256));
+  KLuRbBlack<T><<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1);
+  KLuRbRed<T><<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1, base0, base1, out0, out1);
+}
+
 void LaunchLuRb(hipStream_t st, int nx, int ny, Rows a, const double* k0, const double* k1,
                 double* xb0, double* xb1, const double* base0, const double* base1,
                 double* out0, double* out1) {
-  long n = long(nx) * ny;
-  dim3 grid(unsigned((n + 255) / 256));
-  KLuRbBlack<<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1);
-  KLuRbRed<<<grid, 256, 0, st>>>(nx, ny, a, k0, k1, xb0, xb1, base0, base1, out0, out1);
+  LaunchLuRbT<double>(st, nx, ny, a, k0, k1, xb0, xb1, base0, base1, out0, out1);
+}
+
+void LaunchLuRb(hipStream_t st, int nx, int ny, RowsF a, const float* k0, const float* k1,
+                float* xb0, float* xb1, const double* base0, const double* base1,
+                double* out0, double* out1) {
+  LaunchLuRbT<float>(st, nx, ny, a, k0, k1, xb0, xb1, base0, base1, out0, out1);
 }
 
 }  // namespace hg

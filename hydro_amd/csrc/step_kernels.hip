@@ -1,12 +1,19 @@
-// step_kernels.hip -- FP64 kernels of the hydro2d time step.
+// step_kernels.hip -- kernels of the hydro2d time step.
 //
 // One kernel per stage of FluidSimple::MakeIteration (fluid.hpp:814-1158),
 // AdvectionSolverMultiExplicit::MakeIteration (advection.hpp:440-539) and
 // hydro::UpdateFluidProperties (hydro2d.hpp:1404-1430).  Every value is
 // computed with the reference's operation order (this file is compiled with
 // -ffp-contract=off), so on power-of-two meshes the FP64 results equal the
-// reference's bit for bit.  Fields are SoA FP64, x fastest; faces are the
+// reference's bit for bit.  Fields are SoA, x fastest; faces are the
 // x-faces followed by the y-faces.
+//
+// The state (velocity, pressure, volume flux, partial densities, properties)
+// is FP64.  The intermediates of a SIMPLE iteration -- per-step face fields,
+// cell gradients, momentum rows -- have a storage type T: double in the FP64
+// path, float in the mixed path.  Arithmetic that cancels (differences of
+// state values, residuals, corrections) stays FP64 in both; the red-black
+// velocity sweep and the advection limiter compute in T.
 
 #include "kernels.h"
 
@@ -26,10 +33,11 @@ __device__ __forceinline__ double BVel(const Bc& bc, const Geo& g, int comp, int
 
 // ---------------------------------------------------------------------------
 // Face interpolation of the per-step inputs (fluid.hpp:602-641).
+template <class T>
 __global__ void __launch_bounds__(kBlock)
 KForceFaces(Geo g, const double* __restrict__ fx, const double* __restrict__ fy,
-            const double* __restrict__ mu, double* __restrict__ ffex, double* __restrict__ ffey,
-            double* __restrict__ ffmu) {
+            const double* __restrict__ mu, T* __restrict__ ffex, T* __restrict__ ffey,
+            T* __restrict__ ffmu) {
   const int nf = g.nfx + g.nx * (g.ny + 1);
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
@@ -45,21 +53,22 @@ KForceFaces(Geo g, const double* __restrict__ fx, const double* __restrict__ fy,
     cp = (j < g.ny) ? i + g.nx * j : -1;
   }
   if (cm >= 0 && cp >= 0) {
-    ffex[f] = Mid(fx[cm], fx[cp]);
-    ffey[f] = Mid(fy[cm], fy[cp]);
-    ffmu[f] = Mid(mu[cm], mu[cp]);
+    ffex[f] = T(Mid(fx[cm], fx[cp]));
+    ffey[f] = T(Mid(fy[cm], fy[cp]));
+    ffmu[f] = T(Mid(mu[cm], mu[cp]));
   } else {
     int c = cm >= 0 ? cm : cp;  // zero-derivative condition: cell value
-    ffex[f] = fx[c];
-    ffey[f] = fy[c];
-    ffmu[f] = mu[c];
+    ffex[f] = T(fx[c]);
+    ffey[f] = T(fy[c]);
+    ffmu[f] = T(mu[c]);
   }
 }
 
 // Restored force (fluid.hpp:612-626): sum_f S_f (f_f . n_f) |x_c - x_f| / V.
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-KRestoredForce(Geo g, const double* __restrict__ ffex, const double* __restrict__ ffey,
-               double* __restrict__ frx, double* __restrict__ fry) {
+KRestoredForce(Geo g, const T* __restrict__ ffex, const T* __restrict__ ffey,
+               T* __restrict__ frx, T* __restrict__ fry) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nx * g.ny) return;
   const int i = c % g.nx, j = c / g.nx;
@@ -67,18 +76,19 @@ KRestoredForce(Geo g, const double* __restrict__ ffex, const double* __restrict_
   const int fs = YFace(g, i, j), fn = YFace(g, i, j + 1);
   // x-faces: S = (sx, -0), n = (1, -0): S*(f.n) has only an x component.
   double sx = 0., sy = 0.;
-  sx += g.sx * ffex[fw] * g.rdist_x;
-  sx += g.sx * ffex[fe] * g.rdist_x;
-  sy += g.sy * ffey[fs] * g.rdist_y;
-  sy += g.sy * ffey[fn] * g.rdist_y;
-  frx[c] = OverV(g, sx);
-  fry[c] = OverV(g, sy);
+  sx += g.sx * double(ffex[fw]) * g.rdist_x;
+  sx += g.sx * double(ffex[fe]) * g.rdist_x;
+  sy += g.sy * double(ffey[fs]) * g.rdist_y;
+  sy += g.sy * double(ffey[fn]) * g.rdist_y;
+  frx[c] = T(OverV(g, sx));
+  fry[c] = T(OverV(g, sy));
 }
 
 // Face interpolation with zero-derivative condition for a pair of cell fields.
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-KFacesDeriv0Pair(Geo g, const double* __restrict__ ax, const double* __restrict__ ay,
-                 double* __restrict__ fx, double* __restrict__ fy) {
+KFacesDeriv0Pair(Geo g, const T* __restrict__ ax, const T* __restrict__ ay,
+                 T* __restrict__ fx, T* __restrict__ fy) {
   const int nf = g.nfx + g.nx * (g.ny + 1);
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
@@ -94,8 +104,8 @@ KFacesDeriv0Pair(Geo g, const double* __restrict__ ax, const double* __restrict_
     cp = (j < g.ny) ? i + g.nx * j : -1;
   }
   if (cm >= 0 && cp >= 0) {
-    fx[f] = Mid(ax[cm], ax[cp]);
-    fy[f] = Mid(ay[cm], ay[cp]);
+    fx[f] = T(Mid(double(ax[cm]), double(ax[cp])));
+    fy[f] = T(Mid(double(ay[cm]), double(ay[cp])));
   } else {
     int c = cm >= 0 ? cm : cp;
     fx[f] = ax[c];
@@ -107,12 +117,14 @@ KFacesDeriv0Pair(Geo g, const double* __restrict__ ax, const double* __restrict_
 // condition) at cells (fluid.hpp:827-832, 840-842; conv_diff.hpp:135).
 // FV: the condition has per-face velocities (inlets or outlets); without
 // them the side's constant velocity needs no per-cell load.
-template <bool FV>
+// The differences of the FP64 fields are taken in FP64; the gradients are
+// stored as T.
+template <bool FV, class T>
 __global__ void __launch_bounds__(kBlock)
 KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict__ u0,
-           const double* __restrict__ u1, double* __restrict__ gpx, double* __restrict__ gpy,
-           double* __restrict__ g0x, double* __restrict__ g0y, double* __restrict__ g1x,
-           double* __restrict__ g1y) {
+           const double* __restrict__ u1, T* __restrict__ gpx, T* __restrict__ gpy,
+           T* __restrict__ g0x, T* __restrict__ g0y, T* __restrict__ g1x,
+           T* __restrict__ g1y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nx * g.ny) return;
   const int i = c % g.nx, j = c / g.nx;
@@ -120,9 +132,11 @@ KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict_
   const Cross xp = LoadCross(p, g.nx, g.ny, i, j);
   const Cross x0 = LoadCross(u0, g.nx, g.ny, i, j);
   const Cross x1 = LoadCross(u1, g.nx, g.ny, i, j);
-  double f[4];
+  double f[4], gx, gy;
   CellFacesV<2>(g, xp, i, j, nullptr, f);
-  GradFromFaces(g, f, gpx[c], gpy[c]);
+  GradFromFaces(g, f, gx, gy);
+  gpx[c] = T(gx);
+  gpy[c] = T(gy);
   double bv0[4], bv1[4];
   const int qs[4] = {j, j, i, i};
   for (int s = 0; s < 4; ++s) {
@@ -130,26 +144,36 @@ KCellGrads(Geo g, Bc bc, const double* __restrict__ p, const double* __restrict_
     bv1[s] = FV ? BVel(bc, g, 1, s, qs[s]) : bc.vel[1][s];
   }
   CellFacesV<0>(g, x0, i, j, bv0, f);
-  GradFromFaces(g, f, g0x[c], g0y[c]);
+  GradFromFaces(g, f, gx, gy);
+  g0x[c] = T(gx);
+  g0y[c] = T(gy);
   CellFacesV<0>(g, x1, i, j, bv1, f);
-  GradFromFaces(g, f, g1x[c], g1y[c]);
+  GradFromFaces(g, f, gx, gy);
+  g1x[c] = T(gx);
+  g1y[c] = T(gy);
 }
 
+template <class T>
 struct MomIn {
-  const double *u0, *u1, *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry;
-  const double *ffmu, *flux, *rho, *utp0, *utp1, *utc0, *utc1;
+  const double *u0, *u1;
+  const T *g0x, *g0y, *g1x, *g1y, *gpx, *gpy, *frx, *fry, *ffmu;
+  const double *flux, *rho, *utp0, *utp1, *utc0, *utc1;
   double bdf0, bdf1, bdf2, relax;
 };
+template <class T>
 struct MomOut {
-  double *s, *w, *c, *e, *n, *k0, *k1;
+  T *s, *w, *c, *e, *n, *k0, *k1;
   double* ap;  // optional: a_P of the relaxed rows
 };
 
 // Explicit viscosity + force assembly + momentum rows (fluid.hpp:835-874,
 // conv_diff.hpp:130-243).  Row coefficients are shared by both velocity
 // components (their BCs, fluxes and densities coincide); constants differ.
+// The row constants -- the delta form's residual at the previous iterate --
+// are formed in FP64 from the FP64 coefficients and stored as T with them.
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
+KMomentum(Geo g, Bc bc, MomIn<T> in, MomOut<T> out) {
   const int nx = g.nx, ny = g.ny;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nx * ny) return;
@@ -298,13 +322,13 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
   if (hn) { k0 += u0N * nb[3]; k1 += u1N * nb[3]; }
   const double rs = hs ? nb[2] : 0., rw = hw ? nb[0] : 0., rc = cen / in.relax;
   const double re = he ? nb[1] : 0., rn = hn ? nb[3] : 0.;
-  out.s[c] = rs;
-  out.w[c] = rw;
-  out.c[c] = rc;
-  out.e[c] = re;
-  out.n[c] = rn;
-  out.k0[c] = k0;
-  out.k1[c] = k1;
+  out.s[c] = T(rs);
+  out.w[c] = T(rw);
+  out.c[c] = T(rc);
+  out.e[c] = T(re);
+  out.n[c] = T(rn);
+  out.k0[c] = T(k0);
+  out.k1[c] = T(k1);
   if (out.ap) {
     // a_P = CoeffSum of the relaxed row, averaged over the two velocity
     // components (identical rows): fluid.hpp:876-883
@@ -322,11 +346,12 @@ KMomentum(Geo g, Bc bc, MomIn in, MomOut out) {
 }
 
 // Diagonal coefficient a_P = mean of the rows' CoeffSum (fluid.hpp:876-884).
+template <class T>
 __global__ void __launch_bounds__(kBlock)
 KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict__ u1,
-          const double* __restrict__ gpx, const double* __restrict__ gpy,
-          const double* __restrict__ ffrx, const double* __restrict__ ffry,
-          const double* __restrict__ ffex, const double* __restrict__ ffey,
+          const T* __restrict__ gpx, const T* __restrict__ gpy,
+          const T* __restrict__ ffrx, const T* __restrict__ ffry,
+          const T* __restrict__ ffex, const T* __restrict__ ffey,
           const double* __restrict__ p, const double* __restrict__ ap, double rc, double mvx,
           double mvy, double* __restrict__ fstar, double* __restrict__ fcoef) {
   const int nf = g.nfx + g.nx * (g.ny + 1);
@@ -354,12 +379,13 @@ KRhieChow(Geo g, Bc bc, const double* __restrict__ u0, const double* __restrict_
     // u*_f . S with u*_f = (u[cm] + u[cp]) / 2 component-wise
     const double ufn = xface ? Mid(u0[cm], u0[cp]) : Mid(u1[cm], u1[cp]);
     const double vfi = ufn * S;
-    const double gpf = xface ? Mid(gpx[cm], gpx[cp]) : Mid(gpy[cm], gpy[cp]);
-    const double frf = xface ? ffrx[f] : ffry[f];
+    const double gpf = xface ? Mid(double(gpx[cm]), double(gpx[cp]))
+                             : Mid(double(gpy[cm]), double(gpy[cp]));
+    const double frf = xface ? double(ffrx[f]) : double(ffry[f]);
     const double wide = (gpf - frf) * S;
     const double dn = xface ? g.dnx : g.dny;
     const double A = xface ? g.ax : g.ay;
-    const double fef = xface ? ffex[f] : ffey[f];
+    const double fef = xface ? double(ffex[f]) : double(ffey[f]);
     const double compact = (p[cp] - p[cm]) / dn * A - fef * S;
     const double af = Mid(ap[cm], ap[cp]);
     fs = vfi + rc * (wide - compact) / af + 0;
@@ -494,9 +520,11 @@ KMaxFold(const double* __restrict__ part, int n, double* __restrict__ rs) {
 }
 
 // Divergence-free volume flux: the flux row evaluated at p' (fluid.hpp:1053-1056).
+// flux32 (optional): an FP32 copy, the advection's input in the mixed path.
 __global__ void __launch_bounds__(kBlock)
 KCorrectFaces(Geo g, const double* __restrict__ fstar, const double* __restrict__ fcoef,
-              const double* __restrict__ pc, double* __restrict__ flux) {
+              const double* __restrict__ pc, double* __restrict__ flux,
+              float* __restrict__ flux32) {
   const int nf = g.nfx + g.nx * (g.ny + 1);
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
@@ -517,6 +545,7 @@ KCorrectFaces(Geo g, const double* __restrict__ fstar, const double* __restrict_
     v += pc[cp] * fcoef[f];
   }
   flux[f] = v;
+  if (flux32) flux32[f] = float(v);
 }
 
 // Initial volume flux u_f . S - meshvel . S (fluid.hpp:770-784).
@@ -557,18 +586,29 @@ KInitialFlux(Geo g, Bc bc, const double* __restrict__ u0, const double* __restri
 // Advection (advection.hpp:440-478): gradient of the zero-derivative face
 // interpolation, then Superbee face values and the explicit update.
 
-__device__ __forceinline__ double Superbee(double p, double q) {
-  if (p > 0. && q > 0.) return fmax(fmin(2 * p, q), fmin(p, 2 * q));
-  if (p < 0. && q < 0.) return -fmax(fmin(-2 * p, -q), fmin(-p, -2 * q));
-  return 0.;
+template <class R>
+__device__ __forceinline__ R Superbee(R p, R q) {
+  const R two = R(2), zero = R(0);
+  if (p > zero && q > zero) return fmax(fmin(two * p, q), fmin(p, two * q));
+  if (p < zero && q < zero) return -fmax(fmin(-two * p, -q), fmin(-p, -two * q));
+  return zero;
 }
 
 // Superbee face value between P (cell 0) and E (cell 1) (solver.hpp:576-592).
-__device__ __forceinline__ double SuperbeeFace(double uP, double uE, double gPn, double gEn,
-                                               double r0, double r1, double vf) {
-  if (vf > 1e-8) return uP + 0.5 * Superbee(uE - uP, -4. * (gPn * r0) - (uE - uP));
-  if (vf < -1e-8) return uE - 0.5 * Superbee(uE - uP, 4. * (gEn * r1) - (uE - uP));
-  return 0.5 * (uP + uE);
+template <class R>
+__device__ __forceinline__ R SuperbeeFace(R uP, R uE, R gPn, R gEn, R r0, R r1, R vf) {
+  const R half = R(0.5), four = R(4), eps = R(1e-8);
+  if (vf > eps) return uP + half * Superbee<R>(uE - uP, -four * (gPn * r0) - (uE - uP));
+  if (vf < -eps) return uE - half * Superbee<R>(uE - uP, four * (gEn * r1) - (uE - uP));
+  return half * (uP + uE);
+}
+
+template <class R>
+__device__ __forceinline__ R MidR(R um, R up) { return um * R(0.5) + up * R(0.5); }
+// x / V in R, as a product when that is exact (V a power of two)
+template <class R>
+__device__ __forceinline__ R OverVR(const Geo& g, R x) {
+  return g.pow2V ? x * R(g.invV) : x / R(g.V);
 }
 
 struct AdvFields {
@@ -588,15 +628,20 @@ struct AdvFields {
 // and the y-gradients on the tile plus one row each side, then updates the
 // tile with the Superbee faces.  Every value is computed with the same
 // operations as the unfused form, so the result is bit for bit the same.
+// R is the arithmetic type of the gradients and the Superbee face values,
+// and the type of the flux read (float in the mixed path).  The fields stay
+// FP64 and the update sums the four face terms in FP64: a face term is the
+// same number for the two cells it separates, so the fields' volume sums
+// are conserved to FP64 rounding in both paths.
 constexpr int kAdvTX = 64, kAdvTY = 8;
 
-template <int NF, bool IN>
+template <int NF, bool IN, class R>
 __global__ void __launch_bounds__(kBlock)
-KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
+KAdvStep(Geo g, AdvFields fl, const R* __restrict__ flux, double dt) {
   constexpr int UX = kAdvTX + 4, UY = kAdvTY + 4;
   __shared__ double su[UY][UX];
-  __shared__ double sgx[kAdvTY][kAdvTX + 2];
-  __shared__ double sgy[kAdvTY + 2][kAdvTX];
+  __shared__ R sgx[kAdvTY][kAdvTX + 2];
+  __shared__ R sgy[kAdvTY + 2][kAdvTX];
   const int nx = g.nx, ny = g.ny;
   const int ntx = (nx + kAdvTX - 1) / kAdvTX;
   const int i0 = (blockIdx.x % ntx) * kAdvTX, j0 = (blockIdx.x / ntx) * kAdvTY;
@@ -615,16 +660,16 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       ureg[q][k] = (t < NU && ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? fl.u[q][ii + nx * jj]
                                                                          : 0.;
     }
-  double vw_[KT], ve_[KT], vs_[KT], vn_[KT];
+  R vw_[KT], ve_[KT], vs_[KT], vn_[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int t = threadIdx.x + k * kBlock;
     const int i = i0 + t % kAdvTX, j = j0 + t / kAdvTX;
     const bool in = t < NT && i < nx && j < ny;
-    vw_[k] = in ? flux[XFace(g, i, j)] : 0.;
-    ve_[k] = in ? flux[XFace(g, i + 1, j)] : 0.;
-    vs_[k] = in ? flux[YFace(g, i, j)] : 0.;
-    vn_[k] = in ? flux[YFace(g, i, j + 1)] : 0.;
+    vw_[k] = in ? flux[XFace(g, i, j)] : R(0);
+    ve_[k] = in ? flux[XFace(g, i + 1, j)] : R(0);
+    vs_[k] = in ? flux[YFace(g, i, j)] : R(0);
+    vn_[k] = in ? flux[YFace(g, i, j + 1)] : R(0);
   }
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
@@ -637,21 +682,22 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
     __syncthreads();
     // a boundary face's value: the inlet's, or the cell's (zero derivative)
     const double* bv = fl.bval[q];
-    auto bface = [&](int s, int pos, double uc) {
-      return (IN && bv && fl.inlet[s]) ? bv[BoundaryIndex(g, s, pos)] : uc;
+    auto bface = [&](int s, int pos, R uc) {
+      return (IN && bv && fl.inlet[s]) ? R(bv[BoundaryIndex(g, s, pos)]) : uc;
     };
+    const R gsx = R(g.sx), gsy = R(g.sy);
     // x-gradients on columns i0-1 .. i0+64 of the tile rows
     for (int t = threadIdx.x; t < kAdvTY * (kAdvTX + 2); t += kBlock) {
       const int lj = t / (kAdvTX + 2), hi = t % (kAdvTX + 2);
       const int ii = i0 + hi - 1, jj = j0 + lj;
-      double gv = 0.;
+      R gv = R(0);
       if (ii >= 0 && ii < nx && jj < ny) {
-        const double uc = su[lj + 2][hi + 1];
-        const double fw = ii > 0 ? Mid(su[lj + 2][hi], uc) : bface(kLeft, jj, uc);
-        const double fe = ii < nx - 1 ? Mid(uc, su[lj + 2][hi + 2]) : bface(kRight, jj, uc);
-        double sxv = (-g.sx) * fw;
-        sxv += g.sx * fe;
-        gv = OverV(g, sxv);
+        const R uc = R(su[lj + 2][hi + 1]);
+        const R fw = ii > 0 ? MidR<R>(R(su[lj + 2][hi]), uc) : bface(kLeft, jj, uc);
+        const R fe = ii < nx - 1 ? MidR<R>(uc, R(su[lj + 2][hi + 2])) : bface(kRight, jj, uc);
+        R sxv = (-gsx) * fw;
+        sxv += gsx * fe;
+        gv = OverVR<R>(g, sxv);
       }
       sgx[lj][hi] = gv;
     }
@@ -659,14 +705,14 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
     for (int t = threadIdx.x; t < (kAdvTY + 2) * kAdvTX; t += kBlock) {
       const int hj = t / kAdvTX, li = t % kAdvTX;
       const int ii = i0 + li, jj = j0 + hj - 1;
-      double gv = 0.;
+      R gv = R(0);
       if (ii < nx && jj >= 0 && jj < ny) {
-        const double uc = su[hj + 1][li + 2];
-        const double fs = jj > 0 ? Mid(su[hj][li + 2], uc) : bface(kBottom, ii, uc);
-        const double fn = jj < ny - 1 ? Mid(uc, su[hj + 2][li + 2]) : bface(kTop, ii, uc);
-        double syv = (-g.sy) * fs;
-        syv += g.sy * fn;
-        gv = OverV(g, syv);
+        const R uc = R(su[hj + 1][li + 2]);
+        const R fs = jj > 0 ? MidR<R>(R(su[hj][li + 2]), uc) : bface(kBottom, ii, uc);
+        const R fn = jj < ny - 1 ? MidR<R>(uc, R(su[hj + 2][li + 2])) : bface(kTop, ii, uc);
+        R syv = (-gsy) * fs;
+        syv += gsy * fn;
+        gv = OverVR<R>(g, syv);
       }
       sgy[hj][li] = gv;
     }
@@ -678,23 +724,27 @@ KAdvStep(Geo g, AdvFields fl, const double* __restrict__ flux, double dt) {
       const int i = i0 + li, j = j0 + lj;
       if (t >= NT || i >= nx || j >= ny) continue;
       const int c = i + nx * j;
-      const double vw = vw_[k], ve = ve_[k], vs = vs_[k], vn = vn_[k];
-      const double uc = su[lj + 2][li + 2];
-      const double gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
-      double uw = i > 0 ? uc : bface(kLeft, j, uc), ue = i < nx - 1 ? uc : bface(kRight, j, uc);
-      double us_ = j > 0 ? uc : bface(kBottom, i, uc), uN = j < ny - 1 ? uc : bface(kTop, i, uc);
-      if (i > 0) uw = SuperbeeFace(su[lj + 2][li + 1], uc, sgx[lj][li], gxc, g.tx0, g.tx1, vw);
+      const R vw = vw_[k], ve = ve_[k], vs = vs_[k], vn = vn_[k];
+      const double uc64 = su[lj + 2][li + 2];
+      const R uc = R(uc64);
+      const R gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
+      const R tx0 = R(g.tx0), tx1 = R(g.tx1), ty0 = R(g.ty0), ty1 = R(g.ty1);
+      R uw = i > 0 ? uc : bface(kLeft, j, uc), ue = i < nx - 1 ? uc : bface(kRight, j, uc);
+      R us_ = j > 0 ? uc : bface(kBottom, i, uc), uN = j < ny - 1 ? uc : bface(kTop, i, uc);
+      if (i > 0)
+        uw = SuperbeeFace<R>(R(su[lj + 2][li + 1]), uc, sgx[lj][li], gxc, tx0, tx1, vw);
       if (i < nx - 1)
-        ue = SuperbeeFace(uc, su[lj + 2][li + 3], gxc, sgx[lj][li + 2], g.tx0, g.tx1, ve);
-      if (j > 0) us_ = SuperbeeFace(su[lj + 1][li + 2], uc, sgy[lj][li], gyc, g.ty0, g.ty1, vs);
+        ue = SuperbeeFace<R>(uc, R(su[lj + 2][li + 3]), gxc, sgx[lj][li + 2], tx0, tx1, ve);
+      if (j > 0)
+        us_ = SuperbeeFace<R>(R(su[lj + 1][li + 2]), uc, sgy[lj][li], gyc, ty0, ty1, vs);
       if (j < ny - 1)
-        uN = SuperbeeFace(uc, su[lj + 3][li + 2], gyc, sgy[lj + 2][li], g.ty0, g.ty1, vn);
+        uN = SuperbeeFace<R>(uc, R(su[lj + 3][li + 2]), gyc, sgy[lj + 2][li], ty0, ty1, vn);
       double fsum = 0.;
-      fsum += uw * vw * -1.;
-      fsum += ue * ve * 1.;
-      fsum += us_ * vs * -1.;
-      fsum += uN * vn * 1.;
-      fl.un[q][c] = uc + OverV(g, -dt) * fsum;
+      fsum += double(uw * vw * R(-1));
+      fsum += double(ue * ve * R(1));
+      fsum += double(us_ * vs * R(-1));
+      fsum += double(uN * vn * R(1));
+      fl.un[q][c] = uc64 + OverV(g, -dt) * fsum;
     }
   }
 }
@@ -1377,45 +1427,96 @@ KAutoDtFinal(const double* __restrict__ part, int nblocks, double* __restrict__ 
 
 // ---- launchers --------------------------------------------------------------
 
+template <class T>
+void ForceFacesT(hipStream_t st, const Geo& g, const double* fx, const double* fy,
+                 const double* mu, T* ffex, T* ffey, T* ffmu) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KForceFaces<T><<<Grid(nf), kBlock, 0, st>>>(g, fx, fy, mu, ffex, ffey, ffmu);
+}
 void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
                       const double* mu, double* ffex, double* ffey, double* ffmu) {
-  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
-  KForceFaces<<<Grid(nf), kBlock, 0, st>>>(g, fx, fy, mu, ffex, ffey, ffmu);
+  ForceFacesT<double>(st, g, fx, fy, mu, ffex, ffey, ffmu);
+}
+void LaunchForceFaces(hipStream_t st, const Geo& g, const double* fx, const double* fy,
+                      const double* mu, float* ffex, float* ffey, float* ffmu) {
+  ForceFacesT<float>(st, g, fx, fy, mu, ffex, ffey, ffmu);
 }
 void LaunchRestoredForce(hipStream_t st, const Geo& g, const double* ffex, const double* ffey,
                          double* frx, double* fry) {
-  KRestoredForce<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ffex, ffey, frx, fry);
+  KRestoredForce<double><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ffex, ffey, frx, fry);
+}
+void LaunchRestoredForce(hipStream_t st, const Geo& g, const float* ffex, const float* ffey,
+                         float* frx, float* fry) {
+  KRestoredForce<float><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, ffex, ffey, frx, fry);
 }
 void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const double* ax, const double* ay,
                            double* fx, double* fy) {
   long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
-  KFacesDeriv0Pair<<<Grid(nf), kBlock, 0, st>>>(g, ax, ay, fx, fy);
+  KFacesDeriv0Pair<double><<<Grid(nf), kBlock, 0, st>>>(g, ax, ay, fx, fy);
+}
+void LaunchFacesDeriv0Pair(hipStream_t st, const Geo& g, const float* ax, const float* ay,
+                           float* fx, float* fy) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KFacesDeriv0Pair<float><<<Grid(nf), kBlock, 0, st>>>(g, ax, ay, fx, fy);
+}
+template <class T>
+void CellGradsT(hipStream_t st, const Geo& g, const Bc& bc, const double* p, const double* u0,
+                const double* u1, T* gpx, T* gpy, T* g0x, T* g0y, T* g1x, T* g1y) {
+  if (bc.fvel[0] || bc.fvel[1])
+    KCellGrads<true, T><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy,
+                                                                   g0x, g0y, g1x, g1y);
+  else
+    KCellGrads<false, T><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy,
+                                                                    g0x, g0y, g1x, g1y);
 }
 void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
                      const double* u0, const double* u1, double* gpx, double* gpy, double* g0x,
                      double* g0y, double* g1x, double* g1y) {
-  if (bc.fvel[0] || bc.fvel[1])
-    KCellGrads<true><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x,
-                                                                g0y, g1x, g1y);
-  else
-    KCellGrads<false><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, p, u0, u1, gpx, gpy, g0x,
-                                                                 g0y, g1x, g1y);
+  CellGradsT<double>(st, g, bc, p, u0, u1, gpx, gpy, g0x, g0y, g1x, g1y);
+}
+void LaunchCellGrads(hipStream_t st, const Geo& g, const Bc& bc, const double* p,
+                     const double* u0, const double* u1, float* gpx, float* gpy, float* g0x,
+                     float* g0y, float* g1x, float* g1y) {
+  CellGradsT<float>(st, g, bc, p, u0, u1, gpx, gpy, g0x, g0y, g1x, g1y);
+}
+template <class T>
+void MomentumT(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgsT<T>& a) {
+  MomIn<T> in{a.u0,   a.u1,   a.g0x,  a.g0y,  a.g1x,  a.g1y,  a.gpx,  a.gpy,  a.frx, a.fry,
+              a.ffmu, a.flux, a.rho,  a.utp0, a.utp1, a.utc0, a.utc1, a.bdf0, a.bdf1, a.bdf2,
+              a.relax};
+  MomOut<T> out{a.s, a.w, a.c, a.e, a.n, a.k0, a.k1, a.ap};
+  KMomentum<T><<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, in, out);
 }
 void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgs& a) {
-  MomIn in{a.u0,   a.u1,   a.g0x,  a.g0y,  a.g1x,  a.g1y,  a.gpx,  a.gpy,  a.frx, a.fry,
-           a.ffmu, a.flux, a.rho,  a.utp0, a.utp1, a.utc0, a.utc1, a.bdf0, a.bdf1, a.bdf2,
-           a.relax};
-  MomOut out{a.s, a.w, a.c, a.e, a.n, a.k0, a.k1, a.ap};
-  KMomentum<<<Grid(long(g.nx) * g.ny), kBlock, 0, st>>>(g, bc, in, out);
+  MomentumT<double>(st, g, bc, a);
+}
+void LaunchMomentum(hipStream_t st, const Geo& g, const Bc& bc, const MomentumArgsF& a) {
+  MomentumT<float>(st, g, bc, a);
+}
+template <class T>
+void RhieChowT(hipStream_t st, const Geo& g, const Bc& bc, const double* u0, const double* u1,
+               const T* gpx, const T* gpy, const T* ffrx, const T* ffry, const T* ffex,
+               const T* ffey, const double* p, const double* ap, double rc, double mvx,
+               double mvy, double* fstar, double* fcoef) {
+  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
+  KRhieChow<T><<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p,
+                                            ap, rc, mvx, mvy, fstar, fcoef);
 }
 void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                     const double* u1, const double* gpx, const double* gpy, const double* ffrx,
                     const double* ffry, const double* ffex, const double* ffey, const double* p,
                     const double* ap, double rc, double mvx, double mvy, double* fstar,
                     double* fcoef) {
-  long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
-  KRhieChow<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap,
-                                         rc, mvx, mvy, fstar, fcoef);
+  RhieChowT<double>(st, g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap, rc, mvx, mvy,
+                    fstar, fcoef);
+}
+void LaunchRhieChow(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
+                    const double* u1, const float* gpx, const float* gpy, const float* ffrx,
+                    const float* ffry, const float* ffex, const float* ffey, const double* p,
+                    const double* ap, double rc, double mvx, double mvy, double* fstar,
+                    double* fcoef) {
+  RhieChowT<float>(st, g, bc, u0, u1, gpx, gpy, ffrx, ffry, ffex, ffey, p, ap, rc, mvx, mvy,
+                   fstar, fcoef);
 }
 void LaunchPressureRhs(hipStream_t st, const Geo& g, const double* fstar, double* k,
                        bool negate) {
@@ -1437,18 +1538,26 @@ void LaunchCorrectCells(hipStream_t st, const Geo& g, const double* pc, const do
   KMaxFold<<<1, kFoldThreads, 0, st>>>(part, int((n + kBlock - 1) / kBlock), rs);
 }
 void LaunchCorrectFaces(hipStream_t st, const Geo& g, const double* fstar, const double* fcoef,
-                        const double* pc, double* flux) {
+                        const double* pc, double* flux, float* flux32) {
   long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
-  KCorrectFaces<<<Grid(nf), kBlock, 0, st>>>(g, fstar, fcoef, pc, flux);
+  KCorrectFaces<<<Grid(nf), kBlock, 0, st>>>(g, fstar, fcoef, pc, flux, flux32);
 }
 void LaunchInitialFlux(hipStream_t st, const Geo& g, const Bc& bc, const double* u0,
                        const double* u1, double mvx, double mvy, double* flux) {
   long nf = long(g.nfx) + long(g.nx) * (g.ny + 1);
   KInitialFlux<<<Grid(nf), kBlock, 0, st>>>(g, bc, u0, u1, mvx, mvy, flux);
 }
-void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
-                   const double* flux, double dt, double* const un[3],
-                   const double* const bval[3], const int* inlet) {
+__global__ void __launch_bounds__(kBlock)
+KToFloat(long n, const double* __restrict__ a, float* __restrict__ b) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i < n) b[i] = float(a[i]);
+}
+void LaunchToFloat(hipStream_t st, long n, const double* a, float* b) {
+  KToFloat<<<Grid(n), kBlock, 0, st>>>(n, a, b);
+}
+template <class R>
+void AdvStepT(hipStream_t st, const Geo& g, int nf, const double* const u[3], const R* flux,
+              double dt, double* const un[3], const double* const bval[3], const int* inlet) {
   AdvFields fl{};
   for (int q = 0; q < nf; ++q) {
     fl.u[q] = u[q];
@@ -1464,19 +1573,29 @@ void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3
   const unsigned grid = unsigned(((g.nx + kAdvTX - 1) / kAdvTX) * ((g.ny + kAdvTY - 1) / kAdvTY));
   if (any_in) {
     if (nf == 1)
-      KAdvStep<1, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<1, true, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
     else if (nf == 2)
-      KAdvStep<2, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<2, true, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
     else
-      KAdvStep<3, true><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<3, true, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
   } else {
     if (nf == 1)
-      KAdvStep<1, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<1, false, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
     else if (nf == 2)
-      KAdvStep<2, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<2, false, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
     else
-      KAdvStep<3, false><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
+      KAdvStep<3, false, R><<<grid, kBlock, 0, st>>>(g, fl, flux, dt);
   }
+}
+void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
+                   const double* flux, double dt, double* const un[3],
+                   const double* const bval[3], const int* inlet) {
+  AdvStepT<double>(st, g, nf, u, flux, dt, un, bval, inlet);
+}
+void LaunchAdvStep(hipStream_t st, const Geo& g, int nf, const double* const u[3],
+                   const float* flux, double dt, double* const un[3],
+                   const double* const bval[3], const int* inlet) {
+  AdvStepT<float>(st, g, nf, u, flux, dt, un, bval, inlet);
 }
 void LaunchProperties(hipStream_t st, const Geo& g, const Phases& ph, const double* const pd[3],
                       double* const alpha[3], double* rho, double* mu, double* fx, double* fy) {

@@ -54,6 +54,30 @@ KERNELS = {
                       "read p', a_P, p_prev, u0, u1, u_prev (2); write p, u0, u1"),
     "KCorrectFaces": (lambda nx, ny: _b(nx, ny, cells64=1, faces64=3),
                       "read flux*, flux coefficient, p'; write flux"),
+    # ---- the same stages in the mixed path: FP32 intermediates ----
+    "KForceFaces mixed": (lambda nx, ny: _b(nx, ny, cells64=3, faces32=3),
+                          "read f_x, f_y, mu (FP64); write face f_x, f_y, mu (FP32)"),
+    "KRestoredForce mixed": (lambda nx, ny: _b(nx, ny, faces32=1, cells32=2),
+                             "read face f_x, f_y (FP32); write restored force x, y (FP32)"),
+    "KFacesDeriv0Pair mixed": (lambda nx, ny: _b(nx, ny, cells32=2, faces32=2),
+                               "read 2 cell fields; write 2 face fields (FP32)"),
+    "KCellGrads<false> mixed": (lambda nx, ny: _b(nx, ny, cells64=3, cells32=6),
+                                "read p, u0, u1 (FP64); write 3 gradients x 2 components (FP32)"),
+    "KMomentum mixed": (lambda nx, ny: _b(nx, ny, cells64=8, cells32=15, faces64=1, faces32=1),
+                        "read u0, u1, rho, u at t-1 and t (4), volume flux (FP64) and 4 "
+                        "velocity gradients, grad p (2), restored force (2), face mu (FP32); "
+                        "write 5 row coefficients and 2 constants (FP32) and a_P (FP64)"),
+    "KLuRbBlack mixed": (lambda nx, ny: _b(nx, ny, cells32=8),
+                         "read 5 coefficients, 2 constants; write 2 black corrections (FP32)"),
+    "KLuRbRed mixed": (lambda nx, ny: _b(nx, ny, cells32=9, cells64=4),
+                       "read 5 coefficients, 2 constants, 2 black corrections (FP32), 2 bases "
+                       "(FP64); write 2 results (FP64)"),
+    "KRhieChow mixed": (lambda nx, ny: _b(nx, ny, cells64=4, cells32=2, faces64=2, faces32=2),
+                        "read u0, u1, p, a_P (FP64), grad p (FP32), face restored force and "
+                        "force (FP32); write flux* and flux coefficient (FP64)"),
+    "KCorrectFaces mixed": (lambda nx, ny: _b(nx, ny, cells64=1, faces64=3, faces32=1),
+                            "read flux*, flux coefficient, p'; write flux (FP64) and its FP32 "
+                            "copy"),
     # ---- pressure-solve set-up and closing check (cg.hip) ----
     "KSetOperator": (lambda nx, ny: _b(nx, ny, faces64=2, faces32=1, cells64=1, cells32=1),
                      "read the flux coefficients; write k_f (FP64 and FP32), D^-1 (FP64, FP32)"),
@@ -63,6 +87,9 @@ KERNELS = {
     # ---- advection and properties ----
     "KAdvStep<2, false>": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                     "two phase fields: read both fields and the volume flux; write both fields"),
+    "KAdvStep<2, false> mixed": (lambda nx, ny: _b(nx, ny, cells64=4, faces32=1),
+                                 "two phase fields (FP64): read both and the FP32 volume flux; "
+                                 "write both"),
     "KAdvGrad": (lambda nx, ny: _b(nx, ny, cells64=3), "read field; write gradient (2)"),
     "KAdvUpdate": (lambda nx, ny: _b(nx, ny, cells64=4, faces64=1),
                    "read field, gradient (2), volume flux; write field"),
@@ -136,16 +163,19 @@ def _vcycle_bytes(nx, ny, fcg_dots=False):
     return b
 
 
-def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
+def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10, mixed=True):
     """Algorithmic bytes of one step of the headline path (two phases,
-    lu_rb velocity, FCG + V-cycle pressure in mixed precision): the sum over
-    the step's kernels, launches per step times bytes per launch."""
+    lu_rb velocity, FCG + V-cycle pressure; mixed: the FP32 intermediates of
+    the mixed path): the sum over the step's kernels, launches per step times
+    bytes per launch."""
     n, f, _, _ = sizes(nx, ny)
     w = F64
+    m = " mixed" if mixed else ""
     start = w * 3 * n + kernel_bytes("KExtrapolate", nx, ny) \
-        + kernel_bytes("KForceFaces", nx, ny) + kernel_bytes("KRestoredForce", nx, ny) \
-        + kernel_bytes("KFacesDeriv0Pair", nx, ny)                 # NaN check, guesses, faces
-    per_iter = sum(kernel_bytes(k, nx, ny) for k in (
+        + kernel_bytes("KForceFaces" + m, nx, ny) + kernel_bytes("KRestoredForce" + m, nx, ny) \
+        + kernel_bytes("KFacesDeriv0Pair" + m, nx, ny)             # NaN check, guesses, faces
+    per_iter = sum(kernel_bytes(k + (m if k != "KPressureRhs" and k != "KCorrectCells" else ""),
+                                nx, ny) for k in (
         "KCellGrads<false>", "KMomentum", "KLuRbBlack", "KLuRbRed", "KRhieChow", "KPressureRhs",
         "KCorrectCells", "KCorrectFaces"))
     # operator set-up (the multigrid's finest level shares the FP32 arrays;
@@ -156,7 +186,7 @@ def headline_step_bytes(nx, ny, simple_iters, fcg_iters, substeps=10):
     # (the update of each iteration runs in the next V-cycle's first kernel)
     per_fcg = kernel_bytes("KFcgSpmv", nx, ny) + _vcycle_bytes(nx, ny, fcg_dots=True)
     finish = w * 3 * n                                             # NaN check
-    advection = substeps * kernel_bytes("KAdvStep<2, false>", nx, ny)
+    advection = substeps * kernel_bytes("KAdvStep<2, false>" + m, nx, ny)
     # density and viscosity, two smoothing passes each in one launch
     properties = kernel_bytes("KProperties", nx, ny) + 2 * kernel_bytes("KSmooth2", nx, ny)
     return start + simple_iters * per_iter + fcg_iters * per_fcg + finish + advection + properties

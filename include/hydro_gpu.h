@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define HG_ABI_VERSION 3
+#define HG_ABI_VERSION 4
 #define HG_MAX_PHASES 3
 
 typedef enum hg_status {
@@ -52,7 +52,12 @@ typedef enum hg_solver {
 
 typedef enum hg_precision {
   HG_PREC_FP64 = 0,           /* every kernel in FP64 */
-  HG_PREC_MIXED = 1           /* FP32 stencils / SpMV, FP64 residual, dots and updates */
+  HG_PREC_MIXED = 1           /* the step: FP32 per-step face fields, cell gradients,
+                                 momentum row coefficients, lu_rb velocity correction and
+                                 advection limiter; FP64 state, row constants (residuals),
+                                 pressure system, Krylov vectors, dots and corrections.
+                                 The pressure solve: FP32 preconditioner (cg: FP32 inner
+                                 PCG) under FP64 FCG / iterative refinement */
 } hg_precision;
 
 typedef enum hg_bc_kind {
@@ -151,6 +156,10 @@ typedef struct hg_config {
   int32_t learned_channels_gpu;        /* channels at the finest level (16) */
   int32_t pressure_warm_start_gpu;     /* 1: a step's first cg / fcg solve starts from the
                                           previous step's correction (1) */
+  int32_t fcg_negate_precond_gpu;      /* check knob of the FCG fallback: 1 makes flexible CG
+                                          apply minus its preconditioner, so (r, z) < 0 on
+                                          the first iteration and the solve continues with
+                                          FP64 Jacobi-PCG to the same tolerance (0) */
   /* output side of the module state */
   int32_t meshvel_output;              /* meshvel_output: the output mesh moves with meshvel
                                           (hydro2d.hpp:1526-1528) (1) */

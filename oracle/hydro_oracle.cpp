@@ -1943,6 +1943,115 @@ int or_poisson_solve(int nx, int ny, const double* kx, const double* ky, const d
   return HG_OK;
 }
 
+// ---- CPU baselines of the pressure solve to a relative residual --------------
+// The face-form system A p = b of or_poisson_solve (b projected to zero mean),
+// solved from p = 0 until ||b - A p||_2 <= tol ||b||_2:
+//  method 0: the reference's pressure algorithm, lexicographic SOR
+//            (GaussSeidel::Solve, linear.hpp:692-709: value = -(k + sum) / c,
+//            x += omega (value - x)), single-threaded, the residual checked
+//            every `check_every` sweeps instead of the max-update test;
+//  method 1: Jacobi-preconditioned CG with every loop and dot product under
+//            OpenMP (the strong CPU baseline; no reference counterpart).
+// Stops after max_iters sweeps / iterations.  *seconds: wall time of the
+// solve (residual checks included).
+int or_poisson_baseline(int nx, int ny, const double* kx, const double* ky, const double* rhs,
+                        int method, double omega, double tol, int max_iters, int check_every,
+                        double* p, double* seconds, int* iters, double* rel_res) {
+  const long N = long(nx) * ny;
+  Vec w(N), e(N), s_(N), n(N), c(N), b(N);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const long r = i + long(nx) * j;
+      w[r] = -kx[size_t(i) + size_t(nx + 1) * j];
+      e[r] = -kx[size_t(i + 1) + size_t(nx + 1) * j];
+      s_[r] = -ky[size_t(i) + size_t(nx) * j];
+      n[r] = -ky[size_t(i) + size_t(nx) * (j + 1)];
+      c[r] = -(((w[r] + e[r]) + s_[r]) + n[r]);
+      b[r] = rhs[r];
+    }
+  double mean = 0.;
+  for (double v : b) mean += v;
+  mean /= double(N);
+  for (double& v : b) v -= mean;
+  auto apply = [&](const Vec& x, Vec& y) {
+#pragma omp parallel for
+    for (long r = 0; r < N; ++r) {
+      const int i = int(r % nx), j = int(r / nx);
+      double v = c[r] * x[r];
+      if (j > 0) v += s_[r] * x[r - nx];
+      if (i > 0) v += w[r] * x[r - 1];
+      if (i < nx - 1) v += e[r] * x[r + 1];
+      if (j < ny - 1) v += n[r] * x[r + nx];
+      y[r] = v;
+    }
+  };
+  auto dot = [&](const Vec& a, const Vec& bb) {
+    double t = 0.;
+#pragma omp parallel for reduction(+ : t)
+    for (long r = 0; r < N; ++r) t += a[r] * bb[r];
+    return t;
+  };
+  const double bnorm = std::sqrt(dot(b, b));
+  Vec x(N, 0.), q(N);
+  auto relres = [&]() {
+    apply(x, q);
+    double t = 0.;
+#pragma omp parallel for reduction(+ : t)
+    for (long r = 0; r < N; ++r) t += (b[r] - q[r]) * (b[r] - q[r]);
+    return bnorm > 0 ? std::sqrt(t) / bnorm : 0.;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  int it = 0;
+  double rel = bnorm > 0 ? 1. : 0.;
+  if (method == 0) {
+    while (rel > tol && it < max_iters) {
+      for (long r = 0; r < N; ++r) {
+        const int i = int(r % nx), j = int(r / nx);
+        double sum = 0.;
+        if (j > 0) sum += s_[r] * x[r - nx];
+        if (i > 0) sum += w[r] * x[r - 1];
+        if (i < nx - 1) sum += e[r] * x[r + 1];
+        if (j < ny - 1) sum += n[r] * x[r + nx];
+        const double value = -(-b[r] + sum) / c[r];
+        x[r] += (value - x[r]) * omega;
+      }
+      ++it;
+      if (it % check_every == 0 || it == max_iters) rel = relres();
+    }
+  } else {
+    Vec r(b), z(N), d(N);
+#pragma omp parallel for
+    for (long i = 0; i < N; ++i) z[i] = r[i] / c[i];
+    d = z;
+    double rz = dot(r, z);
+    while (rel > tol && it < max_iters) {
+      apply(d, q);
+      const double alpha = rz / dot(d, q);
+#pragma omp parallel for
+      for (long i = 0; i < N; ++i) {
+        x[i] += alpha * d[i];
+        r[i] -= alpha * q[i];
+      }
+      ++it;
+      rel = std::sqrt(dot(r, r)) / bnorm;
+      if (rel <= tol) break;
+#pragma omp parallel for
+      for (long i = 0; i < N; ++i) z[i] = r[i] / c[i];
+      const double rz2 = dot(r, z);
+      const double beta = rz2 / rz;
+      rz = rz2;
+#pragma omp parallel for
+      for (long i = 0; i < N; ++i) d[i] = z[i] + beta * d[i];
+    }
+    rel = relres();
+  }
+  *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *iters = it;
+  *rel_res = rel;
+  std::copy(x.begin(), x.end(), p);
+  return 0;
+}
+
 int or_geometry(const or_ctx* o, double* cx, double* cy, double* cv, double* fx, double* fy,
                 double* fa, double* sx, double* sy) {
   const Mesh& m = o->m;

@@ -62,6 +62,12 @@ int or_poisson_solve(int nx, int ny, const double* kx, const double* ky, const d
 
 /* Reference geometry of the uniform mesh (mesh.hpp:722-736, mesh2d.hpp:215-245),
  * for the geometry tests: cell centers/volumes, face centers/areas/surfaces. */
+// CPU baselines of the pressure solve to a relative residual: method 0 serial
+// lexicographic SOR (the reference's algorithm, residual checked every
+// check_every sweeps), method 1 OpenMP Jacobi-PCG.
+int or_poisson_baseline(int nx, int ny, const double* kx, const double* ky, const double* rhs,
+                        int method, double omega, double tol, int max_iters, int check_every,
+                        double* p, double* seconds, int* iters, double* rel_res);
 int or_geometry(const or_ctx* ctx, double* cell_cx, double* cell_cy, double* cell_vol,
                 double* face_cx, double* face_cy, double* face_area, double* face_sx,
                 double* face_sy);

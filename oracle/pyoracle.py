@@ -51,6 +51,9 @@ def load():
         "or_poisson_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_dp] * 4 +
                              [_P(capi.HgSolveOpts), _P(capi.HgSolveStats)]),
         "or_geometry": (ctypes.c_int, [vp] + [_dp] * 8),
+        "or_poisson_baseline": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_dp] * 3 +
+                                [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_int, _dp, _dp, _P(ctypes.c_int), _dp]),
         "or_bdf_coeffs": (None, [ctypes.c_double, ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
@@ -182,6 +185,28 @@ def poisson_solve(nx, ny, kx, ky, rhs, tol=1e-8, max_iters=100000):
     if rc != 0:
         raise RuntimeError(f"or_poisson_solve -> {rc}")
     return p, st
+
+
+def poisson_baseline(nx, ny, kx, ky, rhs, method, tol=1e-8, max_iters=100000, omega=1.9,
+                     check_every=10):
+    """CPU pressure solve of the face-form system to a relative residual:
+    method "sor" (the reference's lexicographic SOR, one thread, residual
+    checked every `check_every` sweeps) or "pcg" (OpenMP Jacobi-PCG).
+    Returns (p, seconds, iterations, relative residual)."""
+    lib = load()
+    kx = np.ascontiguousarray(kx, dtype=np.float64)
+    ky = np.ascontiguousarray(ky, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    p = np.zeros(nx * ny)
+    sec, rel = ctypes.c_double(), ctypes.c_double()
+    it = ctypes.c_int()
+    m = {"sor": 0, "pcg": 1}[method]
+    rc = lib.or_poisson_baseline(nx, ny, capi.dptr(kx), capi.dptr(ky), capi.dptr(rhs), m, omega,
+                                 tol, max_iters, check_every, capi.dptr(p), ctypes.byref(sec),
+                                 ctypes.byref(it), ctypes.byref(rel))
+    if rc != 0:
+        raise RuntimeError(f"or_poisson_baseline -> {rc}")
+    return p, sec.value, it.value, rel.value
 
 
 def bdf_coeffs(dt, second_order=True):

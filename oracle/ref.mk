@@ -6,6 +6,8 @@
 # Outputs (git-ignored, travel with the tree):
 #   oracle/_ref/hydro_ref        -O2 -ffp-contract=off: parity fixtures
 #   oracle/_ref/hydro_ref_fast   -O3 -march=x86-64-v3 -fopenmp: CPU baseline
+#   oracle/_ref/hydro_ref_gpu    (make ... gpu) the parity driver with the
+#                                reference's solvers dispatched to libhydrogpu.so
 #                                (the reference builds with -O3 -march=native,
 #                                CMakeLists.txt:69-74; x86-64-v3 keeps the
 #                                binary runnable on other x86 hosts)
@@ -22,6 +24,12 @@ COMMON := -std=c++11 -w -I$(SRC) -DMODULE_HYDRO_2D -fopenmp -pthread
 
 all: $(OUT)/hydro_ref $(OUT)/hydro_ref_fast $(OUT)/general.hydroconf
 
+# hydro_ref_gpu: the same driver with the reference's pressure (and velocity)
+# LinearSolver replaced by the GpuLinearSolver adapter over libhydrogpu.so
+# (oracle/ref_gpu_dispatch.cpp); built when the library exists.
+GPU_LIB := hydro_amd/libhydrogpu.so
+gpu: $(OUT)/hydro_ref_gpu
+
 # The reference binary reads every parameter it uses; its defaults file
 # goes next to it (an output of this build, not a committed file).
 $(OUT)/general.hydroconf: $(REF)/examples/general.hydroconf
@@ -36,4 +44,10 @@ $(OUT)/hydro_ref_fast: oracle/ref_shim.cpp $(REF_SRCS)
 	mkdir -p $(OUT)
 	$(CXX) $(COMMON) -O3 -march=x86-64-v3 -o $@ oracle/ref_shim.cpp $(REF_SRCS)
 
-.PHONY: all
+$(OUT)/hydro_ref_gpu: oracle/ref_shim.cpp oracle/ref_gpu_dispatch.cpp $(REF_SRCS) $(GPU_LIB)
+	mkdir -p $(OUT)
+	$(CXX) $(COMMON) -O2 -ffp-contract=off -Iinclude -o $@ oracle/ref_shim.cpp \
+	  oracle/ref_gpu_dispatch.cpp $(REF_SRCS) -Lhydro_amd -lhydrogpu \
+	  -Wl,-rpath,'$$ORIGIN/../../hydro_amd'
+
+.PHONY: all gpu

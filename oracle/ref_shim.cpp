@@ -10,7 +10,7 @@
 // instantiations (where access checks do not apply), which changes no
 // reference file and no reference behaviour.
 //
-// usage: hydro_ref <script> <nsteps> <dump-prefix> [dump-every]
+// usage: hydro_ref <script> <nsteps> <dump-prefix> [dump-every] [hook argument]
 //   script: a file of `set <type> <name> <value>` lines
 //   writes <prefix>.<step>.bin for each dumped step:
 //     header: int32 nx, ny, num_phases, step; then float64 arrays
@@ -46,6 +46,12 @@
 
 using Mesh = geom::geom2d::MeshStructured<double>;
 using Module = hydro2D_uniform_MPI::hydro<Mesh>;
+
+// Hook run right after the module is constructed (null in hydro_ref; set by
+// oracle/ref_gpu_dispatch.cpp in hydro_ref_gpu), and the driver's optional
+// fifth argument, which the hook reads.
+void (*g_after_construct)(Module&, TExperiment&) = nullptr;
+const char* g_extra_arg = nullptr;
 
 // Member access through explicit instantiation.
 template <class Tag, typename Tag::type M>
@@ -157,6 +163,8 @@ int main(int argc, char** argv) {
       ApplySet(ex, line);
     }
     Module h(&ex);
+    g_extra_arg = argc > 5 ? argv[5] : nullptr;
+    if (g_after_construct) g_after_construct(h, ex);
     if (every > 0 && prefix != "-") Dump(h, prefix + ".0.bin", 0);
     for (int s = 1; s <= nsteps; ++s) {
       auto t0 = std::chrono::steady_clock::now();

@@ -122,6 +122,9 @@ LONG = {
     # the pressure solver on every SIMPLE iteration)
     "ref_cavity64_200": (workloads.cavity(64), 200, (200,)),
     "ref_dam256x128_50": (workloads.dam(256, 128), 50, (50,)),
+    # C2 in full: 500 steps (about 40 minutes of the reference module; from
+    # about step 200 its SOR runs to the 1000-sweep limit every iteration)
+    "ref_dam256x128_500": (workloads.dam(256, 128), 500, (100, 200, 300, 400, 500)),
 }
 
 
