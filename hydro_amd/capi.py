@@ -1,4 +1,5 @@
-"""ctypes mirror of include/hydro_gpu.h and include/hydro_output.h (the C-ABI
+"""ctypes mirror of include/hydro_gpu.h, include/hydro_output.h and
+include/hydro_noalloc.h (the C-ABI
 of libhydrogpu.so).
 
 The structures below follow the header field by field; `HG_ABI_VERSION`
@@ -197,7 +198,8 @@ class HgTrainStats(ctypes.Structure):
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libhydrogpu.so")
 
-# Every entry point include/hydro_gpu.h and include/hydro_output.h declare.
+# Every entry point include/hydro_gpu.h, include/hydro_output.h and
+# include/hydro_noalloc.h declare.
 EXPORTS = [
     "hg_get_config", "hg_get_progress", "hg_get_iteration_history",
     "hg_output_config_default", "hg_output_config_parse", "hg_output_open",
@@ -208,7 +210,7 @@ EXPORTS = [
     "hg_fluid_start_step", "hg_fluid_iterate", "hg_fluid_is_converged",
     "hg_fluid_finish_step", "hg_advect_substep", "hg_update_properties",
     "hg_calc_stats", "hg_step", "hg_last_step_timing", "hg_time_kernel", "hg_batch_run",
-    "hg_stencil_solve", "hg_poisson_create", "hg_poisson_destroy",
+    "hg_stencil_solve", "hg_stencil_solve_noalloc", "hg_poisson_create", "hg_poisson_destroy",
     "hg_poisson_solve", "hg_poisson_time_kernel",
     "hg_train_opts_default", "hg_poisson_train", "hg_train_pressure_net",
     "hg_net_create", "hg_net_destroy", "hg_net_num_weights", "hg_net_configure",
@@ -265,6 +267,8 @@ def _declare(lib):
                                         _P(HgStepStats), _dp]),
         "hg_stencil_solve": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32] + [_dp] * 7 +
                              [_P(HgSolveOpts), _P(HgSolveStats)]),
+        "hg_stencil_solve_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32] + [vp] * 7 +
+                                     [_P(HgSolveOpts), _P(HgSolveStats), vp]),
         "hg_poisson_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _dp, _dp, _P(vp)]),
         "hg_poisson_destroy": (None, [vp]),
         "hg_poisson_solve": (ctypes.c_int, [vp, _dp, _dp, _P(HgSolveOpts), _P(HgSolveStats)]),

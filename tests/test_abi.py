@@ -1,7 +1,8 @@
 """The C-ABI library and its host logic, without a GPU.
 
-* libhydrogpu.so loads and exports every function include/hydro_gpu.h and
-  include/hydro_output.h declare (no compute calls here).
+* libhydrogpu.so loads and exports every function include/hydro_gpu.h,
+  include/hydro_output.h and include/hydro_noalloc.h declare (no compute
+  calls here).
 * The .hydroconf reader keeps the reference's defaults and semantics.
 * Calls that need a device fail loudly when none is visible.
 """
@@ -15,7 +16,8 @@ from hydro_amd import capi
 from tests.conftest import gpu_available
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("hydro_gpu.h", "hydro_output.h")]
+HEADERS = [os.path.join(ROOT, "include", h)
+           for h in ("hydro_gpu.h", "hydro_output.h", "hydro_noalloc.h")]
 
 
 def declared_functions():
