@@ -13,7 +13,7 @@
 #include <cmath>
 #include <vector>
 
-#include "device_common.h"
+#include "geometry.h"
 
 namespace hg {
 
