@@ -125,7 +125,9 @@ class HgStepStats(ctypes.Structure):
                 ("convergence_indicator", ctypes.c_double),
                 ("advection_substeps", ctypes.c_int32),
                 ("pressure_solve_iters_total", ctypes.c_int64),
-                ("time", ctypes.c_double), ("dt", ctypes.c_double)]
+                ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("pressure_solve_diff_max", ctypes.c_double),
+                ("pressure_converged", ctypes.c_int32)]
 
 
 class HgStats(ctypes.Structure):

@@ -312,7 +312,8 @@ __global__ void __launch_bounds__(kB)
 KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict__ ky,
          const float* __restrict__ z, const double* __restrict__ pold, double* __restrict__ pnew,
          double* __restrict__ q, Parts pt, CgScalars* sc) {
-  if (sc->done) return;
+  // the stop flag is tested once the first tile's loads are in flight
+  const int stopped = sc->done;
   const double beta = sc->beta;  // KFcgBeta
   // p is formed once per cell of the tile and its one-cell halo, in LDS, and
   // the stencil reads it there; each thread's loads (two halo-region cells,
@@ -345,6 +346,7 @@ KFcgSpmv(Walk w, int k, const double* __restrict__ kx, const double* __restrict_
     const int fw = in ? i + (nx + 1) * j : 0, fs = in ? i + nx * j : 0;
     const double kw = in ? kx[fw] : 0., ke = in ? kx[fw + 1] : 0.;
     const double ks = in ? ky[fs] : 0., kn = in ? ky[fs + nx] : 0.;
+    if (stopped) return;  // uniform; before the block's first barrier
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const int h = threadIdx.x + s * kB;

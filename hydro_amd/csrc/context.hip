@@ -180,6 +180,13 @@ struct hg_ctx {
   double *fl_tc, *fl_tp, *fl_ic, *fl_ip;
   // buffer freed by the last FinishStep's rotation (time layers)
   double *u_spare[2] = {nullptr, nullptr}, *p_spare = nullptr, *fl_spare = nullptr;
+  // the four buffers of each layered field (u0, u1, p, flux)
+  double* layer_bufs[4][4] = {};
+  // The mixed path with guess_extrapolation 0 takes the step's guess in
+  // place: the first SIMPLE iteration reads time_curr itself as its previous
+  // iterate instead of a copy of it (the copy is KExtrapolate's whole work
+  // when the extrapolation is 0)
+  bool guess_in_place = false;
   // properties
   double *rho, *rho_s, *mu, *mu_s, *fx, *fy, *tmp1, *tmp2;
   double *pd[3], *pd_new[3], *alpha[3];
@@ -598,11 +605,19 @@ void StartStep(hg_ctx* c) {
   if (c->fl_ic == c->fl_tc) c->fl_ic = c->fl_spare;
   for (int d = 0; d < 2; ++d)
     if (c->u_ic[d] == c->u_tc[d]) c->u_ic[d] = c->u_spare[d];
-  ExtrapLayers l{{c->u_tc[0], c->u_tc[1], c->p_tc, c->fl_tc},
-                 {c->u_tp[0], c->u_tp[1], c->p_tp, c->fl_tp},
-                 {c->u_ic[0], c->u_ic[1], c->p_ic, c->fl_ic}};
-  KExtrapolate<<<G(c->nc + c->nf), 256, 0, c->st>>>(c->nc, c->nf, l,
-                                                     c->cfg.guess_extrapolation);
+  if (c->guess_in_place) {
+    // iter_curr = time_curr (the guess a + (a - b) 0 is a, up to the sign of
+    // a zero); Iterate gives the iteration a buffer of its own
+    c->p_ic = c->p_tc;
+    c->fl_ic = c->fl_tc;
+    for (int d = 0; d < 2; ++d) c->u_ic[d] = c->u_tc[d];
+  } else {
+    ExtrapLayers l{{c->u_tc[0], c->u_tc[1], c->p_tc, c->fl_tc},
+                   {c->u_tp[0], c->u_tp[1], c->p_tp, c->fl_tp},
+                   {c->u_ic[0], c->u_ic[1], c->p_ic, c->fl_ic}};
+    KExtrapolate<<<G(c->nc + c->nf), 256, 0, c->st>>>(c->nc, c->nf, l,
+                                                       c->cfg.guess_extrapolation);
+  }
   c->flux32_valid = false;  // iter_curr's flux is the new guess
   // inputs that stay fixed through the SIMPLE loop (fluid.hpp:602-641)
   if (c->mixed) {
@@ -853,6 +868,22 @@ void Iterate(hg_ctx* c, hg_iter_stats* out, bool defer_rs = false) {
   std::swap(c->fl_ip, c->fl_ic);
   std::swap(c->u_ip[0], c->u_ic[0]);
   std::swap(c->u_ip[1], c->u_ic[1]);
+  if (c->guess_in_place) {
+    // with the guess read in place, iter_curr can now name a time layer:
+    // write the iteration into the buffer no other layer holds
+    auto own = [](double*& ic, double* tp, double* tc, double* ip, double* const bufs[4]) {
+      if (ic != tp && ic != tc && ic != ip) return;
+      for (int k = 0; k < 4; ++k)
+        if (bufs[k] != tp && bufs[k] != tc && bufs[k] != ip) {
+          ic = bufs[k];
+          return;
+        }
+    };
+    own(c->u_ic[0], c->u_tp[0], c->u_tc[0], c->u_ip[0], c->layer_bufs[0]);
+    own(c->u_ic[1], c->u_tp[1], c->u_tc[1], c->u_ip[1], c->layer_bufs[1]);
+    own(c->p_ic, c->p_tp, c->p_tc, c->p_ip, c->layer_bufs[2]);
+    own(c->fl_ic, c->fl_tp, c->fl_tc, c->fl_ip, c->layer_bufs[3]);
+  }
   if (c->mixed)
     MixedMomentum(c, is);
   else
@@ -1082,6 +1113,18 @@ hg_status hg_create(const hg_config* cfg_in, hg_ctx** out) {
     }
     c->p_tc = c->Alloc(nc); c->p_tp = c->Alloc(nc); c->p_ic = c->Alloc(nc); c->p_ip = c->Alloc(nc);
     c->fl_tc = c->Alloc(nf); c->fl_tp = c->Alloc(nf); c->fl_ic = c->Alloc(nf); c->fl_ip = c->Alloc(nf);
+    for (int d = 0; d < 2; ++d) {
+      double* b[4] = {c->u_tc[d], c->u_tp[d], c->u_ic[d], c->u_ip[d]};
+      std::copy(b, b + 4, c->layer_bufs[d]);
+    }
+    {
+      double* bp[4] = {c->p_tc, c->p_tp, c->p_ic, c->p_ip};
+      double* bf[4] = {c->fl_tc, c->fl_tp, c->fl_ic, c->fl_ip};
+      std::copy(bp, bp + 4, c->layer_bufs[2]);
+      std::copy(bf, bf + 4, c->layer_bufs[3]);
+    }
+    c->guess_in_place = cfg.precision_gpu == HG_PREC_MIXED && cfg.guess_extrapolation == 0. &&
+                        cfg.fluid_enable;
     for (double** p : {&c->rho, &c->rho_s, &c->mu, &c->mu_s, &c->fx, &c->fy, &c->tmp1, &c->tmp2})
       *p = c->Alloc(nc);
     for (int i = 0; i < 3; ++i) {
@@ -1401,19 +1444,30 @@ hg_status hg_step(hg_ctx* c, hg_step_stats* s) {
     HG_CHECK(hipEventRecord(c->ev[0], c->st));
     if (c->cfg.dt_auto) AutoTimeStep(c);
     StartStep(c);
-    bool iterated = false;
+    bool iterated = false, last_deferred = false;
+    st.pressure_converged = 1;
     if (c->cfg.fluid_enable) {
       while (!IsConverged(c)) {
         hg_iter_stats is;
         // when the iteration limit ends the loop, the indicator is not
         // needed to decide and is read back at a later synchronisation
-        Iterate(c, &is, c->fluid_iter + 1 >= c->cfg.num_iterations_limit);
+        last_deferred = c->fluid_iter + 1 >= c->cfg.num_iterations_limit;
+        Iterate(c, &is, last_deferred);
         st.pressure_solve_iters_total += is.pressure_solve_iters;
+        if (!last_deferred) {
+          st.pressure_solve_diff_max = std::max(st.pressure_solve_diff_max, is.pressure_solve_diff);
+          st.pressure_converged = st.pressure_converged && is.pressure_converged;
+        }
         iterated = true;
       }
     }
     st.simple_iterations = c->fluid_iter;
     FinishStep(c);
+    if (last_deferred) {  // FinishStep's synchronisation settled the last solve
+      st.pressure_solve_diff_max =
+          std::max(st.pressure_solve_diff_max, c->last_iter.pressure_solve_diff);
+      st.pressure_converged = st.pressure_converged && c->last_iter.pressure_converged;
+    }
     HG_CHECK(hipEventRecord(c->ev[1], c->st));
     if (c->cfg.advection_enable) {
       while (c->t_adv < c->t_fluid - 0.5 * c->dt_adv) {

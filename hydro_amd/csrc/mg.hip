@@ -287,8 +287,10 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done,
       FcgFusion fcg, int stop_test) {
   static_assert(!UP || std::is_same<RT, double>::value, "the update is on the FP64 residual");
-  if (Done(done)) return;
-  if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
+  // the stop flag is read with the data: its load is issued first and
+  // tested once the tile's loads are in flight, so a working launch does
+  // not wait on it before the rest
+  const int stopped = done ? *done : 0;
   using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
   __shared__ float s1[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
@@ -302,6 +304,8 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   SlotCell sc[S::N];
   RT rv[S::N];
   LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
+  if (stopped) return;
+  if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
   if constexpr (UP) FoldedUpdate<TY>(fcg, sc, rv, nx);
 #pragma unroll
   for (int s = 0; s < S::N; ++s)
@@ -351,7 +355,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
     int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done, FcgFusion fcg,
     int dots) {
-  if (Done(done)) return;
+  const int stopped = done ? *done : 0;  // tested once the loads are in flight
   using S = TileSlots<TY>;
   constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
   // coarse cells i0/2 - 2 .. i0/2 + kTX/2 + 1 (parents of the halo-2 region
@@ -394,6 +398,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
 #pragma unroll
   for (int s = 0; s < S::NT; ++s)
     qo[s] = (dots && fcg.qold && sc[s].in) ? fcg.qold[sc[s].i + nx * sc[s].j] : 0.;
+  if (stopped) return;
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int q = threadIdx.x + k * kTileThreads;
@@ -547,7 +552,7 @@ static_assert(kInKernelMax % kCcThreads == 0 || kInKernelMax < kCcThreads, "slot
 
 __global__ void __launch_bounds__(kCcThreads)
 KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
-  if (Done(done)) return;
+  const int stopped = done ? *done : 0;  // tested once the loads are in flight
   __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int t = threadIdx.x;
   const int nlev = cs.count - 1;
@@ -595,6 +600,7 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
     tn = cs.ky[nlev][iT + nxT * (jT + 1)];
     tdi = cs.dinv[nlev][t];
   }
+  if (stopped) return;
   const int n0 = cs.nx[0] * cs.ny[0];
 #pragma unroll
   for (int m = 0; m < kCcSlots; ++m) {

@@ -221,6 +221,10 @@ typedef struct hg_step_stats {
   int64_t pressure_solve_iters_total;
   double time;                      /* fluid time after the step */
   double dt;                        /* the step's time step (dt_auto: chosen by the step) */
+  double pressure_solve_diff_max;   /* the largest final pressure_solve_diff of the step's
+                                       solves (cg / fcg: true FP64 relative residual) */
+  int32_t pressure_converged;       /* 1 if every pressure solve of the step met its
+                                       tolerance */
 } hg_step_stats;
 
 typedef struct hg_stats {           /* CalcStat (hydro2d.hpp:1432-1529), per phase */

@@ -1784,7 +1784,7 @@ static double AutoTimeStep(const or_ctx* o) {
 
 int or_step(or_ctx* o, hg_step_stats* stats) {
   ScopedTimer t_step(&o->timers, "step");
-  hg_step_stats s;
+  hg_step_stats s{};
   std::memset(&s, 0, sizeof(s));
   int rc;
   if (o->cfg.dt_auto) {
@@ -1799,6 +1799,7 @@ int or_step(or_ctx* o, hg_step_stats* stats) {
     ScopedTimer t(&o->timers, "step.fluid");
     rc = or_fluid_start_step(o);
     if (rc) return rc;
+    s.pressure_converged = 1;
     if (o->cfg.fluid_enable) {
       while (!or_fluid_is_converged(o)) {
         hg_iter_stats is;
@@ -1806,6 +1807,8 @@ int or_step(or_ctx* o, hg_step_stats* stats) {
         if (rc) return rc;
         s.pressure_solve_iters_total += is.pressure_solve_iters;
         s.convergence_indicator = is.convergence_indicator;
+        s.pressure_solve_diff_max = std::max(s.pressure_solve_diff_max, is.pressure_solve_diff);
+        s.pressure_converged = s.pressure_converged && is.pressure_converged;
       }
     }
     s.simple_iterations = o->fluid_iter;

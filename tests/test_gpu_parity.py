@@ -6,6 +6,8 @@ oracle.  On power-of-two meshes every geometric quantity is exact, so the
 exact-order solvers (lu, gauss_seidel, jacobi, lu_rb) give identical bits;
 the bar written in the north star is 1e-10 relative L2, checked as well.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -186,12 +188,20 @@ def test_stencil_solvers_bitwise(gpu, solver):
     assert np.array_equal(xg, xo)
 
 
-@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50"])
+# C2 in full (500 steps) takes the exact-order SOR through 2.7 M sweeps (the
+# reference's own solver runs to its 1000-sweep limit on every SIMPLE
+# iteration from about step 200): minutes on one workgroup.  It runs when
+# HYDRO_LONG_C2=1 is set, outside the default suite.
+LONG_C2 = pytest.param("ref_dam256x128_500", marks=pytest.mark.skipif(
+    os.environ.get("HYDRO_LONG_C2") != "1", reason="C2 500 steps: set HYDRO_LONG_C2=1"))
+
+
+@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50", LONG_C2])
 def test_long_trajectory_matches_reference(gpu, name):
-    """C1 (cavity 64x64, 200 steps) and the first 50 steps of C2 (dam
-    256x128): every dumped field bit for bit against the reference module's
-    own run (SHA-256 digests in tests/golden), same SIMPLE iterations per
-    step and same total pressure sweeps."""
+    """C1 (cavity 64x64, 200 steps) and C2 (dam 256x128; its first 50 steps
+    by default, all 500 with HYDRO_LONG_C2=1): every dumped field bit for bit
+    against the reference module's own run (SHA-256 digests in tests/golden),
+    same SIMPLE iterations per step and same total pressure sweeps."""
     import json
     import os
     from tests.test_oracle_reference import GOLDEN, check_long_fixture, field_digests

@@ -153,7 +153,10 @@ def check_long_fixture(fix, digests_at_step, step_stats):
         assert not bad, (step, bad)
 
 
-@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50"])
+@pytest.mark.parametrize("name", ["ref_cavity64_200", "ref_dam256x128_50", pytest.param(
+    "ref_dam256x128_500", marks=pytest.mark.skipif(
+        os.environ.get("HYDRO_LONG_C2") != "1",
+        reason="C2 500 steps (about 40 minutes on the CPU): set HYDRO_LONG_C2=1"))])
 def test_oracle_long_trajectory(name):
     import json
     fix = json.load(open(os.path.join(GOLDEN, name + ".json")))
