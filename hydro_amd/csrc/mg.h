@@ -45,6 +45,7 @@ struct CoarseSet {
   int nx[kMaxInKernelLevels], ny[kMaxInKernelLevels];
   const float *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
   float *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];
+  float* out;  // the first level's correction (the caller's `out`)
 };
 
 class MgPrecond : public Precond {

@@ -537,7 +537,7 @@ __device__ __forceinline__ void BlockSync() {
 // coefficients, inverse diagonals and prolongation weights are loaded into
 // registers once, so the phases of the cycle touch only registers and LDS
 // (iterates, residuals) between barriers.  Input: r of the first level
-// (global); output: its correction (global xb of the first level).
+// (global); output: its correction (global, cs.out).
 constexpr int kCcThreads = 1024;
 constexpr int CcCells(int l) { return kInKernelMax >> (2 * l); }
 constexpr int CcSlots(int l) { return CcCells(l) > kCcThreads ? CcCells(l) / kCcThreads : 1; }
@@ -729,7 +729,7 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
 #pragma unroll
   for (int m = 0; m < kCcSlots; ++m) {
     const int c = t + m * kCcThreads;
-    if (c < n0) cs.xb[0][c] = sxa[c];
+    if (c < n0) cs.out[c] = sxa[c];
   }
 }
 
@@ -848,8 +848,7 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
       cs.xa[k] = C.xa;
       cs.xb[k] = C.xb;
     }
-    // the caller prolongs from this level's xb
-    (void)out;
+    cs.out = out;
     KCoarseCycle<<<1, 1024, 0, st>>>(cs, Omega2{omega1, omega2}, done);
     return;
   }
@@ -907,10 +906,16 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
                                                        L.r, Omega2{omega1, omega2}, L.xb, C.nx,
                                                        C.ny, C.r, done, fz,
                                                        (l == 1 && up) ? 1 : 0);
-  Cycle(l + 1, nullptr, C.xb, st, done, (l == 0 && up) ? f : nullptr);
+  // the coarser level's correction goes to its xa: its xb holds the x2 that
+  // its up-leg reads on tile halos while other workgroups write their output
+  Cycle(l + 1, nullptr, C.xa, st, done, (l == 0 && up) ? f : nullptr);
+  // the up-leg reads x2 (L.xb) and the coarse correction on halos that other
+  // workgroups' tiles cover: its output must be a third buffer
+  if (out == L.xb || out == C.xa)
+    throw std::logic_error("MgPrecond: up-leg output aliases its input");
 #define HG_UP(RT, B, TY, TG, R)                                                               \
   KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
-                                                      L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,    \
+                                                      L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,    \
                                                       Omega2{omega1, omega2}, out, done, fz,  \
                                                       fused)
   if (l == 0) {
@@ -953,10 +958,10 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
           C.r, nullptr, FcgFusion{}, 0);
     } else if (k == "KUp<double, true>") {
       KUp<double, true, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xb, C.kx, C.ky, C.nx, C.ny,
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,
           Omega2{omega1, omega2}, L.xa, nullptr, FcgFusion{}, 0);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
-      Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xb, st, nullptr);
+      Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xa, st, nullptr);
     } else {
       throw std::runtime_error("TimeKernel: unknown kernel " + k);
     }

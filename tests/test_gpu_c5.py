@@ -14,7 +14,10 @@ seen.  For every case:
   ||div flux||_2 <= 2e-8 ||b||_2, b the last pressure system's right-hand
   side (the discrete divergence of the predicted flux, fluid.hpp:946-1027);
 * cases with sides <= 128 agree with the CPU oracle (FP64, Jacobi-PCG to the
-  same tolerance) within the mixed path's bounds.
+  same tolerance) within the mixed path's bounds;
+* the eight largest cases, stepped again alone, give bit for bit the fields
+  and per-step pressure iteration counts of the batch (concurrent contexts
+  share nothing, and no kernel's result depends on workgroup timing).
 """
 import re
 
@@ -46,9 +49,11 @@ def test_c5_batch_of_64_cases(gpu):
     hs = [Hydro(t) for t in texts]
     v0 = [compare.phase_volumes(h) for h in hs]
     worst_res = 0.0
+    piters = [[] for _ in hs]
     for s in range(STEPS):
         stats, ms = batch_run(hs, 1, threads=8)
         for i, st in enumerate(stats):
+            piters[i].append(st.pressure_solve_iters_total)
             assert st.simple_iterations >= 1, i
             assert st.pressure_converged == 1, (i, s, st.pressure_solve_diff_max)
             assert st.pressure_solve_diff_max <= 1e-8, (i, s, st.pressure_solve_diff_max)
@@ -85,5 +90,17 @@ def test_c5_batch_of_64_cases(gpu):
     bounds = {"u": 1e-5, "v": 1e-5, "p": 1e-6, "alpha0": 1e-9, "alpha1": 1e-9}
     for k, v in worst.items():
         assert v <= bounds[k], (k, v)
+    # the largest cases alone: the batch's result, bit for bit
+    largest = sorted(range(len(texts)), key=lambda i: -sizes[i][0] * sizes[i][1])[:8]
+    fields = [capi.HG_FIELD_VELOCITY_X, capi.HG_FIELD_VELOCITY_Y, capi.HG_FIELD_PRESSURE,
+              capi.HG_FIELD_VOLUME_FLUX]
+    for i in largest:
+        with Hydro(texts[i]) as h1:
+            alone = [h1.step().pressure_solve_iters_total for _ in range(STEPS)]
+            assert alone == piters[i], (i, sizes[i], alone, piters[i])
+            for f in fields:
+                assert np.array_equal(h1.get(f), hs[i].get(f)), (i, sizes[i], f)
+    print(f"  {len(largest)} largest cases alone: bitwise equal, pressure iterations "
+          f"{[piters[i] for i in largest]}", flush=True)
     for h in hs:
         h.close()
