@@ -25,7 +25,7 @@ ARCH = "gfx950"
 
 # Parity-path sources evaluate every expression in the reference's order:
 # no contraction into FMAs.  The throughput solvers and the network allow it.
-EXACT = {"step_kernels.hip", "solvers.hip", "context.hip", "config.cpp", "output.cpp"}
+EXACT = {"step_kernels.hip", "solvers.hip", "context.hip", "config.cpp", "output.cpp", "step3d.hip"}
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]
@@ -64,7 +64,8 @@ def build_gpu_lib():
 def build_oracle():
     tmp = ORACLE_LIB + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
-          "-o", tmp, os.path.join(ORACLE_DIR, "hydro_oracle.cpp")])
+          "-o", tmp, os.path.join(ORACLE_DIR, "hydro_oracle.cpp"),
+          os.path.join(ORACLE_DIR, "hydro_oracle3d.cpp")])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 
@@ -74,6 +75,7 @@ def build_reference():
     if not os.path.isdir(os.path.join(REFERENCE, "source")):
         return None
     _run(["make", "-s", "-f", "oracle/ref.mk", "-j4", f"REF={REFERENCE}", "all"])
+    _run(["make", "-s", "-f", "oracle/ref.mk", f"REF={REFERENCE}", "ref3d"])
     if os.path.exists(LIB):
         _run(["make", "-s", "-f", "oracle/ref.mk", f"REF={REFERENCE}", "gpu"])
     return os.path.join(ORACLE_DIR, "_ref")

@@ -38,6 +38,10 @@ HG_FIELD_VOLUME_FLUX_ITER = 15
 HG_FIELD_PRESSURE_FACE_COEF = 16
 HG_FIELD_PRESSURE_RHS = 17
 HG_FIELD_TEMPERATURE = 18
+# 3D context (hydro_gpu3d.h)
+HG_FIELD_VELOCITY_Z = 32
+HG_FIELD_FORCE_Z = 33
+HG_SIDE_CLOSE, HG_SIDE_FAR = 4, 5
 
 _d2 = ctypes.c_double * 2
 _d3 = ctypes.c_double * HG_MAX_PHASES
@@ -118,6 +122,16 @@ class HgIterStats(ctypes.Structure):
                 ("pressure_solve_iters", ctypes.c_int32),
                 ("pressure_solve_diff", ctypes.c_double),
                 ("pressure_converged", ctypes.c_int32)]
+
+
+class Hg3Config(ctypes.Structure):
+    """hg3_config (include/hydro_gpu3d.h)."""
+    _fields_ = [("base", HgConfig), ("nz", ctypes.c_int32),
+                ("a_z", ctypes.c_double), ("b_z", ctypes.c_double),
+                ("a1_z", ctypes.c_double), ("b1_z", ctypes.c_double),
+                ("gravity_z", ctypes.c_double), ("force_z", ctypes.c_double),
+                ("initial_velocity_z", ctypes.c_double), ("meshvel_z", ctypes.c_double),
+                ("bc_kind", ctypes.c_int32 * 6), ("bc_velocity", (ctypes.c_double * 3) * 6)]
 
 
 class HgStepStats(ctypes.Structure):
@@ -203,6 +217,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libhydrogpu.so")
 # Every entry point include/hydro_gpu.h, include/hydro_output.h and
 # include/hydro_noalloc.h declare.
 EXPORTS = [
+    "hg3_config_default", "hg3_config_parse", "hg3_create", "hg3_destroy", "hg3_last_error",
+    "hg3_step", "hg3_get_field", "hg3_set_field",
     "hg_get_config", "hg_get_progress", "hg_get_iteration_history",
     "hg_output_config_default", "hg_output_config_parse", "hg_output_open",
     "hg_output_write_results", "hg_output_close", "hg_output_last_error", "hg_vts_write",
@@ -230,6 +246,15 @@ def _declare(lib):
     vp = ctypes.c_void_p
     sig = {
         "hg_config_default": (None, [_P(HgConfig)]),
+        "hg3_config_default": (None, [_P(Hg3Config)]),
+        "hg3_config_parse": (ctypes.c_int, [_P(Hg3Config), ctypes.c_char_p,
+                                            ctypes.c_char_p, ctypes.c_size_t]),
+        "hg3_create": (ctypes.c_int, [_P(Hg3Config), _P(vp)]),
+        "hg3_destroy": (None, [vp]),
+        "hg3_last_error": (ctypes.c_char_p, [vp]),
+        "hg3_step": (ctypes.c_int, [vp, _P(HgStepStats)]),
+        "hg3_get_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
+        "hg3_set_field": (ctypes.c_int, [vp, ctypes.c_int32, _dp, ctypes.c_size_t]),
         "hg_config_parse": (ctypes.c_int, [_P(HgConfig), ctypes.c_char_p,
                                            ctypes.c_char_p, ctypes.c_size_t]),
         "hg_solver_from_name": (ctypes.c_int32, [ctypes.c_char_p]),
@@ -327,6 +352,18 @@ def parse_config(text, cfg=None, lib=None):
     cfg = cfg if cfg is not None else default_config(lib)
     err = ctypes.create_string_buffer(512)
     rc = lib.hg_config_parse(ctypes.byref(cfg), text.encode(), err, 512)
+    if rc != HG_OK:
+        raise ValueError(err.value.decode())
+    return cfg
+
+
+def parse_config3(text, lib=None):
+    """hg3_config from .hydroconf `set` lines (defaults first)."""
+    lib = lib or load()
+    cfg = Hg3Config()
+    lib.hg3_config_default(ctypes.byref(cfg))
+    err = ctypes.create_string_buffer(512)
+    rc = lib.hg3_config_parse(ctypes.byref(cfg), text.encode(), err, 512)
     if rc != HG_OK:
         raise ValueError(err.value.decode())
     return cfg

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/hydro_gpu.h"
+#include "../../include/hydro_gpu3d.h"
 
 namespace {
 
@@ -318,6 +319,80 @@ hg_status hg_config_parse(hg_config* c, const char* text, char* err, size_t errl
       if (!ok) return fail(line_no, "vect " + name + " needs two components");
     } else {
       return fail(line_no, "Unknown type '" + type + "'");
+    }
+  }
+  return HG_OK;
+}
+
+
+// ---- 3D (hydro_gpu3d.h) ------------------------------------------------
+
+void hg3_config_default(hg3_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  hg_config_default(&c->base);
+  c->nz = 10;
+  c->a_z = 0.;
+  c->b_z = 1.;
+  c->a1_z = 0.;
+  c->b1_z = 1.;  // general.hydroconf: A1 (0, 0, 0), B1 (0, 0, 1)
+  for (int s = 0; s < 6; ++s) c->bc_kind[s] = HG_BC_WALL;
+}
+
+// The 2D reader takes every key it knows (x and y of vectors); this pass
+// adds Nz, the z components and the six wall conditions.
+hg_status hg3_config_parse(hg3_config* c, const char* text, char* err, size_t errlen) {
+  const hg_status rc = hg_config_parse(&c->base, text, err, errlen);
+  if (rc != HG_OK) return rc;
+  auto fail = [&](int line_no, const std::string& msg) {
+    if (err && errlen) std::snprintf(err, errlen, "line %d: %s", line_no, msg.c_str());
+    return HG_ERR_PARSE;
+  };
+  std::istringstream all(text ? text : "");
+  std::string line;
+  int line_no = 0;
+  while (std::getline(all, line)) {
+    ++line_no;
+    size_t hash = line.find('#');
+    if (hash != std::string::npos) line = line.substr(0, hash);
+    std::istringstream in(line);
+    std::string cmd, type, name;
+    if (!(in >> cmd) || cmd != "set") continue;
+    if (!(in >> type >> name)) continue;  // refused by the 2D pass
+    if (type == "int" && name == "Nz") {
+      long v;
+      if (!(in >> v)) return fail(line_no, "bad int for Nz");
+      c->nz = int32_t(v);
+    } else if (type == "vect") {
+      std::vector<double> v;
+      if (!ReadVect(in, v)) return fail(line_no, "bad vect for " + name);
+      const double z = v.size() >= 3 ? v[2] : 0.;
+      if (name == "A") c->a_z = z;
+      else if (name == "B") c->b_z = z;
+      else if (name == "A1") c->a1_z = z;
+      else if (name == "B1") c->b1_z = z;
+      else if (name == "gravity") c->gravity_z = z;
+      else if (name == "force") c->force_z = z;
+      else if (name == "initial_velocity") c->initial_velocity_z = z;
+      else if (name == "meshvel") c->meshvel_z = z;
+    } else if (type == "string" && name.rfind("condition_", 0) == 0) {
+      int side = -1;
+      if (name == "condition_left") side = HG_SIDE_LEFT;
+      else if (name == "condition_right") side = HG_SIDE_RIGHT;
+      else if (name == "condition_bottom") side = HG_SIDE_BOTTOM;
+      else if (name == "condition_top") side = HG_SIDE_TOP;
+      else if (name == "condition_close") side = HG_SIDE_CLOSE;
+      else if (name == "condition_far") side = HG_SIDE_FAR;
+      if (side < 0) continue;
+      std::istringstream cs(RestOfLine(in));
+      std::string kind;
+      cs >> kind;
+      double v3[3] = {0., 0., 0.};
+      cs >> v3[0] >> v3[1] >> v3[2];
+      if (kind == "wall") c->bc_kind[side] = HG_BC_WALL;
+      else if (kind == "inlet") c->bc_kind[side] = HG_BC_INLET;
+      else if (kind == "outlet") c->bc_kind[side] = HG_BC_OUTLET;
+      else return fail(line_no, "unknown boundary condition '" + kind + "'");
+      for (int d = 0; d < 3; ++d) c->bc_velocity[side][d] = kind == "outlet" ? 0. : v3[d];
     }
   }
   return HG_OK;

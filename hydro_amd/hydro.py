@@ -397,3 +397,69 @@ def batch_run(cases, steps, threads=0):
                           stats, ctypes.byref(ms))
     _raise(lib, None, rc, "hg_batch_run")
     return list(stats), ms.value
+
+
+def _raise3(lib, ctx, rc, what):
+    if rc == capi.HG_OK:
+        return
+    msg = lib.hg3_last_error(ctx if ctx else None).decode()
+    text = f"{what}: {capi.STATUS_NAMES.get(rc, rc)} {msg}".strip()
+    if rc == capi.HG_ERR_NAN:
+        raise HydroNaN(msg or "NaN")
+    if rc == capi.HG_ERR_UNSUPPORTED:
+        raise Unsupported(text)
+    raise HydroError(text)
+
+
+class Hydro3:
+    """One GPU-resident hydro3d case (include/hydro_gpu3d.h): the same
+    step as Hydro on the structured 3D mesh."""
+
+    def __init__(self, config):
+        self.lib = capi.load()
+        if isinstance(config, str):
+            config = capi.parse_config3(config, lib=self.lib)
+        self.cfg = config
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.hg3_create(ctypes.byref(config), ctypes.byref(self.ctx))
+        if rc != capi.HG_OK:
+            self.ctx = ctypes.c_void_p()
+            _raise3(self.lib, None, rc, "hg3_create")
+        nx, ny, nz = config.base.nx, config.base.ny, config.nz
+        self.nc = nx * ny * nz
+        self.nf = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+
+    def close(self):
+        if self.ctx:
+            self.lib.hg3_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get(self, field):
+        n = self.nf if field == capi.HG_FIELD_VOLUME_FLUX else self.nc
+        out = np.empty(n, dtype=np.float64)
+        _raise3(self.lib, self.ctx, self.lib.hg3_get_field(self.ctx, field, capi.dptr(out), n),
+                "hg3_get_field")
+        return out
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _raise3(self.lib, self.ctx, self.lib.hg3_set_field(self.ctx, field, capi.dptr(v), v.size),
+                "hg3_set_field")
+
+    def step(self):
+        st = capi.HgStepStats()
+        _raise3(self.lib, self.ctx, self.lib.hg3_step(self.ctx, ctypes.byref(st)), "hg3_step")
+        return st
+

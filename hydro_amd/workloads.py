@@ -236,3 +236,31 @@ def c5_cases(count=64, seed=2026, solver_p="fcg_mg", solver_v="lu_rb", precision
                        f"set vect gravity (0, {j(-9.81):.6g})\nset vect A1 (0, 0)\n"
                        f"set vect B1 (0.5, 0.5)\nset double dt 1e-4\n" + solvers)
     return out
+
+# ---- 3D (hydro_gpu3d.h): vectors with three components; the keys whose
+# general.hydroconf defaults have two (initial_velocity, box_A, box_B) are
+# set with three, as the reference's 3D module reads three
+_W3 = "set vect initial_velocity (0, 0, 0)\nset vect box_A (0, 0, 0)\nset vect box_B (0, 0, 0)\n"
+
+
+def cavity3d(nx=16, ny=16, nz=16, dt=1e-3, viscosity=0.01, solver_p="jacobi",
+             solver_v="jacobi", extra=""):
+    """Lid-driven cavity in the unit cube: the lid (top, y = 1) moves in x."""
+    return (f"set int Nx {nx}\nset int Ny {ny}\nset int Nz {nz}\nset vect A (0, 0, 0)\n"
+            f"set vect B (1, 1, 1)\nset double dt {dt}\nset double viscosity_0 {viscosity}\n"
+            f"set string condition_top \"wall 1 0 0\"\n"
+            f"set string linear_solver_pressure {solver_p}\nset string linear_solver_velocity {solver_v}\n"
+            f"set double lu_relaxed_relaxation_factor 0.9\n" + _W3 + extra)
+
+
+def dam3d(nx=32, ny=8, nz=16, dt=1e-4, solver_p="jacobi", solver_v="jacobi", extra=""):
+    """Broken dam in a 2 x 0.5 x 1 box, gravity -9.81 in z, density ratio
+    1000, the water block 0.5 wide and 0.5 high in the corner across the
+    whole depth (the reference's broken_dam_3d layout, smaller)."""
+    return (f"set int Nx {nx}\nset int Ny {ny}\nset int Nz {nz}\nset vect A (0, 0, 0)\n"
+            f"set vect B (2, 0.5, 1)\nset int num_phases 2\nset double density_0 1\n"
+            f"set double density_1 1000\nset double viscosity_0 1e-5\nset double viscosity_1 1e-3\n"
+            f"set vect gravity (0, 0, -9.81)\nset vect A1 (0, 0, 0)\nset vect B1 (0.5, 0.5, 0.5)\n"
+            f"set double dt {dt}\nset string linear_solver_pressure {solver_p}\n"
+            f"set string linear_solver_velocity {solver_v}\n"
+            f"set double lu_relaxed_relaxation_factor 0.9\n" + _W3 + extra)

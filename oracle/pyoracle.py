@@ -14,6 +14,7 @@ ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_BIN = os.path.join(ORACLE_DIR, "_ref", "hydro_ref")
 REF_BIN_FAST = os.path.join(ORACLE_DIR, "_ref", "hydro_ref_fast")
+REF_BIN_3D = os.path.join(ORACLE_DIR, "_ref", "hydro_ref3d")
 
 _P = ctypes.POINTER
 _dp = _P(ctypes.c_double)
@@ -55,6 +56,14 @@ def load():
                                 [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                  ctypes.c_int, _dp, _dp, _P(ctypes.c_int), _dp]),
         "or_bdf_coeffs": (None, [ctypes.c_double, ctypes.c_int, _dp]),
+        "or3_create": (ctypes.c_int, [_P(capi.Hg3Config), _P(vp), ctypes.c_char_p,
+                                      ctypes.c_size_t]),
+        "or3_destroy": (None, [vp]),
+        "or3_num_cells": (ctypes.c_int, [vp]),
+        "or3_num_faces": (ctypes.c_int, [vp]),
+        "or3_get_field": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_size_t]),
+        "or3_set_field": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_size_t]),
+        "or3_step": (ctypes.c_int, [vp, _P(capi.HgStepStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -213,6 +222,75 @@ def bdf_coeffs(dt, second_order=True):
     lib = load()
     out = np.zeros(3)
     lib.or_bdf_coeffs(dt, 1 if second_order else 0, capi.dptr(out))
+    return out
+
+
+class Oracle3:
+    """One 3D oracle case (the CPU counterpart of an hg3 context)."""
+
+    def __init__(self, cfg3):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = self.lib.or3_create(ctypes.byref(cfg3), ctypes.byref(self.ctx), err, 512)
+        if rc != 0:
+            raise ValueError(err.value.decode())
+        self.nc = self.lib.or3_num_cells(self.ctx)
+        self.nf = self.lib.or3_num_faces(self.ctx)
+
+    def close(self):
+        if self.ctx:
+            self.lib.or3_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self):
+        st = capi.HgStepStats()
+        rc = self.lib.or3_step(self.ctx, ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"or3_step: {rc}")
+        return st
+
+    def get(self, field):
+        n = self.nf if field == capi.HG_FIELD_VOLUME_FLUX else self.nc
+        out = np.empty(n)
+        rc = self.lib.or3_get_field(self.ctx, field, out.ctypes.data_as(_dp), n)
+        if rc != 0:
+            raise ValueError(f"or3_get_field {field}: {rc}")
+        return out
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        rc = self.lib.or3_set_field(self.ctx, field, v.ctypes.data_as(_dp), v.size)
+        if rc != 0:
+            raise ValueError(f"or3_set_field {field}: {rc}")
+
+
+def read_ref_dump3(path):
+    """Reads a dump of oracle/_ref/hydro_ref3d (ref_shim.cpp built with
+    HR_DIM=3): header nx, ny, nz, num_phases, step; u, v, w, p, flux, then
+    alpha_i, pd_i per phase, then T."""
+    raw = open(path, "rb").read()
+    nx, ny, nz, np_, step = (int(v) for v in np.frombuffer(raw[:20], dtype=np.int32))
+    nc = nx * ny * nz
+    nf = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+    data = np.frombuffer(raw[20:], dtype=np.float64)
+    out = {"nx": nx, "ny": ny, "nz": nz, "num_phases": np_, "step": step}
+    off = 0
+    for name, n in (("u", nc), ("v", nc), ("w", nc), ("p", nc), ("flux", nf)):
+        out[name] = data[off:off + n].copy()
+        off += n
+    for i in range(np_):
+        out[f"alpha{i}"] = data[off:off + nc].copy()
+        off += nc
+        out[f"pd{i}"] = data[off:off + nc].copy()
+        off += nc
+    out["T"] = data[off:off + nc].copy()
     return out
 
 

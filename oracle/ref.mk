@@ -50,4 +50,12 @@ $(OUT)/hydro_ref_gpu: oracle/ref_shim.cpp oracle/ref_gpu_dispatch.cpp $(REF_SRCS
 	  oracle/ref_gpu_dispatch.cpp $(REF_SRCS) -Lhydro_amd -lhydrogpu \
 	  -Wl,-rpath,'$$ORIGIN/../../hydro_amd'
 
-.PHONY: all gpu
+# hydro_ref3d: the same driver on the reference's 3D module (hydro3d.cpp,
+# mesh3d.cpp; MODULE_HYDRO_3D), fixtures of the 3D step
+ref3d: $(OUT)/hydro_ref3d
+$(OUT)/hydro_ref3d: oracle/ref_shim.cpp $(REF_SRCS)
+	mkdir -p $(OUT)
+	$(CXX) $(COMMON) -DMODULE_HYDRO_3D -DHR_DIM=3 -O2 -ffp-contract=off -o $@ \
+	  oracle/ref_shim.cpp $(REF_SRCS) $(SRC)/hydro2dmpi/hydro3d.cpp $(SRC)/hydro2dmpi/mesh3d.cpp
+
+.PHONY: all gpu ref3d

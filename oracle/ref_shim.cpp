@@ -10,11 +10,15 @@
 // instantiations (where access checks do not apply), which changes no
 // reference file and no reference behaviour.
 //
+// Built with -DHR_DIM=3 (oracle/_ref/hydro_ref3d) it runs the 3D module,
+// hydro<geom3d::MeshStructured<double>> (hydro3d.cpp), instead.
+//
 // usage: hydro_ref <script> <nsteps> <dump-prefix> [dump-every] [hook argument]
 //   script: a file of `set <type> <name> <value>` lines
 //   writes <prefix>.<step>.bin for each dumped step:
-//     header: int32 nx, ny, num_phases, step; then float64 arrays
-//     u[N], v[N], p[N], flux[F], then alpha_i[N], pd_i[N] per phase,
+//     header: int32 nx, ny, num_phases, step (3D: nx, ny, nz, num_phases,
+//     step); then float64 arrays u[N], v[N] (3D: and w[N]), p[N], flux[F],
+//     then alpha_i[N], pd_i[N] per phase,
 //     then the temperature T[N]
 //   and prints one line per step: "STEP s=<n> iters=<SIMPLE iters> t=<seconds>"
 //   dump-every < 0: output mode, no dumps; after each step the module time
@@ -44,7 +48,11 @@
 
 #include "hydro2dmpi/hydro2d.hpp"
 
+#if HR_DIM == 3
+using Mesh = geom::geom3d::MeshStructured<double>;
+#else
 using Mesh = geom::geom2d::MeshStructured<double>;
+#endif
 using Module = hydro2D_uniform_MPI::hydro<Mesh>;
 
 // Hook run right after the module is constructed (null in hydro_ref; set by
@@ -105,13 +113,21 @@ static void Dump(Module& h, const std::string& path, int step) {
   int ny = int(mesh.GetBlockCells().GetDimensions()[1]);
   int np = int(h.*Get(TagNp()));
   std::ofstream f(path, std::ios::binary);
+#if HR_DIM == 3
+  // 3D: header int32 nx, ny, nz, num_phases, step; three velocity components
+  int nz = int(mesh.GetBlockCells().GetDimensions()[2]);
+  int32_t hdr[5] = {nx, ny, nz, np, step};
+  const size_t ncomp = 3;
+#else
   int32_t hdr[4] = {nx, ny, np, step};
+  const size_t ncomp = 2;
+#endif
   f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
   const auto& vel = fluid->GetVelocity();
   const auto& p = fluid->GetPressure();
   const auto& flux = fluid->GetVolumeFlux();
   std::vector<double> buf;
-  for (size_t comp = 0; comp < 2; ++comp) {
+  for (size_t comp = 0; comp < ncomp; ++comp) {
     buf.clear();
     for (auto c : mesh.Cells()) buf.push_back(vel[c][comp]);
     f.write(reinterpret_cast<const char*>(buf.data()), buf.size() * 8);

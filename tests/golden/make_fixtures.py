@@ -19,6 +19,8 @@ not needed afterwards).
 5. sample_*.gz: the field frame and iteration history the reference ships
    with its cavity sample run (examples/cavity/sample), gzip-compressed:
    data only.
+6. ref3d_*.npz / ref3d_*.json: the same for the reference's 3D module
+   (oracle/_ref/hydro_ref3d: hydro<geom3d::MeshStructured<double>>).
 
     python tests/golden/make_fixtures.py [output | only NAME...]
 """
@@ -126,6 +128,72 @@ LONG = {
     # about step 200 its SOR runs to the 1000-sweep limit every iteration)
     "ref_dam256x128_500": (workloads.dam(256, 128), 500, (100, 200, 300, 400, 500)),
 }
+
+
+FIXTURES3 = {
+    # cavity with the jacobi solvers, one SIMPLE iteration per step
+    "ref3d_cavity12x10x8": (workloads.cavity3d(12, 10, 8), 5, (1, 5)),
+    # the SIMPLE loop to convergence (3-4 iterations per step)
+    "ref3d_cavity12x10x8_loop": (workloads.cavity3d(12, 10, 8, extra=(
+        "set double convergence_tolerance 1e-4\n")), 4, (4,)),
+    # the reference's default solvers (gauss_seidel pressure, lu velocity)
+    "ref3d_cavity10x8x6_gs_lu": (workloads.cavity3d(10, 8, 6, solver_p="gauss_seidel",
+                                                      solver_v="lu"), 3, (3,)),
+    # two phases, gravity in z
+    "ref3d_dam16x4x8": (workloads.dam3d(16, 4, 8, dt=1e-3), 4, (1, 4)),
+}
+LONG3 = {
+    # cavity 32^3, 60 steps
+    "ref3d_cavity32_60": (workloads.cavity3d(32, 32, 32), 60, (20, 60)),
+}
+
+
+def _run3(text, steps, every, tmp):
+    script = open(GENERAL).read() + "\n" + text + "\nset bool no_output 1\n"
+    path = os.path.join(tmp, "case.conf")
+    open(path, "w").write(script)
+    r = subprocess.run([pyoracle.REF_BIN_3D, path, str(steps), os.path.join(tmp, "d"),
+                        str(every)], capture_output=True, text=True, cwd=tmp)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr)
+    return r.stdout
+
+
+def reference_dump3(name, text, steps, dumps):
+    with tempfile.TemporaryDirectory() as tmp:
+        stdout = _run3(text, steps, 1, tmp)
+        out = {"text": np.array(text), "steps": np.array(steps)}
+        out["simple_iters"] = np.array([int(m.group(1)) for m in
+                                        re.finditer(r"STEP s=\d+ iters=(\d+)", stdout)])
+        out["pressure_sweeps"] = np.array([int(m.group(1)) for m in
+                                           re.finditer(r"iter = (\d+), diff", stdout)])
+        for s in dumps:
+            d = pyoracle.read_ref_dump3(os.path.join(tmp, f"d.{s}.bin"))
+            for k, v in d.items():
+                if isinstance(v, np.ndarray):
+                    out[f"s{s}_{k}"] = v
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print(f"{name}: {steps} steps, SIMPLE iterations {list(out['simple_iters'])}")
+
+
+def reference_hashes3(name, text, steps, dumps):
+    import hashlib
+    import json
+    with tempfile.TemporaryDirectory() as tmp:
+        stdout = _run3(text, steps, min(dumps), tmp)
+        out = {"text": text, "steps": steps,
+               "simple_iters": [int(m.group(1)) for m in
+                                re.finditer(r"STEP s=\d+ iters=(\d+)", stdout)],
+               "dumps": {}}
+        for s in dumps:
+            d = pyoracle.read_ref_dump3(os.path.join(tmp, f"d.{s}.bin"))
+            out["dumps"][str(s)] = {
+                k: {"sha256": hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest(),
+                    "l2": float(np.linalg.norm(v)), "max": float(np.abs(v).max())}
+                for k, v in d.items() if isinstance(v, np.ndarray)}
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(out, f, indent=1)
+        print(f"{name}: {steps} steps, SIMPLE iterations total {sum(out['simple_iters'])}")
 
 
 def reference_hashes(name, text, steps, dumps):
@@ -268,6 +336,10 @@ def main():
                 reference_output(name, *OUTPUT[name])
             elif name in LONG:
                 reference_hashes(name, *LONG[name])
+            elif name in FIXTURES3:
+                reference_dump3(name, *FIXTURES3[name])
+            elif name in LONG3:
+                reference_hashes3(name, *LONG3[name])
             else:
                 raise SystemExit(f"unknown fixture {name}")
         return
@@ -281,6 +353,10 @@ def main():
         reference_dump(name, text, steps, dumps)
     for name, (text, steps, dumps) in LONG.items():
         reference_hashes(name, text, steps, dumps)
+    for name, (text, steps, dumps) in FIXTURES3.items():
+        reference_dump3(name, text, steps, dumps)
+    for name, (text, steps, dumps) in LONG3.items():
+        reference_hashes3(name, text, steps, dumps)
 
 
 if __name__ == "__main__":

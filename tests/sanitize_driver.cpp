@@ -7,8 +7,10 @@
 //    family the tests use (cavity, two-phase dam, inlet/outlet channel,
 //    SIMPLER, heat, dt_auto, every row solver and CG), the stencil and
 //    Poisson solves and the CPU solve baselines;
-//  * the configuration reader (hydro_amd/csrc/config.cpp) on valid and
-//    malformed scripts;
+//  * the 3D oracle (oracle/hydro_oracle3d.cpp): steps of a cavity with each
+//    of its solvers and of a two-phase dam;
+//  * the configuration reader (hydro_amd/csrc/config.cpp, 2D and 3D) on
+//    valid and malformed scripts;
 //  * the output keys and the .vts writer (hydro_amd/csrc/output.cpp).
 // output.cpp's session entry points read a context through the C-ABI; the
 // sessions are not run here, so those calls resolve to the refusals below.
@@ -22,6 +24,7 @@
 #include "../include/hydro_gpu.h"
 #include "../include/hydro_output.h"
 #include "../oracle/hydro_oracle.h"
+#include "../oracle/hydro_oracle3d.h"
 
 extern "C" {
 // Context entry points output.cpp links against (libhydrogpu.so's
@@ -113,6 +116,44 @@ int main() {
                                  "set double cfl_advection 0.01\n"
                                  "set vect initial_velocity (0.2, 0.1)\n", 2);
   RunOracle("channel", chan, 3);
+
+  // the 3D oracle
+  {
+    const std::string w3 = "set vect initial_velocity (0, 0, 0)\nset int Nx 6\nset int Ny 5\n"
+                           "set int Nz 4\nset vect A (0, 0, 0)\nset vect B (1, 0.8, 0.6)\n"
+                           "set double dt 1e-3\nset double lu_relaxed_relaxation_factor 0.9\n";
+    const char* cases[] = {
+        "set string condition_top \"wall 1 0 0.5\"\nset string linear_solver_pressure jacobi\n"
+        "set string linear_solver_velocity jacobi\n",
+        "set string condition_far \"wall 0 1 0\"\n",  // gauss_seidel + lu
+        "set string linear_solver_pressure cg\nset string linear_solver_velocity jacobi\n"
+        "set double convergence_tolerance 1e-5\n",
+        "set int num_phases 2\nset double density_1 1000\nset vect gravity (0, 0, -9.81)\n"
+        "set vect A1 (0, 0, 0)\nset vect B1 (0.5, 0.8, 0.3)\nset double dt 1e-4\n"};
+    for (const char* extra : cases) {
+      hg3_config c3;
+      hg3_config_default(&c3);
+      char err[256] = {0};
+      Check(hg3_config_parse(&c3, (w3 + extra).c_str(), err, sizeof err) == HG_OK, "parse3");
+      or3_ctx* o = nullptr;
+      Check(or3_create(&c3, &o, err, sizeof err) == 0, "or3_create");
+      if (!o) continue;
+      for (int s = 0; s < 2; ++s) {
+        hg_step_stats st{};
+        Check(or3_step(o, &st) == 0, "or3_step");
+      }
+      std::vector<double> u(size_t(or3_num_cells(o)));
+      Check(or3_get_field(o, HG_FIELD_VELOCITY_Z, u.data(), u.size()) == 0, "or3_get_field");
+      for (double v : u) Check(std::isfinite(v), "3D field finite");
+      or3_destroy(o);
+    }
+    hg3_config c3;
+    hg3_config_default(&c3);
+    char err[64];
+    (void)hg3_config_parse(&c3, "set int Nz\nset vect B (1,\nset string condition_far \"wall\"\n",
+                           err, sizeof err);
+    std::printf("oracle 3D: %d cases\n", int(sizeof cases / sizeof cases[0]));
+  }
 
   // stencil and Poisson solves, and the CPU baselines
   {

@@ -1,8 +1,8 @@
 """The C-ABI library and its host logic, without a GPU.
 
 * libhydrogpu.so loads and exports every function include/hydro_gpu.h,
-  include/hydro_output.h and include/hydro_noalloc.h declare (no compute
-  calls here).
+  include/hydro_output.h, include/hydro_noalloc.h and include/hydro_gpu3d.h
+  declare (no compute calls here).
 * The .hydroconf reader keeps the reference's defaults and semantics.
 * Calls that need a device fail loudly when none is visible.
 """
@@ -17,13 +17,13 @@ from tests.conftest import gpu_available
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", h)
-           for h in ("hydro_gpu.h", "hydro_output.h", "hydro_noalloc.h")]
+           for h in ("hydro_gpu.h", "hydro_output.h", "hydro_noalloc.h", "hydro_gpu3d.h")]
 
 
 def declared_functions():
     text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(hg_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(hg3?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_and_python_mirror_agree():

@@ -17,6 +17,7 @@ def test_host_code_under_asan_and_ubsan(tmp_path):
     exe = tmp_path / "sanitize_driver"
     srcs = [os.path.join(ROOT, "tests", "sanitize_driver.cpp"),
             os.path.join(ROOT, "oracle", "hydro_oracle.cpp"),
+            os.path.join(ROOT, "oracle", "hydro_oracle3d.cpp"),
             os.path.join(ROOT, "hydro_amd", "csrc", "config.cpp"),
             os.path.join(ROOT, "hydro_amd", "csrc", "output.cpp")]
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fopenmp",
