@@ -61,6 +61,22 @@ def build_gpu_lib():
     return LIB
 
 
+MICRO = os.path.join(ROOT, "hydro_amd", "micro")
+
+
+def build_micro():
+    """The launch and barrier microbenchmarks (standalone programs, not
+    part of the library), next to their sources."""
+    out = []
+    for name, extra in (("grid_barrier", []), ("graph_launch", ["-pthread"])):
+        exe = os.path.join(MICRO, name)
+        _run([HIPCC, "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+              "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+              os.path.join(MICRO, name + ".hip"), "-o", exe] + extra)
+        out.append(exe)
+    return out
+
+
 def build_oracle():
     tmp = ORACLE_LIB + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
@@ -86,6 +102,7 @@ def build_all(verbose=True):
     out["oracle"] = build_oracle()
     out["gpu"] = build_gpu_lib()
     out["reference"] = build_reference()
+    out["micro"] = build_micro()
     if verbose:
         for k, v in out.items():
             print(f"[build] {k}: {v}", flush=True)
