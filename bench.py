@@ -51,7 +51,8 @@ def log(msg):
 # WRITE_SIZE on the RT 1024x2048 state, each scaled by what the counter reads
 # for a 256 MiB device-to-device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0).
 MEASURED_TRAFFIC = {"KUp<double, true>": 82.79e6, "KDown<double>": 58.70e6,
-                    "KAdvStep<2, false>": 120.91e6, "KMomentum": 579.68e6}
+                    "KAdvStep<2, false>": 120.91e6, "KMomentum": 579.68e6,
+                    "KAdvStep<2, false> mixed": 104.12e6, "KMomentum mixed": 370.78e6}
 
 
 def run_path(text, steps, warmup, label, train=False):
