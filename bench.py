@@ -225,6 +225,38 @@ def batch_c5(steps=3):
             "ms_concurrent_8_threads": round(ms_conc, 2), "ms_one_after_another": round(ms_seq, 2)}
 
 
+def step_3d(n=128, steps=5, warmup=2, cpu_n=64):
+    """The 3D step (hg3_*, the reference's hydro3d): a lid-driven cavity
+    n^3, FP64, the reference's jacobi as the velocity solver (bitwise the
+    reference's) and FP64 Jacobi-PCG for the pressure; beside it the 3D
+    oracle (one thread) on one step of the same case at cpu_n^3."""
+    from hydro_amd import capi, workloads
+    from hydro_amd.hydro import Hydro3
+    from oracle import pyoracle
+    text = workloads.cavity3d(n, n, n, solver_p="cg", solver_v="jacobi")
+    h = Hydro3(text)
+    for _ in range(warmup):
+        h.step()
+    t0 = time.perf_counter()
+    st = [h.step() for _ in range(steps)]   # hg3_step returns after its work
+    dt = (time.perf_counter() - t0) / steps
+    h.close()
+    small = workloads.cavity3d(cpu_n, cpu_n, cpu_n, solver_p="cg", solver_v="jacobi")
+    o = pyoracle.Oracle3(capi.parse_config3(small))
+    o.step()
+    t1 = time.perf_counter()
+    o.step()
+    cpu_s = time.perf_counter() - t1
+    o.close()
+    return {"workload": f"cavity {n}^3, FP64, jacobi velocity, Jacobi-PCG pressure to 1e-8",
+            "steps_per_s": round(1. / dt, 3), "ms_per_step": round(dt * 1e3, 2),
+            "cell_steps_per_s": round(n ** 3 / dt, 0),
+            "simple_iters_per_step": float(np.mean([s.simple_iterations for s in st])),
+            "pressure_iters_per_step": float(np.mean([s.pressure_solve_iters_total for s in st])),
+            "cpu_oracle_one_thread": {"case": f"cavity {cpu_n}^3, one step", "s": round(cpu_s, 3),
+                                      "cell_steps_per_s": round(cpu_n ** 3 / cpu_s, 0)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +391,9 @@ def main():
     log(f"step roofline: {step_roofline}")
 
     batch = batch_c5()
+    log(f"C5: {batch}")
+    s3d = step_3d()
+    log(f"3D step: {s3d}")
     log(f"batch: {batch}")
 
     net = net_throughput(nx, ny)
@@ -407,6 +442,7 @@ def main():
         "step_roofline": step_roofline,
         "kernels": kernels,
         "batch_c5": batch,
+        "step_3d": s3d,
         "net": net,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -429,17 +429,21 @@ __device__ __forceinline__ double BlockSum(double v, double* red) {
     for (int w = 0; w < kB / 64; ++w) t += red[w];
   return t;
 }
+// grid-stride loops over cells: a grid of at most kCgBlocks blocks leaves
+// few partial sums for the one-block folds
+constexpr int kCgBlocks = 1024;
+#define H3_CELLS(c, n) \
+  for (long c = blockIdx.x * long(blockDim.x) + threadIdx.x; c < (n); c += long(gridDim.x) * blockDim.x)
 __global__ void KApplyRows(G g, Rows A, const double* __restrict__ x, double* __restrict__ y,
                            const double* __restrict__ p2, double* part) {
   __shared__ double red[kB / 64];
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
   double v = 0.;
-  if (c < g.nc) {
+  H3_CELLS(c, g.nc) {
     double s = 0.;
     for (int q = 0; q < 7; ++q)
       if (Has(g, c, q)) s += A.t[q][c] * x[c + Off(g, q)];
     y[c] = s;
-    if (p2) v = p2[c] * s;
+    if (p2) v += p2[c] * s;
   }
   if (part) {
     const double t = BlockSum(v, red);
@@ -455,12 +459,10 @@ __global__ void KFold(const double* part, int n, double* out) {
 }
 // elementwise helpers of the PCG
 __global__ void KCgStart(long n, const double* k, const double* mean, const double* diag,
-                         double* b, double* r, double* z, double* p, double* x, double* part,
-                         const double* rz_part_unused) {
+                         double* b, double* r, double* z, double* p, double* x, double* part) {
   __shared__ double red[kB / 64];
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
   double v = 0.;
-  if (c < n) {
+  H3_CELLS(c, n) {
     const double bv = -k[c] - *mean;
     b[c] = bv;
     r[c] = bv;
@@ -468,15 +470,16 @@ __global__ void KCgStart(long n, const double* k, const double* mean, const doub
     z[c] = zv;
     p[c] = zv;
     x[c] = 0.;
-    v = bv * zv;
+    v += bv * zv;
   }
   const double t = BlockSum(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 __global__ void KSumK(long n, const double* k, double* part) {
   __shared__ double red[kB / 64];
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
-  const double t = BlockSum(c < n ? -k[c] : 0., red);
+  double v = 0.;
+  H3_CELLS(c, n) v += -k[c];
+  const double t = BlockSum(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 __global__ void KDivide(double* v, double n) { *v = *v / n; }
@@ -490,16 +493,15 @@ __global__ void KCgUpdate(long n, const double* p, const double* q, double* x, d
   __shared__ double red[kB / 64];
   if (sc->done) return;
   const double alpha = sc->rz / sc->pq;
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
   double rr = 0., rz = 0.;
-  if (c < n) {
+  H3_CELLS(c, n) {
     x[c] += alpha * p[c];
     const double rv = r[c] - alpha * q[c];
     r[c] = rv;
     const double zv = rv / diag[c];
     z[c] = zv;
-    rr = rv * rv;
-    rz = rv * zv;
+    rr += rv * rv;
+    rz += rv * zv;
   }
   const double t1 = BlockSum(rr, red);
   __syncthreads();
@@ -526,9 +528,8 @@ __global__ void KCgShift(CgScal* sc) {
 }
 __global__ void KCgResidual(long n, const double* b, const double* ax, double* part) {
   __shared__ double red[kB / 64];
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
   double v = 0.;
-  if (c < n) v = (b[c] - ax[c]) * (b[c] - ax[c]);
+  H3_CELLS(c, n) v += (b[c] - ax[c]) * (b[c] - ax[c]);
   const double t = BlockSum(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -1070,7 +1071,8 @@ int SolveJacobi(hg3_ctx* c, const Rows& R, const double* k, double* x) {
 int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   const hg_config& cfg = c->cfg;
   const long n = c->nc;
-  const int nb = int(Grid(n));
+  const int nb = int(std::min<long>(Grid(n), kCgBlocks));  // partial sums
+  const unsigned ne = Grid(n);                              // elementwise
   const double tol = cfg.pressure_tolerance_gpu;
   const int max_iters = cfg.pressure_max_iters_gpu;
   CgScal* sc = c->cs;
@@ -1080,7 +1082,7 @@ int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   KFold<<<1, kB, 0, c->st>>>(c->part, nb, &sc->mean);
   KDivide<<<1, 1, 0, c->st>>>(&sc->mean, double(n));
   KCgStart<<<nb, kB, 0, c->st>>>(n, k, &sc->mean, R.t[3], c->cb, c->cr, c->cz, c->cp, c->cx,
-                                 c->part, nullptr);
+                                 c->part);
   KFold<<<1, kB, 0, c->st>>>(c->part, nb, &sc->rz);
   // ||b||^2
   KApplyRows<<<nb, kB, 0, c->st>>>(c->g, R, c->cx, c->cq, nullptr, nullptr);  // q = A 0 = 0
@@ -1099,7 +1101,7 @@ int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
         KFold<<<1, kB, 0, c->st>>>(c->part, nb, &sc->rr);
         KFold<<<1, kB, 0, c->st>>>(c->part + nb, nb, &sc->rz2);
         KCgCheck<<<1, 1, 0, c->st>>>(sc, tol * tol, max_iters);
-        KCgDir<<<nb, kB, 0, c->st>>>(n, c->cz, c->cp, sc);
+        KCgDir<<<ne, kB, 0, c->st>>>(n, c->cz, c->cp, sc);
         KCgShift<<<1, 1, 0, c->st>>>(sc);
       }
       H3_CHECK(hipMemcpyAsync(c->cs_h, sc, sizeof(CgScal), hipMemcpyDeviceToHost, c->st));

@@ -5,6 +5,7 @@
                                                   # calibration), then the step's
                                                   # dominant kernels
     python -m hydro_amd.profile_targets net       # the BF16 MFMA network alone
+    python -m hydro_amd.profile_targets step3d    # 3D cavity 128^3 steps (hg3_*)
 
 Each prints what it ran; none writes files.
 """
@@ -77,6 +78,22 @@ def net(reps=5):
     n.close()
 
 
+def step3d(n=128, steps=3, warmup=2):
+    """The 3D step: cavity n^3, FP64, jacobi velocity, Jacobi-PCG pressure."""
+    from hydro_amd import workloads
+    from hydro_amd.hydro import Hydro3
+    h = Hydro3(workloads.cavity3d(n, n, n, solver_p="cg", solver_v="jacobi"))
+    for _ in range(warmup):
+        h.step()
+    t0 = time.perf_counter()
+    st = [h.step() for _ in range(steps)]
+    print(f"3D cavity {n}^3: {warmup} warm-up steps, then {steps} steps: "
+          f"{1e3 * (time.perf_counter() - t0) / steps:.2f} ms/step, SIMPLE "
+          f"{[s.simple_iterations for s in st]}, pressure iterations "
+          f"{[s.pressure_solve_iters_total for s in st]}", flush=True)
+    h.close()
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "step"
-    {"step": step, "kernels": kernels, "net": net}[what]()
+    {"step": step, "kernels": kernels, "net": net, "step3d": step3d}[what]()
