@@ -6,7 +6,8 @@
 // not scale with h), restriction = sum of the four children's residuals,
 // prolongation = cell-centred bilinear weighted by the coarse face
 // coefficients (injection across walls and strong density jumps), smoother =
-// two Chebyshev-damped Jacobi sweeps before and after the coarse correction,
+// Chebyshev-damped Jacobi sweeps before and after the coarse correction (two
+// on the finest level, three on the coarser ones),
 // levels of <= 2048 cells in one workgroup down to a coarsest grid of <= 64
 // cells solved by Jacobi-PCG in one wave.
 #pragma once
@@ -32,10 +33,17 @@ struct MgLevel {
   float *r = nullptr, *xa = nullptr, *xb = nullptr;
 };
 
-// The coarse levels handed to the one-workgroup cycle kernel.
-struct Omega2 {
-  float a, b;  // damping of the first and the second Jacobi sweep
+// Damped Jacobi sweeps of each smoothing pass (pre and post): kFineSweeps on
+// the finest level, kSweeps on the coarser ones; a fused kernel's tile
+// carries a halo of as many cells as it sweeps.
+constexpr int kFineSweeps = 2;
+constexpr int kSweeps = 3;
+constexpr int kMaxSweeps = 3;
+struct SweepWeights {
+  float w[kMaxSweeps];  // damping of sweep 1, 2, ... (those past the count unused)
 };
+
+// The coarse levels handed to the one-workgroup cycle kernel.
 
 constexpr int kMaxInKernelLevels = 12;
 constexpr int kLdsCells = 2752;  // LDS cells per vector for all in-kernel levels
@@ -79,12 +87,21 @@ class MgPrecond : public Precond {
   double TimeKernel(const char* name, int reps, const double* r, hipStream_t st);
   const MgLevel& level(int l) const { return lv_[l]; }
   float omega = 0.8f;  // damping of the plain Jacobi sweeps (coarsest-level fallbacks)
-  // Two-sweep Chebyshev smoothing (pre and post): damping 1/r_k at the roots
-  // r_k = 5/4 + 3/4 cos((2k-1) pi / 4) of the degree-2 Chebyshev polynomial
-  // on [1/2, 2], the part of the spectrum of D^-1 A (<= 2 by Gershgorin) the
-  // coarse grid does not correct; the two sweeps' product is at most
-  // 1/T_2(5/3) = 0.22 there, against 0.36 for two sweeps of omega 0.8.
-  float omega1 = 0.56169f, omega2 = 1.38953f;
+  // Chebyshev smoothing (pre and post): damping 1/r_k at the roots r_k of
+  // the Chebyshev polynomial of the sweep count on an interval [a, 2] of the
+  // spectrum of D^-1 A (which is in [0, 2] by Gershgorin), applied in that
+  // order (small damping first).
+  // Finest level, two sweeps on [1/2, 2]: r_k = 5/4 + 3/4 cos((2k-1) pi / 4),
+  // the sweeps' product at most 1/T_2(5/3) = 0.22 on that interval.
+  SweepWeights fine_sweeps{{0.561695f, 1.389532f, 0.f}};
+  // Coarser levels, three sweeps on [1/5, 2]: r_k = 11/10 + 9/10
+  // cos((2k-1) pi / 6), the product at most 1/T_3(11/9) = 0.27 on an
+  // interval that two sweeps ([1/2, 2]) leave undamped below 1/2.  These
+  // levels are latency-bound, so the third sweep (in LDS, one ring more of
+  // halo) costs little; on the Rayleigh-Taylor pressure systems it takes
+  // flexible CG from 7.35 to about 6.2 iterations per solve to 1e-8, where
+  // a third sweep on the finest level too adds nothing more.
+  SweepWeights sweeps{{0.532078f, 0.909091f, 3.119374f}};
   int coarse_sweeps = 40;
   bool bilinear = true;  // prolongation: bilinear (true) or injection
   // levels of <= 2048 cells down to a coarsest grid of <= 64 cells run in

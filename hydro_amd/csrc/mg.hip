@@ -150,18 +150,20 @@ __device__ __forceinline__ float Bilinear(float ec, float ex, float ey, float ed
   return ec + (3.f * wx * (ex - ec) + 3.f * wy * (ey - ec) + wd * (ed - ec)) * 0.0625f;
 }
 
-// The halo-2 region of a 64 x TY tile split into per-thread slots: slots
-// 0..NT-1 are tile cells (tile position threadIdx.x + 256 s), slot NT a cell
-// of the ring at distance 1, slot NT + 1 a cell of the ring at distance 2
-// (threads beyond a ring's length have none).  A thread keeps its cells
+// The halo-SW region of a 64 x TY tile split into per-thread slots: slots
+// 0..NT-1 are tile cells (tile position threadIdx.x + 256 s), slot NT + d - 1
+// a cell of the ring at distance d = 1..SW (threads beyond a ring's length
+// have none).  A thread keeps its cells
 // through the phases of a fused kernel, so the cells' coefficients and
 // right-hand side are loaded once, together, into registers; LDS carries
 // only the iterates that neighbours exchange.
-template <int TY>
+template <int TY, int SW>
 struct TileSlots {
   static constexpr int NT = TY * kTX / kTileThreads;
-  static constexpr int N = NT + 2;
-  static_assert(2 * (kTX + 4) + 2 * (TY + 2) <= kTileThreads, "ring longer than the block");
+  static constexpr int N = NT + SW;
+  static_assert(SW >= 1 && SW <= kMaxSweeps, "sweep count");
+  static_assert(2 * (kTX + 2 * SW) + 2 * (TY + 2 * SW - 2) <= kTileThreads,
+                "ring longer than the block");
   // ring at distance d: rows -d and TY-1+d (with corners), columns -d and
   // kTX-1+d (without); position relative to the tile origin
   __device__ static bool Ring(int r, int d, int& li, int& lj) {
@@ -186,8 +188,8 @@ struct TileSlots {
   }
 };
 
-// One slot's cell: position, grid validity and (for the tile and the first
-// ring) its faces and inverse diagonal.
+// One slot's cell: position, grid validity and (for the tile and the rings
+// inside the outermost) its faces and inverse diagonal.
 struct SlotCell {
   int li, lj, i, j;
   bool mine;  // this thread has a cell in the slot
@@ -195,14 +197,15 @@ struct SlotCell {
   float kw, ke, ks, kn, di;
 };
 
-template <int TY, class RT>
-__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], RT (&rv)[TileSlots<TY>::N],
+// S: the tile's TileSlots<TY, SW>.
+template <class S, class RT>
+__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[S::N], RT (&rv)[S::N],
                                           int nx, int ny, int i0, int j0,
                                           const float* __restrict__ kx,
                                           const float* __restrict__ ky,
                                           const float* __restrict__ dinv,
-                                          const RT* __restrict__ r, bool r_on_ring2) {
-  using S = TileSlots<TY>;
+                                          const RT* __restrict__ r, bool r_on_last_ring) {
+  constexpr int SW = S::N - S::NT;
 #pragma unroll
   for (int s = 0; s < S::N; ++s) {
     SlotCell& c = sc[s];
@@ -212,13 +215,13 @@ __device__ __forceinline__ void LoadSlots(SlotCell (&sc)[TileSlots<TY>::N], RT (
     c.j = j0 + c.lj;
     c.in = c.mine && c.i >= 0 && c.i < nx && c.j >= 0 && c.j < ny;
     const int k = c.i + nx * c.j;
-    const bool coef = c.in && s <= S::NT;  // tile and first ring
+    const bool coef = c.in && s < S::NT + SW - 1;  // tile and inner rings
     c.kw = coef ? kx[c.i + (nx + 1) * c.j] : 0.f;
     c.ke = coef ? kx[c.i + 1 + (nx + 1) * c.j] : 0.f;
     c.ks = coef ? ky[k] : 0.f;
     c.kn = coef ? ky[k + nx] : 0.f;
     c.di = c.in ? dinv[k] : 0.f;
-    rv[s] = (c.in && (s <= S::NT || r_on_ring2)) ? r[k] : RT(0);
+    rv[s] = (c.in && (s < S::NT + SW - 1 || r_on_last_ring)) ? r[k] : RT(0);
   }
 }
 
@@ -239,11 +242,9 @@ __device__ __forceinline__ float SlotApply(const SlotCell& c, const float (*x)[X
 // (FcgFusion::up_on), alpha given: r - alpha q replaces the slots' residual
 // (halo included, so the down-leg smooths the updated residual), and the
 // tile cells write r, x += alpha p and contribute (r, r).
-template <int TY>
-__device__ __forceinline__ void FoldedUpdate(const FcgFusion& f,
-                                             const SlotCell (&sc)[TileSlots<TY>::N],
-                                             double (&rv)[TileSlots<TY>::N], int nx) {
-  using S = TileSlots<TY>;
+template <class S>
+__device__ __forceinline__ void FoldedUpdate(const FcgFusion& f, const SlotCell (&sc)[S::N],
+                                             double (&rv)[S::N], int nx) {
   double qv[S::N];
 #pragma unroll
   for (int s = 0; s < S::N; ++s) qv[s] = sc[s].in ? f.up_q[sc[s].i + nx * sc[s].j] : 0.;
@@ -272,18 +273,19 @@ __device__ __forceinline__ void FoldedUpdate(const FcgFusion& f,
   }
 }
 
-// Fused down-leg of one level: two damped Jacobi sweeps from zero
-// (x1 = w.a D^-1 r, x2 = x1 + w.b D^-1 (r - A x1)) and the restriction of
-// r - A x2 (sum of the four children).  A block of 256 threads works on a
-// 64 x TY tile with a two-cell halo (x1 on tile + 2, x2 on tile + 1 in LDS):
-// one launch and one pass over the level's arrays replace three, and all
+// Fused down-leg of one level: SW damped Jacobi sweeps from zero
+// (x1 = w1 D^-1 r, x_k+1 = x_k + w_k+1 D^-1 (r - A x_k)) and the restriction
+// of r - A x_SW (sum of the four children).  A block of 256 threads works on
+// a 64 x TY tile with an SW-cell halo (x_k on tile + SW - k + 1, in two LDS
+// buffers used in turn): one launch and one pass over the level's arrays
+// replace SW + 1, and all
 // global loads are issued at the start (TileSlots), so a workgroup waits on
 // memory once.  UP (FP64 residual only): the previous FCG update first
 // (FoldedUpdate).
-template <class RT, int TY, bool UP = false>
+template <class RT, int TY, int SW, bool UP = false>
 __global__ void __launch_bounds__(kTileThreads)
 KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
-      const float* __restrict__ dinv, const RT* __restrict__ r, Omega2 w,
+      const float* __restrict__ dinv, const RT* __restrict__ r, SweepWeights w,
       float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done,
       FcgFusion fcg, int stop_test) {
   static_assert(!UP || std::is_same<RT, double>::value, "the update is on the FP64 residual");
@@ -291,9 +293,9 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   // tested once the tile's loads are in flight, so a working launch does
   // not wait on it before the rest
   const int stopped = done ? *done : 0;
-  using S = TileSlots<TY>;
-  constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
-  __shared__ float s1[H2Y][H2X], s2[H1Y][H1X], sres[TY][kTX];
+  using S = TileSlots<TY, SW>;
+  constexpr int H = SW, XW = kTX + 2 * H, YH = TY + 2 * H;
+  __shared__ float sx[2][YH][XW], sres[TY][kTX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) {  // uniform over the block
@@ -303,32 +305,40 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
   SlotCell sc[S::N];
   RT rv[S::N];
-  LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
+  LoadSlots<S, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
   if (stopped) return;
   if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
-  if constexpr (UP) FoldedUpdate<TY>(fcg, sc, rv, nx);
+  if constexpr (UP) FoldedUpdate<S>(fcg, sc, rv, nx);
+  // x1 on the tile and rings 1..H
 #pragma unroll
   for (int s = 0; s < S::N; ++s)
-    if (sc[s].mine) s1[sc[s].lj + 2][sc[s].li + 2] = sc[s].in ? w.a * sc[s].di * float(rv[s]) : 0.f;
+    if (sc[s].mine) sx[0][sc[s].lj + H][sc[s].li + H] = sc[s].in ? w.w[0] * sc[s].di * float(rv[s]) : 0.f;
   __syncthreads();
+  // x_k+1 on the tile and rings 1..H-k, from x_k (one ring wider)
 #pragma unroll
-  for (int s = 0; s <= S::NT; ++s) {
-    const SlotCell& c = sc[s];
-    if (!c.mine) continue;
-    float xv = 0.f;
-    if (c.in)
-      xv = s1[c.lj + 2][c.li + 2] +
-           w.b * c.di * (float(rv[s]) - SlotApply<2, H2X>(c, s1, nx, ny));
-    s2[c.lj + 1][c.li + 1] = xv;
+  for (int k = 1; k < H; ++k) {
+    const float(*src)[XW] = sx[(k - 1) & 1];
+    float(*dst)[XW] = sx[k & 1];
+#pragma unroll
+    for (int s = 0; s < S::NT + H - k; ++s) {
+      const SlotCell& c = sc[s];
+      if (!c.mine) continue;
+      float xv = 0.f;
+      if (c.in)
+        xv = src[c.lj + H][c.li + H] +
+             w.w[k] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
+      dst[c.lj + H][c.li + H] = xv;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  const float(*xs)[XW] = sx[(H - 1) & 1];
 #pragma unroll
   for (int s = 0; s < S::NT; ++s) {
     const SlotCell& c = sc[s];
     float res = 0.f;
     if (c.in) {
-      x2out[c.i + nx * c.j] = s2[c.lj + 1][c.li + 1];
-      res = float(rv[s]) - SlotApply<1, H1X>(c, s2, nx, ny);
+      x2out[c.i + nx * c.j] = xs[c.lj + H][c.li + H];
+      res = float(rv[s]) - SlotApply<H, XW>(c, xs, nx, ny);
     }
     sres[c.lj][c.li] = res;
   }
@@ -342,27 +352,33 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   }
 }
 
-// Fused up-leg of one level: y = x2 + P e, x3 = y + w.a D^-1 (r - A y),
-// out = x3 + w.b D^-1 (r - A x3), on the block's 64 x TY tile with a
-// two-cell halo (y and x3 in LDS).  The coarse corrections and the coarse
+// Fused up-leg of one level: y = x2 + P e (x2: the down-leg's smoothed
+// iterate), then SW damped Jacobi sweeps y_k+1 = y_k + w_k+1 D^-1 (r - A y_k),
+// the last into out, on the block's 64 x TY tile with an SW-cell halo (y_k
+// on tile + SW - k, in two LDS buffers used in turn).  The coarse
+// corrections and the coarse
 // cells' side weights of the covering coarse region are staged in LDS; x2,
 // r and the fine coefficients of the thread's cells (TileSlots) are loaded
 // at the start with them.
-template <class RT, bool BILINEAR, int TY>
+template <class RT, bool BILINEAR, int TY, int SW>
 __global__ void __launch_bounds__(kTileThreads)
 KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
     const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
     const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
-    int nxc, int nyc, Omega2 w, float* __restrict__ out, const int* done, FcgFusion fcg,
+    int nxc, int nyc, SweepWeights w, float* __restrict__ out, const int* done, FcgFusion fcg,
     int dots) {
   const int stopped = done ? *done : 0;  // tested once the loads are in flight
-  using S = TileSlots<TY>;
-  constexpr int H2X = kTX + 4, H2Y = TY + 4, H1X = kTX + 2, H1Y = TY + 2;
-  // coarse cells i0/2 - 2 .. i0/2 + kTX/2 + 1 (parents of the halo-2 region
-  // and their neighbours), likewise in y
+  using S = TileSlots<TY, SW>;
+  constexpr int H = SW, XW = kTX + 2 * H, YH = TY + 2 * H;
+  // coarse cells i0/2 - 2 .. i0/2 + kTX/2 + 1, likewise in y: the parents of
+  // the tile and its halo and their neighbours on the child's side.  A child
+  // i reads parents i/2 and i/2 + 1 (i odd) or i/2 - 1 (i even), so halos of
+  // 2 and 3 cells (i0 - 3 odd, i0 - 2 even; i0 + kTX + 1 odd, i0 + kTX + 2
+  // even) reach the same parents
+  static_assert(SW >= 1 && SW <= 3, "coarse region covers halos of 1 to 3 cells");
   constexpr int CX = kTX / 2 + 4, CY = TY / 2 + 4, NC = CX * CY;
   constexpr int KC = (NC + kTileThreads - 1) / kTileThreads;
-  __shared__ float sy[H2Y][H2X], s3[H1Y][H1X];
+  __shared__ float sx[2][YH][XW];
   __shared__ float se[CY][CX], sw[BILINEAR ? 4 : 1][CY][CX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
@@ -392,7 +408,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   RT rv[S::N];
   float xv2[S::N];
   double qo[S::NT];
-  LoadSlots<TY, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
+  LoadSlots<S, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
 #pragma unroll
   for (int s = 0; s < S::N; ++s) xv2[s] = sc[s].in ? x2[sc[s].i + nx * sc[s].j] : 0.f;
 #pragma unroll
@@ -412,7 +428,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     }
   }
   __syncthreads();
-  // y = x2 + P e on the halo-2 region (zero outside the grid)
+  // y = x2 + P e on the tile and rings 1..H (zero outside the grid)
 #pragma unroll
   for (int s = 0; s < S::N; ++s) {
     const SlotCell& c = sc[s];
@@ -430,27 +446,34 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
       }
       yv = xv2[s] + corr;
     }
-    sy[c.lj + 2][c.li + 2] = yv;
+    sx[0][c.lj + H][c.li + H] = yv;
   }
   __syncthreads();
+  // y_k+1 on the tile and rings 1..H-1-k, from y_k (one ring wider)
 #pragma unroll
-  for (int s = 0; s <= S::NT; ++s) {
-    const SlotCell& c = sc[s];
-    if (!c.mine) continue;
-    float xv = 0.f;
-    if (c.in)
-      xv = sy[c.lj + 2][c.li + 2] +
-           w.a * c.di * (float(rv[s]) - SlotApply<2, H2X>(c, sy, nx, ny));
-    s3[c.lj + 1][c.li + 1] = xv;
+  for (int k = 0; k < H - 1; ++k) {
+    const float(*src)[XW] = sx[k & 1];
+    float(*dst)[XW] = sx[(k + 1) & 1];
+#pragma unroll
+    for (int s = 0; s < S::NT + H - 1 - k; ++s) {
+      const SlotCell& c = sc[s];
+      if (!c.mine) continue;
+      float xv = 0.f;
+      if (c.in)
+        xv = src[c.lj + H][c.li + H] +
+             w.w[k] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
+      dst[c.lj + H][c.li + H] = xv;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  const float(*ys)[XW] = sx[(H - 1) & 1];
   double zq = 0., rz = 0.;
 #pragma unroll
   for (int s = 0; s < S::NT; ++s) {
     const SlotCell& c = sc[s];
     if (!c.in) continue;
-    const float z = s3[c.lj + 1][c.li + 1] +
-                    w.b * c.di * (float(rv[s]) - SlotApply<1, H1X>(c, s3, nx, ny));
+    const float z = ys[c.lj + H][c.li + H] +
+                    w.w[H - 1] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, ys, nx, ny));
     out[c.i + nx * c.j] = z;
     const double zd = double(z);
     zq += zd * qo[s];
@@ -551,7 +574,7 @@ constexpr int kCcSlots = CcSlots(0);
 static_assert(kInKernelMax % kCcThreads == 0 || kInKernelMax < kCcThreads, "slots");
 
 __global__ void __launch_bounds__(kCcThreads)
-KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
+KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   const int stopped = done ? *done : 0;  // tested once the loads are in flight
   __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int t = threadIdx.x;
@@ -617,7 +640,15 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
     if (j < ny - 1) v -= kn[l][m] * x[c + nx];
     return v;
   };
-  // ---- down: two sweeps from zero, residual, restriction (sum of children)
+  // Sweep k (0-based) of a leg writes Buf(k): the two vectors in turn, so
+  // that the last (k = kSweeps - 1) writes xb on the way down (the up-leg's
+  // start) and xa on the way up (the level's result, read by the finer
+  // level's prolongation).
+  auto Buf = [](float* xa, float* xb, int k, bool down) {
+    const bool last_parity = ((kSweeps - 1 - k) & 1) == 0;
+    return (last_parity == down) ? xb : xa;
+  };
+  // ---- down: kSweeps sweeps from zero, residual, restriction (sum of children)
 #pragma unroll
   for (int l = 0; l < kCcLevels; ++l) {
     if (l >= nlev) break;
@@ -625,17 +656,19 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
     const float* r = sr + cs.off[l];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
 #pragma unroll
-    for (int m = 0; m < CcSlots(l); ++m) {
-      const int c = t + m * kCcThreads;
-      if (c < n) xa[c] = w.a * di[l][m] * r[c];
-    }
-    __syncthreads();
+    for (int k = 0; k < kSweeps; ++k) {
+      float* dst = Buf(xa, xb, k, true);
+      const float* src = Buf(xa, xb, k - 1, true);
 #pragma unroll
-    for (int m = 0; m < CcSlots(l); ++m) {
-      const int c = t + m * kCcThreads;
-      if (c < n) xb[c] = xa[c] + w.b * di[l][m] * (r[c] - Apply(xa, l, m, nx, ny, c, c % nx, c / nx));
+      for (int m = 0; m < CcSlots(l); ++m) {
+        const int c = t + m * kCcThreads;
+        if (c >= n) continue;
+        dst[c] = k == 0 ? w.w[0] * di[l][m] * r[c]
+                        : src[c] + w.w[k] * di[l][m] *
+                                       (r[c] - Apply(src, l, m, nx, ny, c, c % nx, c / nx));
+      }
+      __syncthreads();
     }
-    __syncthreads();
     // residual r - A xb into xa (free until the up-leg)
 #pragma unroll
     for (int m = 0; m < CcSlots(l); ++m) {
@@ -690,7 +723,9 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
     if (actT) sxa[cs.off[nlev] + t] = x;
   }
   __syncthreads();
-  // ---- up: y = xb + P e (into xa), two sweeps; the level's result in xa
+  // ---- up: y = xb + P e (into Buf(-1): xb in place for an odd kSweeps, each
+  // thread reading and writing only its own cells), kSweeps sweeps; the
+  // level's result in xa
 #pragma unroll
   for (int l = kCcLevels - 1; l >= 0; --l) {
     if (l >= nlev) continue;
@@ -698,6 +733,7 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
     const float* r = sr + cs.off[l];
     const float* e = sxa + cs.off[l + 1];
     float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
+    float* yb = Buf(xa, xb, -1, false);
 #pragma unroll
     for (int m = 0; m < CcSlots(l); ++m) {
       const int c = t + m * kCcThreads;
@@ -709,22 +745,22 @@ KCoarseCycle(CoarseSet cs, Omega2 w, const int* done) {
       const float ey = wy[l][m] > 0.f ? e[I + nxc * Jy] : ec;
       const float wd = wx[l][m] * wy[l][m];
       const float ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
-      xa[c] = xb[c] + (ec + (3.f * wx[l][m] * (ex - ec) + 3.f * wy[l][m] * (ey - ec) +
+      yb[c] = xb[c] + (ec + (3.f * wx[l][m] * (ex - ec) + 3.f * wy[l][m] * (ey - ec) +
                              wd * (ed - ec)) * 0.0625f);
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < CcSlots(l); ++m) {
-      const int c = t + m * kCcThreads;
-      if (c < n) xb[c] = xa[c] + w.a * di[l][m] * (r[c] - Apply(xa, l, m, nx, ny, c, c % nx, c / nx));
-    }
-    __syncthreads();
+    for (int k = 0; k < kSweeps; ++k) {
+      float* dst = Buf(xa, xb, k, false);
+      const float* src = Buf(xa, xb, k - 1, false);
 #pragma unroll
-    for (int m = 0; m < CcSlots(l); ++m) {
-      const int c = t + m * kCcThreads;
-      if (c < n) xa[c] = xb[c] + w.b * di[l][m] * (r[c] - Apply(xb, l, m, nx, ny, c, c % nx, c / nx));
+      for (int m = 0; m < CcSlots(l); ++m) {
+        const int c = t + m * kCcThreads;
+        if (c < n)
+          dst[c] = src[c] + w.w[k] * di[l][m] * (r[c] - Apply(src, l, m, nx, ny, c, c % nx, c / nx));
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int m = 0; m < kCcSlots; ++m) {
@@ -849,7 +885,7 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
       cs.xb[k] = C.xb;
     }
     cs.out = out;
-    KCoarseCycle<<<1, 1024, 0, st>>>(cs, Omega2{omega1, omega2}, done);
+    KCoarseCycle<<<1, 1024, 0, st>>>(cs, sweeps, done);
     return;
   }
   const bool coarsest = (l == int(lv_.size()) - 1);
@@ -894,16 +930,16 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
   // with the FCG update folded into level 0 its stop test moves to level 1
   const bool up = fz.up_on != 0;
   if (l == 0 && up)
-    KDown<double, kTallY, true><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+    KDown<double, kTallY, kFineSweeps, true><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, fine_sweeps, L.xb, C.nx, C.ny,
         C.r, done, fz, 0);
   else if (l == 0)
-    KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+    KDown<double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+        L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, fine_sweeps, L.xb, C.nx, C.ny,
         C.r, done, fz, fused);
   else
-    KDown<float, kMidY><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
-                                                       L.r, Omega2{omega1, omega2}, L.xb, C.nx,
+    KDown<float, kMidY, kSweeps><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
+                                                       L.r, sweeps, L.xb, C.nx,
                                                        C.ny, C.r, done, fz,
                                                        (l == 1 && up) ? 1 : 0);
   // the coarser level's correction goes to its xa: its xb holds the x2 that
@@ -913,23 +949,23 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
   // workgroups' tiles cover: its output must be a third buffer
   if (out == L.xb || out == C.xa)
     throw std::logic_error("MgPrecond: up-leg output aliases its input");
-#define HG_UP(RT, B, TY, TG, R)                                                               \
-  KUp<RT, B, TY><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, \
-                                                      L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,    \
-                                                      Omega2{omega1, omega2}, out, done, fz,  \
-                                                      fused)
+#define HG_UP(RT, B, TY, SW, WT, TG, R)                                                 \
+  KUp<RT, B, TY, SW><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky,    \
+                                                          L.dinv, R, L.xb, C.xa, C.kx,   \
+                                                          C.ky, C.nx, C.ny, WT, out,     \
+                                                          done, fz, fused)
   if (l == 0) {
     // after the folded update the level's residual is the one it wrote
     const double* r0 = up ? fz.up_r : r64;
     if (bilinear)
-      HG_UP(double, true, kTallY, L.tall, r0);
+      HG_UP(double, true, kTallY, kFineSweeps, fine_sweeps, L.tall, r0);
     else
-      HG_UP(double, false, kTallY, L.tall, r0);
+      HG_UP(double, false, kTallY, kFineSweeps, fine_sweeps, L.tall, r0);
   } else {
     if (bilinear)
-      HG_UP(float, true, kMidY, L.mid, L.r);
+      HG_UP(float, true, kMidY, kSweeps, sweeps, L.mid, L.r);
     else
-      HG_UP(float, false, kMidY, L.mid, L.r);
+      HG_UP(float, false, kMidY, kSweeps, sweeps, L.mid, L.r);
   }
 #undef HG_UP
 }
@@ -953,13 +989,13 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
   MgLevel& C = lv_[1];
   auto launch = [&]() {
     if (k == "KDown<double>") {
-      KDown<double, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
-          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, Omega2{omega1, omega2}, L.xb, C.nx, C.ny,
+      KDown<double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+          L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, fine_sweeps, L.xb, C.nx, C.ny,
           C.r, nullptr, FcgFusion{}, 0);
     } else if (k == "KUp<double, true>") {
-      KUp<double, true, kTallY><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+      KUp<double, true, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,
-          Omega2{omega1, omega2}, L.xa, nullptr, FcgFusion{}, 0);
+          fine_sweeps, L.xa, nullptr, FcgFusion{}, 0);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
       Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xa, st, nullptr);
     } else {
