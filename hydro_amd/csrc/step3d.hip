@@ -12,8 +12,10 @@
 // normals, vectors to cells; cell centres and volumes) is computed on the
 // host with mesh3d.hpp's formulas and kept per face and per cell, exact for
 // any spacing.  Solvers: jacobi (linear.hpp:750-783: grid sweeps, the max
-// update per sweep folded on the device, the reference's stop rule) and cg
-// (FP64 Jacobi-PCG on the 7-point rows to pressure_tolerance_gpu).
+// update per sweep folded on the device, the reference's stop rule), cg
+// (FP64 Jacobi-PCG on the 7-point rows to pressure_tolerance_gpu) and, for
+// the pressure, fcg_mg (FP64 flexible CG with an FP32 multigrid V-cycle,
+// mg3d.h, to the same tolerance).
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +30,7 @@
 #include <vector>
 
 #include "../../include/hydro_gpu3d.h"
+#include "mg3d.h"
 
 #define H3_CHECK(x)                                                                    \
   do {                                                                                 \
@@ -483,6 +486,10 @@ __global__ void KSumK(long n, const double* k, double* part) {
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 __global__ void KDivide(double* v, double n) { *v = *v / n; }
+__global__ void KFormB(long n, const double* k, const double* mean, double* b) {
+  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (c < n) b[c] = -k[c] - *mean;
+}
 struct CgScal {
   double mean, bb, rz, pq, rr, rz2, done_rel;
   int done, iters;
@@ -811,8 +818,10 @@ struct hg3_ctx {
     allocs.push_back(p);
     return (double*)p;
   }
+  std::unique_ptr<hg3::Mg3Fcg> mg;  // fcg_mg pressure solver
   ~hg3_ctx() {
     if (st) (void)hipStreamSynchronize(st);
+    mg.reset();
     for (void* p : allocs) (void)hipFree(p);
     if (state_h) (void)hipHostFree(state_h);
     if (scal_h) (void)hipHostFree(scal_h);
@@ -1114,8 +1123,27 @@ int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   return it;
 }
 
+// fcg_mg: b = -k - mean(-k) as for cg, then FCG with the V-cycle; a
+// breakdown of the preconditioner's descent property finishes with the
+// Jacobi-PCG solve (from zero, to the same tolerance)
+int SolveFcgMg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
+  const hg_config& cfg = c->cfg;
+  const long n = c->nc;
+  const int nb = int(std::min<long>(Grid(n), kCgBlocks));
+  CgScal* sc = c->cs;
+  KSumK<<<nb, kB, 0, c->st>>>(n, k, c->part);
+  KFold<<<1, kB, 0, c->st>>>(c->part, nb, &sc->mean);
+  KDivide<<<1, 1, 0, c->st>>>(&sc->mean, double(n));
+  KFormB<<<Grid(n), kB, 0, c->st>>>(n, k, &sc->mean, c->cb);
+  const hg3::Fcg3Result res = c->mg->Solve(R.t, c->cb, x, cfg.pressure_tolerance_gpu,
+                                           cfg.pressure_max_iters_gpu, c->st);
+  if (res.failed) return res.iterations + SolveCg(c, R, k, x);
+  return res.iterations;
+}
+
 int RunSolver(hg3_ctx* c, int which, const Rows& R, const double* k, double* x) {
   if (which == HG_SOLVER_JACOBI) return SolveJacobi(c, R, k, x);
+  if (which == HG_SOLVER_FCG_MG) return SolveFcgMg(c, R, k, x);
   return SolveCg(c, R, k, x);
 }
 
@@ -1316,8 +1344,13 @@ hg_status hg3_create(const hg3_config* cfg3, hg3_ctx** out) {
                   "point and initial_sin are not run on the 3D path");
   if (cfg.linear_solver_velocity != HG_SOLVER_JACOBI)
     return refuse("linear_solver_velocity: the 3D path runs jacobi");
-  if (cfg.linear_solver_pressure != HG_SOLVER_JACOBI && cfg.linear_solver_pressure != HG_SOLVER_CG)
-    return refuse("linear_solver_pressure: the 3D path runs jacobi and cg");
+  if (cfg.linear_solver_pressure != HG_SOLVER_JACOBI && cfg.linear_solver_pressure != HG_SOLVER_CG &&
+      cfg.linear_solver_pressure != HG_SOLVER_FCG_MG)
+    return refuse("linear_solver_pressure: the 3D path runs jacobi, cg and fcg_mg");
+  if (cfg.linear_solver_pressure == HG_SOLVER_FCG_MG &&
+      !hg3::Mg3Fcg::Supports(cfg.nx, cfg.ny, cfg3->nz))
+    return refuse("linear_solver_pressure fcg_mg: Nx, Ny, Nz must halve (while even) to a "
+                  "coarsest grid of at most 4096 cells");
   hg3_ctx* c = new hg3_ctx();
   const hg_status rc = Guard3(c, [&]() {
     c->cfg3 = *cfg3;
@@ -1398,6 +1431,8 @@ hg_status hg3_create(const hg3_config* cfg3, hg3_ctx** out) {
     H3_CHECK(hipHostMalloc(&c->state_h, 4 * sizeof(int)));
     H3_CHECK(hipHostMalloc(&c->scal_h, 4 * sizeof(double)));
     H3_CHECK(hipHostMalloc(&c->cs_h, sizeof(CgScal)));
+    if (cfg.linear_solver_pressure == HG_SOLVER_FCG_MG)
+      c->mg = std::make_unique<hg3::Mg3Fcg>(cfg.nx, cfg.ny, cfg3->nz);
     c->grav[0] = cfg.gravity[0];
     c->grav[1] = cfg.gravity[1];
     c->grav[2] = cfg3->gravity_z;

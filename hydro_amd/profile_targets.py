@@ -78,11 +78,12 @@ def net(reps=5):
     n.close()
 
 
-def step3d(n=128, steps=3, warmup=2):
-    """The 3D step: cavity n^3, FP64, jacobi velocity, Jacobi-PCG pressure."""
+def step3d(n=128, steps=3, warmup=2, solver_p="fcg_mg"):
+    """The 3D step: cavity n^3, FP64, jacobi velocity; pressure by solver_p
+    (fcg_mg: flexible CG with the FP32 V-cycle; cg: Jacobi-PCG), to 1e-8."""
     from hydro_amd import workloads
     from hydro_amd.hydro import Hydro3
-    h = Hydro3(workloads.cavity3d(n, n, n, solver_p="cg", solver_v="jacobi"))
+    h = Hydro3(workloads.cavity3d(n, n, n, solver_p=solver_p, solver_v="jacobi"))
     for _ in range(warmup):
         h.step()
     t0 = time.perf_counter()
@@ -96,4 +97,6 @@ def step3d(n=128, steps=3, warmup=2):
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "step"
-    {"step": step, "kernels": kernels, "net": net, "step3d": step3d}[what]()
+    targets = {"step": step, "kernels": kernels, "net": net, "step3d": step3d,
+               "step3d_cg": lambda: step3d(solver_p="cg")}
+    targets[what]()

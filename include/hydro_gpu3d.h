@@ -13,7 +13,9 @@
  * walls on all six sides (condition_left/right/bottom/top/close/far
  * "wall vx vy vz"), gravity and a uniform force, the row solvers jacobi,
  * gauss_seidel and lu (the oracle) -- on the GPU jacobi (bitwise the
- * reference's) and cg (FP64 Jacobi-PCG to pressure_tolerance_gpu).
+ * reference's), cg (FP64 Jacobi-PCG to pressure_tolerance_gpu) and, for the
+ * pressure, fcg_mg (FP64 flexible CG with an FP32 multigrid V-cycle, to the
+ * same tolerance; Nx, Ny, Nz must halve while even to at most 4096 cells).
  * Everything else (heat, SIMPLER, dt_auto, inlets and outlets, mesh
  * velocity, surface tension, sharpening, split TVD) is refused by
  * hg3_create with HG_ERR_UNSUPPORTED.

@@ -10,6 +10,13 @@ oracle/_ref/hydro_ref3d) and the 3D oracle.
   recurrences; the 3D path refuses them at hg3_create.
 * cg (FP64 Jacobi-PCG to pressure_tolerance_gpu): cavity 24^3 against the
   oracle's cg, relative L2 per field.
+* fcg_mg (FP64 flexible CG with the FP32 V-cycle, no reference
+  counterpart): cavity 24^3 and the two-phase dam 32x8x16 (density ratio
+  1000) against the oracle's cg, both to a 1e-12 residual, L2 differences
+  relative to the velocity field's and the pressure's norms (p up to a
+  constant: the all-wall operator is singular), with a
+  small fraction of cg's iterations; grids that do not halve to a small
+  coarsest level are refused.
 """
 import numpy as np
 import pytest
@@ -61,3 +68,54 @@ def test_gpu3d_cg_against_oracle(gpu):
     print("GPU cg vs oracle cg: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
     for k, v in errs.items():
         assert v <= 1e-8, (k, v)
+
+
+def _rel_errors(h, o):
+    """Per field: the L2 difference relative to the L2 norm of the whole
+    velocity field (u, v, w together: a component that stays at round-off
+    level, as v across the depth-uniform dam, has no scale of its own) or of
+    the pressure (mean-removed)."""
+    vel = ("u", capi.HG_FIELD_VELOCITY_X), ("v", capi.HG_FIELD_VELOCITY_Y), \
+        ("w", capi.HG_FIELD_VELOCITY_Z)
+    ref = {k: o.get(f) for k, f in vel}
+    scale = float(np.sqrt(sum(np.sum(b * b) for b in ref.values())))
+    errs = {k: float(np.linalg.norm(h.get(f) - ref[k]) / scale) for k, f in vel}
+    a, b = h.get(capi.HG_FIELD_PRESSURE), o.get(capi.HG_FIELD_PRESSURE)
+    a, b = a - a.mean(), b - b.mean()
+    errs["p"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return errs
+
+
+@pytest.mark.parametrize("case", ["cavity24", "dam32x8x16"])
+def test_gpu3d_fcg_mg_against_oracle(gpu, case):
+    tight = "set double pressure_tolerance_gpu 1e-12\nset int pressure_max_iters_gpu 4000\n"
+    if case == "cavity24":
+        make = lambda sp: workloads.cavity3d(24, 24, 24, solver_p=sp, extra=tight)
+        steps = 5
+    else:
+        make = lambda sp: workloads.dam3d(32, 8, 16, solver_p=sp, extra=tight)
+        steps = 4
+    print(f"case: {case} fcg_mg against the oracle's cg, {steps} steps", flush=True)
+    o = pyoracle.Oracle3(capi.parse_config3(make("cg")))
+    it_mg, it_cg = 0, 0
+    with Hydro3(make("fcg_mg")) as h:
+        for _ in range(steps):
+            sg, so = h.step(), o.step()
+            assert sg.simple_iterations == so.simple_iterations
+            it_mg += sg.pressure_solve_iters_total
+            it_cg += so.pressure_solve_iters_total
+        errs = _rel_errors(h, o)
+    o.close()
+    print(f"GPU fcg_mg vs oracle cg ({it_mg} against {it_cg} pressure iterations): " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-8, (k, v)
+    assert 0 < it_mg * 8 < it_cg
+
+
+def test_gpu3d_fcg_mg_refuses_grids_without_a_small_coarsest_level(gpu):
+    # 25^3 is odd at once: its coarsest level would be all 15625 cells
+    with pytest.raises(Unsupported, match="fcg_mg"):
+        Hydro3(workloads.cavity3d(25, 25, 25, solver_p="fcg_mg"))
+    # 26 x 26 x 26 halves once to 13^3 = 2197 cells: accepted
+    Hydro3(workloads.cavity3d(26, 26, 26, solver_p="fcg_mg")).close()
