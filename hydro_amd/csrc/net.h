@@ -7,8 +7,9 @@
 // coarser output plus the encoder output, D2 64->32, D1 32->16 (+ ReLU),
 // D0 16->1.  Convolutions are 'same' 3x3 with zero padding and no bias, so
 // the network is positively homogeneous.  The 16..64-channel convolutions
-// run as BF16 implicit GEMMs on MFMA (v_mfma_f32_16x16x16_bf16) with FP32
-// accumulation; activations are BF16 NHWC, master weights FP32 (Adam).
+// (forward, data gradient and weight gradient) run as BF16 implicit GEMMs on
+// gfx950's MFMA v_mfma_f32_16x16x32_bf16 with FP32 accumulation;
+// activations are BF16 NHWC, master weights FP32 (Adam).
 // Weight packing: per layer W[tap][ci][co], tap = 3*ky+kx, offsets
 // (kx-1, ky-1), layers in the order E0..E3, D3..D0.
 #pragma once

@@ -24,6 +24,7 @@ namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));  // one lane's 16x16x32 operand
 using bf = uint16_t;
 
 __device__ __forceinline__ bool Done(const int* d) { return d && *d; }
@@ -52,23 +53,29 @@ __device__ __forceinline__ uint4 Pack8(const float f[8]) {
   return uint4{w[0], w[1], w[2], w[3]};
 }
 __device__ __forceinline__ uint4 Ld8(const bf* p) { return *reinterpret_cast<const uint4*>(p); }
+// 8 BF16 values (16 bytes, 16-byte aligned) as an MFMA operand
+__device__ __forceinline__ b8 LdOp(const bf* p) { return __builtin_bit_cast(b8, Ld8(p)); }
 __device__ __forceinline__ void St8(bf* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
 
 
 // ---------------------------------------------------------------------------
 // 3x3 'same' convolution, CIN, COUT multiples of 16, as an implicit GEMM on
-// MFMA 16x16x16 BF16 (FP32 accumulate).  A block of 4 waves computes a
-// 64 x 4 pixel tile (one row per wave) for all output channels.  The input
-// tile with its one-pixel halo is staged once in LDS (16-byte global loads,
-// rows padded by 4 channels against bank conflicts); per (tap, 16 input
-// channels) step a wave multiplies each weight block (M = 16 output
-// channels, K = 16 input channels, read once and used kSeg times) by kSeg
-// input blocks from LDS (K x N = 16 pixels each).  Operand lane layouts:
-// A[i][k] in lane i + 16*(k/4), element k%4 (weights wt[tap][co][ci], 8
-// contiguous bytes per lane); B[k][j] in lane j + 16*(k/4); D[i][j] in lane
-// j + 16*(i/4), element i%4, so each lane ends with 4 consecutive output
-// channels of one pixel and stores them as one 8-byte word.  Used for the
-// forward pass and, with tap-flipped weights, for dgrad.
+// gfx950's MFMA 16x16x32 BF16 (FP32 accumulate).  A block of 4 waves
+// computes a 64 x 4 pixel tile (one row per wave) for all output channels.
+// The input tile with its one-pixel halo is staged once in LDS (16-byte
+// global loads and LDS stores; pixels padded to CIN + 8 channels, a 16-byte
+// multiple, so every operand read is one aligned 16-byte LDS load).  The
+// GEMM's K runs over (tap, input channel), k = tap * CIN + ci, 32 at a time:
+// per chunk a wave multiplies each weight block (M = 16 output channels,
+// read once and used kSeg times) by kSeg input blocks from LDS (N = 16
+// pixels each); with CIN = 16 a chunk spans two taps, and the chunk past
+// the ninth tap is zero.  Operand lane layouts (checked on the device):
+// A[i][k] in lane i + 16*(k/8), element k%8 (8 consecutive channels of one
+// tap: 16 contiguous bytes of wt[tap][co][ci]); B[k][j] in lane j +
+// 16*(k/8), element k%8; D[i][j] in lane j + 16*(i/4), element i%4, so each
+// lane ends with 4 consecutive output channels of one pixel and stores them
+// as one 8-byte word.  Used for the forward pass and, with tap-flipped
+// weights, for dgrad.
 constexpr int kSeg = 4;           // 16-pixel blocks per wave: 64-pixel rows
 constexpr int kCTX = 16 * kSeg;   // tile width
 constexpr int kCTY = 4;           // tile height (one row per wave)
@@ -78,7 +85,8 @@ __global__ void __launch_bounds__(256)
 KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
           bf* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  constexpr int CP = CIN + 4;  // padded channel stride in LDS
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+  constexpr int CP = CIN + 8;  // padded channel stride in LDS (16-byte multiple)
   constexpr int HX = kCTX + 2, HY = kCTY + 2;
   __shared__ __attribute__((aligned(16))) bf sin_[HY * HX * CP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -93,12 +101,7 @@ KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
     const int xx = x0 + hx - 1, yy = y0 + hy - 1;
     uint4 v = uint4{0u, 0u, 0u, 0u};
     if (xx >= 0 && xx < W && yy >= 0 && yy < H) v = Ld8(in + (size_t(yy) * W + xx) * CIN + g * 8);
-    // CP*2 bytes per pixel is a multiple of 8: two 8-byte stores
-    unsigned* dst = reinterpret_cast<unsigned*>(sin_ + pix * CP + g * 8);
-    dst[0] = v.x;
-    dst[1] = v.y;
-    dst[2] = v.z;
-    dst[3] = v.w;
+    St8(sin_ + pix * CP + g * 8, v);
   }
   __syncthreads();
   const int ty = y0 + wave;
@@ -109,25 +112,26 @@ KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
   for (int s = 0; s < kSeg; ++s)
 #pragma unroll
     for (int n = 0; n < COUT / 16; ++n) acc[s][n] = f4{0.f, 0.f, 0.f, 0.f};
+  constexpr int NQ = (9 * CIN + 31) / 32;  // K chunks
+  const b8 zero = __builtin_bit_cast(b8, uint4{0u, 0u, 0u, 0u});
 #pragma unroll
-  for (int t = 0; t < 9; ++t) {
+  for (int q = 0; q < NQ; ++q) {
+    // this lane's 8 values of the chunk: channels ci0..ci0+7 of tap t
+    const int kk = 32 * q + 8 * kq;
+    const int t = kk / CIN, ci0 = kk % CIN;
+    const bool live = t < 9;
     const int hy = wave + t / 3;
+    b8 bw[COUT / 16];
 #pragma unroll
-    for (int cc = 0; cc < CIN / 16; ++cc) {
-      s4 bw[COUT / 16];
+    for (int n = 0; n < COUT / 16; ++n)
+      bw[n] = live ? LdOp(wt + size_t(t * COUT + n * 16 + pj) * CIN + ci0) : zero;
+#pragma unroll
+    for (int s = 0; s < kSeg; ++s) {
+      const int hx = 16 * s + pj + t % 3;
+      const b8 xin = live ? LdOp(sin_ + (hy * HX + hx) * CP + ci0) : zero;
 #pragma unroll
       for (int n = 0; n < COUT / 16; ++n)
-        bw[n] = *reinterpret_cast<const s4*>(
-            wt + (size_t(t * COUT + n * 16 + pj) * CIN + cc * 16 + kq * 4));
-#pragma unroll
-      for (int s = 0; s < kSeg; ++s) {
-        const int hx = 16 * s + pj + t % 3;
-        const s4 xin =
-            *reinterpret_cast<const s4*>(sin_ + (hy * HX + hx) * CP + cc * 16 + kq * 4);
-#pragma unroll
-        for (int n = 0; n < COUT / 16; ++n)
-          acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[n], xin, acc[s][n], 0, 0, 0);
-      }
+        acc[s][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[n], xin, acc[s][n], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -149,45 +153,65 @@ KConvMfma(int W, int H, const bf* __restrict__ in, const bf* __restrict__ wt,
 }
 
 // Weight gradient dW[tap][ci][co] += sum_p in[p + off(tap)][ci] g[p][co]:
-// per tap an implicit GEMM with M = 16 input channels, N = COUT, K = pixels.
-// Grid: x = tap * (CIN/16) + channel tile, y = row blocks of `rows` rows.
+// per tap an implicit GEMM with M = 16 input channels, N = COUT, K = pixels,
+// 32 pixels per MFMA 16x16x32 BF16.  Grid: x = tap * (CIN/16) + channel
+// tile, y = row blocks of `rows` rows.  The block walks its rows in chunks of
+// kWgCols pixels: it stages the chunk's input pixels (shifted by the tap's
+// offset, zero outside the grid; 16 channels) and output gradients (COUT
+// channels) in LDS channel-major, from 16-byte global loads, so that each
+// lane's operand -- 8 consecutive pixels of one channel -- is one aligned
+// 16-byte LDS load; wave w takes the chunk's pixels 32 w .. 32 w + 31.
+constexpr int kWgCols = 128;                 // pixels per staged chunk (4 waves x 32)
+constexpr int kWgStride = kWgCols + 8;       // LDS row stride (16-byte multiple)
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256)
 KConvWgrad(int W, int H, int rows, const bf* __restrict__ in, const bf* __restrict__ g,
            float* __restrict__ dw) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+  __shared__ __attribute__((aligned(16))) bf sa[16 * kWgStride];
+  __shared__ __attribute__((aligned(16))) bf sb[COUT * kWgStride];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = blockIdx.x / (CIN / 16), ct = blockIdx.x % (CIN / 16);
   const int dy = t / 3 - 1, dx = t % 3 - 1;
-  const int ci = ct * 16 + (lane & 15);
   const int kq = lane >> 4, pj = lane & 15;
   f4 acc[COUT / 16];
 #pragma unroll
   for (int n = 0; n < COUT / 16; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
-  const int tiles_x = (W + 15) >> 4;
-  const int y0 = blockIdx.y * rows;
-  const int ntile = rows * tiles_x;
-  for (int k = wave; k < ntile; k += 4) {
-    const int y = y0 + k / tiles_x;
-    if (y >= H) break;
-    const int x0 = (k % tiles_x) << 4;
-    s4 a;
+  const int y0 = blockIdx.y * rows, y1 = min(H, y0 + rows);
+  for (int y = y0; y < y1; ++y) {
     const int yy = y + dy;
+    for (int x0 = 0; x0 < W; x0 += kWgCols) {
+      // input: 16 channels = two 8-channel loads per pixel
+      for (int q = threadIdx.x; q < kWgCols * 2; q += 256) {
+        const int px = q >> 1, half = q & 1;
+        const int x = x0 + px, xx = x + dx;
+        uint4 v = uint4{0u, 0u, 0u, 0u};
+        if (x < W && xx >= 0 && xx < W && yy >= 0 && yy < H)
+          v = Ld8(in + (size_t(yy) * W + xx) * CIN + ct * 16 + half * 8);
+        float f[8];
+        Unpack8(v, f);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int px = x0 + kq * 4 + v;
-      const int xx = px + dx;
-      const bool ok = px < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
-      a[v] = ok ? short(in[(size_t(yy) * W + xx) * CIN + ci]) : short(0);
-    }
-#pragma unroll
-    for (int n = 0; n < COUT / 16; ++n) {
-      s4 b;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int px = x0 + kq * 4 + v;
-        b[v] = px < W ? short(g[(size_t(y) * W + px) * COUT + n * 16 + pj]) : short(0);
+        for (int e = 0; e < 8; ++e) sa[(half * 8 + e) * kWgStride + px] = bf(__float_as_uint(f[e]) >> 16);
       }
-      acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[n], 0, 0, 0);
+      // output gradient: COUT channels, 8 per load
+      for (int q = threadIdx.x; q < kWgCols * (COUT / 8); q += 256) {
+        const int px = q / (COUT / 8), grp = q % (COUT / 8);
+        const int x = x0 + px;
+        uint4 v = uint4{0u, 0u, 0u, 0u};
+        if (x < W) v = Ld8(g + (size_t(y) * W + x) * COUT + grp * 8);
+        float f[8];
+        Unpack8(v, f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sb[(grp * 8 + e) * kWgStride + px] = bf(__float_as_uint(f[e]) >> 16);
+      }
+      __syncthreads();
+      const int k0 = 32 * wave + 8 * kq;  // this lane's 8 pixels of the wave's 32
+      const b8 a = LdOp(sa + pj * kWgStride + k0);
+#pragma unroll
+      for (int n = 0; n < COUT / 16; ++n)
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, LdOp(sb + (n * 16 + pj) * kWgStride + k0),
+                                                         acc[n], 0, 0, 0);
+      __syncthreads();  // the chunk's LDS is restaged next
     }
   }
   // reduce the four waves in LDS, then one atomic add per weight
