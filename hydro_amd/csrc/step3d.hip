@@ -66,7 +66,6 @@ struct G {
   const double *fc[3], *surf[3], *nrm[3], *to0[3], *to1[3];
   const double *area, *vol;
   const double *cc[3];
-  const long *fcm, *fcp;
 };
 
 __device__ __forceinline__ long FaceOf(const G& g, long c, int q) {
@@ -81,7 +80,38 @@ __device__ __forceinline__ long FaceOf(const G& g, long c, int q) {
   }
 }
 __device__ __forceinline__ double Out(int q) { return (q % 2 == 0) ? -1. : 1.; }
-__device__ __forceinline__ bool Inner(const G& g, long f) { return g.fcm[f] >= 0 && g.fcp[f] >= 0; }
+// Cells on the minus and plus side of face f (-1 outside the grid), from the
+// face number of the structured mesh (x-faces, then y-faces, then z-faces,
+// each with x fastest): the same maps as the host's fcm / fcp arrays
+// (BuildGeometry), computed instead of loaded -- two 8-byte loads per face
+// were most of the face kernels' memory traffic.  32-bit arithmetic:
+// hg3_create refuses grids of 2^31 or more faces.
+struct FC {
+  long m, p;
+};
+__device__ __forceinline__ FC FaceCells(const G& g, long f) {
+  const unsigned nx = unsigned(g.nx), ny = unsigned(g.ny), nz = unsigned(g.nz);
+  const unsigned nfx = unsigned(g.nfx), nfy = unsigned(g.nfy);
+  const unsigned uf = unsigned(f);
+  if (uf < nfx) {
+    const unsigned i = uf % (nx + 1u), r = uf / (nx + 1u);  // r = j + ny k
+    const long c = long(i) + long(nx) * r;
+    return FC{i > 0u ? c - 1 : -1, i < nx ? c : -1};
+  }
+  if (uf < nfx + nfy) {
+    const unsigned q = uf - nfx;  // i + nx (j + (ny + 1) k)
+    const unsigned i = q % nx, t = q / nx, j = t % (ny + 1u), k = t / (ny + 1u);
+    const long c = long(i) + long(nx) * (long(j) + long(ny) * k);
+    return FC{j > 0u ? c - long(nx) : -1, j < ny ? c : -1};
+  }
+  const unsigned q = uf - nfx - nfy;  // i + nx (j + ny k), k = 0..nz
+  const unsigned nxy = nx * ny, k = q / nxy;
+  return FC{k > 0u ? long(q) - long(nxy) : -1, k < nz ? long(q) : -1};
+}
+__device__ __forceinline__ bool Inner(const G& g, long f) {
+  const FC fc = FaceCells(g, f);
+  return fc.m >= 0 && fc.p >= 0;
+}
 __device__ __forceinline__ double Dot3(double ax, double ay, double az, double bx, double by,
                                        double bz) {
   double s = 0;
@@ -119,7 +149,7 @@ __device__ __forceinline__ int SideOf(const G& g, long f) {
 __global__ void KInterpInner(G g, const double* __restrict__ u, double* __restrict__ res) {
   const long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
   if (f >= g.nf) return;
-  res[f] = Inner(g, f) ? u[g.fcm[f]] * (1. - 0.5) + u[g.fcp[f]] * 0.5 : 0.;
+  res[f] = Inner(g, f) ? u[FaceCells(g, f).m] * (1. - 0.5) + u[FaceCells(g, f).p] * 0.5 : 0.;
 }
 // boundary faces of one direction (x: sides 0,1; y: 2,3; z: 4,5); the
 // directions run in face order, so an extrapolated face sees the values of
@@ -149,8 +179,8 @@ __global__ void KInterpBoundary(G g, int dir, const double* __restrict__ u, doub
   const int side = SideOf(g, f);
   const int kind = bc.kind[side];
   if (kind == BC_NONE) return;
-  const int id = g.fcm[f] >= 0 ? 0 : 1;
-  const long c = id == 0 ? g.fcm[f] : g.fcp[f];
+  const int id = FaceCells(g, f).m >= 0 ? 0 : 1;
+  const long c = id == 0 ? FaceCells(g, f).m : FaceCells(g, f).p;
   const double factor = (id == 0 ? 1. : -1.);
   if (kind == BC_VALUE) {
     res[f] = bc.val[side];
@@ -276,7 +306,7 @@ __global__ void KMomentumFaces(G g, const double* __restrict__ probe, const doub
   if (f >= g.nf) return;
   const double F = probe[f], mu = ffmu[f], A_ = g.area[f];
   if (Inner(g, f)) {
-    const long cm = g.fcm[f], cp = g.fcp[f];
+    const long cm = FaceCells(g, f).m, cp = FaceCells(g, f).p;
     double am, ap, ak;
     if (F > 1e-10) {
       am = 1.;
@@ -300,7 +330,7 @@ __global__ void KMomentumFaces(G g, const double* __restrict__ probe, const doub
     e.dcp[f] = (al * (-mu)) * A_;
     e.dck[f] = (0. * (-mu)) * A_;
   } else {
-    const int id = g.fcm[f] >= 0 ? 0 : 1;
+    const int id = FaceCells(g, f).m >= 0 ? 0 : 1;
     const double val = vbc.val[SideOf(g, f)];
     e.cam[f] = e.cap[f] = 0.;
     e.cak[f] = val * F;
@@ -385,11 +415,38 @@ __global__ void KMomentumRows(G g, FaceExpr e, MomCell m, int write_coefs, Rows 
 }
 
 // ---- Jacobi (linear.hpp:750-783): one sweep x -> xn with the max |corr|
-// folded into diff[s] (non-negative doubles order as their bit patterns)
+// folded into diff[s] (non-negative doubles order as their bit patterns).
+// The stop rule after sweep s (state = {done, iters, buffer holding the
+// result}): converged (iters = s) or over the limit (iters = s + 1).
+__device__ __forceinline__ bool JacobiStop(const unsigned long long* diff, int s, double tol,
+                                           int limit, int& iters) {
+  const double d = __longlong_as_double((long long)diff[s]);
+  if (!(d > tol)) {
+    iters = s;
+    return true;
+  }
+  if (!(s < limit)) {
+    iters = s + 1;
+    return true;
+  }
+  return false;
+}
+// Sweep s first applies the stop rule of sweep s - 1 (complete: its kernel
+// has finished), so no separate check runs between sweeps: every block
+// takes the same decision from diff[s - 1], and block 0 records it.
 __global__ void KJacobiSweep(G g, Rows A, const double* __restrict__ k, const double* __restrict__ x,
                              double* __restrict__ xn, double omega, unsigned long long* diff,
-                             int s, const int* state) {
-  if (state[0]) return;
+                             int s, int* state, double tol, int limit) {
+  if (state[0]) return;  // decided after sweep s - 2 or earlier
+  int iters = 0;
+  if (s > 0 && JacobiStop(diff, s - 1, tol, limit, iters)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      state[0] = 1;
+      state[1] = iters;
+      state[2] = s & 1;  // sweep s - 1 wrote buffer s & 1
+    }
+    return;
+  }
   const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
   double m = 0.;
   if (c < g.nc) {
@@ -405,20 +462,15 @@ __global__ void KJacobiSweep(G g, Rows A, const double* __restrict__ k, const do
   if ((threadIdx.x & 63) == 0 && m > 0.)
     atomicMax(&diff[s], (unsigned long long)__double_as_longlong(m));
 }
-// the stop rule after sweep s: converged (iters = s) or over the limit
-// (iters = s + 1); state = {done, iters, buffer holding the result}
+// the stop rule after the last sweep s of a batch
 __global__ void KJacobiCheck(const unsigned long long* diff, int s, double tol, int limit,
                              int* state) {
   if (state[0]) return;
-  const double d = __longlong_as_double((long long)diff[s]);
-  if (!(d > tol)) {
+  int iters = 0;
+  if (JacobiStop(diff, s, tol, limit, iters)) {
     state[0] = 1;
-    state[1] = s;
-    state[2] = (s + 1) & 1;
-  } else if (!(s < limit)) {
-    state[0] = 1;
-    state[1] = s + 1;
-    state[2] = (s + 1) & 1;
+    state[1] = iters;
+    state[2] = (s + 1) & 1;  // sweep s wrote buffer (s + 1) & 1
   }
 }
 
@@ -570,7 +622,7 @@ __global__ void KRhieChow(G g, Faces3 fu, Faces3 ffgp, Faces3 ffr, Faces3 ffe,
   const double vfi = Dot3(fu.x[f], fu.y[f], fu.z[f], sx, sy, sz);
   double fs = vfi, fc = 0.;
   if (Inner(g, f)) {
-    const long cm = g.fcm[f], cp = g.fcp[f];
+    const long cm = FaceCells(g, f).m, cp = FaceCells(g, f).p;
     const double dn = Norm3(g.to1[0][f] - g.to0[0][f], g.to1[1][f] - g.to0[1][f],
                             g.to1[2][f] - g.to0[2][f]);
     const double wide = Dot3(ffgp.x[f] - ffr.x[f], ffgp.y[f] - ffr.y[f], ffgp.z[f] - ffr.z[f],
@@ -624,8 +676,8 @@ __global__ void KCorrectFlux(G g, const double* fstar, const double* fcoef, cons
   if (f >= g.nf) return;
   double v = fstar[f];
   if (Inner(g, f)) {
-    v += pc[g.fcm[f]] * (-fcoef[f]);
-    v += pc[g.fcp[f]] * fcoef[f];
+    v += pc[FaceCells(g, f).m] * (-fcoef[f]);
+    v += pc[FaceCells(g, f).p] * fcoef[f];
   }
   fl[f] = v;
 }
@@ -679,7 +731,7 @@ __global__ void KAdvFaces(G g, const double* __restrict__ u, const double* __res
   const long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
   if (f >= g.nf) return;
   if (Inner(g, f)) {
-    const long P = g.fcm[f], E = g.fcp[f];
+    const long P = FaceCells(g, f).m, E = FaceCells(g, f).p;
     const double v = vf[f] + 0.;
     if (v > 1e-8) {
       fu[f] = u[P] + 0.5 * Superbee(u[E] - u[P], -4. * Dot3(gx[P], gy[P], gz[P], g.to0[0][f],
@@ -693,8 +745,8 @@ __global__ void KAdvFaces(G g, const double* __restrict__ u, const double* __res
       fu[f] = 0.5 * (u[P] + u[E]);
     }
   } else {
-    const int id = g.fcm[f] >= 0 ? 0 : 1;
-    const long cc = id == 0 ? g.fcm[f] : g.fcp[f];
+    const int id = FaceCells(g, f).m >= 0 ? 0 : 1;
+    const long cc = id == 0 ? FaceCells(g, f).m : FaceCells(g, f).p;
     const double factor = (id == 0 ? 1. : -1.);
     const double al = ToDist(g, f, id) * factor;
     fu[f] = u[cc] + 0. * al;
@@ -983,16 +1035,6 @@ void BuildGeometry(hg3_ctx* c) {
   }
   g.area = up(area);
   g.vol = up(vol);
-  long* dm = nullptr;
-  long* dp = nullptr;
-  H3_CHECK(hipMalloc(&dm, size_t(nf) * sizeof(long)));
-  c->allocs.push_back(dm);
-  H3_CHECK(hipMalloc(&dp, size_t(nf) * sizeof(long)));
-  c->allocs.push_back(dp);
-  H3_CHECK(hipMemcpyAsync(dm, fcm.data(), size_t(nf) * sizeof(long), hipMemcpyHostToDevice, c->st));
-  H3_CHECK(hipMemcpyAsync(dp, fcp.data(), size_t(nf) * sizeof(long), hipMemcpyHostToDevice, c->st));
-  g.fcm = dm;
-  g.fcp = dp;
   c->Sync();  // the host vectors go out of scope
 }
 
@@ -1063,11 +1105,10 @@ int SolveJacobi(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   int s = 0;
   while (true) {
     const int batch = std::min(32, limit + 1 - s);
-    for (int b = 0; b < batch; ++b, ++s) {
+    for (int b = 0; b < batch; ++b, ++s)
       KJacobiSweep<<<Grid(c->nc), kB, 0, c->st>>>(c->g, R, k, buf[s & 1], buf[(s + 1) & 1], omega,
-                                                   c->jdiff, s, c->jstate);
-      KJacobiCheck<<<1, 1, 0, c->st>>>(c->jdiff, s, tol, limit, c->jstate);
-    }
+                                                   c->jdiff, s, c->jstate, tol, limit);
+    KJacobiCheck<<<1, 1, 0, c->st>>>(c->jdiff, s - 1, tol, limit, c->jstate);
     H3_CHECK(hipMemcpyAsync(c->state_h, c->jstate, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     c->Sync();
     if (c->state_h[0]) break;
@@ -1333,6 +1374,8 @@ hg_status hg3_create(const hg3_config* cfg3, hg3_ctx** out) {
   };
   if (cfg.abi_version != HG_ABI_VERSION) return refuse("hg3_config from another ABI version");
   if (cfg.nx < 2 || cfg.ny < 2 || cfg3->nz < 2) return refuse("Nx, Ny, Nz must be >= 2");
+  if (3.0 * (double(cfg.nx) + 1.) * (double(cfg.ny) + 1.) * (double(cfg3->nz) + 1.) >= 2147483648.)
+    return refuse("the 3D path takes grids of fewer than 2^31 faces");
   if (cfg.num_phases < 1 || cfg.num_phases > 2) return refuse("num_phases must be 1 or 2");
   for (int s = 0; s < 6; ++s)
     if (cfg3->bc_kind[s] != HG_BC_WALL) return refuse("the 3D path runs walls only");

@@ -78,7 +78,7 @@ def net(reps=5):
     n.close()
 
 
-def step3d(n=128, steps=3, warmup=2, solver_p="fcg_mg"):
+def step3d(n=128, steps=5, warmup=2, solver_p="fcg_mg"):
     """The 3D step: cavity n^3, FP64, jacobi velocity; pressure by solver_p
     (fcg_mg: flexible CG with the FP32 V-cycle; cg: Jacobi-PCG), to 1e-8."""
     from hydro_amd import workloads
