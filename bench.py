@@ -321,8 +321,9 @@ def main():
     nfx = (nx + 1) * ny
     ps = Poisson(nx, ny, kf[:nfx], kf[nfx:])
     solves = {}
+    # fcg_mg precision 0: the same V-cycle on FP64 levels (FCG is FP64 in both)
     for label, solver, prec in (("fp64_jacobi_pcg", "cg", 0), ("mixed_jacobi_pcg", "cg", 1),
-                                ("mixed_fcg_vcycle", "fcg_mg", 1)):
+                                ("mixed_fcg_vcycle", "fcg_mg", 1), ("fcg_fp64_vcycle", "fcg_mg", 0)):
         ms, its, res = [], None, None
         for rep in range(4):
             _, st = ps.solve(rhs, solver=solver, tol=1e-8, precision=prec)
@@ -456,6 +457,8 @@ def main():
         "learned_correction": train,
         "pressure_solve_ms": dict(solves, speedup_mixed_fcg_vs_fp64=round(
             solves["fp64_jacobi_pcg"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 2),
+            speedup_fp32_vcycle_vs_fp64_vcycle=round(
+                solves["fcg_fp64_vcycle"]["ms"] / solves["mixed_fcg_vcycle"]["ms"], 3),
             cpu=cpu_solves),
         "poisson_c4_512": c4,
         "field_error_vs_fp64": field_err,

@@ -44,6 +44,11 @@ class CgSolver {
   float* kx32() const { return kx32_; }
   float* ky32() const { return ky32_; }
   float* dinv32() const { return dinv32_; }
+  // The FP64 face coefficients and inverse diagonal (the FP64 V-cycle's
+  // finest level reads them in place).
+  double* kx64() const { return kx_; }
+  double* ky64() const { return ky_; }
+  double* dinv64() const { return dinv_; }
   // Solves A x = b (device pointers, FP64) from x = 0, or, with warm, from
   // the x the caller left in d_x (flexible and FP64 CG; the mixed
   // refinement always starts from 0).  Convergence is relative to ||b||.

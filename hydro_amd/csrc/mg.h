@@ -24,14 +24,18 @@ namespace hg {
 constexpr int kTallY = 16;  // tile height of the finest level's fused kernels
 constexpr int kMidY = 4;    // tile height of the coarser levels' fused kernels
 
-struct MgLevel {
+// A level's arrays, in the level type M (float: the V-cycle of the mixed
+// path; double: the same cycle in FP64, for comparison).
+template <class M>
+struct MgLevelT {
   int nx, ny;
   TileGrid tg;    // 64 x 4 tiles
   TileGrid mid;   // 64 x kMidY tiles
   TileGrid tall;  // 64 x kTallY tiles
-  float *kx = nullptr, *ky = nullptr, *dinv = nullptr;
-  float *r = nullptr, *xa = nullptr, *xb = nullptr;
+  M *kx = nullptr, *ky = nullptr, *dinv = nullptr;
+  M *r = nullptr, *xa = nullptr, *xb = nullptr;
 };
+using MgLevel = MgLevelT<float>;
 
 // Damped Jacobi sweeps of each smoothing pass (pre and post): kFineSweeps on
 // the finest level, kSweeps on the coarser ones; a fused kernel's tile
@@ -47,24 +51,30 @@ struct SweepWeights {
 
 constexpr int kMaxInKernelLevels = 12;
 constexpr int kLdsCells = 2752;  // LDS cells per vector for all in-kernel levels
-struct CoarseSet {
+template <class M>
+struct CoarseSetT {
   int count, bilinear;
   int off[kMaxInKernelLevels];  // offset of each level in the LDS vectors
   int nx[kMaxInKernelLevels], ny[kMaxInKernelLevels];
-  const float *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
-  float *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];
-  float* out;  // the first level's correction (the caller's `out`)
+  const M *kx[kMaxInKernelLevels], *ky[kMaxInKernelLevels], *dinv[kMaxInKernelLevels];
+  M *r[kMaxInKernelLevels], *xa[kMaxInKernelLevels], *xb[kMaxInKernelLevels];
+  M* out;  // the first level's correction (the caller's `out`)
 };
 
-class MgPrecond : public Precond {
+// The V-cycle with its levels' arrays and arithmetic in M (the output z is
+// FP32 either way: the Precond interface's).  MgPrecond (M = float) is the
+// preconditioner of fcg_mg / fcg_learned; MgPrecond64 runs the same kernels
+// in FP64 to measure what FP32 buys.
+template <class M>
+class MgPrecondT : public Precond {
  public:
-  // kx0, ky0, dinv0: the finest level's FP32 face coefficients and inverse
-  // diagonal, owned by the caller (CgSolver::kx32() ...) and rewritten by it
-  // before each Build.
-  MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0);
-  ~MgPrecond();
-  MgPrecond(const MgPrecond&) = delete;
-  MgPrecond& operator=(const MgPrecond&) = delete;
+  // kx0, ky0, dinv0: the finest level's face coefficients and inverse
+  // diagonal in M, owned by the caller (CgSolver::kx32() ... for float,
+  // kx64() ... for double) and rewritten by it before each Build.
+  MgPrecondT(int nx, int ny, M* kx0, M* ky0, M* dinv0);
+  ~MgPrecondT();
+  MgPrecondT(const MgPrecondT&) = delete;
+  MgPrecondT& operator=(const MgPrecondT&) = delete;
   // Coarse-level coefficients and inverse diagonals from the finest level's.
   void Build(hipStream_t st);
   // z = V(r): r is the FP64 residual of the finest level; z FP32.  Does
@@ -85,7 +95,7 @@ class MgPrecond : public Precond {
   // times back to back between two HIP events on `st`, reading r (FP64,
   // finest level) and the level buffers.  Throws on an unknown name.
   double TimeKernel(const char* name, int reps, const double* r, hipStream_t st);
-  const MgLevel& level(int l) const { return lv_[l]; }
+  const MgLevelT<M>& level(int l) const { return lv_[l]; }
   float omega = 0.8f;  // damping of the plain Jacobi sweeps (coarsest-level fallbacks)
   // Chebyshev smoothing (pre and post): damping 1/r_k at the roots r_k of
   // the Chebyshev polynomial of the sweep count on an interval [a, 2] of the
@@ -109,10 +119,13 @@ class MgPrecond : public Precond {
   bool use_coarse_kernel = true;
 
  private:
-  std::vector<MgLevel> lv_;
+  std::vector<MgLevelT<M>> lv_;
   int first_in_kernel_ = -1;
-  void Cycle(int l, const double* r64, float* out, hipStream_t st, const int* d_done,
+  // level l's correction: into z (FP32) on level 0, into out on the others
+  void Cycle(int l, const double* r64, float* z, M* out, hipStream_t st, const int* d_done,
              const FcgFusion* f = nullptr);
 };
+using MgPrecond = MgPrecondT<float>;
+using MgPrecond64 = MgPrecondT<double>;
 
 }  // namespace hg

@@ -31,46 +31,48 @@ constexpr int kTinyIters = 8;
 __device__ __forceinline__ bool Done(const int* d) { return d && *d; }
 
 // A x at (i,j) for face-form coefficients.
-template <class T>
-__device__ __forceinline__ float Ax(int nx, int ny, const float* __restrict__ kx,
-                                    const float* __restrict__ ky, const T* __restrict__ x, int i,
+template <class M, class T>
+__device__ __forceinline__ M Ax(int nx, int ny, const M* __restrict__ kx,
+                                    const M* __restrict__ ky, const T* __restrict__ x, int i,
                                     int j) {
   const int c = i + nx * j;
-  const float kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
-  const float ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
-  const float xc = float(x[c]);
-  float v = (((kw + ke) + ks) + kn) * xc;
-  if (i > 0) v -= kw * float(x[c - 1]);
-  if (i < nx - 1) v -= ke * float(x[c + 1]);
-  if (j > 0) v -= ks * float(x[c - nx]);
-  if (j < ny - 1) v -= kn * float(x[c + nx]);
+  const M kw = kx[i + (nx + 1) * j], ke = kx[i + 1 + (nx + 1) * j];
+  const M ks = ky[i + nx * j], kn = ky[i + nx * (j + 1)];
+  const M xc = M(x[c]);
+  M v = (((kw + ke) + ks) + kn) * xc;
+  if (i > 0) v -= kw * M(x[c - 1]);
+  if (i < nx - 1) v -= ke * M(x[c + 1]);
+  if (j > 0) v -= ks * M(x[c - nx]);
+  if (j < ny - 1) v -= kn * M(x[c + nx]);
   return v;
 }
 
 // Coarse cell (I, J) of a level: its west and south face coefficients (and
 // the east / north one on the last column / row), each the mean of the two
 // fine faces it covers, and its inverse diagonal.
-__device__ __forceinline__ void CoarsenCell(int nxf, const float* __restrict__ kxf,
-                                            const float* __restrict__ kyf, int nxc, int nyc,
-                                            float* __restrict__ kxc, float* __restrict__ kyc,
-                                            float* __restrict__ dinvc, int I, int J) {
+template <class M>
+__device__ __forceinline__ void CoarsenCell(int nxf, const M* __restrict__ kxf,
+                                            const M* __restrict__ kyf, int nxc, int nyc,
+                                            M* __restrict__ kxc, M* __restrict__ kyc,
+                                            M* __restrict__ dinvc, int I, int J) {
   auto fx = [&](int i, int j) { return kxf[i + (nxf + 1) * j]; };
   auto fy = [&](int i, int j) { return kyf[i + nxf * j]; };
-  const float kw = 0.5f * (fx(2 * I, 2 * J) + fx(2 * I, 2 * J + 1));
-  const float ke = 0.5f * (fx(2 * I + 2, 2 * J) + fx(2 * I + 2, 2 * J + 1));
-  const float ks = 0.5f * (fy(2 * I, 2 * J) + fy(2 * I + 1, 2 * J));
-  const float kn = 0.5f * (fy(2 * I, 2 * J + 2) + fy(2 * I + 1, 2 * J + 2));
+  const M kw = 0.5f * (fx(2 * I, 2 * J) + fx(2 * I, 2 * J + 1));
+  const M ke = 0.5f * (fx(2 * I + 2, 2 * J) + fx(2 * I + 2, 2 * J + 1));
+  const M ks = 0.5f * (fy(2 * I, 2 * J) + fy(2 * I + 1, 2 * J));
+  const M kn = 0.5f * (fy(2 * I, 2 * J + 2) + fy(2 * I + 1, 2 * J + 2));
   kxc[I + (nxc + 1) * J] = kw;
   kyc[I + nxc * J] = ks;
   if (I == nxc - 1) kxc[I + 1 + (nxc + 1) * J] = ke;
   if (J == nyc - 1) kyc[I + nxc * (J + 1)] = kn;
-  const float d = ((kw + ke) + ks) + kn;
+  const M d = ((kw + ke) + ks) + kn;
   dinvc[I + nxc * J] = d > 0.f ? 1.f / d : 0.f;
 }
 
+template <class M>
 __global__ void __launch_bounds__(256)
-KCoarsen(int nxf, const float* __restrict__ kxf, const float* __restrict__ kyf, int nxc,
-         int nyc, float* __restrict__ kxc, float* __restrict__ kyc, float* __restrict__ dinvc) {
+KCoarsen(int nxf, const M* __restrict__ kxf, const M* __restrict__ kyf, int nxc,
+         int nyc, M* __restrict__ kxc, M* __restrict__ kyc, M* __restrict__ dinvc) {
   const long n = long(nxc) * nyc;
   for (long c = blockIdx.x * long(blockDim.x) + threadIdx.x; c < n;
        c += long(gridDim.x) * blockDim.x)
@@ -80,14 +82,16 @@ KCoarsen(int nxf, const float* __restrict__ kxf, const float* __restrict__ kyf, 
 // Levels 0..count-1 of a chain of small grids: level k (>= 1) coarsened from
 // level k-1, all in one workgroup, a barrier between levels.
 constexpr int kMaxChain = 16;
+template <class M>
 struct ChainSet {
   int count;
   int nx[kMaxChain], ny[kMaxChain];
-  float *kx[kMaxChain], *ky[kMaxChain], *dinv[kMaxChain];
+  M *kx[kMaxChain], *ky[kMaxChain], *dinv[kMaxChain];
 };
 
+template <class M>
 __global__ void __launch_bounds__(1024)
-KCoarsenChain(ChainSet cs) {
+KCoarsenChain(ChainSet<M> cs) {
   for (int k = 1; k < cs.count; ++k) {
     const int nxc = cs.nx[k], nyc = cs.ny[k];
     for (int c = threadIdx.x; c < nxc * nyc; c += blockDim.x)
@@ -99,29 +103,29 @@ KCoarsenChain(ChainSet cs) {
 }
 
 // First pre-smoothing sweep from zero: x = omega D^-1 r.
-template <class RT>
+template <class RT, class M>
 __global__ void __launch_bounds__(kTileThreads)
 KSmoothFirst(int nx, int ny, TileGrid tg, const RT* __restrict__ r,
-             const float* __restrict__ dinv, float omega, float* __restrict__ x,
+             const M* __restrict__ dinv, M omega, M* __restrict__ x,
              const int* done) {
   if (Done(done)) return;
   int i, j;
   if (!TileCell(tg, nx, ny, i, j)) return;
   const int c = i + nx * j;
-  x[c] = omega * dinv[c] * float(r[c]);
+  x[c] = omega * dinv[c] * M(r[c]);
 }
 
 // Damped Jacobi sweep: xn = x + omega D^-1 (r - A x).
-template <class RT>
+template <class RT, class M>
 __global__ void __launch_bounds__(kTileThreads)
-KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
-        const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x,
-        float omega, float* __restrict__ xn, const int* done) {
+KJacobi(int nx, int ny, TileGrid tg, const M* __restrict__ kx, const M* __restrict__ ky,
+        const M* __restrict__ dinv, const RT* __restrict__ r, const M* __restrict__ x,
+        M omega, M* __restrict__ xn, const int* done) {
   if (Done(done)) return;
   int i, j;
   if (!TileCell(tg, nx, ny, i, j)) return;
   const int c = i + nx * j;
-  xn[c] = x[c] + omega * dinv[c] * (float(r[c]) - Ax(nx, ny, kx, ky, x, i, j));
+  xn[c] = x[c] + omega * dinv[c] * (M(r[c]) - Ax(nx, ny, kx, ky, x, i, j));
 }
 
 // Coarse correction at a fine cell.  Injection: the parent's value.
@@ -132,20 +136,22 @@ KJacobi(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* 
 // coefficients within a factor 4 of each other give the 9/16, 3/16, 3/16,
 // 1/16 bilinear weights; a wall (k = 0) or a weakly coupled neighbour across
 // a strong density jump gives injection in that direction.
-__device__ __forceinline__ void SideWeights(float kw, float ke, float ks, float kn,
-                                            float (&wt)[4]) {
-  const float kmax = fmaxf(fmaxf(kw, ke), fmaxf(ks, kn));
+template <class M>
+__device__ __forceinline__ void SideWeights(M kw, M ke, M ks, M kn,
+                                            M (&wt)[4]) {
+  const M kmax = fmax(fmax(kw, ke), fmax(ks, kn));
   const bool any = kmax > 0.f;
-  wt[0] = any ? fminf(1.f, 4.f * kw / kmax) : 0.f;
-  wt[1] = any ? fminf(1.f, 4.f * ke / kmax) : 0.f;
-  wt[2] = any ? fminf(1.f, 4.f * ks / kmax) : 0.f;
-  wt[3] = any ? fminf(1.f, 4.f * kn / kmax) : 0.f;
+  wt[0] = any ? fmin(1.f, 4.f * kw / kmax) : 0.f;
+  wt[1] = any ? fmin(1.f, 4.f * ke / kmax) : 0.f;
+  wt[2] = any ? fmin(1.f, 4.f * ks / kmax) : 0.f;
+  wt[3] = any ? fmin(1.f, 4.f * kn / kmax) : 0.f;
 }
-__device__ __forceinline__ float Bilinear(float ec, float ex, float ey, float ed, float wx,
-                                          float wy) {
+template <class M>
+__device__ __forceinline__ M Bilinear(M ec, M ex, M ey, M ed, M wx,
+                                          M wy) {
   ex = wx > 0.f ? ex : ec;
   ey = wy > 0.f ? ey : ec;
-  const float wd = wx * wy;
+  const M wd = wx * wy;
   ed = wd > 0.f ? ed : ec;
   return ec + (3.f * wx * (ex - ec) + 3.f * wy * (ey - ec) + wd * (ed - ec)) * 0.0625f;
 }
@@ -190,25 +196,26 @@ struct TileSlots {
 
 // One slot's cell: position, grid validity and (for the tile and the rings
 // inside the outermost) its faces and inverse diagonal.
+template <class M>
 struct SlotCell {
   int li, lj, i, j;
   bool mine;  // this thread has a cell in the slot
   bool in;    // ... and it lies inside the grid
-  float kw, ke, ks, kn, di;
+  M kw, ke, ks, kn, di;
 };
 
 // S: the tile's TileSlots<TY, SW>.
-template <class S, class RT>
-__device__ __forceinline__ void LoadSlots(SlotCell (&sc)[S::N], RT (&rv)[S::N],
+template <class S, class RT, class M>
+__device__ __forceinline__ void LoadSlots(SlotCell<M> (&sc)[S::N], RT (&rv)[S::N],
                                           int nx, int ny, int i0, int j0,
-                                          const float* __restrict__ kx,
-                                          const float* __restrict__ ky,
-                                          const float* __restrict__ dinv,
+                                          const M* __restrict__ kx,
+                                          const M* __restrict__ ky,
+                                          const M* __restrict__ dinv,
                                           const RT* __restrict__ r, bool r_on_last_ring) {
   constexpr int SW = S::N - S::NT;
 #pragma unroll
   for (int s = 0; s < S::N; ++s) {
-    SlotCell& c = sc[s];
+    SlotCell<M>& c = sc[s];
     c.li = c.lj = 0;
     c.mine = S::Pos(s, c.li, c.lj);
     c.i = i0 + c.li;
@@ -226,11 +233,11 @@ __device__ __forceinline__ void LoadSlots(SlotCell (&sc)[S::N], RT (&rv)[S::N],
 }
 
 // (A x) at slot cell c from LDS values x, the cell at x[lj + O][li + O].
-template <int O, int XW>
-__device__ __forceinline__ float SlotApply(const SlotCell& c, const float (*x)[XW], int nx,
+template <int O, int XW, class M>
+__device__ __forceinline__ M SlotApply(const SlotCell<M>& c, const M (*x)[XW], int nx,
                                            int ny) {
   const int a = c.lj + O, f = c.li + O;
-  float v = (((c.kw + c.ke) + c.ks) + c.kn) * x[a][f];
+  M v = (((c.kw + c.ke) + c.ks) + c.kn) * x[a][f];
   if (c.i > 0) v -= c.kw * x[a][f - 1];
   if (c.i < nx - 1) v -= c.ke * x[a][f + 1];
   if (c.j > 0) v -= c.ks * x[a - 1][f];
@@ -242,8 +249,8 @@ __device__ __forceinline__ float SlotApply(const SlotCell& c, const float (*x)[X
 // (FcgFusion::up_on), alpha given: r - alpha q replaces the slots' residual
 // (halo included, so the down-leg smooths the updated residual), and the
 // tile cells write r, x += alpha p and contribute (r, r).
-template <class S>
-__device__ __forceinline__ void FoldedUpdate(const FcgFusion& f, const SlotCell (&sc)[S::N],
+template <class S, class M>
+__device__ __forceinline__ void FoldedUpdate(const FcgFusion& f, const SlotCell<M> (&sc)[S::N],
                                              double (&rv)[S::N], int nx) {
   double qv[S::N];
 #pragma unroll
@@ -282,11 +289,11 @@ __device__ __forceinline__ void FoldedUpdate(const FcgFusion& f, const SlotCell 
 // global loads are issued at the start (TileSlots), so a workgroup waits on
 // memory once.  UP (FP64 residual only): the previous FCG update first
 // (FoldedUpdate).
-template <class RT, int TY, int SW, bool UP = false>
+template <class M, class RT, int TY, int SW, bool UP = false>
 __global__ void __launch_bounds__(kTileThreads)
-KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
-      const float* __restrict__ dinv, const RT* __restrict__ r, SweepWeights w,
-      float* __restrict__ x2out, int nxc, int nyc, float* __restrict__ rc, const int* done,
+KDown(int nx, int ny, TileGrid tg, const M* __restrict__ kx, const M* __restrict__ ky,
+      const M* __restrict__ dinv, const RT* __restrict__ r, SweepWeights w,
+      M* __restrict__ x2out, int nxc, int nyc, M* __restrict__ rc, const int* done,
       FcgFusion fcg, int stop_test) {
   static_assert(!UP || std::is_same<RT, double>::value, "the update is on the FP64 residual");
   // the stop flag is read with the data: its load is issued first and
@@ -295,7 +302,7 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
   const int stopped = done ? *done : 0;
   using S = TileSlots<TY, SW>;
   constexpr int H = SW, XW = kTX + 2 * H, YH = TY + 2 * H;
-  __shared__ float sx[2][YH][XW], sres[TY][kTX];
+  __shared__ M sx[2][YH][XW], sres[TY][kTX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) {  // uniform over the block
@@ -303,42 +310,42 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
     return;
   }
   const int i0 = (tile % tg.ntx) * kTX, j0 = (tile / tg.ntx) * TY;
-  SlotCell sc[S::N];
+  SlotCell<M> sc[S::N];
   RT rv[S::N];
-  LoadSlots<S, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
+  LoadSlots<S, RT, M>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, true);
   if (stopped) return;
   if (stop_test && blockIdx.x == 0) FcgStopTest(fcg);
-  if constexpr (UP) FoldedUpdate<S>(fcg, sc, rv, nx);
+  if constexpr (UP) FoldedUpdate<S, M>(fcg, sc, rv, nx);
   // x1 on the tile and rings 1..H
 #pragma unroll
   for (int s = 0; s < S::N; ++s)
-    if (sc[s].mine) sx[0][sc[s].lj + H][sc[s].li + H] = sc[s].in ? w.w[0] * sc[s].di * float(rv[s]) : 0.f;
+    if (sc[s].mine) sx[0][sc[s].lj + H][sc[s].li + H] = sc[s].in ? w.w[0] * sc[s].di * M(rv[s]) : 0.f;
   __syncthreads();
   // x_k+1 on the tile and rings 1..H-k, from x_k (one ring wider)
 #pragma unroll
   for (int k = 1; k < H; ++k) {
-    const float(*src)[XW] = sx[(k - 1) & 1];
-    float(*dst)[XW] = sx[k & 1];
+    const M(*src)[XW] = sx[(k - 1) & 1];
+    M(*dst)[XW] = sx[k & 1];
 #pragma unroll
     for (int s = 0; s < S::NT + H - k; ++s) {
-      const SlotCell& c = sc[s];
+      const SlotCell<M>& c = sc[s];
       if (!c.mine) continue;
-      float xv = 0.f;
+      M xv = 0.f;
       if (c.in)
         xv = src[c.lj + H][c.li + H] +
-             w.w[k] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
+             w.w[k] * c.di * (M(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
       dst[c.lj + H][c.li + H] = xv;
     }
     __syncthreads();
   }
-  const float(*xs)[XW] = sx[(H - 1) & 1];
+  const M(*xs)[XW] = sx[(H - 1) & 1];
 #pragma unroll
   for (int s = 0; s < S::NT; ++s) {
-    const SlotCell& c = sc[s];
-    float res = 0.f;
+    const SlotCell<M>& c = sc[s];
+    M res = 0.f;
     if (c.in) {
       x2out[c.i + nx * c.j] = xs[c.lj + H][c.li + H];
-      res = float(rv[s]) - SlotApply<H, XW>(c, xs, nx, ny);
+      res = M(rv[s]) - SlotApply<H, XW>(c, xs, nx, ny);
     }
     sres[c.lj][c.li] = res;
   }
@@ -360,12 +367,12 @@ KDown(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __
 // cells' side weights of the covering coarse region are staged in LDS; x2,
 // r and the fine coefficients of the thread's cells (TileSlots) are loaded
 // at the start with them.
-template <class RT, bool BILINEAR, int TY, int SW>
+template <class M, class OT, class RT, bool BILINEAR, int TY, int SW>
 __global__ void __launch_bounds__(kTileThreads)
-KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __restrict__ ky,
-    const float* __restrict__ dinv, const RT* __restrict__ r, const float* __restrict__ x2,
-    const float* __restrict__ e, const float* __restrict__ kxc, const float* __restrict__ kyc,
-    int nxc, int nyc, SweepWeights w, float* __restrict__ out, const int* done, FcgFusion fcg,
+KUp(int nx, int ny, TileGrid tg, const M* __restrict__ kx, const M* __restrict__ ky,
+    const M* __restrict__ dinv, const RT* __restrict__ r, const M* __restrict__ x2,
+    const M* __restrict__ e, const M* __restrict__ kxc, const M* __restrict__ kyc,
+    int nxc, int nyc, SweepWeights w, OT* __restrict__ out, const int* done, FcgFusion fcg,
     int dots) {
   const int stopped = done ? *done : 0;  // tested once the loads are in flight
   using S = TileSlots<TY, SW>;
@@ -378,8 +385,8 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   static_assert(SW >= 1 && SW <= 3, "coarse region covers halos of 1 to 3 cells");
   constexpr int CX = kTX / 2 + 4, CY = TY / 2 + 4, NC = CX * CY;
   constexpr int KC = (NC + kTileThreads - 1) / kTileThreads;
-  __shared__ float sx[2][YH][XW];
-  __shared__ float se[CY][CX], sw[BILINEAR ? 4 : 1][CY][CX];
+  __shared__ M sx[2][YH][XW];
+  __shared__ M se[CY][CX], sw[BILINEAR ? 4 : 1][CY][CX];
   const int b = blockIdx.x;
   const int tile = (b % kXcds) * tg.per + b / kXcds;
   if (tile >= tg.ntx * tg.nty) {  // uniform over the block
@@ -390,7 +397,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   const int ci0 = i0 / 2 - 2, cj0 = j0 / 2 - 2;
   // coarse loads, clamped into the coarse grid: cells outside are read only
   // behind a zero weight (walls), where the value is not used
-  float ce[KC], cf[KC][4];
+  M ce[KC], cf[KC][4];
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
     const int q = threadIdx.x + k * kTileThreads;
@@ -404,11 +411,11 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
       cf[k][3] = ok ? kyc[I + nxc * (J + 1)] : 0.f;
     }
   }
-  SlotCell sc[S::N];
+  SlotCell<M> sc[S::N];
   RT rv[S::N];
-  float xv2[S::N];
+  M xv2[S::N];
   double qo[S::NT];
-  LoadSlots<S, RT>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
+  LoadSlots<S, RT, M>(sc, rv, nx, ny, i0, j0, kx, ky, dinv, r, false);
 #pragma unroll
   for (int s = 0; s < S::N; ++s) xv2[s] = sc[s].in ? x2[sc[s].i + nx * sc[s].j] : 0.f;
 #pragma unroll
@@ -421,7 +428,7 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
     if (q >= NC) continue;
     se[q / CX][q % CX] = ce[k];
     if (BILINEAR) {
-      float wt[4];
+      M wt[4];
       SideWeights(cf[k][0], cf[k][1], cf[k][2], cf[k][3], wt);
 #pragma unroll
       for (int m = 0; m < 4; ++m) sw[m][q / CX][q % CX] = wt[m];
@@ -431,16 +438,16 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   // y = x2 + P e on the tile and rings 1..H (zero outside the grid)
 #pragma unroll
   for (int s = 0; s < S::N; ++s) {
-    const SlotCell& c = sc[s];
+    const SlotCell<M>& c = sc[s];
     if (!c.mine) continue;
-    float yv = 0.f;
+    M yv = 0.f;
     if (c.in) {
       const int pi = (c.i >> 1) - ci0, pj = (c.j >> 1) - cj0;  // parent, local
-      const float ec = se[pj][pi];
-      float corr = ec;
+      const M ec = se[pj][pi];
+      M corr = ec;
       if (BILINEAR) {
         const int sx = c.i & 1, sy_ = c.j & 1;
-        const float wx = sw[sx ? 1 : 0][pj][pi], wy = sw[sy_ ? 3 : 2][pj][pi];
+        const M wx = sw[sx ? 1 : 0][pj][pi], wy = sw[sy_ ? 3 : 2][pj][pi];
         const int xi = pi + (sx ? 1 : -1), yj = pj + (sy_ ? 1 : -1);
         corr = Bilinear(ec, se[pj][xi], se[yj][pi], se[yj][xi], wx, wy);
       }
@@ -452,30 +459,31 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
   // y_k+1 on the tile and rings 1..H-1-k, from y_k (one ring wider)
 #pragma unroll
   for (int k = 0; k < H - 1; ++k) {
-    const float(*src)[XW] = sx[k & 1];
-    float(*dst)[XW] = sx[(k + 1) & 1];
+    const M(*src)[XW] = sx[k & 1];
+    M(*dst)[XW] = sx[(k + 1) & 1];
 #pragma unroll
     for (int s = 0; s < S::NT + H - 1 - k; ++s) {
-      const SlotCell& c = sc[s];
+      const SlotCell<M>& c = sc[s];
       if (!c.mine) continue;
-      float xv = 0.f;
+      M xv = 0.f;
       if (c.in)
         xv = src[c.lj + H][c.li + H] +
-             w.w[k] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
+             w.w[k] * c.di * (M(rv[s]) - SlotApply<H, XW>(c, src, nx, ny));
       dst[c.lj + H][c.li + H] = xv;
     }
     __syncthreads();
   }
-  const float(*ys)[XW] = sx[(H - 1) & 1];
+  const M(*ys)[XW] = sx[(H - 1) & 1];
   double zq = 0., rz = 0.;
 #pragma unroll
   for (int s = 0; s < S::NT; ++s) {
-    const SlotCell& c = sc[s];
+    const SlotCell<M>& c = sc[s];
     if (!c.in) continue;
-    const float z = ys[c.lj + H][c.li + H] +
-                    w.w[H - 1] * c.di * (float(rv[s]) - SlotApply<H, XW>(c, ys, nx, ny));
-    out[c.i + nx * c.j] = z;
-    const double zd = double(z);
+    const M z = ys[c.lj + H][c.li + H] +
+                    w.w[H - 1] * c.di * (M(rv[s]) - SlotApply<H, XW>(c, ys, nx, ny));
+    const OT zo = OT(z);  // the dots use z as written
+    out[c.i + nx * c.j] = zo;
+    const double zd = double(zo);
     zq += zd * qo[s];
     rz += double(rv[s]) * zd;
   }
@@ -501,16 +509,17 @@ KUp(int nx, int ny, TileGrid tg, const float* __restrict__ kx, const float* __re
 }
 
 // Coarsest level: Jacobi sweeps in one workgroup, x and r in LDS.
+template <class M>
 __global__ void __launch_bounds__(kCoarsestMax)
-KCoarsest(int nx, int ny, const float* __restrict__ kx, const float* __restrict__ ky,
-          const float* __restrict__ dinv, const float* __restrict__ r, float omega, int sweeps,
-          float* __restrict__ out, const int* done) {
+KCoarsest(int nx, int ny, const M* __restrict__ kx, const M* __restrict__ ky,
+          const M* __restrict__ dinv, const M* __restrict__ r, M omega, int sweeps,
+          M* __restrict__ out, const int* done) {
   if (Done(done)) return;
-  __shared__ float xs[2][kCoarsestMax];
+  __shared__ M xs[2][kCoarsestMax];
   const int n = nx * ny;
   const int c = threadIdx.x;
   const bool act = c < n;
-  float kw = 0, ke = 0, ks = 0, kn = 0, di = 0, rc = 0;
+  M kw = 0, ke = 0, ks = 0, kn = 0, di = 0, rc = 0;
   int i = 0, j = 0;
   if (act) {
     i = c % nx;
@@ -527,8 +536,8 @@ KCoarsest(int nx, int ny, const float* __restrict__ kx, const float* __restrict_
   int cur = 0;
   for (int s = 1; s < sweeps; ++s) {
     if (act) {
-      const float* x = xs[cur];
-      float v = (((kw + ke) + ks) + kn) * x[c];
+      const M* x = xs[cur];
+      M v = (((kw + ke) + ks) + kn) * x[c];
       if (i > 0) v -= kw * x[c - 1];
       if (i < nx - 1) v -= ke * x[c + 1];
       if (j > 0) v -= ks * x[c - nx];
@@ -541,7 +550,8 @@ KCoarsest(int nx, int ny, const float* __restrict__ kx, const float* __restrict_
   if (act) out[c] = xs[cur][c];
 }
 
-__device__ __forceinline__ float WaveSumF(float v) {
+template <class M>
+__device__ __forceinline__ M WaveSumF(M v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
@@ -573,14 +583,15 @@ constexpr int kCcLevels = CcLevels();  // smoothed (non-coarsest) levels at most
 constexpr int kCcSlots = CcSlots(0);
 static_assert(kInKernelMax % kCcThreads == 0 || kInKernelMax < kCcThreads, "slots");
 
+template <class M>
 __global__ void __launch_bounds__(kCcThreads)
-KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
+KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
   const int stopped = done ? *done : 0;  // tested once the loads are in flight
-  __shared__ float sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
+  __shared__ M sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int t = threadIdx.x;
   const int nlev = cs.count - 1;
   // per owned cell: faces w e s n, inverse diagonal, prolongation weights
-  float kw[kCcLevels][kCcSlots], ke[kCcLevels][kCcSlots], ks[kCcLevels][kCcSlots],
+  M kw[kCcLevels][kCcSlots], ke[kCcLevels][kCcSlots], ks[kCcLevels][kCcSlots],
       kn[kCcLevels][kCcSlots], di[kCcLevels][kCcSlots], wx[kCcLevels][kCcSlots],
       wy[kCcLevels][kCcSlots];
 #pragma unroll
@@ -601,12 +612,12 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
         // weights of the coarse neighbours on this child's side (zero across
         // walls, where the coarse face coefficient is zero)
         const int nxc = cs.nx[l + 1], I = i >> 1, J = j >> 1;
-        const float cw = cs.kx[l + 1][I + (nxc + 1) * J], ce = cs.kx[l + 1][I + 1 + (nxc + 1) * J];
-        const float cs_ = cs.ky[l + 1][I + nxc * J], cn = cs.ky[l + 1][I + nxc * (J + 1)];
-        const float kmax = fmaxf(fmaxf(cw, ce), fmaxf(cs_, cn));
+        const M cw = cs.kx[l + 1][I + (nxc + 1) * J], ce = cs.kx[l + 1][I + 1 + (nxc + 1) * J];
+        const M cs_ = cs.ky[l + 1][I + nxc * J], cn = cs.ky[l + 1][I + nxc * (J + 1)];
+        const M kmax = fmax(fmax(cw, ce), fmax(cs_, cn));
         if (kmax > 0.f) {
-          wx[l][m] = fminf(1.f, 4.f * ((i & 1) ? ce : cw) / kmax);
-          wy[l][m] = fminf(1.f, 4.f * ((j & 1) ? cn : cs_) / kmax);
+          wx[l][m] = fmin(1.f, 4.f * ((i & 1) ? ce : cw) / kmax);
+          wy[l][m] = fmin(1.f, 4.f * ((j & 1) ? cn : cs_) / kmax);
         }
       }
     }
@@ -615,7 +626,7 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   const int nxT = cs.nx[nlev], nyT = cs.ny[nlev], nT = nxT * nyT;
   const bool actT = t < nT;
   const int iT = actT ? t % nxT : 0, jT = actT ? t / nxT : 0;
-  float tw = 0.f, te = 0.f, ts = 0.f, tn = 0.f, tdi = 0.f;
+  M tw = 0.f, te = 0.f, ts = 0.f, tn = 0.f, tdi = 0.f;
   if (actT) {
     tw = cs.kx[nlev][iT + (nxT + 1) * jT];
     te = cs.kx[nlev][iT + 1 + (nxT + 1) * jT];
@@ -632,8 +643,8 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   }
   __syncthreads();
   // (A x) of owned cell (l, m) at (i, j) from LDS values x (level-local)
-  auto Apply = [&](const float* x, int l, int m, int nx, int ny, int c, int i, int j) {
-    float v = (((kw[l][m] + ke[l][m]) + ks[l][m]) + kn[l][m]) * x[c];
+  auto Apply = [&](const M* x, int l, int m, int nx, int ny, int c, int i, int j) {
+    M v = (((kw[l][m] + ke[l][m]) + ks[l][m]) + kn[l][m]) * x[c];
     if (i > 0) v -= kw[l][m] * x[c - 1];
     if (i < nx - 1) v -= ke[l][m] * x[c + 1];
     if (j > 0) v -= ks[l][m] * x[c - nx];
@@ -644,7 +655,7 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   // that the last (k = kSweeps - 1) writes xb on the way down (the up-leg's
   // start) and xa on the way up (the level's result, read by the finer
   // level's prolongation).
-  auto Buf = [](float* xa, float* xb, int k, bool down) {
+  auto Buf = [](M* xa, M* xb, int k, bool down) {
     const bool last_parity = ((kSweeps - 1 - k) & 1) == 0;
     return (last_parity == down) ? xb : xa;
   };
@@ -653,12 +664,12 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   for (int l = 0; l < kCcLevels; ++l) {
     if (l >= nlev) break;
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny;
-    const float* r = sr + cs.off[l];
-    float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
+    const M* r = sr + cs.off[l];
+    M *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
 #pragma unroll
     for (int k = 0; k < kSweeps; ++k) {
-      float* dst = Buf(xa, xb, k, true);
-      const float* src = Buf(xa, xb, k - 1, true);
+      M* dst = Buf(xa, xb, k, true);
+      const M* src = Buf(xa, xb, k - 1, true);
 #pragma unroll
       for (int m = 0; m < CcSlots(l); ++m) {
         const int c = t + m * kCcThreads;
@@ -677,10 +688,10 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
     }
     __syncthreads();
     const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
-    float* rc = sr + cs.off[l + 1];
+    M* rc = sr + cs.off[l + 1];
     for (int C = t; C < nc; C += kCcThreads) {
       const int I = C % nxc, J = C / nxc;
-      float sum = 0.f;
+      M sum = 0.f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) sum += xa[2 * I + (q & 1) + nx * (2 * J + (q >> 1))];
       rc[C] = sum;
@@ -689,37 +700,37 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   }
   // ---- coarsest: Jacobi-PCG in wave 0, result into sxa
   if (t < 64) {
-    const float* r = sr + cs.off[nlev];
-    float b = actT ? r[t] : 0.f;
-    const float dg = ((tw + te) + ts) + tn;
+    const M* r = sr + cs.off[nlev];
+    M b = actT ? r[t] : 0.f;
+    const M dg = ((tw + te) + ts) + tn;
     // the face-form operator has the constants as null space: solve with the
     // right-hand side projected to zero mean (consistent system)
-    b -= WaveSumF(b) / float(nT);
+    b -= WaveSumF(b) / M(nT);
     if (!actT) b = 0.f;
-    float x = 0.f, rr = b, z = tdi * rr, p = z;
-    float rz = WaveSumF(rr * z);
-    const float rz0 = rz;
+    M x = 0.f, rr = b, z = tdi * rr, p = z;
+    M rz = WaveSumF(rr * z);
+    const M rz0 = rz;
     for (int it = 0; it < kTinyIters && rz > 1e-12f * rz0; ++it) {
-      const float pw = __shfl(p, (t + 63) & 63, 64), pe = __shfl(p, (t + 1) & 63, 64);
-      const float ps = __shfl(p, (t + 64 - nxT) & 63, 64);
-      const float pn = __shfl(p, (t + nxT) & 63, 64);
-      float ap = dg * p;
+      const M pw = __shfl(p, (t + 63) & 63, 64), pe = __shfl(p, (t + 1) & 63, 64);
+      const M ps = __shfl(p, (t + 64 - nxT) & 63, 64);
+      const M pn = __shfl(p, (t + nxT) & 63, 64);
+      M ap = dg * p;
       if (iT > 0) ap -= tw * pw;
       if (iT < nxT - 1) ap -= te * pe;
       if (jT > 0) ap -= ts * ps;
       if (jT < nyT - 1) ap -= tn * pn;
       if (!actT) ap = 0.f;
-      const float pap = WaveSumF(p * ap);
+      const M pap = WaveSumF(p * ap);
       if (!(pap > 0.f)) break;
-      const float alpha = rz / pap;
+      const M alpha = rz / pap;
       x += alpha * p;
       rr -= alpha * ap;
       z = tdi * rr;
-      const float rzn = WaveSumF(rr * z);
+      const M rzn = WaveSumF(rr * z);
       p = z + (rzn / rz) * p;
       rz = rzn;
     }
-    x -= WaveSumF(actT ? x : 0.f) / float(nT);
+    x -= WaveSumF(actT ? x : 0.f) / M(nT);
     if (actT) sxa[cs.off[nlev] + t] = x;
   }
   __syncthreads();
@@ -730,29 +741,29 @@ KCoarseCycle(CoarseSet cs, SweepWeights w, const int* done) {
   for (int l = kCcLevels - 1; l >= 0; --l) {
     if (l >= nlev) continue;
     const int nx = cs.nx[l], ny = cs.ny[l], n = nx * ny, nxc = cs.nx[l + 1];
-    const float* r = sr + cs.off[l];
-    const float* e = sxa + cs.off[l + 1];
-    float *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
-    float* yb = Buf(xa, xb, -1, false);
+    const M* r = sr + cs.off[l];
+    const M* e = sxa + cs.off[l + 1];
+    M *xa = sxa + cs.off[l], *xb = sxb + cs.off[l];
+    M* yb = Buf(xa, xb, -1, false);
 #pragma unroll
     for (int m = 0; m < CcSlots(l); ++m) {
       const int c = t + m * kCcThreads;
       if (c >= n) continue;
       const int i = c % nx, j = c / nx, I = i >> 1, J = j >> 1;
       const int Ix = I + ((i & 1) ? 1 : -1), Jy = J + ((j & 1) ? 1 : -1);
-      const float ec = e[I + nxc * J];
-      const float ex = wx[l][m] > 0.f ? e[Ix + nxc * J] : ec;
-      const float ey = wy[l][m] > 0.f ? e[I + nxc * Jy] : ec;
-      const float wd = wx[l][m] * wy[l][m];
-      const float ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
+      const M ec = e[I + nxc * J];
+      const M ex = wx[l][m] > 0.f ? e[Ix + nxc * J] : ec;
+      const M ey = wy[l][m] > 0.f ? e[I + nxc * Jy] : ec;
+      const M wd = wx[l][m] * wy[l][m];
+      const M ed = wd > 0.f ? e[Ix + nxc * Jy] : ec;
       yb[c] = xb[c] + (ec + (3.f * wx[l][m] * (ex - ec) + 3.f * wy[l][m] * (ey - ec) +
                              wd * (ed - ec)) * 0.0625f);
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSweeps; ++k) {
-      float* dst = Buf(xa, xb, k, false);
-      const float* src = Buf(xa, xb, k - 1, false);
+      M* dst = Buf(xa, xb, k, false);
+      const M* src = Buf(xa, xb, k - 1, false);
 #pragma unroll
       for (int m = 0; m < CcSlots(l); ++m) {
         const int c = t + m * kCcThreads;
@@ -773,10 +784,11 @@ inline int Blocks1D(long n) { return int(std::min<long>((n + 255) / 256, 4096));
 
 }  // namespace
 
-MgPrecond::MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0) {
+template <class M>
+MgPrecondT<M>::MgPrecondT(int nx, int ny, M* kx0, M* ky0, M* dinv0) {
   int cx = nx, cy = ny;
   while (true) {
-    MgLevel L;
+    MgLevelT<M> L;
     L.nx = cx;
     L.ny = cy;
     L.tg = MakeTileGrid(cx, cy);
@@ -788,13 +800,13 @@ MgPrecond::MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0) {
       L.ky = ky0;
       L.dinv = dinv0;
     } else {
-      HG_CHECK(hipMalloc(&L.kx, size_t(cx + 1) * cy * 4));
-      HG_CHECK(hipMalloc(&L.ky, size_t(cx) * (cy + 1) * 4));
-      HG_CHECK(hipMalloc(&L.dinv, n * 4));
+      HG_CHECK(hipMalloc(&L.kx, size_t(cx + 1) * cy * sizeof(M)));
+      HG_CHECK(hipMalloc(&L.ky, size_t(cx) * (cy + 1) * sizeof(M)));
+      HG_CHECK(hipMalloc(&L.dinv, n * sizeof(M)));
     }
-    HG_CHECK(hipMalloc(&L.r, n * 4));
-    HG_CHECK(hipMalloc(&L.xa, n * 4));
-    HG_CHECK(hipMalloc(&L.xb, n * 4));
+    HG_CHECK(hipMalloc(&L.r, n * sizeof(M)));
+    HG_CHECK(hipMalloc(&L.xa, n * sizeof(M)));
+    HG_CHECK(hipMalloc(&L.xb, n * sizeof(M)));
     lv_.push_back(L);
     if ((cx & 1) || (cy & 1) || n <= kTinyMax || cx < 4 || cy < 4) break;
     cx /= 2;
@@ -823,9 +835,10 @@ MgPrecond::MgPrecond(int nx, int ny, float* kx0, float* ky0, float* dinv0) {
   }
 }
 
-MgPrecond::~MgPrecond() {
+template <class M>
+MgPrecondT<M>::~MgPrecondT() {
   for (size_t l = 0; l < lv_.size(); ++l) {
-    MgLevel& L = lv_[l];
+    MgLevelT<M>& L = lv_[l];
     for (void* p : {(void*)L.r, (void*)L.xa, (void*)L.xb})
       if (p) (void)hipFree(p);
     if (l > 0)  // level 0's coefficients are the caller's
@@ -834,45 +847,47 @@ MgPrecond::~MgPrecond() {
   }
 }
 
-void MgPrecond::Build(hipStream_t st) {
+template <class M>
+void MgPrecondT<M>::Build(hipStream_t st) {
   // grid kernels down to the first level of at most kChainCells cells, then
   // the remaining (small) levels in one workgroup
   constexpr long kChainCells = 4096;
   size_t l = 1;
   for (; l < lv_.size(); ++l) {
-    MgLevel& F = lv_[l - 1];
-    MgLevel& C = lv_[l];
+    MgLevelT<M>& F = lv_[l - 1];
+    MgLevelT<M>& C = lv_[l];
     if (long(C.nx) * C.ny <= kChainCells && lv_.size() - (l - 1) <= size_t(kMaxChain)) break;
-    KCoarsen<<<Blocks1D(long(C.nx) * C.ny), 256, 0, st>>>(F.nx, F.kx, F.ky, C.nx, C.ny, C.kx,
+    KCoarsen<M><<<Blocks1D(long(C.nx) * C.ny), 256, 0, st>>>(F.nx, F.kx, F.ky, C.nx, C.ny, C.kx,
                                                           C.ky, C.dinv);
   }
   if (l < lv_.size()) {
-    ChainSet cs{};
+    ChainSet<M> cs{};
     cs.count = int(lv_.size() - (l - 1));
     for (int k = 0; k < cs.count; ++k) {
-      MgLevel& L = lv_[l - 1 + k];
+      MgLevelT<M>& L = lv_[l - 1 + k];
       cs.nx[k] = L.nx;
       cs.ny[k] = L.ny;
       cs.kx[k] = L.kx;
       cs.ky[k] = L.ky;
       cs.dinv[k] = L.dinv;
     }
-    KCoarsenChain<<<1, 1024, 0, st>>>(cs);
+    KCoarsenChain<M><<<1, 1024, 0, st>>>(cs);
   }
 }
 
-void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, const int* done,
-                      const FcgFusion* f) {
+template <class M>
+void MgPrecondT<M>::Cycle(int l, const double* r64, float* z, M* out, hipStream_t st,
+                          const int* done, const FcgFusion* f) {
   const FcgFusion fz = f ? *f : FcgFusion{};
   const int fused = (f && l == 0) ? 1 : 0;
-  MgLevel& L = lv_[l];
+  MgLevelT<M>& L = lv_[l];
   if (l > 0 && l == first_in_kernel_ && use_coarse_kernel) {
-    CoarseSet cs{};
+    CoarseSetT<M> cs{};
     cs.count = int(lv_.size()) - l;
     cs.bilinear = bilinear ? 1 : 0;
     int off = 0;
     for (int k = 0; k < cs.count; ++k) {
-      const MgLevel& C = lv_[l + k];
+      const MgLevelT<M>& C = lv_[l + k];
       cs.off[k] = off;
       off += C.nx * C.ny;
       cs.nx[k] = C.nx;
@@ -885,7 +900,7 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
       cs.xb[k] = C.xb;
     }
     cs.out = out;
-    KCoarseCycle<<<1, 1024, 0, st>>>(cs, sweeps, done);
+    KCoarseCycle<M><<<1, 1024, 0, st>>>(cs, sweeps, done);
     return;
   }
   const bool coarsest = (l == int(lv_.size()) - 1);
@@ -894,110 +909,122 @@ void MgPrecond::Cycle(int l, const double* r64, float* out, hipStream_t st, cons
     // r of the coarsest level is always FP32 (lv_.size() > 1 is guaranteed for
     // fine grids; a single-level hierarchy relaxes the FP64 residual below)
     if (l > 0) {
-      KCoarsest<<<1, kCoarsestMax, 0, st>>>(L.nx, L.ny, L.kx, L.ky, L.dinv, L.r, omega,
-                                             coarse_sweeps, out, done);
+      KCoarsest<M><<<1, kCoarsestMax, 0, st>>>(L.nx, L.ny, L.kx, L.ky, L.dinv, L.r, omega,
+                                                coarse_sweeps, out, done);
       return;
     }
   }
   if (coarsest) {
+    // the level's result goes to z (FP32) on a single-level hierarchy
+    M* target = out;
+    if (l == 0) {
+      if constexpr (std::is_same<M, float>::value)
+        target = z;
+      else
+        throw std::runtime_error("MgPrecond64: a single-level hierarchy is not supported");
+    }
     if (l == 0)
-      KSmoothFirst<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, r64, L.dinv, omega,
-                                                            L.xa, done);
+      KSmoothFirst<double, M><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, r64, L.dinv,
+                                                               omega, L.xa, done);
     else
-      KSmoothFirst<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.r, L.dinv, omega,
-                                                           L.xa, done);
+      KSmoothFirst<M, M><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.r, L.dinv, omega,
+                                                          L.xa, done);
     // no coarser level (odd or tiny grids): seven more Jacobi sweeps ending
-    // in `out`, ping-ponging between xa and xb
-    float* cur = L.xa;
+    // in the target, ping-ponging between xa and xb
+    M* cur = L.xa;
     const int nsw = 7;
     for (int s = 0; s < nsw; ++s) {
-      float* dst = (s == nsw - 1) ? out : (cur == L.xa ? L.xb : L.xa);
+      M* dst = (s == nsw - 1) ? target : (cur == L.xa ? L.xb : L.xa);
       if (dst == cur) dst = (cur == L.xa) ? L.xb : L.xa;
       if (l == 0)
-        KJacobi<double><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, r64,
-                                                         cur, omega, dst, done);
+        KJacobi<double, M><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv,
+                                                            r64, cur, omega, dst, done);
       else
-        KJacobi<float><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
-                                                        cur, omega, dst, done);
+        KJacobi<M, M><<<blocks, kTileThreads, 0, st>>>(L.nx, L.ny, L.tg, L.kx, L.ky, L.dinv, L.r,
+                                                       cur, omega, dst, done);
       cur = dst;
     }
-    if (cur != out)
-      HG_CHECK(hipMemcpyAsync(out, cur, size_t(L.nx) * L.ny * 4, hipMemcpyDeviceToDevice, st));
+    if (cur != target)
+      HG_CHECK(hipMemcpyAsync(target, cur, size_t(L.nx) * L.ny * sizeof(M),
+                              hipMemcpyDeviceToDevice, st));
     return;
   }
-  MgLevel& C = lv_[l + 1];
+  MgLevelT<M>& C = lv_[l + 1];
   // level 0 (FP64 residual) on 64 x kTallY tiles, the FP32 levels on 64 x kMidY;
   // with the FCG update folded into level 0 its stop test moves to level 1
   const bool up = fz.up_on != 0;
   if (l == 0 && up)
-    KDown<double, kTallY, kFineSweeps, true><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+    KDown<M, double, kTallY, kFineSweeps, true><<<L.tall.blocks(), kTileThreads, 0, st>>>(
         L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, fine_sweeps, L.xb, C.nx, C.ny,
         C.r, done, fz, 0);
   else if (l == 0)
-    KDown<double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+    KDown<M, double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
         L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r64, fine_sweeps, L.xb, C.nx, C.ny,
         C.r, done, fz, fused);
   else
-    KDown<float, kMidY, kSweeps><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
+    KDown<M, M, kMidY, kSweeps><<<L.mid.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, L.mid, L.kx, L.ky, L.dinv,
                                                        L.r, sweeps, L.xb, C.nx,
                                                        C.ny, C.r, done, fz,
                                                        (l == 1 && up) ? 1 : 0);
   // the coarser level's correction goes to its xa: its xb holds the x2 that
   // its up-leg reads on tile halos while other workgroups write their output
-  Cycle(l + 1, nullptr, C.xa, st, done, (l == 0 && up) ? f : nullptr);
+  Cycle(l + 1, nullptr, nullptr, C.xa, st, done, (l == 0 && up) ? f : nullptr);
   // the up-leg reads x2 (L.xb) and the coarse correction on halos that other
   // workgroups' tiles cover: its output must be a third buffer
-  if (out == L.xb || out == C.xa)
+  // (level 0 writes z, a separate FP32 buffer)
+  if (l > 0 && (out == L.xb || out == C.xa))
     throw std::logic_error("MgPrecond: up-leg output aliases its input");
-#define HG_UP(RT, B, TY, SW, WT, TG, R)                                                 \
-  KUp<RT, B, TY, SW><<<TG.blocks(), kTileThreads, 0, st>>>(L.nx, L.ny, TG, L.kx, L.ky,    \
-                                                          L.dinv, R, L.xb, C.xa, C.kx,   \
-                                                          C.ky, C.nx, C.ny, WT, out,     \
-                                                          done, fz, fused)
+#define HG_UP(OT, RT, B, TY, SW, WT, TG, R, OUT)                                         \
+  KUp<M, OT, RT, B, TY, SW><<<TG.blocks(), kTileThreads, 0, st>>>(                       \
+      L.nx, L.ny, TG, L.kx, L.ky, L.dinv, R, L.xb, C.xa, C.kx, C.ky, C.nx, C.ny, WT, OUT, \
+      done, fz, fused)
   if (l == 0) {
     // after the folded update the level's residual is the one it wrote
     const double* r0 = up ? fz.up_r : r64;
     if (bilinear)
-      HG_UP(double, true, kTallY, kFineSweeps, fine_sweeps, L.tall, r0);
+      HG_UP(float, double, true, kTallY, kFineSweeps, fine_sweeps, L.tall, r0, z);
     else
-      HG_UP(double, false, kTallY, kFineSweeps, fine_sweeps, L.tall, r0);
+      HG_UP(float, double, false, kTallY, kFineSweeps, fine_sweeps, L.tall, r0, z);
   } else {
     if (bilinear)
-      HG_UP(float, true, kMidY, kSweeps, sweeps, L.mid, L.r);
+      HG_UP(M, M, true, kMidY, kSweeps, sweeps, L.mid, L.r, out);
     else
-      HG_UP(float, false, kMidY, kSweeps, sweeps, L.mid, L.r);
+      HG_UP(M, M, false, kMidY, kSweeps, sweeps, L.mid, L.r, out);
   }
 #undef HG_UP
 }
 
-void MgPrecond::Apply(const double* r, float* z, hipStream_t st, const int* d_done) {
-  Cycle(0, r, z, st, d_done);
+template <class M>
+void MgPrecondT<M>::Apply(const double* r, float* z, hipStream_t st, const int* d_done) {
+  Cycle(0, r, z, nullptr, st, d_done);
 }
 
-bool MgPrecond::ApplyFcg(const double* r, float* z, hipStream_t st, const int* d_done,
-                         const FcgFusion& f, int* ndir) {
+template <class M>
+bool MgPrecondT<M>::ApplyFcg(const double* r, float* z, hipStream_t st, const int* d_done,
+                             const FcgFusion& f, int* ndir) {
   if (lv_.size() < 2) return false;  // no KDown / KUp pair on the finest level
-  Cycle(0, r, z, st, d_done, &f);
+  Cycle(0, r, z, nullptr, st, d_done, &f);
   *ndir = lv_[0].tall.blocks();
   return true;
 }
 
-double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStream_t st) {
+template <class M>
+double MgPrecondT<M>::TimeKernel(const char* name, int reps, const double* r, hipStream_t st) {
   if (lv_.size() < 2) throw std::runtime_error("TimeKernel: single-level hierarchy");
   const std::string k(name);
-  MgLevel& L = lv_[0];
-  MgLevel& C = lv_[1];
+  MgLevelT<M>& L = lv_[0];
+  MgLevelT<M>& C = lv_[1];
   auto launch = [&]() {
     if (k == "KDown<double>") {
-      KDown<double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+      KDown<M, double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, fine_sweeps, L.xb, C.nx, C.ny,
           C.r, nullptr, FcgFusion{}, 0);
     } else if (k == "KUp<double, true>") {
-      KUp<double, true, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
+      KUp<M, M, double, true, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,
           fine_sweeps, L.xa, nullptr, FcgFusion{}, 0);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
-      Cycle(first_in_kernel_, nullptr, lv_[first_in_kernel_].xa, st, nullptr);
+      Cycle(first_in_kernel_, nullptr, nullptr, lv_[first_in_kernel_].xa, st, nullptr);
     } else {
       throw std::runtime_error("TimeKernel: unknown kernel " + k);
     }
@@ -1016,5 +1043,8 @@ double MgPrecond::TimeKernel(const char* name, int reps, const double* r, hipStr
   (void)hipEventDestroy(e1);
   return double(ms) / reps;
 }
+
+template class MgPrecondT<float>;
+template class MgPrecondT<double>;
 
 }  // namespace hg

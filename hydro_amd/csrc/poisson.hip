@@ -44,12 +44,14 @@ struct hg_poisson {
   std::unique_ptr<hg::CgSolver> cg;
   std::unique_ptr<hg::LearnedNet> net;
   std::unique_ptr<hg::MgPrecond> mg;
+  std::unique_ptr<hg::MgPrecond64> mg64;  // built on the first FP64 fcg_mg solve
   double *b = nullptr, *x = nullptr;
   ~hg_poisson() {
     if (st) (void)hipStreamSynchronize(st);
     cg.reset();
     net.reset();
     mg.reset();
+    mg64.reset();
     if (b) (void)hipFree(b);
     if (x) (void)hipFree(x);
     if (st) (void)hipStreamDestroy(st);
@@ -222,7 +224,15 @@ hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
     if (opts->solver == HG_SOLVER_FCG_LEARNED)
       pc = (ps->net && ps->net->Trained()) ? static_cast<hg::Precond*>(ps->net.get())
                                            : static_cast<hg::Precond*>(ps->mg.get());
-    else if (opts->solver == HG_SOLVER_FCG_MG)
+    else if (opts->solver == HG_SOLVER_FCG_MG && opts->precision == HG_PREC_FP64) {
+      // the same V-cycle with FP64 levels (what FP32 buys, measured)
+      if (!ps->mg64) {
+        ps->mg64.reset(new hg::MgPrecond64(ps->nx, ps->ny, ps->cg->kx64(), ps->cg->ky64(),
+                                           ps->cg->dinv64()));
+        ps->mg64->Build(ps->st);
+      }
+      pc = ps->mg64.get();
+    } else if (opts->solver == HG_SOLVER_FCG_MG)
       pc = ps->mg.get();
     hg::CgResult r = ps->cg->Solve(ps->b, ps->x, opts->tolerance, opts->max_iters,
                                    opts->precision, true, pc);

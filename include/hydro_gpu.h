@@ -391,8 +391,11 @@ hg_status hg_poisson_create(int32_t nx, int32_t ny, const double* kx, const doub
 void hg_poisson_destroy(hg_poisson* ps);
 /* Solves A p = rhs (rhs projected to zero mean) from p = 0 to the relative
  * residual opts->tolerance (true FP64 residual).  opts->solver is
- * HG_SOLVER_CG (Jacobi-PCG) or HG_SOLVER_FCG_LEARNED; opts->precision
- * HG_PREC_MIXED runs FP32 inner solves under FP64 iterative refinement. */
+ * HG_SOLVER_CG (Jacobi-PCG), HG_SOLVER_FCG_LEARNED or HG_SOLVER_FCG_MG.
+ * opts->precision: for cg, HG_PREC_MIXED runs FP32 inner solves under FP64
+ * iterative refinement; for fcg_mg, HG_PREC_MIXED preconditions with the
+ * FP32 V-cycle (the step's) and HG_PREC_FP64 with the same V-cycle on FP64
+ * levels (FCG itself is FP64 either way). */
 hg_status hg_poisson_solve(hg_poisson* ps, const double* rhs, double* p,
                            const hg_solve_opts* opts, hg_solve_stats* stats);
 

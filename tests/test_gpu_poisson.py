@@ -30,7 +30,8 @@ def problem():
     return n, kx, ky, rhs
 
 
-@pytest.mark.parametrize("solver,precision", [("cg", 0), ("cg", 1), ("fcg_learned", 1)])
+@pytest.mark.parametrize("solver,precision", [("cg", 0), ("cg", 1), ("fcg_learned", 1),
+                                              ("fcg_mg", 1), ("fcg_mg", 0)])
 def test_poisson_true_residual(gpu, problem, solver, precision):
     n, kx, ky, rhs = problem
     print(f"case: poisson {n}x{n} {solver} precision {precision}", flush=True)
@@ -61,3 +62,17 @@ def test_multigrid_cuts_iterations(gpu, problem):
     print(f"jacobi-pcg {s_cg.iterations} it {s_cg.ms:.2f} ms; fcg+mg {s_mg.iterations} it "
           f"{s_mg.ms:.2f} ms", flush=True)
     assert s_mg.iterations * 10 < s_cg.iterations
+
+
+def test_fp64_vcycle_same_iterations(gpu, problem):
+    """fcg_mg with the V-cycle on FP64 levels (precision 0) against the FP32
+    one (precision 1): the same kernels, so FP32 costs the preconditioner no
+    strength (iteration counts within one), only halves its bytes."""
+    n, kx, ky, rhs = problem
+    ps = Poisson(n, n, kx, ky)
+    _, s32 = ps.solve(rhs, solver="fcg_mg", tol=1e-8, precision=1)
+    _, s64 = ps.solve(rhs, solver="fcg_mg", tol=1e-8, precision=0)
+    print(f"fcg_mg FP32 V-cycle {s32.iterations} it {s32.ms:.3f} ms; FP64 V-cycle "
+          f"{s64.iterations} it {s64.ms:.3f} ms", flush=True)
+    assert s32.converged and s64.converged
+    assert abs(s32.iterations - s64.iterations) <= 1
