@@ -146,10 +146,31 @@ __device__ __forceinline__ int SideOf(const G& g, long f) {
 }
 
 // ---- interpolation (solver.hpp:392-470)
-__global__ void KInterpInner(G g, const double* __restrict__ u, double* __restrict__ res) {
+// Inner faces: the mean of the two cells.  Boundary faces: with bc_here,
+// the value and zero-derivative conditions (which read only the face's own
+// cell, as in KInterpBoundary), else 0 for KInterpBoundary to fill.
+__global__ void KInterpInner(G g, const double* __restrict__ u, double* __restrict__ res, Bc bc,
+                             int bc_here) {
   const long f = blockIdx.x * long(blockDim.x) + threadIdx.x;
   if (f >= g.nf) return;
-  res[f] = Inner(g, f) ? u[FaceCells(g, f).m] * (1. - 0.5) + u[FaceCells(g, f).p] * 0.5 : 0.;
+  const FC fc = FaceCells(g, f);
+  double v = 0.;
+  if (fc.m >= 0 && fc.p >= 0) {
+    v = u[fc.m] * (1. - 0.5) + u[fc.p] * 0.5;
+  } else if (bc_here) {
+    const int side = SideOf(g, f);
+    const int kind = bc.kind[side];
+    const int id = fc.m >= 0 ? 0 : 1;
+    const long c = id == 0 ? fc.m : fc.p;
+    const double factor = (id == 0 ? 1. : -1.);
+    if (kind == BC_VALUE) {
+      v = bc.val[side];
+    } else if (kind == BC_DERIV) {
+      const double alpha = ToDist(g, f, id) * factor;
+      v = u[c] + bc.val[side] * alpha;
+    }
+  }
+  res[f] = v;
 }
 // boundary faces of one direction (x: sides 0,1; y: 2,3; z: 4,5); the
 // directions run in face order, so an extrapolated face sees the values of
@@ -471,6 +492,80 @@ __global__ void KJacobiCheck(const unsigned long long* diff, int s, double tol, 
     state[0] = 1;
     state[1] = iters;
     state[2] = (s + 1) & 1;  // sweep s wrote buffer (s + 1) & 1
+  }
+}
+
+// The three velocity components' Jacobi solves (one matrix, three right-hand
+// sides, each from zero with its own stop rule) in one kernel per sweep:
+// the seven row entries of a cell are loaded once for the components still
+// running, and each component's update is the single-component sweep's,
+// operation for operation.  Component d keeps its max |corr| in
+// diff[d * stride + s] and its state (done, iters, buffer) at state[4 d].
+struct Jac3 {
+  const double* k[3];
+  const double* x[3];  // sweep s's input iterate, per component
+  double* xn[3];       // its output
+};
+__global__ void KJacobiSweep3(G g, Rows A, Jac3 J, double omega, unsigned long long* diff,
+                              int stride, int s, int* state, double tol, int limit) {
+  bool act[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    act[d] = !state[4 * d];  // decided after sweep s - 2 or earlier: off
+    int iters = 0;
+    if (act[d] && s > 0 && JacobiStop(diff + long(d) * stride, s - 1, tol, limit, iters)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state[4 * d] = 1;
+        state[4 * d + 1] = iters;
+        state[4 * d + 2] = s & 1;  // sweep s - 1 wrote buffer s & 1
+      }
+      act[d] = false;
+    }
+  }
+  if (!act[0] && !act[1] && !act[2]) return;  // uniform over the grid
+  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  double m[3] = {0., 0., 0.};
+  if (c < g.nc) {
+    bool has[7];
+    double t[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      has[q] = q != 3 && Has(g, c, q);
+      t[q] = (q == 3 || has[q]) ? A.t[q][c] : 0.;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!act[d]) continue;
+      const double* x = J.x[d];
+      double sum = 0.;
+#pragma unroll
+      for (int q = 0; q < 7; ++q)
+        if (has[q]) sum += t[q] * x[c + Off(g, q)];
+      const double value = -(J.k[d][c] + sum) / t[3];
+      const double corr = value - x[c];
+      m[d] = fabs(corr);
+      J.xn[d][c] = x[c] + corr * omega;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!act[d]) continue;
+    double v = m[d];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.)
+      atomicMax(&diff[long(d) * stride + s], (unsigned long long)__double_as_longlong(v));
+  }
+}
+// the stop rule after the last sweep s of a batch, for each component
+__global__ void KJacobiCheck3(const unsigned long long* diff, int stride, int s, double tol,
+                              int limit, int* state) {
+  const int d = threadIdx.x;
+  if (d >= 3 || state[4 * d]) return;
+  int iters = 0;
+  if (JacobiStop(diff + long(d) * stride, s, tol, limit, iters)) {
+    state[4 * d] = 1;
+    state[4 * d + 1] = iters;
+    state[4 * d + 2] = (s + 1) & 1;  // sweep s wrote buffer (s + 1) & 1
   }
 }
 
@@ -849,6 +944,7 @@ struct hg3_ctx {
   Rows A{}, P{};
   double* kk[3];
   double *corr, *corr2, *a_p, *ffd, *fu[3], *fstar, *fcoef, *pc, *ffpc;
+  double* jbuf[3][2];  // the velocity components' Jacobi iterates (jacobi velocity solver)
   double *part, *scal_d;
   unsigned long long* jdiff;
   int* jstate;
@@ -1041,7 +1137,14 @@ void BuildGeometry(hg3_ctx* c) {
 // Interpolate (the three boundary directions in face order)
 void Interp(hg3_ctx* c, const double* u, double* res, const Bc& bc) {
   const G& g = c->g;
-  KInterpInner<<<Grid(g.nf), kB, 0, c->st>>>(g, u, res);
+  // extrapolated boundary faces read the values of the cell's other faces
+  // (earlier directions' boundary faces included): they keep the
+  // direction-by-direction kernels; the other conditions are written with
+  // the inner faces in one launch
+  bool extrap = false;
+  for (int s = 0; s < 6; ++s) extrap = extrap || bc.kind[s] == BC_EXTRAP;
+  KInterpInner<<<Grid(g.nf), kB, 0, c->st>>>(g, u, res, bc, extrap ? 0 : 1);
+  if (!extrap) return;
   const long per[3] = {2L * g.ny * g.nz, 2L * g.nx * g.nz, 2L * g.nx * g.ny};
   for (int d = 0; d < 3; ++d) KInterpBoundary<<<Grid(per[d]), kB, 0, c->st>>>(g, d, u, res, bc);
 }
@@ -1116,6 +1219,44 @@ int SolveJacobi(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   }
   Copy(c, x, buf[c->state_h[2]], c->nc);
   return c->state_h[1];
+}
+
+// The velocity components' jacobi solves together (KJacobiSweep3): each
+// component's iterations and result are those of its own SolveJacobi; out
+// u_ic[d] = u_ip[d] + x_d, iters[d] its sweep count.
+void SolveVelocityJacobi3(hg3_ctx* c, const Rows& R, int iters[3]) {
+  const hg_config& cfg = c->cfg;
+  const int limit = cfg.lu_relaxed_num_iters_limit;
+  const double tol = cfg.lu_relaxed_tolerance, omega = cfg.lu_relaxed_relaxation_factor;
+  const int stride = limit + 2;
+  H3_CHECK(hipMemsetAsync(c->jdiff, 0, size_t(3) * stride * 8, c->st));
+  H3_CHECK(hipMemsetAsync(c->jstate, 0, 12 * sizeof(int), c->st));
+  for (int d = 0; d < 3; ++d)
+    H3_CHECK(hipMemsetAsync(c->jbuf[d][0], 0, size_t(c->nc) * 8, c->st));
+  int s = 0;
+  while (true) {
+    const int batch = std::min(32, limit + 1 - s);
+    for (int b = 0; b < batch; ++b, ++s) {
+      Jac3 J;
+      for (int d = 0; d < 3; ++d) {
+        J.k[d] = c->kk[d];
+        J.x[d] = c->jbuf[d][s & 1];
+        J.xn[d] = c->jbuf[d][(s + 1) & 1];
+      }
+      KJacobiSweep3<<<Grid(c->nc), kB, 0, c->st>>>(c->g, R, J, omega, c->jdiff, stride, s,
+                                                    c->jstate, tol, limit);
+    }
+    KJacobiCheck3<<<1, 64, 0, c->st>>>(c->jdiff, stride, s - 1, tol, limit, c->jstate);
+    H3_CHECK(hipMemcpyAsync(c->state_h, c->jstate, 12 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    c->Sync();
+    if (c->state_h[0] && c->state_h[4] && c->state_h[8]) break;
+    if (s > limit) throw Err3(HG_ERR_HIP, "jacobi: stop rule not reached");
+  }
+  for (int d = 0; d < 3; ++d) {
+    iters[d] = c->state_h[4 * d + 1];
+    KAddTo<<<Grid(c->nc), kB, 0, c->st>>>(c->nc, c->u_ip[d], c->jbuf[d][c->state_h[4 * d + 2]],
+                                          c->u_ic[d]);
+  }
 }
 
 int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
@@ -1255,9 +1396,16 @@ void Iterate(hg3_ctx* c, hg_iter_stats* is) {
               cfg.velocity_relaxation_factor};
     KMomentumRows<<<Grid(nc), kB, 0, c->st>>>(g, c->fe, m, d == 0 ? 1 : 0, c->A, c->kk[d]);
   }
-  for (int d = 0; d < 3; ++d) {
-    is->velocity_solve_iters[d < 2 ? d : 1] = RunSolver(c, cfg.linear_solver_velocity, c->A, c->kk[d], c->pc);
-    KAddTo<<<Grid(nc), kB, 0, c->st>>>(nc, c->u_ip[d], c->pc, c->u_ic[d]);
+  if (cfg.linear_solver_velocity == HG_SOLVER_JACOBI) {
+    int it3[3];
+    SolveVelocityJacobi3(c, c->A, it3);
+    for (int d = 0; d < 3; ++d) is->velocity_solve_iters[d < 2 ? d : 1] = it3[d];
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      is->velocity_solve_iters[d < 2 ? d : 1] =
+          RunSolver(c, cfg.linear_solver_velocity, c->A, c->kk[d], c->pc);
+      KAddTo<<<Grid(nc), kB, 0, c->st>>>(nc, c->u_ip[d], c->pc, c->u_ic[d]);
+    }
   }
   c->cd_iter += 1;
   // diagonal and Rhie-Chow
@@ -1468,10 +1616,14 @@ hg_status hg3_create(const hg3_config* cfg3, hg3_ctx** out) {
     c->cx = c->Alloc(nc);
     c->cs = (CgScal*)c->Alloc(long(sizeof(CgScal) / 8 + 1));
     c->jlimit = std::max(1, cfg.lu_relaxed_num_iters_limit);
-    c->jdiff = (unsigned long long*)c->Alloc(c->jlimit + 2);
-    c->jstate = (int*)c->Alloc(2);
+    // three components' max |corr| per sweep and states (done, iters, buffer)
+    c->jdiff = (unsigned long long*)c->Alloc(3L * (c->jlimit + 2));
+    c->jstate = (int*)c->Alloc(6);
+    if (cfg.linear_solver_velocity == HG_SOLVER_JACOBI)
+      for (int d = 0; d < 3; ++d)
+        for (int b = 0; b < 2; ++b) c->jbuf[d][b] = c->Alloc(nc);
     c->flag = (int*)c->Alloc(1);
-    H3_CHECK(hipHostMalloc(&c->state_h, 4 * sizeof(int)));
+    H3_CHECK(hipHostMalloc(&c->state_h, 12 * sizeof(int)));  // three components' jacobi states
     H3_CHECK(hipHostMalloc(&c->scal_h, 4 * sizeof(double)));
     H3_CHECK(hipHostMalloc(&c->cs_h, sizeof(CgScal)));
     if (cfg.linear_solver_pressure == HG_SOLVER_FCG_MG)
