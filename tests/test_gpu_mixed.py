@@ -60,7 +60,7 @@ def test_mixed_and_fp64_vs_oracle_rt128x256(gpu):
     report("GPU mixed vs oracle", emx, compare.volume_drift(mv0, mv1))
     report("oracle", {}, compare.volume_drift(rv0, rv1))
     # FP64 GPU: Jacobi-PCG to the same tolerance with reductions in another
-    # order, within the north star's 1e-10 (measured: u 7.4e-12, v 8.1e-12,
+    # order, within the path's 1e-10 relative L2 (measured: u 7.4e-12, v 8.1e-12,
     # p 6.5e-15, alpha 4.2e-17)
     for k, v in e64.items():
         assert v <= 1e-10, (k, v)

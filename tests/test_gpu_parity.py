@@ -4,7 +4,7 @@ The oracle (oracle/hydro_oracle.cpp) matches the reference module bit for
 bit (tests/test_oracle_reference.py); here the HIP kernels must match the
 oracle.  On power-of-two meshes every geometric quantity is exact, so the
 exact-order solvers (lu, gauss_seidel, jacobi, lu_rb) give identical bits;
-the bar written in the north star is 1e-10 relative L2, checked as well.
+the floating-point bar for the path, 1e-10 relative L2, is checked as well.
 """
 import os
 
