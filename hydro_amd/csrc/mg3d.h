@@ -11,7 +11,8 @@
 //
 // Levels halve all three dimensions while they are even and the level has
 // more than 64 cells; the coarsest (at most kCoarsestMax cells) is solved
-// approximately by Jacobi-PCG in one workgroup.  Coarse face coefficient =
+// approximately by Jacobi-PCG in one workgroup, and the levels below the
+// finest of at most 4096 cells run with it in that one workgroup.  Coarse face coefficient =
 // half the sum of the four fine faces it covers (the rediscretised operator:
 // in 3D k_f = area / (distance a_f) scales with h), restriction = sum of the
 // eight children's residuals, prolongation = cell-centred trilinear weighted
@@ -63,6 +64,7 @@ class Mg3Fcg {
     float *kx, *ky, *kz, *dinv, *r, *xa, *xb;
   };
   std::vector<Level> lv_;
+  int first_fused_ = -1;  // first level (>= 1) run with the coarser ones in one workgroup
   std::vector<void*> allocs_;
   float* sgn_ = nullptr;  // sign of the finest rows' diagonal
   float* z32_ = nullptr;

@@ -227,15 +227,8 @@ KRestrict(Lv L, const float* __restrict__ r, const float* __restrict__ x, int nx
 // coarse face coefficients: trilinear (27, 9, 3, 1) / 64 where the couplings
 // are within a factor 4, collapsing onto the parent across walls (k = 0)
 // and weak couplings.
-__global__ void __launch_bounds__(kB)
-KProlong(Lv C, const float* __restrict__ e, int nx, int ny, int nz, float* __restrict__ x,
-         const int* done) {
-  if (Stopped(done)) return;
-  const long n = long(nx) * ny * nz;
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
-  if (c >= n) return;
-  int i, j, k;
-  Ijk(c, nx, ny, i, j, k);
+// (P e) at fine cell (i, j, k): see KProlong
+__device__ __forceinline__ float Prolong(const Lv& C, const float* e, int i, int j, int k) {
   const int I = i >> 1, J = j >> 1, K = k >> 1;
   const long P = I + long(C.nx) * (J + long(C.ny) * K);
   const Six k6 = Faces(C, P, I, J, K);
@@ -259,7 +252,18 @@ KProlong(Lv C, const float* __restrict__ e, int nx, int ny, int nz, float* __res
   const float d2 = (wx * wy * D(ux && uy, ox + oy) + wx * wz * D(ux && uz, ox + oz)) +
                    wy * wz * D(uy && uz, oy + oz);
   const float d3 = wx * wy * wz * D(ux && uy && uz, ox + oy + oz);
-  x[c] += ec + ((9.f * d1 + 3.f * d2) + d3) * (1.f / 64.f);
+  return ec + ((9.f * d1 + 3.f * d2) + d3) * (1.f / 64.f);
+}
+__global__ void __launch_bounds__(kB)
+KProlong(Lv C, const float* __restrict__ e, int nx, int ny, int nz, float* __restrict__ x,
+         const int* done) {
+  if (Stopped(done)) return;
+  const long n = long(nx) * ny * nz;
+  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (c >= n) return;
+  int i, j, k;
+  Ijk(c, nx, ny, i, j, k);
+  x[c] += Prolong(C, e, i, j, k);
 }
 
 // Coarsest level (n <= kCoarsestMax cells): at most kCoarseIters Jacobi-PCG
@@ -276,11 +280,10 @@ __device__ __forceinline__ float BlockSumF(float v, float* red) {
   return t;
 }
 
-__global__ void __launch_bounds__(kCoarseT)
-KCoarsest(Lv L, const float* __restrict__ r, float* __restrict__ out, const int* done) {
-  if (Stopped(done)) return;
-  __shared__ float sp[Mg3Fcg::kCoarsestMax];
-  __shared__ float red[kCoarseT / 64];
+// The coarsest-level solve by the whole workgroup: r and out may be global or
+// LDS; sp (kCoarsestMax floats) and red are the workgroup's LDS scratch.
+__device__ __forceinline__ void CoarsestPcg(const Lv& L, const float* r, float* out, float* sp,
+                                            float* red) {
   const int n = L.nx * L.ny * L.nz;
   const int t = threadIdx.x;
   Six k6[kCoarseSlots];
@@ -353,6 +356,116 @@ KCoarsest(Lv L, const float* __restrict__ r, float* __restrict__ out, const int*
 #pragma unroll
   for (int m = 0; m < kCoarseSlots; ++m)
     if (t + m * kCoarseT < n) out[t + m * kCoarseT] = x[m] - xm;
+}
+
+__global__ void __launch_bounds__(kCoarseT)
+KCoarsest(Lv L, const float* __restrict__ r, float* __restrict__ out, const int* done) {
+  if (Stopped(done)) return;
+  __shared__ float sp[Mg3Fcg::kCoarsestMax];
+  __shared__ float red[kCoarseT / 64];
+  CoarsestPcg(L, r, out, sp, red);
+}
+
+// The V-cycle's levels of at most kFusedMax cells, down to the coarsest, in
+// one workgroup: one launch instead of eight per level plus the coarsest
+// solve, the levels being too small to fill the GPU or hide a launch.  The
+// same sweeps, restriction and prolongation as the grid kernels (x1 = w0
+// D^-1 r stored instead of formed on the fly: the same values), on
+// residuals and iterates in LDS; the faces are read from global memory,
+// where these small levels stay cached.  The coarsest level gets
+// CoarsestPcg.  Level 0 of the set reads its residual from fs.r and writes
+// its correction to fs.out.
+constexpr int kFusedMax = 4096;
+constexpr int kFusedCells = 4688;  // a chain from 4096 cells by eighths: 4096 + 512 + 64 + 8 + 1
+constexpr int kMaxFused = 8;
+struct Fused3 {
+  int count;  // levels; the last is the coarsest
+  int nx[kMaxFused], ny[kMaxFused], nz[kMaxFused], off[kMaxFused];
+  const float *kx[kMaxFused], *ky[kMaxFused], *kz[kMaxFused], *dinv[kMaxFused];
+  const float* r;
+  float* out;
+};
+
+__global__ void __launch_bounds__(kCoarseT)
+KFused(Fused3 fs, const int* done) {
+  if (Stopped(done)) return;
+  __shared__ float sr[kFusedCells], sxa[kFusedCells], sxb[kFusedCells];
+  __shared__ float sp[Mg3Fcg::kCoarsestMax];
+  __shared__ float red[kCoarseT / 64];
+  const int t = threadIdx.x;
+  auto Lev = [&](int l) {
+    return Lv{fs.nx[l], fs.ny[l], fs.nz[l], fs.kx[l], fs.ky[l], fs.kz[l], fs.dinv[l]};
+  };
+  auto Cells = [&](int l) { return fs.nx[l] * fs.ny[l] * fs.nz[l]; };
+  // out = x + w D^-1 (r - A x) on level l (LDS vectors)
+  auto Sweep = [&](const Lv& L, int n, const float* r, const float* x, float w, float* o) {
+    for (int c = t; c < n; c += kCoarseT) {
+      int i, j, k;
+      Ijk(c, L.nx, L.ny, i, j, k);
+      const Six k6 = Faces(L, c, i, j, k);
+      o[c] = x[c] + w * L.dinv[c] * (r[c] - ApplyAt(L, k6, c, i, j, k, [&](long q) { return x[q]; }));
+    }
+    __syncthreads();
+  };
+  const int last = fs.count - 1;
+  {
+    const int n0 = Cells(0);
+    for (int c = t; c < n0; c += kCoarseT) sr[c] = fs.r[c];
+    __syncthreads();
+  }
+  // down: x1 -> xb, x2 -> xa, x3 -> xb, residual -> xa, restriction
+  for (int l = 0; l < last; ++l) {
+    const Lv L = Lev(l);
+    const int n = Cells(l);
+    const float* r = sr + fs.off[l];
+    float *xa = sxa + fs.off[l], *xb = sxb + fs.off[l];
+    for (int c = t; c < n; c += kCoarseT) xb[c] = kW0 * L.dinv[c] * r[c];
+    __syncthreads();
+    Sweep(L, n, r, xb, kW1, xa);
+    Sweep(L, n, r, xa, kW2, xb);
+    for (int c = t; c < n; c += kCoarseT) {
+      int i, j, k;
+      Ijk(c, L.nx, L.ny, i, j, k);
+      const Six k6 = Faces(L, c, i, j, k);
+      xa[c] = r[c] - ApplyAt(L, k6, c, i, j, k, [&](long q) { return xb[q]; });
+    }
+    __syncthreads();
+    const int nxc = fs.nx[l + 1], nyc = fs.ny[l + 1], nc = Cells(l + 1);
+    float* rc = sr + fs.off[l + 1];
+    for (int C = t; C < nc; C += kCoarseT) {
+      int I, J, K;
+      Ijk(C, nxc, nyc, I, J, K);
+      float sum = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
+        sum += xa[i + L.nx * (j + L.ny * k)];
+      }
+      rc[C] = sum;
+    }
+    __syncthreads();
+  }
+  CoarsestPcg(Lev(last), sr + fs.off[last], sxa + fs.off[last], sp, red);
+  __syncthreads();
+  // up: xb += P e, then xb -> xa -> xb -> xa (the level's correction in xa)
+  for (int l = last - 1; l >= 0; --l) {
+    const Lv L = Lev(l), C = Lev(l + 1);
+    const int n = Cells(l);
+    const float* r = sr + fs.off[l];
+    const float* e = sxa + fs.off[l + 1];
+    float *xa = sxa + fs.off[l], *xb = sxb + fs.off[l];
+    for (int c = t; c < n; c += kCoarseT) {
+      int i, j, k;
+      Ijk(c, L.nx, L.ny, i, j, k);
+      xb[c] += Prolong(C, e, i, j, k);
+    }
+    __syncthreads();
+    Sweep(L, n, r, xb, kW0, xa);
+    Sweep(L, n, r, xa, kW1, xb);
+    Sweep(L, n, r, xb, kW2, xa);
+  }
+  const int n0 = Cells(0);
+  for (int c = t; c < n0; c += kCoarseT) fs.out[c] = sxa[c];
 }
 
 // ---- flexible CG (FP64) -----------------------------------------------------
@@ -553,6 +666,15 @@ Mg3Fcg::Mg3Fcg(int nx, int ny, int nz) {
     cy /= 2;
     cz /= 2;
   }
+  // the small levels in one workgroup: from the first level below the finest
+  // of at most kFusedMax cells, if the chain fits the kernel's LDS
+  for (int l = 1; l < int(lv_.size()); ++l)
+    if (lv_[l].n <= kFusedMax) {
+      long cells = 0;
+      for (int m = l; m < int(lv_.size()); ++m) cells += lv_[m].n;
+      if (cells <= kFusedCells && int(lv_.size()) - l <= kMaxFused) first_fused_ = l;
+      break;
+    }
   const long n = lv_[0].n;
   sgn_ = (float*)Alloc(size_t(n) * 4);
   z32_ = (float*)Alloc(size_t(n) * 4);
@@ -589,6 +711,27 @@ void Mg3Fcg::Build(double* const (&t)[7], hipStream_t st) {
 void Mg3Fcg::Cycle(int l, float* out, hipStream_t st, const int* done) {
   Level& L = lv_[l];
   const Lv lv{L.nx, L.ny, L.nz, L.kx, L.ky, L.kz, L.dinv};
+  if (l > 0 && l == first_fused_) {
+    Fused3 fs{};
+    fs.count = int(lv_.size()) - l;
+    int off = 0;
+    for (int k = 0; k < fs.count; ++k) {
+      const Level& C = lv_[l + k];
+      fs.nx[k] = C.nx;
+      fs.ny[k] = C.ny;
+      fs.nz[k] = C.nz;
+      fs.off[k] = off;
+      off += int(C.n);
+      fs.kx[k] = C.kx;
+      fs.ky[k] = C.ky;
+      fs.kz[k] = C.kz;
+      fs.dinv[k] = C.dinv;
+    }
+    fs.r = L.r;
+    fs.out = out;
+    KFused<<<1, kCoarseT, 0, st>>>(fs, done);
+    return;
+  }
   if (l == int(lv_.size()) - 1) {
     if (l == 0)  // a single-level hierarchy: its FP32 residual first
       KDown12<true><<<Grid(L.n), kB, 0, st>>>(lv, r_, L.r, L.xa, done);
