@@ -377,7 +377,9 @@ KCoarsest(Lv L, const float* __restrict__ r, float* __restrict__ out, const int*
 // its correction to fs.out.
 constexpr int kFusedMax = 4096;
 constexpr int kFusedCells = 4688;  // a chain from 4096 cells by eighths: 4096 + 512 + 64 + 8 + 1
-constexpr int kMaxFused = 8;
+// levels halve while they hold more than 64 cells, so a set starting at <=
+// 4096 cells smooths at most two levels (4096, 512) above its coarsest
+constexpr int kMaxFused = 3;
 struct Fused3 {
   int count;  // levels; the last is the coarsest
   int nx[kMaxFused], ny[kMaxFused], nz[kMaxFused], off[kMaxFused];
@@ -386,48 +388,81 @@ struct Fused3 {
   float* out;
 };
 
+// cells per thread of fused level l: the first holds at most kFusedMax cells,
+// each later one at most an eighth of that
+constexpr int FusedSlots(int l) { return l == 0 ? kFusedMax / kCoarseT : 1; }
+static_assert(kFusedMax / 8 <= kCoarseT, "one slot on the fused levels after the first");
+
 __global__ void __launch_bounds__(kCoarseT)
 KFused(Fused3 fs, const int* done) {
-  if (Stopped(done)) return;
+  const int stopped = done ? *done : 0;  // tested once the loads are in flight
   __shared__ float sr[kFusedCells], sxa[kFusedCells], sxb[kFusedCells];
   __shared__ float sp[Mg3Fcg::kCoarsestMax];
   __shared__ float red[kCoarseT / 64];
   const int t = threadIdx.x;
+  const int last = fs.count - 1;
   auto Lev = [&](int l) {
     return Lv{fs.nx[l], fs.ny[l], fs.nz[l], fs.kx[l], fs.ky[l], fs.kz[l], fs.dinv[l]};
   };
   auto Cells = [&](int l) { return fs.nx[l] * fs.ny[l] * fs.nz[l]; };
-  // out = x + w D^-1 (r - A x) on level l (LDS vectors)
-  auto Sweep = [&](const Lv& L, int n, const float* r, const float* x, float w, float* o) {
-    for (int c = t; c < n; c += kCoarseT) {
+  // the faces and inverse diagonal of this thread's cells on every smoothed
+  // level, loaded once into registers (the phases then touch LDS only)
+  Six k6[kMaxFused - 1][FusedSlots(0)];
+  float di[kMaxFused - 1][FusedSlots(0)];
+#pragma unroll
+  for (int l = 0; l < kMaxFused - 1; ++l)
+#pragma unroll
+    for (int m = 0; m < FusedSlots(l); ++m) {
+      k6[l][m] = Six{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      di[l][m] = 0.f;
+      const int c = t + m * kCoarseT;
+      if (l >= last || c >= Cells(l)) continue;
       int i, j, k;
-      Ijk(c, L.nx, L.ny, i, j, k);
-      const Six k6 = Faces(L, c, i, j, k);
-      o[c] = x[c] + w * L.dinv[c] * (r[c] - ApplyAt(L, k6, c, i, j, k, [&](long q) { return x[q]; }));
+      Ijk(c, fs.nx[l], fs.ny[l], i, j, k);
+      k6[l][m] = Faces(Lev(l), c, i, j, k);
+      di[l][m] = fs.dinv[l][c];
     }
-    __syncthreads();
-  };
-  const int last = fs.count - 1;
   {
     const int n0 = Cells(0);
     for (int c = t; c < n0; c += kCoarseT) sr[c] = fs.r[c];
-    __syncthreads();
   }
+  if (stopped) return;
+  __syncthreads();
   // down: x1 -> xb, x2 -> xa, x3 -> xb, residual -> xa, restriction
-  for (int l = 0; l < last; ++l) {
+#pragma unroll
+  for (int l = 0; l < kMaxFused - 1; ++l) {
+    if (l >= last) break;
     const Lv L = Lev(l);
     const int n = Cells(l);
     const float* r = sr + fs.off[l];
     float *xa = sxa + fs.off[l], *xb = sxb + fs.off[l];
-    for (int c = t; c < n; c += kCoarseT) xb[c] = kW0 * L.dinv[c] * r[c];
-    __syncthreads();
-    Sweep(L, n, r, xb, kW1, xa);
-    Sweep(L, n, r, xa, kW2, xb);
-    for (int c = t; c < n; c += kCoarseT) {
+    // o = x + w D^-1 (r - A x) (x null: the first sweep from zero)
+    auto Sweep = [&](const float* x, float w, float* o) {
+#pragma unroll
+      for (int m = 0; m < FusedSlots(l); ++m) {
+        const int c = t + m * kCoarseT;
+        if (c >= n) continue;
+        if (!x) {
+          o[c] = w * di[l][m] * r[c];
+          continue;
+        }
+        int i, j, k;
+        Ijk(c, L.nx, L.ny, i, j, k);
+        o[c] = x[c] + w * di[l][m] *
+                          (r[c] - ApplyAt(L, k6[l][m], c, i, j, k, [&](long q) { return x[q]; }));
+      }
+      __syncthreads();
+    };
+    Sweep(nullptr, kW0, xb);
+    Sweep(xb, kW1, xa);
+    Sweep(xa, kW2, xb);
+#pragma unroll
+    for (int m = 0; m < FusedSlots(l); ++m) {
+      const int c = t + m * kCoarseT;
+      if (c >= n) continue;
       int i, j, k;
       Ijk(c, L.nx, L.ny, i, j, k);
-      const Six k6 = Faces(L, c, i, j, k);
-      xa[c] = r[c] - ApplyAt(L, k6, c, i, j, k, [&](long q) { return xb[q]; });
+      xa[c] = r[c] - ApplyAt(L, k6[l][m], c, i, j, k, [&](long q) { return xb[q]; });
     }
     __syncthreads();
     const int nxc = fs.nx[l + 1], nyc = fs.ny[l + 1], nc = Cells(l + 1);
@@ -448,21 +483,38 @@ KFused(Fused3 fs, const int* done) {
   CoarsestPcg(Lev(last), sr + fs.off[last], sxa + fs.off[last], sp, red);
   __syncthreads();
   // up: xb += P e, then xb -> xa -> xb -> xa (the level's correction in xa)
-  for (int l = last - 1; l >= 0; --l) {
+#pragma unroll
+  for (int l = kMaxFused - 2; l >= 0; --l) {
+    if (l >= last) continue;
     const Lv L = Lev(l), C = Lev(l + 1);
     const int n = Cells(l);
     const float* r = sr + fs.off[l];
     const float* e = sxa + fs.off[l + 1];
     float *xa = sxa + fs.off[l], *xb = sxb + fs.off[l];
-    for (int c = t; c < n; c += kCoarseT) {
+#pragma unroll
+    for (int m = 0; m < FusedSlots(l); ++m) {
+      const int c = t + m * kCoarseT;
+      if (c >= n) continue;
       int i, j, k;
       Ijk(c, L.nx, L.ny, i, j, k);
       xb[c] += Prolong(C, e, i, j, k);
     }
     __syncthreads();
-    Sweep(L, n, r, xb, kW0, xa);
-    Sweep(L, n, r, xa, kW1, xb);
-    Sweep(L, n, r, xb, kW2, xa);
+    auto Sweep = [&](const float* x, float w, float* o) {
+#pragma unroll
+      for (int m = 0; m < FusedSlots(l); ++m) {
+        const int c = t + m * kCoarseT;
+        if (c >= n) continue;
+        int i, j, k;
+        Ijk(c, L.nx, L.ny, i, j, k);
+        o[c] = x[c] + w * di[l][m] *
+                          (r[c] - ApplyAt(L, k6[l][m], c, i, j, k, [&](long q) { return x[q]; }));
+      }
+      __syncthreads();
+    };
+    Sweep(xb, kW0, xa);
+    Sweep(xa, kW1, xb);
+    Sweep(xb, kW2, xa);
   }
   const int n0 = Cells(0);
   for (int c = t; c < n0; c += kCoarseT) fs.out[c] = sxa[c];

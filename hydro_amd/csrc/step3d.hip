@@ -638,7 +638,8 @@ __global__ void KFormB(long n, const double* k, const double* mean, double* b) {
   if (c < n) b[c] = -k[c] - *mean;
 }
 struct CgScal {
-  double mean, bb, rz, pq, rr, rz2, done_rel;
+  double mean, bb, rz, pq, rr, rz2;
+  double converged;  // 1 when the stop was by the tolerance (set with done)
   int done, iters;
 };
 __global__ void KCgUpdate(long n, const double* p, const double* q, double* x, double* r,
@@ -665,10 +666,16 @@ __global__ void KCgUpdate(long n, const double* p, const double* q, double* x, d
     part_rz[blockIdx.x] = t2;
   }
 }
+// The stop test; converged records the reason at the stop, since the
+// iterations still queued behind it refold partial buffers into rr.
 __global__ void KCgCheck(CgScal* sc, double tol2, int max_iters) {
   if (sc->done) return;
   sc->iters += 1;
-  if (sc->rr <= tol2 * sc->bb || sc->iters >= max_iters) sc->done = 1;
+  const bool conv = sc->rr <= tol2 * sc->bb;
+  if (conv || sc->iters >= max_iters) {
+    sc->done = 1;
+    sc->converged = conv ? 1. : 0.;
+  }
 }
 __global__ void KCgDir(long n, const double* z, double* p, const CgScal* sc) {
   if (sc->done) return;
@@ -954,6 +961,7 @@ struct hg3_ctx {
   CgScal* cs;
   CgScal* cs_h;
   int jlimit = 0;
+  bool p_converged = true;  // the last pressure solve met its tolerance (cg, fcg_mg)
   int fluid_iter = 0, cd_iter = 0;
   double last_ind = 1.;
   int* state_h = nullptr;
@@ -1300,6 +1308,7 @@ int SolveCg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
       if (c->cs_h->done) break;
     }
     it = c->cs_h->iters;
+    c->p_converged = c->cs_h->converged != 0.;
   }
   Copy(c, x, c->cx, n);
   return it;
@@ -1320,6 +1329,7 @@ int SolveFcgMg(hg3_ctx* c, const Rows& R, const double* k, double* x) {
   const hg3::Fcg3Result res = c->mg->Solve(R.t, c->cb, x, cfg.pressure_tolerance_gpu,
                                            cfg.pressure_max_iters_gpu, c->st);
   if (res.failed) return res.iterations + SolveCg(c, R, k, x);
+  c->p_converged = res.converged;
   return res.iterations;
 }
 
@@ -1418,8 +1428,9 @@ void Iterate(hg3_ctx* c, hg_iter_stats* is) {
                                         Faces3{c->ffe[0], c->ffe[1], c->ffe[2]}, c->p_ip, c->ffd,
                                         cfg.rhie_chow_factor, c->fstar, c->fcoef);
   KPressureRows<<<Grid(nc), kB, 0, c->st>>>(g, c->fstar, c->fcoef, c->P);
+  c->p_converged = true;  // jacobi: the reference's stop rule, no flag
   is->pressure_solve_iters = RunSolver(c, cfg.linear_solver_pressure, c->P, c->P.k, c->pc);
-  is->pressure_converged = 1;
+  is->pressure_converged = c->p_converged ? 1 : 0;
   // corrections
   KCorrectPressure<<<Grid(nc), kB, 0, c->st>>>(nc, c->p_ip, c->pc, cfg.pressure_relaxation_factor,
                                                c->p_ic);
@@ -1714,6 +1725,7 @@ hg_status hg3_step(hg3_ctx* c, hg_step_stats* s) {
       while (!IsConverged(c)) {
         hg_iter_stats is{};
         Iterate(c, &is);
+        if (!is.pressure_converged) st.pressure_converged = 0;
         st.pressure_solve_iters_total += is.pressure_solve_iters;
         st.convergence_indicator = is.convergence_indicator;
       }

@@ -119,3 +119,18 @@ def test_gpu3d_fcg_mg_refuses_grids_without_a_small_coarsest_level(gpu):
         Hydro3(workloads.cavity3d(25, 25, 25, solver_p="fcg_mg"))
     # 26 x 26 x 26 halves once to 13^3 = 2197 cells: accepted
     Hydro3(workloads.cavity3d(26, 26, 26, solver_p="fcg_mg")).close()
+
+
+@pytest.mark.parametrize("solver", ["cg", "fcg_mg"])
+def test_gpu3d_iteration_cap_reports_not_converged(gpu, solver):
+    """A pressure solve cut off by pressure_max_iters_gpu clears the step's
+    pressure_converged; the same case with the default cap sets it."""
+    print(f"case: cavity 16^3 {solver}, pressure_max_iters_gpu 1", flush=True)
+    capped = workloads.cavity3d(16, 16, 16, solver_p=solver,
+                                extra="set int pressure_max_iters_gpu 1\n")
+    with Hydro3(capped) as h:
+        st = h.step()
+        assert st.pressure_converged == 0
+    with Hydro3(workloads.cavity3d(16, 16, 16, solver_p=solver)) as h:
+        st = h.step()
+        assert st.pressure_converged == 1
