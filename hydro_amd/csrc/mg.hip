@@ -594,15 +594,20 @@ KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
   M kw[kCcLevels][kCcSlots], ke[kCcLevels][kCcSlots], ks[kCcLevels][kCcSlots],
       kn[kCcLevels][kCcSlots], di[kCcLevels][kCcSlots], wx[kCcLevels][kCcSlots],
       wy[kCcLevels][kCcSlots];
+  // and its position, i in the low and j in the high 16 bits (the phases
+  // would otherwise divide by the level's width in every sweep)
+  int ij[kCcLevels][kCcSlots];
 #pragma unroll
   for (int l = 0; l < kCcLevels; ++l) {
 #pragma unroll
     for (int m = 0; m < CcSlots(l); ++m) {
       kw[l][m] = ke[l][m] = ks[l][m] = kn[l][m] = di[l][m] = wx[l][m] = wy[l][m] = 0.f;
+      ij[l][m] = 0;
       if (l >= nlev) continue;
       const int nx = cs.nx[l], c = t + m * kCcThreads;
       if (c >= nx * cs.ny[l]) continue;
       const int i = c % nx, j = c / nx;
+      ij[l][m] = i | (j << 16);
       kw[l][m] = cs.kx[l][i + (nx + 1) * j];
       ke[l][m] = cs.kx[l][i + 1 + (nx + 1) * j];
       ks[l][m] = cs.ky[l][i + nx * j];
@@ -676,7 +681,8 @@ KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
         if (c >= n) continue;
         dst[c] = k == 0 ? w.w[0] * di[l][m] * r[c]
                         : src[c] + w.w[k] * di[l][m] *
-                                       (r[c] - Apply(src, l, m, nx, ny, c, c % nx, c / nx));
+                                       (r[c] - Apply(src, l, m, nx, ny, c, ij[l][m] & 0xffff,
+                                                     ij[l][m] >> 16));
       }
       __syncthreads();
     }
@@ -684,7 +690,7 @@ KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
 #pragma unroll
     for (int m = 0; m < CcSlots(l); ++m) {
       const int c = t + m * kCcThreads;
-      if (c < n) xa[c] = r[c] - Apply(xb, l, m, nx, ny, c, c % nx, c / nx);
+      if (c < n) xa[c] = r[c] - Apply(xb, l, m, nx, ny, c, ij[l][m] & 0xffff, ij[l][m] >> 16);
     }
     __syncthreads();
     const int nxc = cs.nx[l + 1], nc = nxc * cs.ny[l + 1];
@@ -749,7 +755,7 @@ KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
     for (int m = 0; m < CcSlots(l); ++m) {
       const int c = t + m * kCcThreads;
       if (c >= n) continue;
-      const int i = c % nx, j = c / nx, I = i >> 1, J = j >> 1;
+      const int i = ij[l][m] & 0xffff, j = ij[l][m] >> 16, I = i >> 1, J = j >> 1;
       const int Ix = I + ((i & 1) ? 1 : -1), Jy = J + ((j & 1) ? 1 : -1);
       const M ec = e[I + nxc * J];
       const M ex = wx[l][m] > 0.f ? e[Ix + nxc * J] : ec;
@@ -768,7 +774,9 @@ KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
       for (int m = 0; m < CcSlots(l); ++m) {
         const int c = t + m * kCcThreads;
         if (c < n)
-          dst[c] = src[c] + w.w[k] * di[l][m] * (r[c] - Apply(src, l, m, nx, ny, c, c % nx, c / nx));
+          dst[c] = src[c] + w.w[k] * di[l][m] *
+                                (r[c] - Apply(src, l, m, nx, ny, c, ij[l][m] & 0xffff,
+                                              ij[l][m] >> 16));
       }
       __syncthreads();
     }

@@ -622,34 +622,41 @@ struct AdvFields {
 
 // One advection sub-step (advection.hpp:440-539) of NF fields sharing the
 // volume flux.  The cell gradients with the zero-derivative condition
-// (CellFaces<1> + GradFromFaces) are formed in LDS instead of being written
+// (CellFaces<1> + GradFromFaces) are formed on chip instead of being written
 // to HBM and read back: the block loads u on its 64 x kAdvTY tile with a
-// two-cell halo, forms the x-gradients on the tile plus one column each side
-// and the y-gradients on the tile plus one row each side, then updates the
-// tile with the Superbee faces.  Every value is computed with the same
-// operations as the unfused form, so the result is bit for bit the same.
+// two-cell halo into LDS (every field at once, one barrier); each thread then
+// owns a vertical segment of kAdvSeg cells of one column and forms, from its
+// column's values and its rows' two neighbours either side, the y-gradients
+// of the segment plus one cell each end and the x-gradients of each cell and
+// its west and east neighbours, in registers.  Every value is computed with
+// the same operations as the unfused form, so the result is bit for bit the
+// same; a gradient is formed by each cell that needs it (about three times
+// over), which costs less than a pass of LDS stores, a barrier and LDS
+// reloads (the kernel's limit is LDS traffic, not arithmetic).
 // R is the arithmetic type of the gradients and the Superbee face values,
 // and the type of the flux read (float in the mixed path).  The fields stay
 // FP64 and the update sums the four face terms in FP64: a face term is the
 // same number for the two cells it separates, so the fields' volume sums
 // are conserved to FP64 rounding in both paths.
 constexpr int kAdvTX = 64, kAdvTY = 8;
+constexpr int kAdvSeg = kAdvTY / (kBlock / kAdvTX);  // cells per thread, one column
+static_assert(kBlock % kAdvTX == 0 && kAdvTY % (kBlock / kAdvTX) == 0, "advection tile");
 
 template <int NF, bool IN, class R>
 __global__ void __launch_bounds__(kBlock)
 KAdvStep(Geo g, AdvFields fl, const R* __restrict__ flux, double dt) {
-  constexpr int UX = kAdvTX + 4, UY = kAdvTY + 4;
-  __shared__ double su[UY][UX];
-  __shared__ R sgx[kAdvTY][kAdvTX + 2];
-  __shared__ R sgy[kAdvTY + 2][kAdvTX];
+  constexpr int UX = kAdvTX + 4, UY = kAdvTY + 4, S = kAdvSeg;
+  __shared__ double su[NF][UY][UX];
   const int nx = g.nx, ny = g.ny;
   const int ntx = (nx + kAdvTX - 1) / kAdvTX;
   const int i0 = (blockIdx.x % ntx) * kAdvTX, j0 = (blockIdx.x / ntx) * kAdvTY;
+  // the thread's cells: column li, tile rows lj0 .. lj0 + S - 1
+  const int li = threadIdx.x % kAdvTX, lj0 = (threadIdx.x / kAdvTX) * S;
+  const int i = i0 + li;
   // every global load of the block is issued here, before any value is used:
-  // each field on the halo-2 tile and the four face fluxes of the thread's
-  // tile cells (so the block waits on memory once, not once per element)
+  // each field on the halo-2 tile and the face fluxes of the thread's cells
+  // (so the block waits on memory once, not once per element)
   constexpr int NU = UX * UY, KU = (NU + kBlock - 1) / kBlock;
-  constexpr int NT = kAdvTX * kAdvTY, KT = (NT + kBlock - 1) / kBlock;
   double ureg[NF][KU];
 #pragma unroll
   for (int q = 0; q < NF; ++q)
@@ -660,85 +667,88 @@ KAdvStep(Geo g, AdvFields fl, const R* __restrict__ flux, double dt) {
       ureg[q][k] = (t < NU && ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? fl.u[q][ii + nx * jj]
                                                                          : 0.;
     }
-  R vw_[KT], ve_[KT], vs_[KT], vn_[KT];
+  // x-faces west and east of each cell; the segment's S + 1 y-faces
+  R vw_[S], ve_[S], vy_[S + 1];
 #pragma unroll
-  for (int k = 0; k < KT; ++k) {
-    const int t = threadIdx.x + k * kBlock;
-    const int i = i0 + t % kAdvTX, j = j0 + t / kAdvTX;
-    const bool in = t < NT && i < nx && j < ny;
-    vw_[k] = in ? flux[XFace(g, i, j)] : R(0);
-    ve_[k] = in ? flux[XFace(g, i + 1, j)] : R(0);
-    vs_[k] = in ? flux[YFace(g, i, j)] : R(0);
-    vn_[k] = in ? flux[YFace(g, i, j + 1)] : R(0);
+  for (int s = 0; s < S; ++s) {
+    const int j = j0 + lj0 + s;
+    const bool in = i < nx && j < ny;
+    vw_[s] = in ? flux[XFace(g, i, j)] : R(0);
+    ve_[s] = in ? flux[XFace(g, i + 1, j)] : R(0);
   }
 #pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    if (q > 0) __syncthreads();  // the previous field's LDS is still being read
+  for (int s = 0; s <= S; ++s) {
+    const int j = j0 + lj0 + s;
+    vy_[s] = (i < nx && j <= ny) ? flux[YFace(g, i, j)] : R(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NF; ++q)
 #pragma unroll
     for (int k = 0; k < KU; ++k) {
       const int t = threadIdx.x + k * kBlock;
-      if (t < NU) su[t / UX][t % UX] = ureg[q][k];
+      if (t < NU) su[q][t / UX][t % UX] = ureg[q][k];
     }
-    __syncthreads();
+  __syncthreads();
+  const R gsx = R(g.sx), gsy = R(g.sy);
+  const R tx0 = R(g.tx0), tx1 = R(g.tx1), ty0 = R(g.ty0), ty1 = R(g.ty1);
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
     // a boundary face's value: the inlet's, or the cell's (zero derivative)
     const double* bv = fl.bval[q];
     auto bface = [&](int s, int pos, R uc) {
       return (IN && bv && fl.inlet[s]) ? R(bv[BoundaryIndex(g, s, pos)]) : uc;
     };
-    const R gsx = R(g.sx), gsy = R(g.sy);
-    // x-gradients on columns i0-1 .. i0+64 of the tile rows
-    for (int t = threadIdx.x; t < kAdvTY * (kAdvTX + 2); t += kBlock) {
-      const int lj = t / (kAdvTX + 2), hi = t % (kAdvTX + 2);
-      const int ii = i0 + hi - 1, jj = j0 + lj;
+    // the column's values at tile rows lj0 - 2 .. lj0 + S + 1
+    double col[S + 4];
+#pragma unroll
+    for (int r = 0; r < S + 4; ++r) col[r] = su[q][lj0 + r][li + 2];
+    // y-gradients at rows j0 + lj0 - 1 .. j0 + lj0 + S (zero outside the grid)
+    R gy[S + 2];
+#pragma unroll
+    for (int r = 0; r < S + 2; ++r) {
+      const int jj = j0 + lj0 - 1 + r;
       R gv = R(0);
-      if (ii >= 0 && ii < nx && jj < ny) {
-        const R uc = R(su[lj + 2][hi + 1]);
-        const R fw = ii > 0 ? MidR<R>(R(su[lj + 2][hi]), uc) : bface(kLeft, jj, uc);
-        const R fe = ii < nx - 1 ? MidR<R>(uc, R(su[lj + 2][hi + 2])) : bface(kRight, jj, uc);
-        R sxv = (-gsx) * fw;
-        sxv += gsx * fe;
-        gv = OverVR<R>(g, sxv);
-      }
-      sgx[lj][hi] = gv;
-    }
-    // y-gradients on rows j0-1 .. j0+kAdvTY of the tile columns
-    for (int t = threadIdx.x; t < (kAdvTY + 2) * kAdvTX; t += kBlock) {
-      const int hj = t / kAdvTX, li = t % kAdvTX;
-      const int ii = i0 + li, jj = j0 + hj - 1;
-      R gv = R(0);
-      if (ii < nx && jj >= 0 && jj < ny) {
-        const R uc = R(su[hj + 1][li + 2]);
-        const R fs = jj > 0 ? MidR<R>(R(su[hj][li + 2]), uc) : bface(kBottom, ii, uc);
-        const R fn = jj < ny - 1 ? MidR<R>(uc, R(su[hj + 2][li + 2])) : bface(kTop, ii, uc);
+      if (i < nx && jj >= 0 && jj < ny) {
+        const R uc = R(col[r + 1]);
+        const R fs = jj > 0 ? MidR<R>(R(col[r]), uc) : bface(kBottom, i, uc);
+        const R fn = jj < ny - 1 ? MidR<R>(uc, R(col[r + 2])) : bface(kTop, i, uc);
         R syv = (-gsy) * fs;
         syv += gsy * fn;
         gv = OverVR<R>(g, syv);
       }
-      sgy[hj][li] = gv;
+      gy[r] = gv;
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < KT; ++k) {
-      const int t = threadIdx.x + k * kBlock;
-      const int lj = t / kAdvTX, li = t % kAdvTX;
-      const int i = i0 + li, j = j0 + lj;
-      if (t >= NT || i >= nx || j >= ny) continue;
+    for (int s = 0; s < S; ++s) {
+      const int lj = lj0 + s, j = j0 + lj;
+      if (i >= nx || j >= ny) continue;
       const int c = i + nx * j;
-      const R vw = vw_[k], ve = ve_[k], vs = vs_[k], vn = vn_[k];
-      const double uc64 = su[lj + 2][li + 2];
+      // the row's values at i - 2 .. i + 2
+      const R uW2 = R(su[q][lj + 2][li]), uW = R(su[q][lj + 2][li + 1]);
+      const R uE = R(su[q][lj + 2][li + 3]), uE2 = R(su[q][lj + 2][li + 4]);
+      const double uc64 = col[s + 2];
       const R uc = R(uc64);
-      const R gxc = sgx[lj][li + 1], gyc = sgy[lj + 1][li];
-      const R tx0 = R(g.tx0), tx1 = R(g.tx1), ty0 = R(g.ty0), ty1 = R(g.ty1);
+      // x-gradient of cell ii of this row from its value u0 and its
+      // neighbours' um, up (zero outside the grid)
+      auto gx = [&](int ii, R um, R u0, R up) {
+        R gv = R(0);
+        if (ii >= 0 && ii < nx) {
+          const R fw = ii > 0 ? MidR<R>(um, u0) : bface(kLeft, j, u0);
+          const R fe = ii < nx - 1 ? MidR<R>(u0, up) : bface(kRight, j, u0);
+          R sxv = (-gsx) * fw;
+          sxv += gsx * fe;
+          gv = OverVR<R>(g, sxv);
+        }
+        return gv;
+      };
+      const R gxc = gx(i, uW, uc, uE), gyc = gy[s + 1];
+      const R vw = vw_[s], ve = ve_[s], vs = vy_[s], vn = vy_[s + 1];
       R uw = i > 0 ? uc : bface(kLeft, j, uc), ue = i < nx - 1 ? uc : bface(kRight, j, uc);
       R us_ = j > 0 ? uc : bface(kBottom, i, uc), uN = j < ny - 1 ? uc : bface(kTop, i, uc);
-      if (i > 0)
-        uw = SuperbeeFace<R>(R(su[lj + 2][li + 1]), uc, sgx[lj][li], gxc, tx0, tx1, vw);
-      if (i < nx - 1)
-        ue = SuperbeeFace<R>(uc, R(su[lj + 2][li + 3]), gxc, sgx[lj][li + 2], tx0, tx1, ve);
-      if (j > 0)
-        us_ = SuperbeeFace<R>(R(su[lj + 1][li + 2]), uc, sgy[lj][li], gyc, ty0, ty1, vs);
-      if (j < ny - 1)
-        uN = SuperbeeFace<R>(uc, R(su[lj + 3][li + 2]), gyc, sgy[lj + 2][li], ty0, ty1, vn);
+      if (i > 0) uw = SuperbeeFace<R>(uW, uc, gx(i - 1, uW2, uW, uc), gxc, tx0, tx1, vw);
+      if (i < nx - 1) ue = SuperbeeFace<R>(uc, uE, gxc, gx(i + 1, uc, uE, uE2), tx0, tx1, ve);
+      if (j > 0) us_ = SuperbeeFace<R>(R(col[s + 1]), uc, gy[s], gyc, ty0, ty1, vs);
+      if (j < ny - 1) uN = SuperbeeFace<R>(uc, R(col[s + 3]), gyc, gy[s + 2], ty0, ty1, vn);
       double fsum = 0.;
       fsum += double(uw * vw * R(-1));
       fsum += double(ue * ve * R(1));
