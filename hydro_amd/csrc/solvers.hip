@@ -105,51 +105,136 @@ KLuWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k0,
   }
 }
 
+// Barrier between wavefront steps that orders only the LDS exchange: the
+// global loads a step has in flight (the next row's operands) are not waited
+// for, since no thread writes what they read before that row's step.
+__device__ __forceinline__ void LdsBarrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One cell row's operands for a thread's CW columns: the row coefficients,
+// the constant, and the old values of the cell and of its east and north
+// neighbours.  Within a sweep those three are still the previous sweep's
+// when the row's step comes: the east cell belongs to this thread (later in
+// the row) or to the next thread (one step later), the north cell to the
+// next row of this thread.
+template <int CW>
+struct GsRow {
+  double s[CW], w[CW], e[CW], n[CW], c[CW], k[CW], xo[CW], xe[CW], xn[CW];
+};
+
+// With one column per thread every load is issued, with the row and column
+// clamped into the grid (a clamped value is never used: the sweep skips rows
+// outside the grid and columns past nx, and tests i < nx - 1, j < ny - 1
+// before using the east and north values), so the number of loads in flight
+// is the same on every path and the wait for a row's operands leaves the next
+// row's loads in flight (191 against 220 us per 256 x 128 sweep).  Wider
+// threads load only what they use: the unconditional form needs more
+// registers than they have and spills (2062 against 592 us, 2048 x 8).
+template <int CW>
+__device__ __forceinline__ void GsLoad(GsRow<CW>& o, int nx, int ny, int c0, int j,
+                                       const Rows& a, const double* __restrict__ k,
+                                       const double* __restrict__ x) {
+  if constexpr (CW == 1) {
+    const int jc = min(max(j, 0), ny - 1);
+    const int i = min(c0, nx - 1), r = i + nx * jc;
+    o.s[0] = a.s[r];
+    o.w[0] = a.w[r];
+    o.e[0] = a.e[r];
+    o.n[0] = a.n[r];
+    o.c[0] = a.c[r];
+    o.k[0] = k[r];
+    o.xo[0] = x[r];
+    o.xe[0] = x[i < nx - 1 ? r + 1 : r];
+    o.xn[0] = x[jc < ny - 1 ? r + nx : r];
+  } else {
+    if (j < 0 || j >= ny) return;
+#pragma unroll
+    for (int m = 0; m < CW; ++m) {
+      const int i = c0 + m;
+      if (i >= nx) break;
+      const int r = i + nx * j;
+      o.s[m] = a.s[r];
+      o.w[m] = a.w[r];
+      o.e[m] = a.e[r];
+      o.n[m] = a.n[r];
+      o.c[m] = a.c[r];
+      o.k[m] = k[r];
+      o.xo[m] = x[r];
+      o.xe[m] = i < nx - 1 ? x[r + 1] : 0.;
+      o.xn[m] = j < ny - 1 ? x[r + nx] : 0.;
+    }
+  }
+}
+
 // Lexicographic SOR, every sweep in this workgroup (linear.hpp:685-715).
+// Thread t owns columns c0 .. c0 + CW - 1 and takes row j = step - t at each
+// step; the row of the next step is loaded while this one is computed (two
+// operand sets in turn), and the steps are separated by LDS-only barriers.
+// (Loading two steps ahead measured no faster: the loads are hidden, and a
+// step's time is its chain of LDS exchange, division and barrier.)
+// Each sweep ends with the workgroup maximum of |corr| behind a full
+// barrier, which also makes the sweep's x visible to the next.
+template <int CW>
 __global__ void __launch_bounds__(kMaxThreads)
-KGsWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k,
-             double* __restrict__ x, double tol, int limit, double omega,
-             unsigned long long* __restrict__ info) {
+KGsWavefront(int nx, int ny, Rows a, const double* __restrict__ k, double* __restrict__ x,
+             double tol, int limit, double omega, unsigned long long* __restrict__ info) {
   extern __shared__ double lds[];  // [2][T] west values, then reduction scratch
   double* red = lds + 2 * blockDim.x;
   const int T = blockDim.x;
   const int t = threadIdx.x;
-  const int c0 = t * cw;
-  const int c1 = min(nx, c0 + cw);
-  const bool active = c0 < nx;
+  const int c0 = t * CW;
+  const int nc = max(0, min(nx, c0 + CW) - c0);
   for (int r = t; r < nx * ny; r += T) x[r] = 0.;
   __syncthreads();
-  double south[kMaxCw];
+  double south[CW];
   long iter = 0;
   double diff = 0.;
+  const int nsteps = ny + T - 1;
+  GsRow<CW> ra, rb;
   while (true) {
     double dmax = 0.;
-    for (int step = 0; step < ny + T - 1; ++step) {
+    // step `step` with the operands of its row in `o`
+    auto Step = [&](int step, const GsRow<CW>& o) {
       const int j = step - t;
       const int par = step & 1;
-      if (active && j >= 0 && j < ny) {
+      if (nc > 0 && j >= 0 && j < ny) {
         double west = (t > 0) ? lds[(par ^ 1) * T + t - 1] : 0.;
-        for (int i = c0; i < c1; ++i) {
-          const int r = i + nx * j;
+#pragma unroll
+        for (int m = 0; m < CW; ++m) {
+          if (m >= nc) break;
+          const int i = c0 + m, r = i + nx * j;
           double sum = 0.;
-          if (j > 0) sum += a.s[r] * south[i - c0];
-          if (i > 0) sum += a.w[r] * west;
-          if (i < nx - 1) sum += a.e[r] * x[r + 1];
-          if (j < ny - 1) sum += a.n[r] * x[r + nx];
-          const double xo = x[r];
-          const double value = -(k[r] + sum) / a.c[r];
+          if (j > 0) sum += o.s[m] * south[m];
+          if (i > 0) sum += o.w[m] * west;
+          if (i < nx - 1) sum += o.e[m] * o.xe[m];
+          if (j < ny - 1) sum += o.n[m] * o.xn[m];
+          const double xo = o.xo[m];
+          const double value = -(o.k[m] + sum) / o.c[m];
           const double corr = value - xo;
           const double ac = fabs(corr);
           dmax = (dmax < ac) ? ac : dmax;  // std::max(diff, |corr|)
           const double xn = xo + corr * omega;
           x[r] = xn;
-          south[i - c0] = xn;
+          south[m] = xn;
           west = xn;
         }
         lds[par * T + t] = west;
       }
-      __syncthreads();
+      LdsBarrier();
+    };
+    GsLoad<CW>(ra, nx, ny, c0, -t, a, k, x);
+    for (int step = 0; step < nsteps; step += 2) {
+      GsLoad<CW>(rb, nx, ny, c0, step + 1 - t, a, k, x);
+      Step(step, ra);
+      if (step + 1 < nsteps) {
+        GsLoad<CW>(ra, nx, ny, c0, step + 2 - t, a, k, x);
+        Step(step + 1, rb);
+      }
     }
+    __syncthreads();
     // workgroup max of |corr|
     double m = WaveMax(dmax);
     if ((t & 63) == 0) red[t >> 6] = m;
@@ -416,7 +501,12 @@ void LaunchGsWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k, 
   int cw;
   int T = ThreadsFor(nx, &cw);
   size_t lds = size_t(2) * T * sizeof(double) + 16 * sizeof(double);
-  KGsWavefront<<<1, T, lds, st>>>(nx, ny, cw, a, k, x, tol, limit, omega, info);
+  switch (cw) {
+    case 1: KGsWavefront<1><<<1, T, lds, st>>>(nx, ny, a, k, x, tol, limit, omega, info); break;
+    case 2: KGsWavefront<2><<<1, T, lds, st>>>(nx, ny, a, k, x, tol, limit, omega, info); break;
+    case 3: KGsWavefront<3><<<1, T, lds, st>>>(nx, ny, a, k, x, tol, limit, omega, info); break;
+    default: KGsWavefront<4><<<1, T, lds, st>>>(nx, ny, a, k, x, tol, limit, omega, info); break;
+  }
 }
 
 void LaunchLuRelaxed(hipStream_t st, int nx, int ny, Rows a, const double* k, double* res,

@@ -211,7 +211,7 @@ def test_long_trajectory_matches_reference(gpu, name):
     stats, digests = [], {}
     for s in range(1, fix["steps"] + 1):
         stats.append(g.step())
-        if s % 50 == 0:
+        if s % 10 == 0:
             print(f"  step {s}", flush=True)
         if str(s) in fix["dumps"]:
             digests[s] = field_digests(g.get, g.cfg.num_phases)
