@@ -99,12 +99,30 @@ __device__ __forceinline__ void PutPartials(const double (&v)[NQ], double* part)
   }
 }
 
+// A thread's share of a partial sum: part[t] + part[t + stride] + ... (from
+// 0, in that order) over indices below n.  The first K terms are loaded at
+// once, at indices clamped into the array and selected afterwards, so their
+// memory round trips overlap (a loop with a load per iteration waits on
+// each); the rest, if any, in a loop.
+template <int K>
+__device__ __forceinline__ double StridedSum(const double* part, int n, int t, int stride) {
+  const int last = n > 0 ? n - 1 : 0;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = part[min(t + k * stride, last)];
+  double s = 0.;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (t + k * stride < n) s += v[k];
+  for (int b = t + K * stride; b < n; b += stride) s += part[b];
+  return s;
+}
+
 // Sum of `n` partials, identical in every workgroup and thread.
 __device__ __forceinline__ double Fold(const double* part, int n) {
   __shared__ double red[kB / 64];
   __shared__ double tot;
-  double s = 0.;
-  for (int b = threadIdx.x; b < n; b += kB) s += part[b];
+  double s = StridedSum<8>(part, n, threadIdx.x, kB);
   s = WaveSum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -126,16 +144,8 @@ __device__ __forceinline__ void FoldN(const double* const (&part)[NQ], const int
                                       double (&out)[NQ]) {
   __shared__ double red[NQ][kB / 64];
   double s[NQ];
-  int nmax = 0;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    s[q] = 0.;
-    nmax = n[q] > nmax ? n[q] : nmax;
-  }
-  for (int b = threadIdx.x; b < nmax; b += kB)
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      if (b < n[q]) s[q] += part[q][b];
+  for (int q = 0; q < NQ; ++q) s[q] = StridedSum<8>(part[q], n[q], threadIdx.x, kB);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const double v = WaveSum(s[q]);
@@ -496,20 +506,26 @@ KStartWarmScale(Walk w, const double* __restrict__ bp, double* __restrict__ r,
 }
 
 // Sums of NQ partial arrays by one workgroup of kWide threads, the same in
-// every thread.  The FCG scalars are folded once per iteration this way
+// every thread.  With `stop` (a device flag, the same for the whole
+// workgroup), a set flag returns false before the first barrier; the flag
+// is tested after the partials are loaded, so that the loads do not wait
+// for it.  The FCG scalars are folded once per iteration this way
 // rather than in every workgroup of the kernel that uses them: a few
 // thousand partials, which a wide workgroup folds sooner than 256 threads.
 constexpr int kWide = 1024;
 template <int NQ>
-__device__ __forceinline__ void WideFold(const double* const (&part)[NQ], const int (&n)[NQ],
-                                         double (&out)[NQ]) {
+__device__ __forceinline__ bool WideFold(const double* const (&part)[NQ], const int (&n)[NQ],
+                                         double (&out)[NQ], const int* stop = nullptr) {
   __shared__ double red[NQ][kWide / 64];
+  const int stopped = stop ? *stop : 0;
+  double s[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) s[q] = StridedSum<2>(part[q], n[q], threadIdx.x, kWide);
+  if (stopped) return false;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    double s = 0.;
-    for (int b = threadIdx.x; b < n[q]; b += kWide) s += part[q][b];
-    s = WaveSum(s);
-    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s;
+    s[q] = WaveSum(s[q]);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = s[q];
   }
   __syncthreads();
 #pragma unroll
@@ -518,6 +534,7 @@ __device__ __forceinline__ void WideFold(const double* const (&part)[NQ], const 
     for (int w = 0; w < kWide / 64; ++w) t += red[q][w];
     out[q] = t;
   }
+  return true;
 }
 
 // FCG(1) direction scalars of iteration k from the preconditioner's dots:
@@ -525,9 +542,10 @@ __device__ __forceinline__ void WideFold(const double* const (&part)[NQ], const 
 // (r, z) <= 0 (preconditioner not positive on r) sets fail and done.
 __global__ void __launch_bounds__(kWide)
 KFcgBeta(int k, Parts pt, CgScalars* sc) {
-  if (sc->done) return;
   double f[3];
-  WideFold<3>({pt.dir + pt.ndir, pt.dir, pt.spmv[(k + 1) & 1]}, {pt.ndir, pt.ndir, pt.nb}, f);
+  if (!WideFold<3>({pt.dir + pt.ndir, pt.dir, pt.spmv[(k + 1) & 1]}, {pt.ndir, pt.ndir, pt.nb}, f,
+                   &sc->done))
+    return;
   if (threadIdx.x == 0) {
     if (f[0] > 0.) {
       sc->rz = f[0];
@@ -543,9 +561,8 @@ KFcgBeta(int k, Parts pt, CgScalars* sc) {
 // a later kernel applies; (p, q) <= 0 sets fail and done instead.
 __global__ void __launch_bounds__(kWide)
 KFcgAlpha(int k, Parts pt, CgScalars* sc) {
-  if (sc->done) return;
   double f[1];
-  WideFold<1>({pt.spmv[k & 1]}, {pt.nb}, f);
+  if (!WideFold<1>({pt.spmv[k & 1]}, {pt.nb}, f, &sc->done)) return;
   if (threadIdx.x == 0) {
     if (f[0] > 0.) {
       sc->alpha = sc->rz / f[0];

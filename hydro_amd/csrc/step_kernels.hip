@@ -506,8 +506,18 @@ KCorrectCells(Geo g, const double* __restrict__ pc, const double* __restrict__ a
 constexpr int kFoldThreads = 1024;
 __global__ void __launch_bounds__(kFoldThreads)
 KMaxFold(const double* __restrict__ part, int n, double* __restrict__ rs) {
+  // the first eight partials of the thread are loaded at once (indices
+  // clamped into the array, selected afterwards), the rest in a loop
+  constexpr int K = 8;
+  const int t = threadIdx.x, last = n > 0 ? n - 1 : 0;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = part[min(t + k * kFoldThreads, last)];
   double m = 0.;
-  for (int b = threadIdx.x; b < n; b += kFoldThreads) m = fmax(m, part[b]);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (t + k * kFoldThreads < n) m = fmax(m, v[k]);
+  for (int b = t + K * kFoldThreads; b < n; b += kFoldThreads) m = fmax(m, part[b]);
   m = WaveMax(m);
   __shared__ double red[kFoldThreads / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
