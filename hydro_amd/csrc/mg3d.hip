@@ -540,11 +540,28 @@ __device__ __forceinline__ void PutPartial(double v, double* part) {
   const double t = BlockSumD(v, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
+// A thread's share of a partial sum: part[t] + part[t + stride] + ... (from
+// 0, in that order) over indices below n; the first K terms are loaded at
+// once (indices clamped into the array, selected afterwards) so that their
+// memory round trips overlap, the rest in a loop.
+template <int K>
+__device__ __forceinline__ double StridedSum(const double* part, int n, int t, int stride) {
+  const int last = n > 0 ? n - 1 : 0;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = part[min(t + k * stride, last)];
+  double s = 0.;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (t + k * stride < n) s += v[k];
+  for (int b = t + K * stride; b < n; b += stride) s += part[b];
+  return s;
+}
 // the sum of n partials, the same in every thread of the (kB-thread) block
 __device__ __forceinline__ double FoldPartials(const double* part, int n) {
   __shared__ double red[kB / 64];
-  double v = 0.;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) v += part[q];
+  // n <= kMaxBlocks = 4 kB
+  const double v = StridedSum<4>(part, n, threadIdx.x, blockDim.x);
   return BlockSumD(v, red);
 }
 

@@ -600,10 +600,26 @@ __global__ void KApplyRows(G g, Rows A, const double* __restrict__ x, double* __
     if (threadIdx.x == 0) part[blockIdx.x] = t;
   }
 }
+// A thread's share of a partial sum: part[t] + part[t + stride] + ... (from
+// 0, in that order) over indices below n; the first K terms are loaded at
+// once (indices clamped into the array, selected afterwards) so that their
+// memory round trips overlap, the rest in a loop.
+template <int K>
+__device__ __forceinline__ double StridedSum(const double* part, int n, int t, int stride) {
+  const int last = n > 0 ? n - 1 : 0;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = part[min(t + k * stride, last)];
+  double s = 0.;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (t + k * stride < n) s += v[k];
+  for (int b = t + K * stride; b < n; b += stride) s += part[b];
+  return s;
+}
 __global__ void KFold(const double* part, int n, double* out) {
   __shared__ double red[kB / 64];
-  double v = 0.;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+  const double v = StridedSum<4>(part, n, threadIdx.x, blockDim.x);  // n <= kCgBlocks = 4 kB
   const double t = BlockSum(v, red);
   if (threadIdx.x == 0) *out = t;
 }
@@ -801,8 +817,19 @@ __global__ void KDiffMax(long n, const double* a0, const double* a1, const doubl
 }
 __global__ void KMaxFold(const double* part, int n, double* out) {
   __shared__ double red[kB / 64];
+  // a thread's first 32 partials loaded at once (indices clamped into the
+  // array, selected afterwards; a maximum is exact in any order), the rest
+  // in a loop
+  constexpr int K = 32;
+  const int t = threadIdx.x, last = n > 0 ? n - 1 : 0;
+  double w[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) w[k] = part[min(t + k * kB, last)];
   double v = 0.;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[i]);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (t + k * kB < n) v = fmax(v, w[k]);
+  for (int i = t + K * kB; i < n; i += kB) v = fmax(v, part[i]);
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
