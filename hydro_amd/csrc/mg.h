@@ -121,6 +121,9 @@ class MgPrecondT : public Precond {
  private:
   std::vector<MgLevelT<M>> lv_;
   int first_in_kernel_ = -1;
+  // a device flag that stays 0: the stop flag of cycles run without one, so
+  // that the kernels always read a flag (no null test ahead of their loads)
+  int* never_done_ = nullptr;
   // level l's correction: into z (FP32) on level 0, into out on the others
   void Cycle(int l, const double* r64, float* z, M* out, hipStream_t st, const int* d_done,
              const FcgFusion* f = nullptr);

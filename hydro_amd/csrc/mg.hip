@@ -205,6 +205,10 @@ struct SlotCell {
 };
 
 // S: the tile's TileSlots<TY, SW>.
+// Every load is issued, at indices clamped into the grid, and selected
+// afterwards (zero outside the grid or the slot's use): the number of loads
+// in flight is then the same on every path, so the tile waits on memory once
+// instead of once per conditionally loaded slot.
 template <class S, class RT, class M>
 __device__ __forceinline__ void LoadSlots(SlotCell<M> (&sc)[S::N], RT (&rv)[S::N],
                                           int nx, int ny, int i0, int j0,
@@ -221,14 +225,26 @@ __device__ __forceinline__ void LoadSlots(SlotCell<M> (&sc)[S::N], RT (&rv)[S::N
     c.i = i0 + c.li;
     c.j = j0 + c.lj;
     c.in = c.mine && c.i >= 0 && c.i < nx && c.j >= 0 && c.j < ny;
-    const int k = c.i + nx * c.j;
-    const bool coef = c.in && s < S::NT + SW - 1;  // tile and inner rings
-    c.kw = coef ? kx[c.i + (nx + 1) * c.j] : 0.f;
-    c.ke = coef ? kx[c.i + 1 + (nx + 1) * c.j] : 0.f;
-    c.ks = coef ? ky[k] : 0.f;
-    c.kn = coef ? ky[k + nx] : 0.f;
-    c.di = c.in ? dinv[k] : 0.f;
-    rv[s] = (c.in && (s < S::NT + SW - 1 || r_on_last_ring)) ? r[k] : RT(0);
+    const int ic = min(max(c.i, 0), nx - 1), jc = min(max(c.j, 0), ny - 1);
+    const int k = ic + nx * jc;
+    // the tile and the inner rings carry faces (the outermost ring only
+    // its inverse diagonal and, if asked, its right-hand side)
+    const bool faces = s < S::NT + SW - 1;
+    M kw = 0.f, ke = 0.f, ks = 0.f, kn = 0.f;
+    if (faces) {  // a constant of the unrolled loop
+      kw = kx[ic + (nx + 1) * jc];
+      ke = kx[ic + 1 + (nx + 1) * jc];
+      ks = ky[k];
+      kn = ky[k + nx];
+    }
+    const M di = dinv[k];
+    const RT rk = r[k];
+    c.kw = (faces && c.in) ? kw : M(0);
+    c.ke = (faces && c.in) ? ke : M(0);
+    c.ks = (faces && c.in) ? ks : M(0);
+    c.kn = (faces && c.in) ? kn : M(0);
+    c.di = c.in ? di : M(0);
+    rv[s] = (c.in && (s < S::NT + SW - 1 || r_on_last_ring)) ? rk : RT(0);
   }
 }
 
@@ -299,7 +315,7 @@ KDown(int nx, int ny, TileGrid tg, const M* __restrict__ kx, const M* __restrict
   // the stop flag is read with the data: its load is issued first and
   // tested once the tile's loads are in flight, so a working launch does
   // not wait on it before the rest
-  const int stopped = done ? *done : 0;
+  const int stopped = *done;  // never null (MgPrecondT::never_done_)
   using S = TileSlots<TY, SW>;
   constexpr int H = SW, XW = kTX + 2 * H, YH = TY + 2 * H;
   __shared__ M sx[2][YH][XW], sres[TY][kTX];
@@ -374,7 +390,7 @@ KUp(int nx, int ny, TileGrid tg, const M* __restrict__ kx, const M* __restrict__
     const M* __restrict__ e, const M* __restrict__ kxc, const M* __restrict__ kyc,
     int nxc, int nyc, SweepWeights w, OT* __restrict__ out, const int* done, FcgFusion fcg,
     int dots) {
-  const int stopped = done ? *done : 0;  // tested once the loads are in flight
+  const int stopped = *done;  // tested once the loads are in flight; never null
   using S = TileSlots<TY, SW>;
   constexpr int H = SW, XW = kTX + 2 * H, YH = TY + 2 * H;
   // coarse cells i0/2 - 2 .. i0/2 + kTX/2 + 1, likewise in y: the parents of
@@ -586,7 +602,7 @@ static_assert(kInKernelMax % kCcThreads == 0 || kInKernelMax < kCcThreads, "slot
 template <class M>
 __global__ void __launch_bounds__(kCcThreads)
 KCoarseCycle(CoarseSetT<M> cs, SweepWeights w, const int* done) {
-  const int stopped = done ? *done : 0;  // tested once the loads are in flight
+  const int stopped = *done;  // tested once the loads are in flight; never null
   __shared__ M sxa[kLdsCells], sxb[kLdsCells], sr[kLdsCells];
   const int t = threadIdx.x;
   const int nlev = cs.count - 1;
@@ -820,6 +836,8 @@ MgPrecondT<M>::MgPrecondT(int nx, int ny, M* kx0, M* ky0, M* dinv0) {
     cx /= 2;
     cy /= 2;
   }
+  HG_CHECK(hipMalloc(&never_done_, sizeof(int)));
+  HG_CHECK(hipMemset(never_done_, 0, sizeof(int)));
   const long nlast = long(lv_.back().nx) * lv_.back().ny;
   if (nlast > kCoarsestMax)
     coarse_sweeps = -1;  // coarsest too large for one workgroup: relax with grid kernels
@@ -845,6 +863,7 @@ MgPrecondT<M>::MgPrecondT(int nx, int ny, M* kx0, M* ky0, M* dinv0) {
 
 template <class M>
 MgPrecondT<M>::~MgPrecondT() {
+  if (never_done_) (void)hipFree(never_done_);
   for (size_t l = 0; l < lv_.size(); ++l) {
     MgLevelT<M>& L = lv_[l];
     for (void* p : {(void*)L.r, (void*)L.xa, (void*)L.xb})
@@ -886,6 +905,7 @@ void MgPrecondT<M>::Build(hipStream_t st) {
 template <class M>
 void MgPrecondT<M>::Cycle(int l, const double* r64, float* z, M* out, hipStream_t st,
                           const int* done, const FcgFusion* f) {
+  if (!done) done = never_done_;
   const FcgFusion fz = f ? *f : FcgFusion{};
   const int fused = (f && l == 0) ? 1 : 0;
   MgLevelT<M>& L = lv_[l];
@@ -1026,11 +1046,11 @@ double MgPrecondT<M>::TimeKernel(const char* name, int reps, const double* r, hi
     if (k == "KDown<double>") {
       KDown<M, double, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, fine_sweeps, L.xb, C.nx, C.ny,
-          C.r, nullptr, FcgFusion{}, 0);
+          C.r, never_done_, FcgFusion{}, 0);
     } else if (k == "KUp<double, true>") {
       KUp<M, M, double, true, kTallY, kFineSweeps><<<L.tall.blocks(), kTileThreads, 0, st>>>(
           L.nx, L.ny, L.tall, L.kx, L.ky, L.dinv, r, L.xb, C.xa, C.kx, C.ky, C.nx, C.ny,
-          fine_sweeps, L.xa, nullptr, FcgFusion{}, 0);
+          fine_sweeps, L.xa, never_done_, FcgFusion{}, 0);
     } else if (k == "KCoarseCycle" && first_in_kernel_ > 0) {
       Cycle(first_in_kernel_, nullptr, nullptr, lv_[first_in_kernel_].xa, st, nullptr);
     } else {
