@@ -11,6 +11,8 @@
 // result equals the serial sweep bit for bit.  Compiled with
 // -ffp-contract=off.
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace hg {
@@ -28,83 +30,6 @@ inline int ThreadsFor(int nx, int* cw) {
   return ((t + 63) / 64) * 64;
 }
 
-// Forward half of the symmetric sweep: x = (-k - s x_S - w x_W) / c.
-// Backward half: x -= (n x_N + e x_E) / c.  NR = number of right-hand sides.
-template <int NR>
-__global__ void __launch_bounds__(kMaxThreads)
-KLuWavefront(int nx, int ny, int cw, Rows a, const double* __restrict__ k0,
-             const double* __restrict__ k1, double* __restrict__ x0, double* __restrict__ x1,
-             const double* __restrict__ base0, const double* __restrict__ base1,
-             double* __restrict__ out0, double* __restrict__ out1) {
-  extern __shared__ double lds[];  // [2 parities][NR][T]
-  const int T = blockDim.x;
-  const int t = threadIdx.x;
-  const int c0 = t * cw;
-  const int c1 = min(nx, c0 + cw);
-  const bool active = c0 < nx;
-  const double* kk[2] = {k0, k1};
-  double* xx[2] = {x0, x1};
-  double south[NR][kMaxCw];
-  // ---- forward
-  for (int step = 0; step < ny + T - 1; ++step) {
-    const int j = step - t;
-    const int par = step & 1;
-    if (active && j >= 0 && j < ny) {
-      double west[NR];
-      for (int q = 0; q < NR; ++q) west[q] = (t > 0) ? lds[((par ^ 1) * NR + q) * T + t - 1] : 0.;
-      for (int i = c0; i < c1; ++i) {
-        const int r = i + nx * j;
-        const double cs = a.s[r], cw_ = a.w[r], cc = a.c[r];
-        for (int q = 0; q < NR; ++q) {
-          double sum = 0;
-          if (j > 0) sum += cs * south[q][i - c0];
-          if (i > 0) sum += cw_ * west[q];
-          const double v = (-kk[q][r] - sum) / cc;
-          xx[q][r] = v;
-          south[q][i - c0] = v;
-          west[q] = v;
-        }
-      }
-      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = west[q];
-    }
-    __syncthreads();
-  }
-  // ---- backward (thread T-1 first, rows from the top)
-  double north[NR][kMaxCw];
-  for (int step = 0; step < ny + T - 1; ++step) {
-    const int j = ny - 1 - (step - (T - 1 - t));
-    const int par = step & 1;
-    if (active && j >= 0 && j < ny) {
-      double east[NR];
-      for (int q = 0; q < NR; ++q)
-        east[q] = (t < T - 1) ? lds[((par ^ 1) * NR + q) * T + t + 1] : 0.;
-      for (int i = c1 - 1; i >= c0; --i) {
-        const int r = i + nx * j;
-        const double cn = a.n[r], ce = a.e[r], cc = a.c[r];
-        for (int q = 0; q < NR; ++q) {
-          double sum = 0;
-          if (j < ny - 1) sum += cn * north[q][i - c0];
-          if (i < nx - 1) sum += ce * east[q];
-          double v = xx[q][r];
-          v -= sum / cc;
-          xx[q][r] = v;
-          north[q][i - c0] = v;
-          east[q] = v;
-        }
-      }
-      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = east[q];
-    }
-    __syncthreads();
-  }
-  // ---- delta-form update out = base + x
-  if (base0) {
-    const double* bb[2] = {base0, base1};
-    double* oo[2] = {out0, out1};
-    for (int r = t; r < nx * ny; r += T)
-      for (int q = 0; q < NR; ++q) oo[q][r] = bb[q][r] + xx[q][r];
-  }
-}
-
 // Barrier between wavefront steps that orders only the LDS exchange: the
 // global loads a step has in flight (the next row's operands) are not waited
 // for, since no thread writes what they read before that row's step.
@@ -112,6 +37,151 @@ __device__ __forceinline__ void LdsBarrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Operands of one row of the LU sweep for a thread's CW columns and NR
+// right-hand sides: forward, the south / west coefficients, the diagonal and
+// the constants; backward, the north / east coefficients, the diagonal and
+// the forward pass's values.  Loaded one step ahead (nothing a row's step
+// reads is written by another thread in the same pass; the backward pass
+// reads the thread's own forward results); with one column per thread every
+// load is issued at a clamped index, as in the SOR kernel below.
+template <int NR, int CW>
+struct LuRow {
+  double a[CW], b[CW], c[CW], v[NR][CW];
+};
+
+template <int NR, int CW, bool FWD>
+__device__ __forceinline__ void LuLoad(LuRow<NR, CW>& o, int nx, int ny, int c0, int j,
+                                       const Rows& a, const double* const (&kk)[2],
+                                       double* const (&xx)[2]) {
+  if constexpr (CW == 1) {
+    const int jc = min(max(j, 0), ny - 1), i = min(c0, nx - 1), r = i + nx * jc;
+    o.a[0] = FWD ? a.s[r] : a.n[r];
+    o.b[0] = FWD ? a.w[r] : a.e[r];
+    o.c[0] = a.c[r];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) o.v[q][0] = FWD ? kk[q][r] : xx[q][r];
+  } else {
+    if (j < 0 || j >= ny) return;
+#pragma unroll
+    for (int m = 0; m < CW; ++m) {
+      const int i = c0 + m;
+      if (i >= nx) break;
+      const int r = i + nx * j;
+      o.a[m] = FWD ? a.s[r] : a.n[r];
+      o.b[m] = FWD ? a.w[r] : a.e[r];
+      o.c[m] = a.c[r];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) o.v[q][m] = FWD ? kk[q][r] : xx[q][r];
+    }
+  }
+}
+
+// Forward half of the symmetric sweep: x = (-k - s x_S - w x_W) / c.
+// Backward half: x -= (n x_N + e x_E) / c.  NR = number of right-hand sides.
+template <int NR, int CW>
+__global__ void __launch_bounds__(kMaxThreads)
+KLuWavefront(int nx, int ny, Rows a, const double* __restrict__ k0,
+             const double* __restrict__ k1, double* __restrict__ x0, double* __restrict__ x1,
+             const double* __restrict__ base0, const double* __restrict__ base1,
+             double* __restrict__ out0, double* __restrict__ out1) {
+  extern __shared__ double lds[];  // [2 parities][NR][T]
+  const int T = blockDim.x;
+  const int t = threadIdx.x;
+  const int c0 = t * CW;
+  const int nc = max(0, min(nx, c0 + CW) - c0);
+  const double* const kk[2] = {k0, k1};
+  double* const xx[2] = {x0, x1};
+  const int nsteps = ny + T - 1;
+  LuRow<NR, CW> ra, rb;
+  // ---- forward: row j = step - t, columns left to right
+  double south[NR][CW];
+  auto Fwd = [&](int step, const LuRow<NR, CW>& o) {
+    const int j = step - t;
+    const int par = step & 1;
+    if (nc > 0 && j >= 0 && j < ny) {
+      double west[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) west[q] = (t > 0) ? lds[((par ^ 1) * NR + q) * T + t - 1] : 0.;
+#pragma unroll
+      for (int m = 0; m < CW; ++m) {
+        if (m >= nc) break;
+        const int i = c0 + m, r = i + nx * j;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          double sum = 0;
+          if (j > 0) sum += o.a[m] * south[q][m];
+          if (i > 0) sum += o.b[m] * west[q];
+          const double v = (-o.v[q][m] - sum) / o.c[m];
+          xx[q][r] = v;
+          south[q][m] = v;
+          west[q] = v;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = west[q];
+    }
+    LdsBarrier();
+  };
+  LuLoad<NR, CW, true>(ra, nx, ny, c0, -t, a, kk, xx);
+  for (int step = 0; step < nsteps; step += 2) {
+    LuLoad<NR, CW, true>(rb, nx, ny, c0, step + 1 - t, a, kk, xx);
+    Fwd(step, ra);
+    if (step + 1 < nsteps) {
+      LuLoad<NR, CW, true>(ra, nx, ny, c0, step + 2 - t, a, kk, xx);
+      Fwd(step + 1, rb);
+    }
+  }
+  // ---- backward: thread T-1 first, rows from the top, columns right to left
+  double north[NR][CW];
+  const int lag = T - 1 - t;
+  auto Bwd = [&](int step, const LuRow<NR, CW>& o) {
+    const int j = ny - 1 - (step - lag);
+    const int par = step & 1;
+    if (nc > 0 && j >= 0 && j < ny) {
+      double east[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        east[q] = (t < T - 1) ? lds[((par ^ 1) * NR + q) * T + t + 1] : 0.;
+#pragma unroll
+      for (int m = CW - 1; m >= 0; --m) {
+        if (m >= nc) continue;
+        const int i = c0 + m, r = i + nx * j;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          double sum = 0;
+          if (j < ny - 1) sum += o.a[m] * north[q][m];
+          if (i < nx - 1) sum += o.b[m] * east[q];
+          double v = o.v[q][m];
+          v -= sum / o.c[m];
+          xx[q][r] = v;
+          north[q][m] = v;
+          east[q] = v;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) lds[(par * NR + q) * T + t] = east[q];
+    }
+    LdsBarrier();
+  };
+  LuLoad<NR, CW, false>(ra, nx, ny, c0, ny - 1 + lag, a, kk, xx);
+  for (int step = 0; step < nsteps; step += 2) {
+    LuLoad<NR, CW, false>(rb, nx, ny, c0, ny - 1 - (step + 1 - lag), a, kk, xx);
+    Bwd(step, ra);
+    if (step + 1 < nsteps) {
+      LuLoad<NR, CW, false>(ra, nx, ny, c0, ny - 1 - (step + 2 - lag), a, kk, xx);
+      Bwd(step + 1, rb);
+    }
+  }
+  __syncthreads();  // every thread's x before the delta-form update reads it
+  // ---- delta-form update out = base + x
+  if (base0) {
+    const double* bb[2] = {base0, base1};
+    double* oo[2] = {out0, out1};
+    for (int r = t; r < nx * ny; r += T)
+      for (int q = 0; q < NR; ++q) oo[q][r] = bb[q][r] + xx[q][r];
+  }
 }
 
 // One cell row's operands for a thread's CW columns: the row coefficients,
@@ -487,12 +557,31 @@ void LaunchLuWavefront(hipStream_t st, int nx, int ny, Rows a, const double* k0,
                        const double* base1, double* out0, double* out1) {
   int cw;
   int T = ThreadsFor(nx, &cw);
+  auto launch = [&](auto nr, auto cwc) {
+    constexpr int NR = decltype(nr)::value, CW = decltype(cwc)::value;
+    const size_t lds = size_t(2) * NR * T * sizeof(double);
+    KLuWavefront<NR, CW><<<1, T, lds, st>>>(nx, ny, a, k0, NR == 2 ? k1 : k0, x0,
+                                            NR == 2 ? x1 : x0, base0, NR == 2 ? base1 : base0,
+                                            out0, NR == 2 ? out1 : out0);
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
   if (k1) {
-    size_t lds = size_t(2) * 2 * T * sizeof(double);
-    KLuWavefront<2><<<1, T, lds, st>>>(nx, ny, cw, a, k0, k1, x0, x1, base0, base1, out0, out1);
+    switch (cw) {
+      case 1: launch(I2{}, I1{}); break;
+      case 2: launch(I2{}, I2{}); break;
+      case 3: launch(I2{}, I3{}); break;
+      default: launch(I2{}, I4{}); break;
+    }
   } else {
-    size_t lds = size_t(2) * T * sizeof(double);
-    KLuWavefront<1><<<1, T, lds, st>>>(nx, ny, cw, a, k0, k0, x0, x0, base0, base0, out0, out0);
+    switch (cw) {
+      case 1: launch(I1{}, I1{}); break;
+      case 2: launch(I1{}, I2{}); break;
+      case 3: launch(I1{}, I3{}); break;
+      default: launch(I1{}, I4{}); break;
+    }
   }
 }
 
