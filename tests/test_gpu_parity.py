@@ -190,8 +190,9 @@ def test_stencil_solvers_bitwise(gpu, solver):
 
 # C2 in full (500 steps) takes the exact-order SOR through 2.7 M sweeps (the
 # reference's own solver runs to its 1000-sweep limit on every SIMPLE
-# iteration from about step 200): minutes on one workgroup.  It runs when
-# HYDRO_LONG_C2=1 is set, outside the default suite.
+# iteration from about step 200): about 8 minutes on one workgroup of one
+# MI355X, longer than the default suite should take.  It runs when
+# HYDRO_LONG_C2=1 is set.
 LONG_C2 = pytest.param("ref_dam256x128_500", marks=pytest.mark.skipif(
     os.environ.get("HYDRO_LONG_C2") != "1", reason="C2 500 steps: set HYDRO_LONG_C2=1"))
 
