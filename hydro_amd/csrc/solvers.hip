@@ -333,47 +333,65 @@ KGsWavefront(int nx, int ny, Rows a, const double* __restrict__ k, double* __res
 // (Expression::Evaluate: constant, then S, W, C, E, N).  f starts as the
 // constants and res at zero.  Stops as do { ... } while (diff > tol &&
 // iter++ < limit).
+template <int CW>
 __global__ void __launch_bounds__(kMaxThreads)
-KLuRelaxed(int nx, int ny, int cw, Rows a, const double* __restrict__ k,
+KLuRelaxed(int nx, int ny, Rows a, const double* __restrict__ k,
            double* __restrict__ res, double* __restrict__ corr, double* __restrict__ f,
            double tol, int limit, double relax, unsigned long long* __restrict__ info) {
   extern __shared__ double lds[];  // [2][T] west values, then reduction scratch
   double* red = lds + 2 * blockDim.x;
   const int T = blockDim.x;
   const int t = threadIdx.x;
-  const int c0 = t * cw;
-  const int c1 = min(nx, c0 + cw);
-  const bool active = c0 < nx;
+  const int c0 = t * CW;
+  const int nc = max(0, min(nx, c0 + CW) - c0);
   const int N = nx * ny;
+  const int nsteps = ny + T - 1;
+  // the forward sweep's row operands (s, w, c and f), one step ahead as in
+  // the LU sweep: f changes only between sweeps, behind full barriers
+  const double* const ff[2] = {f, f};
+  double* const cc[2] = {corr, corr};
+  LuRow<1, CW> ra, rb;
   for (int r = t; r < N; r += T) {
     res[r] = 0.;
     f[r] = k[r];
   }
   __syncthreads();
-  double south[kMaxCw];
+  double south[CW];
   long iter = 0;
   double diff = 0.;
+  auto Fwd = [&](int step, const LuRow<1, CW>& o) {
+    const int j = step - t;
+    const int par = step & 1;
+    if (nc > 0 && j >= 0 && j < ny) {
+      double west = (t > 0) ? lds[(par ^ 1) * T + t - 1] : 0.;
+#pragma unroll
+      for (int m = 0; m < CW; ++m) {
+        if (m >= nc) break;
+        const int i = c0 + m, r = i + nx * j;
+        double sum = 0;
+        if (j > 0) sum += o.a[m] * south[m];
+        if (i > 0) sum += o.b[m] * west;
+        const double v = (-o.v[0][m] - sum) / (o.c[m] + relax);
+        corr[r] = v;
+        south[m] = v;
+        west = v;
+      }
+      lds[par * T + t] = west;
+    }
+    LdsBarrier();
+  };
   while (true) {
     // ---- forward (wavefront over i + j)
-    for (int step = 0; step < ny + T - 1; ++step) {
-      const int j = step - t;
-      const int par = step & 1;
-      if (active && j >= 0 && j < ny) {
-        double west = (t > 0) ? lds[(par ^ 1) * T + t - 1] : 0.;
-        for (int i = c0; i < c1; ++i) {
-          const int r = i + nx * j;
-          double sum = 0;
-          if (j > 0) sum += a.s[r] * south[i - c0];
-          if (i > 0) sum += a.w[r] * west;
-          const double v = (-f[r] - sum) / (a.c[r] + relax);
-          corr[r] = v;
-          south[i - c0] = v;
-          west = v;
-        }
-        lds[par * T + t] = west;
+    LuLoad<1, CW, true>(ra, nx, ny, c0, -t, a, ff, cc);
+    for (int step = 0; step < nsteps; step += 2) {
+      LuLoad<1, CW, true>(rb, nx, ny, c0, step + 1 - t, a, ff, cc);
+      Fwd(step, ra);
+      if (step + 1 < nsteps) {
+        LuLoad<1, CW, true>(ra, nx, ny, c0, step + 2 - t, a, ff, cc);
+        Fwd(step + 1, rb);
       }
-      __syncthreads();
     }
+    __syncthreads();  // every thread's corrections before the backward step reads them
     // ---- backward, from the previous iterate (no recurrence)
     for (int r = t; r < N; r += T) {
       const int i = r % nx, j = r / nx;
@@ -604,7 +622,12 @@ void LaunchLuRelaxed(hipStream_t st, int nx, int ny, Rows a, const double* k, do
   int cw;
   int T = ThreadsFor(nx, &cw);
   size_t lds = size_t(2) * T * sizeof(double) + 16 * sizeof(double);
-  KLuRelaxed<<<1, T, lds, st>>>(nx, ny, cw, a, k, res, corr, f, tol, limit, relax, info);
+  switch (cw) {
+    case 1: KLuRelaxed<1><<<1, T, lds, st>>>(nx, ny, a, k, res, corr, f, tol, limit, relax, info); break;
+    case 2: KLuRelaxed<2><<<1, T, lds, st>>>(nx, ny, a, k, res, corr, f, tol, limit, relax, info); break;
+    case 3: KLuRelaxed<3><<<1, T, lds, st>>>(nx, ny, a, k, res, corr, f, tol, limit, relax, info); break;
+    default: KLuRelaxed<4><<<1, T, lds, st>>>(nx, ny, a, k, res, corr, f, tol, limit, relax, info); break;
+  }
 }
 
 void LaunchJacobiSweeps(hipStream_t st, int nx, int ny, Rows a, const double* k, JacobiWork w,
